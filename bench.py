@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Headline benchmark: Laguerre cells/s of the ball-clipped 2D cell pass.
+
+Workload: N = 1,000,000 diracs uniform in [0,1]^2 (seed 0), weights
+w_i ~ U[0, 1e-6] shifted by r0^2 = 1e-6, so each cell is clipped by the disc of
+radius sqrt(w_i + r0^2); domain the unit square with density 1.  One step = one
+full pass on resident HBM inputs: grid build (bounding box, Morton keys,
+counting sort, per-cell max weight), cell kernel (power cell, disc clip, cell
+area, boundary measures and Hessian rows) and the dirac-order CSR build in
+HBM.  Both precisions are measured; `value` is the faster one whose masses
+agree with the FP64 CPU oracle within 1e-6 relative.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Laguerre cells/s (1M diracs 2D/3D); solve s @1e-6 mass err; p99 latency"
+PEAK = {"fp64": 78.6, "mixed": 157.3}        # MI355X vector FP64 / FP32, TFLOP/s
+
+
+def make_inputs(n, seed=0):
+    import sdot_amd
+    u = sdot_amd.rng_uniform(seed, 3 * n)
+    pos = u[: 2 * n].reshape(n, 2).copy()
+    w = 1e-6 * u[2 * n:] + 1e-6
+    return pos, w
+
+
+def run_gpu(pos, w, precision, steps, warmup):
+    import sdot_amd
+    pd = sdot_amd.PowerDiagram(2, precision, ball=True)
+    pd.add_box_shape([0.0, 0.0], [1.0, 1.0])
+    pd.set_diracs(pos, w)
+    for _ in range(warmup):
+        pd.mark_diracs_changed()
+        pd.compute()
+        pd.csr_device()
+    tims = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pd.mark_diracs_changed()
+        pd.compute()
+        pd.csr_device()
+        tims.append(pd.last_timings())
+    t1 = time.perf_counter()
+    masses = pd.integrals()
+    ms_step = (t1 - t0) * 1e3 / steps
+    kern = float(np.mean([t["ms_cell_kernel"] for t in tims]))
+    flops = tims[-1]["cell_flops"]
+    return dict(precision=precision, ms_per_step=ms_step, cells_per_s=len(w) / (ms_step * 1e-3),
+                ms_cell_kernel=kern, ms_grid=float(np.mean([t["ms_grid"] for t in tims])),
+                ms_rows=float(np.mean([t["ms_rows"] for t in tims])),
+                cell_flops=flops, achieved_tflops=flops / (kern * 1e-3) / 1e12,
+                overflow_cells=int(tims[-1]["nb_cells_overflow"]), status=pd.last_status), masses
+
+
+def run_cpu_baseline(pos, w, budget_s=10.0, max_reps=40):
+    from oracle import oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        r = O.cells(pos, w, ball=True, nthreads=cores)
+        O.to_csr(r)
+        reps += 1
+        if time.perf_counter() - t0 >= budget_s or reps >= max_reps:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    return dict(value=len(w) / dt, unit="cells/s", cores=cores, kind="port",
+                sample=f"{reps} full passes of the same 1M-dirac ball workload (grid + cells + CSR) "
+                       f"by the C oracle with {cores} threads, {dt:.3f} s each"), r["masses"]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py measures one GPU only (the cell pass of one GPU's share); --gpus {args.gpus} "
+              "is not run: multi-GPU sharding is a design (ghost-layer exchange), not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    pos, w = make_inputs(args.n)
+    res = {}
+    masses = {}
+    for prec in ("fp64", "mixed"):
+        res[prec], masses[prec] = run_gpu(pos, w, prec, args.steps, args.warmup)
+        print(f"# {prec}: {res[prec]}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu, ref = run_cpu_baseline(pos, w)
+        scale = 1.0 / args.n
+        for prec in res:
+            err = np.abs(masses[prec] - ref)
+            res[prec]["max_mass_err_vs_oracle"] = float((err / (np.abs(ref) + 1e-9 * scale)).max())
+            res[prec]["max_mass_err_x_N"] = float(err.max() / scale)
+
+    def ok(p):
+        return "max_mass_err_vs_oracle" not in res[p] or res[p]["max_mass_err_vs_oracle"] < 1e-6
+
+    cands = [p for p in res if ok(p)] or ["fp64"]
+    best = max(cands, key=lambda p: res[p]["cells_per_s"])
+    b = res[best]
+    achieved = b["achieved_tflops"]
+    line = {
+        "metric": METRIC,
+        "value": b["cells_per_s"],
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": b["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-clip/f64-integrate" if best == "mixed" else "f64",
+        "data": "synthetic (xoshiro256** seed 0)",
+        "config": {"workload": "2D ball-clipped cell pass, N=1e6 uniform in [0,1]^2, w~U[0,1e-6]+r0^2, r0=1e-3, "
+                               "unit-square domain; grid + cells + Hessian CSR per step",
+                   "n_diracs": args.n, "dim": 2, "precision": best},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[best], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK[best], "traffic": None,
+                     "kernel": "k_cells2", "flops_per_launch": b["cell_flops"],
+                     "kernel_ms": b["ms_cell_kernel"]},
+        "cpu_baseline": cpu,
+        "precisions": res,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

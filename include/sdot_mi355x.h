@@ -1,0 +1,153 @@
+/*
+ * sdot_mi355x.h -- C-ABI of the MI355X (gfx950) Laguerre-cell path.
+ *
+ * A flat, handle-based API in front of hand-written HIP kernels.  It replaces,
+ * for the Laguerre-cell hot path, the reference's three template APIs
+ * (paths relative to the sdot tree):
+ *   Visitor  SpZGrid::update / for_each_laguerre_cell   src/sdot/PowerDiagram/Visitors/SpZGrid.h:39-48
+ *   Bounds   ConvexPolyhedronAssembly::add_box ...      src/sdot/Domains/ConvexPolyhedronAssembly.h:26-41
+ *            ScaledImage(min, max, data, sizes, nb_coeffs) src/sdot/Domains/ScaledImage.h:26
+ *   Functions get_integrals                             src/sdot/PowerDiagram/get_integrals.h:13
+ *            get_der_integrals_wrt_weights              src/sdot/PowerDiagram/get_der_integrals_wrt_weights.h:16
+ *            OptimalTransportSolver::solve              src/sdot/Solvers/OptimalTransportSolver.h:18-26
+ *
+ * Conventions (same as the reference):
+ *   positions are interleaved (x0,y0[,z0],x1,...) doubles, Point2/Point3 layout
+ *   (src/sdot/Geometry/Point2.h:14); outputs are in dirac order; CSR rows are
+ *   sorted by column, duplicates merged, diagonal present; the domain cut id is
+ *   -1.  Index types are int64_t (the reference's TI = std::size_t).
+ * Errors are status codes; no entry point aborts (the reference's ASSERT/TODO
+ * call abort(), src/sdot/Support/Assert.h:13-17).
+ * Threading: a handle owns one HIP stream (default priority) and is not
+ * internally locked; use one handle per host thread.
+ */
+#ifndef SDOT_MI355X_H
+#define SDOT_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sdot_status {
+    SDOT_OK = 0,
+    SDOT_EMPTY_CELL = 1,         /* an empty Laguerre cell (reference returns 1) */
+    SDOT_NOT_CONVERGED = 2,
+    SDOT_CAPACITY_OVERFLOW = 3,
+    SDOT_INVALID_ARG = 4,
+    SDOT_HIP_ERROR = 5,
+    SDOT_OOM = 6,
+    SDOT_NOT_SUPPORTED = 7
+} sdot_status;
+
+typedef enum sdot_precision {
+    SDOT_FP64 = 0,               /* clipping, integrals and solve in FP64 */
+    SDOT_MIXED = 1               /* FP32 cell-local clipping and integrals; FP64 plane offsets,
+                                    mass sums, residuals and the Newton/CG solve */
+} sdot_precision;
+
+typedef struct sdot_pd sdot_pd;
+
+/* Handle life cycle.  dim is 2 or 3. */
+sdot_status sdot_pd_create(sdot_pd **pd, int dim, int precision);
+sdot_status sdot_pd_destroy(sdot_pd *pd);
+const char *sdot_status_string(int status);
+/* last HIP / argument error text of this handle ("" if none) */
+const char *sdot_pd_last_error(const sdot_pd *pd);
+
+/* Diracs (host arrays, copied to HBM).  SpZGrid::update(pos, w, N) -- SpZGrid.h:39 */
+sdot_status sdot_pd_set_diracs(sdot_pd *pd, const double *positions, const double *weights, int64_t nb_diracs);
+/* weights only (positions unchanged: the grid's sort is kept) */
+sdot_status sdot_pd_set_weights(sdot_pd *pd, const double *weights);
+/* Same with device pointers already in HBM (copied device-to-device on the handle's stream). */
+sdot_status sdot_pd_set_diracs_device(sdot_pd *pd, const double *d_positions, const double *d_weights, int64_t nb_diracs);
+
+/* Device-resident inputs: the handle's HBM arrays of positions (N*dim) and
+ * weights (N), for producers that write them in place; after writing, call
+ * sdot_pd_mark_diracs_changed so the next pass rebuilds the grid. */
+sdot_status sdot_pd_dirac_buffers(sdot_pd *pd, double **d_positions, double **d_weights);
+sdot_status sdot_pd_mark_diracs_changed(sdot_pd *pd, int positions_changed);
+
+/* Domain ("bounds").  ConvexPolyhedronAssembly::add_box(p0, p1, coeff, cut_id) -- .tcc:46.
+ * One box item is supported on the device path (the configuration every
+ * workload uses); several items return SDOT_NOT_SUPPORTED at evaluation. */
+sdot_status sdot_pd_add_box_shape(sdot_pd *pd, const double *p0, const double *p1, double coeff, int64_t cut_id);
+/* ScaledImage(min_pt, max_pt, data, sizes, nb_coeffs) -- ScaledImage.h:26; sizes x fastest; nb_coeffs = 1 */
+sdot_status sdot_pd_set_image_density(sdot_pd *pd, const double *min_pt, const double *max_pt, const double *data,
+                                      const int64_t *sizes, int64_t nb_coeffs);
+sdot_status sdot_pd_clear_shapes(sdot_pd *pd);
+/* Radial function: 0 = FunctionEnum::Unit, 1 = FunctionEnum::InBallW05 (clip by the
+ * ball of radius sqrt(w_i)) -- src/sdot/Integration/Unit.h, InBallW05.h */
+sdot_status sdot_pd_set_ball_clip(sdot_pd *pd, int on);
+
+/* Run the cell pass (grid build + cells + Hessian rows) on the device; results
+ * stay resident.  Returns SDOT_EMPTY_CELL when a cell is empty or has zero mass
+ * (get_der_integrals_wrt_weights.h:93-96), like the reference's int result. */
+sdot_status sdot_pd_compute(sdot_pd *pd);
+
+/* get_integrals(res, ...) -- get_integrals.h:13.  Runs the pass if needed. */
+sdot_status sdot_pd_integrals(sdot_pd *pd, double *masses);
+/* per-cell boundary measures: ball part (arcs with cut id i) and whole perimeter
+ * (area of the boundary in 3D); either may be NULL */
+sdot_status sdot_pd_boundary_measures(sdot_pd *pd, double *ball_arc, double *perimeter);
+/* get_der_integrals_wrt_weights(offsets, cols, vals, masses, ...) -- get_der_integrals_wrt_weights.h:16.
+ * Two-phase: with cols == NULL only *nnz is written.  offsets has N+1 entries;
+ * cols/vals need room for *nnz entries (cap).  masses may be NULL. */
+sdot_status sdot_pd_der_integrals_wrt_weights(sdot_pd *pd, int64_t *offsets, int64_t *cols, double *vals,
+                                              double *masses, int64_t cap, int64_t *nnz);
+
+/* Builds the dirac-order CSR of the last pass in HBM (no host copy) and returns
+ * handle-owned device pointers: offsets[N+1]; row i occupies
+ * [offsets[i], offsets[i+1]) with ascending columns (a merged duplicate leaves a
+ * trailing column -1, value 0).  Valid until the next call on the handle. */
+sdot_status sdot_pd_csr_device(sdot_pd *pd, const int64_t **d_offsets, const int64_t **d_cols, const double **d_vals);
+
+typedef struct sdot_solve_stats {
+    int newton_iters;
+    int pcg_iters_total;
+    int backtracks;
+    int reserved;
+    double max_rel_mass_err;     /* at the returned weights */
+    double max_abs_dw;           /* last Newton step */
+    int64_t nnz;
+    double seconds_cells;        /* device time in the cell passes */
+    double seconds_solve;        /* device time in the PCG solves */
+    double seconds_total;        /* wall time of the call */
+} sdot_solve_stats;
+
+/* OptimalTransportSolver::solve(pos, w_inout, N, masses, rf) -- OptimalTransportSolver.tcc:27-119.
+ * Newton with the reference's rules (M00 doubled without ball; 0.9/0.1 backtrack on an
+ * empty cell).  stop_rule 0: max_i |m_i - t_i| / t_i < tol; 1: max |dw| < tol (reference).
+ * weights (dirac order) are read as the start and overwritten with the result. */
+sdot_status sdot_ot_solve(sdot_pd *pd, const double *target_masses, double tol, int stop_rule, int max_iter,
+                          double *weights_inout, sdot_solve_stats *stats);
+
+/* Tuning knobs (0 = automatic): mean diracs per grid cell (x100), threads per block. */
+sdot_status sdot_pd_set_tuning(sdot_pd *pd, int diracs_per_cell_x100, int block_size);
+
+typedef struct sdot_timings {
+    double ms_grid;              /* bbox + sort + cell tables */
+    double ms_cells;             /* cell kernel(s), incl. overflow re-runs */
+    double ms_rows;              /* CSR export (0 unless requested) */
+    double ms_total;
+    int64_t nb_cells_overflow;   /* cells re-run on the wide-capacity path */
+    int64_t nnz_offdiag;         /* off-diagonal Hessian entries in the last pass */
+    int64_t grid_cells;          /* grid cells of the last grid build */
+    double ms_cell_kernel;       /* main cell kernel alone (HIP events around the launch) */
+    double cell_flops;           /* algorithmic FLOPs of the pass: sum over cells of
+                                    4K^2 + 37K + 40A (2D) / 12K^2 + 100K - 284 (3D),
+                                    K = faces, A = ball arcs of the final cell */
+    int64_t sum_faces;           /* sum of K */
+    int64_t sum_arcs;            /* sum of A */
+} sdot_timings;
+/* HIP-event timings of the last sdot_pd_compute (on the handle's stream). */
+sdot_status sdot_pd_last_timings(sdot_pd *pd, sdot_timings *t);
+
+/* splitmix64 -> xoshiro256** uniform doubles (x >> 11) * 2^-53: the synthetic-input generator. */
+void sdot_rng_uniform(uint64_t seed, int64_t n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

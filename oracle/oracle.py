@@ -1,0 +1,195 @@
+"""ctypes front-end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the timed CPU
+baseline.  Imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product package never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+OR_OK, OR_EMPTY_CELL, OR_NOT_CONVERGED, OR_CAPACITY, OR_INVALID = 0, 1, 2, 3, 4
+
+_d = ctypes.c_double
+_i64 = ctypes.c_int64
+_p = ctypes.c_void_p
+
+
+class Domain(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("p0", _d * 3), ("p1", _d * 3), ("coeff", _d),
+                ("img", _p), ("img_sizes", _i64 * 3)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("newton_iters", ctypes.c_int), ("pcg_iters_total", ctypes.c_int),
+                ("backtracks", ctypes.c_int), ("max_rel_mass_err", _d),
+                ("max_abs_dw", _d), ("nnz", _i64)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.or_cp2_eval.restype = ctypes.c_int
+        L.or_cp2_eval.argtypes = [_d, _d, _d, _d, ctypes.c_int, _p, ctypes.c_int, _d, _d, _d,
+                                  _p, _p, _p, _p, ctypes.c_int, _p]
+        L.or_cp3_eval.restype = ctypes.c_int
+        L.or_cp3_eval.argtypes = [_p, _p, ctypes.c_int, _p, _p, _p, _p, _p, ctypes.c_int]
+        L.or_cells.restype = ctypes.c_int
+        L.or_cells.argtypes = [ctypes.c_int, _i64, _p, _p, ctypes.POINTER(Domain), ctypes.c_int, ctypes.c_int,
+                               _p, _p, _p, _i64, _p, _p, _p]
+        L.or_rows_to_csr.restype = _i64
+        L.or_rows_to_csr.argtypes = [_i64, _i64, _p, _p, _p, _p, _p, _p]
+        L.or_pcg.restype = ctypes.c_int
+        L.or_pcg.argtypes = [_i64, _p, _p, _p, _p, _p, _d, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.or_ot_solve.restype = ctypes.c_int
+        L.or_ot_solve.argtypes = [ctypes.c_int, _i64, _p, _p, _p, ctypes.POINTER(Domain), ctypes.c_int,
+                                  ctypes.c_int, _d, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SolveStats)]
+        L.or_rng_uniform.restype = None
+        L.or_rng_uniform.argtypes = [ctypes.c_uint64, _i64, _p]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def rng_uniform(seed, n):
+    out = np.empty(int(n), dtype=np.float64)
+    lib().or_rng_uniform(ctypes.c_uint64(seed), int(n), _ptr(out))
+    return out
+
+
+def cp2_eval(box, cuts, ball=None, cap=256):
+    """box=(x0,y0,x1,y1); cuts=[(ox,oy,nx,ny)] with normalised normals; ball=(cx,cy,r)."""
+    c = np.ascontiguousarray(np.asarray(cuts, dtype=np.float64).reshape(-1, 4))
+    area = ctypes.c_double()
+    nbp = ctypes.c_int()
+    nit = ctypes.c_int()
+    meas = np.zeros(cap)
+    ids = np.zeros(cap, dtype=np.int64)
+    b = ball if ball is not None else (0.0, 0.0, 0.0)
+    st = lib().or_cp2_eval(box[0], box[1], box[2], box[3], len(c), _ptr(c), int(ball is not None),
+                           b[0], b[1], b[2], ctypes.byref(area), ctypes.byref(nbp), _ptr(meas), _ptr(ids),
+                           cap, ctypes.byref(nit))
+    n = nit.value
+    return dict(status=st, area=area.value, nb_points=nbp.value, measures=meas[:n].copy(), ids=ids[:n].copy())
+
+
+def cp3_eval(p0, p1, cuts, cap=256):
+    """cuts=[(ox,oy,oz,nx,ny,nz)]; keeps (o-p).n >= 0."""
+    c = np.ascontiguousarray(np.asarray(cuts, dtype=np.float64).reshape(-1, 6))
+    a0 = np.asarray(p0, dtype=np.float64)
+    a1 = np.asarray(p1, dtype=np.float64)
+    vol = ctypes.c_double()
+    nf = ctypes.c_int()
+    areas = np.zeros(cap)
+    ids = np.zeros(cap, dtype=np.int64)
+    st = lib().or_cp3_eval(_ptr(a0), _ptr(a1), len(c), _ptr(c), ctypes.byref(vol), ctypes.byref(nf),
+                           _ptr(areas), _ptr(ids), cap)
+    n = nf.value
+    return dict(status=st, volume=vol.value, nb_faces=n, areas=areas[:n].copy(), ids=ids[:n].copy())
+
+
+def make_domain(dim, p0=None, p1=None, coeff=1.0, img=None):
+    d = Domain()
+    d.dim = dim
+    p0 = p0 if p0 is not None else [0.0] * dim
+    p1 = p1 if p1 is not None else [1.0] * dim
+    for k in range(dim):
+        d.p0[k] = p0[k]
+        d.p1[k] = p1[k]
+    d.coeff = coeff
+    keep = None
+    if img is not None:
+        keep = np.ascontiguousarray(img, dtype=np.float64)
+        d.img = keep.ctypes.data
+        # img indexed [z][y][x] (x fastest)
+        shp = keep.shape[::-1]
+        for k in range(dim):
+            d.img_sizes[k] = shp[k]
+    else:
+        d.img = None
+    return d, keep
+
+
+def cells(pos, w, dim=None, ball=False, brute=False, nthreads=None, domain=None, img=None, cap=None):
+    """Per-cell masses, ball arc, perimeter and raw rows.  Returns dict."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    dim = dim or pos.shape[1]
+    N = pos.shape[0]
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    cap = cap or (64 if dim == 2 else 160)
+    if domain is None:
+        dom, keep = make_domain(dim, img=img)
+    else:
+        dom, keep = make_domain(dim, *domain, img=img)
+    nthreads = nthreads or os.cpu_count() or 1
+    masses = np.zeros(N)
+    arc = np.zeros(N)
+    per = np.zeros(N)
+    rl = np.zeros(N, dtype=np.int64)
+    rc = np.zeros(N * cap, dtype=np.int64)
+    rv = np.zeros(N * cap)
+    flags = (1 if ball else 0) | (2 if brute else 0)
+    st = lib().or_cells(dim, N, _ptr(pos), _ptr(w), ctypes.byref(dom), flags, nthreads,
+                        _ptr(masses), _ptr(arc), _ptr(per), cap, _ptr(rl), _ptr(rc), _ptr(rv))
+    del keep
+    return dict(status=st, masses=masses, ball_arc=arc, perimeter=per, row_len=rl, rcols=rc, rvals=rv, cap=cap)
+
+
+def to_csr(res):
+    N = len(res["masses"])
+    cap = res["cap"]
+    nnz_max = int(res["row_len"].sum())
+    off = np.zeros(N + 1, dtype=np.int64)
+    cols = np.zeros(max(nnz_max, 1), dtype=np.int64)
+    vals = np.zeros(max(nnz_max, 1))
+    nnz = lib().or_rows_to_csr(N, cap, _ptr(res["row_len"]), _ptr(res["rcols"]), _ptr(res["rvals"]),
+                               _ptr(off), _ptr(cols), _ptr(vals))
+    return off, cols[:nnz].copy(), vals[:nnz].copy()
+
+
+def der_integrals_wrt_weights(pos, w, **kw):
+    res = cells(pos, w, **kw)
+    off, cols, vals = to_csr(res)
+    return res["status"], off, cols, vals, res["masses"]
+
+
+def pcg(off, cols, vals, b, rtol=1e-12, max_iter=100000):
+    x = np.zeros(len(b))
+    it = ctypes.c_int()
+    st = lib().or_pcg(len(b), _ptr(off), _ptr(cols), _ptr(vals), _ptr(np.ascontiguousarray(b)), _ptr(x),
+                      rtol, max_iter, ctypes.byref(it))
+    return st, x, it.value
+
+
+def ot_solve(pos, w0, targets, ball=False, tol=1e-6, stop_rule=0, max_iter=150, nthreads=None,
+             domain=None, img=None):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    dim = pos.shape[1]
+    w = np.array(w0, dtype=np.float64, copy=True)
+    t = np.ascontiguousarray(targets, dtype=np.float64)
+    if domain is None:
+        dom, keep = make_domain(dim, img=img)
+    else:
+        dom, keep = make_domain(dim, *domain, img=img)
+    stats = SolveStats()
+    st = lib().or_ot_solve(dim, pos.shape[0], _ptr(pos), _ptr(w), _ptr(t), ctypes.byref(dom),
+                           1 if ball else 0, nthreads or os.cpu_count() or 1, tol, stop_rule, max_iter,
+                           ctypes.byref(stats))
+    del keep
+    return st, w, stats

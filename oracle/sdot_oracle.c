@@ -1,0 +1,1094 @@
+/*
+ * sdot_oracle.c -- CPU restatement of the sdot Laguerre-cell path.
+ *
+ * TEST INFRASTRUCTURE ONLY (checker and timed CPU baseline).  The product
+ * library never links or calls this file.  See sdot_oracle.h for what each
+ * function follows in the reference and how it is pinned.
+ */
+#define _GNU_SOURCE
+#include "sdot_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* 2D polygon: ConvexPolyhedron2 restated                                     */
+/* ------------------------------------------------------------------------- */
+
+/* Box ctor: ConvexPolyhedron2.tcc:65-93 (vertices p0,(p1.x,p0.y),p1,(p0.x,p1.y)) */
+void or_cp2_box(or_cp2 *cp, double x0, double y0, double x1, double y1, int64_t id) {
+    cp->n = 4;
+    cp->x[0] = x0; cp->y[0] = y0;
+    cp->x[1] = x1; cp->y[1] = y0;
+    cp->x[2] = x1; cp->y[2] = y1;
+    cp->x[3] = x0; cp->y[3] = y1;
+    cp->nx[0] = 0;  cp->ny[0] = -1;
+    cp->nx[1] = 1;  cp->ny[1] = 0;
+    cp->nx[2] = 0;  cp->ny[2] = 1;
+    cp->nx[3] = -1; cp->ny[3] = 0;
+    for (int k = 0; k < 4; ++k) { cp->id[k] = id; cp->arc[k] = 0; }
+    cp->scx = 0.5 * (x0 + x1);
+    cp->scy = 0.5 * (y0 + y1);
+    cp->sr = -1;
+    cp->sid = -1;
+}
+
+int or_cp2_empty(const or_cp2 *cp) { return cp->n == 0 && cp->sr <= 0; } /* ConvexPolyhedron2.h:95 */
+
+/* plane_cut: ConvexPolyhedron2.tcc:687-922.  Keeps p with (o-p).n >= 0.  New
+ * points are interpolated from the inside vertex toward the outside one
+ * (:752-783); the edge entering the removed run takes the new id and normal,
+ * the edge leaving it keeps the id of the last removed edge.  The streaming
+ * form below emits the same vertex order as the reference's four cases. */
+int or_cp2_plane_cut(or_cp2 *cp, double ox, double oy, double nx, double ny, int64_t id, int normal_is_normalized) {
+    int n = cp->n;
+    if (n == 0) return 0;
+    double d[OR_CP2_CAP];
+    int nout = 0;
+    for (int k = 0; k < n; ++k) {
+        d[k] = (ox - cp->x[k]) * nx + (oy - cp->y[k]) * ny;
+        nout += d[k] < 0;
+    }
+    if (nout == 0) return 0;
+    if (nout == n) { cp->n = 0; return 0; }
+    if (!normal_is_normalized) {
+        double s = 1 / sqrt(nx * nx + ny * ny);
+        for (int k = 0; k < n; ++k) d[k] *= s;
+        nx *= s; ny *= s;
+    }
+    if (n + 1 > OR_CP2_CAP) return OR_CAPACITY;
+    or_cp2 t;
+    int m = 0;
+    for (int k = 0; k < n; ++k) {
+        int km = (k + n - 1) % n;
+        int ok = d[k] < 0, okm = d[km] < 0;
+        if (ok && !okm) {            /* inside km -> outside k: new point, new edge id */
+            double m0 = d[km] / (d[k] - d[km]);
+            t.x[m] = cp->x[km] - m0 * (cp->x[k] - cp->x[km]);
+            t.y[m] = cp->y[km] - m0 * (cp->y[k] - cp->y[km]);
+            t.nx[m] = nx; t.ny[m] = ny; t.id[m] = id; t.arc[m] = 0;
+            ++m;
+        }
+        if (!ok && okm) {            /* outside km -> inside k: point keeps edge km's id */
+            double m1 = d[k] / (d[km] - d[k]);
+            t.x[m] = cp->x[k] - m1 * (cp->x[km] - cp->x[k]);
+            t.y[m] = cp->y[k] - m1 * (cp->y[km] - cp->y[k]);
+            t.nx[m] = cp->nx[km]; t.ny[m] = cp->ny[km]; t.id[m] = cp->id[km]; t.arc[m] = cp->arc[km];
+            ++m;
+        }
+        if (!ok) {
+            t.x[m] = cp->x[k]; t.y[m] = cp->y[k];
+            t.nx[m] = cp->nx[k]; t.ny[m] = cp->ny[k]; t.id[m] = cp->id[k]; t.arc[m] = cp->arc[k];
+            ++m;
+        }
+    }
+    memcpy(cp->x, t.x, m * sizeof(double));
+    memcpy(cp->y, t.y, m * sizeof(double));
+    memcpy(cp->nx, t.nx, m * sizeof(double));
+    memcpy(cp->ny, t.ny, m * sizeof(double));
+    memcpy(cp->id, t.id, m * sizeof(int64_t));
+    memcpy(cp->arc, t.arc, m);
+    cp->n = m;
+    return 1;
+}
+
+/* ball_cut: ConvexPolyhedron2.tcc:534-665 */
+int or_cp2_ball_cut(or_cp2 *cp, double cx, double cy, double r, int64_t id) {
+    cp->scx = cx; cp->scy = cy; cp->sr = r; cp->sid = id;
+    int n = cp->n;
+    if (n == 0) { cp->sr = 0; return 0; }
+    double dist[OR_CP2_CAP];
+    int all_inside = 1;
+    for (int k = 0; k < n; ++k) {
+        double px = cp->x[k] - cx, py = cp->y[k] - cy;
+        dist[k] = px * px + py * py - r * r;
+        all_inside &= dist[k] <= 0;
+    }
+    if (all_inside) return 0;
+    if (2 * n > OR_CP2_CAP) return OR_CAPACITY;
+
+    or_cp2 t;
+    int m = 0;
+#define PUSH(TYPE, ID, NX, NY, X, Y) do { t.arc[m] = (TYPE); t.id[m] = (ID); t.nx[m] = (NX); t.ny[m] = (NY); t.x[m] = (X); t.y[m] = (Y); ++m; } while (0)
+    for (int i0 = 0; i0 < n; ++i0) {
+        int i1 = (i0 + 1) % n;
+        double p0x = cp->x[i0], p0y = cp->y[i0], p1x = cp->x[i1], p1y = cp->y[i1];
+        if (dist[i0] <= 0) {
+            if (dist[i1] <= 0) {
+                PUSH(0, cp->id[i0], cp->nx[i0], cp->ny[i0], p0x, p0y);
+            } else {
+                /* find_unique_intersection(p0, p1) :574-584 */
+                double ex = p1x - p0x, ey = p1y - p0y;
+                double a = ex * ex + ey * ey;
+                double b = (p0x - cx) * ex + (p0y - cy) * ey;
+                double c = (p0x - cx) * (p0x - cx) + (p0y - cy) * (p0y - cy) - r * r;
+                double dd = sqrt(fmax(0.0, b * b - a * c));
+                double u = (-b + dd) / a, v = (-b - dd) / a;
+                double tt = fabs(u - 0.5) <= fabs(v - 0.5) ? u : v;
+                tt = fmin(1.0, fmax(0.0, tt));
+                PUSH(0, cp->id[i0], cp->nx[i0], cp->ny[i0], p0x, p0y);
+                PUSH(1, id, 0.0, 0.0, p0x + tt * ex, p0y + tt * ey);
+            }
+        } else {
+            if (dist[i1] <= 0) {
+                double ex = p0x - p1x, ey = p0y - p1y;   /* unique(p1, p0) */
+                double a = ex * ex + ey * ey;
+                double b = (p1x - cx) * ex + (p1y - cy) * ey;
+                double c = (p1x - cx) * (p1x - cx) + (p1y - cy) * (p1y - cy) - r * r;
+                double dd = sqrt(fmax(0.0, b * b - a * c));
+                double u = (-b + dd) / a, v = (-b - dd) / a;
+                double tt = fabs(u - 0.5) <= fabs(v - 0.5) ? u : v;
+                tt = fmin(1.0, fmax(0.0, tt));
+                PUSH(0, cp->id[i0], cp->nx[i0], cp->ny[i0], p1x + tt * ex, p1y + tt * ey);
+            } else {
+                /* find_two_cuts :586-612 */
+                double ex = p1x - p0x, ey = p1y - p0y;
+                double a = ex * ex + ey * ey;
+                if (a == 0) continue;
+                double b = (p0x - cx) * ex + (p0y - cy) * ey;
+                double c = (p0x - cx) * (p0x - cx) + (p0y - cy) * (p0y - cy) - r * r;
+                double s = b * b - a * c;
+                if (s <= 0) continue;
+                double dd = sqrt(s);
+                double u = (-b - dd) / a, v = (-b + dd) / a;
+                if (u > 0 && u < 1) v = fmax(0.0, fmin(1.0, v));
+                else if (v > 0 && v < 1) u = fmax(0.0, fmin(1.0, u));
+                else continue;
+                double q0x = p0x + u * ex, q0y = p0y + u * ey;
+                double q1x = p0x + v * ex, q1y = p0y + v * ey;
+                double a0 = atan2(q0y - cy, q0x - cx), a1 = atan2(q1y - cy, q1x - cx);
+                if (a0 == a1) continue;
+                PUSH(0, cp->id[i0], cp->nx[i0], cp->ny[i0], q0x, q0y);
+                PUSH(1, id, 0.0, 0.0, q1x, q1y);
+            }
+        }
+    }
+#undef PUSH
+    if (m == 0) {                       /* :646-653 */
+        for (int i0 = 0; i0 < n; ++i0) {
+            if ((cx - cp->x[i0]) * cp->nx[i0] + (cy - cp->y[i0]) * cp->ny[i0] > 0) {
+                cp->sr = 0;
+                break;
+            }
+        }
+    }
+    memcpy(cp->x, t.x, m * sizeof(double));
+    memcpy(cp->y, t.y, m * sizeof(double));
+    memcpy(cp->nx, t.nx, m * sizeof(double));
+    memcpy(cp->ny, t.ny, m * sizeof(double));
+    memcpy(cp->id, t.id, m * sizeof(int64_t));
+    memcpy(cp->arc, t.arc, m);
+    cp->n = m;
+    return 0;
+}
+
+static double arc_angle(const or_cp2 *cp, int i0, int i1) {
+    double a0 = atan2(cp->y[i0] - cp->scy, cp->x[i0] - cp->scx);
+    double a1 = atan2(cp->y[i1] - cp->scy, cp->x[i1] - cp->scx);
+    if (a1 < a0) a1 += 2 * M_PI;
+    return a1 - a0;
+}
+
+/* integration(Constant, Unit): ConvexPolyhedron2.tcc:1791-1842, _arc_area :2361-2380 */
+double or_cp2_area(const or_cp2 *cp, double coeff) {
+    int n = cp->n;
+    if (n == 0) return cp->sr > 0 ? coeff * M_PI * cp->sr * cp->sr : 0.0;
+    double res = 0;
+    double ax = cp->x[0], ay = cp->y[0];
+    for (int i = 2; i < n; ++i) {
+        double bx = cp->x[i - 1], by = cp->y[i - 1], cx = cp->x[i], cy = cp->y[i];
+        res += ax * (by - cy) + bx * (cy - ay) + cx * (ay - by);
+    }
+    res *= 0.5;
+    for (int i0 = n - 1, i1 = 0; i1 < n; i0 = i1++) {
+        if (!cp->arc[i0]) continue;
+        double da = arc_angle(cp, i0, i1);
+        double ex = cp->x[i0] - cp->x[i1], ey = cp->y[i0] - cp->y[i1];
+        double d0 = sqrt(ex * ex + ey * ey) / 2;
+        double d1 = sqrt(fmax(0.0, cp->sr * cp->sr - d0 * d0));
+        double rs = cp->sr * cp->sr * da / 2;
+        if (da > M_PI) rs += d1 * d0;
+        else rs -= d1 * d0;
+        res += rs;
+    }
+    return coeff * res;
+}
+
+/* for_each_boundary_item(Constant, Unit): ConvexPolyhedron2.tcc:437-478 */
+int or_cp2_boundary_items(const or_cp2 *cp, double coeff, double *measures, int64_t *ids, int cap) {
+    int n = cp->n;
+    if (n == 0) {
+        if (cp->sr >= 0 && cap > 0) {
+            measures[0] = coeff * 2 * M_PI * cp->sr;
+            ids[0] = cp->sid;
+            return 1;
+        }
+        return 0;
+    }
+    int m = 0;
+    for (int i1 = 0, i0 = n - 1; i1 < n; i0 = i1++) {
+        if (m >= cap) return -1;
+        ids[m] = cp->id[i0];
+        if (cp->arc[i0]) {
+            measures[m] = coeff * arc_angle(cp, i0, i1) * cp->sr;
+        } else {
+            double ex = cp->x[i1] - cp->x[i0], ey = cp->y[i1] - cp->y[i0];
+            measures[m] = coeff * sqrt(ex * ex + ey * ey);
+        }
+        ++m;
+    }
+    return m;
+}
+
+int or_cp2_eval(double x0, double y0, double x1, double y1, int ncuts, const double *cuts,
+                int ball, double bcx, double bcy, double br,
+                double *area, int *nb_points, double *measures, int64_t *ids, int cap, int *nitems) {
+    or_cp2 *cp = (or_cp2 *)malloc(sizeof(or_cp2));
+    or_cp2_box(cp, x0, y0, x1, y1, -1);
+    int st = OR_OK;
+    for (int k = 0; k < ncuts && st == OR_OK; ++k) {
+        int r = or_cp2_plane_cut(cp, cuts[4 * k + 0], cuts[4 * k + 1], cuts[4 * k + 2], cuts[4 * k + 3], k, 1);
+        if (r == OR_CAPACITY) st = r;
+    }
+    if (ball && st == OR_OK) st = or_cp2_ball_cut(cp, bcx, bcy, br, -2);
+    *area = or_cp2_area(cp, 1.0);
+    *nb_points = cp->n;
+    *nitems = or_cp2_boundary_items(cp, 1.0, measures, ids, cap);
+    free(cp);
+    return st;
+}
+
+/* ------------------------------------------------------------------------- */
+/* 3D polyhedron (ConvexPolyhedron3 semantics)                                */
+/* ------------------------------------------------------------------------- */
+
+static void cp3_set_face(or_cp3 *cp, int f, const double v[4][3], double nx, double ny, double nz, int64_t id) {
+    cp->nv[f] = 4;
+    for (int k = 0; k < 4; ++k)
+        for (int d = 0; d < 3; ++d) cp->v[f][k][d] = v[k][d];
+    cp->n[f][0] = nx; cp->n[f][1] = ny; cp->n[f][2] = nz;
+    cp->id[f] = id;
+}
+
+/* clear(Box): ConvexPolyhedron3.tcc:799-866 -- six faces, all with cut id `id` */
+void or_cp3_box(or_cp3 *cp, const double *p0, const double *p1, int64_t id) {
+    double x0 = p0[0], y0 = p0[1], z0 = p0[2], x1 = p1[0], y1 = p1[1], z1 = p1[2];
+    cp->nf = 6;
+    cp->overflow = 0;
+    { const double v[4][3] = {{x0,y0,z0},{x0,y1,z0},{x1,y1,z0},{x1,y0,z0}}; cp3_set_face(cp, 0, v, 0, 0, -1, id); }
+    { const double v[4][3] = {{x0,y0,z1},{x1,y0,z1},{x1,y1,z1},{x0,y1,z1}}; cp3_set_face(cp, 1, v, 0, 0, +1, id); }
+    { const double v[4][3] = {{x0,y0,z0},{x1,y0,z0},{x1,y0,z1},{x0,y0,z1}}; cp3_set_face(cp, 2, v, 0, -1, 0, id); }
+    { const double v[4][3] = {{x0,y1,z0},{x0,y1,z1},{x1,y1,z1},{x1,y1,z0}}; cp3_set_face(cp, 3, v, 0, +1, 0, id); }
+    { const double v[4][3] = {{x0,y0,z0},{x0,y0,z1},{x0,y1,z1},{x0,y1,z0}}; cp3_set_face(cp, 4, v, -1, 0, 0, id); }
+    { const double v[4][3] = {{x1,y0,z0},{x1,y1,z0},{x1,y1,z1},{x1,y0,z1}}; cp3_set_face(cp, 5, v, +1, 0, 0, id); }
+}
+
+typedef struct { double p[3]; double ang; } cap_pt;
+
+static int cmp_cap(const void *a, const void *b) {
+    double x = ((const cap_pt *)a)->ang, y = ((const cap_pt *)b)->ang;
+    return (x > y) - (x < y);
+}
+
+/* plane_cut: ConvexPolyhedron3.tcc:614-754 semantics.  Keeps p with
+ * (o-p).n >= 0; every face loop is clipped (intersections from the inside
+ * vertex toward the outside one, as in 2D), removed faces vanish, and the new
+ * face is closed from the crossing points.  Returns 1 on an effective cut. */
+int or_cp3_plane_cut(or_cp3 *cp, const double *origin, const double *normal, int64_t id) {
+    double nn = sqrt(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
+    double nx = normal[0] / nn, ny = normal[1] / nn, nz = normal[2] / nn;
+    double off = origin[0] * nx + origin[1] * ny + origin[2] * nz;
+    int any_out = 0, any_in = 0;
+    for (int f = 0; f < cp->nf; ++f)
+        for (int k = 0; k < cp->nv[f]; ++k) {
+            double d = off - (cp->v[f][k][0] * nx + cp->v[f][k][1] * ny + cp->v[f][k][2] * nz);
+            if (d < 0) any_out = 1; else any_in = 1;
+        }
+    if (!any_out) return 0;
+    if (!any_in) { cp->nf = 0; return 0; }
+
+    static __thread cap_pt capp[2 * OR_CP3_MAXF + 8];
+    int ncap = 0;
+    int nf2 = 0;
+    double tv[OR_CP3_MAXFV * 2][3];
+    for (int f = 0; f < cp->nf; ++f) {
+        int nv = cp->nv[f];
+        double d[OR_CP3_MAXFV];
+        int nout = 0;
+        for (int k = 0; k < nv; ++k) {
+            d[k] = off - (cp->v[f][k][0] * nx + cp->v[f][k][1] * ny + cp->v[f][k][2] * nz);
+            nout += d[k] < 0;
+        }
+        if (nout == nv) continue;                 /* face removed */
+        if (nout == 0) {                          /* face kept as is */
+            if (nf2 != f) {
+                cp->nv[nf2] = nv;
+                memcpy(cp->v[nf2], cp->v[f], sizeof(double) * 3 * nv);
+                memcpy(cp->n[nf2], cp->n[f], sizeof(double) * 3);
+                cp->id[nf2] = cp->id[f];
+            }
+            ++nf2;
+            continue;
+        }
+        int m = 0;
+        for (int k = 0; k < nv; ++k) {
+            int km = (k + nv - 1) % nv;
+            int ok = d[k] < 0, okm = d[km] < 0;
+            if (ok != okm) {
+                int ii = ok ? km : k, io = ok ? k : km;
+                double mm = d[ii] / (d[io] - d[ii]);
+                for (int c = 0; c < 3; ++c)
+                    tv[m][c] = cp->v[f][ii][c] - mm * (cp->v[f][io][c] - cp->v[f][ii][c]);
+                if (ncap < 2 * OR_CP3_MAXF + 8) { memcpy(capp[ncap].p, tv[m], sizeof(double) * 3); ++ncap; }
+                ++m;
+            }
+            if (!ok) { memcpy(tv[m], cp->v[f][k], sizeof(double) * 3); ++m; }
+        }
+        if (m > OR_CP3_MAXFV) { cp->overflow = 1; m = OR_CP3_MAXFV; }
+        cp->nv[nf2] = m;
+        memcpy(cp->v[nf2], tv, sizeof(double) * 3 * m);
+        if (nf2 != f) {
+            memcpy(cp->n[nf2], cp->n[f], sizeof(double) * 3);
+            cp->id[nf2] = cp->id[f];
+        }
+        ++nf2;
+    }
+    cp->nf = nf2;
+    if (ncap < 3) return 1;
+    /* new face: unique crossing points ordered counter-clockwise about +n */
+    double cx = 0, cy = 0, cz = 0;
+    for (int k = 0; k < ncap; ++k) { cx += capp[k].p[0]; cy += capp[k].p[1]; cz += capp[k].p[2]; }
+    cx /= ncap; cy /= ncap; cz /= ncap;
+    double ux, uy, uz;
+    if (fabs(nx) < 0.6) { ux = 0; uy = nz; uz = -ny; } else { ux = -nz; uy = 0; uz = nx; }
+    double un = sqrt(ux * ux + uy * uy + uz * uz);
+    ux /= un; uy /= un; uz /= un;
+    double vx = ny * uz - nz * uy, vy = nz * ux - nx * uz, vz = nx * uy - ny * ux;
+    for (int k = 0; k < ncap; ++k) {
+        double px = capp[k].p[0] - cx, py = capp[k].p[1] - cy, pz = capp[k].p[2] - cz;
+        capp[k].ang = atan2(px * vx + py * vy + pz * vz, px * ux + py * uy + pz * uz);
+    }
+    qsort(capp, ncap, sizeof(cap_pt), cmp_cap);
+    if (cp->nf >= OR_CP3_MAXF) { cp->overflow = 1; return 1; }
+    int f = cp->nf++;
+    int m = 0;
+    for (int k = 0; k < ncap; ++k) {
+        if (m && capp[k].p[0] == cp->v[f][m - 1][0] && capp[k].p[1] == cp->v[f][m - 1][1] && capp[k].p[2] == cp->v[f][m - 1][2])
+            continue;
+        if (m >= OR_CP3_MAXFV) { cp->overflow = 1; break; }
+        memcpy(cp->v[f][m], capp[k].p, sizeof(double) * 3);
+        ++m;
+    }
+    while (m > 1 && cp->v[f][m - 1][0] == cp->v[f][0][0] && cp->v[f][m - 1][1] == cp->v[f][0][1] && cp->v[f][m - 1][2] == cp->v[f][0][2])
+        --m;
+    cp->nv[f] = m;
+    cp->n[f][0] = nx; cp->n[f][1] = ny; cp->n[f][2] = nz;
+    cp->id[f] = id;
+    return 1;
+}
+
+/* area(Face): ConvexPolyhedron3.tcc:1367-1418 -- fan of cross products . n */
+double or_cp3_face_area(const or_cp3 *cp, int f) {
+    double s = 0;
+    const double (*v)[3] = cp->v[f];
+    for (int k = 2; k < cp->nv[f]; ++k) {
+        double ax = v[k - 1][0] - v[0][0], ay = v[k - 1][1] - v[0][1], az = v[k - 1][2] - v[0][2];
+        double bx = v[k][0] - v[0][0], by = v[k][1] - v[0][1], bz = v[k][2] - v[0][2];
+        s += (ay * bz - az * by) * cp->n[f][0] + (az * bx - ax * bz) * cp->n[f][1] + (ax * by - ay * bx) * cp->n[f][2];
+    }
+    return s / 2;
+}
+
+/* integration(Constant, Unit): ConvexPolyhedron3.tcc:1022-1056, taken about `center` */
+double or_cp3_volume(const or_cp3 *cp, const double *center) {
+    double res = 0;
+    for (int f = 0; f < cp->nf; ++f) {
+        if (cp->nv[f] < 3) continue;
+        double h = (cp->v[f][0][0] - center[0]) * cp->n[f][0] + (cp->v[f][0][1] - center[1]) * cp->n[f][1] +
+                   (cp->v[f][0][2] - center[2]) * cp->n[f][2];
+        res += h * or_cp3_face_area(cp, f) / 3;
+    }
+    return res;
+}
+
+int or_cp3_eval(const double *p0, const double *p1, int ncuts, const double *cuts,
+                double *volume, int *nb_faces, double *areas, int64_t *ids, int cap) {
+    or_cp3 *cp = (or_cp3 *)malloc(sizeof(or_cp3));
+    or_cp3_box(cp, p0, p1, -1);
+    for (int k = 0; k < ncuts; ++k) or_cp3_plane_cut(cp, cuts + 6 * k, cuts + 6 * k + 3, k);
+    double c[3] = {0, 0, 0};
+    *volume = or_cp3_volume(cp, c);
+    int m = 0;
+    for (int f = 0; f < cp->nf && m < cap; ++f) {
+        if (cp->nv[f] < 3) continue;
+        areas[m] = or_cp3_face_area(cp, f);
+        ids[m] = cp->id[f];
+        ++m;
+    }
+    *nb_faces = m;
+    int st = cp->overflow ? OR_CAPACITY : OR_OK;
+    free(cp);
+    return st;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Neighbour search: uniform grid with the SpZGrid eviction rule              */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    int dim;
+    int64_t ncell[3];
+    double lo[3], h[3];
+    int64_t *start;    /* ncells + 1 */
+    int64_t *items;    /* N dirac indices, grouped per cell, ascending within cell */
+    double *wmax;      /* per cell max weight */
+    double wglob;
+} or_grid;
+
+static int64_t grid_coord(const or_grid *g, int d, double x) {
+    int64_t c = (int64_t)floor((x - g->lo[d]) / g->h[d]);
+    if (c < 0) c = 0;
+    if (c >= g->ncell[d]) c = g->ncell[d] - 1;
+    return c;
+}
+
+static void grid_build(or_grid *g, int dim, int64_t N, const double *pos, const double *w) {
+    g->dim = dim;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) { lo[d] = INFINITY; hi[d] = -INFINITY; }
+    for (int64_t i = 0; i < N; ++i)
+        for (int d = 0; d < dim; ++d) {
+            lo[d] = fmin(lo[d], pos[dim * i + d]);
+            hi[d] = fmax(hi[d], pos[dim * i + d]);
+        }
+    double vol = 1;
+    for (int d = 0; d < dim; ++d) {
+        double l = hi[d] - lo[d];
+        if (!(l > 0)) l = 1e-300;
+        double pad = 1e-9 * l + 1e-300;
+        lo[d] -= pad; hi[d] += pad;
+        vol *= hi[d] - lo[d];
+    }
+    double h = pow(vol * 2.0 / (double)(N > 0 ? N : 1), 1.0 / dim);
+    int64_t tot = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (d < dim) {
+            int64_t nc = (int64_t)floor((hi[d] - lo[d]) / h);
+            if (nc < 1) nc = 1;
+            if (nc > 4096) nc = 4096;
+            g->ncell[d] = nc;
+            g->lo[d] = lo[d];
+            g->h[d] = (hi[d] - lo[d]) / nc;
+        } else {
+            g->ncell[d] = 1; g->lo[d] = 0; g->h[d] = 1;
+        }
+        tot *= g->ncell[d];
+    }
+    g->start = (int64_t *)calloc(tot + 1, sizeof(int64_t));
+    g->items = (int64_t *)malloc((N > 0 ? N : 1) * sizeof(int64_t));
+    g->wmax = (double *)malloc(tot * sizeof(double));
+    int64_t *cell = (int64_t *)malloc((N > 0 ? N : 1) * sizeof(int64_t));
+    for (int64_t i = 0; i < N; ++i) {
+        int64_t c = 0, mul = 1;
+        for (int d = 0; d < dim; ++d) { c += mul * grid_coord(g, d, pos[dim * i + d]); mul *= g->ncell[d]; }
+        cell[i] = c;
+        g->start[c + 1]++;
+    }
+    for (int64_t c = 0; c < tot; ++c) g->start[c + 1] += g->start[c];
+    int64_t *cur = (int64_t *)malloc(tot * sizeof(int64_t));
+    memcpy(cur, g->start, tot * sizeof(int64_t));
+    for (int64_t i = 0; i < N; ++i) g->items[cur[cell[i]]++] = i;   /* ascending i within cell */
+    g->wglob = -INFINITY;
+    for (int64_t c = 0; c < tot; ++c) {
+        double m = -INFINITY;
+        for (int64_t k = g->start[c]; k < g->start[c + 1]; ++k) m = fmax(m, w[g->items[k]]);
+        g->wmax[c] = m;
+        g->wglob = fmax(g->wglob, m);
+    }
+    free(cur);
+    free(cell);
+}
+
+static void grid_free(or_grid *g) { free(g->start); free(g->items); free(g->wmax); }
+
+/* ------------------------------------------------------------------------- */
+/* Cell evaluation                                                            */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    int dim;
+    int64_t N;
+    const double *pos, *w;
+    const or_domain *dom;
+    int flags;
+    double *masses, *ball_arc, *perimeter;
+    int64_t cap;
+    int64_t *row_len, *cols;
+    double *vals;
+    or_grid grid;
+    int64_t next;          /* shared chunk counter */
+    pthread_mutex_t mtx;
+    int status;
+} cells_job;
+
+typedef struct {
+    int64_t *cols;
+    double *vals;
+    int64_t len, cap;
+    double der0;
+    int overflow;
+} row_acc;
+
+static void row_add(row_acc *r, int64_t j, double v) {
+    for (int64_t k = 1; k < r->len; ++k)
+        if (r->cols[k] == j) { r->vals[k] += v; return; }
+    if (r->len >= r->cap) { r->overflow = 1; return; }
+    r->cols[r->len] = j;
+    r->vals[r->len] = v;
+    r->len++;
+}
+
+static double dist_between(const double *pos, int dim, int64_t i, int64_t j) {
+    double s = 0;
+    for (int d = 0; d < dim; ++d) { double e = pos[dim * i + d] - pos[dim * j + d]; s += e * e; }
+    return sqrt(s);
+}
+
+/* ---- 2D ---- */
+
+/* the per-piece part of the get_der_integrals_wrt_weights callback
+ * (get_der_integrals_wrt_weights.h:52-75) for one 2D piece of density coeff */
+static void piece2_accumulate(const cells_job *J, int64_t i, const or_cp2 *cp, double coeff, double *mass, double *arcm,
+                              double *perim, row_acc *row) {
+    static __thread double meas[2 * OR_CP2_CAP + 2];
+    static __thread int64_t ids[2 * OR_CP2_CAP + 2];
+    *mass += or_cp2_area(cp, coeff);
+    int m = or_cp2_boundary_items(cp, coeff, meas, ids, 2 * OR_CP2_CAP + 2);
+    double wi = J->w[i];
+    for (int k = 0; k < m; ++k) {
+        *perim += meas[k];
+        if (ids[k] == -1) continue;
+        if (ids[k] == i) {
+            *arcm += meas[k];
+            row->der0 += 0.5 * meas[k] / sqrt(wi);
+        } else {
+            double dd = dist_between(J->pos, 2, i, ids[k]);
+            if (dd != 0) {
+                double b = 0.5 * meas[k] / dd;
+                row_add(row, ids[k], -b);
+                row->der0 += b;
+            }
+        }
+    }
+}
+
+static void power_cut2(or_cp2 *cp, const double *pos, const double *w, int64_t i, int64_t j) {
+    double c0x = pos[2 * i], c0y = pos[2 * i + 1];
+    double vx = pos[2 * j] - c0x, vy = pos[2 * j + 1] - c0y;
+    double n = vx * vx + vy * vy;
+    double x = 0.5 + 0.5 * (w[i] - w[j]) / n;                /* SpZGrid.tcc:506-522 */
+    or_cp2_plane_cut(cp, c0x + x * vx, c0y + x * vy, vx, vy, j, 0);
+}
+
+/* can_be_evicted, SpZGrid.tcc:475-480: the box cannot cut if, for every vertex p,
+ * |c0-p|^2 - w0 <= |clamp(p,box)-p|^2 - wmax */
+static int evict2(const or_cp2 *cp, double c0x, double c0y, double w0, double bx0, double by0, double bx1, double by1,
+                  double wmax) {
+    for (int k = 0; k < cp->n; ++k) {
+        double px = cp->x[k], py = cp->y[k];
+        double qx = fmin(bx1, fmax(bx0, px)) - px, qy = fmin(by1, fmax(by0, py)) - py;
+        double ax = c0x - px, ay = c0y - py;
+        if (!(ax * ax + ay * ay - w0 <= qx * qx + qy * qy - wmax)) return 0;
+    }
+    return 1;
+}
+
+static int build_cell2(const cells_job *J, int64_t i, or_cp2 *cp) {
+    const or_domain *dom = J->dom;
+    const double *pos = J->pos, *w = J->w;
+    or_cp2_box(cp, dom->p0[0], dom->p0[1], dom->p1[0], dom->p1[1], -1);
+    double c0x = pos[2 * i], c0y = pos[2 * i + 1], w0 = w[i];
+    if (J->flags & 2) {
+        for (int64_t j = 0; j < J->N; ++j)
+            if (j != i) power_cut2(cp, pos, w, i, j);
+    } else {
+        const or_grid *g = &J->grid;
+        int64_t hx = grid_coord(g, 0, c0x), hy = grid_coord(g, 1, c0y);
+        int64_t maxr = g->ncell[0] > g->ncell[1] ? g->ncell[0] : g->ncell[1];
+        for (int64_t r = 0; r <= maxr && cp->n; ++r) {
+            if (r > 0) {
+                /* every dirac in rings >= r lies outside the box of rings < r */
+                double rx0 = g->lo[0] + (hx - r + 1) * g->h[0], rx1 = g->lo[0] + (hx + r) * g->h[0];
+                double ry0 = g->lo[1] + (hy - r + 1) * g->h[1], ry1 = g->lo[1] + (hy + r) * g->h[1];
+                int covers = hx - r + 1 <= 0 && hy - r + 1 <= 0 && hx + r >= g->ncell[0] && hy + r >= g->ncell[1];
+                if (covers) break;
+                int stop = 1;
+                for (int k = 0; k < cp->n && stop; ++k) {
+                    double px = cp->x[k], py = cp->y[k];
+                    double dm = INFINITY;
+                    if (hx - r + 1 > 0) dm = fmin(dm, px - rx0);
+                    if (hx + r < g->ncell[0]) dm = fmin(dm, rx1 - px);
+                    if (hy - r + 1 > 0) dm = fmin(dm, py - ry0);
+                    if (hy + r < g->ncell[1]) dm = fmin(dm, ry1 - py);
+                    if (dm < 0) dm = 0;
+                    double ax = c0x - px, ay = c0y - py;
+                    if (!(ax * ax + ay * ay - w0 <= dm * dm - g->wglob)) stop = 0;
+                }
+                if (stop) break;
+            }
+            for (int64_t gy = hy - r; gy <= hy + r; ++gy) {
+                if (gy < 0 || gy >= g->ncell[1]) continue;
+                for (int64_t gx = hx - r; gx <= hx + r; ++gx) {
+                    if (gx < 0 || gx >= g->ncell[0]) continue;
+                    if (llabs(gx - hx) != r && llabs(gy - hy) != r) continue;
+                    int64_t c = gy * g->ncell[0] + gx;
+                    if (g->start[c] == g->start[c + 1]) continue;
+                    double bx0 = g->lo[0] + gx * g->h[0], by0 = g->lo[1] + gy * g->h[1];
+                    if (evict2(cp, c0x, c0y, w0, bx0, by0, bx0 + g->h[0], by0 + g->h[1], g->wmax[c])) continue;
+                    for (int64_t k = g->start[c]; k < g->start[c + 1]; ++k) {
+                        int64_t j = g->items[k];
+                        if (j != i) power_cut2(cp, pos, w, i, j);
+                    }
+                }
+            }
+        }
+    }
+    if (J->flags & 1) or_cp2_ball_cut(cp, c0x, c0y, sqrt(w0), i);
+    else { cp->scx = c0x; cp->scy = c0y; }
+    return or_cp2_empty(cp) ? OR_EMPTY_CELL : OR_OK;
+}
+
+static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
+    row_acc row = { J->cols + i * J->cap, J->vals + i * J->cap, 1, J->cap, 0.0, 0 };
+    row.cols[0] = i;
+    double mass = 0, arcm = 0, perim = 0;
+    int st = build_cell2(J, i, cp);
+    const or_domain *dom = J->dom;
+    if (!dom->img) {
+        piece2_accumulate(J, i, cp, dom->coeff, &mass, &arcm, &perim, &row);
+    } else {
+        /* ScaledImage::for_each_intersection, ScaledImage.tcc:98-149 */
+        double mn[2], mx[2];
+        if (cp->n == 0) {
+            double r = cp->sr > 0 ? cp->sr : 0;
+            mn[0] = cp->scx - r; mx[0] = cp->scx + r; mn[1] = cp->scy - r; mx[1] = cp->scy + r;
+        } else {
+            mn[0] = mx[0] = cp->x[0]; mn[1] = mx[1] = cp->y[0];
+            for (int k = 1; k < cp->n; ++k) {
+                mn[0] = fmin(mn[0], cp->x[k]); mx[0] = fmax(mx[0], cp->x[k]);
+                mn[1] = fmin(mn[1], cp->y[k]); mx[1] = fmax(mx[1], cp->y[k]);
+            }
+        }
+        int64_t mi[2], ma[2];
+        double ps[2];
+        for (int d = 0; d < 2; ++d) {
+            double L = dom->p1[d] - dom->p0[d];
+            int64_t S = dom->img_sizes[d];
+            double a = fmax(mn[d], dom->p0[d]), b = fmin(mx[d], dom->p1[d]);
+            int64_t ia = (int64_t)((a - dom->p0[d]) * S / L), ib = (int64_t)((b - dom->p0[d]) * S / L);
+            if (ia < 0) ia = 0;
+            if (ib < 0) ib = 0;
+            mi[d] = (ia > 1 ? ia : 1) - 1;
+            ma[d] = ib + 2 < S ? ib + 2 : S;
+            ps[d] = L / S;
+        }
+        if (!or_cp2_empty(cp)) {
+            for (int64_t py = mi[1]; py < ma[1]; ++py)
+                for (int64_t px = mi[0]; px < ma[0]; ++px) {
+                    double rho = dom->img[py * dom->img_sizes[0] + px];
+                    or_cp2_box(piece, dom->p0[0] + ps[0] * px, dom->p0[1] + ps[1] * py,
+                               dom->p0[0] + ps[0] * (px + 1), dom->p0[1] + ps[1] * (py + 1), -1);
+                    /* intersect_with, ConvexPolyhedron2.tcc:2475-2497 */
+                    for (int k = 0; k < cp->n; ++k)
+                        or_cp2_plane_cut(piece, cp->x[k], cp->y[k], cp->nx[k], cp->ny[k], cp->id[k], 1);
+                    if (cp->sr > 0) or_cp2_ball_cut(piece, cp->scx, cp->scy, cp->sr, cp->sid);
+                    else { piece->scx = cp->scx; piece->scy = cp->scy; piece->sr = cp->sr; piece->sid = cp->sid; }
+                    piece2_accumulate(J, i, piece, rho, &mass, &arcm, &perim, &row);
+                }
+        }
+    }
+    row.vals[0] = row.der0;
+    J->masses[i] = mass;
+    if (J->ball_arc) J->ball_arc[i] = arcm;
+    if (J->perimeter) J->perimeter[i] = perim;
+    J->row_len[i] = row.len;
+    if (row.overflow) st = OR_CAPACITY;
+    if (st == OR_OK && mass == 0) st = OR_EMPTY_CELL;      /* get_der_integrals_wrt_weights.h:93-96 */
+    if (st != OR_OK) {
+        pthread_mutex_lock(&J->mtx);
+        if (J->status == OR_OK || st == OR_CAPACITY) J->status = st;
+        pthread_mutex_unlock(&J->mtx);
+    }
+}
+
+/* ---- 3D ---- */
+
+static void power_cut3(or_cp3 *cp, const double *pos, const double *w, int64_t i, int64_t j) {
+    const double *c0 = pos + 3 * i, *c1 = pos + 3 * j;
+    double V[3] = {c1[0] - c0[0], c1[1] - c0[1], c1[2] - c0[2]};
+    double n = V[0] * V[0] + V[1] * V[1] + V[2] * V[2];
+    double x = 0.5 + 0.5 * (w[i] - w[j]) / n;
+    double o[3] = {c0[0] + x * V[0], c0[1] + x * V[1], c0[2] + x * V[2]};
+    or_cp3_plane_cut(cp, o, V, j);
+}
+
+static int evict3(const or_cp3 *cp, const double *c0, double w0, const double *b0, const double *b1, double wmax) {
+    for (int f = 0; f < cp->nf; ++f)
+        for (int k = 0; k < cp->nv[f]; ++k) {
+            const double *p = cp->v[f][k];
+            double q = 0, a = 0;
+            for (int d = 0; d < 3; ++d) {
+                double e = fmin(b1[d], fmax(b0[d], p[d])) - p[d];
+                q += e * e;
+                double g = c0[d] - p[d];
+                a += g * g;
+            }
+            if (!(a - w0 <= q - wmax)) return 0;
+        }
+    return 1;
+}
+
+static int build_cell3(const cells_job *J, int64_t i, or_cp3 *cp) {
+    const or_domain *dom = J->dom;
+    const double *pos = J->pos, *w = J->w;
+    or_cp3_box(cp, dom->p0, dom->p1, -1);
+    const double *c0 = pos + 3 * i;
+    double w0 = w[i];
+    if (J->flags & 2) {
+        for (int64_t j = 0; j < J->N && cp->nf; ++j)
+            if (j != i) power_cut3(cp, pos, w, i, j);
+    } else {
+        const or_grid *g = &J->grid;
+        int64_t h[3];
+        for (int d = 0; d < 3; ++d) h[d] = grid_coord(g, d, c0[d]);
+        int64_t maxr = g->ncell[0];
+        if (g->ncell[1] > maxr) maxr = g->ncell[1];
+        if (g->ncell[2] > maxr) maxr = g->ncell[2];
+        for (int64_t r = 0; r <= maxr && cp->nf; ++r) {
+            if (r > 0) {
+                int covers = 1;
+                for (int d = 0; d < 3; ++d) covers &= h[d] - r + 1 <= 0 && h[d] + r >= g->ncell[d];
+                if (covers) break;
+                int stop = 1;
+                for (int f = 0; f < cp->nf && stop; ++f)
+                    for (int k = 0; k < cp->nv[f] && stop; ++k) {
+                        const double *p = cp->v[f][k];
+                        double dm = INFINITY, a = 0;
+                        for (int d = 0; d < 3; ++d) {
+                            if (h[d] - r + 1 > 0) dm = fmin(dm, p[d] - (g->lo[d] + (h[d] - r + 1) * g->h[d]));
+                            if (h[d] + r < g->ncell[d]) dm = fmin(dm, g->lo[d] + (h[d] + r) * g->h[d] - p[d]);
+                            double e = c0[d] - p[d];
+                            a += e * e;
+                        }
+                        if (dm < 0) dm = 0;
+                        if (!(a - w0 <= dm * dm - g->wglob)) stop = 0;
+                    }
+                if (stop) break;
+            }
+            for (int64_t gz = h[2] - r; gz <= h[2] + r; ++gz) {
+                if (gz < 0 || gz >= g->ncell[2]) continue;
+                for (int64_t gy = h[1] - r; gy <= h[1] + r; ++gy) {
+                    if (gy < 0 || gy >= g->ncell[1]) continue;
+                    for (int64_t gx = h[0] - r; gx <= h[0] + r; ++gx) {
+                        if (gx < 0 || gx >= g->ncell[0]) continue;
+                        if (llabs(gx - h[0]) != r && llabs(gy - h[1]) != r && llabs(gz - h[2]) != r) continue;
+                        int64_t c = (gz * g->ncell[1] + gy) * g->ncell[0] + gx;
+                        if (g->start[c] == g->start[c + 1]) continue;
+                        double b0[3] = {g->lo[0] + gx * g->h[0], g->lo[1] + gy * g->h[1], g->lo[2] + gz * g->h[2]};
+                        double b1[3] = {b0[0] + g->h[0], b0[1] + g->h[1], b0[2] + g->h[2]};
+                        if (evict3(cp, c0, w0, b0, b1, g->wmax[c])) continue;
+                        for (int64_t k = g->start[c]; k < g->start[c + 1]; ++k) {
+                            int64_t j = g->items[k];
+                            if (j != i) power_cut3(cp, pos, w, i, j);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (cp->overflow) return OR_CAPACITY;
+    return cp->nf == 0 ? OR_EMPTY_CELL : OR_OK;             /* ConvexPolyhedron3.h:142 */
+}
+
+static void piece3_accumulate(const cells_job *J, int64_t i, const or_cp3 *cp, double coeff, double *mass, double *perim,
+                              row_acc *row) {
+    *mass += coeff * or_cp3_volume(cp, J->pos + 3 * i);
+    for (int f = 0; f < cp->nf; ++f) {
+        if (cp->nv[f] < 3) continue;
+        double a = coeff * or_cp3_face_area(cp, f);
+        *perim += a;
+        int64_t j = cp->id[f];
+        if (j == -1) continue;
+        double dd = dist_between(J->pos, 3, i, j);
+        if (dd != 0) {
+            double b = 0.5 * a / dd;
+            row_add(row, j, -b);
+            row->der0 += b;
+        }
+    }
+}
+
+static void eval_cell3(cells_job *J, int64_t i, or_cp3 *cp, or_cp3 *piece) {
+    row_acc row = { J->cols + i * J->cap, J->vals + i * J->cap, 1, J->cap, 0.0, 0 };
+    row.cols[0] = i;
+    double mass = 0, perim = 0;
+    int st = build_cell3(J, i, cp);
+    const or_domain *dom = J->dom;
+    if (!dom->img) {
+        piece3_accumulate(J, i, cp, dom->coeff, &mass, &perim, &row);
+    } else if (cp->nf) {
+        double mn[3], mx[3];
+        for (int d = 0; d < 3; ++d) { mn[d] = INFINITY; mx[d] = -INFINITY; }
+        for (int f = 0; f < cp->nf; ++f)
+            for (int k = 0; k < cp->nv[f]; ++k)
+                for (int d = 0; d < 3; ++d) { mn[d] = fmin(mn[d], cp->v[f][k][d]); mx[d] = fmax(mx[d], cp->v[f][k][d]); }
+        int64_t mi[3], ma[3];
+        double ps[3];
+        for (int d = 0; d < 3; ++d) {
+            double L = dom->p1[d] - dom->p0[d];
+            int64_t S = dom->img_sizes[d];
+            double a = fmax(mn[d], dom->p0[d]), b = fmin(mx[d], dom->p1[d]);
+            int64_t ia = (int64_t)((a - dom->p0[d]) * S / L), ib = (int64_t)((b - dom->p0[d]) * S / L);
+            if (ia < 0) ia = 0;
+            if (ib < 0) ib = 0;
+            mi[d] = (ia > 1 ? ia : 1) - 1;
+            ma[d] = ib + 2 < S ? ib + 2 : S;
+            ps[d] = L / S;
+        }
+        for (int64_t pz = mi[2]; pz < ma[2]; ++pz)
+            for (int64_t py = mi[1]; py < ma[1]; ++py)
+                for (int64_t px = mi[0]; px < ma[0]; ++px) {
+                    double rho = dom->img[(pz * dom->img_sizes[1] + py) * dom->img_sizes[0] + px];
+                    double q0[3] = {dom->p0[0] + ps[0] * px, dom->p0[1] + ps[1] * py, dom->p0[2] + ps[2] * pz};
+                    double q1[3] = {dom->p0[0] + ps[0] * (px + 1), dom->p0[1] + ps[1] * (py + 1), dom->p0[2] + ps[2] * (pz + 1)};
+                    or_cp3_box(piece, q0, q1, -1);
+                    /* intersect_with, ConvexPolyhedron3.tcc:156-160: cut by every face plane of the cell */
+                    for (int f = 0; f < cp->nf && piece->nf; ++f) {
+                        if (cp->nv[f] < 3) continue;
+                        or_cp3_plane_cut(piece, cp->v[f][0], cp->n[f], cp->id[f]);
+                    }
+                    if (piece->nf) piece3_accumulate(J, i, piece, rho, &mass, &perim, &row);
+                }
+    }
+    row.vals[0] = row.der0;
+    J->masses[i] = mass;
+    if (J->ball_arc) J->ball_arc[i] = 0;
+    if (J->perimeter) J->perimeter[i] = perim;
+    J->row_len[i] = row.len;
+    if (row.overflow) st = OR_CAPACITY;
+    if (st == OR_OK && mass == 0) st = OR_EMPTY_CELL;
+    if (st != OR_OK) {
+        pthread_mutex_lock(&J->mtx);
+        if (J->status == OR_OK || st == OR_CAPACITY) J->status = st;
+        pthread_mutex_unlock(&J->mtx);
+    }
+}
+
+static void *cells_worker(void *arg) {
+    cells_job *J = (cells_job *)arg;
+    void *a = NULL, *b = NULL;
+    if (J->dim == 2) { a = malloc(sizeof(or_cp2)); b = malloc(sizeof(or_cp2)); }
+    else { a = malloc(sizeof(or_cp3)); b = malloc(sizeof(or_cp3)); }
+    const int64_t chunk = 64;
+    for (;;) {
+        int64_t beg = __atomic_fetch_add(&J->next, chunk, __ATOMIC_RELAXED);
+        if (beg >= J->N) break;
+        int64_t end = beg + chunk < J->N ? beg + chunk : J->N;
+        for (int64_t i = beg; i < end; ++i) {
+            if (J->dim == 2) eval_cell2(J, i, (or_cp2 *)a, (or_cp2 *)b);
+            else eval_cell3(J, i, (or_cp3 *)a, (or_cp3 *)b);
+        }
+    }
+    free(a);
+    free(b);
+    return NULL;
+}
+
+int or_cells(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom,
+             int flags, int nthreads,
+             double *masses, double *ball_arc, double *perimeter,
+             int64_t cap, int64_t *row_len, int64_t *cols, double *vals) {
+    if ((dim != 2 && dim != 3) || N < 0 || cap < 1 || !dom || dom->dim != dim) return OR_INVALID;
+    if (dim == 3 && (flags & 1)) return OR_INVALID;  /* ConvexPolyhedron3::ball_cut is TODO (:292-293) */
+    if (N == 0) return OR_OK;
+    cells_job J;
+    memset(&J, 0, sizeof(J));
+    J.dim = dim; J.N = N; J.pos = pos; J.w = w; J.dom = dom; J.flags = flags;
+    J.masses = masses; J.ball_arc = ball_arc; J.perimeter = perimeter;
+    J.cap = cap; J.row_len = row_len; J.cols = cols; J.vals = vals;
+    J.next = 0;
+    J.status = OR_OK;
+    pthread_mutex_init(&J.mtx, NULL);
+    if (!(flags & 2)) grid_build(&J.grid, dim, N, pos, w);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    for (int t = 1; t < nthreads; ++t) pthread_create(&th[t], NULL, cells_worker, &J);
+    cells_worker(&J);
+    for (int t = 1; t < nthreads; ++t) pthread_join(th[t], NULL);
+    if (!(flags & 2)) grid_free(&J.grid);
+    pthread_mutex_destroy(&J.mtx);
+    return J.status;
+}
+
+/* ------------------------------------------------------------------------- */
+/* CSR assembly, PCG, Newton                                                  */
+/* ------------------------------------------------------------------------- */
+
+typedef struct { int64_t c; double v; } cv_pair;
+
+static int cmp_cv(const void *a, const void *b) {
+    const cv_pair *x = (const cv_pair *)a, *y = (const cv_pair *)b;
+    if (x->c != y->c) return (x->c > y->c) - (x->c < y->c);
+    return (x->v > y->v) - (x->v < y->v);
+}
+
+int64_t or_rows_to_csr(int64_t N, int64_t cap, const int64_t *row_len, const int64_t *rcols, const double *rvals,
+                       int64_t *offsets, int64_t *cols, double *vals) {
+    cv_pair *tmp = (cv_pair *)malloc(sizeof(cv_pair) * (cap > 0 ? cap : 1));
+    int64_t nnz = 0;
+    for (int64_t i = 0; i < N; ++i) {
+        offsets[i] = nnz;
+        int64_t L = row_len[i];
+        for (int64_t k = 0; k < L; ++k) { tmp[k].c = rcols[i * cap + k]; tmp[k].v = rvals[i * cap + k]; }
+        qsort(tmp, L, sizeof(cv_pair), cmp_cv);                  /* std::sort of (col,val) pairs */
+        for (int64_t k = 0; k < L; ++k) {
+            if (k + 1 < L && tmp[k].c == tmp[k + 1].c) { tmp[k + 1].v += tmp[k].v; continue; }
+            cols[nnz] = tmp[k].c;
+            vals[nnz] = tmp[k].v;
+            ++nnz;
+        }
+    }
+    offsets[N] = nnz;
+    free(tmp);
+    return nnz;
+}
+
+static void spmv(int64_t N, const int64_t *off, const int64_t *cols, const double *vals, const double *x, double *y) {
+    for (int64_t i = 0; i < N; ++i) {
+        double s = 0;
+        for (int64_t k = off[i]; k < off[i + 1]; ++k) s += vals[k] * x[cols[k]];
+        y[i] = s;
+    }
+}
+
+int or_pcg(int64_t N, const int64_t *offsets, const int64_t *cols, const double *vals,
+           const double *b, double *x, double rtol, int max_iter, int *iters) {
+    double *r = (double *)malloc(sizeof(double) * N), *z = (double *)malloc(sizeof(double) * N);
+    double *p = (double *)malloc(sizeof(double) * N), *q = (double *)malloc(sizeof(double) * N);
+    double *dinv = (double *)malloc(sizeof(double) * N);
+    for (int64_t i = 0; i < N; ++i) {
+        double d = 0;
+        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k)
+            if (cols[k] == i) d += vals[k];
+        dinv[i] = d != 0 ? 1 / d : 1;
+        x[i] = 0;
+        r[i] = b[i];
+    }
+    double bn = 0;
+    for (int64_t i = 0; i < N; ++i) bn += b[i] * b[i];
+    bn = sqrt(bn);
+    int it = 0, st = OR_OK;
+    if (bn == 0) { *iters = 0; goto done; }
+    double rz = 0;
+    for (int64_t i = 0; i < N; ++i) { z[i] = dinv[i] * r[i]; p[i] = z[i]; rz += r[i] * z[i]; }
+    for (it = 0; it < max_iter; ++it) {
+        spmv(N, offsets, cols, vals, p, q);
+        double pq = 0;
+        for (int64_t i = 0; i < N; ++i) pq += p[i] * q[i];
+        double alpha = rz / pq;
+        double rn = 0;
+        for (int64_t i = 0; i < N; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; rn += r[i] * r[i]; }
+        if (sqrt(rn) <= rtol * bn) { ++it; break; }
+        double rz2 = 0;
+        for (int64_t i = 0; i < N; ++i) { z[i] = dinv[i] * r[i]; rz2 += r[i] * z[i]; }
+        double beta = rz2 / rz;
+        rz = rz2;
+        for (int64_t i = 0; i < N; ++i) p[i] = z[i] + beta * p[i];
+    }
+    if (it >= max_iter) st = OR_NOT_CONVERGED;
+    *iters = it;
+done:
+    free(r); free(z); free(p); free(q); free(dinv);
+    return st;
+}
+
+int or_ot_solve(int dim, int64_t N, const double *pos, double *w, const double *targets,
+                const or_domain *dom, int flags, int nthreads, double tol, int stop_rule,
+                int max_iter, or_solve_stats *stats) {
+    const int64_t cap = dim == 2 ? 64 : 160;
+    double *old = (double *)malloc(sizeof(double) * N), *m = (double *)malloc(sizeof(double) * N);
+    double *dw = (double *)malloc(sizeof(double) * N), *b = (double *)malloc(sizeof(double) * N);
+    int64_t *rl = (int64_t *)malloc(sizeof(int64_t) * N), *rc = (int64_t *)malloc(sizeof(int64_t) * N * cap);
+    double *rv = (double *)malloc(sizeof(double) * N * cap);
+    int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (N + 1)), *cols = (int64_t *)malloc(sizeof(int64_t) * N * cap);
+    double *vals = (double *)malloc(sizeof(double) * N * cap);
+    memcpy(old, w, sizeof(double) * N);
+    or_solve_stats st;
+    memset(&st, 0, sizeof(st));
+    st.max_rel_mass_err = INFINITY;
+    int status = OR_NOT_CONVERGED;
+    for (int it = 0; it < max_iter; ++it) {
+        int err = or_cells(dim, N, pos, w, dom, flags, nthreads, m, NULL, NULL, cap, rl, rc, rv);
+        st.newton_iters = it + 1;
+        if (err == OR_CAPACITY || err == OR_INVALID) { status = err; break; }
+        if (err) {                                   /* OptimalTransportSolver.tcc:72-79 */
+            for (int64_t i = 0; i < N; ++i) w[i] = 0.9 * old[i] + 0.1 * w[i];
+            st.backtracks++;
+            continue;
+        }
+        memcpy(old, w, sizeof(double) * N);
+        double rel = 0;
+        for (int64_t i = 0; i < N; ++i) {
+            b[i] = m[i] - targets[i];
+            rel = fmax(rel, fabs(b[i]) / targets[i]);
+        }
+        st.max_rel_mass_err = rel;
+        if (stop_rule == 0 && rel < tol) { status = OR_OK; break; }
+        st.nnz = or_rows_to_csr(N, cap, rl, rc, rv, off, cols, vals);
+        if (!(flags & 1)) vals[off[0]] *= 2;          /* :62-63 gauge fix */
+        int pit = 0;
+        double rtol = fmax(1e-14, fmin(1e-3, 1e-3 * rel));
+        or_pcg(N, off, cols, vals, b, dw, rtol, 100000, &pit);
+        st.pcg_iters_total += pit;
+        double mdw = 0;
+        for (int64_t i = 0; i < N; ++i) { mdw = fmax(mdw, fabs(dw[i])); w[i] -= dw[i]; }
+        st.max_abs_dw = mdw;
+        if (stop_rule == 1 && mdw < tol) { status = OR_OK; break; }
+    }
+    if (stats) *stats = st;
+    free(old); free(m); free(dw); free(b); free(rl); free(rc); free(rv); free(off); free(cols); free(vals);
+    return status;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Inputs: splitmix64 -> xoshiro256** (Blackman & Vigna)                      */
+/* ------------------------------------------------------------------------- */
+
+static uint64_t splitmix64(uint64_t *x) {
+    uint64_t z = (*x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+void or_rng_uniform(uint64_t seed, int64_t n, double *out) {
+    uint64_t s[4], sm = seed;
+    for (int k = 0; k < 4; ++k) s[k] = splitmix64(&sm);
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t r = rotl(s[1] * 5, 7) * 9;
+        uint64_t t = s[1] << 17;
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        out[i] = (double)(r >> 11) * 0x1.0p-53;
+    }
+}

@@ -1,0 +1,289 @@
+"""sdot_amd -- MI355X (gfx950) Laguerre-cell path behind the sdot API.
+
+Python front-end over the C-ABI of libsdot_mi355x.so (include/sdot_mi355x.h).
+Mirrors the reference's operator API for this path (paths relative to the
+sdot tree):
+
+  PowerDiagram.add_box_shape / set_image_density   Domains/ConvexPolyhedronAssembly.h:26-41,
+                                                   Domains/ScaledImage.h:26
+  PowerDiagram.integrals                           PowerDiagram/get_integrals.h:13
+  PowerDiagram.der_integrals_wrt_weights           PowerDiagram/get_der_integrals_wrt_weights.h:16
+  OptimalTransport.adjust_weights / solve          Solvers/OptimalTransportSolver.h:18-26
+
+The HIP library is the only compute path: importing this package raises when
+the built library is missing, and every call reports the library's status.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsdot_mi355x.so")
+
+SDOT_OK = 0
+SDOT_EMPTY_CELL = 1
+SDOT_NOT_CONVERGED = 2
+SDOT_CAPACITY_OVERFLOW = 3
+SDOT_INVALID_ARG = 4
+SDOT_HIP_ERROR = 5
+SDOT_OOM = 6
+SDOT_NOT_SUPPORTED = 7
+
+FP64 = 0
+MIXED = 1
+
+_d = ctypes.c_double
+_i64 = ctypes.c_int64
+_p = ctypes.c_void_p
+_int = ctypes.c_int
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("newton_iters", _int), ("pcg_iters_total", _int), ("backtracks", _int), ("reserved", _int),
+                ("max_rel_mass_err", _d), ("max_abs_dw", _d), ("nnz", _i64), ("seconds_cells", _d),
+                ("seconds_solve", _d), ("seconds_total", _d)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class Timings(ctypes.Structure):
+    _fields_ = [("ms_grid", _d), ("ms_cells", _d), ("ms_rows", _d), ("ms_total", _d),
+                ("nb_cells_overflow", _i64), ("nnz_offdiag", _i64), ("grid_cells", _i64),
+                ("ms_cell_kernel", _d), ("cell_flops", _d), ("sum_faces", _i64), ("sum_arcs", _i64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# name -> (restype, argtypes): every symbol the C header declares
+SIGNATURES = {
+    "sdot_pd_create": (_int, [ctypes.POINTER(_p), _int, _int]),
+    "sdot_pd_destroy": (_int, [_p]),
+    "sdot_status_string": (ctypes.c_char_p, [_int]),
+    "sdot_pd_last_error": (ctypes.c_char_p, [_p]),
+    "sdot_pd_set_diracs": (_int, [_p, _p, _p, _i64]),
+    "sdot_pd_set_weights": (_int, [_p, _p]),
+    "sdot_pd_set_diracs_device": (_int, [_p, _p, _p, _i64]),
+    "sdot_pd_add_box_shape": (_int, [_p, _p, _p, _d, _i64]),
+    "sdot_pd_set_image_density": (_int, [_p, _p, _p, _p, _p, _i64]),
+    "sdot_pd_clear_shapes": (_int, [_p]),
+    "sdot_pd_set_ball_clip": (_int, [_p, _int]),
+    "sdot_pd_compute": (_int, [_p]),
+    "sdot_pd_integrals": (_int, [_p, _p]),
+    "sdot_pd_boundary_measures": (_int, [_p, _p, _p]),
+    "sdot_pd_der_integrals_wrt_weights": (_int, [_p, _p, _p, _p, _p, _i64, ctypes.POINTER(_i64)]),
+    "sdot_ot_solve": (_int, [_p, _p, _d, _int, _int, _p, ctypes.POINTER(SolveStats)]),
+    "sdot_pd_set_tuning": (_int, [_p, _int, _int]),
+    "sdot_pd_last_timings": (_int, [_p, ctypes.POINTER(Timings)]),
+    "sdot_rng_uniform": (None, [ctypes.c_uint64, _i64, _p]),
+    "sdot_pd_dirac_buffers": (_int, [_p, ctypes.POINTER(_p), ctypes.POINTER(_p)]),
+    "sdot_pd_mark_diracs_changed": (_int, [_p, _int]),
+    "sdot_pd_csr_device": (_int, [_p, ctypes.POINTER(_p), ctypes.POINTER(_p), ctypes.POINTER(_p)]),
+}
+
+_LIB = None
+
+
+def lib():
+    """The loaded HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `make -C sdot_amd` "
+                              "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+lib()
+
+
+class SdotError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{lib().sdot_status_string(status).decode()}: {msg}")
+        self.status = status
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def rng_uniform(seed, n):
+    """splitmix64 -> xoshiro256** uniform doubles in [0,1): the synthetic-input generator."""
+    out = np.empty(int(n), dtype=np.float64)
+    lib().sdot_rng_uniform(ctypes.c_uint64(seed), int(n), _ptr(out))
+    return out
+
+
+class PowerDiagram:
+    """Laguerre (power) diagram of weighted diracs over a domain, on the GPU.
+
+    `allow_empty=True` makes an empty cell a reported condition (`self.last_status`)
+    rather than an exception, like the reference's int result."""
+
+    def __init__(self, dim=2, precision="fp64", positions=None, weights=None, ball=False):
+        self.dim = int(dim)
+        self.precision = precision
+        prec = FP64 if precision == "fp64" else MIXED
+        h = _p()
+        st = lib().sdot_pd_create(ctypes.byref(h), self.dim, prec)
+        if st != SDOT_OK:
+            raise SdotError(st, "sdot_pd_create")
+        self._h = h
+        self.last_status = SDOT_OK
+        self.N = 0
+        if ball:
+            self.set_ball_clip(True)
+        if positions is not None:
+            self.set_diracs(positions, weights if weights is not None else np.zeros(len(positions)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().sdot_pd_destroy(h)
+            self._h = None
+
+    def _check(self, st, what, allow_empty=False):
+        self.last_status = st
+        if st == SDOT_OK or (allow_empty and st == SDOT_EMPTY_CELL):
+            return st
+        raise SdotError(st, f"{what}: {lib().sdot_pd_last_error(self._h).decode()}")
+
+    # -- inputs --
+    def set_diracs(self, positions, weights):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, self.dim)
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+        if len(w) != len(pos):
+            raise ValueError("positions and weights differ in length")
+        self._check(lib().sdot_pd_set_diracs(self._h, _ptr(pos), _ptr(w), len(w)), "set_diracs")
+        self.N = len(w)
+
+    def set_weights(self, weights):
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+        if len(w) != self.N:
+            raise ValueError("weights length")
+        self._check(lib().sdot_pd_set_weights(self._h, _ptr(w)), "set_weights")
+
+    def set_diracs_device(self, pos_ptr, w_ptr, n):
+        """Device pointers (e.g. torch cuda tensors' data_ptr()) already in HBM."""
+        self._check(lib().sdot_pd_set_diracs_device(self._h, _p(pos_ptr), _p(w_ptr), int(n)), "set_diracs_device")
+        self.N = int(n)
+
+    def add_box_shape(self, p0, p1, coeff=1.0, cut_id=-1):
+        a = np.ascontiguousarray(p0, dtype=np.float64)
+        b = np.ascontiguousarray(p1, dtype=np.float64)
+        self._check(lib().sdot_pd_add_box_shape(self._h, _ptr(a), _ptr(b), float(coeff), int(cut_id)), "add_box_shape")
+
+    def set_image_density(self, min_pt, max_pt, data):
+        """data indexed [z][y][x] / [y][x] (x fastest), piecewise constant."""
+        a = np.ascontiguousarray(min_pt, dtype=np.float64)
+        b = np.ascontiguousarray(max_pt, dtype=np.float64)
+        img = np.ascontiguousarray(data, dtype=np.float64)
+        sizes = np.ascontiguousarray(img.shape[::-1], dtype=np.int64)
+        self._keep_img = img
+        self._check(lib().sdot_pd_set_image_density(self._h, _ptr(a), _ptr(b), _ptr(img), _ptr(sizes), 1),
+                    "set_image_density")
+
+    def clear_shapes(self):
+        self._check(lib().sdot_pd_clear_shapes(self._h), "clear_shapes")
+
+    def set_ball_clip(self, on=True):
+        self._check(lib().sdot_pd_set_ball_clip(self._h, int(bool(on))), "set_ball_clip")
+
+    def set_tuning(self, diracs_per_cell=0.0, block=0):
+        self._check(lib().sdot_pd_set_tuning(self._h, int(round(diracs_per_cell * 100)), int(block)), "set_tuning")
+
+    # -- evaluation --
+    def compute(self, allow_empty=True):
+        return self._check(lib().sdot_pd_compute(self._h), "compute", allow_empty)
+
+    def integrals(self, allow_empty=True):
+        m = np.empty(self.N)
+        self._check(lib().sdot_pd_integrals(self._h, _ptr(m)), "integrals", allow_empty)
+        return m
+
+    def boundary_measures(self, allow_empty=True):
+        a = np.empty(self.N)
+        p = np.empty(self.N)
+        self._check(lib().sdot_pd_boundary_measures(self._h, _ptr(a), _ptr(p)), "boundary_measures", allow_empty)
+        return a, p
+
+    def der_integrals_wrt_weights(self, allow_empty=True):
+        """CSR (offsets[N+1], columns, values) of d mass_i / d w_j and the masses."""
+        nnz = _i64()
+        self._check(lib().sdot_pd_der_integrals_wrt_weights(self._h, None, None, None, None, 0, ctypes.byref(nnz)),
+                    "der_integrals_wrt_weights", allow_empty)
+        cap = max(int(nnz.value), 1)
+        off = np.empty(self.N + 1, dtype=np.int64)
+        cols = np.empty(cap, dtype=np.int64)
+        vals = np.empty(cap)
+        m = np.empty(self.N)
+        self._check(lib().sdot_pd_der_integrals_wrt_weights(self._h, _ptr(off), _ptr(cols), _ptr(vals), _ptr(m), cap,
+                                                            ctypes.byref(nnz)), "der_integrals_wrt_weights", allow_empty)
+        n = int(nnz.value)
+        return off, cols[:n].copy(), vals[:n].copy(), m
+
+    def mark_diracs_changed(self, positions_changed=True):
+        self._check(lib().sdot_pd_mark_diracs_changed(self._h, int(bool(positions_changed))), "mark_diracs_changed")
+
+    def dirac_buffers(self):
+        a, b = _p(), _p()
+        self._check(lib().sdot_pd_dirac_buffers(self._h, ctypes.byref(a), ctypes.byref(b)), "dirac_buffers")
+        return a.value, b.value
+
+    def csr_device(self, allow_empty=True):
+        a, b, c = _p(), _p(), _p()
+        self._check(lib().sdot_pd_csr_device(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+                    "csr_device", allow_empty)
+        return a.value, b.value, c.value
+
+    def last_timings(self):
+        t = Timings()
+        self._check(lib().sdot_pd_last_timings(self._h, ctypes.byref(t)), "last_timings")
+        return t.as_dict()
+
+    # -- Newton solve --
+    def ot_solve(self, targets, weights, tol=1e-6, stop_rule=0, max_iter=150):
+        """OptimalTransportSolver::solve; returns (status, weights, stats dict)."""
+        t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1)
+        w = np.array(weights, dtype=np.float64, copy=True).reshape(-1)
+        stats = SolveStats()
+        st = lib().sdot_ot_solve(self._h, _ptr(t), float(tol), int(stop_rule), int(max_iter), _ptr(w),
+                                 ctypes.byref(stats))
+        self.last_status = st
+        if st not in (SDOT_OK, SDOT_NOT_CONVERGED):
+            raise SdotError(st, f"ot_solve: {lib().sdot_pd_last_error(self._h).decode()}")
+        return st, w, stats.as_dict()
+
+
+class OptimalTransport:
+    """pysdot-style front door (`OptimalTransport(positions).adjust_weights()`)."""
+
+    def __init__(self, positions, masses=None, domain=None, precision="fp64", ball=False, weights=None):
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        self.dim = pos.shape[1]
+        self.positions = pos
+        n = len(pos)
+        self.masses = np.full(n, 1.0 / n) if masses is None else np.asarray(masses, dtype=np.float64)
+        self.weights = np.zeros(n) if weights is None else np.asarray(weights, dtype=np.float64)
+        self.pd = PowerDiagram(self.dim, precision, ball=ball)
+        if domain is None:
+            self.pd.add_box_shape(np.zeros(self.dim), np.ones(self.dim))
+        else:
+            domain(self.pd)
+        self.pd.set_diracs(pos, self.weights)
+        self.stats = None
+
+    def adjust_weights(self, tol=1e-6, stop_rule=0, max_iter=150):
+        st, w, stats = self.pd.ot_solve(self.masses, self.weights, tol, stop_rule, max_iter)
+        self.weights = w
+        self.stats = stats
+        self.status = st
+        return w

@@ -1,0 +1,450 @@
+// C-ABI entry points (include/sdot_mi355x.h): handle life cycle, uploads, the
+// cell pass driver and the result getters.  Every entry point converts HIP
+// failures and invalid input into status codes.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <new>
+
+#include "internal.h"
+
+using namespace sdot_amd;
+
+namespace {
+
+
+template <class F>
+sdot_status guard(sdot_pd *pd, F &&f) {
+    try {
+        return f();
+    } catch (const HipError &e) {
+        if (pd) {
+            char buf[512];
+            snprintf(buf, sizeof buf, "HIP error %d (%s) in %s (line %d)", (int)e.err, hipGetErrorString(e.err), e.what,
+                     e.line);
+            pd->err = buf;
+        }
+        return SDOT_HIP_ERROR;
+    } catch (const OomError &) {
+        if (pd) pd->err = "device allocation failed";
+        return SDOT_OOM;
+    } catch (const StatusError &e) {
+        if (pd) pd->err = e.msg;
+        return e.st;
+    } catch (const std::string &e) {
+        if (pd) pd->err = e;
+        return SDOT_INVALID_ARG;
+    } catch (const std::bad_alloc &) {
+        if (pd) pd->err = "host allocation failed";
+        return SDOT_OOM;
+    }
+}
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return ms;
+}
+
+// the domain as seen by the kernels: one box (with density coefficient) or an image
+void domain_args(sdot_pd *pd, CellArgs &a) {
+    a.img = nullptr;
+    if (pd->has_image) {
+        if (!pd->shapes.empty()) throw StatusError{SDOT_NOT_SUPPORTED, "image density together with shapes"};
+        for (int d = 0; d < 3; ++d) {
+            a.dom_lo[d] = d < pd->dim ? pd->img_lo[d] : 0;
+            a.dom_hi[d] = d < pd->dim ? pd->img_hi[d] : 1;
+            a.img_n[d] = d < pd->dim ? pd->img_n[d] : 1;
+        }
+        a.coeff = 1;
+        a.img = pd->img.p;
+        return;
+    }
+    if (pd->shapes.size() != 1)
+        throw StatusError{SDOT_NOT_SUPPORTED, pd->shapes.empty() ? "no domain shape (add_box_shape)" :
+                                                                   "several domain shapes are not supported on the device path"};
+    const Shape &sh = pd->shapes[0];
+    for (int d = 0; d < 3; ++d) {
+        a.dom_lo[d] = d < pd->dim ? sh.p0[d] : 0;
+        a.dom_hi[d] = d < pd->dim ? sh.p1[d] : 1;
+        a.img_n[d] = 1;
+    }
+    a.coeff = sh.coeff;
+}
+
+}  // namespace
+
+namespace sdot_amd {
+sdot_status compute_pass(sdot_pd *pd) {
+    if (pd->N <= 0) throw StatusError{SDOT_INVALID_ARG, "no diracs"};
+    if (pd->ball && pd->dim == 3)
+        throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping in 3D (ConvexPolyhedron3::ball_cut is TODO in the reference)"};
+    if (pd->ball && pd->has_image) throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping with an image density"};
+    if (pd->N > (int64_t)0x7FFFFFFF) throw StatusError{SDOT_INVALID_ARG, "more than 2^31-1 diracs"};
+    hipStream_t s = pd->stream;
+    CellArgs a{};
+    domain_args(pd, a);
+    for (int k = 0; k < 4; ++k)
+        if (!pd->ev[k]) HIPCK(hipEventCreate(&pd->ev[k]));
+    HIPCK(hipEventRecord(pd->ev[0], s));
+    if (!pd->grid_valid) build_grid(pd);
+    pd->flags.ensure(2);
+    fill_u32(pd->flags.p, 2, 0u, s);
+    refresh_weights(pd);
+    HIPCK(hipEventRecord(pd->ev[1], s));
+
+    const int64_t N = pd->N;
+    pd->ell_cap = pd->dim == 2 ? 32 : 96;
+    pd->mass.ensure(N);
+    pd->arc.ensure(N);
+    pd->perim.ensure(N);
+    pd->diag.ensure(N);
+    pd->row_len.ensure(N);
+    pd->ell_col.ensure((size_t)pd->ell_cap * N);
+    pd->ell_val.ensure((size_t)pd->ell_cap * N);
+    pd->ovf_list.ensure(2 * (size_t)N + 2);
+
+    a.N = N;
+    a.spos = pd->spos.p;
+    a.sw = pd->sw.p;
+    a.cell_start = pd->cell_start.p;
+    a.cell_wmax = pd->cell_wmax.p;
+    a.wglob = pd->wglob.p;
+    a.g = pd->g;
+    a.mass = pd->mass.p;
+    a.arc = pd->arc.p;
+    a.perim = pd->perim.p;
+    a.diag = pd->diag.p;
+    a.row_len = pd->row_len.p;
+    a.ell_col = pd->ell_col.p;
+    a.ell_val = pd->ell_val.p;
+    a.ell_cap = pd->ell_cap;
+    a.ell_stride = N;
+    a.flags = pd->flags.p;
+    a.ovf_list = pd->ovf_list.p;
+    a.ovf_count = pd->flags.p + 1;
+    pd->wctr.ensure(3 * WCTR);
+    fill_u32((uint32_t *)pd->wctr.p, 6 * WCTR, 0u, s);
+    a.ctr = pd->wctr.p;
+    if (pd->dim == 2) run_cells2(pd, a);
+    else run_cells3(pd, a);
+    HIPCK(hipEventRecord(pd->ev[2], s));
+    unsigned fl = 0;
+    std::vector<unsigned long long> ctr(3 * WCTR);
+    HIPCK(hipMemcpyAsync(&fl, pd->flags.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(ctr.data(), pd->wctr.p, 3 * WCTR * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    double flops = 0;
+    int64_t sk = 0, sa = 0;
+    for (int k = 0; k < WCTR; ++k) {
+        flops += (double)ctr[k];
+        sk += (int64_t)ctr[WCTR + k];
+        sa += (int64_t)ctr[2 * WCTR + k];
+    }
+    pd->last.cell_flops = flops;
+    pd->last.sum_faces = sk;
+    pd->last.sum_arcs = sa;
+    pd->last.ms_cell_kernel = elapsed_ms(pd->ev[1], pd->ev[3]);
+    pd->last.ms_grid = elapsed_ms(pd->ev[0], pd->ev[1]);
+    pd->last.ms_cells = elapsed_ms(pd->ev[1], pd->ev[2]);
+    pd->last.ms_rows = 0;
+    pd->last.ms_total = elapsed_ms(pd->ev[0], pd->ev[2]);
+    pd->last.grid_cells = (int64_t)pd->g.ncell[0] * pd->g.ncell[1] * pd->g.ncell[2];
+    pd->last_flags = fl;
+    pd->pass_valid = true;
+    if (fl & FLAG_BAD_INPUT) throw StatusError{SDOT_INVALID_ARG, "non-finite weight or coincident diracs"};
+    if (fl & FLAG_ROW_OVERFLOW) throw StatusError{SDOT_CAPACITY_OVERFLOW, "Hessian row longer than the row capacity"};
+    return (fl & FLAG_EMPTY) ? SDOT_EMPTY_CELL : SDOT_OK;
+}
+}  // namespace sdot_amd
+
+namespace {
+sdot_status ensure_pass(sdot_pd *pd) {
+    if (pd->pass_valid) {
+        if (pd->last_flags & (FLAG_BAD_INPUT | FLAG_ROW_OVERFLOW)) return compute_pass(pd);
+        return (pd->last_flags & FLAG_EMPTY) ? SDOT_EMPTY_CELL : SDOT_OK;
+    }
+    return compute_pass(pd);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sdot_status_string(int st) {
+    switch (st) {
+        case SDOT_OK: return "ok";
+        case SDOT_EMPTY_CELL: return "empty cell";
+        case SDOT_NOT_CONVERGED: return "not converged";
+        case SDOT_CAPACITY_OVERFLOW: return "capacity overflow";
+        case SDOT_INVALID_ARG: return "invalid argument";
+        case SDOT_HIP_ERROR: return "HIP error";
+        case SDOT_OOM: return "out of memory";
+        case SDOT_NOT_SUPPORTED: return "not supported";
+        default: return "unknown status";
+    }
+}
+
+const char *sdot_pd_last_error(const sdot_pd *pd) { return pd ? pd->err.c_str() : "null handle"; }
+
+sdot_status sdot_pd_create(sdot_pd **out, int dim, int precision) {
+    if (!out || (dim != 2 && dim != 3) || (precision != SDOT_FP64 && precision != SDOT_MIXED)) return SDOT_INVALID_ARG;
+    *out = nullptr;
+    sdot_pd *pd = new (std::nothrow) sdot_pd;
+    if (!pd) return SDOT_OOM;
+    pd->dim = dim;
+    pd->precision = precision;
+    sdot_status st = guard(pd, [&] {
+        int ndev = 0;
+        HIPCK(hipGetDeviceCount(&ndev));
+        if (ndev < 1) throw StatusError{SDOT_HIP_ERROR, "no HIP device"};
+        HIPCK(hipStreamCreateWithFlags(&pd->stream, hipStreamNonBlocking));
+        return SDOT_OK;
+    });
+    if (st != SDOT_OK) {
+        delete pd;
+        return st;
+    }
+    *out = pd;
+    return SDOT_OK;
+}
+
+sdot_status sdot_pd_destroy(sdot_pd *pd) {
+    if (!pd) return SDOT_INVALID_ARG;
+    if (pd->stream) (void)hipStreamSynchronize(pd->stream);
+    for (auto &e : pd->ev)
+        if (e) (void)hipEventDestroy(e);
+    pd->img.release();
+    pd->pos.release(); pd->w.release();
+    pd->keys.release(); pd->rank.release(); pd->cell_start.release(); pd->scan_tmp.release();
+    pd->perm.release(); pd->inv_perm.release();
+    pd->spos.release(); pd->sw.release(); pd->cell_wmax.release(); pd->bbox_part.release(); pd->wglob.release();
+    pd->mass.release(); pd->arc.release(); pd->perim.release(); pd->diag.release(); pd->ell_val.release();
+    pd->row_len.release(); pd->ell_col.release(); pd->ovf_list.release(); pd->flags.release(); pd->wctr.release();
+    pd->csr_off.release(); pd->csr_col.release(); pd->csr_cnt.release(); pd->csr_tmp.release();
+    pd->csr_val.release(); pd->out_tmp.release();
+    for (auto &b : pd->sv) b.release();
+    pd->red.release();
+    if (pd->stream) (void)hipStreamDestroy(pd->stream);
+    delete pd;
+    return SDOT_OK;
+}
+
+sdot_status sdot_pd_set_diracs(sdot_pd *pd, const double *positions, const double *weights, int64_t n) {
+    if (!pd || n <= 0 || !positions || !weights) return SDOT_INVALID_ARG;
+    return guard(pd, [&] {
+        pd->N = n;
+        pd->pos.ensure(n * pd->dim);
+        pd->w.ensure(n);
+        HIPCK(hipMemcpyAsync(pd->pos.p, positions, n * pd->dim * sizeof(double), hipMemcpyHostToDevice, pd->stream));
+        HIPCK(hipMemcpyAsync(pd->w.p, weights, n * sizeof(double), hipMemcpyHostToDevice, pd->stream));
+        HIPCK(hipStreamSynchronize(pd->stream));
+        pd->grid_valid = false;
+        pd->pass_valid = false;
+        return SDOT_OK;
+    });
+}
+
+sdot_status sdot_pd_set_diracs_device(sdot_pd *pd, const double *d_positions, const double *d_weights, int64_t n) {
+    if (!pd || n <= 0 || !d_positions || !d_weights) return SDOT_INVALID_ARG;
+    return guard(pd, [&] {
+        pd->N = n;
+        pd->pos.ensure(n * pd->dim);
+        pd->w.ensure(n);
+        HIPCK(hipMemcpyAsync(pd->pos.p, d_positions, n * pd->dim * sizeof(double), hipMemcpyDeviceToDevice, pd->stream));
+        HIPCK(hipMemcpyAsync(pd->w.p, d_weights, n * sizeof(double), hipMemcpyDeviceToDevice, pd->stream));
+        pd->grid_valid = false;
+        pd->pass_valid = false;
+        return SDOT_OK;
+    });
+}
+
+sdot_status sdot_pd_set_weights(sdot_pd *pd, const double *weights) {
+    if (!pd || !weights || pd->N <= 0) return SDOT_INVALID_ARG;
+    return guard(pd, [&] {
+        HIPCK(hipMemcpyAsync(pd->w.p, weights, pd->N * sizeof(double), hipMemcpyHostToDevice, pd->stream));
+        HIPCK(hipStreamSynchronize(pd->stream));
+        pd->pass_valid = false;
+        return SDOT_OK;
+    });
+}
+
+sdot_status sdot_pd_dirac_buffers(sdot_pd *pd, double **d_positions, double **d_weights) {
+    if (!pd || pd->N <= 0 || !d_positions || !d_weights) return SDOT_INVALID_ARG;
+    *d_positions = pd->pos.p;
+    *d_weights = pd->w.p;
+    return SDOT_OK;
+}
+
+sdot_status sdot_pd_mark_diracs_changed(sdot_pd *pd, int positions_changed) {
+    if (!pd) return SDOT_INVALID_ARG;
+    if (positions_changed) pd->grid_valid = false;
+    pd->pass_valid = false;
+    return SDOT_OK;
+}
+
+sdot_status sdot_pd_csr_device(sdot_pd *pd, const int64_t **d_offsets, const int64_t **d_cols, const double **d_vals) {
+    if (!pd || !d_offsets || !d_cols || !d_vals) return SDOT_INVALID_ARG;
+    return guard(pd, [&] {
+        sdot_status st = ensure_pass(pd);
+        for (int k = 0; k < 2; ++k)
+            if (!pd->ev[4 + k]) HIPCK(hipEventCreate(&pd->ev[4 + k]));
+        HIPCK(hipEventRecord(pd->ev[4], pd->stream));
+        build_csr_device(pd);
+        HIPCK(hipEventRecord(pd->ev[5], pd->stream));
+        HIPCK(hipStreamSynchronize(pd->stream));
+        pd->last.ms_rows = elapsed_ms(pd->ev[4], pd->ev[5]);
+        *d_offsets = pd->csr_off.p;
+        *d_cols = pd->csr_col.p;
+        *d_vals = pd->csr_val.p;
+        return st;
+    });
+}
+
+sdot_status sdot_pd_add_box_shape(sdot_pd *pd, const double *p0, const double *p1, double coeff, int64_t cut_id) {
+    if (!pd || !p0 || !p1) return SDOT_INVALID_ARG;
+    if (cut_id != -1) {
+        pd->err = "only the domain cut id -1 is supported";
+        return SDOT_NOT_SUPPORTED;
+    }
+    Shape sh{};
+    for (int d = 0; d < pd->dim; ++d) {
+        if (!(p1[d] > p0[d])) return SDOT_INVALID_ARG;
+        sh.p0[d] = p0[d];
+        sh.p1[d] = p1[d];
+    }
+    sh.coeff = coeff;
+    sh.cut_id = cut_id;
+    pd->shapes.push_back(sh);
+    pd->pass_valid = false;
+    return SDOT_OK;
+}
+
+sdot_status sdot_pd_set_image_density(sdot_pd *pd, const double *min_pt, const double *max_pt, const double *data,
+                                      const int64_t *sizes, int64_t nb_coeffs) {
+    if (!pd || !min_pt || !max_pt || !data || !sizes) return SDOT_INVALID_ARG;
+    if (nb_coeffs != 1) {
+        pd->err = "only piecewise-constant images (nb_coeffs = 1) are supported";
+        return SDOT_NOT_SUPPORTED;
+    }
+    int64_t tot = 1;
+    for (int d = 0; d < pd->dim; ++d) {
+        if (sizes[d] < 1 || sizes[d] > (1 << 20) || !(max_pt[d] > min_pt[d])) return SDOT_INVALID_ARG;
+        tot *= sizes[d];
+    }
+    return guard(pd, [&] {
+        pd->img.ensure(tot);
+        HIPCK(hipMemcpyAsync(pd->img.p, data, tot * sizeof(double), hipMemcpyHostToDevice, pd->stream));
+        HIPCK(hipStreamSynchronize(pd->stream));
+        for (int d = 0; d < 3; ++d) {
+            pd->img_lo[d] = d < pd->dim ? min_pt[d] : 0;
+            pd->img_hi[d] = d < pd->dim ? max_pt[d] : 1;
+            pd->img_n[d] = d < pd->dim ? (int)sizes[d] : 1;
+        }
+        pd->has_image = true;
+        pd->pass_valid = false;
+        return SDOT_OK;
+    });
+}
+
+sdot_status sdot_pd_clear_shapes(sdot_pd *pd) {
+    if (!pd) return SDOT_INVALID_ARG;
+    pd->shapes.clear();
+    pd->has_image = false;
+    pd->pass_valid = false;
+    return SDOT_OK;
+}
+
+sdot_status sdot_pd_set_ball_clip(sdot_pd *pd, int on) {
+    if (!pd) return SDOT_INVALID_ARG;
+    pd->ball = on ? 1 : 0;
+    pd->pass_valid = false;
+    return SDOT_OK;
+}
+
+sdot_status sdot_pd_set_tuning(sdot_pd *pd, int dpc_x100, int block) {
+    if (!pd || dpc_x100 < 0 || block < 0) return SDOT_INVALID_ARG;
+    if (dpc_x100 != pd->dpc_x100) pd->grid_valid = false;
+    pd->dpc_x100 = dpc_x100;
+    pd->block = block;
+    pd->pass_valid = false;
+    return SDOT_OK;
+}
+
+sdot_status sdot_pd_compute(sdot_pd *pd) {
+    if (!pd) return SDOT_INVALID_ARG;
+    return guard(pd, [&] { return compute_pass(pd); });
+}
+
+sdot_status sdot_pd_integrals(sdot_pd *pd, double *masses) {
+    if (!pd || !masses) return SDOT_INVALID_ARG;
+    return guard(pd, [&] {
+        sdot_status st = ensure_pass(pd);
+        export_vec(pd, pd->mass.p, masses);
+        return st;
+    });
+}
+
+sdot_status sdot_pd_boundary_measures(sdot_pd *pd, double *ball_arc, double *perimeter) {
+    if (!pd) return SDOT_INVALID_ARG;
+    return guard(pd, [&] {
+        sdot_status st = ensure_pass(pd);
+        if (ball_arc) export_vec(pd, pd->arc.p, ball_arc);
+        if (perimeter) export_vec(pd, pd->perim.p, perimeter);
+        return st;
+    });
+}
+
+sdot_status sdot_pd_der_integrals_wrt_weights(sdot_pd *pd, int64_t *offsets, int64_t *cols, double *vals,
+                                              double *masses, int64_t cap, int64_t *nnz) {
+    if (!pd || !nnz || (cols && (!offsets || !vals))) return SDOT_INVALID_ARG;
+    return guard(pd, [&] {
+        sdot_status st = ensure_pass(pd);
+        export_csr(pd, offsets, cols, vals, cap, nnz);
+        if (masses) export_vec(pd, pd->mass.p, masses);
+        return st;
+    });
+}
+
+sdot_status sdot_pd_last_timings(sdot_pd *pd, sdot_timings *t) {
+    if (!pd || !t) return SDOT_INVALID_ARG;
+    *t = pd->last;
+    return SDOT_OK;
+}
+
+sdot_status sdot_ot_solve(sdot_pd *pd, const double *targets, double tol, int stop_rule, int max_iter,
+                          double *weights_inout, sdot_solve_stats *stats) {
+    if (!pd || !targets || !weights_inout || !(tol > 0) || max_iter < 1 || (stop_rule != 0 && stop_rule != 1))
+        return SDOT_INVALID_ARG;
+    if (pd->N <= 0) return SDOT_INVALID_ARG;
+    return guard(pd, [&] { return ot_solve(pd, targets, tol, stop_rule, max_iter, weights_inout, stats); });
+}
+
+// splitmix64 seeding xoshiro256** (Blackman & Vigna), u = (x >> 11) * 2^-53
+void sdot_rng_uniform(uint64_t seed, int64_t n, double *out) {
+    auto splitmix = [](uint64_t &x) {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    };
+    auto rotl = [](uint64_t x, int k) { return (x << k) | (x >> (64 - k)); };
+    uint64_t s[4], sm = seed;
+    for (auto &v : s) v = splitmix(sm);
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t r = rotl(s[1] * 5, 7) * 9;
+        uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        out[i] = (double)(r >> 11) * 0x1.0p-53;
+    }
+}
+
+}  // extern "C"

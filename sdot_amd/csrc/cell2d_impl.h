@@ -1,0 +1,492 @@
+// Per-cell routine of the 2D Laguerre-cell kernel (host+device so that the
+// same code can be exercised on the CPU when debugging the kernel).
+//
+// Per cell (semantics of src/sdot/PowerDiagram/Visitors/SpZGrid.tcc:494-645):
+//   * start from the domain box (ConvexPolyhedronAssembly with one box,
+//     ConvexPolyhedronAssembly.tcc:64-65), in coordinates local to the dirac;
+//   * visit grid cells ring by ring; a ring (and every further one) or a grid
+//     cell is skipped when no dirac in it can cut the current polygon -- the
+//     power-distance bound of SpZGrid::can_be_evicted (:475-480) taken over the
+//     disc of radius R (max vertex distance) that holds the polygon (in ball mode
+//     over its intersection with the ball, which is what the result depends on);
+//   * cut by the radical half-plane of each remaining candidate
+//     (SpZGrid.tcc:506-522; ConvexPolyhedron2::plane_cut :687-922: the new
+//     points are interpolated from the inside vertex, the entering edge takes
+//     the new id, the leaving edge keeps the id of the last removed edge);
+//   * in ball mode, clip by the disc of radius sqrt(w_i) streaming the
+//     LINE/ARC items of ConvexPolyhedron2::ball_cut (:534-665) straight into the
+//     integrals (area :1791-1842, _arc_area :2361-2380, boundary items
+//     :437-478) and the Hessian row (get_der_integrals_wrt_weights.h:52-75).
+// T = double (FP64 path) or float (mixed path: FP32 local coordinates; plane
+// offsets, distances and the mass/Hessian outputs in FP64).
+#pragma once
+
+#include <cfloat>
+#include <cmath>
+
+#include "internal.h"
+
+namespace sdot_amd {
+namespace c2 {
+
+template <class T>
+struct Eps;
+template <>
+struct Eps<double> {
+    static constexpr double region = 1e-9, cand = 1e-9;
+};
+template <>
+struct Eps<float> {
+    static constexpr double region = 2e-5, cand = 2e-5;
+};
+
+// Can a region whose points are all at distance >= d from c0 and whose diracs
+// have weight <= wmax hold a dirac that cuts a polygon contained in the disc of
+// radius sqrt(R2) about c0?  Dirac j cuts only if |V|*R > (|V|^2 + w0 - wj)/2.
+template <class T>
+SDOT_HD inline bool region_skippable(double d, double R2, double w0, double wmax) {
+    double R = sqrt(R2);
+    double dw = w0 - wmax;
+    double tol = Eps<T>::region * (d * d + R2 + fabs(w0) + fabs(wmax));
+    if (d >= R) return d * d - 2 * R * d + dw >= tol;
+    return dw - R2 >= tol;
+}
+
+SDOT_HD inline void ring_cell(int r, int k, int &dx, int &dy) {
+    if (r == 0) { dx = dy = 0; return; }
+    int row = 2 * r + 1;
+    if (k < row) { dx = -r + k; dy = r; return; }
+    k -= row;
+    if (k < row) { dx = -r + k; dy = -r; return; }
+    k -= row;
+    if (k < 2 * r - 1) { dx = -r; dy = -r + 1 + k; return; }
+    k -= 2 * r - 1;
+    dx = r;
+    dy = -r + 1 + k;
+}
+
+// Neumaier-compensated sum
+template <class T>
+struct CSum {
+    T s = 0, c = 0;
+    SDOT_HD void add(T x) {
+        T t = s + x;
+        if (fabs(s) >= fabs(x)) c += (s - t) + x;
+        else c += (x - t) + s;
+        s = t;
+    }
+    SDOT_HD double value() const { return (double)s + (double)c; }
+};
+
+template <class T>
+SDOT_HD inline T my_atan2(T y, T x) { return atan2(y, x); }
+template <>
+SDOT_HD inline float my_atan2<float>(float y, float x) { return atan2f(y, x); }
+
+constexpr double TWO_PI = 6.283185307179586476925286766559;
+constexpr double PI_D = 3.1415926535897932384626433832795;
+
+// Accumulates the boundary items of the final cell (line pieces with a cut id,
+// arcs of the ball) into the integrals and the Hessian row.
+template <class T>
+struct Integrator {
+    const CellArgs *a;
+    int64_t t;
+    double c0x, c0y;
+    T r, r2;
+    CSum<T> area2;       // twice the polygonal area (shoelace about the dirac)
+    double arcs = 0;     // circular segment areas
+    double arcm = 0, perim = 0;
+    double der_line = 0;   // sum of 0.5*rho*|G_ij|/|x_i-x_j| (rho included)
+    double der_arc = 0;    // sum of 0.5*da: 0.5*arc/sqrt(w_i) before the density factor
+    int nrow = 0;
+    int nlines = 0, narcs = 0;
+    bool row_ovf = false;
+    // previous / first item
+    int m = 0;
+    T fx, fy, px, py;
+    int ftype, fid, ptype, pid;
+
+    SDOT_HD void line(T x0, T y0, T x1, T y1, int id) {
+        T ex = x1 - x0, ey = y1 - y0;
+        double len = (double)sqrt(ex * ex + ey * ey);
+        perim += len;
+        if (id >= 0) {
+            double vx = a->spos[2 * (int64_t)id] - c0x, vy = a->spos[2 * (int64_t)id + 1] - c0y;
+            double dist = sqrt(vx * vx + vy * vy);
+            if (dist != 0) {
+                double b = 0.5 * a->coeff * len / dist;
+                if (nrow < a->ell_cap) {
+                    a->ell_col[nrow * a->ell_stride + t] = id;
+                    a->ell_val[nrow * a->ell_stride + t] = -b;
+                    ++nrow;
+                } else {
+                    row_ovf = true;
+                }
+                der_line += b;
+            }
+        }
+    }
+    SDOT_HD void arc(T x0, T y0, T x1, T y1) {
+        T a0 = my_atan2(y0, x0), a1 = my_atan2(y1, x1);
+        T da = a1 - a0;
+        if (a1 < a0) da = (a1 + T(TWO_PI)) - a0;
+        double l = (double)da * (double)r;
+        arcm += l;
+        perim += l;
+        der_arc += 0.5 * (double)da;
+        T ex = x1 - x0, ey = y1 - y0;
+        T d0 = sqrt(ex * ex + ey * ey) / 2;
+        T d1 = sqrt(fmax(T(0), r2 - d0 * d0));
+        double seg = (double)r2 * (double)da / 2;
+        if (da > T(PI_D)) seg += (double)d1 * (double)d0;
+        else seg -= (double)d1 * (double)d0;
+        arcs += seg;
+    }
+    SDOT_HD void close(T x1, T y1) {
+        area2.add(px * y1 - x1 * py);
+        if (ptype == 0) {
+            line(px, py, x1, y1, pid);
+            ++nlines;
+        } else {
+            arc(px, py, x1, y1);
+            ++narcs;
+        }
+    }
+    SDOT_HD void push(int type, int id, T x, T y) {
+        if (m == 0) { fx = x; fy = y; ftype = type; fid = id; }
+        else close(x, y);
+        px = x; py = y; ptype = type; pid = id;
+        ++m;
+    }
+    SDOT_HD void finish() {
+        if (m > 0) close(fx, fy);
+    }
+};
+
+// half side of the square that encloses the ball of radius sqrt(w0) with a margin
+SDOT_HD inline double ball_square(double w0) { return sqrt(fmax(w0, 0.0)) * (1 + 1e-4); }
+
+// The line a*x + b*y = c (coordinates local to the dirac) that carries an edge:
+// the radical line of neighbour id >= 0, box side -1 - id (bottom, right, top,
+// left) for ids -1..-4, or a side of the ball's enclosing square for -5..-8.
+SDOT_HD inline void line_of(const CellArgs &a, int id, double c0x, double c0y, double w0, double &la, double &lb,
+                            double &lc) {
+    if (id >= 0) {
+        la = a.spos[2 * (int64_t)id] - c0x;
+        lb = a.spos[2 * (int64_t)id + 1] - c0y;
+        lc = 0.5 * (la * la + lb * lb + w0 - a.sw[id]);
+        return;
+    }
+    const double sq = ball_square(w0);
+    switch (-1 - id) {
+        case 0: la = 0; lb = 1; lc = a.dom_lo[1] - c0y; break;
+        case 1: la = 1; lb = 0; lc = a.dom_hi[0] - c0x; break;
+        case 2: la = 0; lb = 1; lc = a.dom_hi[1] - c0y; break;
+        case 3: la = 1; lb = 0; lc = a.dom_lo[0] - c0x; break;
+        case 4: la = 1; lb = 0; lc = sq; break;
+        case 5: la = 0; lb = 1; lc = sq; break;
+        case 6: la = -1; lb = 0; lc = sq; break;
+        default: la = 0; lb = -1; lc = sq; break;
+    }
+}
+
+// Builds and integrates cell t.  The polygon lives at X/Y/I with stride S
+// (an LDS slot on the device).  Returns 1 when the polygon outgrows CAP
+// vertices (nothing is written then), 0 otherwise.
+template <class T, int CAP, bool BALL>
+SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, int *I, const int S) {
+#define PX(k) X[(k) * S]
+#define PY(k) Y[(k) * S]
+#define PI_(k) I[(k) * S]
+
+    const double c0x = a.spos[2 * t], c0y = a.spos[2 * t + 1], w0 = a.sw[t];
+    const GridInfo &g = a.g;
+    const double W = dbl_unorder(*a.wglob);
+
+    // starting cell: the domain box, vertices p0, (p1.x,p0.y), p1, (p0.x,p1.y)
+    int n = 4;
+    PX(0) = T(a.dom_lo[0] - c0x); PY(0) = T(a.dom_lo[1] - c0y);
+    PX(1) = T(a.dom_hi[0] - c0x); PY(1) = T(a.dom_lo[1] - c0y);
+    PX(2) = T(a.dom_hi[0] - c0x); PY(2) = T(a.dom_hi[1] - c0y);
+    PX(3) = T(a.dom_lo[0] - c0x); PY(3) = T(a.dom_hi[1] - c0y);
+    for (int k = 0; k < 4; ++k) PI_(k) = -1 - k;   // box sides: bottom, right, top, left
+
+    auto radius2 = [&]() {
+        double R2 = 0;
+        for (int k = 0; k < n; ++k) {
+            double x = PX(k), y = PY(k);
+            R2 = fmax(R2, x * x + y * y);
+        }
+        if (BALL) R2 = fmin(R2, fmax(w0, 0.0) * (1 + 1e-6));
+        return R2;
+    };
+    // cut by the half-plane q.V <= H; returns 0 (no change), 1 (cut), 2 (capacity exceeded)
+    auto cut = [&](const T Vx, const T Vy, const T H, const int cid) -> int {
+        // classify vertices: d = H - q.V, outside when d < 0
+        uint64_t mask = 0;
+        T dfirst = 0, dprev = 0, dAin = 0, dAout = 0, dBout = 0, dBin = 0;
+        int i1 = -1, i2 = -1;
+        for (int k = 0; k < n; ++k) {
+            T d = fma(-PX(k), Vx, fma(-PY(k), Vy, H));
+            bool o = d < 0;
+            if (o) mask |= 1ull << k;
+            if (k == 0) {
+                dfirst = d;
+            } else {
+                bool po = dprev < 0;
+                if (o && !po) { i1 = k; dAin = dprev; dAout = d; }
+                if (!o && po) { i2 = k - 1; dBout = dprev; dBin = d; }
+            }
+            dprev = d;
+        }
+        if (!mask) return 0;
+        const int nout = __builtin_popcountll(mask);
+        if (nout == n) { n = 0; return 1; }
+        {
+            bool o0 = dfirst < 0, ol = dprev < 0;
+            if (o0 && !ol) { i1 = 0; dAin = dprev; dAout = dfirst; }
+            if (!o0 && ol) { i2 = n - 1; dBout = dprev; dBin = dfirst; }
+        }
+        const int nn = n - nout + 2;
+        if (nn > CAP) return 2;
+        const int i0 = i1 == 0 ? n - 1 : i1 - 1;
+        const int i3 = i2 == n - 1 ? 0 : i2 + 1;
+        const T m0 = dAin / (dAout - dAin);
+        const T Ax = PX(i0) - m0 * (PX(i1) - PX(i0)), Ay = PY(i0) - m0 * (PY(i1) - PY(i0));
+        const T m2 = dBin / (dBout - dBin);
+        const T Bx = PX(i3) - m2 * (PX(i2) - PX(i3)), By = PY(i3) - m2 * (PY(i2) - PY(i3));
+        const int idB = PI_(i2);
+        if (i1 <= i2) {
+            const int shift = nout - 2;
+            if (shift > 0) {
+                for (int k = i2 + 1; k < n; ++k) { PX(k - shift) = PX(k); PY(k - shift) = PY(k); PI_(k - shift) = PI_(k); }
+            } else if (shift < 0) {
+                for (int k = n - 1; k > i2; --k) { PX(k + 1) = PX(k); PY(k + 1) = PY(k); PI_(k + 1) = PI_(k); }
+            }
+            PX(i1) = Ax; PY(i1) = Ay; PI_(i1) = cid;
+            PX(i1 + 1) = Bx; PY(i1 + 1) = By; PI_(i1 + 1) = idB;
+        } else {
+            const int ni = i1 - i2 - 1;
+            for (int k = 0; k < ni; ++k) { PX(1 + k) = PX(i2 + 1 + k); PY(1 + k) = PY(i2 + 1 + k); PI_(1 + k) = PI_(i2 + 1 + k); }
+            PX(0) = Bx; PY(0) = By; PI_(0) = idB;
+            PX(ni + 1) = Ax; PY(ni + 1) = Ay; PI_(ni + 1) = cid;
+        }
+        n = nn;
+        return 1;
+    };
+    double R2 = radius2();
+    bool overflow = false;
+    bool bad = false;
+
+    if (BALL && !(w0 > 0)) n = 0;   // disc of radius sqrt(w0) <= 0: empty cell
+    if (BALL && n > 0) {
+        // Clip by a square slightly larger than the disc: P & square & disc =
+        // P & disc, and every edge left is short, which keeps the chord/circle
+        // intersections of the ball clip well conditioned.
+        const T sq = T(ball_square(w0));
+        if (cut(T(1), T(0), sq, -5) == 2 || cut(T(0), T(1), sq, -6) == 2 || cut(T(-1), T(0), sq, -7) == 2 ||
+            cut(T(0), T(-1), sq, -8) == 2)
+            overflow = true;
+        R2 = radius2();
+    }
+
+    const int hx = grid_coord(g, 0, c0x), hy = grid_coord(g, 1, c0y);
+    const int maxr = g.ncell[0] > g.ncell[1] ? g.ncell[0] : g.ncell[1];
+    for (int r = 0; r <= maxr && n > 0 && !overflow; ++r) {
+        if (r > 0) {
+            bool lf = hx - r + 1 > 0, rt = hx + r < g.ncell[0], bt = hy - r + 1 > 0, tp = hy + r < g.ncell[1];
+            if (!(lf || rt || bt || tp)) break;
+            double dm = DBL_MAX;
+            if (lf) dm = fmin(dm, c0x - (g.lo[0] + (hx - r + 1) * g.h[0]));
+            if (rt) dm = fmin(dm, (g.lo[0] + (hx + r) * g.h[0]) - c0x);
+            if (bt) dm = fmin(dm, c0y - (g.lo[1] + (hy - r + 1) * g.h[1]));
+            if (tp) dm = fmin(dm, (g.lo[1] + (hy + r) * g.h[1]) - c0y);
+            dm = fmax(dm, 0.0);
+            if (region_skippable<T>(dm, R2, w0, W)) break;
+        }
+        const int ncell_ring = r == 0 ? 1 : 8 * r;
+        for (int kc = 0; kc < ncell_ring && n > 0 && !overflow; ++kc) {
+            int dx, dy;
+            ring_cell(r, kc, dx, dy);
+            const int gx = hx + dx, gy = hy + dy;
+            if (gx < 0 || gy < 0 || gx >= g.ncell[0] || gy >= g.ncell[1]) continue;
+            const uint32_t code = morton2(gx, gy);
+            const uint32_t cs = a.cell_start[code], ce = a.cell_start[code + 1];
+            if (cs == ce) continue;
+            {
+                double bx0 = g.lo[0] + gx * g.h[0], by0 = g.lo[1] + gy * g.h[1];
+                double ex = fmax(0.0, fmax(bx0 - c0x, c0x - (bx0 + g.h[0])));
+                double ey = fmax(0.0, fmax(by0 - c0y, c0y - (by0 + g.h[1])));
+                if (region_skippable<T>(sqrt(ex * ex + ey * ey), R2, w0, a.cell_wmax[code])) continue;
+            }
+            for (uint32_t j = cs; j < ce; ++j) {
+                if ((int64_t)j == t) continue;
+                const double vx = a.spos[2 * (int64_t)j] - c0x, vy = a.spos[2 * (int64_t)j + 1] - c0y;
+                const double vv = vx * vx + vy * vy;
+                const double hh = 0.5 * (vv + w0 - a.sw[j]);
+                if (hh > 0 && R2 * vv <= hh * hh * (1 - Eps<T>::cand)) continue;   // cannot cut
+                if (vv == 0) { bad = true; continue; }                             // coincident diracs
+                const T Vx = T(vx), Vy = T(vy), H = T(hh);
+                const int rc = cut(Vx, Vy, H, (int)j);
+                if (rc == 2) { overflow = true; break; }
+                if (rc == 0) continue;
+                if (n == 0) break;
+                R2 = radius2();
+            }
+        }
+    }
+    if (bad) flag_or(a.flags, FLAG_BAD_INPUT);
+    if (overflow) return 1;
+
+    // ---- integration (and the ball clip, streamed), in FP64 ----
+    // FP64 path: the vertices as clipped.  Mixed path: the FP32 clip fixed
+    // which lines bound the cell; each vertex is rebuilt in FP64 as the
+    // intersection of its two lines, so the integrals carry no FP32 error.
+    Integrator<double> it;
+    it.a = &a;
+    it.t = t;
+    it.c0x = c0x;
+    it.c0y = c0y;
+    it.r2 = BALL ? w0 : 0.0;
+    it.r = BALL ? sqrt(fmax(w0, 0.0)) : 0.0;
+    auto vtx = [&](int k, double &x, double &y) {
+        x = (double)PX(k);
+        y = (double)PY(k);
+        if (sizeof(T) < sizeof(double)) {
+            double a1, b1, e1, a2, b2, e2;
+            line_of(a, PI_(k == 0 ? n - 1 : k - 1), c0x, c0y, w0, a1, b1, e1);
+            line_of(a, PI_(k), c0x, c0y, w0, a2, b2, e2);
+            double det = a1 * b2 - a2 * b1;
+            if (fabs(det) > 1e-12 * (fabs(a1 * b2) + fabs(a2 * b1))) {
+                x = (e1 * b2 - e2 * b1) / det;
+                y = (a1 * e2 - a2 * e1) / det;
+            }
+        }
+    };
+    bool full_disc = false;
+    if (!BALL) {
+        for (int k = 0; k < n; ++k) {
+            double x, y;
+            vtx(k, x, y);
+            it.push(0, PI_(k), x, y);
+        }
+        it.finish();
+    } else if (n > 0) {
+        const double r2 = it.r2;
+        bool all_inside = true;
+        double x0v, y0v;
+        vtx(0, x0v, y0v);
+        {
+            double px = x0v, py = y0v;
+            for (int k = 0; k < n; ++k) {
+                double x = px, y = py;
+                if (k + 1 < n) vtx(k + 1, px, py);
+                all_inside &= x * x + y * y - r2 <= 0;
+            }
+        }
+        if (all_inside) {
+            for (int k = 0; k < n; ++k) {
+                double x, y;
+                vtx(k, x, y);
+                it.push(0, PI_(k), x, y);
+            }
+        } else {
+            double p1x = x0v, p1y = y0v;
+            for (int k0 = 0; k0 < n; ++k0) {
+                const int k1 = k0 + 1 == n ? 0 : k0 + 1;
+                const double p0x = p1x, p0y = p1y;
+                if (k1 == 0) { p1x = x0v; p1y = y0v; }
+                else vtx(k1, p1x, p1y);
+                const double d0 = p0x * p0x + p0y * p0y - r2, d1 = p1x * p1x + p1y * p1y - r2;
+                const int id = PI_(k0);
+                if (d0 <= 0) {
+                    if (d1 <= 0) {
+                        it.push(0, id, p0x, p0y);
+                    } else {
+                        double ex = p1x - p0x, ey = p1y - p0y;
+                        double aa = ex * ex + ey * ey, b = p0x * ex + p0y * ey, c = d0;
+                        double dd = sqrt(fmax(0.0, b * b - aa * c));
+                        double u = (-b + dd) / aa, v = (-b - dd) / aa;
+                        double tt = fabs(u - 0.5) <= fabs(v - 0.5) ? u : v;
+                        tt = fmin(1.0, fmax(0.0, tt));
+                        it.push(0, id, p0x, p0y);
+                        it.push(1, 0, p0x + tt * ex, p0y + tt * ey);
+                    }
+                } else {
+                    if (d1 <= 0) {
+                        double ex = p0x - p1x, ey = p0y - p1y;
+                        double aa = ex * ex + ey * ey, b = p1x * ex + p1y * ey, c = d1;
+                        double dd = sqrt(fmax(0.0, b * b - aa * c));
+                        double u = (-b + dd) / aa, v = (-b - dd) / aa;
+                        double tt = fabs(u - 0.5) <= fabs(v - 0.5) ? u : v;
+                        tt = fmin(1.0, fmax(0.0, tt));
+                        it.push(0, id, p1x + tt * ex, p1y + tt * ey);
+                    } else {
+                        double ex = p1x - p0x, ey = p1y - p0y;
+                        double aa = ex * ex + ey * ey;
+                        if (aa == 0) continue;
+                        double b = p0x * ex + p0y * ey, c = d0;
+                        double s = b * b - aa * c;
+                        if (s <= 0) continue;
+                        double dd = sqrt(s);
+                        double u = (-b - dd) / aa, v = (-b + dd) / aa;
+                        if (u > 0 && u < 1) v = fmax(0.0, fmin(1.0, v));
+                        else if (v > 0 && v < 1) u = fmax(0.0, fmin(1.0, u));
+                        else continue;
+                        double q0x = p0x + u * ex, q0y = p0y + u * ey, q1x = p0x + v * ex, q1y = p0y + v * ey;
+                        if (atan2(q0y, q0x) == atan2(q1y, q1x)) continue;
+                        it.push(0, id, q0x, q0y);
+                        it.push(1, 0, q1x, q1y);
+                    }
+                }
+            }
+            if (it.m == 0) {
+                // no item left: the disc is either inside the polygon or disjoint from it
+                bool outside = false;
+                double qx = x0v, qy = y0v;
+                for (int k0 = 0; k0 < n; ++k0) {
+                    double px = qx, py = qy;
+                    if (k0 + 1 < n) vtx(k0 + 1, qx, qy);
+                    else { qx = x0v; qy = y0v; }
+                    double ex = qx - px, ey = qy - py;
+                    if (ex * py - ey * px > 0) outside = true;   // the dirac lies right of the edge
+                }
+                full_disc = !outside;
+            }
+        }
+        it.finish();
+    }
+    double area = 0.5 * it.area2.value() + it.arcs;
+    if (full_disc) {
+        area = PI_D * (double)w0;
+        double l = TWO_PI * sqrt(w0);
+        it.arcm = l;
+        it.perim = l;
+        it.der_arc = PI_D;
+    }
+    const double coeff = a.coeff;
+    const double mass = coeff * area;
+    a.mass[t] = mass;
+    a.arc[t] = coeff * it.arcm;
+    a.perim[t] = coeff * it.perim;
+    a.diag[t] = it.der_line + coeff * it.der_arc;
+    a.row_len[t] = it.nrow;
+    if (a.ctr) {
+        const unsigned long long K = it.nlines, A = it.narcs;
+        const int slot = (int)(t % WCTR);
+        ctr_add(a.ctr + slot, 4 * K * K + 37 * K + 40 * A);
+        ctr_add(a.ctr + WCTR + slot, K);
+        ctr_add(a.ctr + 2 * WCTR + slot, A);
+    }
+    if (it.row_ovf) flag_or(a.flags, FLAG_ROW_OVERFLOW);
+    if (mass == 0) flag_or(a.flags, FLAG_EMPTY);
+    return 0;
+#undef PX
+#undef PY
+#undef PI_
+}
+
+
+}  // namespace c2
+}  // namespace sdot_amd

@@ -1,0 +1,246 @@
+// 2D Laguerre-cell kernel: one thread per cell (sorted order, so a wavefront
+// covers a compact patch of the grid and its neighbour reads hit the same
+// cache lines), with the cell polygon held in that thread's LDS slot.
+//
+// Per cell (semantics of src/sdot/PowerDiagram/Visitors/SpZGrid.tcc:494-645):
+//   * start from the domain box (ConvexPolyhedronAssembly with one box,
+//     ConvexPolyhedronAssembly.tcc:64-65), in coordinates local to the dirac;
+//   * visit grid cells ring by ring; a ring (and every further one) or a grid
+//     cell is skipped when no dirac in it can cut the current polygon -- the
+//     exact power-distance bound of SpZGrid::can_be_evicted (:475-480) taken
+//     over the disc of radius R (max vertex distance) that holds the polygon;
+//   * cut by the radical half-plane of each remaining candidate
+//     (SpZGrid.tcc:506-522; ConvexPolyhedron2::plane_cut :687-922);
+//   * in ball mode, clip by the disc of radius sqrt(w_i) streaming the
+//     LINE/ARC items of ConvexPolyhedron2::ball_cut (:534-665) straight into the
+//     integrals (area :1791-1842, _arc_area :2361-2380, boundary items
+//     :437-478) and the Hessian row (get_der_integrals_wrt_weights.h:52-75).
+// T = double (FP64 path) or float (mixed path: FP32 local coordinates; plane
+// offsets, distances and the mass/Hessian outputs in FP64).
+#include <cfloat>
+#include <cmath>
+
+#include "cell2d_impl.h"
+
+namespace sdot_amd {
+
+namespace {
+
+using namespace c2;
+
+template <class T, int CAP, int BS, bool BALL>
+__global__ __launch_bounds__(BS) void k_cells2(CellArgs a) {
+    __shared__ T sx[CAP * BS];
+    __shared__ T sy[CAP * BS];
+    __shared__ int sid[CAP * BS];
+    int64_t t;
+    if (a.work) {
+        int64_t k = blockIdx.x * (int64_t)BS + threadIdx.x;
+        if (k >= a.work_n) return;
+        t = a.work[k];
+    } else {
+        t = xcd_tile(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+        if (t >= a.N) return;
+    }
+    if (cell2_body<T, CAP, BALL>(a, t, sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x, BS)) {
+        unsigned k = atomicAdd(a.ovf_count, 1u);
+        a.ovf_list[k] = (int)t;
+    }
+}
+
+// ---- dirac-order export ----
+__global__ void k_export_vec(const double *v, const int *inv_perm, int64_t n, double *out) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = v[inv_perm[i]];
+}
+
+__global__ void k_csr_count(const int *row_len, const int *inv_perm, int64_t n, int64_t *cnt) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) cnt[i] = row_len[inv_perm[i]] + 1;
+}
+
+template <int MAXE>
+__global__ void k_csr_fill(const int *row_len, const int *ell_col, const double *ell_val, int64_t stride,
+                           const double *diag, const int *perm, const int *inv_perm, int64_t n, const int64_t *off,
+                           int64_t *cols, double *vals) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t t = inv_perm[i];
+    const int L = row_len[t];
+    int64_t c[MAXE];
+    double v[MAXE];
+    int m = 0;
+    c[m] = i;
+    v[m] = diag[t];
+    ++m;
+    for (int e = 0; e < L && m < MAXE; ++e) {
+        int64_t cc = perm[ell_col[e * stride + t]];
+        double vv = ell_val[e * stride + t];
+        // insertion by (col, val)
+        int k = m;
+        while (k > 0 && (c[k - 1] > cc || (c[k - 1] == cc && v[k - 1] > vv))) {
+            c[k] = c[k - 1];
+            v[k] = v[k - 1];
+            --k;
+        }
+        c[k] = cc;
+        v[k] = vv;
+        ++m;
+    }
+    // the diagonal went in first; keep (col,val) order for it as well
+    for (int k = 1; k < m; ++k) {
+        if (c[k - 1] > c[k] || (c[k - 1] == c[k] && v[k - 1] > v[k])) {
+            int64_t tc = c[k];
+            double tv = v[k];
+            int q = k;
+            while (q > 0 && (c[q - 1] > tc || (c[q - 1] == tc && v[q - 1] > tv))) {
+                c[q] = c[q - 1];
+                v[q] = v[q - 1];
+                --q;
+            }
+            c[q] = tc;
+            v[q] = tv;
+        }
+    }
+    int64_t o = off[i];
+    int w = 0;
+    for (int k = 0; k < m; ++k) {
+        if (k + 1 < m && c[k] == c[k + 1]) {
+            v[k + 1] += v[k];
+            continue;
+        }
+        cols[o + w] = c[k];
+        vals[o + w] = v[k];
+        ++w;
+    }
+    for (int64_t k = w; k < m; ++k) {   // merged duplicates leave padding: mark it
+        cols[o + k] = -1;
+        vals[o + k] = 0;
+    }
+}
+
+template <class T, int CAP, int BS, bool BALL>
+void launch2(const CellArgs &a, int64_t nthreads, hipStream_t s) {
+    unsigned nb = (unsigned)((nthreads + BS - 1) / BS);
+    if (nb == 0) return;
+    hipLaunchKernelGGL((k_cells2<T, CAP, BS, BALL>), dim3(nb), dim3(BS), 0, s, a);
+    HIPCK(hipGetLastError());
+}
+
+}  // namespace
+
+// dispatch over precision / ball: main pass with a small LDS polygon, then the
+// overflowed cells again with a wide one
+void run_cells2(sdot_pd *pd, CellArgs &a) {
+    hipStream_t s = pd->stream;
+    const bool ball = pd->ball != 0;
+    const bool mixed = pd->precision == SDOT_MIXED;
+    a.work = nullptr;
+    a.work_n = 0;
+    if (mixed) {
+        if (ball) launch2<float, 16, 64, true>(a, a.N, s);
+        else launch2<float, 16, 64, false>(a, a.N, s);
+    } else {
+        if (ball) launch2<double, 16, 64, true>(a, a.N, s);
+        else launch2<double, 16, 64, false>(a, a.N, s);
+    }
+    HIPCK(hipEventRecord(pd->ev[3], s));
+    unsigned cnt[2] = {0, 0};
+    HIPCK(hipMemcpyAsync(cnt, pd->flags.p, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    pd->last.nb_cells_overflow = cnt[1];
+    if (cnt[1] > 0) {
+        // the overflow list is copied so the wide pass can append to a fresh counter
+        pd->ovf_list.ensure(2 * (size_t)a.N + 2);
+        int *list2 = pd->ovf_list.p + a.N + 1;
+        HIPCK(hipMemcpyAsync(list2, pd->ovf_list.p, cnt[1] * sizeof(int), hipMemcpyDeviceToDevice, s));
+        unsigned zero = 0;
+        HIPCK(hipMemcpyAsync(pd->flags.p + 1, &zero, sizeof(unsigned), hipMemcpyHostToDevice, s));
+        a.work = list2;
+        a.work_n = cnt[1];
+        if (mixed) {
+            if (ball) launch2<float, 64, 64, true>(a, cnt[1], s);
+            else launch2<float, 64, 64, false>(a, cnt[1], s);
+        } else {
+            if (ball) launch2<double, 64, 64, true>(a, cnt[1], s);
+            else launch2<double, 64, 64, false>(a, cnt[1], s);
+        }
+        unsigned c2 = 0;
+        HIPCK(hipMemcpyAsync(&c2, pd->flags.p + 1, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        if (c2 > 0) throw std::string("cell polygon exceeds 64 vertices");
+        a.work = nullptr;
+    }
+}
+
+void export_vec(sdot_pd *pd, const double *d_sorted, double *h_out) {
+    const int64_t N = pd->N;
+    pd->out_tmp.ensure(N);
+    hipLaunchKernelGGL(k_export_vec, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, pd->stream, d_sorted, pd->inv_perm.p,
+                       N, pd->out_tmp.p);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(h_out, pd->out_tmp.p, N * sizeof(double), hipMemcpyDeviceToHost, pd->stream));
+    HIPCK(hipStreamSynchronize(pd->stream));
+}
+
+int64_t build_csr_device(sdot_pd *pd) {
+    const int64_t N = pd->N;
+    hipStream_t s = pd->stream;
+    pd->csr_cnt.ensure(N);
+    pd->csr_off.ensure(N + 1);
+    pd->csr_tmp.ensure(scan_tmp_size(N));
+    const unsigned nb = (unsigned)((N + 255) / 256);
+    hipLaunchKernelGGL(k_csr_count, dim3(nb), dim3(256), 0, s, pd->row_len.p, pd->inv_perm.p, N, pd->csr_cnt.p);
+    HIPCK(hipGetLastError());
+    exclusive_scan<int64_t>(pd->csr_cnt.p, pd->csr_off.p, N, pd->csr_tmp.p, s);
+    // upper bound of the entries: every ELL slot of every row plus the diagonal
+    const int64_t bound = (int64_t)N * (pd->ell_cap + 1);
+    pd->csr_col.ensure(bound);
+    pd->csr_val.ensure(bound);
+    if (pd->ell_cap + 1 <= 33)
+        hipLaunchKernelGGL(k_csr_fill<33>, dim3(nb), dim3(256), 0, s, pd->row_len.p, pd->ell_col.p, pd->ell_val.p,
+                           (int64_t)N, pd->diag.p, pd->perm.p, pd->inv_perm.p, N, pd->csr_off.p, pd->csr_col.p,
+                           pd->csr_val.p);
+    else
+        hipLaunchKernelGGL(k_csr_fill<97>, dim3(nb), dim3(256), 0, s, pd->row_len.p, pd->ell_col.p, pd->ell_val.p,
+                           (int64_t)N, pd->diag.p, pd->perm.p, pd->inv_perm.p, N, pd->csr_off.p, pd->csr_col.p,
+                           pd->csr_val.p);
+    HIPCK(hipGetLastError());
+    return bound;
+}
+
+void export_csr(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, int64_t cap, int64_t *nnz) {
+    const int64_t N = pd->N;
+    hipStream_t s = pd->stream;
+    build_csr_device(pd);
+    int64_t total = 0;
+    HIPCK(hipMemcpyAsync(&total, pd->csr_off.p + N, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    if (!h_col) {
+        *nnz = total;
+        return;
+    }
+    std::vector<int64_t> off(N + 1);
+    std::vector<int64_t> col(total);
+    std::vector<double> val(total);
+    HIPCK(hipMemcpyAsync(off.data(), pd->csr_off.p, (N + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(col.data(), pd->csr_col.p, total * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(val.data(), pd->csr_val.p, total * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    // rows whose duplicate columns were merged carry -1 padding: squeeze it out
+    int64_t w = 0;
+    for (int64_t i = 0; i < N; ++i) {
+        h_off[i] = w;
+        for (int64_t k = off[i]; k < off[i + 1]; ++k) {
+            if (col[k] < 0) continue;
+            if (w >= cap) throw std::string("CSR capacity too small");
+            h_col[w] = col[k];
+            h_val[w] = val[k];
+            ++w;
+        }
+    }
+    h_off[N] = w;
+    *nnz = w;
+}
+
+}  // namespace sdot_amd

@@ -1,0 +1,256 @@
+// Internal declarations of the MI355X Laguerre-cell library (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sdot_mi355x.h"
+
+namespace sdot_amd {
+
+// ---------------------------------------------------------------------------
+// error handling: every HIP call goes through HIPCK, which records the error
+// text on the handle and returns SDOT_HIP_ERROR from the entry point.
+// ---------------------------------------------------------------------------
+struct HipError {
+    hipError_t err;
+    const char *what;
+    int line;
+};
+
+#define HIPCK(call)                                                        \
+    do {                                                                   \
+        hipError_t e_ = (call);                                            \
+        if (e_ != hipSuccess) throw ::sdot_amd::HipError{e_, #call, __LINE__}; \
+    } while (0)
+
+struct OomError {};
+
+struct StatusError {
+    sdot_status st;
+    std::string msg;
+};
+
+// grow-only device buffer
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void ensure(size_t m) {
+        if (m <= n && p) return;
+        if (p) HIPCK(hipFree(p));
+        p = nullptr;
+        n = 0;
+        size_t want = m > 0 ? m : 1;
+        if (hipMalloc((void **)&p, want * sizeof(T)) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            throw OomError{};
+        }
+        n = want;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// Uniform grid over the dirac bounding box; cells are addressed by the Morton
+// code of their integer coordinates, so a sort by code groups each cell's
+// diracs contiguously and keeps nearby cells close in memory.
+struct GridInfo {
+    int dim;
+    int bits;             // bits per axis of the Morton code
+    int ncell[3];
+    double lo[3];
+    double h[3];
+    double inv_h[3];
+    int64_t ncodes;       // 2^(dim*bits)
+};
+
+__host__ __device__ inline uint32_t spread2(uint32_t x) {
+    x &= 0xFFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+__host__ __device__ inline uint32_t spread3(uint32_t x) {
+    x &= 0x3FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+__host__ __device__ inline uint32_t morton2(int x, int y) { return spread2(x) | (spread2(y) << 1); }
+__host__ __device__ inline uint32_t morton3(int x, int y, int z) {
+    return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+}
+__host__ __device__ inline int grid_coord(const GridInfo &g, int d, double x) {
+    double f = floor((x - g.lo[d]) * g.inv_h[d]);
+    int c = (int)f;
+    if (f < 0) c = 0;
+    if (c >= g.ncell[d]) c = g.ncell[d] - 1;
+    return c;
+}
+
+// blockIdx -> work tile mapping that gives each of the 8 XCDs (dispatched
+// round-robin) one contiguous range of tiles, so neighbouring tiles share an L2.
+__device__ inline int64_t xcd_tile(int64_t b, int64_t nb) {
+    int64_t q = nb / 8, rem = nb % 8, x = b % 8, i = b / 8;
+    return x < rem ? x * (q + 1) + i : rem * (q + 1) + (x - rem) * q + i;
+}
+
+// order-preserving map double -> uint64 (for atomicMax over doubles)
+__host__ __device__ inline unsigned long long dbl_order(double v) {
+    unsigned long long b;
+    memcpy(&b, &v, 8);
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double dbl_unorder(unsigned long long b) {
+    b = (b & 0x8000000000000000ull) ? (b & 0x7FFFFFFFFFFFFFFFull) : ~b;
+    double v;
+    memcpy(&v, &b, 8);
+    return v;
+}
+
+enum : unsigned { FLAG_EMPTY = 1u, FLAG_ROW_OVERFLOW = 2u, FLAG_BAD_INPUT = 4u };
+
+#define SDOT_HD __host__ __device__
+
+SDOT_HD inline void ctr_add(unsigned long long *p, unsigned long long v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicAdd(p, v);
+#else
+    *p += v;
+#endif
+}
+
+// flag update usable from host+device code (the device side is atomic)
+SDOT_HD inline void flag_or(unsigned *p, unsigned v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicOr(p, v);
+#else
+    *p |= v;
+#endif
+}
+
+// arguments of the cell kernels
+struct CellArgs {
+    int64_t N;
+    const double *spos;          // sorted positions, interleaved
+    const double *sw;            // sorted weights
+    const uint32_t *cell_start;  // by Morton code, ncodes + 1 entries
+    const double *cell_wmax;     // by Morton code
+    const unsigned long long *wglob;  // ordered bits of the global max weight
+    GridInfo g;
+    double dom_lo[3], dom_hi[3];
+    double coeff;
+    // piecewise-constant image density (3D voxel / 2D pixel), null when constant
+    const double *img;
+    int img_n[3];
+    // outputs, sorted order
+    double *mass, *arc, *perim, *diag;
+    int *row_len;
+    int *ell_col;
+    double *ell_val;
+    int ell_cap;
+    int64_t ell_stride;
+    unsigned *flags;
+    int *ovf_list;
+    unsigned *ovf_count;
+    const int *work;             // optional list of sorted indices to process
+    int64_t work_n;
+    unsigned long long *ctr;     // work counters: [k] = sum F, [WCTR + k] = sum K, [2*WCTR + k] = sum A
+};
+constexpr int WCTR = 1024;       // counters are spread over WCTR slots to avoid contention
+
+// ---------------------------------------------------------------------------
+// the handle
+// ---------------------------------------------------------------------------
+struct Shape {
+    double p0[3], p1[3];
+    double coeff;
+    int64_t cut_id;
+};
+
+}  // namespace sdot_amd
+
+struct sdot_pd {
+    int dim = 2;
+    int precision = SDOT_FP64;
+    int ball = 0;
+    int dpc_x100 = 0;  // tuning: diracs per grid cell x100 (0 = auto)
+    int block = 0;     // tuning: threads per block (0 = auto)
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // domain
+    std::vector<sdot_amd::Shape> shapes;
+    bool has_image = false;
+    double img_lo[3] = {0, 0, 0}, img_hi[3] = {1, 1, 1};
+    int img_n[3] = {1, 1, 1};
+    sdot_amd::DevBuf<double> img;
+
+    // diracs (original order)
+    int64_t N = 0;
+    sdot_amd::DevBuf<double> pos, w;
+    bool grid_valid = false;    // positions unchanged since the last sort
+    bool pass_valid = false;    // results match the current diracs/weights/settings
+
+    // grid
+    sdot_amd::GridInfo g{};
+    sdot_amd::DevBuf<uint32_t> keys, rank, cell_start, scan_tmp;
+    sdot_amd::DevBuf<int> perm, inv_perm;
+    sdot_amd::DevBuf<double> spos, sw, cell_wmax, bbox_part;
+    sdot_amd::DevBuf<unsigned long long> wglob;
+
+    // pass outputs (sorted order)
+    sdot_amd::DevBuf<double> mass, arc, perim, diag, ell_val;
+    sdot_amd::DevBuf<int> row_len, ell_col, ovf_list;
+    sdot_amd::DevBuf<unsigned> flags;  // [0] flags, [1] overflow count
+    int ell_cap = 0;
+    unsigned last_flags = 0;
+    sdot_amd::DevBuf<unsigned long long> wctr;
+
+    // CSR export scratch
+    sdot_amd::DevBuf<int64_t> csr_off, csr_col, csr_cnt, csr_tmp;
+    sdot_amd::DevBuf<double> csr_val, out_tmp;
+
+    // solver scratch
+    sdot_amd::DevBuf<double> sv[10];
+    sdot_amd::DevBuf<double> red;
+
+    // timing
+    hipEvent_t ev[8] = {};
+    sdot_timings last{};
+};
+
+namespace sdot_amd {
+// grid.hip
+void build_grid(sdot_pd *pd);         // bbox, keys, sort, sorted positions (positions changed)
+void refresh_weights(sdot_pd *pd);    // sorted weights, per-cell and global max weight
+template <class T>
+void exclusive_scan(const T *in, T *out, int64_t n, T *tmp, hipStream_t s);  // out has n+1 entries
+size_t scan_tmp_size(int64_t n);
+// cells.hip
+void run_cells(sdot_pd *pd);          // cell kernels for the current dim / precision / ball
+int64_t build_csr_device(sdot_pd *pd);  // dirac-order CSR in HBM (csr_off/csr_col/csr_val); returns slot bound
+void run_cells2(sdot_pd *pd, CellArgs &a);
+void run_cells3(sdot_pd *pd, CellArgs &a);
+void fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
+void export_csr(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, int64_t cap, int64_t *nnz);
+void export_vec(sdot_pd *pd, const double *d_sorted, double *h_out);  // sorted -> dirac order, to host
+// capi.hip
+sdot_status compute_pass(sdot_pd *pd);  // grid (if needed) + weights + cell pass; flags -> status
+// solver.hip
+sdot_status ot_solve(sdot_pd *pd, const double *targets, double tol, int stop_rule, int max_iter, double *w_inout,
+                     sdot_solve_stats *stats);
+}  // namespace sdot_amd

@@ -1,0 +1,97 @@
+"""GPU parity of the 2D cell pass (masses, boundary measures, Hessian CSR)
+against the CPU oracle, FP64 and mixed precision, with and without the ball."""
+import math
+
+import numpy as np
+import pytest
+
+from tests.gpu_parity import compare, csr_matrix, gpu_pass, inputs, oracle_pass
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("ball", [False, True])
+@pytest.mark.parametrize("n,seed", [(2000, 0), (30000, 1)])
+def test_cells2d_parity(precision, ball, n, seed):
+    if ball:
+        pos, w = inputs(2, n, seed, wscale=1.0 / n, wshift=1.0 / n)      # C2 shape: w in [1,2]/N
+    else:
+        pos, w = inputs(2, n, seed, wscale=0.3 / n)
+    g = gpu_pass(pos, w, 2, precision, ball)
+    o = oracle_pass(pos, w, 2, ball)
+    hess_rtol = 1e-6 if precision == "fp64" else 1e-4
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol)
+    print(f"[parity2d] {precision} ball={ball} n={n}: {err}")
+
+
+def test_cells2d_c2_full_size_fp64_and_mixed():
+    """1M diracs, w ~ U[0,1e-6] + r0^2 with r0 = 1e-3 (the ball-clipped workload)."""
+    n = 1_000_000
+    pos, w = inputs(2, n, 0, wscale=1e-6, wshift=1e-6)
+    o = oracle_pass(pos, w, 2, ball=True)
+    for prec in ("fp64", "mixed"):
+        g = gpu_pass(pos, w, 2, prec, ball=True)
+        err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if prec == "fp64" else 1e-4)
+        print(f"[parity2d C2] {prec}: {err} timings {g['timings']}")
+        # size-independent properties: symmetry, row sums = ball terms, disc bound
+        M = csr_matrix(g["csr"], n)
+        assert abs(M - M.T).max() < 1e-9 * abs(M).max()
+        rs = np.asarray(M.sum(axis=1)).ravel()
+        assert np.abs(rs - 0.5 * g["ball_arc"] / np.sqrt(w)).max() < 1e-9
+        assert (g["masses"] <= math.pi * w * (1 + 1e-12)).all()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cells2d_edge_cases(precision):
+    # a single dirac: the cell is the whole box
+    g = gpu_pass(np.array([[0.3, 0.6]]), np.array([0.0]), 2, precision)
+    assert abs(g["masses"][0] - 1) < 1e-14 and len(g["csr"][1]) == 1
+    # two diracs, unequal weights: exact split x = 0.5 + (w0 - w1)/(2*0.5) shifted
+    pos = np.array([[0.25, 0.5], [0.75, 0.5]])
+    g = gpu_pass(pos, np.array([0.05, 0.0]), 2, precision)
+    xs = 0.5 + 0.5 * 0.05 / 0.5
+    assert abs(g["masses"][0] - xs) < 1e-13
+    assert abs(g["csr"][2][1] + 0.5 * 1 / 0.5) < 1e-13
+    # a regular lattice (every vertex is a 4-way tie)
+    k = 40
+    gx = (np.arange(k) + 0.5) / k
+    pos = np.array(np.meshgrid(gx, gx, indexing="ij")).reshape(2, -1).T.copy()
+    g = gpu_pass(pos, np.zeros(k * k), 2, precision)
+    assert np.abs(g["masses"] * k * k - 1).max() < 1e-9
+    M = csr_matrix(g["csr"], k * k).toarray()
+    off = M[~np.eye(k * k, dtype=bool)]
+    assert np.allclose(off[np.abs(off) > 1e-9], -0.5, atol=1e-9)
+    # diracs outside the domain box and a clustered distribution
+    rng = np.random.default_rng(4)
+    pos = np.concatenate([rng.uniform(-0.2, 1.2, size=(3000, 2)), 0.5 + 0.01 * rng.normal(size=(3000, 2))])
+    w = np.zeros(len(pos))
+    g = gpu_pass(pos, w, 2, precision)
+    o = oracle_pass(pos, w, 2)
+    compare(g, o, len(pos), 1.0 / len(pos), hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+
+
+def test_cells2d_empty_cell_and_bad_input():
+    import sdot_amd
+    pos = np.array([[0.3, 0.5], [0.5, 0.5], [0.7, 0.5]])
+    g = gpu_pass(pos, np.array([0.0, -1.0, 0.0]), 2)
+    assert g["status"] == sdot_amd.SDOT_EMPTY_CELL
+    assert g["masses"][1] == 0
+    pd = sdot_amd.PowerDiagram(2)
+    pd.add_box_shape([0, 0], [1, 1])
+    pd.set_diracs(np.array([[0.5, 0.5], [0.5, 0.5]]), np.zeros(2))
+    with pytest.raises(sdot_amd.SdotError):
+        pd.compute()
+
+
+def test_cells2d_density_coefficient_and_domain():
+    pos, w = inputs(2, 5000, 8, wscale=1e-5)
+    for coeff, lo, hi in ((2.5, [0, 0], [1, 1]), (1.0, [-1, 0.5], [2, 1.5])):
+        import sdot_amd
+        pd = sdot_amd.PowerDiagram(2)
+        pd.add_box_shape(lo, hi, coeff)
+        p = pos * (np.array(hi) - lo) + lo
+        pd.set_diracs(p, w)
+        m = pd.integrals()
+        o = __import__("oracle.oracle", fromlist=["x"]).cells(p, w, domain=(lo, hi, coeff))
+        assert np.abs(m - o["masses"]).max() < 1e-12 * coeff * np.prod(np.array(hi) - lo)

@@ -1,0 +1,60 @@
+"""GPU Newton solve (device FP64 Jacobi-PCG) against the oracle's solve: the
+converged weights are unique up to an additive constant, so they are compared
+after removing the mean; the masses at the returned weights must hit the
+targets."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_parity import inputs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_newton_2d_c1(precision):
+    """N = 10^4 uniform, w0 = 0, uniform target, stop at max relative mass error 1e-6."""
+    import sdot_amd
+    n = 10_000
+    pos, _ = inputs(2, n, 0)
+    t = np.full(n, 1.0 / n)
+    ot = sdot_amd.OptimalTransport(pos, t, precision=precision)
+    w = ot.adjust_weights(tol=1e-6)
+    assert ot.status == sdot_amd.SDOT_OK, ot.stats
+    assert ot.stats["max_rel_mass_err"] < 1e-6
+    # masses recomputed by the FP64 oracle at the GPU's weights
+    m = O.cells(pos, w)["masses"]
+    assert np.abs(m / t - 1).max() < 1.5e-6
+    st, wo, so = O.ot_solve(pos, np.zeros(n), t, tol=1e-6)
+    assert st == 0
+    d1, d2 = w - w.mean(), wo - wo.mean()
+    # both are 1e-6 solutions: weights agree to the conditioning of the problem
+    assert np.abs(d1 - d2).max() < 1e-3 * np.abs(d2).max()
+    print(f"[newton C1] {precision}: iters {ot.stats['newton_iters']} pcg {ot.stats['pcg_iters_total']} "
+          f"err {ot.stats['max_rel_mass_err']:.2e} t {ot.stats['seconds_total']:.3f}s; oracle iters {so.newton_iters}")
+
+
+def test_newton_2d_tight_tolerance_matches_oracle_weights():
+    import sdot_amd
+    n = 3000
+    pos, _ = inputs(2, n, 5)
+    t = 1 + 0.5 * O.rng_uniform(6, n)
+    t /= t.sum()
+    ot = sdot_amd.OptimalTransport(pos, t)
+    w = ot.adjust_weights(tol=1e-11)
+    st, wo, _ = O.ot_solve(pos, np.zeros(n), t, tol=1e-11)
+    assert ot.status == 0 and st == 0
+    d1, d2 = w - w.mean(), wo - wo.mean()
+    assert np.abs(d1 - d2).max() < 1e-6 * np.abs(d2).max()
+
+
+def test_newton_ball_mode():
+    import sdot_amd
+    n = 2000
+    pos, _ = inputs(2, n, 7)
+    t = np.full(n, 0.5 / n)        # half the box covered by the discs
+    ot = sdot_amd.OptimalTransport(pos, t, ball=True, weights=np.full(n, 0.6 / n / np.pi))
+    w = ot.adjust_weights(tol=1e-8)
+    assert ot.status == 0, ot.stats
+    m = O.cells(pos, w, ball=True)["masses"]
+    assert np.abs(m / t - 1).max() < 1e-7

@@ -1,0 +1,151 @@
+"""The CPU oracle's cell pass, Hessian and Newton solve against brute force,
+closed forms, invariants and finite differences."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import oracle as O
+
+
+def csr(off, cols, vals, n):
+    return sp.csr_matrix((vals, cols, off), shape=(n, n))
+
+
+def inputs(dim, n, seed, wscale=0.0, wshift=0.0):
+    u = O.rng_uniform(seed, n * dim + n)
+    return u[: n * dim].reshape(n, dim), wshift + wscale * u[n * dim:]
+
+
+@pytest.mark.parametrize("dim,ball", [(2, False), (2, True), (3, False)])
+def test_grid_equals_brute_force(dim, ball):
+    n = 400
+    pos, w = inputs(dim, n, 11, wscale=(0.3 / n) ** (2 / dim), wshift=(0.5 / n) ** (2 / dim) if ball else 0)
+    rb = O.cells(pos, w, ball=ball, brute=True)
+    rg = O.cells(pos, w, ball=ball)
+    assert rb["status"] == rg["status"]
+    assert np.abs(rb["masses"] - rg["masses"]).max() < 1e-14
+    Mb = csr(*O.to_csr(rb), n)
+    Mg = csr(*O.to_csr(rg), n)
+    assert abs(Mb - Mg).max() < 1e-11
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_mass_sum_symmetry_rowsums(dim):
+    n = 2000
+    pos, w = inputs(dim, n, 2, wscale=0.02 * (1.0 / n) ** (2 / dim))
+    r = O.cells(pos, w)
+    assert r["status"] == 0
+    assert abs(r["masses"].sum() - 1) < 1e-12
+    M = csr(*O.to_csr(r), n)
+    assert abs(M - M.T).max() < 1e-10
+    assert np.abs(np.asarray(M.sum(axis=1))).max() < 1e-10 * abs(M).max()
+    assert (M.diagonal() > 0).all()
+
+
+def test_ball_rowsum_is_arc_term():
+    # with the ball, row sum = 0.5 * arc_i / sqrt(w_i) (get_der_integrals_wrt_weights.h:58-59)
+    n = 3000
+    pos, w = inputs(2, n, 3, wscale=1.0 / n, wshift=1.0 / n)
+    r = O.cells(pos, w, ball=True)
+    M = csr(*O.to_csr(r), n)
+    rs = np.asarray(M.sum(axis=1)).ravel()
+    assert np.abs(rs - 0.5 * r["ball_arc"] / np.sqrt(w)).max() < 1e-10
+    assert (r["masses"] <= math.pi * w * (1 + 1e-12)).all()
+
+
+def test_isolated_dirac_ball():
+    # one dirac far from the box edges: m = pi w, arc = 2 pi sqrt(w), M_ii = pi
+    pos = np.array([[0.5, 0.5]])
+    for w in (0.01, 0.04, 0.2):
+        st, off, cols, vals, m = O.der_integrals_wrt_weights(pos, np.array([w]), ball=True)
+        assert abs(m[0] - math.pi * w) < 1e-13
+        assert abs(vals[0] - math.pi) < 1e-13
+        r = O.cells(pos, np.array([w]), ball=True)
+        assert abs(r["ball_arc"][0] - 2 * math.pi * math.sqrt(w)) < 1e-13
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lattice_closed_form(dim):
+    k = 6
+    g = (np.arange(k) + 0.5) / k
+    pos = np.array(np.meshgrid(*([g] * dim), indexing="ij")).reshape(dim, -1).T.copy()
+    n = len(pos)
+    r = O.cells(pos, np.zeros(n))
+    assert np.abs(r["masses"] - 1.0 / n).max() < 1e-14
+    M = csr(*O.to_csr(r), n).toarray()
+    h = 1.0 / k
+    off = M[~np.eye(n, dtype=bool)]
+    face = 0.5 * h ** (dim - 1) / h
+    nz = off[np.abs(off) > 1e-12]
+    assert np.allclose(nz, -face, atol=1e-12)
+
+
+@pytest.mark.parametrize("dim,ball", [(2, False), (2, True), (3, False)])
+def test_finite_differences(dim, ball):
+    n = 150
+    pos, w = inputs(dim, n, 9, wscale=0.3 * (1.0 / n) ** (2 / dim), wshift=(0.6 / n) ** (2 / dim) if ball else 0)
+    r = O.cells(pos, w, ball=ball)
+    M = csr(*O.to_csr(r), n).toarray()
+    eps = 1e-7 * (1.0 / n) ** (2 / dim)
+    for j in (3, 57, 101):
+        wp, wm = w.copy(), w.copy()
+        wp[j] += eps
+        wm[j] -= eps
+        fd = (O.cells(pos, wp, ball=ball)["masses"] - O.cells(pos, wm, ball=ball)["masses"]) / (2 * eps)
+        assert np.abs(fd - M[:, j]).max() < 1e-5 * np.abs(M[:, j]).max()
+
+
+def test_image_density_closed_form_2d():
+    # 2x2 image, 4 diracs on the quadrant centres -> each cell is one quadrant
+    img = np.array([[1.0, 2.0], [3.0, 4.0]])        # [y][x]
+    pos = np.array([[0.25, 0.25], [0.75, 0.25], [0.25, 0.75], [0.75, 0.75]])
+    r = O.cells(pos, np.zeros(4), img=img)
+    assert np.allclose(r["masses"], np.array([1, 2, 3, 4]) / 4.0, atol=1e-15)
+    # finer image, off-centre diracs: sum of masses = mean density x box area
+    rng = np.random.default_rng(0)
+    img = rng.uniform(0.5, 2, size=(7, 5))
+    pos, w = inputs(2, 300, 4, wscale=1e-4)
+    r = O.cells(pos, w, img=img)
+    assert abs(r["masses"].sum() - img.mean()) < 1e-12
+
+
+def test_image_density_3d_sum_and_hessian():
+    rng = np.random.default_rng(1)
+    img = np.exp(0.5 * rng.normal(size=(4, 5, 3)))
+    n = 300
+    pos, w = inputs(3, n, 5, wscale=0.1 * (1.0 / n) ** (2 / 3))
+    r = O.cells(pos, w, img=img)
+    assert abs(r["masses"].sum() - img.mean()) < 1e-12
+    M = csr(*O.to_csr(r), n)
+    assert abs(M - M.T).max() < 1e-10
+    assert np.abs(np.asarray(M.sum(axis=1))).max() < 1e-9
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_newton_reaches_targets(dim):
+    n = 500
+    pos, _ = inputs(dim, n, 21)
+    st, w, stats = O.ot_solve(pos, np.zeros(n), np.full(n, 1.0 / n), tol=1e-9)
+    assert st == 0, stats.max_rel_mass_err
+    m = O.cells(pos, w)["masses"]
+    assert np.abs(m * n - 1).max() < 1e-9
+
+
+def test_newton_non_uniform_targets_unique_up_to_constant():
+    n = 400
+    pos, _ = inputs(2, n, 22)
+    t = 1 + 0.5 * O.rng_uniform(23, n)
+    t /= t.sum()
+    st1, w1, _ = O.ot_solve(pos, np.zeros(n), t, tol=1e-10)
+    st2, w2, _ = O.ot_solve(pos, 0.0002 * O.rng_uniform(24, n), t, tol=1e-10)
+    assert st1 == 0 and st2 == 0
+    d1, d2 = w1 - w1.mean(), w2 - w2.mean()
+    assert np.abs(d1 - d2).max() < 1e-8 * np.abs(d1).max()
+
+
+def test_empty_cell_status():
+    pos = np.array([[0.3, 0.5], [0.5, 0.5], [0.7, 0.5]])
+    w = np.array([0.0, -1.0, 0.0])
+    assert O.cells(pos, w)["status"] == O.OR_EMPTY_CELL
