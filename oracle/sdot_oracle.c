@@ -467,11 +467,12 @@ static void grid_build(or_grid *g, int dim, int64_t N, const double *pos, const 
             lo[d] = fmin(lo[d], pos[dim * i + d]);
             hi[d] = fmax(hi[d], pos[dim * i + d]);
         }
-    double vol = 1;
+    double vol = 1, lmax = 0;
+    for (int d = 0; d < dim; ++d) lmax = fmax(lmax, hi[d] - lo[d]);
     for (int d = 0; d < dim; ++d) {
         double l = hi[d] - lo[d];
-        if (!(l > 0)) l = 1e-300;
         double pad = 1e-9 * l + 1e-300;
+        if (l + 2 * pad < 1e-3 * lmax) pad = 0.5 * (1e-3 * lmax - l);   /* flat inputs: keep cells isotropic */
         lo[d] -= pad; hi[d] += pad;
         vol *= hi[d] - lo[d];
     }
@@ -642,11 +643,10 @@ static int build_cell2(const cells_job *J, int64_t i, or_cp2 *cp) {
                 }
                 if (stop) break;
             }
-            for (int64_t gy = hy - r; gy <= hy + r; ++gy) {
-                if (gy < 0 || gy >= g->ncell[1]) continue;
-                for (int64_t gx = hx - r; gx <= hx + r; ++gx) {
+            for (int64_t gy = hy - r < 0 ? 0 : hy - r; gy <= hy + r && gy < g->ncell[1]; ++gy) {
+                int full = llabs(gy - hy) == r;
+                for (int64_t gx = hx - r; gx <= hx + r; gx += (full || r == 0) ? 1 : 2 * r) {
                     if (gx < 0 || gx >= g->ncell[0]) continue;
-                    if (llabs(gx - hx) != r && llabs(gy - hy) != r) continue;
                     int64_t c = gy * g->ncell[0] + gx;
                     if (g->start[c] == g->start[c + 1]) continue;
                     double bx0 = g->lo[0] + gx * g->h[0], by0 = g->lo[1] + gy * g->h[1];
@@ -791,13 +791,11 @@ static int build_cell3(const cells_job *J, int64_t i, or_cp3 *cp) {
                     }
                 if (stop) break;
             }
-            for (int64_t gz = h[2] - r; gz <= h[2] + r; ++gz) {
-                if (gz < 0 || gz >= g->ncell[2]) continue;
-                for (int64_t gy = h[1] - r; gy <= h[1] + r; ++gy) {
-                    if (gy < 0 || gy >= g->ncell[1]) continue;
-                    for (int64_t gx = h[0] - r; gx <= h[0] + r; ++gx) {
+            for (int64_t gz = h[2] - r < 0 ? 0 : h[2] - r; gz <= h[2] + r && gz < g->ncell[2]; ++gz) {
+                for (int64_t gy = h[1] - r < 0 ? 0 : h[1] - r; gy <= h[1] + r && gy < g->ncell[1]; ++gy) {
+                    int full = llabs(gz - h[2]) == r || llabs(gy - h[1]) == r;
+                    for (int64_t gx = h[0] - r; gx <= h[0] + r; gx += (full || r == 0) ? 1 : 2 * r) {
                         if (gx < 0 || gx >= g->ncell[0]) continue;
-                        if (llabs(gx - h[0]) != r && llabs(gy - h[1]) != r && llabs(gz - h[2]) != r) continue;
                         int64_t c = (gz * g->ncell[1] + gy) * g->ncell[0] + gx;
                         if (g->start[c] == g->start[c + 1]) continue;
                         double b0[3] = {g->lo[0] + gx * g->h[0], g->lo[1] + gy * g->h[1], g->lo[2] + gz * g->h[2]};
@@ -822,8 +820,8 @@ static void piece3_accumulate(const cells_job *J, int64_t i, const or_cp3 *cp, d
     for (int f = 0; f < cp->nf; ++f) {
         if (cp->nv[f] < 3) continue;
         double a = coeff * or_cp3_face_area(cp, f);
-        *perim += a;
         int64_t j = cp->id[f];
+        if (!J->dom->img || j >= 0) *perim += a;   /* image density: faces shared with neighbours only */
         if (j == -1) continue;
         double dd = dist_between(J->pos, 3, i, j);
         if (dd != 0) {

@@ -225,7 +225,7 @@ sdot_status sdot_pd_destroy(sdot_pd *pd) {
     pd->mass.release(); pd->arc.release(); pd->perim.release(); pd->diag.release(); pd->ell_val.release();
     pd->row_len.release(); pd->ell_col.release(); pd->ovf_list.release(); pd->flags.release(); pd->wctr.release();
     pd->csr_off.release(); pd->csr_col.release(); pd->csr_cnt.release(); pd->csr_tmp.release();
-    pd->csr_val.release(); pd->out_tmp.release();
+    pd->csr_val.release(); pd->out_tmp.release(); pd->scratch3.release();
     for (auto &b : pd->sv) b.release();
     pd->red.release();
     if (pd->stream) (void)hipStreamDestroy(pd->stream);

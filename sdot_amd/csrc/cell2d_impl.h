@@ -52,19 +52,6 @@ SDOT_HD inline bool region_skippable(double d, double R2, double w0, double wmax
     return dw - R2 >= tol;
 }
 
-SDOT_HD inline void ring_cell(int r, int k, int &dx, int &dy) {
-    if (r == 0) { dx = dy = 0; return; }
-    int row = 2 * r + 1;
-    if (k < row) { dx = -r + k; dy = r; return; }
-    k -= row;
-    if (k < row) { dx = -r + k; dy = -r; return; }
-    k -= row;
-    if (k < 2 * r - 1) { dx = -r; dy = -r + 1 + k; return; }
-    k -= 2 * r - 1;
-    dx = r;
-    dy = -r + 1 + k;
-}
-
 // Neumaier-compensated sum
 template <class T>
 struct CSum {
@@ -305,12 +292,15 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             dm = fmax(dm, 0.0);
             if (region_skippable<T>(dm, R2, w0, W)) break;
         }
-        const int ncell_ring = r == 0 ? 1 : 8 * r;
-        for (int kc = 0; kc < ncell_ring && n > 0 && !overflow; ++kc) {
-            int dx, dy;
-            ring_cell(r, kc, dx, dy);
-            const int gx = hx + dx, gy = hy + dy;
-            if (gx < 0 || gy < 0 || gx >= g.ncell[0] || gy >= g.ncell[1]) continue;
+        // the in-range cells of ring r: full rows at dy = +-r, the two side columns in between
+        const int gy0 = hy - r < 0 ? 0 : hy - r, gy1 = hy + r >= g.ncell[1] ? g.ncell[1] - 1 : hy + r;
+        for (int gy = gy0; gy <= gy1 && n > 0 && !overflow; ++gy) {
+          const bool full_row = gy == hy - r || gy == hy + r;
+          const int gxa = full_row ? (hx - r < 0 ? 0 : hx - r) : hx - r;
+          const int gxb = full_row ? (hx + r >= g.ncell[0] ? g.ncell[0] - 1 : hx + r) : hx + r;
+          const int step = full_row ? 1 : 2 * r;
+          for (int gx = gxa; gx <= gxb && n > 0 && !overflow; gx += step) {
+            if (gx < 0 || gx >= g.ncell[0]) continue;
             const uint32_t code = morton2(gx, gy);
             const uint32_t cs = a.cell_start[code], ce = a.cell_start[code + 1];
             if (cs == ce) continue;
@@ -334,6 +324,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                 if (n == 0) break;
                 R2 = radius2();
             }
+          }
         }
     }
     if (bad) flag_or(a.flags, FLAG_BAD_INPUT);

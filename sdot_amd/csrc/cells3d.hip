@@ -1,10 +1,584 @@
-// 3D Laguerre-cell kernel (placeholder until the polyhedron kernel lands).
-#include "internal.h"
+// 3D Laguerre-cell pass.  This file holds the thread-per-cell kernel: each
+// thread owns one polyhedron slot in HBM (element k of slot s at k * S + s,
+// so a wavefront's accesses coalesce) and runs the per-cell routine of
+// cell3d_impl.h over a grid-stride range of cells in grid order.
+#include "cell3d_impl.h"
 
 namespace sdot_amd {
-void run_cells3(sdot_pd *pd, CellArgs &a) {
-    (void)pd;
-    (void)a;
-    throw StatusError{SDOT_NOT_SUPPORTED, "3D cells are not built yet"};
+
+namespace {
+
+constexpr int VCAP3 = 160;   // vertices per polyhedron
+constexpr int PCAP3 = 96;    // planes per polyhedron (indices fit in 8 bits)
+constexpr int BS3 = 256;
+
+template <class T>
+size_t slot_bytes() {
+    return (size_t)VCAP3 * (7 * sizeof(T) + 2 * sizeof(int)) + (size_t)PCAP3 * (4 * sizeof(T) + sizeof(int) + 2 * sizeof(double));
 }
+
+template <class T>
+__host__ __device__ c3::Poly3Mem<T> slot(char *base, int64_t S, int64_t s) {
+    // field-major layout: [field][k][slot]
+    c3::Poly3Mem<T> m;
+    char *p = base;
+    auto take = [&](auto *&dst, size_t elem, int cap) {
+        using P = typename std::remove_reference<decltype(dst)>::type;
+        dst = (P)(p) + s;
+        p += elem * cap * S;
+    };
+    take(m.px, sizeof(T), VCAP3);
+    take(m.py, sizeof(T), VCAP3);
+    take(m.pz, sizeof(T), VCAP3);
+    take(m.d, sizeof(T), VCAP3);
+    take(m.sx, sizeof(T), VCAP3);
+    take(m.sy, sizeof(T), VCAP3);
+    take(m.sz, sizeof(T), VCAP3);
+    take(m.tri, sizeof(int), VCAP3);
+    take(m.stri, sizeof(int), VCAP3);
+    take(m.nx, sizeof(T), PCAP3);
+    take(m.ny, sizeof(T), PCAP3);
+    take(m.nz, sizeof(T), PCAP3);
+    take(m.nh, sizeof(T), PCAP3);
+    take(m.pid, sizeof(int), PCAP3);
+    take(m.acc, sizeof(double), PCAP3);
+    take(m.hacc, sizeof(double), PCAP3);
+    m.S = (int)S;
+    return m;
+}
+
+template <class T>
+__global__ __launch_bounds__(BS3) void k_cells3_thread(CellArgs a, char *scratch, int64_t nslots) {
+    const int64_t s = blockIdx.x * (int64_t)BS3 + threadIdx.x;
+    if (s >= nslots) return;
+    const c3::Poly3Mem<T> m = slot<T>(scratch, nslots, s);
+    const int64_t count = a.work ? a.work_n : a.N;
+    for (int64_t k = s; k < count; k += nslots) {
+        const int64_t t = a.work ? a.work[k] : k;
+        if (c3::cell3_body<T>(a, t, m, VCAP3, PCAP3)) flag_or(a.flags, FLAG_ROW_OVERFLOW);
+    }
+}
+
+template <class T>
+void launch_thread3(sdot_pd *pd, const CellArgs &a) {
+    int dev = 0, ncu = 0;
+    HIPCK(hipGetDevice(&dev));
+    HIPCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int64_t count = a.work ? a.work_n : a.N;
+    int64_t nslots = (int64_t)ncu * 4 * 64;     // four wavefronts per CU
+    if (nslots > count) nslots = count;
+    if (nslots < 1) return;
+    pd->scratch3.ensure(slot_bytes<T>() * nslots);
+    const unsigned nb = (unsigned)((nslots + BS3 - 1) / BS3);
+    hipLaunchKernelGGL(k_cells3_thread<T>, dim3(nb), dim3(BS3), 0, pd->stream, a, pd->scratch3.p, nslots);
+    HIPCK(hipGetLastError());
+}
+
+
+// ---------------------------------------------------------------------------
+// Wavefront-cooperative kernel: one cell per wavefront, one polyhedron vertex
+// per lane (position and plane triple in registers), planes and the dual-edge
+// map in the wavefront's LDS slice.  A candidate costs one dot product per
+// lane and a ballot; a cut builds the new vertices on the crossed edges with
+// lane shuffles, compacts the vertex list with a wavefront prefix sum and
+// rewrites the edge map.  Cells that need more than 64 vertices or planes go
+// to the thread kernel above.
+// ---------------------------------------------------------------------------
+constexpr int WV = 64;            // lanes = vertex capacity = plane capacity
+constexpr int WPB = 4;            // wavefronts per block
+
+struct WaveLds {
+    double pn[WV][4];             // planes (FP64): n.q <= h
+    int pid[WV];
+    unsigned char edge[WV * WV];  // edge[x * 64 + y] = lane holding the dual edge x -> y
+    double acc[WV];               // face-area accumulators
+    double hacc[WV];              // density-weighted face areas (image mode)
+    double sx[WV], sy[WV], sz[WV];
+    int stri[WV];
+};
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ inline int wave_sum_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ inline int wave_excl_scan(int v, int lane) {
+    int x = v;
+    for (int o = 1; o < WV; o <<= 1) {
+        int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    return x - v;
+}
+
+template <class T>
+struct WaveCell {
+    const CellArgs &a;
+    WaveLds &L;
+    const int lane;
+    int nv = 0, np = 0;
+    // this lane's vertex
+    T x = 0, y = 0, z = 0;
+    int tri = 0;
+    double c0x, c0y, c0z, w0;
+    double vlo[3], vhi[3];
+
+    __device__ WaveCell(const CellArgs &a_, WaveLds &L_, int lane_) : a(a_), L(L_), lane(lane_) {}
+
+    __device__ void set_plane(int k, double nx, double ny, double nz, double h, int id) {
+        if (lane == 0) {
+            L.pn[k][0] = nx;
+            L.pn[k][1] = ny;
+            L.pn[k][2] = nz;
+            L.pn[k][3] = h;
+            L.pid[k] = id;
+        }
+    }
+
+    __device__ void write_edges() {
+        if (lane < nv) {
+            const int p0 = tri & 255, p1 = (tri >> 8) & 255, p2 = (tri >> 16) & 255;
+            L.edge[p0 * WV + p1] = (unsigned char)lane;
+            L.edge[p1 * WV + p2] = (unsigned char)lane;
+            L.edge[p2 * WV + p0] = (unsigned char)lane;
+        }
+        wave_sync();
+    }
+
+    __device__ void init_box() {
+        const double c0[3] = {c0x, c0y, c0z};
+        if (lane < 6) {
+            const int axis = lane >> 1, up = lane & 1;
+            const double off = (up ? a.dom_hi[axis] : a.dom_lo[axis]) - c0[axis];
+            L.pn[lane][0] = axis == 0 ? (up ? 1.0 : -1.0) : 0.0;
+            L.pn[lane][1] = axis == 1 ? (up ? 1.0 : -1.0) : 0.0;
+            L.pn[lane][2] = axis == 2 ? (up ? 1.0 : -1.0) : 0.0;
+            L.pn[lane][3] = up ? off : -off;
+            L.pid[lane] = -1 - lane;
+        }
+        np = 6;
+        nv = 8;
+        if (lane < 8) {
+            const int sx = lane & 1, sy = (lane >> 1) & 1, sz = (lane >> 2) & 1;
+            int pa = sx, pb = 2 + sy, pc = 4 + sz;
+            if ((sx ? 1 : -1) * (sy ? 1 : -1) * (sz ? 1 : -1) < 0) {
+                int t = pb;
+                pb = pc;
+                pc = t;
+            }
+            tri = pa | (pb << 8) | (pc << 16);
+            x = T((sx ? a.dom_hi[0] : a.dom_lo[0]) - c0x);
+            y = T((sy ? a.dom_hi[1] : a.dom_lo[1]) - c0y);
+            z = T((sz ? a.dom_hi[2] : a.dom_lo[2]) - c0z);
+        }
+        write_edges();
+    }
+
+    __device__ double radius2() const {
+        double r = lane < nv ? (double)x * x + (double)y * y + (double)z * z : 0.0;
+        return wave_max(r);
+    }
+
+    // cut by q.n <= h: 0 no change, 1 cut (nv may be 0), 2 capacity exceeded
+    __device__ int cut(T nx, T ny, T nz, T h, int id, double nx64, double ny64, double nz64, double h64) {
+        const bool valid = lane < nv;
+        const T d = valid ? fma(-x, nx, fma(-y, ny, fma(-z, nz, h))) : T(1);
+        const bool out = valid && d < 0;
+        const unsigned long long mo = __ballot(out);
+        if (!mo) return 0;
+        if (__popcll(mo) == nv) {
+            nv = 0;
+            return 1;
+        }
+        if (np >= WV) return 2;
+        const int pn = np;
+        const int p[3] = {tri & 255, (tri >> 8) & 255, (tri >> 16) & 255};
+        int cnt = out ? 0 : (valid ? 1 : 0);
+        T nxv[3], nyv[3], nzv[3];
+        int ntri[3];
+        bool mk[3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const int xa = p[e], yb = p[e == 2 ? 0 : e + 1];
+            const int nb = out ? (int)L.edge[yb * WV + xa] : lane;
+            const T dn = __shfl(d, nb), xn = __shfl(x, nb), yn = __shfl(y, nb), zn = __shfl(z, nb);
+            mk[e] = out && !(dn < 0);
+            const T mm = dn / (d - dn);
+            nxv[e] = xn - mm * (x - xn);
+            nyv[e] = yn - mm * (y - yn);
+            nzv[e] = zn - mm * (z - zn);
+            ntri[e] = xa | (yb << 8) | (pn << 16);
+            cnt += mk[e] ? 1 : 0;
+        }
+        const int pre = wave_excl_scan(cnt, lane);
+        const int total = __shfl(pre + cnt, WV - 1);
+        if (total > WV) return 2;
+        wave_sync();
+        if (valid && !out) {
+            L.sx[pre] = x;
+            L.sy[pre] = y;
+            L.sz[pre] = z;
+            L.stri[pre] = tri;
+        }
+        int o = pre;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            if (mk[e]) {
+                L.sx[o] = nxv[e];
+                L.sy[o] = nyv[e];
+                L.sz[o] = nzv[e];
+                L.stri[o] = ntri[e];
+                ++o;
+            }
+        }
+        set_plane(pn, nx64, ny64, nz64, h64, id);
+        wave_sync();
+        nv = total;
+        np = pn + 1;
+        if (lane < nv) {
+            x = T(L.sx[lane]);
+            y = T(L.sy[lane]);
+            z = T(L.sz[lane]);
+            tri = L.stri[lane];
+        }
+        wave_sync();
+        write_edges();
+        return 1;
+    }
+
+    __device__ void plane64(int k, double &nx, double &ny, double &nz, double &h) const {
+        const int id = L.pid[k];
+        if (id <= -7) {   // voxel side
+            const int s = -7 - id, axis = s >> 1, up = s & 1;
+            const double off = up ? vhi[axis] : vlo[axis];
+            nx = axis == 0 ? (up ? 1.0 : -1.0) : 0.0;
+            ny = axis == 1 ? (up ? 1.0 : -1.0) : 0.0;
+            nz = axis == 2 ? (up ? 1.0 : -1.0) : 0.0;
+            h = up ? off : -off;
+            return;
+        }
+        nx = L.pn[k][0];
+        ny = L.pn[k][1];
+        nz = L.pn[k][2];
+        h = L.pn[k][3];
+    }
+
+    // FP64 position of this lane's vertex (intersection of its planes in the mixed path)
+    __device__ void vertex64(double &X, double &Y, double &Z) const {
+        X = x;
+        Y = y;
+        Z = z;
+        if (sizeof(T) == sizeof(double) || lane >= nv) return;
+        double n0[4], n1[4], n2[4];
+        plane64(tri & 255, n0[0], n0[1], n0[2], n0[3]);
+        plane64((tri >> 8) & 255, n1[0], n1[1], n1[2], n1[3]);
+        plane64((tri >> 16) & 255, n2[0], n2[1], n2[2], n2[3]);
+        const double c12x = n1[1] * n2[2] - n1[2] * n2[1], c12y = n1[2] * n2[0] - n1[0] * n2[2], c12z = n1[0] * n2[1] - n1[1] * n2[0];
+        const double c20x = n2[1] * n0[2] - n2[2] * n0[1], c20y = n2[2] * n0[0] - n2[0] * n0[2], c20z = n2[0] * n0[1] - n2[1] * n0[0];
+        const double c01x = n0[1] * n1[2] - n0[2] * n1[1], c01y = n0[2] * n1[0] - n0[0] * n1[2], c01z = n0[0] * n1[1] - n0[1] * n1[0];
+        const double det = n0[0] * c12x + n0[1] * c12y + n0[2] * c12z;
+        const double nrm = sqrt((n0[0] * n0[0] + n0[1] * n0[1] + n0[2] * n0[2]) * (n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]) *
+                                (n2[0] * n2[0] + n2[1] * n2[1] + n2[2] * n2[2]));
+        if (!(fabs(det) > 1e-9 * nrm)) return;
+        X = (n0[3] * c12x + n1[3] * c20x + n2[3] * c01x) / det;
+        Y = (n0[3] * c12y + n1[3] * c20y + n2[3] * c01y) / det;
+        Z = (n0[3] * c12z + n1[3] * c20z + n2[3] * c01z) / det;
+    }
+
+    // face areas into L.acc[0..np), returns the volume (all lanes)
+    __device__ double integrate() {
+        if (lane < np) L.acc[lane] = 0;
+        wave_sync();
+        double X, Y, Z;
+        vertex64(X, Y, Z);
+        const bool valid = lane < nv;
+        const int p[3] = {tri & 255, (tri >> 8) & 255, (tri >> 16) & 255};
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const int f = p[e], gq = p[e == 2 ? 0 : e + 1];
+            const int q = valid ? (int)L.edge[gq * WV + f] : lane;
+            const double Xq = __shfl(X, q), Yq = __shfl(Y, q), Zq = __shfl(Z, q);
+            if (valid) {
+                double fx, fy, fz, fh;
+                plane64(f, fx, fy, fz, fh);
+                const double cx = Y * Zq - Z * Yq, cy = Z * Xq - X * Zq, cz = X * Yq - Y * Xq;
+                atomicAdd(&L.acc[f], (cx * fx + cy * fy + cz * fz) / sqrt(fx * fx + fy * fy + fz * fz));
+            }
+        }
+        wave_sync();
+        double vol = 0;
+        if (lane < np) {
+            const double ar = 0.5 * fabs(L.acc[lane]);
+            L.acc[lane] = ar;
+            double fx, fy, fz, fh;
+            plane64(lane, fx, fy, fz, fh);
+            vol = fh / sqrt(fx * fx + fy * fy + fz * fz) * ar / 3;
+        }
+        wave_sync();
+        return wave_sum(vol);
+    }
+
+    __device__ unsigned long long used_planes() const {
+        unsigned long long m = 0;
+        if (lane < nv) m = (1ull << (tri & 255)) | (1ull << ((tri >> 8) & 255)) | (1ull << ((tri >> 16) & 255));
+        for (int o = 32; o > 0; o >>= 1) m |= __shfl_xor(m, o);
+        return m;
+    }
+};
+
+template <class T>
+__global__ __launch_bounds__(WV * WPB) void k_cells3_wave(CellArgs a) {
+    __shared__ WaveLds lds[WPB];
+    const int lane = threadIdx.x & (WV - 1);
+    const int wid = threadIdx.x / WV;
+    WaveLds &L = lds[wid];
+    const int64_t nwaves = (int64_t)gridDim.x * WPB;
+    const int64_t w0id = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * WPB + wid);
+    const GridInfo &g = a.g;
+    const double W = dbl_unorder(*a.wglob);
+    int maxr = g.ncell[0];
+    if (g.ncell[1] > maxr) maxr = g.ncell[1];
+    if (g.ncell[2] > maxr) maxr = g.ncell[2];
+    // cells in grid order, one contiguous range per wavefront
+    const int64_t per = (a.N + nwaves - 1) / nwaves;
+    const int64_t tb = w0id * per, te = tb + per < a.N ? tb + per : a.N;
+    for (int64_t t = tb; t < te; ++t) {
+        WaveCell<T> C(a, L, lane);
+        C.c0x = a.spos[3 * t];
+        C.c0y = a.spos[3 * t + 1];
+        C.c0z = a.spos[3 * t + 2];
+        C.w0 = a.sw[t];
+        const double c0x = C.c0x, c0y = C.c0y, c0z = C.c0z, w0 = C.w0;
+        const double c0[3] = {c0x, c0y, c0z};
+        C.init_box();
+        double R2 = C.radius2();
+        bool overflow = false, bad = false;
+        const int hh[3] = {grid_coord(g, 0, c0x), grid_coord(g, 1, c0y), grid_coord(g, 2, c0z)};
+        for (int r = 0; r <= maxr && C.nv > 0 && !overflow; ++r) {
+            if (r > 0) {
+                double dm = DBL_MAX;
+                bool any = false;
+                for (int d = 0; d < 3; ++d) {
+                    if (hh[d] - r + 1 > 0) { dm = fmin(dm, c0[d] - (g.lo[d] + (hh[d] - r + 1) * g.h[d])); any = true; }
+                    if (hh[d] + r < g.ncell[d]) { dm = fmin(dm, (g.lo[d] + (hh[d] + r) * g.h[d]) - c0[d]); any = true; }
+                }
+                if (!any) break;
+                if (c3::region_skippable3<T>(fmax(dm, 0.0), R2, w0, W)) break;
+            }
+            const int z0 = hh[2] - r < 0 ? 0 : hh[2] - r, z1 = hh[2] + r >= g.ncell[2] ? g.ncell[2] - 1 : hh[2] + r;
+            const int y0 = hh[1] - r < 0 ? 0 : hh[1] - r, y1 = hh[1] + r >= g.ncell[1] ? g.ncell[1] - 1 : hh[1] + r;
+            for (int gz = z0; gz <= z1 && C.nv > 0 && !overflow; ++gz) {
+                for (int gy = y0; gy <= y1 && C.nv > 0 && !overflow; ++gy) {
+                    const bool full = gz == hh[2] - r || gz == hh[2] + r || gy == hh[1] - r || gy == hh[1] + r;
+                    const int step = full || r == 0 ? 1 : 2 * r;
+                    for (int gx = hh[0] - r; gx <= hh[0] + r && C.nv > 0 && !overflow; gx += step) {
+                        if (gx < 0 || gx >= g.ncell[0]) continue;
+                        const uint32_t code = morton3(gx, gy, gz);
+                        const uint32_t cs = a.cell_start[code], ce = a.cell_start[code + 1];
+                        if (cs == ce) continue;
+                        {
+                            const int gg[3] = {gx, gy, gz};
+                            double e2 = 0;
+                            for (int d = 0; d < 3; ++d) {
+                                const double b0 = g.lo[d] + gg[d] * g.h[d];
+                                const double e = fmax(0.0, fmax(b0 - c0[d], c0[d] - (b0 + g.h[d])));
+                                e2 += e * e;
+                            }
+                            if (c3::region_skippable3<T>(sqrt(e2), R2, w0, a.cell_wmax[code])) continue;
+                        }
+                        for (uint32_t j = cs; j < ce; ++j) {
+                            if ((int64_t)j == t) continue;
+                            const double vx = a.spos[3 * (int64_t)j] - c0x, vy = a.spos[3 * (int64_t)j + 1] - c0y,
+                                         vz = a.spos[3 * (int64_t)j + 2] - c0z;
+                            const double vv = vx * vx + vy * vy + vz * vz;
+                            const double hv = 0.5 * (vv + w0 - a.sw[j]);
+                            if (hv > 0 && R2 * vv <= hv * hv * (1 - c3::Eps3<T>::cand)) continue;
+                            if (vv == 0) { bad = true; continue; }
+                            const int rc = C.cut(T(vx), T(vy), T(vz), T(hv), (int)j, vx, vy, vz, hv);
+                            if (rc == 2) { overflow = true; break; }
+                            if (rc == 1) {
+                                if (C.nv == 0) break;
+                                R2 = C.radius2();
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (bad && lane == 0) flag_or(a.flags, FLAG_BAD_INPUT);
+        if (overflow) {
+            if (lane == 0) {
+                unsigned k = atomicAdd(a.ovf_count, 1u);
+                a.ovf_list[k] = (int)t;
+            }
+            wave_sync();
+            continue;
+        }
+        // ---- integrals ----
+        double mass = 0;
+        const int npc = C.np;
+        if (lane < npc) L.hacc[lane] = 0;
+        wave_sync();
+        if (C.nv > 0) {
+            if (!a.img) {
+                const double vol = C.integrate();
+                mass = a.coeff * vol;
+                if (lane < npc) L.hacc[lane] = a.coeff * L.acc[lane];
+            } else {
+                double X, Y, Z;
+                C.vertex64(X, Y, Z);
+                const bool valid = lane < C.nv;
+                double mn[3], mx[3];
+                mn[0] = -wave_max(valid ? -X : -DBL_MAX); mx[0] = wave_max(valid ? X : -DBL_MAX);
+                mn[1] = -wave_max(valid ? -Y : -DBL_MAX); mx[1] = wave_max(valid ? Y : -DBL_MAX);
+                mn[2] = -wave_max(valid ? -Z : -DBL_MAX); mx[2] = wave_max(valid ? Z : -DBL_MAX);
+                int i0[3], i1[3];
+                double vs[3];
+                for (int d = 0; d < 3; ++d) {
+                    vs[d] = (a.dom_hi[d] - a.dom_lo[d]) / a.img_n[d];
+                    const double lo = mn[d] + c0[d] - a.dom_lo[d], hi = mx[d] + c0[d] - a.dom_lo[d];
+                    const int u0 = (int)floor(lo / vs[d]), u1 = (int)floor(hi / vs[d]);
+                    i0[d] = u0 < 0 ? 0 : (u0 >= a.img_n[d] ? a.img_n[d] - 1 : u0);
+                    i1[d] = u1 < 0 ? 0 : (u1 >= a.img_n[d] ? a.img_n[d] - 1 : u1);
+                }
+                const int nv0 = C.nv, np0 = C.np;
+                const T sx = C.x, sy = C.y, sz = C.z;
+                const int stri = C.tri;
+                for (int iz = i0[2]; iz <= i1[2] && !overflow; ++iz)
+                    for (int iy = i0[1]; iy <= i1[1] && !overflow; ++iy)
+                        for (int ix = i0[0]; ix <= i1[0] && !overflow; ++ix) {
+                            const int iv[3] = {ix, iy, iz};
+                            const double rho = a.img[((int64_t)iz * a.img_n[1] + iy) * a.img_n[0] + ix];
+                            for (int d = 0; d < 3; ++d) {
+                                C.vlo[d] = a.dom_lo[d] + vs[d] * iv[d] - c0[d];
+                                C.vhi[d] = a.dom_lo[d] + vs[d] * (iv[d] + 1) - c0[d];
+                            }
+                            bool empty = false;
+                            for (int s = 0; s < 6 && !empty; ++s) {
+                                const int axis = s >> 1, up = s & 1;
+                                if (up ? (iv[axis] == i1[axis]) : (iv[axis] == i0[axis])) continue;
+                                const double off = up ? C.vhi[axis] : C.vlo[axis];
+                                const double nxx = axis == 0 ? (up ? 1.0 : -1.0) : 0.0, nyy = axis == 1 ? (up ? 1.0 : -1.0) : 0.0,
+                                             nzz = axis == 2 ? (up ? 1.0 : -1.0) : 0.0, hq = up ? off : -off;
+                                const int rc = C.cut(T(nxx), T(nyy), T(nzz), T(hq), -7 - s, nxx, nyy, nzz, hq);
+                                if (rc == 2) { overflow = true; break; }
+                                if (C.nv == 0) empty = true;
+                            }
+                            if (!overflow && !empty) {
+                                const double vol = C.integrate();
+                                mass += rho * vol;
+                                if (lane < npc) L.hacc[lane] += rho * L.acc[lane];
+                            }
+                            // restore the cell
+                            C.nv = nv0;
+                            C.np = np0;
+                            C.x = sx;
+                            C.y = sy;
+                            C.z = sz;
+                            C.tri = stri;
+                            C.write_edges();
+                        }
+                if (overflow) {
+                    if (lane == 0) {
+                        unsigned k = atomicAdd(a.ovf_count, 1u);
+                        a.ovf_list[k] = (int)t;
+                    }
+                    wave_sync();
+                    continue;
+                }
+            }
+        }
+        wave_sync();
+        // ---- Hessian row, boundary measure ----
+        const unsigned long long used = C.nv > 0 ? C.used_planes() : 0ull;
+        const bool mine = lane < npc && ((used >> lane) & 1ull);
+        const int id = lane < npc ? L.pid[lane] : -1;
+        const double ar = mine ? L.hacc[lane] : 0.0;
+        const bool nb = mine && id >= 0;
+        double b = 0;
+        if (nb) {
+            double nx, ny, nz, hv;
+            C.plane64(lane, nx, ny, nz, hv);
+            b = 0.5 * ar / sqrt(nx * nx + ny * ny + nz * nz);
+        }
+        const int slot = wave_excl_scan(nb ? 1 : 0, lane);
+        const int nrow = __shfl(slot + (nb ? 1 : 0), WV - 1);
+        if (nb && slot < a.ell_cap) {
+            a.ell_col[slot * a.ell_stride + t] = id;
+            a.ell_val[slot * a.ell_stride + t] = -b;
+        }
+        const double der = wave_sum(b);
+        const double perim = wave_sum(mine && (!a.img || id >= 0) ? ar : 0.0);
+        if (lane == 0) {
+            a.mass[t] = mass;
+            a.arc[t] = 0;
+            a.perim[t] = perim;
+            a.diag[t] = der;
+            a.row_len[t] = nrow < a.ell_cap ? nrow : a.ell_cap;
+            if (nrow > a.ell_cap) flag_or(a.flags, FLAG_ROW_OVERFLOW);
+            if (mass == 0) flag_or(a.flags, FLAG_EMPTY);
+            if (a.ctr) {
+                const unsigned long long K = __popcll(used);
+                const int sl = (int)(t % WCTR);
+                const unsigned long long F = 12 * K * K + 100 * K;
+                ctr_add(a.ctr + sl, F > 284 ? F - 284 : 0);
+                ctr_add(a.ctr + WCTR + sl, K);
+            }
+        }
+        wave_sync();
+    }
+}
+
+template <class T>
+void launch_wave3(sdot_pd *pd, const CellArgs &a) {
+    int dev = 0, ncu = 0;
+    HIPCK(hipGetDevice(&dev));
+    HIPCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    int64_t nblocks = (int64_t)ncu * 8;
+    const int64_t need = (a.N + WPB - 1) / WPB;
+    if (nblocks > need) nblocks = need;
+    if (nblocks < 1) return;
+    hipLaunchKernelGGL(k_cells3_wave<T>, dim3((unsigned)nblocks), dim3(WV * WPB), 0, pd->stream, a);
+    HIPCK(hipGetLastError());
+}
+
+}  // namespace
+
+void run_cells3(sdot_pd *pd, CellArgs &a) {
+    hipStream_t s = pd->stream;
+    a.work = nullptr;
+    a.work_n = 0;
+    const bool mixed = pd->precision == SDOT_MIXED;
+    if (mixed) launch_wave3<float>(pd, a);
+    else launch_wave3<double>(pd, a);
+    HIPCK(hipEventRecord(pd->ev[3], s));
+    unsigned cnt[2] = {0, 0};
+    HIPCK(hipMemcpyAsync(cnt, pd->flags.p, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    pd->last.nb_cells_overflow = cnt[1];
+    if (cnt[1] > 0) {
+        // cells beyond 64 vertices / planes: the thread kernel with wide slots
+        int *list2 = pd->ovf_list.p + a.N + 1;
+        HIPCK(hipMemcpyAsync(list2, pd->ovf_list.p, cnt[1] * sizeof(int), hipMemcpyDeviceToDevice, s));
+        a.work = list2;
+        a.work_n = cnt[1];
+        if (mixed) launch_thread3<float>(pd, a);
+        else launch_thread3<double>(pd, a);
+        a.work = nullptr;
+    }
+}
+
 }  // namespace sdot_amd

@@ -237,10 +237,15 @@ void build_grid(sdot_pd *pd) {
     g.dim = dim;
     double vol = 1;
     double L[3];
+    // extents are floored at 1e-3 of the largest one so that diracs on a line
+    // or a plane still get cells of comparable size along every axis
+    double lmax = 0;
+    for (int d = 0; d < dim; ++d) lmax = fmax(lmax, hi[d] - lo[d]);
     for (int d = 0; d < dim; ++d) {
         double l = hi[d] - lo[d];
         double scale = fmax(fabs(lo[d]), fabs(hi[d]));
         double pad = 1e-7 * l + 1e-12 * scale + 1e-300;
+        if (l + 2 * pad < 1e-3 * lmax) pad = 0.5 * (1e-3 * lmax - l);
         lo[d] -= pad;
         hi[d] += pad;
         L[d] = hi[d] - lo[d];

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -219,6 +220,7 @@ struct sdot_pd {
     int ell_cap = 0;
     unsigned last_flags = 0;
     sdot_amd::DevBuf<unsigned long long> wctr;
+    sdot_amd::DevBuf<char> scratch3;   // per-thread polyhedron slots of the 3D thread kernel
 
     // CSR export scratch
     sdot_amd::DevBuf<int64_t> csr_off, csr_col, csr_cnt, csr_tmp;
