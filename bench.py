@@ -80,14 +80,80 @@ def run_cpu_baseline(pos, w, budget_s=10.0, max_reps=40):
                        f"by the C oracle with {cores} threads, {dt:.3f} s each"), r["masses"]
 
 
+def run_c3(n, img_size, steps):
+    """3D Newton solve: n diracs uniform in [0,1]^3, lognormal(0, 0.5) voxel density on an
+    img_size^3 grid normalised to mass 1, w0 = 0, uniform targets, stop at 1e-6 relative mass error."""
+    import sdot_amd
+    from sdot_amd import workloads
+    pos = workloads.uniform_diracs(n, 3, seed=0)
+    img = workloads.lognormal_image(img_size, 3, seed=1)
+    t = np.full(n, img.mean() / n)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        pd = sdot_amd.PowerDiagram(3, prec)
+        pd.set_image_density([0, 0, 0], [1, 1, 1], img)
+        pd.set_diracs(pos, np.zeros(n))
+        runs = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            st, w, stats = pd.ot_solve(t, np.zeros(n), tol=1e-6)
+            runs.append(time.perf_counter() - t0)
+        stats["seconds"] = float(np.median(runs))
+        stats["status"] = st
+        stats["cells_per_s_pass"] = n * stats["newton_iters"] / stats["seconds_cells"] if stats["seconds_cells"] else None
+        out[prec] = stats
+        out[prec + "_weights"] = w
+        print(f"# c3 {prec}: {stats}", file=sys.stderr, flush=True)
+    return out, pos, img
+
+
+def main_c3(args):
+    from oracle import oracle as O
+    res, pos, img = run_c3(args.n, args.img, args.steps)
+    w = res["fp64_weights"]
+    sample = min(args.n, 4000)
+    dt, st, _ = O.cells_sample_seconds(pos, w, 0, sample, img=img, nthreads=min(16, os.cpu_count() or 1))
+    cpu_cells = sample / dt
+    iters = res["fp64"]["newton_iters"]
+    cpu_solve = iters * args.n / cpu_cells
+    best = min(("fp64", "mixed"), key=lambda p: res[p]["seconds"] if res[p]["status"] == 0 else 1e30)
+    line = {
+        "metric": "solve s @1e-6 mass err (3D Newton, 1M diracs, 64^3 lognormal density)",
+        "value": res[best]["seconds"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": res[best]["seconds"] * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-clip/f64-solve" if best == "mixed" else "f64",
+        "data": "synthetic (xoshiro256** seed 0 positions, seed 1 density)",
+        "config": {"workload": f"3D Newton OT solve, N={args.n} uniform in [0,1]^3, {args.img}^3 lognormal(0,0.5) "
+                               "voxel density (mass 1), w0=0, uniform targets, stop at max relative mass error 1e-6",
+                   "n_diracs": args.n, "dim": 3, "precision": best},
+        "solves": {p: {k: v for k, v in res[p].items()} for p in ("fp64", "mixed")},
+        "mixed_speedup_vs_fp64": res["fp64"]["seconds"] / res["mixed"]["seconds"],
+        "cpu_baseline": {"value": cpu_cells, "unit": "cells/s (one image pass)", "cores": min(16, os.cpu_count() or 1),
+                         "kind": "port",
+                         "sample": f"oracle cell pass (voxel pieces, Hessian rows) on {sample} of the {args.n} cells at "
+                                   f"the converged weights: {dt:.2f} s",
+                         "solve_seconds_estimate": cpu_solve,
+                         "estimate": f"{iters} Newton passes x N / (cells/s), CPU linear solves not counted"},
+        "speedup_vs_cpu_estimate": cpu_solve / res[best]["seconds"],
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["c2", "c3"], default="c2")
+    ap.add_argument("--img", type=int, default=64)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
+    if args.workload == "c3":
+        if args.gpus != 1:
+            sys.exit(2)
+        args.steps = min(args.steps, 3)
+        return main_c3(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU only (the cell pass of one GPU's share); --gpus {args.gpus} "
               "is not run: multi-GPU sharding is a design (ghost-layer exchange), not part of this build.",

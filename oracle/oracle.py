@@ -50,6 +50,9 @@ def lib():
         L.or_cells.restype = ctypes.c_int
         L.or_cells.argtypes = [ctypes.c_int, _i64, _p, _p, ctypes.POINTER(Domain), ctypes.c_int, ctypes.c_int,
                                _p, _p, _p, _i64, _p, _p, _p]
+        L.or_cells_range.restype = ctypes.c_int
+        L.or_cells_range.argtypes = [ctypes.c_int, _i64, _p, _p, ctypes.POINTER(Domain), ctypes.c_int, ctypes.c_int,
+                                     _i64, _i64, _p, _p, _p, _i64, _p, _p, _p]
         L.or_rows_to_csr.restype = _i64
         L.or_rows_to_csr.argtypes = [_i64, _i64, _p, _p, _p, _p, _p, _p]
         L.or_pcg.restype = ctypes.c_int
@@ -149,6 +152,28 @@ def cells(pos, w, dim=None, ball=False, brute=False, nthreads=None, domain=None,
                         _ptr(masses), _ptr(arc), _ptr(per), cap, _ptr(rl), _ptr(rc), _ptr(rv))
     del keep
     return dict(status=st, masses=masses, ball_arc=arc, perimeter=per, row_len=rl, rcols=rc, rvals=rv, cap=cap)
+
+
+def cells_sample_seconds(pos, w, i_begin, i_end, ball=False, nthreads=None, img=None, cap=None):
+    """Wall time of the oracle evaluating cells [i_begin, i_end) of the full problem."""
+    import time
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    dim = pos.shape[1]
+    N = pos.shape[0]
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    cap = cap or (64 if dim == 2 else 160)
+    dom, keep = make_domain(dim, img=img)
+    masses = np.zeros(N)
+    rl = np.zeros(N, dtype=np.int64)
+    rc = np.zeros(N * cap, dtype=np.int64)
+    rv = np.zeros(N * cap)
+    t0 = time.perf_counter()
+    st = lib().or_cells_range(dim, N, _ptr(pos), _ptr(w), ctypes.byref(dom), 1 if ball else 0,
+                              nthreads or os.cpu_count() or 1, int(i_begin), int(i_end), _ptr(masses), None, None,
+                              cap, _ptr(rl), _ptr(rc), _ptr(rv))
+    dt = time.perf_counter() - t0
+    del keep
+    return dt, st, masses
 
 
 def to_csr(res):

@@ -534,6 +534,7 @@ typedef struct {
     double *vals;
     or_grid grid;
     int64_t next;          /* shared chunk counter */
+    int64_t end;           /* cells [next0, end) are evaluated */
     pthread_mutex_t mtx;
     int status;
 } cells_job;
@@ -896,8 +897,8 @@ static void *cells_worker(void *arg) {
     const int64_t chunk = 64;
     for (;;) {
         int64_t beg = __atomic_fetch_add(&J->next, chunk, __ATOMIC_RELAXED);
-        if (beg >= J->N) break;
-        int64_t end = beg + chunk < J->N ? beg + chunk : J->N;
+        if (beg >= J->end) break;
+        int64_t end = beg + chunk < J->end ? beg + chunk : J->end;
         for (int64_t i = beg; i < end; ++i) {
             if (J->dim == 2) eval_cell2(J, i, (or_cp2 *)a, (or_cp2 *)b);
             else eval_cell3(J, i, (or_cp3 *)a, (or_cp3 *)b);
@@ -912,6 +913,14 @@ int or_cells(int dim, int64_t N, const double *pos, const double *w, const or_do
              int flags, int nthreads,
              double *masses, double *ball_arc, double *perimeter,
              int64_t cap, int64_t *row_len, int64_t *cols, double *vals) {
+    return or_cells_range(dim, N, pos, w, dom, flags, nthreads, 0, N, masses, ball_arc, perimeter, cap, row_len, cols, vals);
+}
+
+int or_cells_range(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom,
+                   int flags, int nthreads, int64_t i_begin, int64_t i_end,
+                   double *masses, double *ball_arc, double *perimeter,
+                   int64_t cap, int64_t *row_len, int64_t *cols, double *vals) {
+    if (i_begin < 0 || i_end > N || i_begin > i_end) return OR_INVALID;
     if ((dim != 2 && dim != 3) || N < 0 || cap < 1 || !dom || dom->dim != dim) return OR_INVALID;
     if (dim == 3 && (flags & 1)) return OR_INVALID;  /* ConvexPolyhedron3::ball_cut is TODO (:292-293) */
     if (N == 0) return OR_OK;
@@ -920,7 +929,8 @@ int or_cells(int dim, int64_t N, const double *pos, const double *w, const or_do
     J.dim = dim; J.N = N; J.pos = pos; J.w = w; J.dom = dom; J.flags = flags;
     J.masses = masses; J.ball_arc = ball_arc; J.perimeter = perimeter;
     J.cap = cap; J.row_len = row_len; J.cols = cols; J.vals = vals;
-    J.next = 0;
+    J.next = i_begin;
+    J.end = i_end;
     J.status = OR_OK;
     pthread_mutex_init(&J.mtx, NULL);
     if (!(flags & 2)) grid_build(&J.grid, dim, N, pos, w);

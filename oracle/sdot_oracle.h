@@ -117,6 +117,13 @@ int or_cells(int dim, int64_t N, const double *pos, const double *w, const or_do
              double *masses, double *ball_arc, double *perimeter,
              int64_t cap, int64_t *row_len, int64_t *cols, double *vals);
 
+/* Same, evaluating only the cells [i_begin, i_end) (the grid is built over all
+ * N diracs): a bounded sample of a large pass for timing. */
+int or_cells_range(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom,
+                   int flags, int nthreads, int64_t i_begin, int64_t i_end,
+                   double *masses, double *ball_arc, double *perimeter,
+                   int64_t cap, int64_t *row_len, int64_t *cols, double *vals);
+
 /* Assemble sorted/merged CSR in dirac order from the raw rows
  * (get_der_integrals_wrt_weights.h:76-88,106-119).  offsets[N+1]; cols/vals
  * need room for sum(row_len).  Returns nnz. */

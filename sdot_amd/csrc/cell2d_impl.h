@@ -157,8 +157,8 @@ SDOT_HD inline double ball_square(double w0) { return sqrt(fmax(w0, 0.0)) * (1 +
 // The line a*x + b*y = c (coordinates local to the dirac) that carries an edge:
 // the radical line of neighbour id >= 0, box side -1 - id (bottom, right, top,
 // left) for ids -1..-4, or a side of the ball's enclosing square for -5..-8.
-SDOT_HD inline void line_of(const CellArgs &a, int id, double c0x, double c0y, double w0, double &la, double &lb,
-                            double &lc) {
+SDOT_HD inline void line_of(const CellArgs &a, const ProbView &pv, int id, double c0x, double c0y, double w0,
+                            double &la, double &lb, double &lc) {
     if (id >= 0) {
         la = a.spos[2 * (int64_t)id] - c0x;
         lb = a.spos[2 * (int64_t)id + 1] - c0y;
@@ -167,10 +167,10 @@ SDOT_HD inline void line_of(const CellArgs &a, int id, double c0x, double c0y, d
     }
     const double sq = ball_square(w0);
     switch (-1 - id) {
-        case 0: la = 0; lb = 1; lc = a.dom_lo[1] - c0y; break;
-        case 1: la = 1; lb = 0; lc = a.dom_hi[0] - c0x; break;
-        case 2: la = 0; lb = 1; lc = a.dom_hi[1] - c0y; break;
-        case 3: la = 1; lb = 0; lc = a.dom_lo[0] - c0x; break;
+        case 0: la = 0; lb = 1; lc = pv.lo[1] - c0y; break;
+        case 1: la = 1; lb = 0; lc = pv.hi[0] - c0x; break;
+        case 2: la = 0; lb = 1; lc = pv.hi[1] - c0y; break;
+        case 3: la = 1; lb = 0; lc = pv.lo[0] - c0x; break;
         case 4: la = 1; lb = 0; lc = sq; break;
         case 5: la = 0; lb = 1; lc = sq; break;
         case 6: la = -1; lb = 0; lc = sq; break;
@@ -188,15 +188,18 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
 #define PI_(k) I[(k) * S]
 
     const double c0x = a.spos[2 * t], c0y = a.spos[2 * t + 1], w0 = a.sw[t];
-    const GridInfo &g = a.g;
-    const double W = dbl_unorder(*a.wglob);
+    ProbView pv;
+    prob_view(a, t, pv);
+    if (a.pdone && a.pdone[pv.p]) return 0;   // converged problem of a batch: nothing to do
+    const GridInfo &g = pv.g;
+    const double W = pv.W;
 
     // starting cell: the domain box, vertices p0, (p1.x,p0.y), p1, (p0.x,p1.y)
     int n = 4;
-    PX(0) = T(a.dom_lo[0] - c0x); PY(0) = T(a.dom_lo[1] - c0y);
-    PX(1) = T(a.dom_hi[0] - c0x); PY(1) = T(a.dom_lo[1] - c0y);
-    PX(2) = T(a.dom_hi[0] - c0x); PY(2) = T(a.dom_hi[1] - c0y);
-    PX(3) = T(a.dom_lo[0] - c0x); PY(3) = T(a.dom_hi[1] - c0y);
+    PX(0) = T(pv.lo[0] - c0x); PY(0) = T(pv.lo[1] - c0y);
+    PX(1) = T(pv.hi[0] - c0x); PY(1) = T(pv.lo[1] - c0y);
+    PX(2) = T(pv.hi[0] - c0x); PY(2) = T(pv.hi[1] - c0y);
+    PX(3) = T(pv.lo[0] - c0x); PY(3) = T(pv.hi[1] - c0y);
     for (int k = 0; k < 4; ++k) PI_(k) = -1 - k;   // box sides: bottom, right, top, left
 
     auto radius2 = [&]() {
@@ -209,23 +212,45 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         return R2;
     };
     // cut by the half-plane q.V <= H; returns 0 (no change), 1 (cut), 2 (capacity exceeded)
-    auto cut = [&](const T Vx, const T Vy, const T H, const int cid) -> int {
+    // exact-ish side of vertex k (FP64, from its two lines) for the line q.v <= h
+    auto side64 = [&](int k, double vx, double vy, double hh) -> double {
+        double a1, b1, e1, a2, b2, e2;
+        line_of(a, pv, PI_(k == 0 ? n - 1 : k - 1), c0x, c0y, w0, a1, b1, e1);
+        line_of(a, pv, PI_(k), c0x, c0y, w0, a2, b2, e2);
+        double x = (double)PX(k), y = (double)PY(k);
+        const double det = a1 * b2 - a2 * b1;
+        if (fabs(det) > 1e-12 * (fabs(a1 * b2) + fabs(a2 * b1))) {
+            x = (e1 * b2 - e2 * b1) / det;
+            y = (a1 * e2 - a2 * e1) / det;
+        }
+        return hh - (x * vx + y * vy);
+    };
+    auto cut = [&](const T Vx, const T Vy, const T H, const int cid, double vx64, double vy64, double h64) -> int {
         // classify vertices: d = H - q.V, outside when d < 0
         uint64_t mask = 0;
         T dfirst = 0, dprev = 0, dAin = 0, dAout = 0, dBout = 0, dBin = 0;
-        int i1 = -1, i2 = -1;
+        int i1 = -1, i2 = -1, ntr = 0;
+        bool exact = false;
+    classify:
+        mask = 0;
+        ntr = 0;
         for (int k = 0; k < n; ++k) {
-            T d = fma(-PX(k), Vx, fma(-PY(k), Vy, H));
+            T d = exact ? T(side64(k, vx64, vy64, h64)) : fma(-PX(k), Vx, fma(-PY(k), Vy, H));
             bool o = d < 0;
             if (o) mask |= 1ull << k;
             if (k == 0) {
                 dfirst = d;
             } else {
                 bool po = dprev < 0;
-                if (o && !po) { i1 = k; dAin = dprev; dAout = d; }
+                if (o && !po) { i1 = k; dAin = dprev; dAout = d; ++ntr; }
                 if (!o && po) { i2 = k - 1; dBout = dprev; dBin = d; }
             }
             dprev = d;
+        }
+        if (n > 0 && dfirst < 0 && !(dprev < 0)) ++ntr;
+        if (ntr > 1 && !exact) {   // a convex polygon crosses a line once: rounding noise, redo in FP64
+            exact = true;
+            goto classify;
         }
         if (!mask) return 0;
         const int nout = __builtin_popcountll(mask);
@@ -272,8 +297,9 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         // P & disc, and every edge left is short, which keeps the chord/circle
         // intersections of the ball clip well conditioned.
         const T sq = T(ball_square(w0));
-        if (cut(T(1), T(0), sq, -5) == 2 || cut(T(0), T(1), sq, -6) == 2 || cut(T(-1), T(0), sq, -7) == 2 ||
-            cut(T(0), T(-1), sq, -8) == 2)
+        const double sqd = ball_square(w0);
+        if (cut(T(1), T(0), sq, -5, 1.0, 0.0, sqd) == 2 || cut(T(0), T(1), sq, -6, 0.0, 1.0, sqd) == 2 ||
+            cut(T(-1), T(0), sq, -7, -1.0, 0.0, sqd) == 2 || cut(T(0), T(-1), sq, -8, 0.0, -1.0, sqd) == 2)
             overflow = true;
         R2 = radius2();
     }
@@ -302,13 +328,13 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
           for (int gx = gxa; gx <= gxb && n > 0 && !overflow; gx += step) {
             if (gx < 0 || gx >= g.ncell[0]) continue;
             const uint32_t code = morton2(gx, gy);
-            const uint32_t cs = a.cell_start[code], ce = a.cell_start[code + 1];
+            const uint32_t cs = pv.cell_start[code], ce = pv.cell_start[code + 1];
             if (cs == ce) continue;
             {
                 double bx0 = g.lo[0] + gx * g.h[0], by0 = g.lo[1] + gy * g.h[1];
                 double ex = fmax(0.0, fmax(bx0 - c0x, c0x - (bx0 + g.h[0])));
                 double ey = fmax(0.0, fmax(by0 - c0y, c0y - (by0 + g.h[1])));
-                if (region_skippable<T>(sqrt(ex * ex + ey * ey), R2, w0, a.cell_wmax[code])) continue;
+                if (region_skippable<T>(sqrt(ex * ex + ey * ey), R2, w0, pv.cell_wmax[code])) continue;
             }
             for (uint32_t j = cs; j < ce; ++j) {
                 if ((int64_t)j == t) continue;
@@ -318,7 +344,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                 if (hh > 0 && R2 * vv <= hh * hh * (1 - Eps<T>::cand)) continue;   // cannot cut
                 if (vv == 0) { bad = true; continue; }                             // coincident diracs
                 const T Vx = T(vx), Vy = T(vy), H = T(hh);
-                const int rc = cut(Vx, Vy, H, (int)j);
+                const int rc = cut(Vx, Vy, H, (int)j, vx, vy, hh);
                 if (rc == 2) { overflow = true; break; }
                 if (rc == 0) continue;
                 if (n == 0) break;
@@ -346,8 +372,8 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         y = (double)PY(k);
         if (sizeof(T) < sizeof(double)) {
             double a1, b1, e1, a2, b2, e2;
-            line_of(a, PI_(k == 0 ? n - 1 : k - 1), c0x, c0y, w0, a1, b1, e1);
-            line_of(a, PI_(k), c0x, c0y, w0, a2, b2, e2);
+            line_of(a, pv, PI_(k == 0 ? n - 1 : k - 1), c0x, c0y, w0, a1, b1, e1);
+            line_of(a, pv, PI_(k), c0x, c0y, w0, a2, b2, e2);
             double det = a1 * b2 - a2 * b1;
             if (fabs(det) > 1e-12 * (fabs(a1 * b2) + fabs(a2 * b1))) {
                 x = (e1 * b2 - e2 * b1) / det;
@@ -471,7 +497,10 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         ctr_add(a.ctr + 2 * WCTR + slot, A);
     }
     if (it.row_ovf) flag_or(a.flags, FLAG_ROW_OVERFLOW);
-    if (mass == 0) flag_or(a.flags, FLAG_EMPTY);
+    if (mass == 0) {
+        flag_or(a.flags, FLAG_EMPTY);
+        if (a.pflags) flag_or(a.pflags + pv.p, FLAG_EMPTY);
+    }
     return 0;
 #undef PX
 #undef PY

@@ -86,8 +86,8 @@ struct Cell3 {
         const int axis = s >> 1, up = s & 1;
         const double c0[3] = {c0x, c0y, c0z};
         double off;
-        if (id >= -6) off = (up ? a.dom_hi[axis] : a.dom_lo[axis]) - c0[axis];
-        else off = up ? vhi[axis] : vlo[axis];
+        if (id >= -6) off = (up ? sel3(a.dom_hi, axis) : sel3(a.dom_lo, axis)) - sel3(c0, axis);
+        else off = up ? sel3(vhi, axis) : sel3(vlo, axis);
         nx = axis == 0 ? (up ? 1.0 : -1.0) : 0.0;
         ny = axis == 1 ? (up ? 1.0 : -1.0) : 0.0;
         nz = axis == 2 ? (up ? 1.0 : -1.0) : 0.0;
@@ -107,7 +107,7 @@ struct Cell3 {
         const double c0[3] = {c0x, c0y, c0z};
         for (int s = 0; s < 6; ++s) {
             const int axis = s >> 1, up = s & 1;
-            double off = (up ? a.dom_hi[axis] : a.dom_lo[axis]) - c0[axis];
+            double off = (up ? sel3(a.dom_hi, axis) : sel3(a.dom_lo, axis)) - sel3(c0, axis);
             add_plane(s, axis == 0 ? (up ? 1 : -1) : 0, axis == 1 ? (up ? 1 : -1) : 0, axis == 2 ? (up ? 1 : -1) : 0,
                       up ? off : -off, -1 - s);
         }
@@ -149,13 +149,74 @@ struct Cell3 {
         return -1;
     }
 
+    // Side of vertex k for the plane n.q <= h, exactly from the planes: with rows
+    // [n, h] of the vertex's planes a, b, c and of the new plane,
+    // det4 = (h - n.v) det3 and det3 = det(n_a, n_b, n_c) > 0 (orientation).
+    SDOT_HD double tie_distance(int k, double nx, double ny, double nz, double h) const {
+        const int t = V_T(k);
+        double q[4][4];
+        plane64(t & 255, q[0][0], q[0][1], q[0][2], q[0][3]);
+        plane64((t >> 8) & 255, q[1][0], q[1][1], q[1][2], q[1][3]);
+        plane64((t >> 16) & 255, q[2][0], q[2][1], q[2][2], q[2][3]);
+        q[3][0] = nx;
+        q[3][1] = ny;
+        q[3][2] = nz;
+        q[3][3] = h;
+        const double s0 = q[0][0] * q[1][1] - q[1][0] * q[0][1], s1 = q[0][0] * q[1][2] - q[1][0] * q[0][2];
+        const double s2 = q[0][0] * q[1][3] - q[1][0] * q[0][3], s3 = q[0][1] * q[1][2] - q[1][1] * q[0][2];
+        const double s4 = q[0][1] * q[1][3] - q[1][1] * q[0][3], s5 = q[0][2] * q[1][3] - q[1][2] * q[0][3];
+        const double c5 = q[2][2] * q[3][3] - q[3][2] * q[2][3], c4 = q[2][1] * q[3][3] - q[3][1] * q[2][3];
+        const double c3 = q[2][1] * q[3][2] - q[3][1] * q[2][2], c2 = q[2][0] * q[3][3] - q[3][0] * q[2][3];
+        const double c1 = q[2][0] * q[3][2] - q[3][0] * q[2][2], c0 = q[2][0] * q[3][1] - q[3][0] * q[2][1];
+        const double det4 = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+        const double det3 = q[0][0] * (q[1][1] * q[2][2] - q[1][2] * q[2][1]) - q[0][1] * (q[1][0] * q[2][2] - q[1][2] * q[2][0]) +
+                            q[0][2] * (q[1][0] * q[2][1] - q[1][1] * q[2][0]);
+        return det3 > 0 ? det4 / det3 : (double)m.d[k * m.S];
+    }
+
+    // renumber the planes still bounding a face, freeing the slots of removed ones
+    SDOT_HD void compact_planes() {
+        int newidx[256];
+        for (int k = 0; k < np; ++k) newidx[k] = -1;
+        for (int k = 0; k < nv; ++k) {
+            const int t = V_T(k);
+            newidx[t & 255] = newidx[(t >> 8) & 255] = newidx[(t >> 16) & 255] = 0;
+        }
+        int w = 0;
+        for (int k = 0; k < np; ++k) {
+            if (newidx[k] < 0) continue;
+            newidx[k] = w;
+            if (w != k) {
+                P_X(w) = P_X(k);
+                P_Y(w) = P_Y(k);
+                P_Z(w) = P_Z(k);
+                P_H(w) = P_H(k);
+                P_I(w) = P_I(k);
+            }
+            ++w;
+        }
+        for (int k = 0; k < nv; ++k) {
+            const int t = V_T(k);
+            V_T(k) = newidx[t & 255] | (newidx[(t >> 8) & 255] << 8) | (newidx[(t >> 16) & 255] << 16);
+        }
+        np = w;
+    }
+
     // cut by q.n <= h (plane id `id`): 0 = no change, 1 = cut (nv may become 0),
     // 2 = capacity exceeded, 3 = inconsistent topology
-    SDOT_HD int cut(T nx, T ny, T nz, T h, int id, double nx64, double ny64, double nz64, double h64) {
+    SDOT_HD int cut(T nx, T ny, T nz, T h, int id, double nx64, double ny64, double nz64, double h64,
+                    double tie_tol = 0, bool may_compact = true) {
         int nout = 0;
         for (int k = 0; k < nv; ++k) {
             T dk = fma(-V_X(k), nx, fma(-V_Y(k), ny, fma(-V_Z(k), nz, h)));
             V_D(k) = dk;
+            if (fabs((double)dk) <= tie_tol) {   // near-tie: decide from the planes
+                const double dd = tie_distance(k, nx64, ny64, nz64, h64);
+                dk = T(dd);
+                if (dd < 0 && !(dk < 0)) dk = -T(1e-30);
+                if (!(dd < 0) && dk < 0) dk = 0;
+                V_D(k) = dk;
+            }
             nout += dk < 0;
         }
         if (nout == 0) return 0;
@@ -163,6 +224,7 @@ struct Cell3 {
             nv = 0;
             return 1;
         }
+        if (np >= PCAP && may_compact) compact_planes();
         if (np >= PCAP) return 2;
         const int pn = np;
         // new vertices are appended after the current ones, then the list is compacted
@@ -284,11 +346,11 @@ template <class T>
 struct Eps3;
 template <>
 struct Eps3<double> {
-    static constexpr double region = 1e-9, cand = 1e-9;
+    static constexpr double region = 1e-9, cand = 1e-9, tie = 1e-10;
 };
 template <>
 struct Eps3<float> {
-    static constexpr double region = 2e-5, cand = 2e-5;
+    static constexpr double region = 2e-5, cand = 2e-5, tie = 1e-4;
 };
 
 template <class T>
@@ -360,7 +422,7 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
                         const double hv = 0.5 * (vv + w0 - a.sw[j]);
                         if (hv > 0 && R2 * vv <= hv * hv * (1 - Eps3<T>::cand)) continue;
                         if (vv == 0) { bad = true; continue; }
-                        const int rc = C.cut(T(vx), T(vy), T(vz), T(hv), (int)j, vx, vy, vz, hv);
+                        const int rc = C.cut(T(vx), T(vy), T(vz), T(hv), (int)j, vx, vy, vz, hv, Eps3<T>::tie * sqrt(vv * R2));
                         if (rc >= 2) { overflow = true; break; }
                         if (rc == 1) {
                             if (C.nv == 0) break;
@@ -373,6 +435,7 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
     }
     if (bad) flag_or(a.flags, FLAG_BAD_INPUT);
     if (overflow) return 2;
+    if (C.nv > 0) C.compact_planes();   // voxel cuts then start from the face count
 
     // ---- integrals ----
     double mass = 0, perim = 0;
@@ -416,11 +479,12 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
                         bool empty = false;
                         for (int s = 0; s < 6 && !empty; ++s) {
                             const int axis = s >> 1, up = s & 1;
-                            if (up ? (iv[axis] == i1[axis]) : (iv[axis] == i0[axis])) continue;  // side beyond the cell
-                            const double off = up ? C.vhi[axis] : C.vlo[axis];
+                            if (up ? (sel3(iv, axis) == sel3(i1, axis)) : (sel3(iv, axis) == sel3(i0, axis))) continue;
+                            const double off = up ? sel3(C.vhi, axis) : sel3(C.vlo, axis);
                             const double nxx = axis == 0 ? (up ? 1.0 : -1.0) : 0.0, nyy = axis == 1 ? (up ? 1.0 : -1.0) : 0.0,
                                          nzz = axis == 2 ? (up ? 1.0 : -1.0) : 0.0, hq = up ? off : -off;
-                            const int rc = C.cut(T(nxx), T(nyy), T(nzz), T(hq), -7 - s, nxx, nyy, nzz, hq);
+                            const int rc = C.cut(T(nxx), T(nyy), T(nzz), T(hq), -7 - s, nxx, nyy, nzz, hq,
+                                                 Eps3<T>::tie * sqrt(R2), false);
                             if (rc >= 2) { overflow = true; break; }
                             if (C.nv == 0) empty = true;
                         }

@@ -59,61 +59,47 @@ __global__ void k_csr_count(const int *row_len, const int *inv_perm, int64_t n, 
     if (i < n) cnt[i] = row_len[inv_perm[i]] + 1;
 }
 
-template <int MAXE>
-__global__ void k_csr_fill(const int *row_len, const int *ell_col, const double *ell_val, int64_t stride,
-                           const double *diag, const int *perm, const int *inv_perm, int64_t n, const int64_t *off,
-                           int64_t *cols, double *vals) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+// one row per thread: gather the ELL row (columns mapped to dirac indices) and
+// the diagonal into this thread's LDS slot, insertion-sort by column, write
+template <int MAXE, int BS>
+__global__ __launch_bounds__(BS) void k_csr_fill(const int *row_len, const int *ell_col, const double *ell_val,
+                                                 int64_t stride, const double *diag, const int *perm,
+                                                 const int *inv_perm, int64_t n, const int64_t *off, int64_t *cols,
+                                                 double *vals) {
+    __shared__ int sc[MAXE * BS];
+    __shared__ double sv[MAXE * BS];
+    const int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x;
     if (i >= n) return;
+    int *c = sc + threadIdx.x;
+    double *v = sv + threadIdx.x;
     const int64_t t = inv_perm[i];
     const int L = row_len[t];
-    int64_t c[MAXE];
-    double v[MAXE];
     int m = 0;
-    c[m] = i;
-    v[m] = diag[t];
-    ++m;
-    for (int e = 0; e < L && m < MAXE; ++e) {
-        int64_t cc = perm[ell_col[e * stride + t]];
-        double vv = ell_val[e * stride + t];
-        // insertion by (col, val)
+    auto insert = [&](int cc, double vv) {
         int k = m;
-        while (k > 0 && (c[k - 1] > cc || (c[k - 1] == cc && v[k - 1] > vv))) {
-            c[k] = c[k - 1];
-            v[k] = v[k - 1];
+        while (k > 0 && c[(k - 1) * BS] > cc) {
+            c[k * BS] = c[(k - 1) * BS];
+            v[k * BS] = v[(k - 1) * BS];
             --k;
         }
-        c[k] = cc;
-        v[k] = vv;
+        c[k * BS] = cc;
+        v[k * BS] = vv;
         ++m;
-    }
-    // the diagonal went in first; keep (col,val) order for it as well
-    for (int k = 1; k < m; ++k) {
-        if (c[k - 1] > c[k] || (c[k - 1] == c[k] && v[k - 1] > v[k])) {
-            int64_t tc = c[k];
-            double tv = v[k];
-            int q = k;
-            while (q > 0 && (c[q - 1] > tc || (c[q - 1] == tc && v[q - 1] > tv))) {
-                c[q] = c[q - 1];
-                v[q] = v[q - 1];
-                --q;
-            }
-            c[q] = tc;
-            v[q] = tv;
-        }
-    }
-    int64_t o = off[i];
+    };
+    insert((int)i, diag[t]);
+    for (int e = 0; e < L && m < MAXE; ++e) insert(perm[ell_col[e * stride + t]], ell_val[e * stride + t]);
+    const int64_t o = off[i];
     int w = 0;
     for (int k = 0; k < m; ++k) {
-        if (k + 1 < m && c[k] == c[k + 1]) {
-            v[k + 1] += v[k];
+        if (k + 1 < m && c[k * BS] == c[(k + 1) * BS]) {
+            v[(k + 1) * BS] += v[k * BS];
             continue;
         }
-        cols[o + w] = c[k];
-        vals[o + w] = v[k];
+        cols[o + w] = c[k * BS];
+        vals[o + w] = v[k * BS];
         ++w;
     }
-    for (int64_t k = w; k < m; ++k) {   // merged duplicates leave padding: mark it
+    for (int k = w; k < m; ++k) {   // merged duplicates leave padding: mark it
         cols[o + k] = -1;
         vals[o + k] = 0;
     }
@@ -198,13 +184,13 @@ int64_t build_csr_device(sdot_pd *pd) {
     pd->csr_col.ensure(bound);
     pd->csr_val.ensure(bound);
     if (pd->ell_cap + 1 <= 33)
-        hipLaunchKernelGGL(k_csr_fill<33>, dim3(nb), dim3(256), 0, s, pd->row_len.p, pd->ell_col.p, pd->ell_val.p,
-                           (int64_t)N, pd->diag.p, pd->perm.p, pd->inv_perm.p, N, pd->csr_off.p, pd->csr_col.p,
-                           pd->csr_val.p);
+        hipLaunchKernelGGL((k_csr_fill<33, 64>), dim3((unsigned)((N + 63) / 64)), dim3(64), 0, s, pd->row_len.p,
+                           pd->ell_col.p, pd->ell_val.p, (int64_t)N, pd->diag.p, pd->perm.p, pd->inv_perm.p, N,
+                           pd->csr_off.p, pd->csr_col.p, pd->csr_val.p);
     else
-        hipLaunchKernelGGL(k_csr_fill<97>, dim3(nb), dim3(256), 0, s, pd->row_len.p, pd->ell_col.p, pd->ell_val.p,
-                           (int64_t)N, pd->diag.p, pd->perm.p, pd->inv_perm.p, N, pd->csr_off.p, pd->csr_col.p,
-                           pd->csr_val.p);
+        hipLaunchKernelGGL((k_csr_fill<97, 32>), dim3((unsigned)((N + 31) / 32)), dim3(32), 0, s, pd->row_len.p,
+                           pd->ell_col.p, pd->ell_val.p, (int64_t)N, pd->diag.p, pd->perm.p, pd->inv_perm.p, N,
+                           pd->csr_off.p, pd->csr_col.p, pd->csr_val.p);
     HIPCK(hipGetLastError());
     return bound;
 }

@@ -48,7 +48,8 @@ __host__ __device__ c3::Poly3Mem<T> slot(char *base, int64_t S, int64_t s) {
 }
 
 template <class T>
-__global__ __launch_bounds__(BS3) void k_cells3_thread(CellArgs a, char *scratch, int64_t nslots) {
+__global__ __launch_bounds__(BS3) void k_cells3_thread(const CellArgs *__restrict__ ap, char *scratch, int64_t nslots) {
+    const CellArgs &a = *ap;
     const int64_t s = blockIdx.x * (int64_t)BS3 + threadIdx.x;
     if (s >= nslots) return;
     const c3::Poly3Mem<T> m = slot<T>(scratch, nslots, s);
@@ -57,6 +58,16 @@ __global__ __launch_bounds__(BS3) void k_cells3_thread(CellArgs a, char *scratch
         const int64_t t = a.work ? a.work[k] : k;
         if (c3::cell3_body<T>(a, t, m, VCAP3, PCAP3)) flag_or(a.flags, FLAG_ROW_OVERFLOW);
     }
+}
+
+// the arguments go to HBM and the kernels read them through a pointer: a by-value
+// argument block whose address is taken would be copied to scratch per lane
+const CellArgs *upload_args(sdot_pd *pd, const CellArgs &a, int slot) {
+    pd->dargs.ensure(2);
+    pd->hargs = a;
+    HIPCK(hipMemcpyAsync(pd->dargs.p + slot, &pd->hargs, sizeof(CellArgs), hipMemcpyHostToDevice, pd->stream));
+    HIPCK(hipStreamSynchronize(pd->stream));
+    return pd->dargs.p + slot;
 }
 
 template <class T>
@@ -70,7 +81,8 @@ void launch_thread3(sdot_pd *pd, const CellArgs &a) {
     if (nslots < 1) return;
     pd->scratch3.ensure(slot_bytes<T>() * nslots);
     const unsigned nb = (unsigned)((nslots + BS3 - 1) / BS3);
-    hipLaunchKernelGGL(k_cells3_thread<T>, dim3(nb), dim3(BS3), 0, pd->stream, a, pd->scratch3.p, nslots);
+    const CellArgs *da = upload_args(pd, a, 1);
+    hipLaunchKernelGGL(k_cells3_thread<T>, dim3(nb), dim3(BS3), 0, pd->stream, da, pd->scratch3.p, nslots);
     HIPCK(hipGetLastError());
 }
 
@@ -95,6 +107,7 @@ struct WaveLds {
     double hacc[WV];              // density-weighted face areas (image mode)
     double sx[WV], sy[WV], sz[WV];
     int stri[WV];
+    int cpre[WV], ccs[WV], ccnt[WV];   // staged grid cells: candidate prefix, first dirac, count
 };
 
 __device__ inline void wave_sync() {
@@ -134,7 +147,10 @@ struct WaveCell {
     T x = 0, y = 0, z = 0;
     int tri = 0;
     double c0x, c0y, c0z, w0;
-    double vlo[3], vhi[3];
+    // current voxel box (local coordinates), kept as scalars so the object stays in registers
+    double vlo0 = 0, vlo1 = 0, vlo2 = 0, vhi0 = 0, vhi1 = 0, vhi2 = 0;
+    __device__ double vlo(int axis) const { return axis == 0 ? vlo0 : (axis == 1 ? vlo1 : vlo2); }
+    __device__ double vhi(int axis) const { return axis == 0 ? vhi0 : (axis == 1 ? vhi1 : vhi2); }
 
     __device__ WaveCell(const CellArgs &a_, WaveLds &L_, int lane_) : a(a_), L(L_), lane(lane_) {}
 
@@ -162,7 +178,7 @@ struct WaveCell {
         const double c0[3] = {c0x, c0y, c0z};
         if (lane < 6) {
             const int axis = lane >> 1, up = lane & 1;
-            const double off = (up ? a.dom_hi[axis] : a.dom_lo[axis]) - c0[axis];
+            const double off = (up ? sel3(a.dom_hi, axis) : sel3(a.dom_lo, axis)) - sel3(c0, axis);
             L.pn[lane][0] = axis == 0 ? (up ? 1.0 : -1.0) : 0.0;
             L.pn[lane][1] = axis == 1 ? (up ? 1.0 : -1.0) : 0.0;
             L.pn[lane][2] = axis == 2 ? (up ? 1.0 : -1.0) : 0.0;
@@ -192,17 +208,85 @@ struct WaveCell {
         return wave_max(r);
     }
 
+    // Side of this lane's vertex for the plane n.q <= h, exactly from the planes:
+    // with rows [n_k, h_k] of the vertex's planes a, b, c and the new plane p,
+    // det4 = (h_p - n_p.v) det3 and det3 = det(n_a, n_b, n_c) > 0 (orientation).
+    __device__ double tie_distance(double nx, double ny, double nz, double h) const {
+        double m[4][4];
+        plane64(tri & 255, m[0][0], m[0][1], m[0][2], m[0][3]);
+        plane64((tri >> 8) & 255, m[1][0], m[1][1], m[1][2], m[1][3]);
+        plane64((tri >> 16) & 255, m[2][0], m[2][1], m[2][2], m[2][3]);
+        m[3][0] = nx;
+        m[3][1] = ny;
+        m[3][2] = nz;
+        m[3][3] = h;
+        const double s0 = m[0][0] * m[1][1] - m[1][0] * m[0][1], s1 = m[0][0] * m[1][2] - m[1][0] * m[0][2];
+        const double s2 = m[0][0] * m[1][3] - m[1][0] * m[0][3], s3 = m[0][1] * m[1][2] - m[1][1] * m[0][2];
+        const double s4 = m[0][1] * m[1][3] - m[1][1] * m[0][3], s5 = m[0][2] * m[1][3] - m[1][2] * m[0][3];
+        const double c5 = m[2][2] * m[3][3] - m[3][2] * m[2][3], c4 = m[2][1] * m[3][3] - m[3][1] * m[2][3];
+        const double c3 = m[2][1] * m[3][2] - m[3][1] * m[2][2], c2 = m[2][0] * m[3][3] - m[3][0] * m[2][3];
+        const double c1 = m[2][0] * m[3][2] - m[3][0] * m[2][2], c0 = m[2][0] * m[3][1] - m[3][0] * m[2][1];
+        const double det4 = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+        const double det3 = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                            m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+        return det3 > 0 ? det4 / det3 : (double)(h - (nx * x + ny * y + nz * z));
+    }
+
+    // renumber the planes still bounding a face (slots of removed planes are freed)
+    __device__ void compact_planes() {
+        const unsigned long long used = used_planes();
+        const bool keep = lane < np && ((used >> lane) & 1ull);
+        double q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+        int qid = 0;
+        if (keep) {
+            q0 = L.pn[lane][0];
+            q1 = L.pn[lane][1];
+            q2 = L.pn[lane][2];
+            q3 = L.pn[lane][3];
+            qid = L.pid[lane];
+        }
+        wave_sync();
+        const int dst = __popcll(used & ((1ull << lane) - 1ull));
+        if (keep) {
+            L.pn[dst][0] = q0;
+            L.pn[dst][1] = q1;
+            L.pn[dst][2] = q2;
+            L.pn[dst][3] = q3;
+            L.pid[dst] = qid;
+        }
+        if (lane < nv) {
+            const int a0 = tri & 255, a1 = (tri >> 8) & 255, a2 = (tri >> 16) & 255;
+            const int b0 = __popcll(used & ((1ull << a0) - 1ull)), b1 = __popcll(used & ((1ull << a1) - 1ull)),
+                      b2 = __popcll(used & ((1ull << a2) - 1ull));
+            tri = b0 | (b1 << 8) | (b2 << 16);
+        }
+        np = __popcll(used);
+        wave_sync();
+        write_edges();
+    }
+
     // cut by q.n <= h: 0 no change, 1 cut (nv may be 0), 2 capacity exceeded
-    __device__ int cut(T nx, T ny, T nz, T h, int id, double nx64, double ny64, double nz64, double h64) {
+    __device__ int cut(T nx, T ny, T nz, T h, int id, double nx64, double ny64, double nz64, double h64,
+                       double tie_tol, bool may_compact) {
         const bool valid = lane < nv;
-        const T d = valid ? fma(-x, nx, fma(-y, ny, fma(-z, nz, h))) : T(1);
-        const bool out = valid && d < 0;
+        T d = valid ? fma(-x, nx, fma(-y, ny, fma(-z, nz, h))) : T(1);
+        bool out = valid && d < 0;
+        const bool near = valid && fabs((double)d) <= tie_tol;
+        if (__ballot(near)) {
+            if (near) {
+                const double dd = tie_distance(nx64, ny64, nz64, h64);
+                out = dd < 0;
+                d = T(dd);
+                if (out && !(d < 0)) d = -T(1e-30);
+            }
+        }
         const unsigned long long mo = __ballot(out);
         if (!mo) return 0;
         if (__popcll(mo) == nv) {
             nv = 0;
             return 1;
         }
+        if (np >= WV && may_compact) compact_planes();
         if (np >= WV) return 2;
         const int pn = np;
         const int p[3] = {tri & 255, (tri >> 8) & 255, (tri >> 16) & 255};
@@ -263,7 +347,7 @@ struct WaveCell {
         const int id = L.pid[k];
         if (id <= -7) {   // voxel side
             const int s = -7 - id, axis = s >> 1, up = s & 1;
-            const double off = up ? vhi[axis] : vlo[axis];
+            const double off = up ? vhi(axis) : vlo(axis);
             nx = axis == 0 ? (up ? 1.0 : -1.0) : 0.0;
             ny = axis == 1 ? (up ? 1.0 : -1.0) : 0.0;
             nz = axis == 2 ? (up ? 1.0 : -1.0) : 0.0;
@@ -340,7 +424,8 @@ struct WaveCell {
 };
 
 template <class T>
-__global__ __launch_bounds__(WV * WPB) void k_cells3_wave(CellArgs a) {
+__global__ __launch_bounds__(WV * WPB) void k_cells3_wave(const CellArgs *__restrict__ ap) {
+    const CellArgs &a = *ap;
     __shared__ WaveLds lds[WPB];
     const int lane = threadIdx.x & (WV - 1);
     const int wid = threadIdx.x / WV;
@@ -371,6 +456,7 @@ __global__ __launch_bounds__(WV * WPB) void k_cells3_wave(CellArgs a) {
             if (r > 0) {
                 double dm = DBL_MAX;
                 bool any = false;
+#pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     if (hh[d] - r + 1 > 0) { dm = fmin(dm, c0[d] - (g.lo[d] + (hh[d] - r + 1) * g.h[d])); any = true; }
                     if (hh[d] + r < g.ncell[d]) { dm = fmin(dm, (g.lo[d] + (hh[d] + r) * g.h[d]) - c0[d]); any = true; }
@@ -378,46 +464,90 @@ __global__ __launch_bounds__(WV * WPB) void k_cells3_wave(CellArgs a) {
                 if (!any) break;
                 if (c3::region_skippable3<T>(fmax(dm, 0.0), R2, w0, W)) break;
             }
-            const int z0 = hh[2] - r < 0 ? 0 : hh[2] - r, z1 = hh[2] + r >= g.ncell[2] ? g.ncell[2] - 1 : hh[2] + r;
-            const int y0 = hh[1] - r < 0 ? 0 : hh[1] - r, y1 = hh[1] + r >= g.ncell[1] ? g.ncell[1] - 1 : hh[1] + r;
-            for (int gz = z0; gz <= z1 && C.nv > 0 && !overflow; ++gz) {
-                for (int gy = y0; gy <= y1 && C.nv > 0 && !overflow; ++gy) {
-                    const bool full = gz == hh[2] - r || gz == hh[2] + r || gy == hh[1] - r || gy == hh[1] + r;
-                    const int step = full || r == 0 ? 1 : 2 * r;
-                    for (int gx = hh[0] - r; gx <= hh[0] + r && C.nv > 0 && !overflow; gx += step) {
-                        if (gx < 0 || gx >= g.ncell[0]) continue;
-                        const uint32_t code = morton3(gx, gy, gz);
-                        const uint32_t cs = a.cell_start[code], ce = a.cell_start[code + 1];
-                        if (cs == ce) continue;
-                        {
-                            const int gg[3] = {gx, gy, gz};
-                            double e2 = 0;
-                            for (int d = 0; d < 3; ++d) {
-                                const double b0 = g.lo[d] + gg[d] * g.h[d];
-                                const double e = fmax(0.0, fmax(b0 - c0[d], c0[d] - (b0 + g.h[d])));
-                                e2 += e * e;
-                            }
-                            if (c3::region_skippable3<T>(sqrt(e2), R2, w0, a.cell_wmax[code])) continue;
-                        }
-                        for (uint32_t j = cs; j < ce; ++j) {
-                            if ((int64_t)j == t) continue;
-                            const double vx = a.spos[3 * (int64_t)j] - c0x, vy = a.spos[3 * (int64_t)j + 1] - c0y,
-                                         vz = a.spos[3 * (int64_t)j + 2] - c0z;
-                            const double vv = vx * vx + vy * vy + vz * vz;
-                            const double hv = 0.5 * (vv + w0 - a.sw[j]);
-                            if (hv > 0 && R2 * vv <= hv * hv * (1 - c3::Eps3<T>::cand)) continue;
-                            if (vv == 0) { bad = true; continue; }
-                            const int rc = C.cut(T(vx), T(vy), T(vz), T(hv), (int)j, vx, vy, vz, hv);
-                            if (rc == 2) { overflow = true; break; }
-                            if (rc == 1) {
-                                if (C.nv == 0) break;
-                                R2 = C.radius2();
-                            }
-                        }
+            // The grid cells of ring r, 64 at a time (one per lane): the cell tests run in
+            // parallel, then the diracs of the surviving cells are staged 64 at a time
+            // (one per lane, loaded in parallel), tested against the current radius,
+            // and only the ballot of possible cutters is applied, in order.
+            const int side = 2 * r + 1;
+            const int ncube = side * side * side;
+            for (int base = 0; base < ncube && C.nv > 0 && !overflow; base += WV) {
+                const int c = base + lane;
+                int dz = c / (side * side) - r, dy = (c / side) % side - r, dx = c % side - r;
+                bool ok = c < ncube && (dx == r || dx == -r || dy == r || dy == -r || dz == r || dz == -r);
+                const int gx = hh[0] + dx, gy = hh[1] + dy, gz = hh[2] + dz;
+                ok = ok && gx >= 0 && gy >= 0 && gz >= 0 && gx < g.ncell[0] && gy < g.ncell[1] && gz < g.ncell[2];
+                uint32_t cs = 0, ce = 0;
+                if (ok) {
+                    const uint32_t code = morton3(gx, gy, gz);
+                    cs = a.cell_start[code];
+                    ce = a.cell_start[code + 1];
+                    if (cs == ce) {
+                        ok = false;
+                    } else {
+                        const double bx = g.lo[0] + gx * g.h[0], by = g.lo[1] + gy * g.h[1], bz = g.lo[2] + gz * g.h[2];
+                        const double ex = fmax(0.0, fmax(bx - c0x, c0x - (bx + g.h[0])));
+                        const double ey = fmax(0.0, fmax(by - c0y, c0y - (by + g.h[1])));
+                        const double ez = fmax(0.0, fmax(bz - c0z, c0z - (bz + g.h[2])));
+                        ok = !c3::region_skippable3<T>(sqrt(ex * ex + ey * ey + ez * ez), R2, w0, a.cell_wmax[code]);
                     }
                 }
+                const int cnt = ok ? (int)(ce - cs) : 0;
+                const int pre = wave_excl_scan(cnt, lane);
+                const int tot = __shfl(pre + cnt, WV - 1);
+                if (tot == 0) continue;
+                L.cpre[lane] = pre;
+                L.ccs[lane] = (int)cs;
+                L.ccnt[lane] = cnt;
+                wave_sync();
+                for (int b0 = 0; b0 < tot && C.nv > 0 && !overflow; b0 += WV) {
+                    const int gi = b0 + lane;
+                    int j = -1;
+                    if (gi < tot) {
+                        int lo = 0, hi = WV - 1;   // last cell slot with cpre <= gi and a non-empty range
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (L.cpre[mid] <= gi) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        while (L.ccnt[lo] == 0) --lo;
+                        j = L.ccs[lo] + (gi - L.cpre[lo]);
+                    }
+                    double vx = 0, vy = 0, vz = 0, hv = 0, vv = 0;
+                    bool cand = j >= 0 && (int64_t)j != t;
+                    if (cand) {
+                        vx = a.spos[3 * (int64_t)j] - c0x;
+                        vy = a.spos[3 * (int64_t)j + 1] - c0y;
+                        vz = a.spos[3 * (int64_t)j + 2] - c0z;
+                        vv = vx * vx + vy * vy + vz * vz;
+                        hv = 0.5 * (vv + w0 - a.sw[j]);
+                        if (vv == 0) {
+                            bad = true;
+                            cand = false;
+                        } else if (hv > 0 && R2 * vv <= hv * hv * (1 - c3::Eps3<T>::cand)) {
+                            cand = false;
+                        }
+                    }
+                    unsigned long long mask = __ballot(cand);
+                    while (mask && C.nv > 0) {
+                        const int k = __ffsll((long long)mask) - 1;
+                        mask &= mask - 1;
+                        const double ux = __shfl(vx, k), uy = __shfl(vy, k), uz = __shfl(vz, k), uh = __shfl(hv, k),
+                                     uv = __shfl(vv, k);
+                        const int uj = __shfl(j, k);
+                        if (uh > 0 && R2 * uv <= uh * uh * (1 - c3::Eps3<T>::cand)) continue;   // R shrank
+                        const double tie = c3::Eps3<T>::tie * sqrt(uv * R2);
+                        const int rc = C.cut(T(ux), T(uy), T(uz), T(uh), uj, ux, uy, uz, uh, tie, true);
+                        if (rc == 2) {
+                            overflow = true;
+                            break;
+                        }
+                        if (rc == 1 && C.nv > 0) R2 = C.radius2();
+                    }
+                }
+                wave_sync();
             }
         }
+        if (C.nv > 0 && !overflow) C.compact_planes();   // voxel cuts then start from the face count
         if (bad && lane == 0) flag_or(a.flags, FLAG_BAD_INPUT);
         if (overflow) {
             if (lane == 0) {
@@ -447,6 +577,7 @@ __global__ __launch_bounds__(WV * WPB) void k_cells3_wave(CellArgs a) {
                 mn[2] = -wave_max(valid ? -Z : -DBL_MAX); mx[2] = wave_max(valid ? Z : -DBL_MAX);
                 int i0[3], i1[3];
                 double vs[3];
+#pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     vs[d] = (a.dom_hi[d] - a.dom_lo[d]) / a.img_n[d];
                     const double lo = mn[d] + c0[d] - a.dom_lo[d], hi = mx[d] + c0[d] - a.dom_lo[d];
@@ -462,18 +593,23 @@ __global__ __launch_bounds__(WV * WPB) void k_cells3_wave(CellArgs a) {
                         for (int ix = i0[0]; ix <= i1[0] && !overflow; ++ix) {
                             const int iv[3] = {ix, iy, iz};
                             const double rho = a.img[((int64_t)iz * a.img_n[1] + iy) * a.img_n[0] + ix];
-                            for (int d = 0; d < 3; ++d) {
-                                C.vlo[d] = a.dom_lo[d] + vs[d] * iv[d] - c0[d];
-                                C.vhi[d] = a.dom_lo[d] + vs[d] * (iv[d] + 1) - c0[d];
-                            }
+                            C.vlo0 = a.dom_lo[0] + vs[0] * ix - c0[0];
+                            C.vhi0 = a.dom_lo[0] + vs[0] * (ix + 1) - c0[0];
+                            C.vlo1 = a.dom_lo[1] + vs[1] * iy - c0[1];
+                            C.vhi1 = a.dom_lo[1] + vs[1] * (iy + 1) - c0[1];
+                            C.vlo2 = a.dom_lo[2] + vs[2] * iz - c0[2];
+                            C.vhi2 = a.dom_lo[2] + vs[2] * (iz + 1) - c0[2];
                             bool empty = false;
-                            for (int s = 0; s < 6 && !empty; ++s) {
+#pragma unroll
+                            for (int s = 0; s < 6; ++s) {
                                 const int axis = s >> 1, up = s & 1;
+                                if (empty || overflow) continue;
                                 if (up ? (iv[axis] == i1[axis]) : (iv[axis] == i0[axis])) continue;
-                                const double off = up ? C.vhi[axis] : C.vlo[axis];
+                                const double off = up ? C.vhi(axis) : C.vlo(axis);
                                 const double nxx = axis == 0 ? (up ? 1.0 : -1.0) : 0.0, nyy = axis == 1 ? (up ? 1.0 : -1.0) : 0.0,
                                              nzz = axis == 2 ? (up ? 1.0 : -1.0) : 0.0, hq = up ? off : -off;
-                                const int rc = C.cut(T(nxx), T(nyy), T(nzz), T(hq), -7 - s, nxx, nyy, nzz, hq);
+                                const int rc = C.cut(T(nxx), T(nyy), T(nzz), T(hq), -7 - s, nxx, nyy, nzz, hq,
+                                                     c3::Eps3<T>::tie * sqrt(R2), false);
                                 if (rc == 2) { overflow = true; break; }
                                 if (C.nv == 0) empty = true;
                             }
@@ -551,7 +687,8 @@ void launch_wave3(sdot_pd *pd, const CellArgs &a) {
     const int64_t need = (a.N + WPB - 1) / WPB;
     if (nblocks > need) nblocks = need;
     if (nblocks < 1) return;
-    hipLaunchKernelGGL(k_cells3_wave<T>, dim3((unsigned)nblocks), dim3(WV * WPB), 0, pd->stream, a);
+    const CellArgs *da = upload_args(pd, a, 0);
+    hipLaunchKernelGGL(k_cells3_wave<T>, dim3((unsigned)nblocks), dim3(WV * WPB), 0, pd->stream, da);
     HIPCK(hipGetLastError());
 }
 

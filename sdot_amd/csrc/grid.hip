@@ -98,13 +98,23 @@ __global__ __launch_bounds__(BS) void k_gather_pos(const double *pos, const int 
     inv_perm[i] = (int)p;
 }
 
+// Sorted weights.  Each weight gets a deterministic offset of at most `tie`
+// (about 1e-12 of the squared dirac spacing, a hash of the dirac index): the
+// diagram stays a consistent power diagram, moved by ~1e-12 relative, and
+// exact ties (lattices: four or more planes through one point, three planes
+// through one line) no longer reach the FP64 clipping.
 __global__ __launch_bounds__(BS) void k_gather_w(const double *w, const int *perm, int64_t n, double *sw,
-                                                 unsigned *flags) {
+                                                 unsigned *flags, double tie) {
     int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x;
     if (p >= n) return;
-    double v = w[perm[p]];
+    const int i = perm[p];
+    double v = w[i];
     if (!isfinite(v)) atomicOr(flags, FLAG_BAD_INPUT);
-    sw[p] = v;
+    uint64_t z = (uint64_t)i * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    sw[p] = v + tie * ((double)(z >> 11) * 0x1.0p-53);
 }
 
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
@@ -231,8 +241,53 @@ void build_grid(sdot_pd *pd) {
             lo[d] = fmin(lo[d], part[b * 2 * dim + d]);
             hi[d] = fmax(hi[d], part[b * 2 * dim + dim + d]);
         }
-    // grid parameters: about `dpc` diracs per cell
     double dpc = pd->dpc_x100 > 0 ? pd->dpc_x100 / 100.0 : (dim == 2 ? 3.0 : 2.0);
+    const GridInfo g = grid_params(dim, lo, hi, N, dpc);
+    pd->g = g;
+
+    pd->keys.ensure(N);
+    pd->rank.ensure(N);
+    pd->cell_start.ensure(g.ncodes + 1);
+    pd->scan_tmp.ensure(g.ncodes + scan_tmp_size(g.ncodes));
+    pd->perm.ensure(N);
+    pd->inv_perm.ensure(N);
+    pd->spos.ensure(N * dim);
+    uint32_t *cnt = pd->scan_tmp.p;
+    uint32_t *tmp = pd->scan_tmp.p + g.ncodes;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, cnt, g.ncodes, 0u);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_keys<2>, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->pos.p, N, g, pd->keys.p, pd->rank.p, cnt);
+    else
+        hipLaunchKernelGGL(k_keys<3>, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->pos.p, N, g, pd->keys.p, pd->rank.p, cnt);
+    HIPCK(hipGetLastError());
+    (void)tmp;
+    sort_by_keys(pd, g.ncodes);
+}
+
+void sort_by_keys(sdot_pd *pd, int64_t ncodes) {
+    const int dim = pd->dim;
+    const int64_t N = pd->N;
+    hipStream_t s = pd->stream;
+    uint32_t *cnt = pd->scan_tmp.p;
+    uint32_t *tmp = pd->scan_tmp.p + ncodes;
+    exclusive_scan<uint32_t>(cnt, pd->cell_start.p, ncodes, tmp, s);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->keys.p, pd->rank.p, pd->cell_start.p, N, pd->perm.p);
+    hipLaunchKernelGGL(k_binsort, dim3(nblk(ncodes, BS)), dim3(BS), 0, s, pd->cell_start.p, ncodes, pd->perm.p);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_gather_pos<2>, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->pos.p, pd->perm.p, N, pd->spos.p, pd->inv_perm.p);
+    else
+        hipLaunchKernelGGL(k_gather_pos<3>, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->pos.p, pd->perm.p, N, pd->spos.p, pd->inv_perm.p);
+    HIPCK(hipGetLastError());
+    pd->grid_valid = true;
+}
+
+GridInfo grid_params(int dim, const double *lo_in, const double *hi_in, int64_t N, double dpc) {
+    double lo[3], hi[3];
+    for (int d = 0; d < dim; ++d) {
+        lo[d] = lo_in[d];
+        hi[d] = hi_in[d];
+    }
+    // grid parameters: about `dpc` diracs per cell
     GridInfo g{};
     g.dim = dim;
     double vol = 1;
@@ -268,32 +323,8 @@ void build_grid(sdot_pd *pd) {
     while ((1 << bits) < mx) ++bits;
     g.bits = bits;
     g.ncodes = (int64_t)1 << (dim * bits);
-    pd->g = g;
 
-    pd->keys.ensure(N);
-    pd->rank.ensure(N);
-    pd->cell_start.ensure(g.ncodes + 1);
-    pd->scan_tmp.ensure(g.ncodes + scan_tmp_size(g.ncodes));
-    pd->perm.ensure(N);
-    pd->inv_perm.ensure(N);
-    pd->spos.ensure(N * dim);
-    uint32_t *cnt = pd->scan_tmp.p;
-    uint32_t *tmp = pd->scan_tmp.p + g.ncodes;
-    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, cnt, g.ncodes, 0u);
-    if (dim == 2)
-        hipLaunchKernelGGL(k_keys<2>, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->pos.p, N, g, pd->keys.p, pd->rank.p, cnt);
-    else
-        hipLaunchKernelGGL(k_keys<3>, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->pos.p, N, g, pd->keys.p, pd->rank.p, cnt);
-    HIPCK(hipGetLastError());
-    exclusive_scan<uint32_t>(cnt, pd->cell_start.p, g.ncodes, tmp, s);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->keys.p, pd->rank.p, pd->cell_start.p, N, pd->perm.p);
-    hipLaunchKernelGGL(k_binsort, dim3(nblk(g.ncodes, BS)), dim3(BS), 0, s, pd->cell_start.p, g.ncodes, pd->perm.p);
-    if (dim == 2)
-        hipLaunchKernelGGL(k_gather_pos<2>, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->pos.p, pd->perm.p, N, pd->spos.p, pd->inv_perm.p);
-    else
-        hipLaunchKernelGGL(k_gather_pos<3>, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->pos.p, pd->perm.p, N, pd->spos.p, pd->inv_perm.p);
-    HIPCK(hipGetLastError());
-    pd->grid_valid = true;
+    return g;
 }
 
 void refresh_weights(sdot_pd *pd) {
@@ -303,7 +334,12 @@ void refresh_weights(sdot_pd *pd) {
     pd->sw.ensure(N);
     pd->cell_wmax.ensure(g.ncodes);
     pd->wglob.ensure(1);
-    hipLaunchKernelGGL(k_gather_w, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->w.p, pd->perm.p, N, pd->sw.p, pd->flags.p);
+    // tie-breaking offset scale: 1e-12 of the squared mean spacing of the grid
+    double vol = 1;
+    for (int d = 0; d < pd->dim; ++d) vol *= g.h[d] * g.ncell[d];
+    const double spacing2 = pow(vol / (double)(N > 0 ? N : 1), 2.0 / pd->dim);
+    hipLaunchKernelGGL(k_gather_w, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->w.p, pd->perm.p, N, pd->sw.p, pd->flags.p,
+                       1e-12 * spacing2);
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, pd->wglob.p, dbl_order(-DBL_MAX));
     hipLaunchKernelGGL(k_cell_wmax, dim3(nblk(g.ncodes, BS)), dim3(BS), 0, s, pd->cell_start.p, g.ncodes, pd->sw.p,
                        pd->cell_wmax.p, pd->wglob.p);

@@ -126,6 +126,14 @@ enum : unsigned { FLAG_EMPTY = 1u, FLAG_ROW_OVERFLOW = 2u, FLAG_BAD_INPUT = 4u }
 
 #define SDOT_HD __host__ __device__
 
+// element i (0..2) of a 3-array with constant indices only: a per-lane index
+// into an array held in a kernel argument would make the compiler copy the
+// whole argument block to scratch memory
+template <class A>
+SDOT_HD inline auto sel3(const A &v, int i) -> decltype(v[0]) {
+    return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
+}
+
 SDOT_HD inline void ctr_add(unsigned long long *p, unsigned long long v) {
 #if defined(__HIP_DEVICE_COMPILE__)
     atomicAdd(p, v);
@@ -170,7 +178,50 @@ struct CellArgs {
     const int *work;             // optional list of sorted indices to process
     int64_t work_n;
     unsigned long long *ctr;     // work counters: [k] = sum F, [WCTR + k] = sum K, [2*WCTR + k] = sum A
+    // batches of independent problems (all null for a single problem): problem of
+    // each sorted index, per-problem grid, code offset into cell_start/cell_wmax,
+    // domain box (lo[dim], hi[dim] per problem, stride 6), flags and done marks
+    const int *prob;
+    const GridInfo *grids;
+    const int64_t *code_base;
+    const double *pdom;
+    unsigned *pflags;
+    const int *pdone;
 };
+
+// the per-problem view of CellArgs for sorted index t
+struct ProbView {
+    int p;
+    GridInfo g;
+    const uint32_t *cell_start;
+    const double *cell_wmax;
+    double W;
+    double lo[3], hi[3];
+};
+SDOT_HD inline void prob_view(const CellArgs &a, int64_t t, ProbView &v) {
+    if (!a.prob) {
+        v.p = 0;
+        v.g = a.g;
+        v.cell_start = a.cell_start;
+        v.cell_wmax = a.cell_wmax;
+        v.W = dbl_unorder(a.wglob[0]);
+        for (int d = 0; d < 3; ++d) {
+            v.lo[d] = a.dom_lo[d];
+            v.hi[d] = a.dom_hi[d];
+        }
+        return;
+    }
+    const int p = a.prob[t];
+    v.p = p;
+    v.g = a.grids[p];
+    v.cell_start = a.cell_start + a.code_base[p];
+    v.cell_wmax = a.cell_wmax + a.code_base[p];
+    v.W = dbl_unorder(a.wglob[p]);
+    for (int d = 0; d < 3; ++d) {
+        v.lo[d] = a.pdom[6 * p + d];
+        v.hi[d] = a.pdom[6 * p + 3 + d];
+    }
+}
 constexpr int WCTR = 1024;       // counters are spread over WCTR slots to avoid contention
 
 // ---------------------------------------------------------------------------
@@ -221,6 +272,8 @@ struct sdot_pd {
     unsigned last_flags = 0;
     sdot_amd::DevBuf<unsigned long long> wctr;
     sdot_amd::DevBuf<char> scratch3;   // per-thread polyhedron slots of the 3D thread kernel
+    sdot_amd::DevBuf<sdot_amd::CellArgs> dargs;  // kernel arguments of the 3D kernels, in HBM
+    sdot_amd::CellArgs hargs{};
 
     // CSR export scratch
     sdot_amd::DevBuf<int64_t> csr_off, csr_col, csr_cnt, csr_tmp;
@@ -239,6 +292,8 @@ namespace sdot_amd {
 // grid.hip
 void build_grid(sdot_pd *pd);         // bbox, keys, sort, sorted positions (positions changed)
 void refresh_weights(sdot_pd *pd);    // sorted weights, per-cell and global max weight
+void sort_by_keys(sdot_pd *pd, int64_t ncodes);  // counts (scan_tmp) + keys/rank -> cell_start, perm, sorted positions
+GridInfo grid_params(int dim, const double *lo, const double *hi, int64_t N, double dpc);
 template <class T>
 void exclusive_scan(const T *in, T *out, int64_t n, T *tmp, hipStream_t s);  // out has n+1 entries
 size_t scan_tmp_size(int64_t n);

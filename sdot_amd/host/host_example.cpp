@@ -1,0 +1,60 @@
+// Reference-style driver (the shape of benchmarks/bench_solve_2D.cpp) written
+// against the C++ host layer: 2D diracs uniform in the unit square, unit box
+// domain, Newton solve to uniform target masses, then the Hessian CSR.
+// Prints one JSON line; exits non-zero on failure.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "sdot_mi355x/PowerDiagram.h"
+
+struct Pc {
+    enum { dim = 2, allow_ball_cut = 1, allow_translations = 0 };
+    using TI = std::size_t;
+    using TF = double;
+    using CI = std::size_t;
+};
+
+int main(int argc, char **argv) {
+    using namespace sdot_mi355x;
+    const std::size_t n = argc > 1 ? std::strtoul(argv[1], nullptr, 10) : 10000;
+    std::vector<double> u(2 * n);
+    sdot_rng_uniform(0, 2 * n, u.data());
+    std::vector<Point2> positions(n);
+    for (std::size_t i = 0; i < n; ++i) positions[i] = Point2{{u[2 * i], u[2 * i + 1]}};
+    std::vector<double> weights(n, 0.0), masses(n, 1.0 / n);
+
+    SpZGrid<Pc> grid(11);
+    ConvexPolyhedronAssembly<Pc> bounds;
+    bounds.add_box({{0, 0}}, {{1, 1}});
+    grid.update(positions.data(), weights.data(), n);
+
+    OptimalTransportSolver<SpZGrid<Pc>, ConvexPolyhedronAssembly<Pc>> solver(&grid, &bounds);
+    solver.stop_rule = 0;
+    solver.tol = 1e-6;
+    int st = solver.solve(positions.data(), weights.data(), n, masses.data());
+
+    std::vector<std::size_t> offsets, columns;
+    std::vector<double> values, v;
+    int err = get_der_integrals_wrt_weights(offsets, columns, values, v, grid, bounds, positions.data(), weights.data(), n,
+                                            FunctionEnum::Unit());
+    double max_rel = 0, sum = 0;
+    for (std::size_t i = 0; i < n; ++i) {
+        max_rel = std::fmax(max_rel, std::fabs(v[i] * n - 1));
+        sum += v[i];
+    }
+    // symmetric rows with zero sums (no ball)
+    double max_rowsum = 0;
+    for (std::size_t i = 0; i < n; ++i) {
+        double s = 0;
+        for (std::size_t k = offsets[i]; k < offsets[i + 1]; ++k) s += values[k];
+        max_rowsum = std::fmax(max_rowsum, std::fabs(s));
+    }
+    std::vector<double> ball(n);
+    get_integrals(ball.data(), grid, bounds, positions.data(), weights.data(), n, FunctionEnum::InBallW05());
+    std::printf("{\"status\": %d, \"der_status\": %d, \"newton_iters\": %d, \"max_rel_mass_err\": %.3e, \"mass_sum\": %.15f, "
+                "\"nnz\": %zu, \"max_row_sum\": %.3e, \"ball_mass_0\": %.6e}\n",
+                st, err, solver.stats.newton_iters, max_rel, sum, columns.size(), max_rowsum, ball[0]);
+    return (st == 0 && err == 0 && max_rel < 1e-6 && max_rowsum < 1e-9) ? 0 : 1;
+}

@@ -49,3 +49,11 @@ def test_invalid_arguments_are_status_codes():
     assert L.sdot_pd_create(ctypes.byref(h), 2, 9) == sdot_amd.SDOT_INVALID_ARG
     assert L.sdot_pd_destroy(None) == sdot_amd.SDOT_INVALID_ARG
     assert L.sdot_pd_compute(None) == sdot_amd.SDOT_INVALID_ARG
+
+
+def test_host_layer_header_compiles():
+    import subprocess
+    src = os.path.join(ROOT, "sdot_amd", "host", "host_example.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I" + os.path.join(ROOT, "sdot_amd", "host"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

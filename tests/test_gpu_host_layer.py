@@ -1,0 +1,23 @@
+"""The C++ host layer (reference-style SpZGrid / ConvexPolyhedronAssembly /
+OptimalTransportSolver / get_der_integrals_wrt_weights over the C-ABI) driven
+by a compiled program, as a reference user would."""
+import json
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+pytestmark = pytest.mark.gpu
+
+
+def test_host_example_solves_and_exports_csr():
+    exe = os.path.join(ROOT, "sdot_amd", "host", "host_example")
+    assert os.path.exists(exe), "build first (make -C sdot_amd)"
+    out = subprocess.run([exe, "10000"], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    assert out.returncode == 0, r
+    assert r["status"] == 0 and r["der_status"] == 0
+    assert r["max_rel_mass_err"] < 1e-6
+    assert abs(r["mass_sum"] - 1) < 1e-12
