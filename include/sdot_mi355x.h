@@ -144,6 +144,50 @@ typedef struct sdot_timings {
 /* HIP-event timings of the last sdot_pd_compute (on the handle's stream). */
 sdot_status sdot_pd_last_timings(sdot_pd *pd, sdot_timings *t);
 
+/* ---------------------------------------------------------------------------
+ * Batches: K independent 2D problems solved together (one launch per kernel
+ * over all problems).  The reference solves one problem per
+ * OptimalTransportSolver::solve call (OptimalTransportSolver.tcc:27-119); a batch
+ * applies the same Newton rules per problem, each problem with its own grid,
+ * domain box, convergence test and iteration count.  A problem's weights do not
+ * depend on the other problems of the batch (bitwise).
+ * ------------------------------------------------------------------------- */
+typedef struct sdot_batch sdot_batch;
+
+typedef struct sdot_batch_stats {
+    int newton_iters_max;        /* most Newton passes of any problem */
+    int nb_converged;
+    int pcg_iters_total;         /* sum over problems and Newton steps */
+    int backtracks;              /* sum over problems */
+    int graph_replays;           /* launches of the captured PCG graph */
+    int tuned_dpc_x100;          /* grid resolution used (diracs per grid cell x100) */
+    double max_rel_mass_err;     /* over problems, at the returned weights */
+    double seconds_cells;        /* device time of the batched cell passes */
+    double seconds_solve;        /* wall time of the batched PCG solves */
+    double seconds_total;        /* wall time of the call */
+    double ms_pcg_iteration;     /* device time of one captured PCG iteration (all problems) */
+} sdot_batch_stats;
+
+/* dim must be 2 (3 returns SDOT_NOT_SUPPORTED). */
+sdot_status sdot_batch_create(sdot_batch **b, int dim, int precision);
+sdot_status sdot_batch_destroy(sdot_batch *b);
+const char *sdot_batch_last_error(const sdot_batch *b);
+/* Problem p owns diracs [offsets[p], offsets[p+1]) of the concatenated arrays
+ * (positions interleaved, weights = start, targets > 0); boxes: (x0, y0, x1, y1)
+ * per problem, NULL = unit square for all; ball: 1 = InBallW05 for all problems. */
+sdot_status sdot_batch_set_problems(sdot_batch *b, int64_t nb_problems, const int64_t *offsets, const double *positions,
+                                    const double *weights, const double *targets, const double *boxes, int ball);
+/* diracs_per_cell_x100: grid resolution (0 = autotune on the first solve of a size
+ * class, then cached); use_graph: replay the PCG iterations from a captured hipGraph. */
+sdot_status sdot_batch_set_tuning(sdot_batch *b, int diracs_per_cell_x100, int use_graph);
+/* Newton on every problem, starting from the weights given to set_problems; stop_rule
+ * as sdot_ot_solve; the autotuning passes (first solve of a size class) use the start
+ * weights.  weights_out (N), masses_out
+ * (N, may be NULL): dirac order; status_out / iters_out (nb_problems, may be NULL).
+ * Returns SDOT_OK when every problem converged, SDOT_NOT_CONVERGED otherwise. */
+sdot_status sdot_batch_solve(sdot_batch *b, double tol, int stop_rule, int max_iter, double *weights_out,
+                             double *masses_out, int *status_out, int *iters_out, sdot_batch_stats *stats);
+
 /* splitmix64 -> xoshiro256** uniform doubles (x >> 11) * 2^-53: the synthetic-input generator. */
 void sdot_rng_uniform(uint64_t seed, int64_t n, double *out);
 

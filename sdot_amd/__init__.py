@@ -57,6 +57,15 @@ class Timings(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [("newton_iters_max", _int), ("nb_converged", _int), ("pcg_iters_total", _int), ("backtracks", _int),
+                ("graph_replays", _int), ("tuned_dpc_x100", _int), ("max_rel_mass_err", _d), ("seconds_cells", _d),
+                ("seconds_solve", _d), ("seconds_total", _d), ("ms_pcg_iteration", _d)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # name -> (restype, argtypes): every symbol the C header declares
 SIGNATURES = {
     "sdot_pd_create": (_int, [ctypes.POINTER(_p), _int, _int]),
@@ -81,6 +90,12 @@ SIGNATURES = {
     "sdot_pd_dirac_buffers": (_int, [_p, ctypes.POINTER(_p), ctypes.POINTER(_p)]),
     "sdot_pd_mark_diracs_changed": (_int, [_p, _int]),
     "sdot_pd_csr_device": (_int, [_p, ctypes.POINTER(_p), ctypes.POINTER(_p), ctypes.POINTER(_p)]),
+    "sdot_batch_create": (_int, [ctypes.POINTER(_p), _int, _int]),
+    "sdot_batch_destroy": (_int, [_p]),
+    "sdot_batch_last_error": (ctypes.c_char_p, [_p]),
+    "sdot_batch_set_problems": (_int, [_p, _i64, _p, _p, _p, _p, _p, _int]),
+    "sdot_batch_set_tuning": (_int, [_p, _int, _int]),
+    "sdot_batch_solve": (_int, [_p, _d, _int, _int, _p, _p, _p, _p, ctypes.POINTER(BatchStats)]),
 }
 
 _LIB = None
@@ -287,3 +302,72 @@ class OptimalTransport:
         self.stats = stats
         self.status = st
         return w
+
+
+class Batch:
+    """K independent 2D problems solved together (one launch per kernel over all
+    problems); per problem the rules of OptimalTransportSolver::solve
+    (Solvers/OptimalTransportSolver.tcc:27-119).  A problem's result does not
+    depend on the other problems of the batch."""
+
+    def __init__(self, precision="mixed", diracs_per_cell=0.0, use_graph=True):
+        h = _p()
+        st = lib().sdot_batch_create(ctypes.byref(h), 2, FP64 if precision == "fp64" else MIXED)
+        if st != SDOT_OK:
+            raise SdotError(st, "sdot_batch_create")
+        self._h = h
+        self.precision = precision
+        self.set_tuning(diracs_per_cell, use_graph)
+        self.offsets = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().sdot_batch_destroy(h)
+            self._h = None
+
+    def _check(self, st, what, ok=(SDOT_OK,)):
+        if st in ok:
+            return st
+        raise SdotError(st, f"{what}: {lib().sdot_batch_last_error(self._h).decode()}")
+
+    def set_tuning(self, diracs_per_cell=0.0, use_graph=True):
+        self._check(lib().sdot_batch_set_tuning(self._h, int(round(diracs_per_cell * 100)), int(bool(use_graph))),
+                    "set_tuning")
+
+    def set_problems(self, positions, weights=None, targets=None, boxes=None, ball=False):
+        """positions: list of (N_k, 2) arrays; weights/targets: lists of (N_k,) arrays
+        (default 0 and 1/N_k * box area); boxes: (K, 4) array of (x0, y0, x1, y1)."""
+        sizes = [len(p) for p in positions]
+        off = np.zeros(len(sizes) + 1, dtype=np.int64)
+        off[1:] = np.cumsum(sizes)
+        pos = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.float64).reshape(-1, 2) for p in positions]))
+        if weights is None:
+            w = np.zeros(off[-1])
+        else:
+            w = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in weights]))
+        bx = None if boxes is None else np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 4)
+        if targets is None:
+            area = np.ones(len(sizes)) if bx is None else (bx[:, 2] - bx[:, 0]) * (bx[:, 3] - bx[:, 1])
+            t = np.concatenate([np.full(n, a / n) for n, a in zip(sizes, area)])
+        else:
+            t = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in targets]))
+        self._keep = (pos, w, t, bx, off)
+        self._check(lib().sdot_batch_set_problems(self._h, len(sizes), _ptr(off), _ptr(pos), _ptr(w), _ptr(t),
+                                                  None if bx is None else _ptr(bx), int(bool(ball))), "set_problems")
+        self.offsets = off
+
+    def solve(self, tol=1e-6, stop_rule=0, max_iter=150):
+        """Returns (status, list of weights, list of masses, per-problem status, per-problem iterations, stats)."""
+        off = self.offsets
+        n, K = int(off[-1]), len(off) - 1
+        w = np.empty(n)
+        m = np.empty(n)
+        pst = np.empty(K, dtype=np.int32)
+        its = np.empty(K, dtype=np.int32)
+        stats = BatchStats()
+        st = lib().sdot_batch_solve(self._h, float(tol), int(stop_rule), int(max_iter), _ptr(w), _ptr(m), _ptr(pst),
+                                    _ptr(its), ctypes.byref(stats))
+        self._check(st, "solve", ok=(SDOT_OK, SDOT_NOT_CONVERGED))
+        split = lambda a: [a[off[k]:off[k + 1]].copy() for k in range(K)]
+        return st, split(w), split(m), pst, its, stats.as_dict()

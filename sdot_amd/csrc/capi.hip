@@ -97,7 +97,8 @@ sdot_status compute_pass(sdot_pd *pd) {
     HIPCK(hipEventRecord(pd->ev[1], s));
 
     const int64_t N = pd->N;
-    pd->ell_cap = pd->dim == 2 ? 32 : 96;
+    const int cap0 = pd->dim == 2 ? ELL_CAP2 : ELL_CAP3, cap_wide = pd->dim == 2 ? ELL_CAP2_WIDE : ELL_CAP3_WIDE;
+    if (pd->ell_cap < cap0) pd->ell_cap = cap0;
     pd->mass.ensure(N);
     pd->arc.ensure(N);
     pd->perim.ensure(N);
@@ -156,7 +157,15 @@ sdot_status compute_pass(sdot_pd *pd) {
     pd->last_flags = fl;
     pd->pass_valid = true;
     if (fl & FLAG_BAD_INPUT) throw StatusError{SDOT_INVALID_ARG, "non-finite weight or coincident diracs"};
-    if (fl & FLAG_ROW_OVERFLOW) throw StatusError{SDOT_CAPACITY_OVERFLOW, "Hessian row longer than the row capacity"};
+    if (fl & FLAG_CELL_OVERFLOW)
+        throw StatusError{SDOT_CAPACITY_OVERFLOW, "a Laguerre cell exceeds the polyhedron capacity (255 faces, 508 vertices)"};
+    if (fl & FLAG_ROW_OVERFLOW) {
+        if (pd->ell_cap < cap_wide) {   // rerun with wide rows (kept for the next passes)
+            pd->ell_cap = cap_wide;
+            return compute_pass(pd);
+        }
+        throw StatusError{SDOT_CAPACITY_OVERFLOW, "Hessian row longer than the row capacity"};
+    }
     return (fl & FLAG_EMPTY) ? SDOT_EMPTY_CELL : SDOT_OK;
 }
 }  // namespace sdot_amd

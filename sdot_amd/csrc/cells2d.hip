@@ -187,8 +187,12 @@ int64_t build_csr_device(sdot_pd *pd) {
         hipLaunchKernelGGL((k_csr_fill<33, 64>), dim3((unsigned)((N + 63) / 64)), dim3(64), 0, s, pd->row_len.p,
                            pd->ell_col.p, pd->ell_val.p, (int64_t)N, pd->diag.p, pd->perm.p, pd->inv_perm.p, N,
                            pd->csr_off.p, pd->csr_col.p, pd->csr_val.p);
-    else
+    else if (pd->ell_cap + 1 <= 97)
         hipLaunchKernelGGL((k_csr_fill<97, 32>), dim3((unsigned)((N + 31) / 32)), dim3(32), 0, s, pd->row_len.p,
+                           pd->ell_col.p, pd->ell_val.p, (int64_t)N, pd->diag.p, pd->perm.p, pd->inv_perm.p, N,
+                           pd->csr_off.p, pd->csr_col.p, pd->csr_val.p);
+    else
+        hipLaunchKernelGGL((k_csr_fill<256, 16>), dim3((unsigned)((N + 15) / 16)), dim3(16), 0, s, pd->row_len.p,
                            pd->ell_col.p, pd->ell_val.p, (int64_t)N, pd->diag.p, pd->perm.p, pd->inv_perm.p, N,
                            pd->csr_off.p, pd->csr_col.p, pd->csr_val.p);
     HIPCK(hipGetLastError());

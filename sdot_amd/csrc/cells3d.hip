@@ -8,17 +8,20 @@ namespace sdot_amd {
 
 namespace {
 
-constexpr int VCAP3 = 160;   // vertices per polyhedron
-constexpr int PCAP3 = 96;    // planes per polyhedron (indices fit in 8 bits)
+// capacity tiers of the thread kernel: cells that outgrow the first tier (or the
+// 64-lane wavefront kernel) are re-run on the second, whose plane indices still
+// fit the 8-bit fields of the vertex plane triples
+constexpr int VCAP3 = 160, PCAP3 = 96;      // tier 1: vertices / planes per polyhedron
+constexpr int VCAP3W = 508, PCAP3W = 255;   // tier 2
 constexpr int BS3 = 256;
 
 template <class T>
-size_t slot_bytes() {
-    return (size_t)VCAP3 * (7 * sizeof(T) + 2 * sizeof(int)) + (size_t)PCAP3 * (4 * sizeof(T) + sizeof(int) + 2 * sizeof(double));
+size_t slot_bytes(int vcap, int pcap) {
+    return (size_t)vcap * (7 * sizeof(T) + 2 * sizeof(int)) + (size_t)pcap * (4 * sizeof(T) + sizeof(int) + 2 * sizeof(double));
 }
 
 template <class T>
-__host__ __device__ c3::Poly3Mem<T> slot(char *base, int64_t S, int64_t s) {
+__host__ __device__ c3::Poly3Mem<T> slot(char *base, int64_t S, int64_t s, int vcap, int pcap) {
     // field-major layout: [field][k][slot]
     c3::Poly3Mem<T> m;
     char *p = base;
@@ -27,36 +30,45 @@ __host__ __device__ c3::Poly3Mem<T> slot(char *base, int64_t S, int64_t s) {
         dst = (P)(p) + s;
         p += elem * cap * S;
     };
-    take(m.px, sizeof(T), VCAP3);
-    take(m.py, sizeof(T), VCAP3);
-    take(m.pz, sizeof(T), VCAP3);
-    take(m.d, sizeof(T), VCAP3);
-    take(m.sx, sizeof(T), VCAP3);
-    take(m.sy, sizeof(T), VCAP3);
-    take(m.sz, sizeof(T), VCAP3);
-    take(m.tri, sizeof(int), VCAP3);
-    take(m.stri, sizeof(int), VCAP3);
-    take(m.nx, sizeof(T), PCAP3);
-    take(m.ny, sizeof(T), PCAP3);
-    take(m.nz, sizeof(T), PCAP3);
-    take(m.nh, sizeof(T), PCAP3);
-    take(m.pid, sizeof(int), PCAP3);
-    take(m.acc, sizeof(double), PCAP3);
-    take(m.hacc, sizeof(double), PCAP3);
+    take(m.px, sizeof(T), vcap);
+    take(m.py, sizeof(T), vcap);
+    take(m.pz, sizeof(T), vcap);
+    take(m.d, sizeof(T), vcap);
+    take(m.sx, sizeof(T), vcap);
+    take(m.sy, sizeof(T), vcap);
+    take(m.sz, sizeof(T), vcap);
+    take(m.tri, sizeof(int), vcap);
+    take(m.stri, sizeof(int), vcap);
+    take(m.nx, sizeof(T), pcap);
+    take(m.ny, sizeof(T), pcap);
+    take(m.nz, sizeof(T), pcap);
+    take(m.nh, sizeof(T), pcap);
+    take(m.pid, sizeof(int), pcap);
+    take(m.acc, sizeof(double), pcap);
+    take(m.hacc, sizeof(double), pcap);
     m.S = (int)S;
     return m;
 }
 
-template <class T>
+// LAST: overflowing cells raise FLAG_CELL_OVERFLOW; otherwise they are appended
+// to ovf_list (counter ovf_count) for the next tier
+template <class T, int VC, int PC, bool LAST>
 __global__ __launch_bounds__(BS3) void k_cells3_thread(const CellArgs *__restrict__ ap, char *scratch, int64_t nslots) {
     const CellArgs &a = *ap;
     const int64_t s = blockIdx.x * (int64_t)BS3 + threadIdx.x;
     if (s >= nslots) return;
-    const c3::Poly3Mem<T> m = slot<T>(scratch, nslots, s);
+    const c3::Poly3Mem<T> m = slot<T>(scratch, nslots, s, VC, PC);
     const int64_t count = a.work ? a.work_n : a.N;
     for (int64_t k = s; k < count; k += nslots) {
         const int64_t t = a.work ? a.work[k] : k;
-        if (c3::cell3_body<T>(a, t, m, VCAP3, PCAP3)) flag_or(a.flags, FLAG_ROW_OVERFLOW);
+        if (c3::cell3_body<T>(a, t, m, VC, PC)) {
+            if (LAST) {
+                flag_or(a.flags, FLAG_CELL_OVERFLOW);
+            } else {
+                const unsigned q = atomicAdd(a.ovf_count, 1u);
+                a.ovf_list[q] = (int)t;
+            }
+        }
     }
 }
 
@@ -70,7 +82,7 @@ const CellArgs *upload_args(sdot_pd *pd, const CellArgs &a, int slot) {
     return pd->dargs.p + slot;
 }
 
-template <class T>
+template <class T, int VC, int PC, bool LAST>
 void launch_thread3(sdot_pd *pd, const CellArgs &a) {
     int dev = 0, ncu = 0;
     HIPCK(hipGetDevice(&dev));
@@ -79,10 +91,10 @@ void launch_thread3(sdot_pd *pd, const CellArgs &a) {
     int64_t nslots = (int64_t)ncu * 4 * 64;     // four wavefronts per CU
     if (nslots > count) nslots = count;
     if (nslots < 1) return;
-    pd->scratch3.ensure(slot_bytes<T>() * nslots);
+    pd->scratch3.ensure(slot_bytes<T>(VC, PC) * nslots);
     const unsigned nb = (unsigned)((nslots + BS3 - 1) / BS3);
     const CellArgs *da = upload_args(pd, a, 1);
-    hipLaunchKernelGGL(k_cells3_thread<T>, dim3(nb), dim3(BS3), 0, pd->stream, da, pd->scratch3.p, nslots);
+    hipLaunchKernelGGL((k_cells3_thread<T, VC, PC, LAST>), dim3(nb), dim3(BS3), 0, pd->stream, da, pd->scratch3.p, nslots);
     HIPCK(hipGetLastError());
 }
 
@@ -706,15 +718,25 @@ void run_cells3(sdot_pd *pd, CellArgs &a) {
     HIPCK(hipMemcpyAsync(cnt, pd->flags.p, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
     pd->last.nb_cells_overflow = cnt[1];
-    if (cnt[1] > 0) {
-        // cells beyond 64 vertices / planes: the thread kernel with wide slots
-        int *list2 = pd->ovf_list.p + a.N + 1;
+    // cells beyond 64 vertices / planes: the thread kernel, tier 1 then tier 2; each
+    // tier works on a copy of the list while the next tier's list refills ovf_list
+    int *list2 = pd->ovf_list.p + a.N + 1;
+    for (int tier = 1; tier <= 2 && cnt[1] > 0; ++tier) {
         HIPCK(hipMemcpyAsync(list2, pd->ovf_list.p, cnt[1] * sizeof(int), hipMemcpyDeviceToDevice, s));
+        fill_u32(pd->flags.p + 1, 1, 0u, s);
         a.work = list2;
         a.work_n = cnt[1];
-        if (mixed) launch_thread3<float>(pd, a);
-        else launch_thread3<double>(pd, a);
+        if (tier == 1) {
+            if (mixed) launch_thread3<float, VCAP3, PCAP3, false>(pd, a);
+            else launch_thread3<double, VCAP3, PCAP3, false>(pd, a);
+        } else {
+            if (mixed) launch_thread3<float, VCAP3W, PCAP3W, true>(pd, a);
+            else launch_thread3<double, VCAP3W, PCAP3W, true>(pd, a);
+        }
+        HIPCK(hipMemcpyAsync(&cnt[1], pd->flags.p + 1, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
         a.work = nullptr;
+        if (tier == 1) pd->last.nb_cells_overflow += cnt[1];
     }
 }
 

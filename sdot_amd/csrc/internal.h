@@ -122,7 +122,11 @@ __host__ __device__ inline double dbl_unorder(unsigned long long b) {
     return v;
 }
 
-enum : unsigned { FLAG_EMPTY = 1u, FLAG_ROW_OVERFLOW = 2u, FLAG_BAD_INPUT = 4u };
+enum : unsigned { FLAG_EMPTY = 1u, FLAG_ROW_OVERFLOW = 2u, FLAG_BAD_INPUT = 4u, FLAG_CELL_OVERFLOW = 8u };
+
+// Hessian row capacity (ELL slots per row): the first pass uses the narrow one;
+// a row overflow re-runs the pass with the wide one, which the handle then keeps
+constexpr int ELL_CAP2 = 32, ELL_CAP2_WIDE = 64, ELL_CAP3 = 96, ELL_CAP3_WIDE = 255;
 
 #define SDOT_HD __host__ __device__
 

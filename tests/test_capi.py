@@ -49,6 +49,15 @@ def test_invalid_arguments_are_status_codes():
     assert L.sdot_pd_create(ctypes.byref(h), 2, 9) == sdot_amd.SDOT_INVALID_ARG
     assert L.sdot_pd_destroy(None) == sdot_amd.SDOT_INVALID_ARG
     assert L.sdot_pd_compute(None) == sdot_amd.SDOT_INVALID_ARG
+    # batches: 2D only; null handles are argument errors
+    assert L.sdot_batch_create(ctypes.byref(h), 3, 0) == sdot_amd.SDOT_NOT_SUPPORTED
+    assert L.sdot_batch_create(ctypes.byref(h), 2, 9) == sdot_amd.SDOT_INVALID_ARG
+    assert L.sdot_batch_create(None, 2, 0) == sdot_amd.SDOT_INVALID_ARG
+    assert L.sdot_batch_destroy(None) == sdot_amd.SDOT_INVALID_ARG
+    assert L.sdot_batch_set_tuning(None, 0, 1) == sdot_amd.SDOT_INVALID_ARG
+    assert L.sdot_batch_set_problems(None, 1, None, None, None, None, None, 0) == sdot_amd.SDOT_INVALID_ARG
+    assert L.sdot_batch_solve(None, 1e-6, 0, 10, None, None, None, None, None) == sdot_amd.SDOT_INVALID_ARG
+    assert L.sdot_batch_last_error(None) == b"null handle"
 
 
 def test_host_layer_header_compiles():

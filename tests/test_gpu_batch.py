@@ -1,0 +1,116 @@
+"""Batched 2D solves (sdot_batch_*): every problem of a batch against the
+oracle's own solve of that problem, and the determinism contract -- a problem's
+weights, masses and iteration count are bitwise the same alone, in a batch, in
+another order and with or without the captured PCG graph."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make_problems(K, seed0, lo=200, hi=3000):
+    from sdot_amd import workloads
+    sizes = workloads.batch_sizes(K, lo, hi, seed=seed0)
+    return [workloads.uniform_diracs(int(n), 2, seed=seed0 + 1 + k) for k, n in enumerate(sizes)]
+
+
+def solve(pos, precision="mixed", dpc=2.5, graph=True, **kw):
+    import sdot_amd
+    b = sdot_amd.Batch(precision, diracs_per_cell=dpc, use_graph=graph)
+    b.set_problems(pos, **{k: v for k, v in kw.items() if k in ("weights", "targets", "boxes", "ball")})
+    return b.solve(**{k: v for k, v in kw.items() if k in ("tol", "stop_rule", "max_iter")})
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_matches_oracle_per_problem(precision):
+    pos = make_problems(12, 100)
+    st, ws, ms, pst, its, stats = solve(pos, precision, tol=1e-6)
+    assert st == 0 and (pst == 0).all(), (pst, stats)
+    assert stats["nb_converged"] == len(pos) and stats["max_rel_mass_err"] < 1e-6
+    for k, p in enumerate(pos):
+        n = len(p)
+        t = np.full(n, 1.0 / n)
+        assert np.abs(ms[k] / t - 1).max() < 1e-6
+        m = O.cells(p, ws[k])["masses"]            # FP64 oracle masses at the batch weights
+        assert np.abs(m / t - 1).max() < 1.5e-6, k
+        sto, wo, so = O.ot_solve(p, np.zeros(n), t, tol=1e-6)
+        assert sto == 0
+        d1, d2 = ws[k] - ws[k].mean(), wo - wo.mean()
+        assert np.abs(d1 - d2).max() < 1e-3 * np.abs(d2).max(), k
+        assert abs(int(its[k]) - so.newton_iters) <= 2, (k, its[k], so.newton_iters)
+    print(f"[batch] {precision}: iters {list(its)} stats {stats}")
+
+
+def test_batch_result_independent_of_the_mix():
+    pos = make_problems(10, 200)
+    _, ws, ms, _, its, _ = solve(pos)
+    # reversed order: every problem moves to another index range
+    _, wr, mr, _, itr, _ = solve(pos[::-1])
+    for k in range(len(pos)):
+        j = len(pos) - 1 - k
+        assert np.array_equal(ws[k], wr[j]) and np.array_equal(ms[k], mr[j]) and its[k] == itr[j], k
+    # alone (a batch of one) and without the graph
+    for k in (0, 4, 9):
+        _, w1, m1, _, it1, _ = solve([pos[k]])
+        assert np.array_equal(w1[0], ws[k]) and np.array_equal(m1[0], ms[k]) and it1[0] == its[k], k
+    _, wg, mg, _, itg, stg = solve(pos, graph=False)
+    assert stg["graph_replays"] == 0
+    for k in range(len(pos)):
+        assert np.array_equal(wg[k], ws[k]) and np.array_equal(mg[k], ms[k]) and itg[k] == its[k], k
+
+
+def test_batch_boxes_and_ball_mode():
+    rng_pos = make_problems(4, 300, 300, 1500)
+    # problem 1 lives in the box [1,3] x [-1,1] (area 4)
+    pos = [p.copy() for p in rng_pos]
+    pos[1] = pos[1] * 2 + np.array([1.0, -1.0])
+    boxes = np.array([[0, 0, 1, 1], [1, -1, 3, 1], [0, 0, 1, 1], [0, 0, 1, 1]], dtype=np.float64)
+    st, ws, ms, pst, _, _ = solve(pos, "fp64", boxes=boxes, tol=1e-8)
+    assert st == 0 and (pst == 0).all()
+    for k, p in enumerate(pos):
+        n = len(p)
+        area = (boxes[k, 2] - boxes[k, 0]) * (boxes[k, 3] - boxes[k, 1])
+        m = O.cells(p, ws[k], domain=(boxes[k, :2], boxes[k, 2:]))["masses"]
+        assert np.abs(m / (area / n) - 1).max() < 1e-7, k
+    # ball mode: half of the unit box covered by the discs
+    pos = rng_pos
+    t = [np.full(len(p), 0.5 / len(p)) for p in pos]
+    w0 = [np.full(len(p), 0.6 / len(p) / np.pi) for p in pos]
+    st, ws, ms, pst, _, _ = solve(pos, "mixed", ball=True, targets=t, weights=w0, tol=1e-8)
+    assert st == 0 and (pst == 0).all()
+    for k, p in enumerate(pos):
+        m = O.cells(p, ws[k], ball=True)["masses"]
+        assert np.abs(m / t[k] - 1).max() < 1e-7, k
+
+
+def test_batch_stop_rule_dw_and_autotune():
+    import sdot_amd
+    pos = make_problems(6, 400)
+    st, ws, ms, pst, its, stats = solve(pos, "fp64", dpc=0.0, tol=1e-6, stop_rule=1)
+    assert st == 0 and (pst == 0).all()
+    assert stats["tuned_dpc_x100"] in (150, 250, 400)
+    # same stop rule on the single-problem path: same number of Newton passes
+    for k in (0, 5):
+        n = len(pos[k])
+        pd = sdot_amd.PowerDiagram(2, "fp64")
+        pd.add_box_shape([0, 0], [1, 1])
+        pd.set_diracs(pos[k], np.zeros(n))
+        s1, w1, st1 = pd.ot_solve(np.full(n, 1.0 / n), np.zeros(n), tol=1e-6, stop_rule=1)
+        assert s1 == 0 and st1["newton_iters"] == its[k], (k, st1, its[k])
+        d1, d2 = ws[k] - ws[k].mean(), w1 - w1.mean()
+        assert np.abs(d1 - d2).max() < 1e-6 * np.abs(d2).max()
+
+
+def test_batch_not_converged_and_reuse():
+    pos = make_problems(3, 500)
+    st, ws, ms, pst, its, stats = solve(pos, tol=1e-12, max_iter=2)
+    assert st == 2 and (pst == 2).all() and (its == 2).all()
+    import sdot_amd
+    b = sdot_amd.Batch("mixed", diracs_per_cell=2.5)
+    b.set_problems(pos)
+    r1 = b.solve(tol=1e-6)
+    r2 = b.solve(tol=1e-6)          # same handle again: same result
+    for k in range(3):
+        assert np.array_equal(r1[1][k], r2[1][k])
