@@ -196,10 +196,117 @@ def main_c4(args):
     print(json.dumps(line))
 
 
+def c5_schedule(rate, duration, seed=7):
+    """Poisson arrivals (rate/s over `duration` s), 70 % 2D / 30 % 3D, N = floor(exp(U[ln 1e3, ln 2e6]))."""
+    import sdot_amd
+    m = int(rate * duration * 2 + 64)
+    u = sdot_amd.rng_uniform(seed, 3 * m).reshape(m, 3)
+    t = np.cumsum(-np.log1p(-u[:, 0]) / rate)
+    keep = t < duration
+    dims = np.where(u[:, 1] < 0.7, 2, 3)
+    sizes = np.floor(np.exp(np.log(1e3) + u[:, 2] * (np.log(2e6) - np.log(1e3)))).astype(np.int64)
+    return t[keep], dims[keep], sizes[keep]
+
+
+def c5_run(rate, duration, deadline, precision, seed=7, label=""):
+    """Open-loop load: submit each request at its arrival time, collect until `deadline` s."""
+    import threading
+    import sdot_amd
+    from sdot_amd import workloads
+    t_arr, dims, sizes = c5_schedule(rate, duration, seed)
+    srv = sdot_amd.Server(precision, diracs_per_cell=2.5, workers_3d=2, tol=1e-6)
+    ids = [None] * len(t_arr)
+    done_submit = threading.Event()
+    t0 = time.perf_counter()
+
+    def submitter():
+        for k in range(len(t_arr)):
+            dt = t_arr[k] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            if time.perf_counter() - t0 > deadline:
+                break
+            pos = workloads.uniform_diracs(int(sizes[k]), int(dims[k]), seed=10_000 + k)
+            ids[k] = srv.submit(pos)
+        done_submit.set()
+
+    th = threading.Thread(target=submitter, daemon=True)
+    th.start()
+    res = {}
+    last = time.perf_counter()
+    k = 0
+    while k < len(t_arr):
+        if ids[k] is None:
+            if done_submit.is_set():
+                break
+            time.sleep(0.005)
+            continue
+        left = deadline - (time.perf_counter() - t0)
+        if left <= 0:
+            break
+        r = srv.wait(ids[k], timeout=min(left, 5.0))
+        if r is None:
+            if time.perf_counter() - last > 20:
+                print(f"# c5{label}: waiting on request {k}, stats {srv.stats()}", file=sys.stderr, flush=True)
+                last = time.perf_counter()
+            continue
+        res[k] = r[1]
+        k += 1
+        if time.perf_counter() - last > 20:
+            print(f"# c5{label}: {len(res)} done, stats {srv.stats()}", file=sys.stderr, flush=True)
+            last = time.perf_counter()
+    wall = time.perf_counter() - t0
+    th.join(timeout=1.0)
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res.values()])
+    ok = sum(1 for r in res.values() if r["status"] == 0)
+    submitted = sum(1 for i in ids if i is not None)
+    cells = st["cells_evaluated"]
+    out = {"rate_per_s": rate, "arrival_window_s": duration, "offered": int(len(t_arr)), "submitted": submitted,
+           "completed": len(res), "converged": ok, "wall_s": wall,
+           "throughput_req_per_s": len(res) / wall, "cells_per_s": cells / wall,
+           "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+           "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+           "max_s": float(lat.max()) if len(lat) else None,
+           "backlog_at_end": submitted - len(res), "server": st,
+           "offered_diracs": int(sizes.sum()), "completed_diracs": int(sum(sizes[i] for i in res))}
+    print(f"# c5{label}: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def main_c5(args):
+    """Serving loop.  Phase 1: the offered load (Poisson 200/s, 70 % 2D / 30 % 3D,
+    N log-uniform in [1e3, 2e6]) for a bounded arrival window; the queue grows
+    without bound at this rate, so completed requests, throughput and backlog
+    are reported.  Phase 2: a sustainable rate (half the phase-1 throughput),
+    every request served; p50/p99 time-to-converged-weights."""
+    over = c5_run(args.rate, args.window, args.window + args.drain, "mixed", label=" offered")
+    sus_rate = max(0.2, 0.5 * over["throughput_req_per_s"])
+    sus = c5_run(sus_rate, args.sustain, args.sustain + 240.0, "mixed", seed=8, label=" sustainable")
+    line = {
+        "metric": "p99 latency s (serving loop, time to converged weights @1e-6)",
+        "value": sus["p99_s"], "unit": "s", "n_gpus": 1, "steps": sus["completed"], "warmup": 0,
+        "ms_per_step": (sus["p50_s"] or 0) * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-clip/f64-solve", "data": "synthetic (xoshiro256** seeds 7/8, 10000+k)",
+        "config": {"workload": "serving loop: Poisson arrivals, 70 % 2D / 30 % 3D, N = floor(exp(U[ln 1e3, ln 2e6])), "
+                               "unit box, uniform targets, w0 = 0, stop at max relative mass error 1e-6; "
+                               f"value = p99 at a sustainable {sus_rate:.2f} req/s",
+                   "precision": "mixed"},
+        "sustainable": sus, "offered_load": over,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2")
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--window", type=float, default=10.0)
+    ap.add_argument("--drain", type=float, default=50.0)
+    ap.add_argument("--sustain", type=float, default=30.0)
     ap.add_argument("--problems", type=int, default=512)
     ap.add_argument("--img", type=int, default=64)
     ap.add_argument("--steps", type=int, default=20)
@@ -219,6 +326,8 @@ def main():
         args.steps = min(args.steps, 5)
         args.warmup = max(1, min(args.warmup, 2))
         return main_c4(args)
+    if args.workload == "c5":
+        return main_c5(args)
 
     pos, w = make_inputs(args.n)
     res = {}

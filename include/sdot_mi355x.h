@@ -38,7 +38,8 @@ typedef enum sdot_status {
     SDOT_INVALID_ARG = 4,
     SDOT_HIP_ERROR = 5,
     SDOT_OOM = 6,
-    SDOT_NOT_SUPPORTED = 7
+    SDOT_NOT_SUPPORTED = 7,
+    SDOT_PENDING = 8             /* serving loop: the request has not finished yet */
 } sdot_status;
 
 typedef enum sdot_precision {
@@ -107,7 +108,7 @@ typedef struct sdot_solve_stats {
     int newton_iters;
     int pcg_iters_total;
     int backtracks;
-    int reserved;
+    int precond_levels;          /* multigrid levels of the last linear solve (0: Jacobi) */
     double max_rel_mass_err;     /* at the returned weights */
     double max_abs_dw;           /* last Newton step */
     int64_t nnz;
@@ -122,6 +123,12 @@ typedef struct sdot_solve_stats {
  * weights (dirac order) are read as the start and overwritten with the result. */
 sdot_status sdot_ot_solve(sdot_pd *pd, const double *target_masses, double tol, int stop_rule, int max_iter,
                           double *weights_inout, sdot_solve_stats *stats);
+
+/* Preconditioner of the linear solves (the reference's EigenSolver is a sparse Cholesky,
+ * EigenSolver.cpp:28-29; its optional AmgclSolver, AmgclSolver.cpp:19-37, an AMG-preconditioned
+ * Krylov solver): aggregation multigrid V-cycle (default) or Jacobi. */
+typedef enum sdot_precond { SDOT_PRECOND_AMG = 0, SDOT_PRECOND_JACOBI = 1 } sdot_precond;
+sdot_status sdot_pd_set_linear_solver(sdot_pd *pd, int precond);
 
 /* Tuning knobs (0 = automatic): mean diracs per grid cell (x100), threads per block. */
 sdot_status sdot_pd_set_tuning(sdot_pd *pd, int diracs_per_cell_x100, int block_size);
@@ -166,6 +173,8 @@ typedef struct sdot_batch_stats {
     double seconds_solve;        /* wall time of the batched PCG solves */
     double seconds_total;        /* wall time of the call */
     double ms_pcg_iteration;     /* device time of one captured PCG iteration (all problems) */
+    int precond_levels;          /* deepest multigrid hierarchy (0: Jacobi) */
+    int precond_fallbacks;       /* problem solves whose hierarchy could not be built (Jacobi used) */
 } sdot_batch_stats;
 
 /* dim must be 2 (3 returns SDOT_NOT_SUPPORTED). */
@@ -180,6 +189,8 @@ sdot_status sdot_batch_set_problems(sdot_batch *b, int64_t nb_problems, const in
 /* diracs_per_cell_x100: grid resolution (0 = autotune on the first solve of a size
  * class, then cached); use_graph: replay the PCG iterations from a captured hipGraph. */
 sdot_status sdot_batch_set_tuning(sdot_batch *b, int diracs_per_cell_x100, int use_graph);
+/* as sdot_pd_set_linear_solver */
+sdot_status sdot_batch_set_linear_solver(sdot_batch *b, int precond);
 /* Newton on every problem, starting from the weights given to set_problems; stop_rule
  * as sdot_ot_solve; the autotuning passes (first solve of a size class) use the start
  * weights.  weights_out (N), masses_out
@@ -187,6 +198,66 @@ sdot_status sdot_batch_set_tuning(sdot_batch *b, int diracs_per_cell_x100, int u
  * Returns SDOT_OK when every problem converged, SDOT_NOT_CONVERGED otherwise. */
 sdot_status sdot_batch_solve(sdot_batch *b, double tol, int stop_rule, int max_iter, double *weights_out,
                              double *masses_out, int *status_out, int *iters_out, sdot_batch_stats *stats);
+
+/* ---------------------------------------------------------------------------
+ * Serving loop: independent OT solve requests (2D and 3D, unit box or a given
+ * box, FunctionEnum::Unit, density 1) submitted at any time and answered with
+ * their converged weights.  No counterpart in the reference, whose solver runs
+ * one problem per call (OptimalTransportSolver.tcc:27-119).
+ *  - 2D requests join one continuous batch (the batched solve above) at the next
+ *    Newton-iteration boundary and leave it when they converge;
+ *  - 3D requests run one per worker, each worker with its own device handle and
+ *    stream, concurrently with the 2D batch;
+ *  - admission by HBM footprint: a request waits in its lane's FIFO queue until
+ *    its estimated device footprint fits the budget beside the resident ones;
+ *  - a request's weights do not depend on what else is being served (bitwise),
+ *    given the same configuration.
+ * ------------------------------------------------------------------------- */
+typedef struct sdot_srv sdot_srv;
+
+typedef struct sdot_srv_config {
+    int precision;               /* SDOT_FP64 / SDOT_MIXED */
+    double hbm_budget_bytes;     /* admission limit for resident requests (0: 80 % of free HBM at creation) */
+    int max_batch_problems;      /* 2D requests advanced together (0: 4096) */
+    int workers_3d;              /* concurrent 3D solves (0: 2) */
+    int diracs_per_cell_x100;    /* 2D grid resolution (0: 250) */
+    double tol;                  /* stop tolerance (0: 1e-6) */
+    int stop_rule;               /* as sdot_ot_solve */
+    int max_iter;                /* Newton iterations per request (0: 150) */
+} sdot_srv_config;
+
+typedef struct sdot_srv_result {
+    int status;                  /* SDOT_OK / SDOT_NOT_CONVERGED / an error of the solve */
+    int newton_iters;
+    double max_rel_mass_err;
+    double t_submit, t_start, t_done;   /* seconds on the server's clock */
+} sdot_srv_result;
+
+typedef struct sdot_srv_stats {
+    int64_t queued_2d, queued_3d, running, completed;
+    double resident_bytes, peak_resident_bytes, budget_bytes;
+    int64_t cells_evaluated;     /* cell evaluations of every pass (2D ticks and 3D passes) */
+    int64_t ticks_2d;            /* Newton iterations of the 2D batch */
+    double seconds;              /* server clock */
+} sdot_srv_stats;
+
+sdot_status sdot_srv_create(sdot_srv **srv, const sdot_srv_config *cfg);
+/* Copies the inputs and returns at once.  positions n*dim interleaved; weights
+ * (start, may be NULL = 0); targets (n, > 0); box: lo[dim], hi[dim] or NULL = unit box. */
+sdot_status sdot_srv_submit(sdot_srv *srv, int dim, int64_t n, const double *positions, const double *weights,
+                            const double *targets, const double *box, int64_t *request_id);
+/* state: 0 queued, 1 running, 2 finished */
+sdot_status sdot_srv_poll(sdot_srv *srv, int64_t request_id, int *state);
+/* Waits until the request finishes (timeout_s < 0: no limit); then copies its weights
+ * (n, dirac order; may be NULL) and result, and forgets the request.  Returns
+ * SDOT_PENDING on a timeout, SDOT_INVALID_ARG for an unknown id, SDOT_OK otherwise
+ * (the solve's own status is result->status). */
+sdot_status sdot_srv_wait(sdot_srv *srv, int64_t request_id, double timeout_s, double *weights_out,
+                          sdot_srv_result *result);
+sdot_status sdot_srv_get_stats(sdot_srv *srv, sdot_srv_stats *stats);
+const char *sdot_srv_last_error(const sdot_srv *srv);
+/* Stops the lanes after their current step; unfinished requests are dropped. */
+sdot_status sdot_srv_destroy(sdot_srv *srv);
 
 /* splitmix64 -> xoshiro256** uniform doubles (x >> 11) * 2^-53: the synthetic-input generator. */
 void sdot_rng_uniform(uint64_t seed, int64_t n, double *out);

@@ -29,6 +29,10 @@ SDOT_INVALID_ARG = 4
 SDOT_HIP_ERROR = 5
 SDOT_OOM = 6
 SDOT_NOT_SUPPORTED = 7
+SDOT_PENDING = 8
+
+PRECOND_AMG = 0
+PRECOND_JACOBI = 1
 
 FP64 = 0
 MIXED = 1
@@ -40,12 +44,12 @@ _int = ctypes.c_int
 
 
 class SolveStats(ctypes.Structure):
-    _fields_ = [("newton_iters", _int), ("pcg_iters_total", _int), ("backtracks", _int), ("reserved", _int),
+    _fields_ = [("newton_iters", _int), ("pcg_iters_total", _int), ("backtracks", _int), ("precond_levels", _int),
                 ("max_rel_mass_err", _d), ("max_abs_dw", _d), ("nnz", _i64), ("seconds_cells", _d),
                 ("seconds_solve", _d), ("seconds_total", _d)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Timings(ctypes.Structure):
@@ -60,7 +64,30 @@ class Timings(ctypes.Structure):
 class BatchStats(ctypes.Structure):
     _fields_ = [("newton_iters_max", _int), ("nb_converged", _int), ("pcg_iters_total", _int), ("backtracks", _int),
                 ("graph_replays", _int), ("tuned_dpc_x100", _int), ("max_rel_mass_err", _d), ("seconds_cells", _d),
-                ("seconds_solve", _d), ("seconds_total", _d), ("ms_pcg_iteration", _d)]
+                ("seconds_solve", _d), ("seconds_total", _d), ("ms_pcg_iteration", _d), ("precond_levels", _int),
+                ("precond_fallbacks", _int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("precision", _int), ("hbm_budget_bytes", _d), ("max_batch_problems", _int), ("workers_3d", _int),
+                ("diracs_per_cell_x100", _int), ("tol", _d), ("stop_rule", _int), ("max_iter", _int)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("status", _int), ("newton_iters", _int), ("max_rel_mass_err", _d), ("t_submit", _d),
+                ("t_start", _d), ("t_done", _d)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("queued_2d", _i64), ("queued_3d", _i64), ("running", _i64), ("completed", _i64),
+                ("resident_bytes", _d), ("peak_resident_bytes", _d), ("budget_bytes", _d),
+                ("cells_evaluated", _i64), ("ticks_2d", _i64), ("seconds", _d)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -85,6 +112,8 @@ SIGNATURES = {
     "sdot_pd_der_integrals_wrt_weights": (_int, [_p, _p, _p, _p, _p, _i64, ctypes.POINTER(_i64)]),
     "sdot_ot_solve": (_int, [_p, _p, _d, _int, _int, _p, ctypes.POINTER(SolveStats)]),
     "sdot_pd_set_tuning": (_int, [_p, _int, _int]),
+    "sdot_pd_set_linear_solver": (_int, [_p, _int]),
+    "sdot_batch_set_linear_solver": (_int, [_p, _int]),
     "sdot_pd_last_timings": (_int, [_p, ctypes.POINTER(Timings)]),
     "sdot_rng_uniform": (None, [ctypes.c_uint64, _i64, _p]),
     "sdot_pd_dirac_buffers": (_int, [_p, ctypes.POINTER(_p), ctypes.POINTER(_p)]),
@@ -96,6 +125,13 @@ SIGNATURES = {
     "sdot_batch_set_problems": (_int, [_p, _i64, _p, _p, _p, _p, _p, _int]),
     "sdot_batch_set_tuning": (_int, [_p, _int, _int]),
     "sdot_batch_solve": (_int, [_p, _d, _int, _int, _p, _p, _p, _p, ctypes.POINTER(BatchStats)]),
+    "sdot_srv_create": (_int, [ctypes.POINTER(_p), ctypes.POINTER(ServerConfig)]),
+    "sdot_srv_submit": (_int, [_p, _int, _i64, _p, _p, _p, _p, ctypes.POINTER(_i64)]),
+    "sdot_srv_poll": (_int, [_p, _i64, ctypes.POINTER(_int)]),
+    "sdot_srv_wait": (_int, [_p, _i64, _d, _p, ctypes.POINTER(ServerResult)]),
+    "sdot_srv_get_stats": (_int, [_p, ctypes.POINTER(ServerStats)]),
+    "sdot_srv_last_error": (ctypes.c_char_p, [_p]),
+    "sdot_srv_destroy": (_int, [_p]),
 }
 
 _LIB = None
@@ -215,6 +251,11 @@ class PowerDiagram:
     def set_tuning(self, diracs_per_cell=0.0, block=0):
         self._check(lib().sdot_pd_set_tuning(self._h, int(round(diracs_per_cell * 100)), int(block)), "set_tuning")
 
+    def set_linear_solver(self, precond="amg"):
+        """'amg' (aggregation multigrid V-cycle, default) or 'jacobi' preconditioned CG."""
+        code = PRECOND_AMG if precond == "amg" else PRECOND_JACOBI
+        self._check(lib().sdot_pd_set_linear_solver(self._h, code), "set_linear_solver")
+
     # -- evaluation --
     def compute(self, allow_empty=True):
         return self._check(lib().sdot_pd_compute(self._h), "compute", allow_empty)
@@ -310,7 +351,7 @@ class Batch:
     (Solvers/OptimalTransportSolver.tcc:27-119).  A problem's result does not
     depend on the other problems of the batch."""
 
-    def __init__(self, precision="mixed", diracs_per_cell=0.0, use_graph=True):
+    def __init__(self, precision="mixed", diracs_per_cell=0.0, use_graph=True, precond="amg"):
         h = _p()
         st = lib().sdot_batch_create(ctypes.byref(h), 2, FP64 if precision == "fp64" else MIXED)
         if st != SDOT_OK:
@@ -318,6 +359,8 @@ class Batch:
         self._h = h
         self.precision = precision
         self.set_tuning(diracs_per_cell, use_graph)
+        code = PRECOND_AMG if precond == "amg" else PRECOND_JACOBI
+        self._check(lib().sdot_batch_set_linear_solver(self._h, code), "set_linear_solver")
         self.offsets = None
 
     def __del__(self):
@@ -371,3 +414,72 @@ class Batch:
         self._check(st, "solve", ok=(SDOT_OK, SDOT_NOT_CONVERGED))
         split = lambda a: [a[off[k]:off[k + 1]].copy() for k in range(K)]
         return st, split(w), split(m), pst, its, stats.as_dict()
+
+
+class Server:
+    """Serving loop: submit independent OT solve requests (2D or 3D), collect
+    their converged weights.  2D requests share one continuous batch; 3D
+    requests run on `workers_3d` concurrent device handles."""
+
+    def __init__(self, precision="mixed", hbm_budget_bytes=0.0, max_batch_problems=0, workers_3d=0,
+                 diracs_per_cell=0.0, tol=1e-6, stop_rule=0, max_iter=150):
+        cfg = ServerConfig(FP64 if precision == "fp64" else MIXED, float(hbm_budget_bytes), int(max_batch_problems),
+                           int(workers_3d), int(round(diracs_per_cell * 100)), float(tol), int(stop_rule),
+                           int(max_iter))
+        h = _p()
+        st = lib().sdot_srv_create(ctypes.byref(h), ctypes.byref(cfg))
+        if st != SDOT_OK:
+            raise SdotError(st, "sdot_srv_create")
+        self._h = h
+        self._n = {}
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().sdot_srv_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, positions, targets=None, weights=None, box=None):
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        n, dim = pos.shape
+        t = np.full(n, 1.0 / n) if targets is None else np.ascontiguousarray(targets, dtype=np.float64)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        b = None if box is None else np.ascontiguousarray(box, dtype=np.float64)
+        rid = _i64()
+        st = lib().sdot_srv_submit(self._h, dim, n, _ptr(pos), None if w is None else _ptr(w), _ptr(t),
+                                   None if b is None else _ptr(b), ctypes.byref(rid))
+        if st != SDOT_OK:
+            raise SdotError(st, "sdot_srv_submit")
+        self._n[rid.value] = n
+        return rid.value
+
+    def poll(self, rid):
+        state = _int()
+        st = lib().sdot_srv_poll(self._h, int(rid), ctypes.byref(state))
+        if st != SDOT_OK:
+            raise SdotError(st, "sdot_srv_poll")
+        return state.value
+
+    def wait(self, rid, timeout=-1.0):
+        """Returns (weights, result dict), or None on a timeout."""
+        n = self._n[rid]
+        w = np.empty(n)
+        res = ServerResult()
+        st = lib().sdot_srv_wait(self._h, int(rid), float(timeout), _ptr(w), ctypes.byref(res))
+        if st == SDOT_PENDING:
+            return None
+        if st != SDOT_OK:
+            raise SdotError(st, "sdot_srv_wait")
+        del self._n[rid]
+        return w, res.as_dict()
+
+    def stats(self):
+        s = ServerStats()
+        lib().sdot_srv_get_stats(self._h, ctypes.byref(s))
+        return s.as_dict()
+
+    def last_error(self):
+        return lib().sdot_srv_last_error(self._h).decode()

@@ -26,40 +26,14 @@
 #include <mutex>
 #include <new>
 
-#include "internal.h"
+#include "batch_engine.h"
+#include "pcg.h"
 
 using namespace sdot_amd;
-
-struct sdot_batch {
-    sdot_pd *pd = nullptr;      // holds the concatenated dirac arrays, the pass outputs and the stream
-    int ball = 0;
-    int64_t K = 0;
-    int use_graph = 1;
-    int dpc_x100 = 0;           // 0: autotune on the first solve of a size class
-    std::vector<int64_t> off;   // problem offsets (K+1)
-    std::vector<double> hpos;   // host copy of the positions (per-problem bounding boxes)
-    std::vector<GridInfo> grids;
-    std::vector<int64_t> code_base;
-    std::vector<int64_t> blk_off;
-    int64_t nblk = 0;           // problem-aligned row blocks
-    int grid_dpc = -1;          // resolution of the current grid (-1: none)
-    DevBuf<int> prob, blk_prob, pdone, act;
-    DevBuf<int64_t> dcode_base, doff, dblk;
-    DevBuf<GridInfo> dgrids;
-    DevBuf<double> pdom, tie, w0, tgt, tsort, old, b, red, S, perr;
-    DevBuf<double> x, r, z, p, q;
-    DevBuf<unsigned> pflags;
-    DevBuf<unsigned long long> wglob;
-    hipGraphExec_t graph = nullptr;
-    hipEvent_t gev[2] = {};
-    std::string err;
-};
 
 namespace {
 
 constexpr int BB = 256;         // rows per block
-constexpr int CHUNK = 8;        // PCG iterations per graph launch
-enum { S_RZ = 0, S_BB, S_ALPHA, S_BETA, S_RR, S_DONE, S_IT, S_RTOL2, S_NS };
 enum { ACT_NONE = 0, ACT_ACCEPT = 1, ACT_BACKTRACK = 2 };
 
 __device__ inline double bsum(double v, double *sm) {
@@ -84,14 +58,6 @@ __device__ inline double bmax(double v, double *sm) {
     __syncthreads();
     return r;
 }
-
-struct BMat {                   // ELL Hessian rows in grid order (column-major slots)
-    const double *diag;
-    const int *len;
-    const int *col;
-    const double *val;
-    int64_t stride;
-};
 
 struct Blocks {                 // problem-aligned row blocks
     const int *blk_prob;
@@ -228,188 +194,258 @@ __global__ void kb_export(const double *sorted, const int *inv_perm, int64_t n, 
 }
 
 // ---- PCG on problem-aligned blocks --------------------------------------------
-// S[p * S_NS + k]: per-problem scalars; S_DONE != 0 freezes the problem.
-__global__ __launch_bounds__(BB) void kb_init(BMat A, Blocks B, const double *b, double *x, double *r, double *z,
-                                              double *p, double *part, const double *S) {
-    __shared__ double sm[BB];
-    const BlockRows br = block_rows(B, blockIdx.x);
-    double rz = 0, bb = 0;
-    if (S[br.p * S_NS + S_DONE] == 0)
-        for (int64_t t = br.r0 + threadIdx.x; t < br.r1; t += BB) {
-            const double d = A.diag[t], rv = b[t], zv = d != 0 ? rv / d : rv;
-            x[t] = 0;
-            r[t] = rv;
-            z[t] = zv;
-            p[t] = zv;
-            rz += rv * zv;
-            bb += rv * rv;
-        }
-    rz = bsum(rz, sm);
-    bb = bsum(bb, sm);
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = rz;
-        part[2 * blockIdx.x + 1] = bb;
-    }
+// relayout: kept problems move from their old offset to the new one (dirac order)
+__global__ __launch_bounds__(BB) void kb_move(const int *newprob, int64_t n, const int64_t *src_off,
+                                              const int64_t *dst_off, const double *pos, const double *w,
+                                              const double *old, const double *w0, const double *tgt, double *pos2,
+                                              double *w2, double *old2, double *w02, double *tgt2) {
+    const int64_t i = blockIdx.x * (int64_t)BB + threadIdx.x;
+    if (i >= n) return;
+    const int p = newprob[i];
+    const int64_t so = src_off[p];
+    if (so < 0) return;   // uploaded from the host
+    const int64_t j = so + (i - dst_off[p]);
+    pos2[2 * i] = pos[2 * j];
+    pos2[2 * i + 1] = pos[2 * j + 1];
+    w2[i] = w[j];
+    old2[i] = old[j];
+    w02[i] = w0[j];
+    tgt2[i] = tgt[j];
 }
 
-// rtol per problem: max(1e-14, min(1e-3, 1e-3 * relative mass error))
-__global__ __launch_bounds__(BB) void kb_init_red(const double *part, const int64_t *blk_off, const double *perr,
-                                                  double *S) {
-    __shared__ double sm[BB];
-    const int p = blockIdx.x;
-    double *sp = S + p * S_NS;
-    if (sp[S_DONE] != 0) return;
-    double a = 0, c = 0;
-    for (int64_t k = blk_off[p] + threadIdx.x; k < blk_off[p + 1]; k += BB) {
-        a += part[2 * k];
-        c += part[2 * k + 1];
+// idx[i] = the k with off[k] <= i < off[k+1] (binary search over K+1 offsets)
+__global__ __launch_bounds__(BB) void kb_owner(const int64_t *off, int64_t K, int64_t n, int *idx) {
+    const int64_t i = blockIdx.x * (int64_t)BB + threadIdx.x;
+    if (i >= n) return;
+    int64_t lo = 0, hi = K;   // off[lo] <= i < off[hi]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) / 2;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
     }
-    a = bsum(a, sm);
-    c = bsum(c, sm);
-    if (threadIdx.x == 0) {
-        const double rtol = fmax(1e-14, fmin(1e-3, 1e-3 * perr[p]));
-        sp[S_RZ] = a;
-        sp[S_BB] = c;
-        sp[S_RR] = c;
-        sp[S_RTOL2] = rtol * rtol;
-        sp[S_IT] = 0;
-        sp[S_DONE] = (c == 0 || !(a > 0)) ? 1.0 : 0.0;
-    }
-}
-
-__global__ __launch_bounds__(BB) void kb_spmv_dot(BMat A, Blocks B, const double *p, double *q, double *part,
-                                                  const double *S) {
-    __shared__ double sm[BB];
-    const BlockRows br = block_rows(B, blockIdx.x);
-    if (S[br.p * S_NS + S_DONE] != 0) return;   // uniform per block
-    double pq = 0;
-    for (int64_t t = br.r0 + threadIdx.x; t < br.r1; t += BB) {
-        double v = A.diag[t] * p[t];
-        const int L = A.len[t];
-        for (int e = 0; e < L; ++e) v += A.val[e * A.stride + t] * p[A.col[e * A.stride + t]];
-        q[t] = v;
-        pq += p[t] * v;
-    }
-    pq = bsum(pq, sm);
-    if (threadIdx.x == 0) part[2 * blockIdx.x] = pq;
-}
-
-__global__ __launch_bounds__(BB) void kb_alpha(const double *part, const int64_t *blk_off, double *S) {
-    __shared__ double sm[BB];
-    const int p = blockIdx.x;
-    double *sp = S + p * S_NS;
-    if (sp[S_DONE] != 0) return;
-    double a = 0;
-    for (int64_t k = blk_off[p] + threadIdx.x; k < blk_off[p + 1]; k += BB) a += part[2 * k];
-    a = bsum(a, sm);
-    if (threadIdx.x == 0) sp[S_ALPHA] = a != 0 ? sp[S_RZ] / a : 0.0;
-}
-
-__global__ __launch_bounds__(BB) void kb_axpy(BMat A, Blocks B, const double *p, const double *q, double *x, double *r,
-                                              double *z, double *part, const double *S) {
-    __shared__ double sm[BB];
-    const BlockRows br = block_rows(B, blockIdx.x);
-    const double *sp = S + br.p * S_NS;
-    if (sp[S_DONE] != 0) return;
-    const double alpha = sp[S_ALPHA];
-    double rz = 0, rr = 0;
-    for (int64_t t = br.r0 + threadIdx.x; t < br.r1; t += BB) {
-        x[t] += alpha * p[t];
-        const double rv = r[t] - alpha * q[t];
-        r[t] = rv;
-        const double d = A.diag[t], zv = d != 0 ? rv / d : rv;
-        z[t] = zv;
-        rz += rv * zv;
-        rr += rv * rv;
-    }
-    rz = bsum(rz, sm);
-    rr = bsum(rr, sm);
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = rz;
-        part[2 * blockIdx.x + 1] = rr;
-    }
-}
-
-__global__ __launch_bounds__(BB) void kb_beta(const double *part, const int64_t *blk_off, double *S, int max_iter) {
-    __shared__ double sm[BB];
-    const int p = blockIdx.x;
-    double *sp = S + p * S_NS;
-    if (sp[S_DONE] != 0) return;
-    double a = 0, c = 0;
-    for (int64_t k = blk_off[p] + threadIdx.x; k < blk_off[p + 1]; k += BB) {
-        a += part[2 * k];
-        c += part[2 * k + 1];
-    }
-    a = bsum(a, sm);
-    c = bsum(c, sm);
-    if (threadIdx.x == 0) {
-        sp[S_BETA] = sp[S_RZ] != 0 ? a / sp[S_RZ] : 0.0;
-        sp[S_RZ] = a;
-        sp[S_RR] = c;
-        sp[S_IT] += 1;
-        if (c <= sp[S_RTOL2] * sp[S_BB] || sp[S_IT] >= max_iter || !(a > 0)) sp[S_DONE] = 1;
-    }
-}
-
-__global__ __launch_bounds__(BB) void kb_pupdate(Blocks B, const double *z, double *p, const double *S) {
-    const BlockRows br = block_rows(B, blockIdx.x);
-    const double *sp = S + br.p * S_NS;
-    if (sp[S_DONE] != 0) return;
-    const double beta = sp[S_BETA];
-    for (int64_t t = br.r0 + threadIdx.x; t < br.r1; t += BB) p[t] = z[t] + beta * p[t];
-}
-
-__global__ __launch_bounds__(BB) void kb_count_active(const double *S, int64_t K, int *out) {
-    __shared__ double sm[BB];
-    double c = 0;
-    for (int64_t p = threadIdx.x; p < K; p += BB) c += S[p * S_NS + S_DONE] == 0 ? 1.0 : 0.0;
-    c = bsum(c, sm);
-    if (threadIdx.x == 0) *out = (int)c;
+    idx[i] = (int)lo;
 }
 
 std::mutex tune_mutex;
 std::map<int, int> tune_cache;  // size class -> diracs per grid cell x100
 constexpr int TUNE_CANDIDATES[] = {150, 250, 400};
 
-void drop_graph(sdot_batch *B) {
-    if (B->graph) (void)hipGraphExecDestroy(B->graph);
-    B->graph = nullptr;
-}
-
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-Blocks blocks(sdot_batch *B) { return Blocks{B->blk_prob.p, B->dblk.p, B->doff.p}; }
+}  // namespace
 
-// per-problem grids in one code space, the counting sort, sorted positions
-void batch_grid(sdot_batch *B, int dpc_x100) {
-    sdot_pd *pd = B->pd;
-    const int64_t K = B->K, N = pd->N;
-    hipStream_t s = pd->stream;
-    B->grids.resize(K);
-    B->code_base.assign(K + 1, 0);
-    std::vector<double> tie(K);
-    for (int64_t p = 0; p < K; ++p) {
-        double lo[2] = {DBL_MAX, DBL_MAX}, hi[2] = {-DBL_MAX, -DBL_MAX};
-        for (int64_t i = B->off[p]; i < B->off[p + 1]; ++i)
-            for (int d = 0; d < 2; ++d) {
-                lo[d] = std::min(lo[d], B->hpos[2 * i + d]);
-                hi[d] = std::max(hi[d], B->hpos[2 * i + d]);
-            }
-        const int64_t np = B->off[p + 1] - B->off[p];
-        const GridInfo g = grid_params(2, lo, hi, np, dpc_x100 / 100.0);
-        B->grids[p] = g;
-        B->code_base[p + 1] = B->code_base[p] + g.ncodes;
-        // same scale as the single-problem path: 1e-12 of the squared mean spacing
-        tie[p] = 1e-12 * (g.h[0] * g.ncell[0]) * (g.h[1] * g.ncell[1]) / (double)np;
+namespace sdot_amd {
+
+GridInfo problem_grid(const double *pos, int64_t n, int dpc_x100, double *tie) {
+    double lo[2] = {DBL_MAX, DBL_MAX}, hi[2] = {-DBL_MAX, -DBL_MAX};
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 2; ++d) {
+            lo[d] = std::min(lo[d], pos[2 * i + d]);
+            hi[d] = std::max(hi[d], pos[2 * i + d]);
+        }
+    const GridInfo g = grid_params(2, lo, hi, n, dpc_x100 / 100.0);
+    // same scale as the single-problem path: 1e-12 of the squared mean spacing
+    if (tie) *tie = 1e-12 * (g.h[0] * g.ncell[0]) * (g.h[1] * g.ncell[1]) / (double)n;
+    return g;
+}
+
+struct BatchEngine::Impl {
+    DevBuf<int> prob, blk_prob, pdone, act;
+    DevBuf<int64_t> dcode_base, doff, dblk, src_off;
+    DevBuf<GridInfo> dgrids;
+    DevBuf<double> pdom, tie, w0, tgt, tsort, old, b, red, perr;
+    DevBuf<double> x, q;
+    DevBuf<double> pos2, w2, old2, w02, tgt2;   // relayout targets (swapped in)
+    DevBuf<unsigned> pflags;
+    DevBuf<unsigned long long> wglob;
+    std::vector<int64_t> blk_off;
+    int64_t nblk = 0, ncodes = 0;
+    PcgSolver pcg;
+};
+
+BatchEngine::BatchEngine(int precision, int ball, int dpc_x100, bool use_graph)
+    : im(new Impl), ball_(ball ? 1 : 0), dpc_x100_(dpc_x100 > 0 ? dpc_x100 : 250), use_graph_(use_graph) {
+    im->pcg.set_options(true, use_graph);
+    const sdot_status st = sdot_pd_create(&pd, 2, precision);
+    if (st != SDOT_OK) {
+        delete im;
+        throw StatusError{st, "device handle creation failed"};
     }
-    const int64_t nc = B->code_base[K];
+    pd->ball = ball_;
+    for (int k = 0; k < 4; ++k)
+        if (!pd->ev[k]) HIPCK(hipEventCreate(&pd->ev[k]));
+    pd->flags.ensure(2);
+}
+
+BatchEngine::~BatchEngine() {
+    for (auto *b : {&im->prob, &im->blk_prob, &im->pdone, &im->act}) b->release();
+    for (auto *b : {&im->dcode_base, &im->doff, &im->dblk, &im->src_off}) b->release();
+    for (auto *b : {&im->pdom, &im->tie, &im->w0, &im->tgt, &im->tsort, &im->old, &im->b, &im->red, &im->perr,
+                    &im->x, &im->q, &im->pos2, &im->w2, &im->old2, &im->w02, &im->tgt2})
+        b->release();
+    im->dgrids.release();
+    im->pflags.release();
+    im->wglob.release();
+    delete im;
+    if (pd) sdot_pd_destroy(pd);
+}
+
+double BatchEngine::bytes_per_dirac() {
+    // dirac arrays (pos, w, w0, old, tgt, spare copies for a relayout), sort keys and
+    // permutations, sorted copies, pass outputs, ELL rows at the wide capacity, PCG vectors
+    return 16 + 8 * 4 + 8 * 5 + 8 + 8 + 24 + 32 + 4 + ELL_CAP2_WIDE * 12.0 + 8 * 7 + 8 + 16;
+}
+
+size_t BatchEngine::device_bytes() const { return (size_t)(bytes_per_dirac() * (double)N_); }
+
+void BatchEngine::set_use_graph(bool on) {
+    use_graph_ = on;
+    im->pcg.set_options(im->pcg.amg(), on);
+}
+
+void BatchEngine::set_preconditioner(bool amg) { im->pcg.set_options(amg, use_graph_); }
+
+void BatchEngine::relayout(const std::vector<int> &keep, const std::vector<NewProblem> &add) {
+    hipStream_t s = pd->stream;
+    std::vector<BatchProblem> np;
+    np.reserve(keep.size() + add.size());
+    std::vector<int64_t> src;
+    int64_t off = 0;
+    for (int k : keep) {
+        BatchProblem b = probs.at(k);
+        src.push_back(b.off);
+        b.off = off;
+        off += b.n;
+        np.push_back(b);
+    }
+    for (const NewProblem &a : add) {
+        BatchProblem b;
+        b.n = a.n;
+        b.hpos = a.pos;
+        for (int d = 0; d < 4; ++d) b.box[d] = a.box[d];
+        b.g = problem_grid(a.pos, a.n, dpc_x100_, &b.tie);
+        b.off = off;
+        b.tag = a.tag;
+        off += a.n;
+        src.push_back(-1);
+        np.push_back(b);
+    }
+    const int64_t N = off, K = (int64_t)np.size();
+    if (N > 0x7FFFFFFF) throw StatusError{SDOT_INVALID_ARG, "more than 2^31-1 diracs resident"};
+    if (K == 0) {
+        probs.clear();
+        N_ = 0;
+        pd->N = 0;
+        return;
+    }
+    // per-problem tables
+    std::vector<int64_t> offs(K + 1);
+    for (int64_t k = 0; k < K; ++k) offs[k] = np[k].off;
+    offs[K] = N;
+    im->doff.ensure(K + 1);
+    im->src_off.ensure(K);
+    HIPCK(hipMemcpyAsync(im->doff.p, offs.data(), (K + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HIPCK(hipMemcpyAsync(im->src_off.p, src.data(), K * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    im->prob.ensure(N);
+    hipLaunchKernelGGL(kb_owner, dim3(nblocks(N, BB)), dim3(BB), 0, s, im->doff.p, K, N, im->prob.p);
+    HIPCK(hipGetLastError());
+    // dirac arrays: move the kept problems, upload the new ones, swap
+    for (auto *b : {&im->w2, &im->old2, &im->w02, &im->tgt2}) b->ensure(N);
+    im->pos2.ensure(2 * N);
+    if (!keep.empty())
+        hipLaunchKernelGGL(kb_move, dim3(nblocks(N, BB)), dim3(BB), 0, s, im->prob.p, N, im->src_off.p, im->doff.p,
+                           pd->pos.p, pd->w.p, im->old.p, im->w0.p, im->tgt.p, im->pos2.p, im->w2.p, im->old2.p,
+                           im->w02.p, im->tgt2.p);
+    HIPCK(hipGetLastError());
+    for (size_t a = 0; a < add.size(); ++a) {
+        const BatchProblem &b = np[keep.size() + a];
+        const NewProblem &q = add[a];
+        HIPCK(hipMemcpyAsync(im->pos2.p + 2 * b.off, q.pos, 2 * b.n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(im->w02.p + b.off, q.w0, b.n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(im->w2.p + b.off, q.w0, b.n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(im->old2.p + b.off, q.w0, b.n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(im->tgt2.p + b.off, q.tgt, b.n * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    std::swap(pd->pos, im->pos2);
+    std::swap(pd->w, im->w2);
+    std::swap(im->old, im->old2);
+    std::swap(im->w0, im->w02);
+    std::swap(im->tgt, im->tgt2);
+    // problem-aligned blocks
+    im->blk_off.assign(K + 1, 0);
+    for (int64_t k = 0; k < K; ++k) im->blk_off[k + 1] = im->blk_off[k] + (np[k].n + BB - 1) / BB;
+    im->nblk = im->blk_off[K];
+    im->dblk.ensure(K + 1);
+    HIPCK(hipMemcpyAsync(im->dblk.p, im->blk_off.data(), (K + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    im->blk_prob.ensure(im->nblk);
+    hipLaunchKernelGGL(kb_owner, dim3(nblocks(im->nblk, BB)), dim3(BB), 0, s, im->dblk.p, K, im->nblk, im->blk_prob.p);
+    HIPCK(hipGetLastError());
+    std::vector<double> dom(6 * K);
+    for (int64_t k = 0; k < K; ++k) {
+        dom[6 * k + 0] = np[k].box[0];
+        dom[6 * k + 1] = np[k].box[1];
+        dom[6 * k + 2] = 0.0;
+        dom[6 * k + 3] = np[k].box[2];
+        dom[6 * k + 4] = np[k].box[3];
+        dom[6 * k + 5] = 1.0;
+    }
+    im->pdom.ensure(6 * K);
+    HIPCK(hipMemcpyAsync(im->pdom.p, dom.data(), 6 * K * sizeof(double), hipMemcpyHostToDevice, s));
+    // per-dirac work arrays
+    for (auto *b : {&im->tsort, &im->b, &im->x, &im->q}) b->ensure(N);
+    im->red.ensure(2 * im->nblk + 16);
+    im->pcg.set_layout(offs, s);
+    im->perr.ensure(K);
+    im->pflags.ensure(K);
+    im->pdone.ensure(K);
+    im->act.ensure(K);
+    im->wglob.ensure(K);
+    if (pd->ell_cap < ELL_CAP2) pd->ell_cap = ELL_CAP2;
+    pd->N = N;
+    pd->mass.ensure(N);
+    pd->arc.ensure(N);
+    pd->perim.ensure(N);
+    pd->diag.ensure(N);
+    pd->row_len.ensure(N);
+    pd->ell_col.ensure((size_t)pd->ell_cap * N);
+    pd->ell_val.ensure((size_t)pd->ell_cap * N);
+    pd->ovf_list.ensure(2 * (size_t)N + 2);
+    probs.swap(np);
+    N_ = N;
+    std::vector<int> done(K);
+    for (int64_t k = 0; k < K; ++k) done[k] = probs[k].done;
+    HIPCK(hipMemcpyAsync(im->pdone.p, done.data(), K * sizeof(int), hipMemcpyHostToDevice, s));
+    rebuild_grid();
+}
+
+void BatchEngine::set_resolution(int dpc_x100) {
+    if (dpc_x100 == dpc_x100_) return;
+    dpc_x100_ = dpc_x100;
+    for (auto &b : probs) b.g = problem_grid(b.hpos, b.n, dpc_x100_, &b.tie);
+    if (!probs.empty()) rebuild_grid();
+}
+
+// per-problem grids in one code space, the counting sort, sorted positions, targets in grid order
+void BatchEngine::rebuild_grid() {
+    hipStream_t s = pd->stream;
+    const int64_t K = (int64_t)probs.size(), N = N_;
+    std::vector<GridInfo> grids(K);
+    std::vector<int64_t> cb(K + 1, 0);
+    std::vector<double> tie(K);
+    for (int64_t k = 0; k < K; ++k) {
+        grids[k] = probs[k].g;
+        tie[k] = probs[k].tie;
+        cb[k + 1] = cb[k] + grids[k].ncodes;
+    }
+    const int64_t nc = cb[K];
     if (nc >= (int64_t)0xFFFFFFFF) throw StatusError{SDOT_CAPACITY_OVERFLOW, "batch grid code space exceeds 32 bits"};
-    B->dgrids.ensure(K);
-    B->dcode_base.ensure(K + 1);
-    B->tie.ensure(K);
-    HIPCK(hipMemcpyAsync(B->dgrids.p, B->grids.data(), K * sizeof(GridInfo), hipMemcpyHostToDevice, s));
-    HIPCK(hipMemcpyAsync(B->dcode_base.p, B->code_base.data(), (K + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-    HIPCK(hipMemcpyAsync(B->tie.p, tie.data(), K * sizeof(double), hipMemcpyHostToDevice, s));
+    im->ncodes = nc;
+    im->dgrids.ensure(K);
+    im->dcode_base.ensure(K + 1);
+    im->tie.ensure(K);
+    HIPCK(hipMemcpyAsync(im->dgrids.p, grids.data(), K * sizeof(GridInfo), hipMemcpyHostToDevice, s));
+    HIPCK(hipMemcpyAsync(im->dcode_base.p, cb.data(), (K + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HIPCK(hipMemcpyAsync(im->tie.p, tie.data(), K * sizeof(double), hipMemcpyHostToDevice, s));
     pd->keys.ensure(N);
     pd->rank.ensure(N);
     pd->cell_start.ensure(nc + 1);
@@ -420,34 +456,49 @@ void batch_grid(sdot_batch *B, int dpc_x100) {
     pd->spos.ensure(2 * N);
     pd->sw.ensure(N);
     fill_u32(pd->scan_tmp.p, nc, 0u, s);
-    hipLaunchKernelGGL(kb_keys, dim3(nblocks(N, BB)), dim3(BB), 0, s, pd->pos.p, B->prob.p, N, B->dgrids.p,
-                       B->dcode_base.p, pd->keys.p, pd->rank.p, pd->scan_tmp.p);
+    hipLaunchKernelGGL(kb_keys, dim3(nblocks(N, BB)), dim3(BB), 0, s, pd->pos.p, im->prob.p, N, im->dgrids.p,
+                       im->dcode_base.p, pd->keys.p, pd->rank.p, pd->scan_tmp.p);
     HIPCK(hipGetLastError());
     sort_by_keys(pd, nc);
-    // the handle's own grid descriptor covers the whole code space (the kernels read the per-problem grids)
-    pd->g = B->grids[0];
+    // the handle's own grid descriptor spans the whole code space (the kernels read the per-problem grids)
+    pd->g = grids[0];
     pd->g.ncodes = nc;
-    // targets in grid order
-    hipLaunchKernelGGL(kb_gather, dim3(nblocks(N, 256)), dim3(256), 0, s, B->tgt.p, pd->perm.p, N, B->tsort.p);
+    hipLaunchKernelGGL(kb_gather, dim3(nblocks(N, 256)), dim3(256), 0, s, im->tgt.p, pd->perm.p, N, im->tsort.p);
     HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(s));
     pd->last.grid_cells = nc;
-    B->grid_dpc = dpc_x100;
 }
 
-// one cell pass over the problems not marked done; returns the device time (ms)
-double batch_pass(sdot_batch *B) {
-    sdot_pd *pd = B->pd;
-    const int64_t K = B->K, N = pd->N, nc = pd->g.ncodes;
+void BatchEngine::restart() {
     hipStream_t s = pd->stream;
+    const int64_t K = (int64_t)probs.size();
+    if (K == 0) return;
+    HIPCK(hipMemcpyAsync(pd->w.p, im->w0.p, N_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCK(hipMemcpyAsync(im->old.p, im->w0.p, N_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+    fill_u32(reinterpret_cast<uint32_t *>(im->pdone.p), K, 0u, s);
+    for (auto &b : probs) {
+        b.iters = b.done = b.dw_ok = b.backtracks = b.pcg_iters = 0;
+        b.status = SDOT_NOT_CONVERGED;
+        b.err = INFINITY;
+    }
+    HIPCK(hipStreamSynchronize(s));
+}
+
+namespace {
+
+// one cell pass over the problems not marked done; device ms
+double engine_pass(sdot_pd *pd, BatchEngine::Impl *im, int64_t K, int64_t N, bool &regrow) {
+    hipStream_t s = pd->stream;
+    const int64_t nc = im->ncodes;
+    regrow = false;
     HIPCK(hipEventRecord(pd->ev[0], s));
     fill_u32(pd->flags.p, 2, 0u, s);
-    fill_u32(B->pflags.p, K, 0u, s);
-    hipLaunchKernelGGL(kb_gather_w, dim3(nblocks(N, BB)), dim3(BB), 0, s, pd->w.p, pd->perm.p, B->prob.p, B->doff.p,
-                       B->tie.p, N, pd->sw.p, pd->flags.p);
+    fill_u32(im->pflags.p, K, 0u, s);
+    hipLaunchKernelGGL(kb_gather_w, dim3(nblocks(N, BB)), dim3(BB), 0, s, pd->w.p, pd->perm.p, im->prob.p, im->doff.p,
+                       im->tie.p, N, pd->sw.p, pd->flags.p);
     hipLaunchKernelGGL(kb_cell_wmax, dim3(nblocks(nc, BB)), dim3(BB), 0, s, pd->cell_start.p, nc, pd->sw.p,
                        pd->cell_wmax.p);
-    hipLaunchKernelGGL(kb_wmax, dim3((unsigned)K), dim3(BB), 0, s, pd->sw.p, B->doff.p, B->wglob.p);
+    hipLaunchKernelGGL(kb_wmax, dim3((unsigned)K), dim3(BB), 0, s, pd->sw.p, im->doff.p, im->wglob.p);
     HIPCK(hipGetLastError());
     HIPCK(hipEventRecord(pd->ev[1], s));
     CellArgs a{};
@@ -456,7 +507,7 @@ double batch_pass(sdot_batch *B) {
     a.sw = pd->sw.p;
     a.cell_start = pd->cell_start.p;
     a.cell_wmax = pd->cell_wmax.p;
-    a.wglob = B->wglob.p;
+    a.wglob = im->wglob.p;
     a.g = pd->g;
     a.coeff = 1;
     a.mass = pd->mass.p;
@@ -471,12 +522,12 @@ double batch_pass(sdot_batch *B) {
     a.flags = pd->flags.p;
     a.ovf_list = pd->ovf_list.p;
     a.ovf_count = pd->flags.p + 1;
-    a.prob = B->prob.p;
-    a.grids = B->dgrids.p;
-    a.code_base = B->dcode_base.p;
-    a.pdom = B->pdom.p;
-    a.pflags = B->pflags.p;
-    a.pdone = B->pdone.p;
+    a.prob = im->prob.p;
+    a.grids = im->dgrids.p;
+    a.code_base = im->dcode_base.p;
+    a.pdom = im->pdom.p;
+    a.pflags = im->pflags.p;
+    a.pdone = im->pdone.p;
     run_cells2(pd, a);   // synchronises the stream
     HIPCK(hipEventRecord(pd->ev[2], s));
     unsigned fl = 0;
@@ -485,87 +536,162 @@ double batch_pass(sdot_batch *B) {
     if (fl & FLAG_BAD_INPUT) throw StatusError{SDOT_INVALID_ARG, "non-finite weight or coincident diracs in the batch"};
     if (fl & FLAG_ROW_OVERFLOW) {
         if (pd->ell_cap >= ELL_CAP2_WIDE) throw StatusError{SDOT_CAPACITY_OVERFLOW, "Hessian row longer than the row capacity"};
-        // rerun with wide rows; the captured PCG graph read the old row arrays
+        // rerun with wide rows (kept from now on); a captured PCG graph read the old row arrays
         pd->ell_cap = ELL_CAP2_WIDE;
         pd->ell_col.ensure((size_t)pd->ell_cap * N);
         pd->ell_val.ensure((size_t)pd->ell_cap * N);
-        drop_graph(B);
-        return batch_pass(B);
+        im->pcg.invalidate();
+        regrow = true;
+        return 0;
     }
     float ms = 0;
     if (hipEventElapsedTime(&ms, pd->ev[0], pd->ev[2]) != hipSuccess) (void)hipGetLastError();
     return ms;
 }
 
-void pcg_iteration(sdot_batch *B, const BMat &A, int max_iter) {
-    hipStream_t s = B->pd->stream;
-    const Blocks bl = blocks(B);
-    const unsigned nb = (unsigned)B->nblk, K = (unsigned)B->K;
-    hipLaunchKernelGGL(kb_spmv_dot, dim3(nb), dim3(BB), 0, s, A, bl, B->p.p, B->q.p, B->red.p, B->S.p);
-    hipLaunchKernelGGL(kb_alpha, dim3(K), dim3(BB), 0, s, B->red.p, B->dblk.p, B->S.p);
-    hipLaunchKernelGGL(kb_axpy, dim3(nb), dim3(BB), 0, s, A, bl, B->p.p, B->q.p, B->x.p, B->r.p, B->z.p, B->red.p,
-                       B->S.p);
-    hipLaunchKernelGGL(kb_beta, dim3(K), dim3(BB), 0, s, B->red.p, B->dblk.p, B->S.p, max_iter);
-    hipLaunchKernelGGL(kb_pupdate, dim3(nb), dim3(BB), 0, s, bl, B->z.p, B->p.p, B->S.p);
+Blocks blocks_of(BatchEngine::Impl *im) { return Blocks{im->blk_prob.p, im->dblk.p, im->doff.p}; }
+
+}  // namespace
+
+double BatchEngine::pass_only() {
+    bool regrow = false;
+    double ms = engine_pass(pd, im, (int64_t)probs.size(), N_, regrow);
+    if (regrow) ms = engine_pass(pd, im, (int64_t)probs.size(), N_, regrow);
+    return ms;
 }
 
-// solves M x = b for the problems with S_DONE == 0; returns the summed iteration counts
-int batch_pcg(sdot_batch *B, int max_iter, sdot_batch_stats &st, double &graph_ms) {
-    sdot_pd *pd = B->pd;
-    hipStream_t s = pd->stream;
-    const int64_t K = B->K;
-    const BMat A{pd->diag.p, pd->row_len.p, pd->ell_col.p, pd->ell_val.p, pd->N};
-    hipLaunchKernelGGL(kb_init, dim3((unsigned)B->nblk), dim3(BB), 0, s, A, blocks(B), B->b.p, B->x.p, B->r.p, B->z.p,
-                       B->p.p, B->red.p, B->S.p);
-    hipLaunchKernelGGL(kb_init_red, dim3((unsigned)K), dim3(BB), 0, s, B->red.p, B->dblk.p, B->perr.p, B->S.p);
+int BatchEngine::tick(const TickParams &tp, std::vector<int> &finished, TickStats &ts) {
+    sdot_pd *pd_ = pd;
+    hipStream_t s = pd_->stream;
+    const int64_t K = (int64_t)probs.size(), N = N_;
+    if (K == 0) return 0;
+    bool regrow = false;
+    double ms = engine_pass(pd_, im, K, N, regrow);
+    if (regrow) ms = engine_pass(pd_, im, K, N, regrow);
+    ts.ms_pass += ms;
+    std::vector<unsigned> pfl(K);
+    HIPCK(hipMemcpyAsync(pfl.data(), im->pflags.p, K * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(kb_residual, dim3((unsigned)im->nblk), dim3(BB), 0, s, blocks_of(im), pd_->mass.p, im->tsort.p,
+                       im->b.p, im->red.p, im->pdone.p);
+    hipLaunchKernelGGL(kb_pmax, dim3((unsigned)K), dim3(BB), 0, s, im->red.p, im->dblk.p, im->perr.p);
     HIPCK(hipGetLastError());
-    if (B->use_graph && !B->graph) {
-        hipGraph_t g = nullptr;
-        HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < CHUNK; ++k) pcg_iteration(B, A, max_iter);
-        HIPCK(hipStreamEndCapture(s, &g));
-        const hipError_t e = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        HIPCK(e);
-    }
-    int *dcount = reinterpret_cast<int *>(B->red.p + 2 * B->nblk + 8);
-    int active = 1;
-    for (int it = 0; it < max_iter && active > 0; it += CHUNK) {
-        if (B->graph) {
-            HIPCK(hipEventRecord(B->gev[0], s));
-            HIPCK(hipGraphLaunch(B->graph, s));
-            HIPCK(hipEventRecord(B->gev[1], s));
-            ++st.graph_replays;
-        } else {
-            for (int k = 0; k < CHUNK; ++k) pcg_iteration(B, A, max_iter);
+    std::vector<double> e(K);
+    HIPCK(hipMemcpyAsync(e.data(), im->perr.p, K * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    // per-problem decisions (OptimalTransportSolver.tcc:60-112)
+    std::vector<int> act(K, ACT_NONE), done(K);
+    int nsolve = 0, running = 0;
+    for (int64_t k = 0; k < K; ++k) {
+        BatchProblem &b = probs[k];
+        if (!b.done) {
+            ts.cells += b.n;
+            const bool empty = (pfl[k] & FLAG_EMPTY) != 0;
+            if (!empty) b.err = e[k];
+            if (b.dw_ok || (tp.stop_rule == 0 && !empty && e[k] < tp.tol)) {   // converged at these weights
+                b.done = 1;
+                b.status = SDOT_OK;
+                finished.push_back((int)k);
+            } else if (b.iters >= tp.max_iter) {   // this pass only evaluated the last update
+                b.done = 1;
+                b.status = SDOT_NOT_CONVERGED;
+                finished.push_back((int)k);
+            } else {
+                ++b.iters;
+                ++running;
+                if (empty) {   // :72-79
+                    act[k] = ACT_BACKTRACK;
+                    ++b.backtracks;
+                    ++ts.backtracks;
+                } else {
+                    act[k] = ACT_ACCEPT;
+                    ++nsolve;
+                }
+            }
         }
-        hipLaunchKernelGGL(kb_count_active, dim3(1), dim3(BB), 0, s, B->S.p, K, dcount);
-        HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(&active, dcount, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCK(hipStreamSynchronize(s));
-        float ms = 0;
-        if (B->graph && hipEventElapsedTime(&ms, B->gev[0], B->gev[1]) == hipSuccess) graph_ms += ms;
+        done[k] = b.done;
     }
-    std::vector<double> S(K * S_NS);
-    HIPCK(hipMemcpyAsync(S.data(), B->S.p, S.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCK(hipStreamSynchronize(s));
-    int tot = 0;
-    for (int64_t p = 0; p < K; ++p) tot += (int)S[p * S_NS + S_IT];
-    return tot;
+    HIPCK(hipMemcpyAsync(im->act.p, act.data(), K * sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(kb_accept, dim3(nblocks(N, BB)), dim3(BB), 0, s, im->prob.p, N, im->act.p, pd_->w.p, im->old.p);
+    HIPCK(hipMemcpyAsync(im->pdone.p, done.data(), K * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCK(hipGetLastError());
+    if (nsolve > 0) {
+        if (!ball_)   // :62-63
+            hipLaunchKernelGGL(kb_gauge, dim3(nblocks(K, 256)), dim3(256), 0, s, pd_->diag.p, pd_->inv_perm.p, im->doff.p,
+                               im->act.p, K);
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<int> active(K), its;
+        for (int64_t k = 0; k < K; ++k) active[k] = act[k] == ACT_ACCEPT;
+        PcgStats ps;
+        im->pcg.solve(PcgMat{pd_->diag.p, pd_->row_len.p, pd_->ell_col.p, pd_->ell_val.p, N}, im->b.p, im->x.p,
+                      im->perr.p, active, 100000, its, ps);
+        hipLaunchKernelGGL(kb_update, dim3((unsigned)im->nblk), dim3(BB), 0, s, blocks_of(im), im->x.p, pd_->perm.p,
+                           pd_->w.p, im->act.p, im->red.p);
+        hipLaunchKernelGGL(kb_pmax, dim3((unsigned)K), dim3(BB), 0, s, im->red.p, im->dblk.p, im->perr.p);
+        HIPCK(hipGetLastError());
+        std::vector<double> mdw(K);
+        HIPCK(hipMemcpyAsync(mdw.data(), im->perr.p, K * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        ts.ms_pcg += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        ts.ms_pcg_graph += ps.ms_graph;
+        ts.graph_replays += ps.graph_replays;
+        ts.graph_iterations += ps.graph_iterations;
+        ts.amg_levels = std::max(ts.amg_levels, ps.amg_levels);
+        ts.amg_fallbacks += ps.amg_fallbacks;
+        ts.solved += nsolve;
+        for (int64_t k = 0; k < K; ++k) {
+            if (act[k] != ACT_ACCEPT) continue;
+            probs[k].pcg_iters += its[k];
+            ts.pcg_iters += its[k];
+            if (tp.stop_rule == 1 && mdw[k] < tp.tol) probs[k].dw_ok = 1;   // done once the next pass evaluates it
+        }
+    }
+    return running;
 }
 
-// residuals of the last pass -> per-problem max relative mass error (host)
-void batch_errors(sdot_batch *B, std::vector<double> &e) {
-    sdot_pd *pd = B->pd;
-    hipStream_t s = pd->stream;
-    hipLaunchKernelGGL(kb_residual, dim3((unsigned)B->nblk), dim3(BB), 0, s, blocks(B), pd->mass.p, B->tsort.p, B->b.p,
-                       B->red.p, B->pdone.p);
-    hipLaunchKernelGGL(kb_pmax, dim3((unsigned)B->K), dim3(BB), 0, s, B->red.p, B->dblk.p, B->perr.p);
-    HIPCK(hipGetLastError());
-    e.resize(B->K);
-    HIPCK(hipMemcpyAsync(e.data(), B->perr.p, B->K * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCK(hipStreamSynchronize(s));
+void BatchEngine::read_weights(int k, double *out) {
+    const BatchProblem &b = probs.at(k);
+    HIPCK(hipMemcpyAsync(out, pd->w.p + b.off, b.n * sizeof(double), hipMemcpyDeviceToHost, pd->stream));
+    HIPCK(hipStreamSynchronize(pd->stream));
 }
+
+void BatchEngine::read_weights_all(double *out) {
+    HIPCK(hipMemcpyAsync(out, pd->w.p, N_ * sizeof(double), hipMemcpyDeviceToHost, pd->stream));
+    HIPCK(hipStreamSynchronize(pd->stream));
+}
+
+void BatchEngine::read_masses_all(double *out) {
+    hipLaunchKernelGGL(kb_export, dim3(nblocks(N_, 256)), dim3(256), 0, pd->stream, pd->mass.p, pd->inv_perm.p, N_,
+                       im->q.p);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(out, im->q.p, N_ * sizeof(double), hipMemcpyDeviceToHost, pd->stream));
+    HIPCK(hipStreamSynchronize(pd->stream));
+}
+
+void BatchEngine::read_masses(int k, double *out) {
+    const BatchProblem &b = probs.at(k);
+    hipLaunchKernelGGL(kb_export, dim3(nblocks(N_, 256)), dim3(256), 0, pd->stream, pd->mass.p, pd->inv_perm.p, N_,
+                       im->q.p);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(out, im->q.p + b.off, b.n * sizeof(double), hipMemcpyDeviceToHost, pd->stream));
+    HIPCK(hipStreamSynchronize(pd->stream));
+}
+
+}  // namespace sdot_amd
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+struct sdot_batch {
+    BatchEngine *eng = nullptr;
+    int precision = SDOT_FP64;
+    int dpc_x100 = 0;             // 0: autotune on the first solve of a size class
+    int use_graph = 1;
+    int precond = SDOT_PRECOND_AMG;
+    std::vector<double> hpos;     // host copy of the positions (grids are built from it)
+    std::string err;
+};
+
+namespace {
 
 template <class F>
 sdot_status batch_guard(sdot_batch *B, F &&f) {
@@ -600,36 +726,17 @@ sdot_status sdot_batch_create(sdot_batch **out, int dim, int precision) {
     if (dim != 2 || (precision != SDOT_FP64 && precision != SDOT_MIXED)) return SDOT_INVALID_ARG;
     sdot_batch *B = new (std::nothrow) sdot_batch;
     if (!B) return SDOT_OOM;
-    sdot_status st = sdot_pd_create(&B->pd, dim, precision);
-    if (st == SDOT_OK)
-        st = batch_guard(B, [&] {
-            for (auto &e : B->gev) HIPCK(hipEventCreate(&e));
-            for (int k = 0; k < 4; ++k)
-                if (!B->pd->ev[k]) HIPCK(hipEventCreate(&B->pd->ev[k]));
-            return SDOT_OK;
-        });
-    if (st != SDOT_OK) {
-        sdot_batch_destroy(B);
-        return st;
-    }
+    B->precision = precision;
     *out = B;
     return SDOT_OK;
 }
 
 sdot_status sdot_batch_destroy(sdot_batch *B) {
     if (!B) return SDOT_INVALID_ARG;
-    drop_graph(B);
-    for (auto &e : B->gev)
-        if (e) (void)hipEventDestroy(e);
-    for (auto *b : {&B->prob, &B->blk_prob, &B->pdone, &B->act}) b->release();
-    for (auto *b : {&B->dcode_base, &B->doff, &B->dblk}) b->release();
-    for (auto *b : {&B->pdom, &B->tie, &B->w0, &B->tgt, &B->tsort, &B->old, &B->b, &B->red, &B->S, &B->perr, &B->x, &B->r, &B->z,
-                    &B->p, &B->q})
-        b->release();
-    B->dgrids.release();
-    B->pflags.release();
-    B->wglob.release();
-    if (B->pd) sdot_pd_destroy(B->pd);
+    try {
+        delete B->eng;
+    } catch (...) {
+    }
     delete B;
     return SDOT_OK;
 }
@@ -643,8 +750,14 @@ sdot_status sdot_batch_set_tuning(sdot_batch *B, int dpc_x100, int use_graph) {
     if (!B || dpc_x100 < 0 || (dpc_x100 > 0 && dpc_x100 < 25) || dpc_x100 > 10000) return SDOT_INVALID_ARG;
     B->dpc_x100 = dpc_x100;
     B->use_graph = use_graph ? 1 : 0;
-    B->grid_dpc = -1;
-    drop_graph(B);
+    if (B->eng) B->eng->set_use_graph(B->use_graph != 0);
+    return SDOT_OK;
+}
+
+sdot_status sdot_batch_set_linear_solver(sdot_batch *B, int precond) {
+    if (!B || (precond != SDOT_PRECOND_AMG && precond != SDOT_PRECOND_JACOBI)) return SDOT_INVALID_ARG;
+    B->precond = precond;
+    if (B->eng) B->eng->set_preconditioner(precond == SDOT_PRECOND_AMG);
     return SDOT_OK;
 }
 
@@ -663,102 +776,56 @@ sdot_status sdot_batch_set_problems(sdot_batch *B, int64_t K, const int64_t *off
         for (int64_t p = 0; p < K; ++p)
             if (!(boxes[4 * p + 2] > boxes[4 * p]) || !(boxes[4 * p + 3] > boxes[4 * p + 1])) return SDOT_INVALID_ARG;
     return batch_guard(B, [&] {
-        sdot_pd *pd = B->pd;
-        hipStream_t s = pd->stream;
-        drop_graph(B);   // buffers may move
-        B->K = K;
-        B->ball = ball ? 1 : 0;
-        B->grid_dpc = -1;
-        B->off.assign(offsets, offsets + K + 1);
-        B->hpos.assign(positions, positions + 2 * N);
-        pd->N = N;
-        pd->ball = B->ball;
-        pd->pos.ensure(2 * N);
-        pd->w.ensure(N);
-        HIPCK(hipMemcpyAsync(pd->pos.p, positions, 2 * N * sizeof(double), hipMemcpyHostToDevice, s));
-        B->w0.ensure(N);
-        HIPCK(hipMemcpyAsync(B->w0.p, weights, N * sizeof(double), hipMemcpyHostToDevice, s));
-        std::vector<int> prob(N);
-        for (int64_t p = 0; p < K; ++p) std::fill(prob.begin() + offsets[p], prob.begin() + offsets[p + 1], (int)p);
-        B->prob.ensure(N);
-        HIPCK(hipMemcpyAsync(B->prob.p, prob.data(), N * sizeof(int), hipMemcpyHostToDevice, s));
-        std::vector<double> dom(6 * K);
-        for (int64_t p = 0; p < K; ++p) {
-            dom[6 * p + 0] = boxes ? boxes[4 * p + 0] : 0.0;
-            dom[6 * p + 1] = boxes ? boxes[4 * p + 1] : 0.0;
-            dom[6 * p + 2] = 0.0;
-            dom[6 * p + 3] = boxes ? boxes[4 * p + 2] : 1.0;
-            dom[6 * p + 4] = boxes ? boxes[4 * p + 3] : 1.0;
-            dom[6 * p + 5] = 1.0;
+        if (!B->eng || B->eng->pd->ball != (ball ? 1 : 0)) {
+            delete B->eng;
+            B->eng = nullptr;
+            B->eng = new BatchEngine(B->precision, ball, B->dpc_x100 ? B->dpc_x100 : 250, B->use_graph != 0);
+            B->eng->set_preconditioner(B->precond == SDOT_PRECOND_AMG);
         }
-        B->pdom.ensure(6 * K);
-        HIPCK(hipMemcpyAsync(B->pdom.p, dom.data(), 6 * K * sizeof(double), hipMemcpyHostToDevice, s));
-        B->doff.ensure(K + 1);
-        HIPCK(hipMemcpyAsync(B->doff.p, offsets, (K + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-        B->blk_off.assign(K + 1, 0);
-        for (int64_t p = 0; p < K; ++p) B->blk_off[p + 1] = B->blk_off[p] + (offsets[p + 1] - offsets[p] + BB - 1) / BB;
-        B->nblk = B->blk_off[K];
-        std::vector<int> bp(B->nblk);
-        for (int64_t p = 0; p < K; ++p) std::fill(bp.begin() + B->blk_off[p], bp.begin() + B->blk_off[p + 1], (int)p);
-        B->blk_prob.ensure(B->nblk);
-        B->dblk.ensure(K + 1);
-        HIPCK(hipMemcpyAsync(B->blk_prob.p, bp.data(), B->nblk * sizeof(int), hipMemcpyHostToDevice, s));
-        HIPCK(hipMemcpyAsync(B->dblk.p, B->blk_off.data(), (K + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
-        B->tgt.ensure(N);
-        HIPCK(hipMemcpyAsync(B->tgt.p, targets, N * sizeof(double), hipMemcpyHostToDevice, s));
-        for (auto *b : {&B->tsort, &B->old, &B->b, &B->x, &B->r, &B->z, &B->p, &B->q}) b->ensure(N);
-        B->red.ensure(2 * B->nblk + 16);
-        B->S.ensure(K * S_NS);
-        B->perr.ensure(K);
-        B->pflags.ensure(K);
-        B->pdone.ensure(K);
-        B->act.ensure(K);
-        B->wglob.ensure(K);
-        pd->flags.ensure(2);
-        if (pd->ell_cap < ELL_CAP2) pd->ell_cap = ELL_CAP2;
-        pd->mass.ensure(N);
-        pd->arc.ensure(N);
-        pd->perim.ensure(N);
-        pd->diag.ensure(N);
-        pd->row_len.ensure(N);
-        pd->ell_col.ensure((size_t)pd->ell_cap * N);
-        pd->ell_val.ensure((size_t)pd->ell_cap * N);
-        pd->ovf_list.ensure(2 * (size_t)N + 2);
-        HIPCK(hipStreamSynchronize(s));
+        B->hpos.assign(positions, positions + 2 * N);
+        std::vector<NewProblem> add(K);
+        for (int64_t p = 0; p < K; ++p) {
+            NewProblem &a = add[p];
+            a.n = offsets[p + 1] - offsets[p];
+            a.pos = B->hpos.data() + 2 * offsets[p];
+            a.w0 = weights + offsets[p];
+            a.tgt = targets + offsets[p];
+            for (int d = 0; d < 4; ++d) a.box[d] = boxes ? boxes[4 * p + d] : (d < 2 ? 0.0 : 1.0);
+            a.tag = p;
+        }
+        B->eng->relayout({}, add);
         return SDOT_OK;
     });
 }
 
 sdot_status sdot_batch_solve(sdot_batch *B, double tol, int stop_rule, int max_iter, double *weights_out,
                              double *masses_out, int *status_out, int *iters_out, sdot_batch_stats *stats) {
-    if (!B || !weights_out || !(tol > 0) || max_iter < 1 || B->K < 1 || (stop_rule != 0 && stop_rule != 1))
+    if (!B || !weights_out || !(tol > 0) || max_iter < 1 || (stop_rule != 0 && stop_rule != 1)) return SDOT_INVALID_ARG;
+    if (!B->eng || B->eng->probs.empty()) {
+        B->err = "no problems set";
         return SDOT_INVALID_ARG;
+    }
     return batch_guard(B, [&] {
-        sdot_pd *pd = B->pd;
-        const int64_t K = B->K, N = pd->N;
-        hipStream_t s = pd->stream;
+        BatchEngine &E = *B->eng;
+        const int64_t K = (int64_t)E.probs.size(), N = E.total();
         const auto t0 = std::chrono::steady_clock::now();
         sdot_batch_stats st{};
-        fill_u32(reinterpret_cast<uint32_t *>(B->pdone.p), K, 0u, s);
-        // every solve starts from the weights given to set_problems
-        HIPCK(hipMemcpyAsync(pd->w.p, B->w0.p, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCK(hipMemcpyAsync(B->old.p, B->w0.p, N * sizeof(double), hipMemcpyDeviceToDevice, s));
-
         // grid resolution: given, cached for this size class, or tuned now on the start weights
         int dpc = B->dpc_x100;
-        const int size_class = ((int)std::lround(std::log2((double)N / (double)K)) * 2 + B->ball) * 2 +
-                               (pd->precision == SDOT_MIXED);
+        const int size_class = ((int)std::lround(std::log2((double)N / (double)K)) * 2 + E.pd->ball) * 2 +
+                               (B->precision == SDOT_MIXED);
         if (!dpc) {
             std::lock_guard<std::mutex> lk(tune_mutex);
             auto it = tune_cache.find(size_class);
             if (it != tune_cache.end()) dpc = it->second;
         }
+        E.restart();
         if (!dpc) {
             double best = DBL_MAX;
             for (int cand : TUNE_CANDIDATES) {
-                batch_grid(B, cand);
-                (void)batch_pass(B);   // first touch
-                const double ms = batch_pass(B);
+                E.set_resolution(cand);
+                (void)E.pass_only();   // first touch
+                const double ms = E.pass_only();
                 if (ms < best) {
                     best = ms;
                     dpc = cand;
@@ -768,91 +835,38 @@ sdot_status sdot_batch_solve(sdot_batch *B, double tol, int stop_rule, int max_i
             tune_cache[size_class] = dpc;
         }
         st.tuned_dpc_x100 = dpc;
-        if (B->grid_dpc != dpc) batch_grid(B, dpc);
-
-        std::vector<int> done(K, 0), iters(K, 0), pst(K, SDOT_NOT_CONVERGED), dw_ok(K, 0), act(K);
-        std::vector<unsigned> pfl(K);
-        std::vector<double> err(K, INFINITY), e, S(K * S_NS), mdw;
-        double ms_cells = 0, ms_solve = 0, graph_ms = 0;
-        bool evaluated = false;   // the last pass saw the current weights of every active problem
-        for (int it = 0; it <= max_iter; ++it) {
-            ms_cells += batch_pass(B);
-            evaluated = true;
-            HIPCK(hipMemcpyAsync(pfl.data(), B->pflags.p, K * sizeof(unsigned), hipMemcpyDeviceToHost, s));
-            batch_errors(B, e);
-            // per-problem decisions
-            int nsolve = 0;
-            for (int64_t p = 0; p < K; ++p) {
-                act[p] = ACT_NONE;
-                S[p * S_NS + S_DONE] = 1;
-                if (done[p]) continue;
-                const bool empty = (pfl[p] & FLAG_EMPTY) != 0;
-                if (!empty) err[p] = e[p];
-                if (dw_ok[p] || (stop_rule == 0 && !empty && e[p] < tol)) {   // converged at these weights
-                    done[p] = 1;
-                    pst[p] = SDOT_OK;
-                    continue;
-                }
-                if (it == max_iter) continue;   // this pass only evaluates the last update
-                iters[p] = it + 1;
-                if (empty) {
-                    act[p] = ACT_BACKTRACK;
-                    st.backtracks++;
-                    continue;
-                }
-                act[p] = ACT_ACCEPT;
-                S[p * S_NS + S_DONE] = 0;
-                ++nsolve;
-            }
-            HIPCK(hipMemcpyAsync(B->act.p, act.data(), K * sizeof(int), hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(kb_accept, dim3(nblocks(N, BB)), dim3(BB), 0, s, B->prob.p, N, B->act.p, pd->w.p,
-                               B->old.p);
-            HIPCK(hipMemcpyAsync(B->pdone.p, done.data(), K * sizeof(int), hipMemcpyHostToDevice, s));
-            HIPCK(hipGetLastError());
-            if (std::all_of(done.begin(), done.end(), [](int d) { return d != 0; }) || it == max_iter) break;
-            if (nsolve == 0) continue;   // only backtracks this round
-            HIPCK(hipMemcpyAsync(B->S.p, S.data(), S.size() * sizeof(double), hipMemcpyHostToDevice, s));
-            if (!B->ball)
-                hipLaunchKernelGGL(kb_gauge, dim3(nblocks(K, 256)), dim3(256), 0, s, pd->diag.p, pd->inv_perm.p,
-                                   B->doff.p, B->act.p, K);
-            const auto ts = std::chrono::steady_clock::now();
-            st.pcg_iters_total += batch_pcg(B, 100000, st, graph_ms);
-            hipLaunchKernelGGL(kb_update, dim3((unsigned)B->nblk), dim3(BB), 0, s, blocks(B), B->x.p, pd->perm.p,
-                               pd->w.p, B->act.p, B->red.p);
-            hipLaunchKernelGGL(kb_pmax, dim3((unsigned)K), dim3(BB), 0, s, B->red.p, B->dblk.p, B->perr.p);
-            HIPCK(hipGetLastError());
-            mdw.resize(K);
-            HIPCK(hipMemcpyAsync(mdw.data(), B->perr.p, K * sizeof(double), hipMemcpyDeviceToHost, s));
-            HIPCK(hipStreamSynchronize(s));
-            ms_solve += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts).count();
-            evaluated = false;
-            if (stop_rule == 1)
-                for (int64_t p = 0; p < K; ++p)
-                    if (act[p] == ACT_ACCEPT && mdw[p] < tol) dw_ok[p] = 1;   // done once the next pass evaluates it
-        }
-        (void)evaluated;
-        HIPCK(hipMemcpyAsync(weights_out, pd->w.p, N * sizeof(double), hipMemcpyDeviceToHost, s));
-        if (masses_out) {
-            hipLaunchKernelGGL(kb_export, dim3(nblocks(N, 256)), dim3(256), 0, s, pd->mass.p, pd->inv_perm.p, N, B->q.p);
-            HIPCK(hipGetLastError());
-            HIPCK(hipMemcpyAsync(masses_out, B->q.p, N * sizeof(double), hipMemcpyDeviceToHost, s));
-        }
-        HIPCK(hipStreamSynchronize(s));
+        E.set_resolution(dpc);
+        TickParams tp;
+        tp.tol = tol;
+        tp.stop_rule = stop_rule;
+        tp.max_iter = max_iter;
+        TickStats ts;
+        std::vector<int> fin;
+        for (int guard = 0; guard <= max_iter + 1; ++guard)
+            if (E.tick(tp, fin, ts) == 0) break;
+        E.read_weights_all(weights_out);
+        if (masses_out) E.read_masses_all(masses_out);
         int nconv = 0, maxit = 0;
         double maxerr = 0;
-        for (int64_t p = 0; p < K; ++p) {
-            nconv += done[p];
-            maxit = std::max(maxit, iters[p]);
-            maxerr = std::max(maxerr, err[p]);
-            if (status_out) status_out[p] = pst[p];
-            if (iters_out) iters_out[p] = iters[p];
+        for (int64_t k = 0; k < K; ++k) {
+            const BatchProblem &b = E.probs[k];
+            nconv += b.status == SDOT_OK;
+            maxit = std::max(maxit, b.iters);
+            maxerr = std::max(maxerr, b.err);
+            if (status_out) status_out[k] = b.status;
+            if (iters_out) iters_out[k] = b.iters;
         }
         st.newton_iters_max = maxit;
         st.nb_converged = nconv;
+        st.pcg_iters_total = ts.pcg_iters;
+        st.backtracks = ts.backtracks;
+        st.graph_replays = ts.graph_replays;
         st.max_rel_mass_err = maxerr;
-        st.seconds_cells = ms_cells * 1e-3;
-        st.seconds_solve = ms_solve * 1e-3;
-        st.ms_pcg_iteration = st.graph_replays ? graph_ms / (st.graph_replays * (double)CHUNK) : 0.0;
+        st.seconds_cells = ts.ms_pass * 1e-3;
+        st.seconds_solve = ts.ms_pcg * 1e-3;
+        st.ms_pcg_iteration = ts.graph_iterations ? ts.ms_pcg_graph / (double)ts.graph_iterations : 0.0;
+        st.precond_levels = ts.amg_levels;
+        st.precond_fallbacks = ts.amg_fallbacks;
         st.seconds_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (stats) *stats = st;
         return nconv == K ? SDOT_OK : SDOT_NOT_CONVERGED;

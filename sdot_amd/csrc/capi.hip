@@ -7,6 +7,7 @@
 #include <new>
 
 #include "internal.h"
+#include "pcg.h"
 
 using namespace sdot_amd;
 
@@ -193,6 +194,7 @@ const char *sdot_status_string(int st) {
         case SDOT_HIP_ERROR: return "HIP error";
         case SDOT_OOM: return "out of memory";
         case SDOT_NOT_SUPPORTED: return "not supported";
+        case SDOT_PENDING: return "pending";
         default: return "unknown status";
     }
 }
@@ -237,6 +239,8 @@ sdot_status sdot_pd_destroy(sdot_pd *pd) {
     pd->csr_val.release(); pd->out_tmp.release(); pd->scratch3.release();
     for (auto &b : pd->sv) b.release();
     pd->red.release();
+    delete pd->pcg;
+    pd->pcg = nullptr;
     if (pd->stream) (void)hipStreamDestroy(pd->stream);
     delete pd;
     return SDOT_OK;
@@ -380,6 +384,12 @@ sdot_status sdot_pd_set_tuning(sdot_pd *pd, int dpc_x100, int block) {
     pd->dpc_x100 = dpc_x100;
     pd->block = block;
     pd->pass_valid = false;
+    return SDOT_OK;
+}
+
+sdot_status sdot_pd_set_linear_solver(sdot_pd *pd, int precond) {
+    if (!pd || (precond != SDOT_PRECOND_AMG && precond != SDOT_PRECOND_JACOBI)) return SDOT_INVALID_ARG;
+    pd->precond = precond;
     return SDOT_OK;
 }
 

@@ -237,6 +237,7 @@ struct Shape {
     int64_t cut_id;
 };
 
+class PcgSolver;
 }  // namespace sdot_amd
 
 struct sdot_pd {
@@ -286,6 +287,10 @@ struct sdot_pd {
     // solver scratch
     sdot_amd::DevBuf<double> sv[10];
     sdot_amd::DevBuf<double> red;
+    sdot_amd::PcgSolver *pcg = nullptr;   // created on the first solve
+    int64_t pcg_n = -1;                   // layout size of pcg
+    int precond = SDOT_PRECOND_AMG;
+    int last_precond_levels = 0;          // multigrid levels of the last solve (0: Jacobi)
 
     // timing
     hipEvent_t ev[8] = {};

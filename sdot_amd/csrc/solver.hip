@@ -10,8 +10,11 @@
 #include <chrono>
 #include <cmath>
 #include <cfloat>
+#include <cstdio>
+#include <cstdlib>
 
 #include "internal.h"
+#include "pcg.h"
 
 namespace sdot_amd {
 
@@ -20,17 +23,7 @@ namespace {
 constexpr int BS = 256;
 constexpr int MAXB = 1024;   // reduction blocks
 
-struct Mat {
-    const double *diag;
-    const int *len;
-    const int *col;
-    const double *val;
-    int64_t stride;
-    int64_t n;
-};
-
-// scalars kept on the device: [0] rz, [1] bb, [2] alpha, [3] beta, [4] rr, [5] done, [6] iters, [7] rtol^2
-enum { S_RZ = 0, S_BB, S_ALPHA, S_BETA, S_RR, S_DONE, S_IT, S_RTOL2, S_NS };
+constexpr int S_NS = 8;   // scalar slots kept after the reduction partials
 
 __device__ inline double block_sum(double v, double *sm) {
     sm[threadIdx.x] = v;
@@ -56,129 +49,6 @@ __device__ inline double block_max(double v, double *sm) {
     return r;
 }
 
-__device__ inline double row_dot(const Mat &A, int64_t t, const double *x) {
-    double s = A.diag[t] * x[t];
-    const int L = A.len[t];
-    for (int e = 0; e < L; ++e) s += A.val[e * A.stride + t] * x[A.col[e * A.stride + t]];
-    return s;
-}
-
-__global__ __launch_bounds__(BS) void k_init(Mat A, const double *b, double *x, double *r, double *z, double *p,
-                                             double *part) {
-    __shared__ double sm[BS];
-    double rz = 0, bb = 0;
-    for (int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x; t < A.n; t += (int64_t)gridDim.x * BS) {
-        double d = A.diag[t];
-        double rv = b[t];
-        double zv = d != 0 ? rv / d : rv;
-        x[t] = 0;
-        r[t] = rv;
-        z[t] = zv;
-        p[t] = zv;
-        rz += rv * zv;
-        bb += rv * rv;
-    }
-    rz = block_sum(rz, sm);
-    bb = block_sum(bb, sm);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = rz;
-        part[MAXB + blockIdx.x] = bb;
-    }
-}
-
-__global__ __launch_bounds__(BS) void k_init_reduce(const double *part, int nb, double *S, double rtol) {
-    __shared__ double sm[BS];
-    double a = 0, c = 0;
-    for (int k = threadIdx.x; k < nb; k += BS) {
-        a += part[k];
-        c += part[MAXB + k];
-    }
-    a = block_sum(a, sm);
-    c = block_sum(c, sm);
-    if (threadIdx.x == 0) {
-        S[S_RZ] = a;
-        S[S_BB] = c;
-        S[S_RR] = c;
-        S[S_RTOL2] = rtol * rtol;
-        S[S_IT] = 0;
-        S[S_DONE] = (c == 0 || !(a > 0)) ? 1.0 : 0.0;
-    }
-}
-
-// q = A p, partial p.q
-__global__ __launch_bounds__(BS) void k_spmv_dot(Mat A, const double *p, double *q, double *part, const double *S) {
-    __shared__ double sm[BS];
-    if (S[S_DONE] != 0) return;
-    double pq = 0;
-    for (int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x; t < A.n; t += (int64_t)gridDim.x * BS) {
-        double v = row_dot(A, t, p);
-        q[t] = v;
-        pq += p[t] * v;
-    }
-    pq = block_sum(pq, sm);
-    if (threadIdx.x == 0) part[blockIdx.x] = pq;
-}
-
-__global__ __launch_bounds__(BS) void k_alpha(const double *part, int nb, double *S) {
-    __shared__ double sm[BS];
-    if (S[S_DONE] != 0) return;
-    double a = 0;
-    for (int k = threadIdx.x; k < nb; k += BS) a += part[k];
-    a = block_sum(a, sm);
-    if (threadIdx.x == 0) S[S_ALPHA] = a != 0 ? S[S_RZ] / a : 0.0;
-}
-
-// x += alpha p; r -= alpha q; z = D^-1 r; partial r.z, r.r
-__global__ __launch_bounds__(BS) void k_axpy(Mat A, const double *p, const double *q, double *x, double *r, double *z,
-                                             double *part, const double *S) {
-    __shared__ double sm[BS];
-    if (S[S_DONE] != 0) return;
-    const double alpha = S[S_ALPHA];
-    double rz = 0, rr = 0;
-    for (int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x; t < A.n; t += (int64_t)gridDim.x * BS) {
-        x[t] += alpha * p[t];
-        double rv = r[t] - alpha * q[t];
-        r[t] = rv;
-        double d = A.diag[t];
-        double zv = d != 0 ? rv / d : rv;
-        z[t] = zv;
-        rz += rv * zv;
-        rr += rv * rv;
-    }
-    rz = block_sum(rz, sm);
-    rr = block_sum(rr, sm);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = rz;
-        part[MAXB + blockIdx.x] = rr;
-    }
-}
-
-__global__ __launch_bounds__(BS) void k_beta(const double *part, int nb, double *S, int max_iter) {
-    __shared__ double sm[BS];
-    if (S[S_DONE] != 0) return;
-    double a = 0, c = 0;
-    for (int k = threadIdx.x; k < nb; k += BS) {
-        a += part[k];
-        c += part[MAXB + k];
-    }
-    a = block_sum(a, sm);
-    c = block_sum(c, sm);
-    if (threadIdx.x == 0) {
-        S[S_BETA] = S[S_RZ] != 0 ? a / S[S_RZ] : 0.0;
-        S[S_RZ] = a;
-        S[S_RR] = c;
-        S[S_IT] += 1;
-        if (c <= S[S_RTOL2] * S[S_BB] || S[S_IT] >= max_iter || !(a > 0)) S[S_DONE] = 1;
-    }
-}
-
-__global__ __launch_bounds__(BS) void k_pupdate(int64_t n, const double *z, double *p, const double *S) {
-    if (S[S_DONE] != 0) return;
-    const double beta = S[S_BETA];
-    for (int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x; t < n; t += (int64_t)gridDim.x * BS) p[t] = z[t] + beta * p[t];
-}
-
-// Newton helpers (sorted order t, dirac order i = perm[t])
 __global__ __launch_bounds__(BS) void k_gather(const double *src, const int *perm, int64_t n, double *dst) {
     int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x;
     if (t < n) dst[t] = src[perm[t]];
@@ -231,34 +101,20 @@ inline int red_blocks(int64_t n) { return (int)std::min<int64_t>(MAXB, std::max<
 
 // Solves A x = b (A: diag + ELL rows of the last pass) by Jacobi-PCG to the
 // relative residual rtol.  Returns the iteration count.
-int pcg_solve(sdot_pd *pd, const double *b, double *x, double rtol, int max_iter) {
-    const int64_t n = pd->N;
-    hipStream_t s = pd->stream;
-    for (int k = 5; k < 9; ++k) pd->sv[k].ensure(n);
-    double *r = pd->sv[5].p, *z = pd->sv[6].p, *p = pd->sv[7].p, *q = pd->sv[8].p;
-    pd->red.ensure(2 * MAXB + S_NS + 8);
-    double *part = pd->red.p, *S = pd->red.p + 2 * MAXB;
-    Mat A{pd->diag.p, pd->row_len.p, pd->ell_col.p, pd->ell_val.p, n, n};
-    const int nb = red_blocks(n);
-    hipLaunchKernelGGL(k_init, dim3(nb), dim3(BS), 0, s, A, b, x, r, z, p, part);
-    hipLaunchKernelGGL(k_init_reduce, dim3(1), dim3(BS), 0, s, part, nb, S, rtol);
-    HIPCK(hipGetLastError());
-    const int chunk = 16;
-    double hs[S_NS];
-    for (int it = 0; it < max_iter; it += chunk) {
-        for (int k = 0; k < chunk; ++k) {
-            hipLaunchKernelGGL(k_spmv_dot, dim3(nb), dim3(BS), 0, s, A, p, q, part, S);
-            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(BS), 0, s, part, nb, S);
-            hipLaunchKernelGGL(k_axpy, dim3(nb), dim3(BS), 0, s, A, p, q, x, r, z, part, S);
-            hipLaunchKernelGGL(k_beta, dim3(1), dim3(BS), 0, s, part, nb, S, max_iter);
-            hipLaunchKernelGGL(k_pupdate, dim3(nb), dim3(BS), 0, s, n, z, p, S);
-        }
-        HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(hs, S, sizeof hs, hipMemcpyDeviceToHost, s));
-        HIPCK(hipStreamSynchronize(s));
-        if (hs[S_DONE] != 0) break;
+// M x = b on the handle's Hessian rows; relative tolerance from the mass error at d_err
+int pcg_solve(sdot_pd *pd, const double *b, double *x, const double *d_err) {
+    if (!pd->pcg) pd->pcg = new PcgSolver();
+    if (pd->pcg_n != pd->N) {
+        pd->pcg->set_layout({0, pd->N}, pd->stream);
+        pd->pcg_n = pd->N;
     }
-    return (int)hs[S_IT];
+    pd->pcg->set_options(pd->precond == SDOT_PRECOND_AMG, true);
+    std::vector<int> its;
+    PcgStats ps;
+    pd->pcg->solve(PcgMat{pd->diag.p, pd->row_len.p, pd->ell_col.p, pd->ell_val.p, pd->N}, b, x, d_err, {1}, 100000,
+                   its, ps);
+    pd->last_precond_levels = ps.amg_fallbacks ? 0 : ps.amg_levels;
+    return its[0];
 }
 
 sdot_status ot_solve(sdot_pd *pd, const double *targets, double tol, int stop_rule, int max_iter, double *w_inout,
@@ -287,10 +143,21 @@ sdot_status ot_solve(sdot_pd *pd, const double *targets, double tol, int stop_ru
     const int nb = red_blocks(n);
     const unsigned nbg = (unsigned)((n + BS - 1) / BS);
     double ms_cells = 0, ms_solve = 0;
+    // SDOT_TRACE=1: one line per Newton iteration on stderr (pass / PCG timings)
+    const char *trace_env = getenv("SDOT_TRACE");
+    const bool trace = trace_env && trace_env[0] == '1';
+    auto wall_ms = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count(); };
+    auto trace_pass = [&](int it, sdot_status ps) {
+        if (trace)
+            fprintf(stderr, "[sdot] it %d pass %.2f ms (grid %.2f, cell kernel %.2f, all cell kernels %.2f, wide cells %lld) status %d wall %.1f ms\n",
+                    it, pd->last.ms_total, pd->last.ms_grid, pd->last.ms_cell_kernel, pd->last.ms_cells,
+                    (long long)pd->last.nb_cells_overflow, (int)ps, wall_ms());
+    };
     for (int it = 0; it < max_iter; ++it) {
         sdot_status ps = compute_pass(pd);
         ms_cells += pd->last.ms_total;
         st.newton_iters = it + 1;
+        trace_pass(it, ps);
         if (ps == SDOT_EMPTY_CELL) {   // OptimalTransportSolver.tcc:72-79
             hipLaunchKernelGGL(k_backtrack, dim3(nb), dim3(BS), 0, s, old, n, pd->w.p);
             HIPCK(hipGetLastError());
@@ -319,8 +186,7 @@ sdot_status ot_solve(sdot_pd *pd, const double *targets, double tol, int stop_ru
             HIPCK(hipGetLastError());
         }
         HIPCK(hipEventRecord(pd->ev[4], s));
-        double rtol = std::max(1e-14, std::min(1e-3, 1e-3 * rel));
-        st.pcg_iters_total += pcg_solve(pd, b, dw, rtol, 100000);
+        st.pcg_iters_total += pcg_solve(pd, b, dw, scal);   // rtol = max(1e-14, min(1e-3, 1e-3 rel))
         HIPCK(hipEventRecord(pd->ev[5], s));
         hipLaunchKernelGGL(k_update_w, dim3(nb), dim3(BS), 0, s, dw, pd->perm.p, n, pd->w.p, part);
         hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(BS), 0, s, part, nb, scal);
@@ -332,6 +198,9 @@ sdot_status ot_solve(sdot_pd *pd, const double *targets, double tol, int stop_ru
         if (hipEventElapsedTime(&ms, pd->ev[4], pd->ev[5]) == hipSuccess) ms_solve += ms;
         pd->pass_valid = false;
         st.max_abs_dw = mdw;
+        if (trace)
+            fprintf(stderr, "[sdot] it %d rel mass err %.3e, pcg %d iters %.2f ms, max |dw| %.3e, wall %.1f ms\n", it, rel,
+                    st.pcg_iters_total, ms, mdw, wall_ms());
         if (stop_rule == 1 && mdw < tol) {
             result = SDOT_OK;
             break;
@@ -359,6 +228,7 @@ sdot_status ot_solve(sdot_pd *pd, const double *targets, double tol, int stop_ru
     HIPCK(hipMemcpyAsync(w_inout, pd->w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
     st.nnz = 0;
+    st.precond_levels = pd->last_precond_levels;
     st.seconds_cells = ms_cells * 1e-3;
     st.seconds_solve = ms_solve * 1e-3;
     st.seconds_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

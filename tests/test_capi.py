@@ -36,7 +36,7 @@ def test_rng_matches_fixture():
 def test_status_strings():
     import sdot_amd
     L = sdot_amd.lib()
-    for st in range(8):
+    for st in range(9):
         assert len(L.sdot_status_string(st)) > 0
 
 

@@ -114,3 +114,26 @@ def test_batch_not_converged_and_reuse():
     r2 = b.solve(tol=1e-6)          # same handle again: same result
     for k in range(3):
         assert np.array_equal(r1[1][k], r2[1][k])
+
+
+def test_batch_multigrid_matches_jacobi_and_is_independent_of_the_mix():
+    import sdot_amd
+    pos = make_problems(8, 600, 30, 20000)     # includes problems below the coarsest-level size
+    out = {}
+    for pc in ("amg", "jacobi"):
+        b = sdot_amd.Batch("fp64", diracs_per_cell=2.5, precond=pc)
+        b.set_problems(pos)
+        out[pc] = b.solve(tol=1e-9)
+    (st_a, wa, _, pa, ia, sa), (st_j, wj, _, pj, ij, sj) = out["amg"], out["jacobi"]
+    assert st_a == 0 and st_j == 0
+    assert sa["precond_levels"] >= 2 and sa["precond_fallbacks"] == 0
+    assert sa["pcg_iters_total"] * 3 < sj["pcg_iters_total"], (sa, sj)
+    for k in range(len(pos)):
+        d1, d2 = wa[k] - wa[k].mean(), wj[k] - wj[k].mean()
+        assert np.abs(d1 - d2).max() < 1e-6 * np.abs(d2).max(), k
+    # alone: bitwise the same with the multigrid too
+    b = sdot_amd.Batch("fp64", diracs_per_cell=2.5, precond="amg")
+    b.set_problems([pos[5]])
+    r1 = b.solve(tol=1e-9)
+    assert np.array_equal(r1[1][0], wa[5]) and r1[4][0] == ia[5]
+    print(f"[batch precond] amg {sa['pcg_iters_total']} ({sa['precond_levels']} levels) jacobi {sj['pcg_iters_total']}")
