@@ -63,6 +63,19 @@ def run_gpu(pos, w, precision, steps, warmup):
                 overflow_cells=int(tims[-1]["nb_cells_overflow"]), status=pd.last_status), masses
 
 
+def copy_calibration(mib, reps=3):
+    """Plain device copies of a known size (reads and writes mib MiB each) for
+    scaling FETCH_SIZE / WRITE_SIZE counters; torch provides the buffers."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.empty(n, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"# copy calibration: {reps} copies of {mib} MiB", file=sys.stderr, flush=True)
+
+
 def run_cpu_baseline(pos, w, budget_s=10.0, max_reps=40):
     from oracle import oracle as O
     cores = min(16, os.cpu_count() or 1)
@@ -214,7 +227,7 @@ def c5_run(rate, duration, deadline, precision, seed=7, label=""):
     import sdot_amd
     from sdot_amd import workloads
     t_arr, dims, sizes = c5_schedule(rate, duration, seed)
-    srv = sdot_amd.Server(precision, diracs_per_cell=2.5, workers_3d=2, tol=1e-6)
+    srv = sdot_amd.Server(precision, diracs_per_cell=2.5, workers_3d=2, tol=1e-6, hbm_budget_bytes=64e9)
     ids = [None] * len(t_arr)
     done_submit = threading.Event()
     t0 = time.perf_counter()
@@ -313,6 +326,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run device-to-device copies of MIB MiB (known HBM bytes) so hardware "
+                         "counters of the cell kernel can be scaled by the copy's counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU only (the cell pass of one GPU's share); --gpus {args.gpus} "
@@ -329,6 +345,8 @@ def main():
     if args.workload == "c5":
         return main_c5(args)
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     pos, w = make_inputs(args.n)
     res = {}
     masses = {}

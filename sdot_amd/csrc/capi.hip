@@ -79,6 +79,12 @@ void domain_args(sdot_pd *pd, CellArgs &a) {
 }  // namespace
 
 namespace sdot_amd {
+
+std::recursive_mutex &device_alloc_mutex() {
+    static std::recursive_mutex m;
+    return m;
+}
+
 sdot_status compute_pass(sdot_pd *pd) {
     if (pd->N <= 0) throw StatusError{SDOT_INVALID_ARG, "no diracs"};
     if (pd->ball && pd->dim == 3)
@@ -115,6 +121,7 @@ sdot_status compute_pass(sdot_pd *pd) {
     a.cell_start = pd->cell_start.p;
     a.cell_wmax = pd->cell_wmax.p;
     a.wglob = pd->wglob.p;
+    a.wpyr = pd->dim == 3 ? pd->wpyr.p : nullptr;
     a.g = pd->g;
     a.mass = pd->mass.p;
     a.arc = pd->arc.p;
@@ -232,7 +239,7 @@ sdot_status sdot_pd_destroy(sdot_pd *pd) {
     pd->pos.release(); pd->w.release();
     pd->keys.release(); pd->rank.release(); pd->cell_start.release(); pd->scan_tmp.release();
     pd->perm.release(); pd->inv_perm.release();
-    pd->spos.release(); pd->sw.release(); pd->cell_wmax.release(); pd->bbox_part.release(); pd->wglob.release();
+    pd->spos.release(); pd->sw.release(); pd->cell_wmax.release(); pd->bbox_part.release(); pd->wpyr.release(); pd->wglob.release();
     pd->mass.release(); pd->arc.release(); pd->perim.release(); pd->diag.release(); pd->ell_val.release();
     pd->row_len.release(); pd->ell_col.release(); pd->ovf_list.release(); pd->flags.release(); pd->wctr.release();
     pd->csr_off.release(); pd->csr_col.release(); pd->csr_cnt.release(); pd->csr_tmp.release();

@@ -43,6 +43,9 @@ struct Poly3Mem {
     T *sx, *sy, *sz;
     int *stri;
     int S;
+    // dual-edge map: vertex holding the directed edge (x -> y) at [(x * PCAP + y) * S];
+    // null: edges are found by a scan of the vertices
+    unsigned short *emap = nullptr;
 };
 
 // voxel / box plane ids: box side k -> -1 - k (k = 0..5: -x,+x,-y,+y,-z,+z);
@@ -128,6 +131,7 @@ struct Cell3 {
             V_Y(k) = T((sy ? a.dom_hi[1] : a.dom_lo[1]) - c0y);
             V_Z(k) = T((sz ? a.dom_hi[2] : a.dom_lo[2]) - c0z);
         }
+        write_edges();
     }
 
     SDOT_HD double radius2() const {
@@ -139,8 +143,28 @@ struct Cell3 {
         return R2;
     }
 
+    // record the three dual edges of every vertex (after any change of the vertex list)
+    SDOT_HD void write_edges() {
+        if (!m.emap) return;
+        for (int k = 0; k < nv; ++k) {
+            const int t = V_T(k);
+            const int p0 = t & 255, p1 = (t >> 8) & 255, p2 = (t >> 16) & 255;
+            m.emap[(p0 * PCAP + p1) * m.S] = (unsigned short)k;
+            m.emap[(p1 * PCAP + p2) * m.S] = (unsigned short)k;
+            m.emap[(p2 * PCAP + p0) * m.S] = (unsigned short)k;
+        }
+    }
+
     // vertex holding the directed dual edge (x -> y)
     SDOT_HD int find_edge(int x, int y) const {
+        if (m.emap) {   // the map holds every current edge; stale entries are rejected by the check
+            const int k = m.emap[(x * PCAP + y) * m.S];
+            if (k < nv) {
+                const int t = V_T(k);
+                const int p0 = t & 255, p1 = (t >> 8) & 255, p2 = (t >> 16) & 255;
+                if ((p0 == x && p1 == y) || (p1 == x && p2 == y) || (p2 == x && p0 == y)) return k;
+            }
+        }
         for (int k = 0; k < nv; ++k) {
             const int t = V_T(k);
             const int p0 = t & 255, p1 = (t >> 8) & 255, p2 = (t >> 16) & 255;
@@ -200,6 +224,7 @@ struct Cell3 {
             V_T(k) = newidx[t & 255] | (newidx[(t >> 8) & 255] << 8) | (newidx[(t >> 16) & 255] << 16);
         }
         np = w;
+        write_edges();
     }
 
     // cut by q.n <= h (plane id `id`): 0 = no change, 1 = cut (nv may become 0),
@@ -263,6 +288,7 @@ struct Cell3 {
         nv = w;
         add_plane(pn, nx64, ny64, nz64, h64, id);
         np = pn + 1;
+        write_edges();
         return 1;
     }
 
@@ -339,6 +365,7 @@ struct Cell3 {
             V_Z(k) = m.sz[k * m.S];
             V_T(k) = m.stri[k * m.S];
         }
+        write_edges();
     }
 };
 

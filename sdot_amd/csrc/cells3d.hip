@@ -17,7 +17,8 @@ constexpr int BS3 = 256;
 
 template <class T>
 size_t slot_bytes(int vcap, int pcap) {
-    return (size_t)vcap * (7 * sizeof(T) + 2 * sizeof(int)) + (size_t)pcap * (4 * sizeof(T) + sizeof(int) + 2 * sizeof(double));
+    return (size_t)vcap * (7 * sizeof(T) + 2 * sizeof(int)) + (size_t)pcap * (4 * sizeof(T) + sizeof(int) + 2 * sizeof(double)) +
+           (size_t)pcap * pcap * sizeof(unsigned short);
 }
 
 template <class T>
@@ -46,6 +47,7 @@ __host__ __device__ c3::Poly3Mem<T> slot(char *base, int64_t S, int64_t s, int v
     take(m.pid, sizeof(int), pcap);
     take(m.acc, sizeof(double), pcap);
     take(m.hacc, sizeof(double), pcap);
+    take(m.emap, sizeof(unsigned short), pcap * pcap);
     m.S = (int)S;
     return m;
 }
@@ -109,6 +111,8 @@ void launch_thread3(sdot_pd *pd, const CellArgs &a) {
 // to the thread kernel above.
 // ---------------------------------------------------------------------------
 constexpr int WV = 64;            // lanes = vertex capacity = plane capacity
+constexpr int FRONT = 256;        // far-field traversal: frontier capacity (nodes)
+constexpr int NEAR_RINGS = 1;     // rings of grid cells visited before the far-field traversal
 constexpr int WPB = 4;            // wavefronts per block
 
 struct WaveLds {
@@ -120,6 +124,7 @@ struct WaveLds {
     double sx[WV], sy[WV], sz[WV];
     int stri[WV];
     int cpre[WV], ccs[WV], ccnt[WV];   // staged grid cells: candidate prefix, first dirac, count
+    uint32_t fr[2][FRONT];        // far-field traversal: node frontier (double buffered)
 };
 
 __device__ inline void wave_sync() {
@@ -464,101 +469,179 @@ __global__ __launch_bounds__(WV * WPB) void k_cells3_wave(const CellArgs *__rest
         double R2 = C.radius2();
         bool overflow = false, bad = false;
         const int hh[3] = {grid_coord(g, 0, c0x), grid_coord(g, 1, c0y), grid_coord(g, 2, c0z)};
-        for (int r = 0; r <= maxr && C.nv > 0 && !overflow; ++r) {
-            if (r > 0) {
-                double dm = DBL_MAX;
-                bool any = false;
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    if (hh[d] - r + 1 > 0) { dm = fmin(dm, c0[d] - (g.lo[d] + (hh[d] - r + 1) * g.h[d])); any = true; }
-                    if (hh[d] + r < g.ncell[d]) { dm = fmin(dm, (g.lo[d] + (hh[d] + r) * g.h[d]) - c0[d]); any = true; }
+        // The diracs of the grid cells held by the lanes (ok, [cs, ce)) are staged 64 at a
+        // time (one per lane, loaded in parallel), tested against the current radius,
+        // and only the ballot of possible cutters is applied, in order.
+        auto process_cells = [&](bool ok, uint32_t cs, uint32_t ce) {
+            const int cnt = ok ? (int)(ce - cs) : 0;
+            const int pre = wave_excl_scan(cnt, lane);
+            const int tot = __shfl(pre + cnt, WV - 1);
+            if (tot == 0) return;
+            L.cpre[lane] = pre;
+            L.ccs[lane] = (int)cs;
+            L.ccnt[lane] = cnt;
+            wave_sync();
+            for (int b0 = 0; b0 < tot && C.nv > 0 && !overflow; b0 += WV) {
+                const int gi = b0 + lane;
+                int j = -1;
+                if (gi < tot) {
+                    int lo = 0, hi = WV - 1;   // last cell slot with cpre <= gi and a non-empty range
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (L.cpre[mid] <= gi) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    while (L.ccnt[lo] == 0) --lo;
+                    j = L.ccs[lo] + (gi - L.cpre[lo]);
                 }
-                if (!any) break;
-                if (c3::region_skippable3<T>(fmax(dm, 0.0), R2, w0, W)) break;
+                double vx = 0, vy = 0, vz = 0, hv = 0, vv = 0;
+                bool cand = j >= 0 && (int64_t)j != t;
+                if (cand) {
+                    vx = a.spos[3 * (int64_t)j] - c0x;
+                    vy = a.spos[3 * (int64_t)j + 1] - c0y;
+                    vz = a.spos[3 * (int64_t)j + 2] - c0z;
+                    vv = vx * vx + vy * vy + vz * vz;
+                    hv = 0.5 * (vv + w0 - a.sw[j]);
+                    if (vv == 0) {
+                        bad = true;
+                        cand = false;
+                    } else if (hv > 0 && R2 * vv <= hv * hv * (1 - c3::Eps3<T>::cand)) {
+                        cand = false;
+                    }
+                }
+                unsigned long long mask = __ballot(cand);
+                while (mask && C.nv > 0) {
+                    const int k = __ffsll((long long)mask) - 1;
+                    mask &= mask - 1;
+                    const double ux = __shfl(vx, k), uy = __shfl(vy, k), uz = __shfl(vz, k), uh = __shfl(hv, k),
+                                 uv = __shfl(vv, k);
+                    const int uj = __shfl(j, k);
+                    if (uh > 0 && R2 * uv <= uh * uh * (1 - c3::Eps3<T>::cand)) continue;   // R shrank
+                    const double tie = c3::Eps3<T>::tie * sqrt(uv * R2);
+                    const int rc = C.cut(T(ux), T(uy), T(uz), T(uh), uj, ux, uy, uz, uh, tie, true);
+                    if (rc == 2) {
+                        overflow = true;
+                        break;
+                    }
+                    if (rc == 1 && C.nv > 0) R2 = C.radius2();
+                }
             }
-            // The grid cells of ring r, 64 at a time (one per lane): the cell tests run in
-            // parallel, then the diracs of the surviving cells are staged 64 at a time
-            // (one per lane, loaded in parallel), tested against the current radius,
-            // and only the ballot of possible cutters is applied, in order.
-            const int side = 2 * r + 1;
-            const int ncube = side * side * side;
-            for (int base = 0; base < ncube && C.nv > 0 && !overflow; base += WV) {
-                const int c = base + lane;
-                int dz = c / (side * side) - r, dy = (c / side) % side - r, dx = c % side - r;
-                bool ok = c < ncube && (dx == r || dx == -r || dy == r || dy == -r || dz == r || dz == -r);
-                const int gx = hh[0] + dx, gy = hh[1] + dy, gz = hh[2] + dz;
-                ok = ok && gx >= 0 && gy >= 0 && gz >= 0 && gx < g.ncell[0] && gy < g.ncell[1] && gz < g.ncell[2];
+            wave_sync();
+        };
+        // Rings r0..r1 of grid cells around the dirac's cell, 64 cells at a time (one per
+        // lane, tested in parallel).  `finished`: no dirac beyond the last ring can cut.
+        bool finished = false;
+        auto rings = [&](int r0, int r1) {
+            for (int r = r0; r <= r1 && C.nv > 0 && !overflow; ++r) {
+                if (r > 0) {
+                    double dm = DBL_MAX;
+                    bool any = false;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        if (hh[d] - r + 1 > 0) { dm = fmin(dm, c0[d] - (g.lo[d] + (hh[d] - r + 1) * g.h[d])); any = true; }
+                        if (hh[d] + r < g.ncell[d]) { dm = fmin(dm, (g.lo[d] + (hh[d] + r) * g.h[d]) - c0[d]); any = true; }
+                    }
+                    if (!any || c3::region_skippable3<T>(fmax(dm, 0.0), R2, w0, W)) {
+                        finished = true;
+                        return;
+                    }
+                }
+                const int side = 2 * r + 1;
+                const int ncube = side * side * side;
+                for (int base = 0; base < ncube && C.nv > 0 && !overflow; base += WV) {
+                    const int c = base + lane;
+                    int dz = c / (side * side) - r, dy = (c / side) % side - r, dx = c % side - r;
+                    bool ok = c < ncube && (dx == r || dx == -r || dy == r || dy == -r || dz == r || dz == -r);
+                    const int gx = hh[0] + dx, gy = hh[1] + dy, gz = hh[2] + dz;
+                    ok = ok && gx >= 0 && gy >= 0 && gz >= 0 && gx < g.ncell[0] && gy < g.ncell[1] && gz < g.ncell[2];
+                    uint32_t cs = 0, ce = 0;
+                    if (ok) {
+                        const uint32_t code = morton3(gx, gy, gz);
+                        cs = a.cell_start[code];
+                        ce = a.cell_start[code + 1];
+                        if (cs == ce) {
+                            ok = false;
+                        } else {
+                            const double bx = g.lo[0] + gx * g.h[0], by = g.lo[1] + gy * g.h[1], bz = g.lo[2] + gz * g.h[2];
+                            const double ex = fmax(0.0, fmax(bx - c0x, c0x - (bx + g.h[0])));
+                            const double ey = fmax(0.0, fmax(by - c0y, c0y - (by + g.h[1])));
+                            const double ez = fmax(0.0, fmax(bz - c0z, c0z - (bz + g.h[2])));
+                            ok = !c3::region_skippable3<T>(sqrt(ex * ex + ey * ey + ez * ez), R2, w0, a.cell_wmax[code]);
+                        }
+                    }
+                    process_cells(ok, cs, ce);
+                }
+            }
+        };
+        // Far field: the cells beyond the near rings, found by a breadth-first descent of
+        // the max-weight pyramid (Morton nodes of 2^k cells per axis); a node is dropped
+        // when no dirac under it can cut the current cell (its max weight, its box) or when
+        // it lies inside the near rings.  Returns false when the frontier outgrows its LDS
+        // capacity (nothing has been cut yet by this pass then).
+        auto far_field = [&]() -> bool {
+            int cur = 0, na = 1;
+            if (lane == 0) L.fr[0][0] = 0u;
+            wave_sync();
+            for (int k = g.bits; k >= 1; --k) {
+                const int span = 1 << (k - 1);
+                const double *wl = a.wpyr + pyr_off(g.ncodes, 3, k - 1);
+                int nb = 0;
+                for (int base = 0; base < na * 8; base += WV) {
+                    const int idx = base + lane;
+                    bool keep = false;
+                    uint32_t child = 0;
+                    if (idx < na * 8) {
+                        child = L.fr[cur][idx >> 3] * 8u + (uint32_t)(idx & 7);
+                        const int cc[3] = {(int)compact3(child) * span, (int)compact3(child >> 1) * span,
+                                           (int)compact3(child >> 2) * span};
+                        if (cc[0] < g.ncell[0] && cc[1] < g.ncell[1] && cc[2] < g.ncell[2]) {
+                            bool inside = true;
+#pragma unroll
+                            for (int d = 0; d < 3; ++d)
+                                inside = inside && cc[d] >= hh[d] - NEAR_RINGS && cc[d] + span - 1 <= hh[d] + NEAR_RINGS;
+                            const double wm = wl[child];
+                            if (!inside && wm > -DBL_MAX) {
+                                double e2 = 0;
+#pragma unroll
+                                for (int d = 0; d < 3; ++d) {
+                                    const int c1 = cc[d] + span < g.ncell[d] ? cc[d] + span : g.ncell[d];
+                                    const double lo = g.lo[d] + cc[d] * g.h[d], hi = g.lo[d] + c1 * g.h[d];
+                                    const double e = fmax(0.0, fmax(lo - c0[d], c0[d] - hi));
+                                    e2 += e * e;
+                                }
+                                keep = !c3::region_skippable3<T>(sqrt(e2), R2, w0, wm);
+                            }
+                        }
+                    }
+                    const unsigned long long m = __ballot(keep);
+                    const int pos = nb + __popcll(m & ((1ull << lane) - 1ull));
+                    if (keep && pos < FRONT) L.fr[cur ^ 1][pos] = child;
+                    nb += __popcll(m);
+                }
+                if (nb > FRONT) return false;
+                wave_sync();
+                cur ^= 1;
+                na = nb;
+                if (na == 0) return true;
+            }
+            // level 0: grid cells
+            for (int base = 0; base < na && C.nv > 0 && !overflow; base += WV) {
+                const int idx = base + lane;
                 uint32_t cs = 0, ce = 0;
+                bool ok = idx < na;
                 if (ok) {
-                    const uint32_t code = morton3(gx, gy, gz);
+                    const uint32_t code = L.fr[cur][idx];
                     cs = a.cell_start[code];
                     ce = a.cell_start[code + 1];
-                    if (cs == ce) {
-                        ok = false;
-                    } else {
-                        const double bx = g.lo[0] + gx * g.h[0], by = g.lo[1] + gy * g.h[1], bz = g.lo[2] + gz * g.h[2];
-                        const double ex = fmax(0.0, fmax(bx - c0x, c0x - (bx + g.h[0])));
-                        const double ey = fmax(0.0, fmax(by - c0y, c0y - (by + g.h[1])));
-                        const double ez = fmax(0.0, fmax(bz - c0z, c0z - (bz + g.h[2])));
-                        ok = !c3::region_skippable3<T>(sqrt(ex * ex + ey * ey + ez * ez), R2, w0, a.cell_wmax[code]);
-                    }
+                    ok = cs != ce;
                 }
-                const int cnt = ok ? (int)(ce - cs) : 0;
-                const int pre = wave_excl_scan(cnt, lane);
-                const int tot = __shfl(pre + cnt, WV - 1);
-                if (tot == 0) continue;
-                L.cpre[lane] = pre;
-                L.ccs[lane] = (int)cs;
-                L.ccnt[lane] = cnt;
-                wave_sync();
-                for (int b0 = 0; b0 < tot && C.nv > 0 && !overflow; b0 += WV) {
-                    const int gi = b0 + lane;
-                    int j = -1;
-                    if (gi < tot) {
-                        int lo = 0, hi = WV - 1;   // last cell slot with cpre <= gi and a non-empty range
-                        while (lo < hi) {
-                            const int mid = (lo + hi + 1) >> 1;
-                            if (L.cpre[mid] <= gi) lo = mid;
-                            else hi = mid - 1;
-                        }
-                        while (L.ccnt[lo] == 0) --lo;
-                        j = L.ccs[lo] + (gi - L.cpre[lo]);
-                    }
-                    double vx = 0, vy = 0, vz = 0, hv = 0, vv = 0;
-                    bool cand = j >= 0 && (int64_t)j != t;
-                    if (cand) {
-                        vx = a.spos[3 * (int64_t)j] - c0x;
-                        vy = a.spos[3 * (int64_t)j + 1] - c0y;
-                        vz = a.spos[3 * (int64_t)j + 2] - c0z;
-                        vv = vx * vx + vy * vy + vz * vz;
-                        hv = 0.5 * (vv + w0 - a.sw[j]);
-                        if (vv == 0) {
-                            bad = true;
-                            cand = false;
-                        } else if (hv > 0 && R2 * vv <= hv * hv * (1 - c3::Eps3<T>::cand)) {
-                            cand = false;
-                        }
-                    }
-                    unsigned long long mask = __ballot(cand);
-                    while (mask && C.nv > 0) {
-                        const int k = __ffsll((long long)mask) - 1;
-                        mask &= mask - 1;
-                        const double ux = __shfl(vx, k), uy = __shfl(vy, k), uz = __shfl(vz, k), uh = __shfl(hv, k),
-                                     uv = __shfl(vv, k);
-                        const int uj = __shfl(j, k);
-                        if (uh > 0 && R2 * uv <= uh * uh * (1 - c3::Eps3<T>::cand)) continue;   // R shrank
-                        const double tie = c3::Eps3<T>::tie * sqrt(uv * R2);
-                        const int rc = C.cut(T(ux), T(uy), T(uz), T(uh), uj, ux, uy, uz, uh, tie, true);
-                        if (rc == 2) {
-                            overflow = true;
-                            break;
-                        }
-                        if (rc == 1 && C.nv > 0) R2 = C.radius2();
-                    }
-                }
-                wave_sync();
+                process_cells(ok, cs, ce);
             }
-        }
+            return true;
+        };
+        const bool pyramid = a.wpyr != nullptr && g.bits >= 2;
+        rings(0, pyramid ? NEAR_RINGS : maxr);
+        if (pyramid && !finished && C.nv > 0 && !overflow && !far_field()) rings(NEAR_RINGS + 1, maxr);
         if (C.nv > 0 && !overflow) C.compact_planes();   // voxel cuts then start from the face count
         if (bad && lane == 0) flag_or(a.flags, FLAG_BAD_INPUT);
         if (overflow) {

@@ -119,6 +119,15 @@ __global__ __launch_bounds__(BS) void k_gather_w(const double *w, const int *per
 
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
+// pyramid level k from level k-1: node n = max of its 2^dim children (Morton order)
+__global__ __launch_bounds__(BS) void k_pyr_level(const double *fine, int64_t nnodes, int children, double *coarse) {
+    const int64_t n = blockIdx.x * (int64_t)BS + threadIdx.x;
+    if (n >= nnodes) return;
+    double m = -DBL_MAX;
+    for (int c = 0; c < children; ++c) m = fmax(m, fine[n * children + c]);
+    coarse[n] = m;
+}
+
 __global__ __launch_bounds__(BS) void k_cell_wmax(const uint32_t *start, int64_t ncodes, const double *sw,
                                                   double *cell_wmax, unsigned long long *wglob) {
     __shared__ double sm[BS];
@@ -344,6 +353,16 @@ void refresh_weights(sdot_pd *pd) {
     hipLaunchKernelGGL(k_cell_wmax, dim3(nblk(g.ncodes, BS)), dim3(BS), 0, s, pd->cell_start.p, g.ncodes, pd->sw.p,
                        pd->cell_wmax.p, pd->wglob.p);
     HIPCK(hipGetLastError());
+    if (pd->dim == 3) {   // max-weight pyramid (level 0 = a copy of cell_wmax) for the far-field traversal
+        pd->wpyr.ensure(pyr_off(g.ncodes, 3, g.bits + 1));
+        HIPCK(hipMemcpyAsync(pd->wpyr.p, pd->cell_wmax.p, g.ncodes * sizeof(double), hipMemcpyDeviceToDevice, s));
+        for (int k = 1; k <= g.bits; ++k) {
+            const int64_t nn = g.ncodes >> (3 * k);
+            hipLaunchKernelGGL(k_pyr_level, dim3(nblk(nn, BS)), dim3(BS), 0, s, pd->wpyr.p + pyr_off(g.ncodes, 3, k - 1), nn,
+                               8, pd->wpyr.p + pyr_off(g.ncodes, 3, k));
+        }
+        HIPCK(hipGetLastError());
+    }
 }
 
 }  // namespace sdot_amd

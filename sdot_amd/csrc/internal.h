@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -36,6 +37,11 @@ struct StatusError {
     std::string msg;
 };
 
+// Device allocation and stream capture exclude each other across host threads:
+// hipFree synchronises the device, which must not happen while another thread
+// records a graph (the capture holds the stream the synchronisation waits on).
+std::recursive_mutex &device_alloc_mutex();
+
 // grow-only device buffer
 template <class T>
 struct DevBuf {
@@ -43,6 +49,7 @@ struct DevBuf {
     size_t n = 0;
     void ensure(size_t m) {
         if (m <= n && p) return;
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
         if (p) HIPCK(hipFree(p));
         p = nullptr;
         n = 0;
@@ -55,7 +62,10 @@ struct DevBuf {
         n = want;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
+            (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
@@ -163,6 +173,9 @@ struct CellArgs {
     const uint32_t *cell_start;  // by Morton code, ncodes + 1 entries
     const double *cell_wmax;     // by Morton code
     const unsigned long long *wglob;  // ordered bits of the global max weight
+    // max-weight pyramid over the Morton code space: level k (0 = cell_wmax) holds
+    // ncodes >> (dim k) nodes at wpyr + pyr_off(k); null when not built
+    const double *wpyr;
     GridInfo g;
     double dom_lo[3], dom_hi[3];
     double coeff;
@@ -226,6 +239,20 @@ SDOT_HD inline void prob_view(const CellArgs &a, int64_t t, ProbView &v) {
         v.hi[d] = a.pdom[6 * p + 3 + d];
     }
 }
+// offset of level k of the max-weight pyramid (level 0 has ncodes nodes)
+SDOT_HD inline int64_t pyr_off(int64_t ncodes, int dim, int k) {
+    int64_t o = 0;
+    for (int j = 0; j < k; ++j) o += ncodes >> (dim * j);
+    return o;
+}
+SDOT_HD inline uint32_t compact3(uint32_t x) {
+    x &= 0x09249249u;
+    x = (x | (x >> 2)) & 0x030C30C3u;
+    x = (x | (x >> 4)) & 0x0300F00Fu;
+    x = (x | (x >> 8)) & 0x030000FFu;
+    x = (x | (x >> 16)) & 0x000003FFu;
+    return x;
+}
 constexpr int WCTR = 1024;       // counters are spread over WCTR slots to avoid contention
 
 // ---------------------------------------------------------------------------
@@ -266,7 +293,7 @@ struct sdot_pd {
     sdot_amd::GridInfo g{};
     sdot_amd::DevBuf<uint32_t> keys, rank, cell_start, scan_tmp;
     sdot_amd::DevBuf<int> perm, inv_perm;
-    sdot_amd::DevBuf<double> spos, sw, cell_wmax, bbox_part;
+    sdot_amd::DevBuf<double> spos, sw, cell_wmax, bbox_part, wpyr;
     sdot_amd::DevBuf<unsigned long long> wglob;
 
     // pass outputs (sorted order)
@@ -290,6 +317,7 @@ struct sdot_pd {
     sdot_amd::PcgSolver *pcg = nullptr;   // created on the first solve
     int64_t pcg_n = -1;                   // layout size of pcg
     int precond = SDOT_PRECOND_AMG;
+    int use_graph = 1;                    // replay the PCG iterations from a captured graph
     int last_precond_levels = 0;          // multigrid levels of the last solve (0: Jacobi)
 
     // timing

@@ -440,7 +440,9 @@ struct PcgSolver::Impl {
     int graph_max_iter = 0;
     hipEvent_t gev[2] = {};
     void drop_graph() {
-        if (graph) (void)hipGraphExecDestroy(graph);
+        if (!graph) return;
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
+        (void)hipGraphExecDestroy(graph);
         graph = nullptr;
     }
     Blocks blocks() const { return Blocks{blk_prob.p, dblk.p, doff.p}; }
@@ -663,11 +665,13 @@ void PcgSolver::solve(const PcgMat &A, const double *b, double *x, const double 
                            I.graph_A.val == A.val && I.graph_A.stride == A.stride && I.graph_x == x &&
                            I.graph_max_iter == max_iter && I.graph_amg == amg;
     if (use_graph_ && (!I.graph || !same_args)) {
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());   // no device free while recording
         I.drop_graph();
         hipGraph_t g = nullptr;
         HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (int k = 0; k < CHUNK; ++k) cg_iteration(I, A, x, amg, max_iter);
-        HIPCK(hipStreamEndCapture(s, &g));
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        HIPCK(ec);
         const hipError_t e = hipGraphInstantiate(&I.graph, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         HIPCK(e);

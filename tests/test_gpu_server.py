@@ -15,13 +15,21 @@ def problem(dim, n, seed):
     return workloads.uniform_diracs(n, dim, seed)
 
 
+def wait(srv, rid, what):
+    """Waits with a limit so a stuck request fails the test with the server state."""
+    r = srv.wait(rid, timeout=90.0)
+    assert r is not None, (what, srv.stats(), srv.last_error())
+    print(f"[server] {what}: {r[1]}", flush=True)
+    return r
+
+
 def test_server_results_identical_alone_and_in_the_mix():
     import sdot_amd
     x2 = problem(2, 3000, 11)
     x3 = problem(3, 2000, 12)
     srv = sdot_amd.Server("mixed", diracs_per_cell=2.5, workers_3d=2)
-    a2 = srv.wait(srv.submit(x2))
-    a3 = srv.wait(srv.submit(x3))
+    a2 = wait(srv, srv.submit(x2), "2D alone")
+    a3 = wait(srv, srv.submit(x3), "3D alone")
     assert a2[1]["status"] == 0 and a3[1]["status"] == 0, (a2[1], a3[1], srv.last_error())
     # the same two requests inside a stream of others (2D and 3D, several sizes)
     ids = []
@@ -31,9 +39,9 @@ def test_server_results_identical_alone_and_in_the_mix():
         if k == 5:
             i2 = srv.submit(x2)
             i3 = srv.submit(x3)
-    b2 = srv.wait(i2)
-    b3 = srv.wait(i3)
-    rest = [srv.wait(i) for i in ids]
+    b2 = wait(srv, i2, "2D in the mix")
+    b3 = wait(srv, i3, "3D in the mix")
+    rest = [wait(srv, i, f"mix {k}") for k, i in enumerate(ids)]
     assert all(r[1]["status"] == 0 for r in rest)
     assert np.array_equal(a2[0], b2[0]) and a2[1]["newton_iters"] == b2[1]["newton_iters"]
     assert np.array_equal(a3[0], b3[0]) and a3[1]["newton_iters"] == b3[1]["newton_iters"]
@@ -62,7 +70,7 @@ def test_server_admission_respects_the_budget():
     per = n * 1400.0          # about two 2D requests of this size fit
     srv = sdot_amd.Server("fp64", hbm_budget_bytes=2.5 * per, workers_3d=1, diracs_per_cell=2.5)
     ids = [srv.submit(problem(2, n, 200 + k)) for k in range(6)]
-    out = [srv.wait(i) for i in ids]
+    out = [wait(srv, i, f"budget {k}") for k, i in enumerate(ids)]
     st = srv.stats()
     assert all(r[1]["status"] == 0 for r in out)
     assert st["peak_resident_bytes"] <= st["budget_bytes"] + 1
