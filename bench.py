@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Laguerre cells/s (1M diracs 2D/3D); solve s @1e-6 mass err; p99 latency"
 PEAK = {"fp64": 78.6, "mixed": 157.3}        # MI355X vector FP64 / FP32, TFLOP/s
+# HBM bytes per launch of the cell kernel on the 1M ball workload: rocprofv3 FETCH_SIZE and
+# WRITE_SIZE of k_cells2, each scaled by the same counter on a 1 GiB device copy
+# (bench.py --copy-calibration 1024: FETCH reads half the copied bytes, WRITE all of them)
+TRAFFIC = {"mixed": 185.0e6, "fp64": 184.1e6}
 
 
 def make_inputs(n, seed=0):
@@ -93,7 +97,7 @@ def run_cpu_baseline(pos, w, budget_s=10.0, max_reps=40):
                        f"by the C oracle with {cores} threads, {dt:.3f} s each"), r["masses"]
 
 
-def run_c3(n, img_size, steps):
+def run_c3(n, img_size, steps, precisions=("fp64", "mixed")):
     """3D Newton solve: n diracs uniform in [0,1]^3, lognormal(0, 0.5) voxel density on an
     img_size^3 grid normalised to mass 1, w0 = 0, uniform targets, stop at 1e-6 relative mass error."""
     import sdot_amd
@@ -102,7 +106,7 @@ def run_c3(n, img_size, steps):
     img = workloads.lognormal_image(img_size, 3, seed=1)
     t = np.full(n, img.mean() / n)
     out = {}
-    for prec in ("fp64", "mixed"):
+    for prec in precisions:
         pd = sdot_amd.PowerDiagram(3, prec)
         pd.set_image_density([0, 0, 0], [1, 1, 1], img)
         pd.set_diracs(pos, np.zeros(n))
@@ -122,14 +126,23 @@ def run_c3(n, img_size, steps):
 
 def main_c3(args):
     from oracle import oracle as O
-    res, pos, img = run_c3(args.n, args.img, args.steps)
-    w = res["fp64_weights"]
-    sample = min(args.n, 4000)
-    dt, st, _ = O.cells_sample_seconds(pos, w, 0, sample, img=img, nthreads=min(16, os.cpu_count() or 1))
-    cpu_cells = sample / dt
-    iters = res["fp64"]["newton_iters"]
-    cpu_solve = iters * args.n / cpu_cells
-    best = min(("fp64", "mixed"), key=lambda p: res[p]["seconds"] if res[p]["status"] == 0 else 1e30)
+    precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
+    res, pos, img = run_c3(args.n, args.img, args.steps, precs)
+    best = min(precs, key=lambda p: res[p]["seconds"] if res[p]["status"] == 0 else 1e30)
+    cpu = None
+    if not args.no_cpu_baseline:
+        w = res[best + "_weights"]
+        sample = min(args.n, 4000)
+        dt, st, _ = O.cells_sample_seconds(pos, w, 0, sample, img=img, nthreads=min(16, os.cpu_count() or 1))
+        cpu_cells = sample / dt
+        iters = res[best]["newton_iters"]
+        cpu_solve = iters * args.n / cpu_cells
+        cpu = {"value": cpu_cells, "unit": "cells/s (one image pass)", "cores": min(16, os.cpu_count() or 1),
+               "kind": "port",
+               "sample": f"oracle cell pass (voxel pieces, Hessian rows) on {sample} of the {args.n} cells at "
+                         f"the converged weights: {dt:.2f} s",
+               "solve_seconds_estimate": cpu_solve,
+               "estimate": f"{iters} Newton passes x N / (cells/s), CPU linear solves not counted"}
     line = {
         "metric": "solve s @1e-6 mass err (3D Newton, 1M diracs, 64^3 lognormal density)",
         "value": res[best]["seconds"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
@@ -139,15 +152,10 @@ def main_c3(args):
         "config": {"workload": f"3D Newton OT solve, N={args.n} uniform in [0,1]^3, {args.img}^3 lognormal(0,0.5) "
                                "voxel density (mass 1), w0=0, uniform targets, stop at max relative mass error 1e-6",
                    "n_diracs": args.n, "dim": 3, "precision": best},
-        "solves": {p: {k: v for k, v in res[p].items()} for p in ("fp64", "mixed")},
-        "mixed_speedup_vs_fp64": res["fp64"]["seconds"] / res["mixed"]["seconds"],
-        "cpu_baseline": {"value": cpu_cells, "unit": "cells/s (one image pass)", "cores": min(16, os.cpu_count() or 1),
-                         "kind": "port",
-                         "sample": f"oracle cell pass (voxel pieces, Hessian rows) on {sample} of the {args.n} cells at "
-                                   f"the converged weights: {dt:.2f} s",
-                         "solve_seconds_estimate": cpu_solve,
-                         "estimate": f"{iters} Newton passes x N / (cells/s), CPU linear solves not counted"},
-        "speedup_vs_cpu_estimate": cpu_solve / res[best]["seconds"],
+        "solves": {p: {k: v for k, v in res[p].items()} for p in precs},
+        "mixed_speedup_vs_fp64": (res["fp64"]["seconds"] / res["mixed"]["seconds"]) if len(precs) == 2 else None,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu_estimate": (cpu["solve_seconds_estimate"] / res[best]["seconds"]) if cpu else None,
     }
     print(json.dumps(line))
 
@@ -321,6 +329,7 @@ def main():
     ap.add_argument("--drain", type=float, default=50.0)
     ap.add_argument("--sustain", type=float, default=30.0)
     ap.add_argument("--problems", type=int, default=512)
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--img", type=int, default=64)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -387,7 +396,7 @@ def main():
                                "unit-square domain; grid + cells + Hessian CSR per step",
                    "n_diracs": args.n, "dim": 2, "precision": best},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[best], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[best], "traffic": None,
+                     "frac": achieved / PEAK[best], "traffic": TRAFFIC.get(best) if args.n == 1_000_000 else None,
                      "kernel": "k_cells2", "flops_per_launch": b["cell_flops"],
                      "kernel_ms": b["ms_cell_kernel"]},
         "cpu_baseline": cpu,

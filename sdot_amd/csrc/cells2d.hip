@@ -59,8 +59,10 @@ __global__ void k_csr_count(const int *row_len, const int *inv_perm, int64_t n, 
     if (i < n) cnt[i] = row_len[inv_perm[i]] + 1;
 }
 
-// one row per thread: gather the ELL row (columns mapped to dirac indices) and
-// the diagonal into this thread's LDS slot, insertion-sort by column, write
+// one row per thread, threads in grid order (the ELL slots [e * stride + t] of a
+// wavefront are contiguous): gather the row (columns mapped to dirac indices) and
+// the diagonal into this thread's LDS slot, insertion-sort by column, write the
+// dirac-order row i = perm[t] at its CSR offset
 template <int MAXE, int BS>
 __global__ __launch_bounds__(BS) void k_csr_fill(const int *row_len, const int *ell_col, const double *ell_val,
                                                  int64_t stride, const double *diag, const int *perm,
@@ -68,11 +70,12 @@ __global__ __launch_bounds__(BS) void k_csr_fill(const int *row_len, const int *
                                                  double *vals) {
     __shared__ int sc[MAXE * BS];
     __shared__ double sv[MAXE * BS];
-    const int64_t i = blockIdx.x * (int64_t)BS + threadIdx.x;
-    if (i >= n) return;
+    const int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x;
+    if (t >= n) return;
+    (void)inv_perm;
     int *c = sc + threadIdx.x;
     double *v = sv + threadIdx.x;
-    const int64_t t = inv_perm[i];
+    const int64_t i = perm[t];
     const int L = row_len[t];
     int m = 0;
     auto insert = [&](int cc, double vv) {

@@ -184,28 +184,38 @@ __device__ inline bool skip(const double *S, const int *amg_off, int p) {
     return S[p * S_NS + S_DONE] != 0 || amg_off[p];
 }
 
-// pre-smooth level l from zero (x = w r / D) and restrict its residual to level l+1
-__global__ __launch_bounds__(BB) void kp_down(LevelView F, LevelView C, const double *S, const int *amg_off) {
+// pre-smooth level l from zero (x = w r / D) and its residual r - A x, one thread per
+// row (rows of problems that have a coarser level)
+__global__ __launch_bounds__(BB) void kp_smooth_res(LevelView F, const int *plev, double *res, const double *S,
+                                                    const int *amg_off) {
+    const int64_t t = blockIdx.x * (int64_t)BB + threadIdx.x;
+    if (t >= F.n) return;
+    const int p = F.prob[t];
+    if (skip(S, amg_off, p) || plev[p] <= F.lev) return;
+    const double rt = F.r[t], dt = F.A.diag[t];
+    const double xt = OMEGA * rt / dt;
+    F.xpre[t] = xt;
+    double ax = dt * xt;
+    const int L = F.A.len[t];
+    for (int e = 0; e < L; ++e) {
+        const int c = F.A.col[e * F.A.stride + t];
+        ax += F.A.val[e * F.A.stride + t] * (OMEGA * F.r[c] / F.A.diag[c]);
+    }
+    res[t] = rt - ax;
+}
+
+// restriction: coarse row I = sum of the residuals of its aggregate, in row order
+__global__ __launch_bounds__(BB) void kp_restrict(LevelView F, LevelView C, const double *res, const double *S,
+                                                  const int *amg_off) {
     const int64_t I = blockIdx.x * (int64_t)BB + threadIdx.x;
     if (I >= C.n) return;
     const int p = C.prob[I];
     if (skip(S, amg_off, p)) return;
     const int64_t f0 = F.off[p] + AGG * (I - C.off[p]);
     const int64_t f1 = f0 + AGG < F.off[p + 1] ? f0 + AGG : F.off[p + 1];
-    double res = 0;
-    for (int64_t t = f0; t < f1; ++t) {
-        const double rt = F.r[t], dt = F.A.diag[t];
-        const double xt = OMEGA * rt / dt;
-        F.xpre[t] = xt;
-        double ax = dt * xt;
-        const int L = F.A.len[t];
-        for (int e = 0; e < L; ++e) {
-            const int c = F.A.col[e * F.A.stride + t];
-            ax += F.A.val[e * F.A.stride + t] * (OMEGA * F.r[c] / F.A.diag[c]);
-        }
-        res += rt - ax;
-    }
-    C.r[I] = res;
+    double sum = 0;
+    for (int64_t t = f0; t < f1; ++t) sum += res[t];
+    C.r[I] = sum;
 }
 
 // coarsest level of each problem: out = inverse * r (level 0 problems write z)
@@ -418,7 +428,7 @@ struct PcgSolver::Impl {
     std::vector<int64_t> off, blk_off;
     DevBuf<int64_t> doff, dblk;
     DevBuf<int> blk_prob, row_prob;
-    DevBuf<double> r, z, p, q, red, S, xpre0;
+    DevBuf<double> r, z, p, q, red, S, xpre0, res;
     // multigrid: levels 1..L (level 0 is the caller's matrix)
     struct Level {
         int64_t n = 0, base = 0;
@@ -458,7 +468,7 @@ PcgSolver::~PcgSolver() {
         if (e) (void)hipEventDestroy(e);
     for (auto *b : {&im->doff, &im->dblk, &im->dcoff, &im->dcbase}) b->release();
     for (auto *b : {&im->blk_prob, &im->row_prob, &im->dplev, &im->dcn, &im->amg_off}) b->release();
-    for (auto *b : {&im->r, &im->z, &im->p, &im->q, &im->red, &im->S, &im->xpre0, &im->vr, &im->vxpre, &im->vout, &im->inv})
+    for (auto *b : {&im->r, &im->z, &im->p, &im->q, &im->red, &im->S, &im->xpre0, &im->res, &im->vr, &im->vxpre, &im->vout, &im->inv})
         b->release();
     for (auto &l : im->lv) {
         l.doff.release();
@@ -499,7 +509,7 @@ void PcgSolver::set_layout(const std::vector<int64_t> &off, hipStream_t s) {
     hipLaunchKernelGGL(kp_owner, dim3(nblocks(I.nblk, BB)), dim3(BB), 0, s, I.dblk.p, K, I.nblk, I.blk_prob.p);
     hipLaunchKernelGGL(kp_owner, dim3(nblocks(N, BB)), dim3(BB), 0, s, I.doff.p, K, N, I.row_prob.p);
     HIPCK(hipGetLastError());
-    for (auto *b : {&I.r, &I.z, &I.p, &I.q, &I.xpre0}) b->ensure(N);
+    for (auto *b : {&I.r, &I.z, &I.p, &I.q, &I.xpre0, &I.res}) b->ensure(N);
     I.red.ensure(2 * I.nblk + 16);
     I.S.ensure(K * S_NS);
     // multigrid levels: problem sizes shrink by AGG until at most CMIN rows
@@ -593,8 +603,10 @@ void apply_prec(PcgSolver::Impl &I, const PcgMat &A0, bool amg, bool init) {
     const int L = amg ? I.L : 0;
     for (int l = 0; l < L; ++l) {
         LevelView F = level_view(I, A0, l, I.r.p, I.z.p), C = level_view(I, A0, l + 1, I.r.p, I.z.p);
-        if (C.n > 0)
-            hipLaunchKernelGGL(kp_down, dim3(nblocks(C.n, BB)), dim3(BB), 0, s, F, C, I.S.p, I.amg_off.p);
+        if (C.n == 0) continue;
+        hipLaunchKernelGGL(kp_smooth_res, dim3(nblocks(F.n, BB)), dim3(BB), 0, s, F, I.dplev.p, I.res.p, I.S.p,
+                           I.amg_off.p);
+        hipLaunchKernelGGL(kp_restrict, dim3(nblocks(C.n, BB)), dim3(BB), 0, s, F, C, I.res.p, I.S.p, I.amg_off.p);
     }
     if (amg)
         hipLaunchKernelGGL(kp_coarse_solve, dim3((unsigned)I.K), dim3(CMIN), 0, s, I.dcbase.p, I.dcn.p, I.dplev.p,
