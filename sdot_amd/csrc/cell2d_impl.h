@@ -194,13 +194,26 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     const GridInfo &g = pv.g;
     const double W = pv.W;
 
-    // starting cell: the domain box, vertices p0, (p1.x,p0.y), p1, (p0.x,p1.y)
+    // starting cell: the domain box, vertices p0, (p1.x,p0.y), p1, (p0.x,p1.y), edges
+    // bottom, right, top, left (ids -1..-4).  In ball mode it is intersected with a
+    // square slightly larger than the disc (sides -5..-8): P & square & disc = P & disc,
+    // and every edge left is short, which keeps the chord/circle intersections of the
+    // ball clip well conditioned.
+    double bx0 = pv.lo[0] - c0x, by0 = pv.lo[1] - c0y, bx1 = pv.hi[0] - c0x, by1 = pv.hi[1] - c0y;
+    int eb = -1, er = -2, et = -3, el = -4;
     int n = 4;
-    PX(0) = T(pv.lo[0] - c0x); PY(0) = T(pv.lo[1] - c0y);
-    PX(1) = T(pv.hi[0] - c0x); PY(1) = T(pv.lo[1] - c0y);
-    PX(2) = T(pv.hi[0] - c0x); PY(2) = T(pv.hi[1] - c0y);
-    PX(3) = T(pv.lo[0] - c0x); PY(3) = T(pv.hi[1] - c0y);
-    for (int k = 0; k < 4; ++k) PI_(k) = -1 - k;   // box sides: bottom, right, top, left
+    if (BALL) {
+        const double sq = ball_square(w0);
+        if (-sq > by0) { by0 = -sq; eb = -8; }
+        if (sq < bx1) { bx1 = sq; er = -5; }
+        if (sq < by1) { by1 = sq; et = -6; }
+        if (-sq > bx0) { bx0 = -sq; el = -7; }
+        if (!(w0 > 0) || !(bx0 < bx1) || !(by0 < by1)) n = 0;   // no disc, or disjoint from the box
+    }
+    PX(0) = T(bx0); PY(0) = T(by0); PI_(0) = eb;
+    PX(1) = T(bx1); PY(1) = T(by0); PI_(1) = er;
+    PX(2) = T(bx1); PY(2) = T(by1); PI_(2) = et;
+    PX(3) = T(bx0); PY(3) = T(by1); PI_(3) = el;
 
     auto radius2 = [&]() {
         double R2 = 0;
@@ -290,19 +303,6 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     double R2 = radius2();
     bool overflow = false;
     bool bad = false;
-
-    if (BALL && !(w0 > 0)) n = 0;   // disc of radius sqrt(w0) <= 0: empty cell
-    if (BALL && n > 0) {
-        // Clip by a square slightly larger than the disc: P & square & disc =
-        // P & disc, and every edge left is short, which keeps the chord/circle
-        // intersections of the ball clip well conditioned.
-        const T sq = T(ball_square(w0));
-        const double sqd = ball_square(w0);
-        if (cut(T(1), T(0), sq, -5, 1.0, 0.0, sqd) == 2 || cut(T(0), T(1), sq, -6, 0.0, 1.0, sqd) == 2 ||
-            cut(T(-1), T(0), sq, -7, -1.0, 0.0, sqd) == 2 || cut(T(0), T(-1), sq, -8, 0.0, -1.0, sqd) == 2)
-            overflow = true;
-        R2 = radius2();
-    }
 
     const int hx = grid_coord(g, 0, c0x), hy = grid_coord(g, 1, c0y);
     const int maxr = g.ncell[0] > g.ncell[1] ? g.ncell[0] : g.ncell[1];

@@ -19,6 +19,7 @@
 // offsets, distances and the mass/Hessian outputs in FP64).
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 
 #include "cell2d_impl.h"
 

@@ -136,7 +136,7 @@ enum : unsigned { FLAG_EMPTY = 1u, FLAG_ROW_OVERFLOW = 2u, FLAG_BAD_INPUT = 4u, 
 
 // Hessian row capacity (ELL slots per row): the first pass uses the narrow one;
 // a row overflow re-runs the pass with the wide one, which the handle then keeps
-constexpr int ELL_CAP2 = 32, ELL_CAP2_WIDE = 64, ELL_CAP3 = 96, ELL_CAP3_WIDE = 255;
+constexpr int ELL_CAP2 = 32, ELL_CAP2_WIDE = 64, ELL_CAP3 = 128, ELL_CAP3_WIDE = 255;
 
 #define SDOT_HD __host__ __device__
 
