@@ -90,7 +90,9 @@ sdot_status sdot_pd_compute(sdot_pd *pd);
 /* get_integrals(res, ...) -- get_integrals.h:13.  Runs the pass if needed. */
 sdot_status sdot_pd_integrals(sdot_pd *pd, double *masses);
 /* per-cell boundary measures: ball part (arcs with cut id i) and whole perimeter
- * (area of the boundary in 3D); either may be NULL */
+ * (area of the boundary in 3D); either may be NULL.  With an image density the
+ * perimeter is the density-weighted measure of the faces shared with other
+ * cells only (the domain boundary is not counted). */
 sdot_status sdot_pd_boundary_measures(sdot_pd *pd, double *ball_arc, double *perimeter);
 /* get_der_integrals_wrt_weights(offsets, cols, vals, masses, ...) -- get_der_integrals_wrt_weights.h:16.
  * Two-phase: with cols == NULL only *nnz is written.  offsets has N+1 entries;

@@ -574,7 +574,7 @@ static void piece2_accumulate(const cells_job *J, int64_t i, const or_cp2 *cp, d
     int m = or_cp2_boundary_items(cp, coeff, meas, ids, 2 * OR_CP2_CAP + 2);
     double wi = J->w[i];
     for (int k = 0; k < m; ++k) {
-        *perim += meas[k];
+        if (!J->dom->img || ids[k] != -1) *perim += meas[k];   /* image density: pixel sides are not boundary */
         if (ids[k] == -1) continue;
         if (ids[k] == i) {
             *arcm += meas[k];

@@ -107,7 +107,9 @@ typedef struct or_domain {
 /* Per-cell evaluation (get_der_integrals_wrt_weights.h:16-122 restated).
  *  masses[N]            integral of the density over cell i
  *  ball_arc[N]          measure of the ball part of the cell boundary (may be NULL)
- *  perimeter[N]         measure of the whole cell boundary (may be NULL)
+ *  perimeter[N]         measure of the whole cell boundary (may be NULL); with an
+ *                       image density, the density-weighted measure of the faces
+ *                       shared with other cells (and of the ball part)
  *  row_len[N], cols[N*cap], vals[N*cap]:  unsorted raw row of cell i:
  *        slot 0 = diagonal (col i), then (j, -0.5*rho*|G_ij|/|x_i-x_j|) per face
  *  flags: bit0 ball clip (radius sqrt(w_i)); bit1 brute-force candidates (all j)

@@ -181,7 +181,44 @@ SDOT_HD inline void line_of(const CellArgs &a, const ProbView &pv, int id, doubl
 // Builds and integrates cell t.  The polygon lives at X/Y/I with stride S
 // (an LDS slot on the device).  Returns 1 when the polygon outgrows CAP
 // vertices (nothing is written then), 0 otherwise.
-template <class T, int CAP, bool BALL>
+// a polygon with one id per edge (edge k runs from vertex k to vertex k+1)
+template <int PC>
+struct IdPoly2 {
+    double x[PC], y[PC];
+    int e[PC];
+    int n;
+};
+
+// one side of a convex polygon clip (Sutherland-Hodgman keeping edge ids): keeps
+// s * (bound - coord) >= 0, the new edge along the clipping line gets `id`;
+// returns false if the result would not fit
+template <int PC>
+SDOT_HD inline bool clip_axis(const IdPoly2<PC> &in, IdPoly2<PC> &out, int axis, double bound, double s, int id) {
+    int m = 0;
+    bool fits = true;
+    auto put = [&](double x, double y, int e) {
+        if (m < PC) { out.x[m] = x; out.y[m] = y; out.e[m] = e; ++m; } else fits = false;
+    };
+    for (int k = 0; k < in.n; ++k) {
+        const int k1 = k + 1 == in.n ? 0 : k + 1;
+        const double dk = s * (bound - (axis == 0 ? in.x[k] : in.y[k]));
+        const double d1 = s * (bound - (axis == 0 ? in.x[k1] : in.y[k1]));
+        if (dk >= 0) {
+            put(in.x[k], in.y[k], in.e[k]);
+            if (d1 < 0) {
+                const double u = dk / (dk - d1);
+                put(in.x[k] + u * (in.x[k1] - in.x[k]), in.y[k] + u * (in.y[k1] - in.y[k]), id);
+            }
+        } else if (d1 >= 0) {
+            const double u = d1 / (d1 - dk);
+            put(in.x[k1] + u * (in.x[k] - in.x[k1]), in.y[k1] + u * (in.y[k] - in.y[k1]), in.e[k]);
+        }
+    }
+    out.n = m;
+    return fits;
+}
+
+template <class T, int CAP, bool BALL, bool IMG = false>
 SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, int *I, const int S) {
 #define PX(k) X[(k) * S]
 #define PY(k) Y[(k) * S]
@@ -382,6 +419,102 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         }
     };
     bool full_disc = false;
+    if (IMG) {
+        // pixel pieces (ScaledImage::for_each_intersection, ScaledImage.tcc:98-149): the cell
+        // clipped by every pixel that overlaps its bounding box, each piece integrated with
+        // the pixel's density; a neighbour's entry sums density x length over its edge's pieces
+        constexpr int PC = CAP + 8;
+        IdPoly2<PC> cell, pa, pb;
+        double edge_rho_len[CAP];
+        double mn[2] = {DBL_MAX, DBL_MAX}, mx[2] = {-DBL_MAX, -DBL_MAX};
+        cell.n = n;
+        for (int k = 0; k < n; ++k) {
+            vtx(k, cell.x[k], cell.y[k]);
+            cell.e[k] = PI_(k);
+            edge_rho_len[k] = 0;
+            mn[0] = fmin(mn[0], cell.x[k]); mx[0] = fmax(mx[0], cell.x[k]);
+            mn[1] = fmin(mn[1], cell.y[k]); mx[1] = fmax(mx[1], cell.y[k]);
+        }
+        const double c0[2] = {c0x, c0y};
+        int i0[2] = {0, 0}, i1[2] = {-1, -1};
+        double ps[2] = {0, 0};
+        for (int d = 0; d < 2 && n > 0; ++d) {
+            ps[d] = (a.dom_hi[d] - a.dom_lo[d]) / a.img_n[d];
+            const int u0 = (int)floor((mn[d] + c0[d] - a.dom_lo[d]) / ps[d]);
+            const int u1 = (int)floor((mx[d] + c0[d] - a.dom_lo[d]) / ps[d]);
+            i0[d] = u0 < 0 ? 0 : (u0 >= a.img_n[d] ? a.img_n[d] - 1 : u0);
+            i1[d] = u1 < 0 ? 0 : (u1 >= a.img_n[d] ? a.img_n[d] - 1 : u1);
+        }
+        double mass_sum = 0;
+        bool fits = true;
+        for (int py = i0[1]; py <= i1[1]; ++py)
+            for (int px = i0[0]; px <= i1[0]; ++px) {
+                const double rho = a.img[(int64_t)py * a.img_n[0] + px];
+                // sides of the pixel that lie inside the bounding box cut the cell; the
+                // others cannot, so the piece of an edge pixel keeps the cell's own edges
+                const IdPoly2<PC> *src = &cell;
+                IdPoly2<PC> *dst = &pa;
+                auto clip = [&](int axis, double bound, double sgn) {
+                    fits &= clip_axis(*src, *dst, axis, bound, sgn, -9);
+                    src = dst;
+                    dst = dst == &pa ? &pb : &pa;
+                };
+                if (px > i0[0]) clip(0, a.dom_lo[0] + ps[0] * px - c0x, -1.0);
+                if (px < i1[0]) clip(0, a.dom_lo[0] + ps[0] * (px + 1) - c0x, 1.0);
+                if (py > i0[1]) clip(1, a.dom_lo[1] + ps[1] * py - c0y, -1.0);
+                if (py < i1[1]) clip(1, a.dom_lo[1] + ps[1] * (py + 1) - c0y, 1.0);
+                const IdPoly2<PC> &q = *src;
+                double a2 = 0;
+                for (int k = 0; k < q.n; ++k) {
+                    const int k1 = k + 1 == q.n ? 0 : k + 1;
+                    a2 += q.x[k] * q.y[k1] - q.x[k1] * q.y[k];
+                    if (q.e[k] < 0) continue;
+                    const double ex = q.x[k1] - q.x[k], ey = q.y[k1] - q.y[k];
+                    const double len = sqrt(ex * ex + ey * ey);
+                    for (int j = 0; j < n; ++j)
+                        if (cell.e[j] == q.e[k]) { edge_rho_len[j] += rho * len; break; }
+                }
+                mass_sum += rho * 0.5 * a2;
+            }
+        if (!fits) flag_or(a.flags, FLAG_CELL_OVERFLOW);
+        double der_line = 0, perim = 0;
+        int nrow = 0;
+        bool row_ovf = false;
+        for (int k = 0; k < n; ++k) {
+            const int id = cell.e[k];
+            if (id < 0) continue;
+            perim += edge_rho_len[k];   // image density: faces shared with neighbours only
+            const double vx = a.spos[2 * (int64_t)id] - c0x, vy = a.spos[2 * (int64_t)id + 1] - c0y;
+            const double dist = sqrt(vx * vx + vy * vy);
+            if (dist == 0) continue;
+            const double b = 0.5 * edge_rho_len[k] / dist;
+            if (nrow < a.ell_cap) {
+                a.ell_col[nrow * a.ell_stride + t] = id;
+                a.ell_val[nrow * a.ell_stride + t] = -b;
+                ++nrow;
+            } else {
+                row_ovf = true;
+            }
+            der_line += b;
+        }
+        a.mass[t] = mass_sum;
+        a.arc[t] = 0;
+        a.perim[t] = perim;
+        a.diag[t] = der_line;
+        a.row_len[t] = nrow;
+        if (a.ctr) {
+            const unsigned long long K = n;
+            const int slot = (int)(t % WCTR);
+            ctr_add(a.ctr + slot, 4 * K * K + 37 * K);
+            ctr_add(a.ctr + WCTR + slot, K);
+        }
+        if (row_ovf) flag_or(a.flags, FLAG_ROW_OVERFLOW);
+        if (mass_sum == 0) {
+            flag_or(a.flags, FLAG_EMPTY);
+            if (a.pflags) flag_or(a.pflags + pv.p, FLAG_EMPTY);
+        }
+        return 0;
+    }
     if (!BALL) {
         for (int k = 0; k < n; ++k) {
             double x, y;

@@ -29,7 +29,7 @@ namespace {
 
 using namespace c2;
 
-template <class T, int CAP, int BS, bool BALL>
+template <class T, int CAP, int BS, bool BALL, bool IMG>
 __global__ __launch_bounds__(BS) void k_cells2(CellArgs a) {
     __shared__ T sx[CAP * BS];
     __shared__ T sy[CAP * BS];
@@ -43,7 +43,7 @@ __global__ __launch_bounds__(BS) void k_cells2(CellArgs a) {
         t = xcd_tile(blockIdx.x, gridDim.x) * BS + threadIdx.x;
         if (t >= a.N) return;
     }
-    if (cell2_body<T, CAP, BALL>(a, t, sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x, BS)) {
+    if (cell2_body<T, CAP, BALL, IMG>(a, t, sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x, BS)) {
         unsigned k = atomicAdd(a.ovf_count, 1u);
         a.ovf_list[k] = (int)t;
     }
@@ -109,17 +109,31 @@ __global__ __launch_bounds__(BS) void k_csr_fill(const int *row_len, const int *
     }
 }
 
-template <class T, int CAP, int BS, bool BALL>
+template <class T, int CAP, int BS, bool BALL, bool IMG>
 void launch2(const CellArgs &a, int64_t nthreads, hipStream_t s) {
     unsigned nb = (unsigned)((nthreads + BS - 1) / BS);
     if (nb == 0) return;
-    hipLaunchKernelGGL((k_cells2<T, CAP, BS, BALL>), dim3(nb), dim3(BS), 0, s, a);
+    hipLaunchKernelGGL((k_cells2<T, CAP, BS, BALL, IMG>), dim3(nb), dim3(BS), 0, s, a);
     HIPCK(hipGetLastError());
+}
+
+// ball and image density exclude each other (rejected before the pass)
+template <int CAP>
+void dispatch2(bool mixed, bool ball, bool img, const CellArgs &a, int64_t nthreads, hipStream_t s) {
+    if (mixed) {
+        if (ball) launch2<float, CAP, 64, true, false>(a, nthreads, s);
+        else if (img) launch2<float, CAP, 64, false, true>(a, nthreads, s);
+        else launch2<float, CAP, 64, false, false>(a, nthreads, s);
+    } else {
+        if (ball) launch2<double, CAP, 64, true, false>(a, nthreads, s);
+        else if (img) launch2<double, CAP, 64, false, true>(a, nthreads, s);
+        else launch2<double, CAP, 64, false, false>(a, nthreads, s);
+    }
 }
 
 }  // namespace
 
-// dispatch over precision / ball: main pass with a small LDS polygon, then the
+// dispatch over precision / ball / image: main pass with a small LDS polygon, then the
 // overflowed cells again with a wide one
 void run_cells2(sdot_pd *pd, CellArgs &a) {
     hipStream_t s = pd->stream;
@@ -127,13 +141,8 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
     const bool mixed = pd->precision == SDOT_MIXED;
     a.work = nullptr;
     a.work_n = 0;
-    if (mixed) {
-        if (ball) launch2<float, 16, 64, true>(a, a.N, s);
-        else launch2<float, 16, 64, false>(a, a.N, s);
-    } else {
-        if (ball) launch2<double, 16, 64, true>(a, a.N, s);
-        else launch2<double, 16, 64, false>(a, a.N, s);
-    }
+    const bool img = a.img != nullptr;
+    dispatch2<16>(mixed, ball, img, a, a.N, s);
     HIPCK(hipEventRecord(pd->ev[3], s));
     unsigned cnt[2] = {0, 0};
     HIPCK(hipMemcpyAsync(cnt, pd->flags.p, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
@@ -148,13 +157,7 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
         HIPCK(hipMemcpyAsync(pd->flags.p + 1, &zero, sizeof(unsigned), hipMemcpyHostToDevice, s));
         a.work = list2;
         a.work_n = cnt[1];
-        if (mixed) {
-            if (ball) launch2<float, 64, 64, true>(a, cnt[1], s);
-            else launch2<float, 64, 64, false>(a, cnt[1], s);
-        } else {
-            if (ball) launch2<double, 64, 64, true>(a, cnt[1], s);
-            else launch2<double, 64, 64, false>(a, cnt[1], s);
-        }
+        dispatch2<64>(mixed, ball, img, a, cnt[1], s);
         unsigned c2 = 0;
         HIPCK(hipMemcpyAsync(&c2, pd->flags.p + 1, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         HIPCK(hipStreamSynchronize(s));

@@ -95,3 +95,65 @@ def test_cells2d_density_coefficient_and_domain():
         m = pd.integrals()
         o = __import__("oracle.oracle", fromlist=["x"]).cells(p, w, domain=(lo, hi, coeff))
         assert np.abs(m - o["masses"]).max() < 1e-12 * coeff * np.prod(np.array(hi) - lo)
+
+
+def lognormal_image_2d(nx, ny, seed=5):
+    z = np.random.default_rng(seed).normal(size=(ny, nx))
+    img = np.exp(0.5 * z)
+    return img / img.mean()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("shape", [(5, 7), (64, 48), (300, 300)])
+def test_cells2d_parity_image(precision, shape):
+    """pixel density (ScaledImage::for_each_intersection): pieces clipped per pixel"""
+    n = 20000
+    pos, w = inputs(2, n, 6, wscale=0.3 / n)
+    img = lognormal_image_2d(*shape)
+    g = gpu_pass(pos, w, 2, precision, img=img)
+    o = oracle_pass(pos, w, 2, img=img)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    assert abs(g["masses"].sum() - img.mean()) < 1e-10
+    print(f"[parity2d img {shape}] {precision}: {err}")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cells2d_image_strip_closed_form(precision):
+    # densities a, b, c in three pixel columns; the shared edge x = 0.5 + delta
+    # lies in the middle pixel: masses, Hessian and perimeter in closed form
+    a, b, c = 1.5, 2.0, 0.5
+    img = np.array([[a, b, c]])
+    pos = np.array([[0.25, 0.5], [0.75, 0.5]])
+    for delta in (0.0, 0.04, -0.1):
+        g = gpu_pass(pos, np.array([delta, 0.0]), 2, precision, img=img)
+        xe = 0.5 + delta
+        assert np.allclose(g["masses"], [a / 3 + b * (xe - 1 / 3), b * (2 / 3 - xe) + c / 3], atol=1e-14)
+        M = csr_matrix(g["csr"], 2).toarray()
+        assert np.allclose(M, [[b, -b], [-b, b]], atol=1e-13)
+        assert np.allclose(g["perimeter"], [b, b], atol=1e-13)
+
+
+def test_cells2d_image_with_ball_is_rejected():
+    import sdot_amd
+    pd = sdot_amd.PowerDiagram(2, ball=True)
+    pd.set_image_density([0, 0], [1, 1], np.ones((4, 4)))
+    pd.set_diracs(np.array([[0.3, 0.4], [0.6, 0.7]]), np.full(2, 0.01))
+    with pytest.raises(sdot_amd.SdotError):
+        pd.compute()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_newton_2d_image(precision):
+    import sdot_amd
+    from oracle import oracle as O
+    n = 20000
+    pos, _ = inputs(2, n, 9)
+    img = lognormal_image_2d(96, 80)
+    t = np.full(n, img.mean() / n)
+    ot = sdot_amd.OptimalTransport(pos, t, precision=precision,
+                                   domain=lambda pd: pd.set_image_density([0, 0], [1, 1], img))
+    w = ot.adjust_weights(tol=1e-6)
+    assert ot.status == 0, ot.stats
+    m = O.cells(pos, w, img=img)["masses"]
+    assert np.abs(m / t - 1).max() < 1.5e-6
+    print(f"[newton2d img] {precision}: {ot.stats}")

@@ -111,6 +111,25 @@ def test_image_density_closed_form_2d():
     assert abs(r["masses"].sum() - img.mean()) < 1e-12
 
 
+@pytest.mark.parametrize("delta", [0.0, 0.04, -0.1])
+def test_image_density_2d_strip_closed_form(delta):
+    # three pixel columns of densities a, b, c; two diracs at x = 0.25, 0.75 with
+    # w0 - w1 = delta put the shared edge at x = 0.5 + delta inside the middle pixel
+    a, b, c = 1.5, 2.0, 0.5
+    img = np.array([[a, b, c]])                       # [y][x]
+    pos = np.array([[0.25, 0.5], [0.75, 0.5]])
+    r = O.cells(pos, np.array([delta, 0.0]), img=img)
+    xe = 0.5 + delta
+    m0 = a / 3 + b * (xe - 1 / 3)
+    m1 = b * (2 / 3 - xe) + c / 3
+    assert np.allclose(r["masses"], [m0, m1], atol=1e-14)
+    M = csr(*O.to_csr(r), 2).toarray()
+    # edge of length 1 in density b, dirac distance 0.5: M01 = -1/2 * b / 0.5
+    assert np.allclose(M, [[b, -b], [-b, b]], atol=1e-14)
+    # only the shared edge is boundary, weighted by its density
+    assert np.allclose(r["perimeter"], [b, b], atol=1e-14)
+
+
 def test_image_density_3d_sum_and_hessian():
     rng = np.random.default_rng(1)
     img = np.exp(0.5 * rng.normal(size=(4, 5, 3)))
