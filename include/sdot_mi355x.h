@@ -74,7 +74,10 @@ sdot_status sdot_pd_mark_diracs_changed(sdot_pd *pd, int positions_changed);
  * One box item is supported on the device path (the configuration every
  * workload uses); several items return SDOT_NOT_SUPPORTED at evaluation. */
 sdot_status sdot_pd_add_box_shape(sdot_pd *pd, const double *p0, const double *p1, double coeff, int64_t cut_id);
-/* ScaledImage(min_pt, max_pt, data, sizes, nb_coeffs) -- ScaledImage.h:26; sizes x fastest; nb_coeffs = 1 */
+/* ScaledImage(min_pt, max_pt, data, sizes, nb_coeffs) -- ScaledImage.h:26; sizes x fastest.
+ * nb_coeffs = 1: constant per pixel.  nb_coeffs = 6 (2D only, no ball): the density in
+ * pixel p is c0 + c1 x + c2 y + c3 x^2 + c4 x y + c5 y^2 in domain coordinates, with
+ * c_k = data[k * nb_pixels + p] (apply_poly, Polynomial.h:37-50). */
 sdot_status sdot_pd_set_image_density(sdot_pd *pd, const double *min_pt, const double *max_pt, const double *data,
                                       const int64_t *sizes, int64_t nb_coeffs);
 sdot_status sdot_pd_clear_shapes(sdot_pd *pd);

@@ -22,7 +22,7 @@ _p = ctypes.c_void_p
 
 class Domain(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("p0", _d * 3), ("p1", _d * 3), ("coeff", _d),
-                ("img", _p), ("img_sizes", _i64 * 3)]
+                ("img", _p), ("img_sizes", _i64 * 3), ("img_nc", _i64)]
 
 
 class SolveStats(ctypes.Structure):
@@ -120,12 +120,15 @@ def make_domain(dim, p0=None, p1=None, coeff=1.0, img=None):
     if img is not None:
         keep = np.ascontiguousarray(img, dtype=np.float64)
         d.img = keep.ctypes.data
-        # img indexed [z][y][x] (x fastest)
-        shp = keep.shape[::-1]
+        # img indexed [z][y][x] (x fastest); a leading axis holds the coefficients
+        # of a per-pixel polynomial ([k][y][x], 6 of them in 2D)
+        shp = keep.shape[-dim:][::-1]
         for k in range(dim):
             d.img_sizes[k] = shp[k]
+        d.img_nc = keep.shape[0] if keep.ndim == dim + 1 else 1
     else:
         d.img = None
+        d.img_nc = 1
     return d, keep
 
 

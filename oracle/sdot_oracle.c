@@ -220,6 +220,37 @@ double or_cp2_area(const or_cp2 *cp, double coeff) {
     return coeff * res;
 }
 
+/* integration(Polynomial<6>, Unit), ConvexPolyhedron2.tcc:2181-2225: the integral of
+ * c0 + c1 x + c2 y + c3 x^2 + c4 x y + c5 y^2 over the polygon, from its area moments
+ * (Green's theorem over the edges, cr = x_k y_k+1 - x_k+1 y_k). */
+double or_cp2_poly_area(const or_cp2 *cp, const double *c6) {
+    int n = cp->n;
+    double A = 0, Sx = 0, Sy = 0, Sxx = 0, Sxy = 0, Syy = 0;
+    for (int k = 0; k < n; ++k) {
+        int k1 = k + 1 == n ? 0 : k + 1;
+        double x0 = cp->x[k], y0 = cp->y[k], x1 = cp->x[k1], y1 = cp->y[k1];
+        double cr = x0 * y1 - x1 * y0;
+        A += cr;
+        Sx += (x0 + x1) * cr;
+        Sy += (y0 + y1) * cr;
+        Sxx += (x0 * x0 + x0 * x1 + x1 * x1) * cr;
+        Syy += (y0 * y0 + y0 * y1 + y1 * y1) * cr;
+        Sxy += (x0 * y1 + 2 * x0 * y0 + 2 * x1 * y1 + x1 * y0) * cr;
+    }
+    return c6[0] * A / 2 + c6[1] * Sx / 6 + c6[2] * Sy / 6 + c6[3] * Sxx / 12 + c6[4] * Sxy / 24 + c6[5] * Syy / 12;
+}
+
+static double poly_at(const double *c, double x, double y) {
+    return c[0] + c[1] * x + c[2] * y + c[3] * x * x + c[4] * x * y + c[5] * y * y;
+}
+
+/* for_each_boundary_item(Polynomial<6>, Unit), ConvexPolyhedron2.tcc:180-214: the
+ * integral along a segment, by Simpson's rule (exact for a quadratic) */
+double or_seg_poly(double x0, double y0, double x1, double y1, const double *c6) {
+    double len = sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+    return len * (poly_at(c6, x0, y0) + 4 * poly_at(c6, 0.5 * (x0 + x1), 0.5 * (y0 + y1)) + poly_at(c6, x1, y1)) / 6;
+}
+
 /* for_each_boundary_item(Constant, Unit): ConvexPolyhedron2.tcc:437-478 */
 int or_cp2_boundary_items(const or_cp2 *cp, double coeff, double *measures, int64_t *ids, int cap) {
     int n = cp->n;
@@ -566,12 +597,23 @@ static double dist_between(const double *pos, int dim, int64_t i, int64_t j) {
 
 /* the per-piece part of the get_der_integrals_wrt_weights callback
  * (get_der_integrals_wrt_weights.h:52-75) for one 2D piece of density coeff */
-static void piece2_accumulate(const cells_job *J, int64_t i, const or_cp2 *cp, double coeff, double *mass, double *arcm,
-                              double *perim, row_acc *row) {
+static void piece2_accumulate(const cells_job *J, int64_t i, const or_cp2 *cp, double coeff, const double *poly,
+                              double *mass, double *arcm, double *perim, row_acc *row) {
     static __thread double meas[2 * OR_CP2_CAP + 2];
     static __thread int64_t ids[2 * OR_CP2_CAP + 2];
-    *mass += or_cp2_area(cp, coeff);
-    int m = or_cp2_boundary_items(cp, coeff, meas, ids, 2 * OR_CP2_CAP + 2);
+    int m;
+    if (!poly) {
+        *mass += or_cp2_area(cp, coeff);
+        m = or_cp2_boundary_items(cp, coeff, meas, ids, 2 * OR_CP2_CAP + 2);
+    } else {                                     /* straight edges only (no ball with a polynomial) */
+        *mass += or_cp2_poly_area(cp, poly);
+        m = cp->n;
+        for (int k = 0; k < m; ++k) {
+            int k1 = k + 1 == m ? 0 : k + 1;
+            ids[k] = cp->id[k];
+            meas[k] = or_seg_poly(cp->x[k], cp->y[k], cp->x[k1], cp->y[k1], poly);
+        }
+    }
     double wi = J->w[i];
     for (int k = 0; k < m; ++k) {
         if (!J->dom->img || ids[k] != -1) *perim += meas[k];   /* image density: pixel sides are not boundary */
@@ -672,7 +714,7 @@ static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
     int st = build_cell2(J, i, cp);
     const or_domain *dom = J->dom;
     if (!dom->img) {
-        piece2_accumulate(J, i, cp, dom->coeff, &mass, &arcm, &perim, &row);
+        piece2_accumulate(J, i, cp, dom->coeff, NULL, &mass, &arcm, &perim, &row);
     } else {
         /* ScaledImage::for_each_intersection, ScaledImage.tcc:98-149 */
         double mn[2], mx[2];
@@ -702,7 +744,9 @@ static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
         if (!or_cp2_empty(cp)) {
             for (int64_t py = mi[1]; py < ma[1]; ++py)
                 for (int64_t px = mi[0]; px < ma[0]; ++px) {
-                    double rho = dom->img[py * dom->img_sizes[0] + px];
+                    int64_t pix = py * dom->img_sizes[0] + px, npix = dom->img_sizes[0] * dom->img_sizes[1];
+                    double rho = dom->img[pix], c6[6];
+                    for (int k = 0; k < 6 && dom->img_nc == 6; ++k) c6[k] = dom->img[k * npix + pix];
                     or_cp2_box(piece, dom->p0[0] + ps[0] * px, dom->p0[1] + ps[1] * py,
                                dom->p0[0] + ps[0] * (px + 1), dom->p0[1] + ps[1] * (py + 1), -1);
                     /* intersect_with, ConvexPolyhedron2.tcc:2475-2497 */
@@ -710,7 +754,7 @@ static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
                         or_cp2_plane_cut(piece, cp->x[k], cp->y[k], cp->nx[k], cp->ny[k], cp->id[k], 1);
                     if (cp->sr > 0) or_cp2_ball_cut(piece, cp->scx, cp->scy, cp->sr, cp->sid);
                     else { piece->scx = cp->scx; piece->scy = cp->scy; piece->sr = cp->sr; piece->sid = cp->sid; }
-                    piece2_accumulate(J, i, piece, rho, &mass, &arcm, &perim, &row);
+                    piece2_accumulate(J, i, piece, rho, dom->img_nc == 6 ? c6 : NULL, &mass, &arcm, &perim, &row);
                 }
         }
     }
@@ -923,6 +967,8 @@ int or_cells_range(int dim, int64_t N, const double *pos, const double *w, const
     if (i_begin < 0 || i_end > N || i_begin > i_end) return OR_INVALID;
     if ((dim != 2 && dim != 3) || N < 0 || cap < 1 || !dom || dom->dim != dim) return OR_INVALID;
     if (dim == 3 && (flags & 1)) return OR_INVALID;  /* ConvexPolyhedron3::ball_cut is TODO (:292-293) */
+    if (dom->img && !(dom->img_nc == 1 || (dim == 2 && dom->img_nc == 6 && !(flags & 1))))
+        return OR_INVALID;   /* apply_poly (Polynomial.h:27-58): 6 coefficients in 2D only, no ball (TODO there) */
     if (N == 0) return OR_OK;
     cells_job J;
     memset(&J, 0, sizeof(J));

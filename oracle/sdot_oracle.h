@@ -67,6 +67,8 @@ void    or_cp2_box(or_cp2 *cp, double x0, double y0, double x1, double y1, int64
 int     or_cp2_plane_cut(or_cp2 *cp, double ox, double oy, double nx, double ny, int64_t id, int normal_is_normalized);
 int     or_cp2_ball_cut(or_cp2 *cp, double cx, double cy, double r, int64_t id);
 double  or_cp2_area(const or_cp2 *cp, double coeff);
+double  or_cp2_poly_area(const or_cp2 *cp, const double *c6);   /* integral of the quadratic over a straight-edged polygon */
+double  or_seg_poly(double x0, double y0, double x1, double y1, const double *c6);   /* integral along a segment */
 int     or_cp2_boundary_items(const or_cp2 *cp, double coeff, double *measures, int64_t *ids, int cap);
 int     or_cp2_empty(const or_cp2 *cp);
 /* convenience for ctypes tests: box + k plane cuts (o,n as 4 doubles each) + optional ball */
@@ -102,6 +104,9 @@ typedef struct or_domain {
     double coeff;
     const double *img;       /* NULL: constant box */
     int64_t img_sizes[3];
+    int64_t img_nc;          /* coefficients per pixel: 1 (constant) or, in 2D, 6:
+                              * c0 + c1 x + c2 y + c3 x^2 + c4 x y + c5 y^2 in domain
+                              * coordinates, coefficient k of pixel p at img[k * npix + p] */
 } or_domain;
 
 /* Per-cell evaluation (get_der_integrals_wrt_weights.h:16-122 restated).

@@ -233,13 +233,15 @@ class PowerDiagram:
         self._check(lib().sdot_pd_add_box_shape(self._h, _ptr(a), _ptr(b), float(coeff), int(cut_id)), "add_box_shape")
 
     def set_image_density(self, min_pt, max_pt, data):
-        """data indexed [z][y][x] / [y][x] (x fastest), piecewise constant."""
+        """data indexed [z][y][x] / [y][x] (x fastest), constant per pixel; in 2D an
+        array [6][y][x] gives per-pixel quadratics c0 + c1 x + c2 y + c3 x^2 + c4 xy + c5 y^2."""
         a = np.ascontiguousarray(min_pt, dtype=np.float64)
         b = np.ascontiguousarray(max_pt, dtype=np.float64)
         img = np.ascontiguousarray(data, dtype=np.float64)
-        sizes = np.ascontiguousarray(img.shape[::-1], dtype=np.int64)
+        nc = img.shape[0] if img.ndim == self.dim + 1 else 1
+        sizes = np.ascontiguousarray(img.shape[-self.dim:][::-1], dtype=np.int64)
         self._keep_img = img
-        self._check(lib().sdot_pd_set_image_density(self._h, _ptr(a), _ptr(b), _ptr(img), _ptr(sizes), 1),
+        self._check(lib().sdot_pd_set_image_density(self._h, _ptr(a), _ptr(b), _ptr(img), _ptr(sizes), nc),
                     "set_image_density")
 
     def clear_shapes(self):

@@ -53,6 +53,7 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
 // the domain as seen by the kernels: one box (with density coefficient) or an image
 void domain_args(sdot_pd *pd, CellArgs &a) {
     a.img = nullptr;
+    a.img_nc = 1;
     if (pd->has_image) {
         if (!pd->shapes.empty()) throw StatusError{SDOT_NOT_SUPPORTED, "image density together with shapes"};
         for (int d = 0; d < 3; ++d) {
@@ -62,6 +63,7 @@ void domain_args(sdot_pd *pd, CellArgs &a) {
         }
         a.coeff = 1;
         a.img = pd->img.p;
+        a.img_nc = pd->img_nc;
         return;
     }
     if (pd->shapes.size() != 1)
@@ -346,11 +348,12 @@ sdot_status sdot_pd_add_box_shape(sdot_pd *pd, const double *p0, const double *p
 sdot_status sdot_pd_set_image_density(sdot_pd *pd, const double *min_pt, const double *max_pt, const double *data,
                                       const int64_t *sizes, int64_t nb_coeffs) {
     if (!pd || !min_pt || !max_pt || !data || !sizes) return SDOT_INVALID_ARG;
-    if (nb_coeffs != 1) {
-        pd->err = "only piecewise-constant images (nb_coeffs = 1) are supported";
+    // apply_poly (Polynomial.h:27-58): constant pixels, or 6 coefficients in 2D
+    if (!(nb_coeffs == 1 || (nb_coeffs == 6 && pd->dim == 2))) {
+        pd->err = "image coefficients per pixel: 1, or 6 in 2D";
         return SDOT_NOT_SUPPORTED;
     }
-    int64_t tot = 1;
+    int64_t tot = nb_coeffs;
     for (int d = 0; d < pd->dim; ++d) {
         if (sizes[d] < 1 || sizes[d] > (1 << 20) || !(max_pt[d] > min_pt[d])) return SDOT_INVALID_ARG;
         tot *= sizes[d];
@@ -364,6 +367,7 @@ sdot_status sdot_pd_set_image_density(sdot_pd *pd, const double *min_pt, const d
             pd->img_hi[d] = d < pd->dim ? max_pt[d] : 1;
             pd->img_n[d] = d < pd->dim ? (int)sizes[d] : 1;
         }
+        pd->img_nc = (int)nb_coeffs;
         pd->has_image = true;
         pd->pass_valid = false;
         return SDOT_OK;

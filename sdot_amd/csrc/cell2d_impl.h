@@ -447,9 +447,28 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         }
         double mass_sum = 0;
         bool fits = true;
+        const bool poly = a.img_nc == 6;
+        const int64_t npix = (int64_t)a.img_n[0] * a.img_n[1];
         for (int py = i0[1]; py <= i1[1]; ++py)
             for (int px = i0[0]; px <= i1[0]; ++px) {
-                const double rho = a.img[(int64_t)py * a.img_n[0] + px];
+                const int64_t pix = (int64_t)py * a.img_n[0] + px;
+                const double rho = a.img[pix];
+                // quadratic pixel (Polynomial<6>, Polynomial.h:37-50): the coefficients are
+                // given in domain coordinates; q re-expands them about the dirac
+                double q[6] = {rho, 0, 0, 0, 0, 0};
+                if (poly) {
+                    double c[6];
+                    for (int k = 0; k < 6; ++k) c[k] = a.img[k * npix + pix];
+                    q[0] = c[0] + c0x * (c[1] + c[3] * c0x + c[4] * c0y) + c0y * (c[2] + c[5] * c0y);
+                    q[1] = c[1] + 2 * c[3] * c0x + c[4] * c0y;
+                    q[2] = c[2] + c[4] * c0x + 2 * c[5] * c0y;
+                    q[3] = c[3];
+                    q[4] = c[4];
+                    q[5] = c[5];
+                }
+                auto q_at = [&](double x, double y) {
+                    return q[0] + x * (q[1] + q[3] * x + q[4] * y) + y * (q[2] + q[5] * y);
+                };
                 // sides of the pixel that lie inside the bounding box cut the cell; the
                 // others cannot, so the piece of an edge pixel keeps the cell's own edges
                 const IdPoly2<PC> *src = &cell;
@@ -463,18 +482,33 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                 if (px < i1[0]) clip(0, a.dom_lo[0] + ps[0] * (px + 1) - c0x, 1.0);
                 if (py > i0[1]) clip(1, a.dom_lo[1] + ps[1] * py - c0y, -1.0);
                 if (py < i1[1]) clip(1, a.dom_lo[1] + ps[1] * (py + 1) - c0y, 1.0);
-                const IdPoly2<PC> &q = *src;
-                double a2 = 0;
-                for (int k = 0; k < q.n; ++k) {
-                    const int k1 = k + 1 == q.n ? 0 : k + 1;
-                    a2 += q.x[k] * q.y[k1] - q.x[k1] * q.y[k];
-                    if (q.e[k] < 0) continue;
-                    const double ex = q.x[k1] - q.x[k], ey = q.y[k1] - q.y[k];
+                const IdPoly2<PC> &pc = *src;
+                // area moments of the piece (Green's theorem over its edges)
+                double m0 = 0, sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0;
+                for (int k = 0; k < pc.n; ++k) {
+                    const int k1 = k + 1 == pc.n ? 0 : k + 1;
+                    const double x0 = pc.x[k], y0 = pc.y[k], x1 = pc.x[k1], y1 = pc.y[k1];
+                    const double cr = x0 * y1 - x1 * y0;
+                    m0 += cr;
+                    if (poly) {
+                        sx += (x0 + x1) * cr;
+                        sy += (y0 + y1) * cr;
+                        sxx += (x0 * x0 + x0 * x1 + x1 * x1) * cr;
+                        syy += (y0 * y0 + y0 * y1 + y1 * y1) * cr;
+                        sxy += (x0 * y1 + 2 * x0 * y0 + 2 * x1 * y1 + x1 * y0) * cr;
+                    }
+                    if (pc.e[k] < 0) continue;
+                    const double ex = x1 - x0, ey = y1 - y0;
                     const double len = sqrt(ex * ex + ey * ey);
+                    // density integrated along the edge piece (Simpson, exact for a quadratic)
+                    const double dens = poly ? (q_at(x0, y0) + 4 * q_at(0.5 * (x0 + x1), 0.5 * (y0 + y1)) + q_at(x1, y1)) / 6
+                                             : rho;
                     for (int j = 0; j < n; ++j)
-                        if (cell.e[j] == q.e[k]) { edge_rho_len[j] += rho * len; break; }
+                        if (cell.e[j] == pc.e[k]) { edge_rho_len[j] += dens * len; break; }
                 }
-                mass_sum += rho * 0.5 * a2;
+                mass_sum += poly ? q[0] * m0 / 2 + q[1] * sx / 6 + q[2] * sy / 6 + q[3] * sxx / 12 + q[4] * sxy / 24 +
+                                       q[5] * syy / 12
+                                 : rho * 0.5 * m0;
             }
         if (!fits) flag_or(a.flags, FLAG_CELL_OVERFLOW);
         double der_line = 0, perim = 0;

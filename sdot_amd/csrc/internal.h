@@ -182,6 +182,7 @@ struct CellArgs {
     // piecewise-constant image density (3D voxel / 2D pixel), null when constant
     const double *img;
     int img_n[3];
+    int img_nc;   // coefficients per pixel: 1, or 6 (2D quadratic, coefficient-major)
     // outputs, sorted order
     double *mass, *arc, *perim, *diag;
     int *row_len;
@@ -281,6 +282,7 @@ struct sdot_pd {
     bool has_image = false;
     double img_lo[3] = {0, 0, 0}, img_hi[3] = {1, 1, 1};
     int img_n[3] = {1, 1, 1};
+    int img_nc = 1;
     sdot_amd::DevBuf<double> img;
 
     // diracs (original order)

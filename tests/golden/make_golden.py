@@ -2,7 +2,8 @@
 
 reference_known_values.json holds the known answers that the reference's own
 tests assert for the 2D/3D clipper (data only: boxes, cuts, balls and the
-expected measures / vertex counts), with the file:line each comes from.  The
+expected measures / vertex counts) and for the pixel density (coefficients,
+diracs and expected integrals), with the file:line each comes from.  The
 values are transcribed from those assertions; nothing of the reference is run.
 
 rng_first8.json pins the synthetic-input generator (splitmix64 -> xoshiro256**,
@@ -87,7 +88,15 @@ def known_values():
                     cuts3.append(v + v)
     cp3.append(dict(src="tests/test_ConvexPolyhedron3.cpp:62-76", p0=[-5, -5, -5], p1=[5, 5, 5], cuts=cuts3,
                     volume=4.77730, area=14.3319, tol=1e-4))
-    return dict(cp2=cases, cp3=cp3)
+    # tests/test_scaled_image.cpp:56-92: one pixel over [0,1]^2 carrying the quadratic
+    # 2 + 3x + 4y + 5x^2 + 6xy + 7y^2 (6 coefficients), unit weights
+    img = []
+    img.append(dict(src="tests/test_scaled_image.cpp:56-69", lo=[0, 0], hi=[1, 1], sizes=[1, 1], nb_coeffs=6,
+                    data=[2, 3, 4, 5, 6, 7], positions=[[0.5, 0.5]], weights=[1.0], integrals=[11.0], tol=1e-6))
+    img.append(dict(src="tests/test_scaled_image.cpp:71-92", lo=[0, 0], hi=[1, 1], sizes=[1, 1], nb_coeffs=6,
+                    data=[2, 3, 4, 5, 6, 7], positions=[[0.25, 0.25], [0.75, 0.25], [0.25, 0.75], [0.75, 0.75]],
+                    weights=[1.0] * 4, integrals=[1.28125, 2.46875, 2.84375, 4.40625], tol=1e-6))
+    return dict(cp2=cases, cp3=cp3, scaled_image=img)
 
 
 def rng_first8():

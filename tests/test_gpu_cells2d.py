@@ -157,3 +157,41 @@ def test_newton_2d_image(precision):
     m = O.cells(pos, w, img=img)["masses"]
     assert np.abs(m / t - 1).max() < 1.5e-6
     print(f"[newton2d img] {precision}: {ot.stats}")
+
+
+def test_cells2d_scaled_image_known_values():
+    """the reference's scaled-image answers (one pixel carrying a quadratic)"""
+    import json
+    import os
+    import sdot_amd
+    kv = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_known_values.json")))
+    for case in kv["scaled_image"]:
+        nx, ny = case["sizes"]
+        img = np.array(case["data"], dtype=np.float64).reshape(case["nb_coeffs"], ny, nx)
+        for prec in ("fp64", "mixed"):
+            pd = sdot_amd.PowerDiagram(2, prec)
+            pd.set_image_density(case["lo"], case["hi"], img)
+            pd.set_diracs(np.array(case["positions"]), np.array(case["weights"]))
+            m = pd.integrals()
+            assert np.abs(m - np.array(case["integrals"])).max() < case["tol"], (case["src"], prec, m)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("shape", [(1, 1), (17, 9), (64, 48)])
+def test_cells2d_parity_quadratic_image(precision, shape):
+    n = 20000
+    pos, w = inputs(2, n, 10, wscale=0.3 / n)
+    rng = np.random.default_rng(11)
+    img = rng.uniform(-0.5, 0.5, size=(6,) + shape[::-1])
+    img[0] += 2.0
+    g = gpu_pass(pos, w, 2, precision, img=img)
+    o = oracle_pass(pos, w, 2, img=img)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    print(f"[parity2d quad img {shape}] {precision}: {err}")
+
+
+def test_quadratic_image_rejected_in_3d():
+    import sdot_amd
+    pd = sdot_amd.PowerDiagram(3)
+    with pytest.raises(sdot_amd.SdotError):
+        pd.set_image_density([0, 0, 0], [1, 1, 1], np.ones((6, 2, 2, 2)))

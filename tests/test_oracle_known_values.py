@@ -6,11 +6,16 @@ import os
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from oracle import oracle as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 KV = json.load(open(os.path.join(HERE, "golden", "reference_known_values.json")))
+
+
+def csr(off, cols, vals, n):
+    return sp.csr_matrix((vals, cols, off), shape=(n, n))
 
 
 @pytest.mark.parametrize("case", KV["cp2"], ids=[c["src"] for c in KV["cp2"]])
@@ -106,3 +111,95 @@ def test_cp3_random_polytopes_match_halfspace_hull():
         assert abs(ref - r["volume"]) < 1e-10 * max(1, ref)
         # faces: the area vector sum of a closed surface vanishes; areas positive
         assert (r["areas"] > -1e-15).all()
+
+
+def scaled_image_array(case):
+    nx, ny = case["sizes"]
+    return np.array(case["data"], dtype=np.float64).reshape(case["nb_coeffs"], ny, nx)
+
+
+@pytest.mark.parametrize("case", KV["scaled_image"], ids=[c["src"] for c in KV["scaled_image"]])
+def test_scaled_image_known_values(case):
+    img = scaled_image_array(case)
+    r = O.cells(np.array(case["positions"]), np.array(case["weights"]), img=img,
+                domain=(case["lo"], case["hi"], 1.0))
+    assert r["status"] == 0
+    assert np.abs(r["masses"] - np.array(case["integrals"])).max() < case["tol"]
+
+
+def quadratic_image(nx, ny, seed):
+    """6 x ny x nx coefficients of positive quadratics on [0,1]^2"""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-0.5, 0.5, size=(6, ny, nx))
+    c[0] += 2.0
+    return c
+
+
+def quadratic_sampled(c, k):
+    """the same density as a constant image of k x k sub-pixels per pixel, sampled
+    at sub-pixel centres (midpoint rule, error O(1/k^2))"""
+    _, ny, nx = c.shape
+    xs = (np.arange(nx * k) + 0.5) / (nx * k)
+    ys = (np.arange(ny * k) + 0.5) / (ny * k)
+    X, Y = np.meshgrid(xs, ys)
+    cc = np.repeat(np.repeat(c, k, axis=1), k, axis=2)
+    return cc[0] + cc[1] * X + cc[2] * Y + cc[3] * X * X + cc[4] * X * Y + cc[5] * Y * Y
+
+
+def test_quadratic_image_total_mass_closed_form():
+    c = quadratic_image(5, 3, 0)
+    _, ny, nx = c.shape
+    n = 400
+    u = O.rng_uniform(3, 3 * n)
+    pos, w = u[: 2 * n].reshape(n, 2).copy(), 1e-4 * u[2 * n:]
+    r = O.cells(pos, w, img=c)
+    # integral of a quadratic over a pixel of centre (xc, yc) and sides (dx, dy)
+    dx, dy = 1.0 / nx, 1.0 / ny
+    xc = (np.arange(nx) + 0.5) * dx
+    yc = (np.arange(ny) + 0.5) * dy
+    X, Y = np.meshgrid(xc, yc)
+    tot = (dx * dy * (c[0] + c[1] * X + c[2] * Y + c[3] * (X * X + dx * dx / 12) + c[4] * X * Y
+                      + c[5] * (Y * Y + dy * dy / 12))).sum()
+    assert abs(r["masses"].sum() - tot) < 1e-12
+
+
+def test_quadratic_image_converges_to_fine_constant_image():
+    c = quadratic_image(4, 3, 1)
+    n = 60
+    u = O.rng_uniform(4, 3 * n)
+    pos, w = u[: 2 * n].reshape(n, 2).copy(), 1e-3 * u[2 * n:]
+    exact = O.cells(pos, w, img=c)
+    M = csr(*O.to_csr(exact), n).toarray()
+    errs = []
+    for k in (8, 16):
+        approx = O.cells(pos, w, img=quadratic_sampled(c, k))
+        Ma = csr(*O.to_csr(approx), n).toarray()
+        errs.append((np.abs(approx["masses"] - exact["masses"]).max(), np.abs(Ma - M).max()))
+    # second order: halving the sub-pixel size divides both errors by about 4
+    assert errs[1][0] < 0.4 * errs[0][0] and errs[1][1] < 0.4 * errs[0][1]
+    assert errs[1][0] < 1e-3 / n and errs[1][1] < 1e-3 * np.abs(M).max()
+
+
+def test_quadratic_image_hessian_is_the_mass_derivative():
+    c = quadratic_image(6, 5, 2)
+    n = 150
+    u = O.rng_uniform(5, 3 * n)
+    pos, w = u[: 2 * n].reshape(n, 2).copy(), 0.3 / n * u[2 * n:]
+    r = O.cells(pos, w, img=c)
+    M = csr(*O.to_csr(r), n).toarray()
+    assert np.abs(M - M.T).max() < 1e-12 * np.abs(M).max()
+    eps = 1e-7 / n
+    for j in (3, 57, 101):
+        wp, wm = w.copy(), w.copy()
+        wp[j] += eps
+        wm[j] -= eps
+        fd = (O.cells(pos, wp, img=c)["masses"] - O.cells(pos, wm, img=c)["masses"]) / (2 * eps)
+        assert np.abs(fd - M[:, j]).max() < 1e-5 * np.abs(M[:, j]).max()
+
+
+def test_quadratic_image_rejected_in_3d_and_with_ball():
+    c3 = np.ones((6, 2, 2, 2))
+    r = O.cells(np.array([[0.5, 0.5, 0.5]]), np.zeros(1), img=c3)
+    assert r["status"] == O.OR_INVALID
+    r = O.cells(np.array([[0.5, 0.5]]), np.ones(1), img=np.ones((6, 1, 1)), ball=True)
+    assert r["status"] == O.OR_INVALID
