@@ -143,10 +143,11 @@ typedef struct sdot_timings {
     double ms_cells;             /* cell kernel(s), incl. overflow re-runs */
     double ms_rows;              /* CSR export (0 unless requested) */
     double ms_total;
-    int64_t nb_cells_overflow;   /* cells re-run on the wide-capacity path */
+    int64_t nb_cells_overflow;   /* cells re-run with a larger polygon / polyhedron capacity */
     int64_t nnz_offdiag;         /* off-diagonal Hessian entries in the last pass */
     int64_t grid_cells;          /* grid cells of the last grid build */
-    double ms_cell_kernel;       /* main cell kernel alone (HIP events around the launch) */
+    double ms_cell_kernel;       /* the cell kernels (2D: all capacity tiers; 3D: the main
+                                    kernel alone), HIP events around the launches */
     double cell_flops;           /* algorithmic FLOPs of the pass: sum over cells of
                                     4K^2 + 37K + 40A (2D) / 12K^2 + 100K - 284 (3D),
                                     K = faces, A = ball arcs of the final cell */

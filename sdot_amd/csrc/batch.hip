@@ -274,7 +274,7 @@ BatchEngine::BatchEngine(int precision, int ball, int dpc_x100, bool use_graph)
     pd->ball = ball_;
     for (int k = 0; k < 4; ++k)
         if (!pd->ev[k]) HIPCK(hipEventCreate(&pd->ev[k]));
-    pd->flags.ensure(2);
+    pd->flags.ensure(3);
 }
 
 BatchEngine::~BatchEngine() {
@@ -492,7 +492,7 @@ double engine_pass(sdot_pd *pd, BatchEngine::Impl *im, int64_t K, int64_t N, boo
     const int64_t nc = im->ncodes;
     regrow = false;
     HIPCK(hipEventRecord(pd->ev[0], s));
-    fill_u32(pd->flags.p, 2, 0u, s);
+    fill_u32(pd->flags.p, 3, 0u, s);
     fill_u32(im->pflags.p, K, 0u, s);
     hipLaunchKernelGGL(kb_gather_w, dim3(nblocks(N, BB)), dim3(BB), 0, s, pd->w.p, pd->perm.p, im->prob.p, im->doff.p,
                        im->tie.p, N, pd->sw.p, pd->flags.p);
@@ -528,12 +528,13 @@ double engine_pass(sdot_pd *pd, BatchEngine::Impl *im, int64_t K, int64_t N, boo
     a.pdom = im->pdom.p;
     a.pflags = im->pflags.p;
     a.pdone = im->pdone.p;
-    run_cells2(pd, a);   // synchronises the stream
+    run_cells2(pd, a);
     HIPCK(hipEventRecord(pd->ev[2], s));
     unsigned fl = 0;
     HIPCK(hipMemcpyAsync(&fl, pd->flags.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
     if (fl & FLAG_BAD_INPUT) throw StatusError{SDOT_INVALID_ARG, "non-finite weight or coincident diracs in the batch"};
+    if (fl & FLAG_CELL_OVERFLOW) throw StatusError{SDOT_CAPACITY_OVERFLOW, "cell polygon exceeds 64 vertices"};
     if (fl & FLAG_ROW_OVERFLOW) {
         if (pd->ell_cap >= ELL_CAP2_WIDE) throw StatusError{SDOT_CAPACITY_OVERFLOW, "Hessian row longer than the row capacity"};
         // rerun with wide rows (kept from now on); a captured PCG graph read the old row arrays

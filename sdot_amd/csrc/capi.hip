@@ -100,8 +100,8 @@ sdot_status compute_pass(sdot_pd *pd) {
         if (!pd->ev[k]) HIPCK(hipEventCreate(&pd->ev[k]));
     HIPCK(hipEventRecord(pd->ev[0], s));
     if (!pd->grid_valid) build_grid(pd);
-    pd->flags.ensure(2);
-    fill_u32(pd->flags.p, 2, 0u, s);
+    pd->flags.ensure(3);
+    fill_u32(pd->flags.p, 3, 0u, s);
     refresh_weights(pd);
     HIPCK(hipEventRecord(pd->ev[1], s));
 
@@ -143,11 +143,13 @@ sdot_status compute_pass(sdot_pd *pd) {
     if (pd->dim == 2) run_cells2(pd, a);
     else run_cells3(pd, a);
     HIPCK(hipEventRecord(pd->ev[2], s));
-    unsigned fl = 0;
+    unsigned fls[3] = {0, 0, 0};
     std::vector<unsigned long long> ctr(3 * WCTR);
-    HIPCK(hipMemcpyAsync(&fl, pd->flags.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(fls, pd->flags.p, 3 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
     HIPCK(hipMemcpyAsync(ctr.data(), pd->wctr.p, 3 * WCTR * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
+    const unsigned fl = fls[0];
+    if (pd->dim == 2) pd->last.nb_cells_overflow = fls[1];
     double flops = 0;
     int64_t sk = 0, sa = 0;
     for (int k = 0; k < WCTR; ++k) {
@@ -168,7 +170,8 @@ sdot_status compute_pass(sdot_pd *pd) {
     pd->pass_valid = true;
     if (fl & FLAG_BAD_INPUT) throw StatusError{SDOT_INVALID_ARG, "non-finite weight or coincident diracs"};
     if (fl & FLAG_CELL_OVERFLOW)
-        throw StatusError{SDOT_CAPACITY_OVERFLOW, "a Laguerre cell exceeds the polyhedron capacity (255 faces, 508 vertices)"};
+        throw StatusError{SDOT_CAPACITY_OVERFLOW, pd->dim == 2 ? "a Laguerre cell exceeds the polygon capacity (64 vertices)"
+                                                              : "a Laguerre cell exceeds the polyhedron capacity (255 faces, 508 vertices)"};
     if (fl & FLAG_ROW_OVERFLOW) {
         if (pd->ell_cap < cap_wide) {   // rerun with wide rows (kept for the next passes)
             pd->ell_cap = cap_wide;

@@ -17,6 +17,7 @@
 //     :437-478) and the Hessian row (get_der_integrals_wrt_weights.h:52-75).
 // T = double (FP64 path) or float (mixed path: FP32 local coordinates; plane
 // offsets, distances and the mass/Hessian outputs in FP64).
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
@@ -29,23 +30,38 @@ namespace {
 
 using namespace c2;
 
-template <class T, int CAP, int BS, bool BALL, bool IMG>
-__global__ __launch_bounds__(BS) void k_cells2(CellArgs a) {
+// One thread per cell, polygon in the thread's LDS slot (CAP vertices).  Without
+// a work list: every cell, in grid order.  With one: the cells listed, count in
+// device memory, grid-stride.  A cell that outgrows CAP goes to ovf_list for the
+// next tier; in the last tier (no ovf_list) it raises FLAG_CELL_OVERFLOW.
+// WAVES: the wavefronts per SIMD the register allocation must allow.
+template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES>
+__global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
     __shared__ T sx[CAP * BS];
     __shared__ T sy[CAP * BS];
     __shared__ int sid[CAP * BS];
-    int64_t t;
+    // one call site of the cell body (so it is inlined): without a work list the
+    // loop runs once
+    int64_t k, count, step;
     if (a.work) {
-        int64_t k = blockIdx.x * (int64_t)BS + threadIdx.x;
-        if (k >= a.work_n) return;
-        t = a.work[k];
+        k = blockIdx.x * (int64_t)BS + threadIdx.x;
+        count = *a.work_count;
+        step = (int64_t)gridDim.x * BS;
     } else {
-        t = xcd_tile(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-        if (t >= a.N) return;
+        k = xcd_tile(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+        count = a.N;
+        step = a.N;
     }
-    if (cell2_body<T, CAP, BALL, IMG>(a, t, sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x, BS)) {
-        unsigned k = atomicAdd(a.ovf_count, 1u);
-        a.ovf_list[k] = (int)t;
+    for (; k < count; k += step) {
+        const int64_t t = a.work ? a.work[k] : k;
+        if (cell2_body<T, CAP, BALL, IMG>(a, t, sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x, BS)) {
+            if (a.ovf_list) {
+                const unsigned q = atomicAdd(a.ovf_count, 1u);
+                a.ovf_list[q] = (int)t;
+            } else {
+                flag_or(a.flags, FLAG_CELL_OVERFLOW);
+            }
+        }
     }
 }
 
@@ -109,61 +125,65 @@ __global__ __launch_bounds__(BS) void k_csr_fill(const int *row_len, const int *
     }
 }
 
-template <class T, int CAP, int BS, bool BALL, bool IMG>
-void launch2(const CellArgs &a, int64_t nthreads, hipStream_t s) {
-    unsigned nb = (unsigned)((nthreads + BS - 1) / BS);
+template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES>
+void launch2(const CellArgs &a, unsigned nb, hipStream_t s) {
     if (nb == 0) return;
-    hipLaunchKernelGGL((k_cells2<T, CAP, BS, BALL, IMG>), dim3(nb), dim3(BS), 0, s, a);
+    hipLaunchKernelGGL((k_cells2<T, CAP, BS, BALL, IMG, WAVES>), dim3(nb), dim3(BS), 0, s, a);
     HIPCK(hipGetLastError());
 }
 
 // ball and image density exclude each other (rejected before the pass)
-template <int CAP>
-void dispatch2(bool mixed, bool ball, bool img, const CellArgs &a, int64_t nthreads, hipStream_t s) {
-    if (mixed) {
-        if (ball) launch2<float, CAP, 64, true, false>(a, nthreads, s);
-        else if (img) launch2<float, CAP, 64, false, true>(a, nthreads, s);
-        else launch2<float, CAP, 64, false, false>(a, nthreads, s);
-    } else {
-        if (ball) launch2<double, CAP, 64, true, false>(a, nthreads, s);
-        else if (img) launch2<double, CAP, 64, false, true>(a, nthreads, s);
-        else launch2<double, CAP, 64, false, false>(a, nthreads, s);
-    }
+template <class T, int CAP, int WAVES>
+void dispatch2(bool ball, bool img, const CellArgs &a, unsigned nb, hipStream_t s) {
+    if (ball) launch2<T, CAP, 64, true, false, WAVES>(a, nb, s);
+    else if (img) launch2<T, CAP, 64, false, true, WAVES>(a, nb, s);
+    else launch2<T, CAP, 64, false, false, WAVES>(a, nb, s);
 }
 
 }  // namespace
 
-// dispatch over precision / ball / image: main pass with a small LDS polygon, then the
-// overflowed cells again with a wide one
+// Capacity tiers, all on the stream with no host round trip: every cell with a
+// small polygon slot in LDS, the cells that outgrew it again with a larger one,
+// and so on up to 64 vertices (a cell beyond that raises FLAG_CELL_OVERFLOW).  A
+// 2D power cell averages about 6 vertices.  Mixed: 16 vertices (FP32 slots,
+// three wavefronts per SIMD), then 64.  FP64: 8 vertices (four wavefronts per
+// SIMD; a few percent of the cells outgrow it while they are being clipped),
+// then 16, then 64.
 void run_cells2(sdot_pd *pd, CellArgs &a) {
     hipStream_t s = pd->stream;
     const bool ball = pd->ball != 0;
     const bool mixed = pd->precision == SDOT_MIXED;
-    a.work = nullptr;
-    a.work_n = 0;
     const bool img = a.img != nullptr;
-    dispatch2<16>(mixed, ball, img, a, a.N, s);
-    HIPCK(hipEventRecord(pd->ev[3], s));
-    unsigned cnt[2] = {0, 0};
-    HIPCK(hipMemcpyAsync(cnt, pd->flags.p, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    HIPCK(hipStreamSynchronize(s));
-    pd->last.nb_cells_overflow = cnt[1];
-    if (cnt[1] > 0) {
-        // the overflow list is copied so the wide pass can append to a fresh counter
-        pd->ovf_list.ensure(2 * (size_t)a.N + 2);
-        int *list2 = pd->ovf_list.p + a.N + 1;
-        HIPCK(hipMemcpyAsync(list2, pd->ovf_list.p, cnt[1] * sizeof(int), hipMemcpyDeviceToDevice, s));
-        unsigned zero = 0;
-        HIPCK(hipMemcpyAsync(pd->flags.p + 1, &zero, sizeof(unsigned), hipMemcpyHostToDevice, s));
-        a.work = list2;
-        a.work_n = cnt[1];
-        dispatch2<64>(mixed, ball, img, a, cnt[1], s);
-        unsigned c2 = 0;
-        HIPCK(hipMemcpyAsync(&c2, pd->flags.p + 1, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        HIPCK(hipStreamSynchronize(s));
-        if (c2 > 0) throw std::string("cell polygon exceeds 64 vertices");
-        a.work = nullptr;
+    const int64_t N = a.N;
+    const unsigned nb_all = (unsigned)((N + 63) / 64);
+    int *lists[2] = {pd->ovf_list.p, pd->ovf_list.p + N + 1};
+    int tier = 0;
+    // tier `tier` reads the previous tier's list (none for the first) and appends to
+    // its own unless it is the last
+    auto next = [&](bool last) {
+        a.work = tier == 0 ? nullptr : lists[(tier - 1) & 1];
+        a.work_count = tier == 0 ? nullptr : pd->flags.p + 1 + ((tier - 1) & 1);
+        a.ovf_list = last ? nullptr : lists[tier & 1];
+        a.ovf_count = last ? nullptr : pd->flags.p + 1 + (tier & 1);
+        ++tier;
+        return tier == 1 ? nb_all : std::min(nb_all, 4096u);
+    };
+    if (mixed) {
+        unsigned nb = next(false);
+        dispatch2<float, 16, 3>(ball, img, a, nb, s);
+        nb = next(true);
+        dispatch2<float, 64, 1>(ball, img, a, nb, s);
+    } else {
+        unsigned nb = next(false);
+        dispatch2<double, 8, 4>(ball, img, a, nb, s);
+        nb = next(false);
+        dispatch2<double, 16, 2>(ball, img, a, nb, s);
+        nb = next(true);
+        dispatch2<double, 64, 1>(ball, img, a, nb, s);
     }
+    HIPCK(hipEventRecord(pd->ev[3], s));
+    a.work = nullptr;
+    a.work_count = nullptr;
 }
 
 void export_vec(sdot_pd *pd, const double *d_sorted, double *h_out) {

@@ -230,9 +230,9 @@ void build_grid(sdot_pd *pd) {
     const int dim = pd->dim;
     const int64_t N = pd->N;
     hipStream_t s = pd->stream;
-    pd->flags.ensure(2);
+    pd->flags.ensure(3);
     pd->bbox_part.ensure(RED_BLOCKS * 2 * 3);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, pd->flags.p, (int64_t)2, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, pd->flags.p, (int64_t)3, 0u);
     if (dim == 2)
         hipLaunchKernelGGL(k_bbox<2>, dim3(RED_BLOCKS), dim3(BS), 0, s, pd->pos.p, N, pd->bbox_part.p, pd->flags.p);
     else

@@ -194,7 +194,8 @@ struct CellArgs {
     int *ovf_list;
     unsigned *ovf_count;
     const int *work;             // optional list of sorted indices to process
-    int64_t work_n;
+    int64_t work_n;              // its length (3D; host-known)
+    const unsigned *work_count;  // its length in device memory (2D tiers; no host round trip)
     unsigned long long *ctr;     // work counters: [k] = sum F, [WCTR + k] = sum K, [2*WCTR + k] = sum A
     // batches of independent problems (all null for a single problem): problem of
     // each sorted index, per-problem grid, code offset into cell_start/cell_wmax,
@@ -301,7 +302,7 @@ struct sdot_pd {
     // pass outputs (sorted order)
     sdot_amd::DevBuf<double> mass, arc, perim, diag, ell_val;
     sdot_amd::DevBuf<int> row_len, ell_col, ovf_list;
-    sdot_amd::DevBuf<unsigned> flags;  // [0] flags, [1] overflow count
+    sdot_amd::DevBuf<unsigned> flags;  // [0] flags, [1] overflow count, [2] second-tier overflow count
     int ell_cap = 0;
     unsigned last_flags = 0;
     sdot_amd::DevBuf<unsigned long long> wctr;
