@@ -135,6 +135,7 @@ sdot_status compute_pass(sdot_pd *pd) {
     a.ell_cap = pd->ell_cap;
     a.ell_stride = N;
     a.flags = pd->flags.p;
+    a.stop_if_void = pd->stop_if_void ? 1 : 0;
     a.ovf_list = pd->ovf_list.p;
     a.ovf_count = pd->flags.p + 1;
     pd->wctr.ensure(3 * WCTR);
@@ -169,6 +170,10 @@ sdot_status compute_pass(sdot_pd *pd) {
     pd->last_flags = fl;
     pd->pass_valid = true;
     if (fl & FLAG_BAD_INPUT) throw StatusError{SDOT_INVALID_ARG, "non-finite weight or coincident diracs"};
+    if (pd->stop_if_void && (fl & FLAG_EMPTY)) {   // stopped early: the per-cell results are partial
+        pd->pass_valid = false;
+        return SDOT_EMPTY_CELL;
+    }
     if (fl & FLAG_CELL_OVERFLOW)
         throw StatusError{SDOT_CAPACITY_OVERFLOW, pd->dim == 2 ? "a Laguerre cell exceeds the polygon capacity (64 vertices)"
                                                               : "a Laguerre cell exceeds the polyhedron capacity (255 faces, 508 vertices)"};

@@ -228,6 +228,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     ProbView pv;
     prob_view(a, t, pv);
     if (a.pdone && a.pdone[pv.p]) return 0;   // converged problem of a batch: nothing to do
+    if (void_seen(a, pv.p)) return 0;         // the pass is a backtrack signal already
     const GridInfo &g = pv.g;
     const double W = pv.W;
 

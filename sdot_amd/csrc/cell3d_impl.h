@@ -392,6 +392,7 @@ SDOT_HD inline bool region_skippable3(double d, double R2, double w0, double wma
 // Builds and integrates cell t.  Returns 0, or 2 when the capacities are exceeded.
 template <class T>
 SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem<T> &mem, int VCAP, int PCAP) {
+    if (void_seen(a, 0)) return 0;   // the pass is a backtrack signal already
     Cell3<T> C(a, mem, VCAP, PCAP);
     C.c0x = a.spos[3 * t];
     C.c0y = a.spos[3 * t + 1];

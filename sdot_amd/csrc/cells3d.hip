@@ -458,6 +458,8 @@ __global__ __launch_bounds__(WV * WPB) void k_cells3_wave(const CellArgs *__rest
     const int64_t per = (a.N + nwaves - 1) / nwaves;
     const int64_t tb = w0id * per, te = tb + per < a.N ? tb + per : a.N;
     for (int64_t t = tb; t < te; ++t) {
+        // stop_if_void: one read per cell, made uniform across the wavefront
+        if (a.stop_if_void && __builtin_amdgcn_readfirstlane((int)void_seen(a, 0))) break;
         WaveCell<T> C(a, L, lane);
         C.c0x = a.spos[3 * t];
         C.c0y = a.spos[3 * t + 1];

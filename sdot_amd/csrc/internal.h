@@ -191,6 +191,10 @@ struct CellArgs {
     int ell_cap;
     int64_t ell_stride;
     unsigned *flags;
+    // stop_if_void (for_each_laguerre_cell, SpZGrid.h:44): once a cell has come out
+    // empty (FLAG_EMPTY in flags, or in pflags for a batch problem) the pass result
+    // is only a backtrack signal, so the remaining cells are skipped
+    int stop_if_void;
     int *ovf_list;
     unsigned *ovf_count;
     const int *work;             // optional list of sorted indices to process
@@ -207,6 +211,14 @@ struct CellArgs {
     unsigned *pflags;
     const int *pdone;
 };
+
+// true when the pass may stop: stop_if_void is set and an empty cell has been
+// seen (in the problem's own flags for a batch)
+SDOT_HD inline bool void_seen(const CellArgs &a, int p) {
+    if (!a.stop_if_void) return false;
+    const unsigned *f = a.pflags ? a.pflags + p : a.flags;
+    return (__atomic_load_n(f, __ATOMIC_RELAXED) & FLAG_EMPTY) != 0;
+}
 
 // the per-problem view of CellArgs for sorted index t
 struct ProbView {
@@ -291,6 +303,7 @@ struct sdot_pd {
     sdot_amd::DevBuf<double> pos, w;
     bool grid_valid = false;    // positions unchanged since the last sort
     bool pass_valid = false;    // results match the current diracs/weights/settings
+    bool stop_if_void = false;  // passes of a Newton solve: an empty cell ends the pass early
 
     // grid
     sdot_amd::GridInfo g{};

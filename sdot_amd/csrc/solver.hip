@@ -153,6 +153,13 @@ sdot_status ot_solve(sdot_pd *pd, const double *targets, double tol, int stop_ru
                     it, pd->last.ms_total, pd->last.ms_grid, pd->last.ms_cell_kernel, pd->last.ms_cells,
                     (long long)pd->last.nb_cells_overflow, (int)ps, wall_ms());
     };
+    // an empty cell only triggers a backtrack (OptimalTransportSolver.tcc:72-79): the
+    // passes stop at the first one (get_der_integrals_wrt_weights' stop_if_void)
+    struct StopIfVoid {
+        sdot_pd *pd;
+        explicit StopIfVoid(sdot_pd *p) : pd(p) { pd->stop_if_void = true; }
+        ~StopIfVoid() { pd->stop_if_void = false; }
+    } stop_guard(pd);
     for (int it = 0; it < max_iter; ++it) {
         sdot_status ps = compute_pass(pd);
         ms_cells += pd->last.ms_total;
@@ -207,6 +214,8 @@ sdot_status ot_solve(sdot_pd *pd, const double *targets, double tol, int stop_ru
         }
     }
     // the masses at the returned weights: one more pass when the loop ended on an update
+    // (a complete one: the caller reads its per-cell results)
+    pd->stop_if_void = false;
     if (!pd->pass_valid) {
         sdot_status ps = compute_pass(pd);
         ms_cells += pd->last.ms_total;
