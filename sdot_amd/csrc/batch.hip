@@ -250,7 +250,7 @@ GridInfo problem_grid(const double *pos, int64_t n, int dpc_x100, double *tie) {
 }
 
 struct BatchEngine::Impl {
-    DevBuf<int> prob, blk_prob, pdone, act;
+    DevBuf<int> prob, blk_prob, pdone, pstop, act;
     DevBuf<int64_t> dcode_base, doff, dblk, src_off;
     DevBuf<GridInfo> dgrids;
     DevBuf<double> pdom, tie, w0, tgt, tsort, old, b, red, perr;
@@ -278,7 +278,7 @@ BatchEngine::BatchEngine(int precision, int ball, int dpc_x100, bool use_graph)
 }
 
 BatchEngine::~BatchEngine() {
-    for (auto *b : {&im->prob, &im->blk_prob, &im->pdone, &im->act}) b->release();
+    for (auto *b : {&im->prob, &im->blk_prob, &im->pdone, &im->pstop, &im->act}) b->release();
     for (auto *b : {&im->dcode_base, &im->doff, &im->dblk, &im->src_off}) b->release();
     for (auto *b : {&im->pdom, &im->tie, &im->w0, &im->tgt, &im->tsort, &im->old, &im->b, &im->red, &im->perr,
                     &im->x, &im->q, &im->pos2, &im->w2, &im->old2, &im->w02, &im->tgt2})
@@ -398,6 +398,7 @@ void BatchEngine::relayout(const std::vector<int> &keep, const std::vector<NewPr
     im->perr.ensure(K);
     im->pflags.ensure(K);
     im->pdone.ensure(K);
+    im->pstop.ensure(K);
     im->act.ensure(K);
     im->wglob.ensure(K);
     if (pd->ell_cap < ELL_CAP2) pd->ell_cap = ELL_CAP2;
@@ -415,6 +416,7 @@ void BatchEngine::relayout(const std::vector<int> &keep, const std::vector<NewPr
     std::vector<int> done(K);
     for (int64_t k = 0; k < K; ++k) done[k] = probs[k].done;
     HIPCK(hipMemcpyAsync(im->pdone.p, done.data(), K * sizeof(int), hipMemcpyHostToDevice, s));
+    fill_u32(reinterpret_cast<uint32_t *>(im->pstop.p), K, 0u, s);   // complete passes until a tick decides
     rebuild_grid();
 }
 
@@ -476,6 +478,7 @@ void BatchEngine::restart() {
     HIPCK(hipMemcpyAsync(pd->w.p, im->w0.p, N_ * sizeof(double), hipMemcpyDeviceToDevice, s));
     HIPCK(hipMemcpyAsync(im->old.p, im->w0.p, N_ * sizeof(double), hipMemcpyDeviceToDevice, s));
     fill_u32(reinterpret_cast<uint32_t *>(im->pdone.p), K, 0u, s);
+    fill_u32(reinterpret_cast<uint32_t *>(im->pstop.p), K, 0u, s);
     for (auto &b : probs) {
         b.iters = b.done = b.dw_ok = b.backtracks = b.pcg_iters = 0;
         b.status = SDOT_NOT_CONVERGED;
@@ -528,6 +531,8 @@ double engine_pass(sdot_pd *pd, BatchEngine::Impl *im, int64_t K, int64_t N, boo
     a.pdom = im->pdom.p;
     a.pflags = im->pflags.p;
     a.pdone = im->pdone.p;
+    a.pstop = im->pstop.p;
+    a.stop_if_void = 1;
     run_cells2(pd, a);
     HIPCK(hipEventRecord(pd->ev[2], s));
     unsigned fl = 0;
@@ -646,6 +651,15 @@ int BatchEngine::tick(const TickParams &tp, std::vector<int> &finished, TickStat
             if (tp.stop_rule == 1 && mdw[k] < tp.tol) probs[k].dw_ok = 1;   // done once the next pass evaluates it
         }
     }
+    // stop_if_void for the next pass: only problems that cannot finish on it (an empty
+    // cell there means a backtrack, whose pass results are not read)
+    std::vector<int> stop(K);
+    for (int64_t k = 0; k < K; ++k) {
+        const BatchProblem &b = probs[k];
+        stop[k] = !b.done && !b.dw_ok && b.iters < tp.max_iter;
+    }
+    HIPCK(hipMemcpyAsync(im->pstop.p, stop.data(), K * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCK(hipStreamSynchronize(s));
     return running;
 }
 

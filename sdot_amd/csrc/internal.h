@@ -210,12 +210,14 @@ struct CellArgs {
     const double *pdom;
     unsigned *pflags;
     const int *pdone;
+    const int *pstop;   // batch: problems whose pass may stop at an empty cell (null: none)
 };
 
 // true when the pass may stop: stop_if_void is set and an empty cell has been
 // seen (in the problem's own flags for a batch)
 SDOT_HD inline bool void_seen(const CellArgs &a, int p) {
     if (!a.stop_if_void) return false;
+    if (a.pflags && !(a.pstop && a.pstop[p])) return false;
     const unsigned *f = a.pflags ? a.pflags + p : a.flags;
     return (__atomic_load_n(f, __ATOMIC_RELAXED) & FLAG_EMPTY) != 0;
 }
