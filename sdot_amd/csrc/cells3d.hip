@@ -111,7 +111,7 @@ void launch_thread3(sdot_pd *pd, const CellArgs &a) {
 // to the thread kernel above.
 // ---------------------------------------------------------------------------
 constexpr int WV = 64;            // lanes = vertex capacity = plane capacity
-constexpr int FRONT = 256;        // far-field traversal: frontier capacity (nodes)
+constexpr int FRONT = 192;        // far-field traversal: frontier capacity (nodes)
 constexpr int NEAR_RINGS = 1;     // rings of grid cells visited before the far-field traversal
 constexpr int WPB = 4;            // wavefronts per block
 
@@ -192,10 +192,13 @@ struct WaveCell {
     }
 
     __device__ void init_box() {
-        const double c0[3] = {c0x, c0y, c0z};
         if (lane < 6) {
+            // (selects, not an indexed array: a per-lane index would put the array in memory)
             const int axis = lane >> 1, up = lane & 1;
-            const double off = (up ? sel3(a.dom_hi, axis) : sel3(a.dom_lo, axis)) - sel3(c0, axis);
+            const double c0a = axis == 0 ? c0x : (axis == 1 ? c0y : c0z);
+            const double lo = axis == 0 ? a.dom_lo[0] : (axis == 1 ? a.dom_lo[1] : a.dom_lo[2]);
+            const double hi = axis == 0 ? a.dom_hi[0] : (axis == 1 ? a.dom_hi[1] : a.dom_hi[2]);
+            const double off = (up ? hi : lo) - c0a;
             L.pn[lane][0] = axis == 0 ? (up ? 1.0 : -1.0) : 0.0;
             L.pn[lane][1] = axis == 1 ? (up ? 1.0 : -1.0) : 0.0;
             L.pn[lane][2] = axis == 2 ? (up ? 1.0 : -1.0) : 0.0;
@@ -441,7 +444,8 @@ struct WaveCell {
 };
 
 template <class T>
-__global__ __launch_bounds__(WV * WPB) void k_cells3_wave(const CellArgs *__restrict__ ap) {
+// three wavefronts per SIMD: registers capped at 168, LDS 3 x 4 x 11.5 KB per CU
+__global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__restrict__ ap) {
     const CellArgs &a = *ap;
     __shared__ WaveLds lds[WPB];
     const int lane = threadIdx.x & (WV - 1);
