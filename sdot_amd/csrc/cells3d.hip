@@ -145,6 +145,57 @@ __device__ inline int wave_sum_i(int v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
+// Cross-lane helpers without LDS traffic (the shuffles above go through the LDS
+// crossbar, one dependent round trip per step).
+// Value of v in lane k, k uniform across the wavefront (v_readlane).
+__device__ inline int lane_value(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ inline float lane_value(float v, int k) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k)); }
+__device__ inline double lane_value(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// DPP move within rows of 16 lanes; CTRL is a full quad permutation or a row
+// rotation, so every lane has a source lane
+template <int CTRL>
+__device__ inline double dpp_row(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+constexpr int DPP_QUAD_SWAP1 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int DPP_QUAD_SWAP2 = 0x4E;   // quad_perm [2,3,0,1]
+constexpr int DPP_ROW_ROR4 = 0x124, DPP_ROW_ROR8 = 0x128;
+// reductions over the whole wavefront (every lane active): within each row by
+// quad swaps and row rotations (after them every lane holds its row's result),
+// then the four rows from lanes 0, 16, 32, 48
+__device__ inline double wave_max_dpp(double v) {
+    v = fmax(v, dpp_row<DPP_QUAD_SWAP1>(v));
+    v = fmax(v, dpp_row<DPP_QUAD_SWAP2>(v));
+    v = fmax(v, dpp_row<DPP_ROW_ROR4>(v));
+    v = fmax(v, dpp_row<DPP_ROW_ROR8>(v));
+    return fmax(fmax(lane_value(v, 0), lane_value(v, 16)), fmax(lane_value(v, 32), lane_value(v, 48)));
+}
+__device__ inline double wave_sum_dpp(double v) {
+    v += dpp_row<DPP_QUAD_SWAP1>(v);
+    v += dpp_row<DPP_QUAD_SWAP2>(v);
+    v += dpp_row<DPP_ROW_ROR4>(v);
+    v += dpp_row<DPP_ROW_ROR8>(v);
+    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+// exclusive prefix sum of small values (0 <= v < 8) from three ballots
+__device__ inline int wave_excl_scan_small(int v) {
+    int r = 0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const unsigned long long m = __ballot((v >> b) & 1);
+        r += (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u)) << b;
+    }
+    return r;
+}
+
 __device__ inline int wave_excl_scan(int v, int lane) {
     int x = v;
     for (int o = 1; o < WV; o <<= 1) {
@@ -225,7 +276,7 @@ struct WaveCell {
 
     __device__ double radius2() const {
         double r = lane < nv ? (double)x * x + (double)y * y + (double)z * z : 0.0;
-        return wave_max(r);
+        return wave_max_dpp(r);
     }
 
     // Side of this lane's vertex for the plane n.q <= h, exactly from the planes:
@@ -327,8 +378,8 @@ struct WaveCell {
             ntri[e] = xa | (yb << 8) | (pn << 16);
             cnt += mk[e] ? 1 : 0;
         }
-        const int pre = wave_excl_scan(cnt, lane);
-        const int total = __shfl(pre + cnt, WV - 1);
+        const int pre = wave_excl_scan_small(cnt);   // cnt <= 3
+        const int total = lane_value(pre + cnt, WV - 1);
         if (total > WV) return 2;
         wave_sync();
         if (valid && !out) {
@@ -432,7 +483,7 @@ struct WaveCell {
             vol = fh / sqrt(fx * fx + fy * fy + fz * fz) * ar / 3;
         }
         wave_sync();
-        return wave_sum(vol);
+        return wave_sum_dpp(vol);
     }
 
     __device__ unsigned long long used_planes() const {
@@ -519,9 +570,9 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
                 while (mask && C.nv > 0) {
                     const int k = __ffsll((long long)mask) - 1;
                     mask &= mask - 1;
-                    const double ux = __shfl(vx, k), uy = __shfl(vy, k), uz = __shfl(vz, k), uh = __shfl(hv, k),
-                                 uv = __shfl(vv, k);
-                    const int uj = __shfl(j, k);
+                    const double ux = lane_value(vx, k), uy = lane_value(vy, k), uz = lane_value(vz, k),
+                                 uh = lane_value(hv, k), uv = lane_value(vv, k);
+                    const int uj = lane_value(j, k);
                     if (uh > 0 && R2 * uv <= uh * uh * (1 - c3::Eps3<T>::cand)) continue;   // R shrank
                     const double tie = c3::Eps3<T>::tie * sqrt(uv * R2);
                     const int rc = C.cut(T(ux), T(uy), T(uz), T(uh), uj, ux, uy, uz, uh, tie, true);
@@ -751,14 +802,14 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
             C.plane64(lane, nx, ny, nz, hv);
             b = 0.5 * ar / sqrt(nx * nx + ny * ny + nz * nz);
         }
-        const int slot = wave_excl_scan(nb ? 1 : 0, lane);
-        const int nrow = __shfl(slot + (nb ? 1 : 0), WV - 1);
+        const int slot = wave_excl_scan_small(nb ? 1 : 0);
+        const int nrow = lane_value(slot + (nb ? 1 : 0), WV - 1);
         if (nb && slot < a.ell_cap) {
             a.ell_col[slot * a.ell_stride + t] = id;
             a.ell_val[slot * a.ell_stride + t] = -b;
         }
-        const double der = wave_sum(b);
-        const double perim = wave_sum(mine && (!a.img || id >= 0) ? ar : 0.0);
+        const double der = wave_sum_dpp(b);
+        const double perim = wave_sum_dpp(mine && (!a.img || id >= 0) ? ar : 0.0);
         if (lane == 0) {
             a.mass[t] = mass;
             a.arc[t] = 0;
