@@ -218,7 +218,9 @@ SDOT_HD inline bool clip_axis(const IdPoly2<PC> &in, IdPoly2<PC> &out, int axis,
     return fits;
 }
 
-template <class T, int CAP, bool BALL, bool IMG = false>
+// BATCH = false: a single problem (CellArgs::prob is null), whose grid and domain
+// then stay uniform values (scalar registers) instead of per-thread copies.
+template <class T, int CAP, bool BALL, bool IMG = false, bool BATCH = true>
 SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, int *I, const int S) {
 #define PX(k) X[(k) * S]
 #define PY(k) Y[(k) * S]
@@ -226,7 +228,8 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
 
     const double c0x = a.spos[2 * t], c0y = a.spos[2 * t + 1], w0 = a.sw[t];
     ProbView pv;
-    prob_view(a, t, pv);
+    if (BATCH) prob_view(a, t, pv);
+    else prob_view_single(a, pv);
     if (a.pdone && a.pdone[pv.p]) return 0;   // converged problem of a batch: nothing to do
     if (void_seen(a, pv.p)) return 0;         // the pass is a backtrack signal already
     const GridInfo &g = pv.g;
@@ -281,28 +284,28 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         uint64_t mask = 0;
         T dfirst = 0, dprev = 0, dAin = 0, dAout = 0, dBout = 0, dBin = 0;
         int i1 = -1, i2 = -1, ntr = 0;
-        bool exact = false;
-    classify:
-        mask = 0;
-        ntr = 0;
-        for (int k = 0; k < n; ++k) {
-            T d = exact ? T(side64(k, vx64, vy64, h64)) : fma(-PX(k), Vx, fma(-PY(k), Vy, H));
-            bool o = d < 0;
-            if (o) mask |= 1ull << k;
-            if (k == 0) {
-                dfirst = d;
-            } else {
-                bool po = dprev < 0;
-                if (o && !po) { i1 = k; dAin = dprev; dAout = d; ++ntr; }
-                if (!o && po) { i2 = k - 1; dBout = dprev; dBin = d; }
+        auto classify = [&](auto &&dist) {
+            mask = 0;
+            ntr = 0;
+            for (int k = 0; k < n; ++k) {
+                const T d = dist(k);
+                const bool o = d < 0;
+                if (o) mask |= 1ull << k;
+                if (k == 0) {
+                    dfirst = d;
+                } else {
+                    const bool po = dprev < 0;
+                    if (o && !po) { i1 = k; dAin = dprev; dAout = d; ++ntr; }
+                    if (!o && po) { i2 = k - 1; dBout = dprev; dBin = d; }
+                }
+                dprev = d;
             }
-            dprev = d;
-        }
-        if (n > 0 && dfirst < 0 && !(dprev < 0)) ++ntr;
-        if (ntr > 1 && !exact) {   // a convex polygon crosses a line once: rounding noise, redo in FP64
-            exact = true;
-            goto classify;
-        }
+            if (n > 0 && dfirst < 0 && !(dprev < 0)) ++ntr;
+        };
+        // the fast loop stays free of the exact path (a separate, rarely run loop)
+        classify([&](int k) { return fma(-PX(k), Vx, fma(-PY(k), Vy, H)); });
+        if (ntr > 1)   // a convex polygon crosses a line once: rounding noise, redo in FP64
+            classify([&](int k) { return T(side64(k, vx64, vy64, h64)); });
         if (!mask) return 0;
         const int nout = __builtin_popcountll(mask);
         if (nout == n) { n = 0; return 1; }

@@ -231,17 +231,20 @@ struct ProbView {
     double W;
     double lo[3], hi[3];
 };
+SDOT_HD inline void prob_view_single(const CellArgs &a, ProbView &v) {
+    v.p = 0;
+    v.g = a.g;
+    v.cell_start = a.cell_start;
+    v.cell_wmax = a.cell_wmax;
+    v.W = dbl_unorder(a.wglob[0]);
+    for (int d = 0; d < 3; ++d) {
+        v.lo[d] = a.dom_lo[d];
+        v.hi[d] = a.dom_hi[d];
+    }
+}
 SDOT_HD inline void prob_view(const CellArgs &a, int64_t t, ProbView &v) {
     if (!a.prob) {
-        v.p = 0;
-        v.g = a.g;
-        v.cell_start = a.cell_start;
-        v.cell_wmax = a.cell_wmax;
-        v.W = dbl_unorder(a.wglob[0]);
-        for (int d = 0; d < 3; ++d) {
-            v.lo[d] = a.dom_lo[d];
-            v.hi[d] = a.dom_hi[d];
-        }
+        prob_view_single(a, v);
         return;
     }
     const int p = a.prob[t];
