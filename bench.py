@@ -25,10 +25,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Laguerre cells/s (1M diracs 2D/3D); solve s @1e-6 mass err; p99 latency"
 PEAK = {"fp64": 78.6, "mixed": 157.3}        # MI355X vector FP64 / FP32, TFLOP/s
-# HBM bytes per launch of the cell kernel on the 1M ball workload: rocprofv3 FETCH_SIZE and
-# WRITE_SIZE of k_cells2, each scaled by the same counter on a 1 GiB device copy
+# HBM bytes per pass of the cell kernels on the 1M ball workload (all capacity tiers): rocprofv3
+# FETCH_SIZE and WRITE_SIZE of k_cells2, each scaled by the same counter on a 1 GiB device copy
 # (bench.py --copy-calibration 1024: FETCH reads half the copied bytes, WRITE all of them)
-TRAFFIC = {"mixed": 185.0e6, "fp64": 184.1e6}
+TRAFFIC = {"mixed": 184.9e6, "fp64": 636.1e6}
 
 
 def make_inputs(n, seed=0):
