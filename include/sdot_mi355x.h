@@ -92,6 +92,11 @@ sdot_status sdot_pd_compute(sdot_pd *pd);
 
 /* get_integrals(res, ...) -- get_integrals.h:13.  Runs the pass if needed. */
 sdot_status sdot_pd_integrals(sdot_pd *pd, double *masses);
+/* get_centroids(grid, bounds, positions, weights, N, radial_func, cb) -- get_centroids.h:14
+ * (rand_ratio = 0): centroids[N*dim] (dirac order) of the density over each cell,
+ * 0 for an empty cell; masses may be NULL.  Runs the pass (with first moments) if
+ * needed.  SDOT_NOT_SUPPORTED with quadratic pixels. */
+sdot_status sdot_pd_centroids(sdot_pd *pd, double *centroids, double *masses);
 /* per-cell boundary measures: ball part (arcs with cut id i) and whole perimeter
  * (area of the boundary in 3D); either may be NULL.  With an image density the
  * perimeter is the density-weighted measure of the faces shared with other

@@ -60,6 +60,12 @@ def lib():
         L.or_ot_solve.restype = ctypes.c_int
         L.or_ot_solve.argtypes = [ctypes.c_int, _i64, _p, _p, _p, ctypes.POINTER(Domain), ctypes.c_int,
                                   ctypes.c_int, _d, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SolveStats)]
+        L.or_cp2_centroid.restype = ctypes.c_int
+        L.or_cp2_centroid.argtypes = [_d, _d, _d, _d, ctypes.c_int, _p, ctypes.c_int, _d, _d, _d, _p,
+                                      ctypes.POINTER(_d)]
+        L.or_centroids.restype = ctypes.c_int
+        L.or_centroids.argtypes = [ctypes.c_int, _i64, _p, _p, ctypes.POINTER(Domain), ctypes.c_int, ctypes.c_int,
+                                   _p, _p]
         L.or_rng_uniform.restype = None
         L.or_rng_uniform.argtypes = [ctypes.c_uint64, _i64, _p]
         _LIB = L
@@ -90,6 +96,34 @@ def cp2_eval(box, cuts, ball=None, cap=256):
                            cap, ctypes.byref(nit))
     n = nit.value
     return dict(status=st, area=area.value, nb_points=nbp.value, measures=meas[:n].copy(), ids=ids[:n].copy())
+
+
+def cp2_centroid(box, cuts, ball=None):
+    c = np.ascontiguousarray(np.asarray(cuts, dtype=np.float64).reshape(-1, 4))
+    out = np.zeros(2)
+    area = ctypes.c_double()
+    b = ball if ball is not None else (0.0, 0.0, 0.0)
+    st = lib().or_cp2_centroid(box[0], box[1], box[2], box[3], len(c), _ptr(c), int(ball is not None),
+                               b[0], b[1], b[2], _ptr(out), ctypes.byref(area))
+    return dict(status=st, centroid=out, area=area.value)
+
+
+def centroids(pos, w, dim=None, ball=False, nthreads=None, domain=None, img=None):
+    """get_centroids: (status, centroids[N, dim], masses[N])."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    dim = dim or pos.shape[1]
+    N = pos.shape[0]
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    if domain is None:
+        dom, keep = make_domain(dim, img=img)
+    else:
+        dom, keep = make_domain(dim, *domain, img=img)
+    c = np.zeros(N * dim)
+    m = np.zeros(N)
+    st = lib().or_centroids(dim, N, _ptr(pos), _ptr(w), ctypes.byref(dom), 1 if ball else 0,
+                            nthreads or os.cpu_count() or 1, _ptr(c), _ptr(m))
+    del keep
+    return st, c.reshape(N, dim), m
 
 
 def cp3_eval(p0, p1, cuts, cap=256):

@@ -220,6 +220,39 @@ double or_cp2_area(const or_cp2 *cp, double coeff) {
     return coeff * res;
 }
 
+/* add_centroid_contrib(Constant, Unit), ConvexPolyhedron2.tcc:1047-1073, _centroid_arc
+ * :2296-2330: area and first moments of the polygon, as signed triangles and
+ * circular sectors seen from (ox, oy) (the ball centre for arcs), relative to it */
+void or_cp2_moments(const or_cp2 *cp, double coeff, double ox, double oy, double *m0, double *mx, double *my) {
+    double a = 0, sx = 0, sy = 0;
+    if (cp->n == 0) {
+        if (cp->sr > 0) {
+            a = M_PI * cp->sr * cp->sr;
+            sx = a * (cp->scx - ox);
+            sy = a * (cp->scy - oy);
+        }
+    } else {
+        for (int i0 = cp->n - 1, i1 = 0; i1 < cp->n; i0 = i1++) {
+            double x0 = cp->x[i0] - ox, y0 = cp->y[i0] - oy, x1 = cp->x[i1] - ox, y1 = cp->y[i1] - oy;
+            if (cp->arc[i0]) {   /* sector of the ball (centred at the origin of the moments) */
+                double a0 = atan2(y0, x0), t = arc_angle(cp, i0, i1), r = cp->sr;
+                double mid = a0 + t / 2, k = 2.0 / 3.0 * r * r * r * sin(t / 2);
+                a += r * r * t / 2;
+                sx += k * cos(mid);
+                sy += k * sin(mid);
+            } else {
+                double cr = x0 * y1 - x1 * y0;
+                a += cr / 2;
+                sx += cr * (x0 + x1) / 6;
+                sy += cr * (y0 + y1) / 6;
+            }
+        }
+    }
+    *m0 += coeff * a;
+    *mx += coeff * sx;
+    *my += coeff * sy;
+}
+
 /* integration(Polynomial<6>, Unit), ConvexPolyhedron2.tcc:2181-2225: the integral of
  * c0 + c1 x + c2 y + c3 x^2 + c4 x y + c5 y^2 over the polygon, from its area moments
  * (Green's theorem over the edges, cr = x_k y_k+1 - x_k+1 y_k). */
@@ -291,6 +324,26 @@ int or_cp2_eval(double x0, double y0, double x1, double y1, int ncuts, const dou
     *area = or_cp2_area(cp, 1.0);
     *nb_points = cp->n;
     *nitems = or_cp2_boundary_items(cp, 1.0, measures, ids, cap);
+    free(cp);
+    return st;
+}
+
+int or_cp2_centroid(double x0, double y0, double x1, double y1, int ncuts, const double *cuts,
+                    int ball, double bcx, double bcy, double br, double *centroid, double *area) {
+    or_cp2 *cp = (or_cp2 *)malloc(sizeof(or_cp2));
+    or_cp2_box(cp, x0, y0, x1, y1, -1);
+    int st = OR_OK;
+    for (int k = 0; k < ncuts && st == OR_OK; ++k) {
+        int r = or_cp2_plane_cut(cp, cuts[4 * k + 0], cuts[4 * k + 1], cuts[4 * k + 2], cuts[4 * k + 3], k, 1);
+        if (r == OR_CAPACITY) st = r;
+    }
+    if (ball && st == OR_OK) st = or_cp2_ball_cut(cp, bcx, bcy, br, -2);
+    const double ox = ball ? bcx : 0.0, oy = ball ? bcy : 0.0;
+    double m0 = 0, mx = 0, my = 0;
+    or_cp2_moments(cp, 1.0, ox, oy, &m0, &mx, &my);
+    *area = m0;
+    centroid[0] = m0 != 0 ? ox + mx / m0 : 0.0;
+    centroid[1] = m0 != 0 ? oy + my / m0 : 0.0;
     free(cp);
     return st;
 }
@@ -448,6 +501,24 @@ double or_cp3_volume(const or_cp3 *cp, const double *center) {
     return res;
 }
 
+/* add_centroid_contrib(Constant, Unit), ConvexPolyhedron3.tcc:884-924: volume and first
+ * moments as signed tetrahedra (centre, fan triangle of each face), relative to centre */
+void or_cp3_moments(const or_cp3 *cp, const double *c, double coeff, double *m0, double *m) {
+    for (int f = 0; f < cp->nf; ++f) {
+        if (cp->nv[f] < 3) continue;
+        const double (*v)[3] = cp->v[f];
+        double a[3] = {v[0][0] - c[0], v[0][1] - c[1], v[0][2] - c[2]};
+        for (int k = 2; k < cp->nv[f]; ++k) {
+            double b[3] = {v[k - 1][0] - c[0], v[k - 1][1] - c[1], v[k - 1][2] - c[2]};
+            double d[3] = {v[k][0] - c[0], v[k][1] - c[1], v[k][2] - c[2]};
+            double vol = (a[0] * (b[1] * d[2] - b[2] * d[1]) - a[1] * (b[0] * d[2] - b[2] * d[0]) +
+                          a[2] * (b[0] * d[1] - b[1] * d[0])) / 6;
+            *m0 += coeff * vol;
+            for (int q = 0; q < 3; ++q) m[q] += coeff * vol * (a[q] + b[q] + d[q]) / 4;
+        }
+    }
+}
+
 int or_cp3_eval(const double *p0, const double *p1, int ncuts, const double *cuts,
                 double *volume, int *nb_faces, double *areas, int64_t *ids, int cap) {
     or_cp3 *cp = (or_cp3 *)malloc(sizeof(or_cp3));
@@ -560,6 +631,7 @@ typedef struct {
     const or_domain *dom;
     int flags;
     double *masses, *ball_arc, *perimeter;
+    double *centroids;     /* N*dim or NULL: mass-weighted centroid of each cell */
     int64_t cap;
     int64_t *row_len, *cols;
     double *vals;
@@ -598,7 +670,11 @@ static double dist_between(const double *pos, int dim, int64_t i, int64_t j) {
 /* the per-piece part of the get_der_integrals_wrt_weights callback
  * (get_der_integrals_wrt_weights.h:52-75) for one 2D piece of density coeff */
 static void piece2_accumulate(const cells_job *J, int64_t i, const or_cp2 *cp, double coeff, const double *poly,
-                              double *mass, double *arcm, double *perim, row_acc *row) {
+                              double *mass, double *arcm, double *perim, row_acc *row, double *mom) {
+    if (J->centroids && !poly) {
+        double m0 = 0;
+        or_cp2_moments(cp, coeff, J->pos[2 * i], J->pos[2 * i + 1], &m0, mom, mom + 1);
+    }
     static __thread double meas[2 * OR_CP2_CAP + 2];
     static __thread int64_t ids[2 * OR_CP2_CAP + 2];
     int m;
@@ -710,11 +786,11 @@ static int build_cell2(const cells_job *J, int64_t i, or_cp2 *cp) {
 static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
     row_acc row = { J->cols + i * J->cap, J->vals + i * J->cap, 1, J->cap, 0.0, 0 };
     row.cols[0] = i;
-    double mass = 0, arcm = 0, perim = 0;
+    double mass = 0, arcm = 0, perim = 0, mom[2] = {0, 0};
     int st = build_cell2(J, i, cp);
     const or_domain *dom = J->dom;
     if (!dom->img) {
-        piece2_accumulate(J, i, cp, dom->coeff, NULL, &mass, &arcm, &perim, &row);
+        piece2_accumulate(J, i, cp, dom->coeff, NULL, &mass, &arcm, &perim, &row, mom);
     } else {
         /* ScaledImage::for_each_intersection, ScaledImage.tcc:98-149 */
         double mn[2], mx[2];
@@ -754,12 +830,15 @@ static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
                         or_cp2_plane_cut(piece, cp->x[k], cp->y[k], cp->nx[k], cp->ny[k], cp->id[k], 1);
                     if (cp->sr > 0) or_cp2_ball_cut(piece, cp->scx, cp->scy, cp->sr, cp->sid);
                     else { piece->scx = cp->scx; piece->scy = cp->scy; piece->sr = cp->sr; piece->sid = cp->sid; }
-                    piece2_accumulate(J, i, piece, rho, dom->img_nc == 6 ? c6 : NULL, &mass, &arcm, &perim, &row);
+                    piece2_accumulate(J, i, piece, rho, dom->img_nc == 6 ? c6 : NULL, &mass, &arcm, &perim, &row, mom);
                 }
         }
     }
     row.vals[0] = row.der0;
     J->masses[i] = mass;
+    if (J->centroids) {   /* get_centroids.h:53-58: centroid / (mass + (mass == 0)) */
+        for (int d = 0; d < 2; ++d) J->centroids[2 * i + d] = mass != 0 ? J->pos[2 * i + d] + mom[d] / mass : 0.0;
+    }
     if (J->ball_arc) J->ball_arc[i] = arcm;
     if (J->perimeter) J->perimeter[i] = perim;
     J->row_len[i] = row.len;
@@ -860,8 +939,12 @@ static int build_cell3(const cells_job *J, int64_t i, or_cp3 *cp) {
 }
 
 static void piece3_accumulate(const cells_job *J, int64_t i, const or_cp3 *cp, double coeff, double *mass, double *perim,
-                              row_acc *row) {
+                              row_acc *row, double *mom) {
     *mass += coeff * or_cp3_volume(cp, J->pos + 3 * i);
+    if (J->centroids) {
+        double m0 = 0;
+        or_cp3_moments(cp, J->pos + 3 * i, coeff, &m0, mom);
+    }
     for (int f = 0; f < cp->nf; ++f) {
         if (cp->nv[f] < 3) continue;
         double a = coeff * or_cp3_face_area(cp, f);
@@ -880,11 +963,11 @@ static void piece3_accumulate(const cells_job *J, int64_t i, const or_cp3 *cp, d
 static void eval_cell3(cells_job *J, int64_t i, or_cp3 *cp, or_cp3 *piece) {
     row_acc row = { J->cols + i * J->cap, J->vals + i * J->cap, 1, J->cap, 0.0, 0 };
     row.cols[0] = i;
-    double mass = 0, perim = 0;
+    double mass = 0, perim = 0, mom[3] = {0, 0, 0};
     int st = build_cell3(J, i, cp);
     const or_domain *dom = J->dom;
     if (!dom->img) {
-        piece3_accumulate(J, i, cp, dom->coeff, &mass, &perim, &row);
+        piece3_accumulate(J, i, cp, dom->coeff, &mass, &perim, &row, mom);
     } else if (cp->nf) {
         double mn[3], mx[3];
         for (int d = 0; d < 3; ++d) { mn[d] = INFINITY; mx[d] = -INFINITY; }
@@ -916,11 +999,14 @@ static void eval_cell3(cells_job *J, int64_t i, or_cp3 *cp, or_cp3 *piece) {
                         if (cp->nv[f] < 3) continue;
                         or_cp3_plane_cut(piece, cp->v[f][0], cp->n[f], cp->id[f]);
                     }
-                    if (piece->nf) piece3_accumulate(J, i, piece, rho, &mass, &perim, &row);
+                    if (piece->nf) piece3_accumulate(J, i, piece, rho, &mass, &perim, &row, mom);
                 }
     }
     row.vals[0] = row.der0;
     J->masses[i] = mass;
+    if (J->centroids) {
+        for (int d = 0; d < 3; ++d) J->centroids[3 * i + d] = mass != 0 ? J->pos[3 * i + d] + mom[d] / mass : 0.0;
+    }
     if (J->ball_arc) J->ball_arc[i] = 0;
     if (J->perimeter) J->perimeter[i] = perim;
     J->row_len[i] = row.len;
@@ -960,10 +1046,39 @@ int or_cells(int dim, int64_t N, const double *pos, const double *w, const or_do
     return or_cells_range(dim, N, pos, w, dom, flags, nthreads, 0, N, masses, ball_arc, perimeter, cap, row_len, cols, vals);
 }
 
+static int cells_run(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom,
+                     int flags, int nthreads, int64_t i_begin, int64_t i_end,
+                     double *masses, double *ball_arc, double *perimeter, double *centroids,
+                     int64_t cap, int64_t *row_len, int64_t *cols, double *vals);
+
 int or_cells_range(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom,
                    int flags, int nthreads, int64_t i_begin, int64_t i_end,
                    double *masses, double *ball_arc, double *perimeter,
                    int64_t cap, int64_t *row_len, int64_t *cols, double *vals) {
+    return cells_run(dim, N, pos, w, dom, flags, nthreads, i_begin, i_end, masses, ball_arc, perimeter, NULL, cap,
+                     row_len, cols, vals);
+}
+
+int or_centroids(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom, int flags,
+                 int nthreads, double *centroids, double *masses) {
+    if (!centroids || !masses || N < 0) return OR_INVALID;
+    if (dom && dom->img && dom->img_nc != 1) return OR_INVALID;   /* quadratic pixels: not restated */
+    const int64_t cap = dim == 2 ? 64 : 160;
+    int64_t *row_len = (int64_t *)malloc(sizeof(int64_t) * (N + 1));
+    int64_t *cols = (int64_t *)malloc(sizeof(int64_t) * (N * cap + 1));
+    double *vals = (double *)malloc(sizeof(double) * (N * cap + 1));
+    int st = cells_run(dim, N, pos, w, dom, flags, nthreads, 0, N, masses, NULL, NULL, centroids, cap, row_len, cols,
+                       vals);
+    free(row_len);
+    free(cols);
+    free(vals);
+    return st;
+}
+
+static int cells_run(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom,
+                     int flags, int nthreads, int64_t i_begin, int64_t i_end,
+                     double *masses, double *ball_arc, double *perimeter, double *centroids,
+                     int64_t cap, int64_t *row_len, int64_t *cols, double *vals) {
     if (i_begin < 0 || i_end > N || i_begin > i_end) return OR_INVALID;
     if ((dim != 2 && dim != 3) || N < 0 || cap < 1 || !dom || dom->dim != dim) return OR_INVALID;
     if (dim == 3 && (flags & 1)) return OR_INVALID;  /* ConvexPolyhedron3::ball_cut is TODO (:292-293) */
@@ -973,7 +1088,7 @@ int or_cells_range(int dim, int64_t N, const double *pos, const double *w, const
     cells_job J;
     memset(&J, 0, sizeof(J));
     J.dim = dim; J.N = N; J.pos = pos; J.w = w; J.dom = dom; J.flags = flags;
-    J.masses = masses; J.ball_arc = ball_arc; J.perimeter = perimeter;
+    J.masses = masses; J.ball_arc = ball_arc; J.perimeter = perimeter; J.centroids = centroids;
     J.cap = cap; J.row_len = row_len; J.cols = cols; J.vals = vals;
     J.next = i_begin;
     J.end = i_end;

@@ -67,6 +67,8 @@ void    or_cp2_box(or_cp2 *cp, double x0, double y0, double x1, double y1, int64
 int     or_cp2_plane_cut(or_cp2 *cp, double ox, double oy, double nx, double ny, int64_t id, int normal_is_normalized);
 int     or_cp2_ball_cut(or_cp2 *cp, double cx, double cy, double r, int64_t id);
 double  or_cp2_area(const or_cp2 *cp, double coeff);
+/* area and first moments relative to (ox, oy), which must be the ball centre when there are arcs */
+void    or_cp2_moments(const or_cp2 *cp, double coeff, double ox, double oy, double *m0, double *mx, double *my);
 double  or_cp2_poly_area(const or_cp2 *cp, const double *c6);   /* integral of the quadratic over a straight-edged polygon */
 double  or_seg_poly(double x0, double y0, double x1, double y1, const double *c6);   /* integral along a segment */
 int     or_cp2_boundary_items(const or_cp2 *cp, double coeff, double *measures, int64_t *ids, int cap);
@@ -75,6 +77,10 @@ int     or_cp2_empty(const or_cp2 *cp);
 int     or_cp2_eval(double x0, double y0, double x1, double y1, int ncuts, const double *cuts,
                     int ball, double bcx, double bcy, double br,
                     double *area, int *nb_points, double *measures, int64_t *ids, int cap, int *nitems);
+
+/* same polygon as or_cp2_eval: its centroid and area */
+int     or_cp2_centroid(double x0, double y0, double x1, double y1, int ncuts, const double *cuts,
+                        int ball, double bcx, double bcy, double br, double *centroid, double *area);
 
 /* 3D polyhedron as a list of faces, each a CCW (seen from outside) vertex loop. */
 #define OR_CP3_MAXF 96
@@ -92,6 +98,8 @@ void    or_cp3_box(or_cp3 *cp, const double *p0, const double *p1, int64_t id);
 int     or_cp3_plane_cut(or_cp3 *cp, const double *origin, const double *normal, int64_t id);
 double  or_cp3_face_area(const or_cp3 *cp, int f);
 double  or_cp3_volume(const or_cp3 *cp, const double *center);
+/* volume and first moments relative to c */
+void    or_cp3_moments(const or_cp3 *cp, const double *c, double coeff, double *m0, double *m);
 int     or_cp3_eval(const double *p0, const double *p1, int ncuts, const double *cuts,
                     double *volume, int *nb_faces, double *areas, int64_t *ids, int cap);
 
@@ -123,6 +131,12 @@ int or_cells(int dim, int64_t N, const double *pos, const double *w, const or_do
              int flags, int nthreads,
              double *masses, double *ball_arc, double *perimeter,
              int64_t cap, int64_t *row_len, int64_t *cols, double *vals);
+
+/* get_centroids (get_centroids.h:14-60, rand_ratio = 0): per cell the centroid of
+ * the density over the cell (0 for an empty cell) and its mass.  Constant density
+ * or constant pixels/voxels (quadratic pixels: OR_INVALID). */
+int or_centroids(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom, int flags,
+                 int nthreads, double *centroids, double *masses);
 
 /* Same, evaluating only the cells [i_begin, i_end) (the grid is built over all
  * N diracs): a bounded sample of a large pass for timing. */

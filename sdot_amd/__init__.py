@@ -108,6 +108,7 @@ SIGNATURES = {
     "sdot_pd_set_ball_clip": (_int, [_p, _int]),
     "sdot_pd_compute": (_int, [_p]),
     "sdot_pd_integrals": (_int, [_p, _p]),
+    "sdot_pd_centroids": (_int, [_p, _p, _p]),
     "sdot_pd_boundary_measures": (_int, [_p, _p, _p]),
     "sdot_pd_der_integrals_wrt_weights": (_int, [_p, _p, _p, _p, _p, _i64, ctypes.POINTER(_i64)]),
     "sdot_ot_solve": (_int, [_p, _p, _d, _int, _int, _p, ctypes.POINTER(SolveStats)]),
@@ -266,6 +267,13 @@ class PowerDiagram:
         m = np.empty(self.N)
         self._check(lib().sdot_pd_integrals(self._h, _ptr(m)), "integrals", allow_empty)
         return m
+
+    def centroids(self, allow_empty=True):
+        """get_centroids (rand_ratio 0): (centroids[N, dim], masses[N]); 0 for empty cells."""
+        c = np.empty((self.N, self.dim))
+        m = np.empty(self.N)
+        self._check(lib().sdot_pd_centroids(self._h, _ptr(c), _ptr(m)), "centroids", allow_empty)
+        return c, m
 
     def boundary_measures(self, allow_empty=True):
         a = np.empty(self.N)

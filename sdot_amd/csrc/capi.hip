@@ -136,6 +136,12 @@ sdot_status compute_pass(sdot_pd *pd) {
     a.ell_stride = N;
     a.flags = pd->flags.p;
     a.stop_if_void = pd->stop_if_void ? 1 : 0;
+    a.mom = nullptr;
+    if (pd->want_mom) {
+        pd->mom.ensure((size_t)pd->dim * N);
+        a.mom = pd->mom.p;
+    }
+    pd->pass_has_mom = pd->want_mom;
     a.ovf_list = pd->ovf_list.p;
     a.ovf_count = pd->flags.p + 1;
     pd->wctr.ensure(3 * WCTR);
@@ -190,6 +196,7 @@ sdot_status compute_pass(sdot_pd *pd) {
 
 namespace {
 sdot_status ensure_pass(sdot_pd *pd) {
+    if (pd->pass_valid && pd->want_mom && !pd->pass_has_mom) pd->pass_valid = false;
     if (pd->pass_valid) {
         if (pd->last_flags & (FLAG_BAD_INPUT | FLAG_ROW_OVERFLOW)) return compute_pass(pd);
         return (pd->last_flags & FLAG_EMPTY) ? SDOT_EMPTY_CELL : SDOT_OK;
@@ -251,7 +258,7 @@ sdot_status sdot_pd_destroy(sdot_pd *pd) {
     pd->perm.release(); pd->inv_perm.release();
     pd->spos.release(); pd->sw.release(); pd->cell_wmax.release(); pd->bbox_part.release(); pd->wpyr.release(); pd->wglob.release();
     pd->mass.release(); pd->arc.release(); pd->perim.release(); pd->diag.release(); pd->ell_val.release();
-    pd->row_len.release(); pd->ell_col.release(); pd->ovf_list.release(); pd->flags.release(); pd->wctr.release();
+    pd->row_len.release(); pd->mom.release(); pd->ell_col.release(); pd->ovf_list.release(); pd->flags.release(); pd->wctr.release();
     pd->csr_off.release(); pd->csr_col.release(); pd->csr_cnt.release(); pd->csr_tmp.release();
     pd->csr_val.release(); pd->out_tmp.release(); pd->scratch3.release();
     for (auto &b : pd->sv) b.release();
@@ -422,6 +429,25 @@ sdot_status sdot_pd_integrals(sdot_pd *pd, double *masses) {
     return guard(pd, [&] {
         sdot_status st = ensure_pass(pd);
         export_vec(pd, pd->mass.p, masses);
+        return st;
+    });
+}
+
+sdot_status sdot_pd_centroids(sdot_pd *pd, double *centroids, double *masses) {
+    if (!pd || !centroids) return SDOT_INVALID_ARG;
+    if (pd->has_image && pd->img_nc != 1) {
+        pd->err = "centroids of quadratic pixels are not offered";
+        return SDOT_NOT_SUPPORTED;
+    }
+    return guard(pd, [&] {
+        struct Want {
+            sdot_pd *pd;
+            explicit Want(sdot_pd *p) : pd(p) { pd->want_mom = true; }
+            ~Want() { pd->want_mom = false; }
+        } want(pd);
+        sdot_status st = ensure_pass(pd);
+        export_centroids(pd, centroids);
+        if (masses) export_vec(pd, pd->mass.p, masses);
         return st;
     });
 }

@@ -86,6 +86,7 @@ struct Integrator {
     double arcm = 0, perim = 0;
     double der_line = 0;   // sum of 0.5*rho*|G_ij|/|x_i-x_j| (rho included)
     double der_arc = 0;    // sum of 0.5*da: 0.5*arc/sqrt(w_i) before the density factor
+    double mx = 0, my = 0; // first moments about the dirac (when a->mom): triangles and ball sectors
     int nrow = 0;
     int nlines = 0, narcs = 0;
     bool row_ovf = false;
@@ -98,6 +99,11 @@ struct Integrator {
         T ex = x1 - x0, ey = y1 - y0;
         double len = (double)sqrt(ex * ex + ey * ey);
         perim += len;
+        if (a->mom) {   // triangle (dirac, p0, p1): area cr/2, centroid (p0 + p1)/3
+            const double cr = (double)x0 * y1 - (double)x1 * y0;
+            mx += cr * ((double)x0 + x1) / 6;
+            my += cr * ((double)y0 + y1) / 6;
+        }
         if (id >= 0) {
             double vx = a->spos[2 * (int64_t)id] - c0x, vy = a->spos[2 * (int64_t)id + 1] - c0y;
             double dist = sqrt(vx * vx + vy * vy);
@@ -122,6 +128,12 @@ struct Integrator {
         arcm += l;
         perim += l;
         der_arc += 0.5 * (double)da;
+        if (a->mom) {   // ball sector (_centroid_arc, ConvexPolyhedron2.tcc:2296-2330)
+            const double mid = (double)a0 + 0.5 * (double)da;
+            const double k = 2.0 / 3.0 * (double)r * r * r * sin(0.5 * (double)da);
+            mx += k * cos(mid);
+            my += k * sin(mid);
+        }
         T ex = x1 - x0, ey = y1 - y0;
         T d0 = sqrt(ex * ex + ey * ey) / 2;
         T d1 = sqrt(fmax(T(0), r2 - d0 * d0));
@@ -449,7 +461,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             i0[d] = u0 < 0 ? 0 : (u0 >= a.img_n[d] ? a.img_n[d] - 1 : u0);
             i1[d] = u1 < 0 ? 0 : (u1 >= a.img_n[d] ? a.img_n[d] - 1 : u1);
         }
-        double mass_sum = 0;
+        double mass_sum = 0, mom_x = 0, mom_y = 0;
         bool fits = true;
         const bool poly = a.img_nc == 6;
         const int64_t npix = (int64_t)a.img_n[0] * a.img_n[1];
@@ -494,7 +506,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                     const double x0 = pc.x[k], y0 = pc.y[k], x1 = pc.x[k1], y1 = pc.y[k1];
                     const double cr = x0 * y1 - x1 * y0;
                     m0 += cr;
-                    if (poly) {
+                    if (poly || a.mom) {
                         sx += (x0 + x1) * cr;
                         sy += (y0 + y1) * cr;
                         sxx += (x0 * x0 + x0 * x1 + x1 * x1) * cr;
@@ -513,6 +525,10 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                 mass_sum += poly ? q[0] * m0 / 2 + q[1] * sx / 6 + q[2] * sy / 6 + q[3] * sxx / 12 + q[4] * sxy / 24 +
                                        q[5] * syy / 12
                                  : rho * 0.5 * m0;
+                if (!poly) {   // first moments of a constant piece (quadratic pixels: not offered)
+                    mom_x += rho * sx / 6;
+                    mom_y += rho * sy / 6;
+                }
             }
         if (!fits) flag_or(a.flags, FLAG_CELL_OVERFLOW);
         double der_line = 0, perim = 0;
@@ -536,6 +552,10 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             der_line += b;
         }
         a.mass[t] = mass_sum;
+        if (a.mom) {
+            a.mom[2 * t] = mom_x;
+            a.mom[2 * t + 1] = mom_y;
+        }
         a.arc[t] = 0;
         a.perim[t] = perim;
         a.diag[t] = der_line;
@@ -656,6 +676,10 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     const double coeff = a.coeff;
     const double mass = coeff * area;
     a.mass[t] = mass;
+    if (a.mom) {
+        a.mom[2 * t] = coeff * it.mx;
+        a.mom[2 * t + 1] = coeff * it.my;
+    }
     a.arc[t] = coeff * it.arcm;
     a.perim[t] = coeff * it.perim;
     a.diag[t] = it.der_line + coeff * it.der_arc;

@@ -317,8 +317,11 @@ struct Cell3 {
         z = (n0[3] * c12z + n1[3] * c20z + n2[3] * c01z) / det;
     }
 
-    // face areas of the current polyhedron into m.acc[0..np), returns the volume
-    SDOT_HD double integrate() {
+    // face areas of the current polyhedron into m.acc[0..np), returns the volume; with
+    // `off`, also the centroid offset from the dirac (signed tetrahedra: dirac, foot of
+    // the face plane, directed edge; add_centroid_contrib, ConvexPolyhedron3.tcc:884-924)
+    SDOT_HD double integrate(double *off = nullptr) {
+        double tv = 0, tm[3] = {0, 0, 0};
         for (int k = 0; k < np; ++k) m.acc[k * m.S] = 0;
         for (int k = 0; k < nv; ++k) {
             const int t = V_T(k);
@@ -334,9 +337,19 @@ struct Cell3 {
                 double fx, fy, fz, fh;
                 plane64(f, fx, fy, fz, fh);
                 const double cx = y0 * z1 - z0 * y1, cy = z0 * x1 - x0 * z1, cz = x0 * y1 - y0 * x1;
-                m.acc[f * m.S] += (cx * fx + cy * fy + cz * fz) / sqrt(fx * fx + fy * fy + fz * fz);
+                const double cn = cx * fx + cy * fy + cz * fz, n2 = fx * fx + fy * fy + fz * fz;
+                m.acc[f * m.S] += cn / sqrt(n2);
+                if (off) {
+                    const double s = fh / n2, v = s * cn / 6;
+                    tv += v;
+                    tm[0] += v * (s * fx + x0 + x1) / 4;
+                    tm[1] += v * (s * fy + y0 + y1) / 4;
+                    tm[2] += v * (s * fz + z0 + z1) / 4;
+                }
             }
         }
+        if (off)
+            for (int d = 0; d < 3; ++d) off[d] = tv != 0 ? tm[d] / tv : 0.0;
         double vol = 0;
         for (int k = 0; k < np; ++k) {
             const double ar = 0.5 * fabs(m.acc[k * m.S]);
@@ -466,13 +479,16 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
     if (C.nv > 0) C.compact_planes();   // voxel cuts then start from the face count
 
     // ---- integrals ----
-    double mass = 0, perim = 0;
+    double mass = 0, perim = 0, mom[3] = {0, 0, 0}, off[3];
+    double *const want = a.mom ? off : nullptr;
     for (int k = 0; k < PCAP; ++k) C.m.hacc[k * C.m.S] = 0;
     const int npc = C.np;
     if (C.nv > 0) {
         if (!a.img) {
-            const double vol = C.integrate();
+            const double vol = C.integrate(want);
             mass = a.coeff * vol;
+            if (want)
+                for (int d = 0; d < 3; ++d) mom[d] = mass * off[d];
             for (int k = 0; k < npc; ++k) C.m.hacc[k * C.m.S] = a.coeff * C.m.acc[k * C.m.S];
         } else {
             // voxel pieces: the cell clipped by each overlapping voxel (ScaledImage.tcc:98-149)
@@ -518,8 +534,10 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
                         }
                         if (overflow) break;
                         if (!empty) {
-                            const double vol = C.integrate();
+                            const double vol = C.integrate(want);
                             mass += rho * vol;
+                            if (want)
+                                for (int d = 0; d < 3; ++d) mom[d] += rho * vol * off[d];
                             for (int k = 0; k < npc; ++k) C.m.hacc[k * C.m.S] += rho * C.m.acc[k * C.m.S];
                         }
                         C.restore(nv0, np0);
@@ -562,6 +580,8 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
         der += b;
     }
     a.mass[t] = mass;
+    if (a.mom)
+        for (int d = 0; d < 3; ++d) a.mom[3 * t + d] = mom[d];
     a.arc[t] = 0;
     a.perim[t] = perim;
     a.diag[t] = der;

@@ -192,6 +192,30 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
     a.work_count = nullptr;
 }
 
+namespace {
+// get_centroids.h:53-58: centroid / (mass + (mass == 0)); the moments are about the dirac
+__global__ void k_export_centroids(const double *spos, const double *mass, const double *mom, const int *inv_perm,
+                                   int64_t n, int dim, double *out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t t = inv_perm[i];
+    const double m = mass[t];
+    for (int d = 0; d < dim; ++d) out[dim * i + d] = m != 0 ? spos[dim * t + d] + mom[dim * t + d] / m : 0.0;
+}
+}  // namespace
+
+void export_centroids(sdot_pd *pd, double *h_out) {
+    const int64_t N = pd->N;
+    const int dim = pd->dim;
+    if (N == 0) return;
+    pd->out_tmp.ensure((size_t)dim * N);
+    hipLaunchKernelGGL(k_export_centroids, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, pd->stream, pd->spos.p,
+                       pd->mass.p, pd->mom.p, pd->inv_perm.p, N, dim, pd->out_tmp.p);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(h_out, pd->out_tmp.p, (size_t)dim * N * sizeof(double), hipMemcpyDeviceToHost, pd->stream));
+    HIPCK(hipStreamSynchronize(pd->stream));
+}
+
 void export_vec(sdot_pd *pd, const double *d_sorted, double *h_out) {
     const int64_t N = pd->N;
     pd->out_tmp.ensure(N);

@@ -453,14 +453,18 @@ struct WaveCell {
         Z = (n0[3] * c12z + n1[3] * c20z + n2[3] * c01z) / det;
     }
 
-    // face areas into L.acc[0..np), returns the volume (all lanes)
-    __device__ double integrate() {
+    // face areas into L.acc[0..np), returns the volume (all lanes).  With `mom`, also
+    // the first moments about the dirac (all lanes): signed tetrahedra (dirac, foot
+    // of the face plane, edge) summed over the directed edges of every face, scaled
+    // to the volume (add_centroid_contrib, ConvexPolyhedron3.tcc:884-924)
+    __device__ double integrate(double *mom = nullptr) {
         if (lane < np) L.acc[lane] = 0;
         wave_sync();
         double X, Y, Z;
         vertex64(X, Y, Z);
         const bool valid = lane < nv;
         const int p[3] = {tri & 255, (tri >> 8) & 255, (tri >> 16) & 255};
+        double tv = 0, tmx = 0, tmy = 0, tmz = 0;
 #pragma unroll
         for (int e = 0; e < 3; ++e) {
             const int f = p[e], gq = p[e == 2 ? 0 : e + 1];
@@ -470,8 +474,23 @@ struct WaveCell {
                 double fx, fy, fz, fh;
                 plane64(f, fx, fy, fz, fh);
                 const double cx = Y * Zq - Z * Yq, cy = Z * Xq - X * Zq, cz = X * Yq - Y * Xq;
-                atomicAdd(&L.acc[f], (cx * fx + cy * fy + cz * fz) / sqrt(fx * fx + fy * fy + fz * fz));
+                const double cn = cx * fx + cy * fy + cz * fz, n2 = fx * fx + fy * fy + fz * fz;
+                atomicAdd(&L.acc[f], cn / sqrt(n2));
+                if (mom) {
+                    const double s = fh / n2, v = s * cn / 6;   // foot F = s n, volume F.(P x Q)/6
+                    tv += v;
+                    tmx += v * (s * fx + X + Xq) / 4;
+                    tmy += v * (s * fy + Y + Yq) / 4;
+                    tmz += v * (s * fz + Z + Zq) / 4;
+                }
             }
+        }
+        if (mom) {
+            const double sv = wave_sum_dpp(tv);
+            const double k = sv != 0 ? 1.0 / sv : 0.0;   // moments / signed volume = centroid offset
+            mom[0] = wave_sum_dpp(tmx) * k;
+            mom[1] = wave_sum_dpp(tmy) * k;
+            mom[2] = wave_sum_dpp(tmz) * k;
         }
         wave_sync();
         double vol = 0;
@@ -710,14 +729,17 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
             continue;
         }
         // ---- integrals ----
-        double mass = 0;
+        double mass = 0, mom[3] = {0, 0, 0}, off[3];
+        double *const want = a.mom ? off : nullptr;
         const int npc = C.np;
         if (lane < npc) L.hacc[lane] = 0;
         wave_sync();
         if (C.nv > 0) {
             if (!a.img) {
-                const double vol = C.integrate();
+                const double vol = C.integrate(want);
                 mass = a.coeff * vol;
+                if (want)
+                    for (int d = 0; d < 3; ++d) mom[d] = mass * off[d];
                 if (lane < npc) L.hacc[lane] = a.coeff * L.acc[lane];
             } else {
                 double X, Y, Z;
@@ -766,8 +788,10 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
                                 if (C.nv == 0) empty = true;
                             }
                             if (!overflow && !empty) {
-                                const double vol = C.integrate();
+                                const double vol = C.integrate(want);
                                 mass += rho * vol;
+                                if (want)
+                                    for (int d = 0; d < 3; ++d) mom[d] += rho * vol * off[d];
                                 if (lane < npc) L.hacc[lane] += rho * L.acc[lane];
                             }
                             // restore the cell
@@ -812,6 +836,8 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
         const double perim = wave_sum_dpp(mine && (!a.img || id >= 0) ? ar : 0.0);
         if (lane == 0) {
             a.mass[t] = mass;
+            if (a.mom)
+                for (int d = 0; d < 3; ++d) a.mom[3 * t + d] = mom[d];
             a.arc[t] = 0;
             a.perim[t] = perim;
             a.diag[t] = der;

@@ -185,6 +185,7 @@ struct CellArgs {
     int img_nc;   // coefficients per pixel: 1, or 6 (2D quadratic, coefficient-major)
     // outputs, sorted order
     double *mass, *arc, *perim, *diag;
+    double *mom;   // density-weighted first moments about the dirac, dim per cell (null: not wanted)
     int *row_len;
     int *ell_col;
     double *ell_val;
@@ -319,6 +320,9 @@ struct sdot_pd {
 
     // pass outputs (sorted order)
     sdot_amd::DevBuf<double> mass, arc, perim, diag, ell_val;
+    sdot_amd::DevBuf<double> mom;   // first moments (centroids), dim per cell
+    bool want_mom = false;          // the next pass computes the first moments
+    bool pass_has_mom = false;      // the current pass did
     sdot_amd::DevBuf<int> row_len, ell_col, ovf_list;
     sdot_amd::DevBuf<unsigned> flags;  // [0] flags, [1] overflow count, [2] second-tier overflow count
     int ell_cap = 0;
@@ -363,6 +367,7 @@ void run_cells3(sdot_pd *pd, CellArgs &a);
 void fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
 void export_csr(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, int64_t cap, int64_t *nnz);
 void export_vec(sdot_pd *pd, const double *d_sorted, double *h_out);  // sorted -> dirac order, to host
+void export_centroids(sdot_pd *pd, double *h_out);   // position + moment / mass (0 when empty), dirac order
 // capi.hip
 sdot_status compute_pass(sdot_pd *pd);  // grid (if needed) + weights + cell pass; flags -> status
 // solver.hip

@@ -96,7 +96,22 @@ def known_values():
     img.append(dict(src="tests/test_scaled_image.cpp:71-92", lo=[0, 0], hi=[1, 1], sizes=[1, 1], nb_coeffs=6,
                     data=[2, 3, 4, 5, 6, 7], positions=[[0.25, 0.25], [0.75, 0.25], [0.25, 0.75], [0.75, 0.75]],
                     weights=[1.0] * 4, integrals=[1.28125, 2.46875, 2.84375, 4.40625], tol=1e-6))
-    return dict(cp2=cases, cp3=cp3, scaled_image=img)
+    # tests/test_ConvexPolyhedron2.cpp:265-310 "centroid" and :322-336 "integration_only_lines":
+    # polygon / disc pieces and their centroids
+    B = [-10, -10, 10, 10]
+    cen = [
+        dict(src=kv + ":273-277 full ball", box=B, cuts=[], ball=[0, 0, 1], centroid=[0.0, 0.0], tol=1e-6),
+        dict(src=kv + ":279-284 half disc", box=B, cuts=[[0, 0, 1, 0]], ball=[0, 0, 1], centroid=[-0.424413, 0.0],
+             tol=1e-4),
+        dict(src=kv + ":286-292 quarter disc", box=B, cuts=[[0, 0, 1, 0], [0, 0, 0, 1]], ball=[0, 0, 1],
+             centroid=[-0.424413, -0.424413], tol=1e-4),
+        dict(src=kv + ":294-300 quarter disc r 0.5", box=B, cuts=[[0, 0, 1, 0], [0, 0, 0, 1]], ball=[0, 0, 0.5],
+             centroid=[-0.212207, -0.212207], tol=1e-4),
+        dict(src=kv + ":302-310 only lines", box=B, cuts=[[1, 0, 1, 0], [0, 1, 0, 1], [-1, 0, -1, 0], [0, -1, 0, -1]],
+             ball=[0, 0, 10], centroid=[0.0, 0.0], tol=1e-6),
+        dict(src=kv + ":322-336 box", box=[0, 0, 2, 1], cuts=[], ball=None, centroid=[1.0, 0.5], tol=1e-5),
+    ]
+    return dict(cp2=cases, cp3=cp3, scaled_image=img, cp2_centroid=cen)
 
 
 def rng_first8():

@@ -83,3 +83,28 @@ def test_multigrid_and_jacobi_preconditioners_agree(dim, n):
     assert np.abs(d1 - d2).max() < 1e-6 * np.abs(d2).max()
     print(f"[precond {dim}D n={n}] amg {sa['pcg_iters_total']} iters ({sa['precond_levels']} levels), "
           f"jacobi {sj['pcg_iters_total']}; newton {sa['newton_iters']} / {sj['newton_iters']}")
+
+
+@pytest.mark.parametrize("dim,precision", [(2, "mixed"), (3, "fp64")])
+def test_newton_with_backtracks_reaches_the_oracle_solution(dim, precision):
+    """Targets far from the start: the first Newton steps leave cells empty, so
+    the solve backtracks (and its passes stop at the first empty cell); the
+    weights it returns still solve the mass equations, as the oracle's own
+    solve (complete passes) does."""
+    import sdot_amd
+    n = 6000 if dim == 2 else 4000
+    pos, _ = inputs(dim, n, 31)
+    t = 1 + 20 * O.rng_uniform(32, n) ** 4
+    t /= t.sum()
+    ot = sdot_amd.OptimalTransport(pos, t, precision=precision)
+    w = ot.adjust_weights(tol=1e-7)
+    assert ot.status == 0, ot.stats
+    assert ot.stats["backtracks"] >= 1, ot.stats
+    m = O.cells(pos, w)["masses"]
+    assert np.abs(m / t - 1).max() < 2e-7
+    st, wo, so = O.ot_solve(pos, np.zeros(n), t, tol=1e-7)
+    assert st == 0 and so.backtracks >= 1
+    d1, d2 = w - w.mean(), wo - wo.mean()
+    assert np.abs(d1 - d2).max() < 1e-4 * np.abs(d2).max()
+    print(f"[newton backtracks {dim}D] {precision}: iters {ot.stats['newton_iters']} backtracks "
+          f"{ot.stats['backtracks']}; oracle iters {so.newton_iters} backtracks {so.backtracks}")

@@ -168,3 +168,57 @@ def test_empty_cell_status():
     pos = np.array([[0.3, 0.5], [0.5, 0.5], [0.7, 0.5]])
     w = np.array([0.0, -1.0, 0.0])
     assert O.cells(pos, w)["status"] == O.OR_EMPTY_CELL
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_centroids_lattice_and_split(dim):
+    # a lattice with equal weights: the cells are the lattice cubes, centroids their centres
+    k = 6
+    g = (np.arange(k) + 0.5) / k
+    pos = np.array(np.meshgrid(*([g] * dim), indexing="ij")).reshape(dim, -1).T.copy()
+    st, c, m = O.centroids(pos, np.zeros(len(pos)))
+    assert st == 0
+    assert np.abs(c - pos).max() < 1e-12 and np.abs(m - k ** -dim).max() < 1e-14
+    # two diracs on the x axis with w0 - w1 = delta: slabs split at x = 0.5 + delta
+    delta = 0.07
+    pos = np.full((2, dim), 0.5)
+    pos[0, 0], pos[1, 0] = 0.25, 0.75
+    st, c, m = O.centroids(pos, np.array([delta, 0.0]))
+    xe = 0.5 + delta
+    assert np.allclose(c[0], [xe / 2] + [0.5] * (dim - 1), atol=1e-13)
+    assert np.allclose(c[1], [(1 + xe) / 2] + [0.5] * (dim - 1), atol=1e-13)
+
+
+@pytest.mark.parametrize("dim,ball,img", [(2, False, False), (2, True, False), (2, False, True), (3, False, False),
+                                          (3, False, True)])
+def test_centroids_first_moment_of_the_domain(dim, ball, img):
+    # sum_i m_i c_i is the first moment of the density over the covered region:
+    # the box (or image) moment without ball; with disjoint discs, sum m_i x_i
+    n = 400
+    pos, w = inputs(dim, n, 41, wscale=0.2 * n ** (-2 / dim))
+    image = None
+    if img:
+        rng = np.random.default_rng(3)
+        image = rng.uniform(0.5, 2.0, size=(5,) * dim)
+    if ball:
+        w = np.full(n, 0.05 / n)     # small discs, mostly away from their neighbours
+    st, c, m = O.centroids(pos, w, ball=ball, img=image)
+    assert st in (0, 1)
+    total = (m[:, None] * c).sum(axis=0)
+    if ball:
+        # every cell is a disc clipped by neighbours' radical lines; the whole-cell
+        # check is against the oracle's own masses: a cell that is a full disc has its
+        # centroid at the dirac
+        full = np.abs(m - np.pi * w) < 1e-15
+        assert full.sum() > n // 4
+        assert np.abs(c[full] - pos[full]).max() < 1e-12
+        return
+    if image is None:
+        expected = np.full(dim, 0.5)
+    else:
+        s = image.shape[0]
+        ctr = (np.arange(s) + 0.5) / s
+        # image indexed [z][y][x]: axis d of the point is array axis dim-1-d
+        expected = np.array([(image * np.expand_dims(ctr, tuple(a for a in range(dim) if a != dim - 1 - d))).mean()
+                             for d in range(dim)])
+    assert np.abs(total - expected).max() < 1e-12

@@ -203,3 +203,10 @@ def test_quadratic_image_rejected_in_3d_and_with_ball():
     assert r["status"] == O.OR_INVALID
     r = O.cells(np.array([[0.5, 0.5]]), np.ones(1), img=np.ones((6, 1, 1)), ball=True)
     assert r["status"] == O.OR_INVALID
+
+
+@pytest.mark.parametrize("case", KV["cp2_centroid"], ids=[c["src"] for c in KV["cp2_centroid"]])
+def test_cp2_centroid_known_values(case):
+    r = O.cp2_centroid(case["box"], case["cuts"], ball=case["ball"])
+    assert r["status"] == 0
+    assert np.abs(r["centroid"] - np.array(case["centroid"])).max() < case["tol"]
