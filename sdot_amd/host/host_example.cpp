@@ -53,8 +53,17 @@ int main(int argc, char **argv) {
     }
     std::vector<double> ball(n);
     get_integrals(ball.data(), grid, bounds, positions.data(), weights.data(), n, FunctionEnum::InBallW05());
+    // centroids (get_centroids.h:14): the mass-weighted mean of the centroids is the
+    // centre of the unit square
+    double cm[2] = {0, 0}, csum = 0;
+    get_centroids(grid, bounds, positions.data(), weights.data(), n, [&](const Point2 &c, double m, std::size_t) {
+        cm[0] += m * c[0];
+        cm[1] += m * c[1];
+        csum += m;
+    });
+    const double cerr = std::fmax(std::fabs(cm[0] / csum - 0.5), std::fabs(cm[1] / csum - 0.5));
     std::printf("{\"status\": %d, \"der_status\": %d, \"newton_iters\": %d, \"max_rel_mass_err\": %.3e, \"mass_sum\": %.15f, "
-                "\"nnz\": %zu, \"max_row_sum\": %.3e, \"ball_mass_0\": %.6e}\n",
-                st, err, solver.stats.newton_iters, max_rel, sum, columns.size(), max_rowsum, ball[0]);
-    return (st == 0 && err == 0 && max_rel < 1e-6 && max_rowsum < 1e-9) ? 0 : 1;
+                "\"nnz\": %zu, \"max_row_sum\": %.3e, \"ball_mass_0\": %.6e, \"centroid_mean_err\": %.3e}\n",
+                st, err, solver.stats.newton_iters, max_rel, sum, columns.size(), max_rowsum, ball[0], cerr);
+    return (st == 0 && err == 0 && max_rel < 1e-6 && max_rowsum < 1e-9 && cerr < 1e-12) ? 0 : 1;
 }

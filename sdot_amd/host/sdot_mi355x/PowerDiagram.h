@@ -9,6 +9,7 @@
 //   ScaledImage<Pc>(min, max, data, sizes, nb_coeffs) src/sdot/Domains/ScaledImage.h:26
 //   FunctionEnum::Unit, FunctionEnum::InBallW05     src/sdot/Integration/Unit.h, InBallW05.h
 //   get_integrals(res, grid, bounds, pos, w, N, rf) src/sdot/PowerDiagram/get_integrals.h:13
+//   get_centroids(grid, bounds, pos, w, N, rf, cb)  src/sdot/PowerDiagram/get_centroids.h:14
 //   get_der_integrals_wrt_weights(...) -> int       src/sdot/PowerDiagram/get_der_integrals_wrt_weights.h:16
 //   OptimalTransportSolver<Grid,Bounds>::solve(...) src/sdot/Solvers/OptimalTransportSolver.h:18-26
 //
@@ -181,6 +182,30 @@ void get_integrals(TF *res, Grid &grid, Bounds &bounds, const Pt *positions, con
 template <class TF, class Grid, class Bounds, class Pt>
 void get_integrals(TF *res, Grid &grid, Bounds &bounds, const Pt *positions, const TF *weights, std::size_t nb_diracs) {
     get_integrals(res, grid, bounds, positions, weights, nb_diracs, FunctionEnum::Unit());
+}
+
+/// get_centroids (get_centroids.h:14, rand_ratio = 0): cb(centroid, mass, num_dirac) for
+/// every cell, in dirac order; the centroid of an empty cell is 0 (get_centroids.h:58).
+template <class Grid, class Bounds, class Pt, class TF, class TI, class Func, class CB>
+void get_centroids(Grid &grid, Bounds &bounds, const Pt *positions, const TF *weights, TI nb_diracs, const Func &,
+                   const CB &cb, TF rand_ratio = TF(0)) {
+    if (rand_ratio != TF(0)) throw Error{SDOT_NOT_SUPPORTED, "get_centroids: rand_ratio (random points) is not offered"};
+    constexpr int dim = sizeof(Pt) / sizeof(double);
+    grid.update(positions, weights, nb_diracs, false, true);
+    bounds.apply(grid.handle());
+    check(grid.handle(), sdot_pd_set_ball_clip(grid.handle(), Func::ball), "set_ball_clip");
+    std::vector<double> c((std::size_t)nb_diracs * dim), m(nb_diracs);
+    check(grid.handle(), sdot_pd_centroids(grid.handle(), c.data(), m.data()), "get_centroids", true);
+    for (std::size_t i = 0; i < (std::size_t)nb_diracs; ++i) {
+        Pt p;
+        for (int d = 0; d < dim; ++d) p[d] = c[dim * i + d];
+        cb(p, (TF)m[i], i);
+    }
+}
+
+template <class Grid, class Bounds, class Pt, class TF, class TI, class CB>
+void get_centroids(Grid &grid, Bounds &bounds, const Pt *positions, const TF *weights, TI nb_diracs, const CB &cb) {
+    get_centroids(grid, bounds, positions, weights, nb_diracs, FunctionEnum::Unit(), cb);
 }
 
 /// get_der_integrals_wrt_weights (get_der_integrals_wrt_weights.h:16): CSR of d mass / d w

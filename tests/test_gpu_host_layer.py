@@ -21,3 +21,5 @@ def test_host_example_solves_and_exports_csr():
     assert r["status"] == 0 and r["der_status"] == 0
     assert r["max_rel_mass_err"] < 1e-6
     assert abs(r["mass_sum"] - 1) < 1e-12
+    # get_centroids: the mass-weighted mean centroid is the centre of the unit square
+    assert r["centroid_mean_err"] < 1e-12
