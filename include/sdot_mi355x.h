@@ -74,6 +74,17 @@ sdot_status sdot_pd_mark_diracs_changed(sdot_pd *pd, int positions_changed);
  * One box item is supported on the device path (the configuration every
  * workload uses); several items return SDOT_NOT_SUPPORTED at evaluation. */
 sdot_status sdot_pd_add_box_shape(sdot_pd *pd, const double *p0, const double *p1, double coeff, int64_t cut_id);
+/* add_convex_polyhedron(positions, normals, coeff, cut_id) -- ConvexPolyhedronAssembly.tcc:7:
+ * the region (x - positions[i]) . normals[i] <= 0 for i < n (n*dim values each), inside
+ * the box of the positions widened by its extent on each side.  cut_id = -1. */
+sdot_status sdot_pd_add_convex_polyhedron_shape(sdot_pd *pd, const double *positions, const double *normals, int64_t n,
+                                                double coeff, int64_t cut_id);
+/* add_simplex(points, coeff, cut_id) -- ConvexPolyhedronAssembly.tcc:36: (dim+1)*dim points. */
+sdot_status sdot_pd_add_simplex_shape(sdot_pd *pd, const double *points, double coeff, int64_t cut_id);
+/* Several shapes: each cell is integrated piece by piece over the shapes
+ * (for_each_intersection, ConvexPolyhedronAssembly.tcc:132-142), densities adding up
+ * where shapes overlap; the perimeter then counts the faces shared with other cells
+ * only.  Not with the ball clip. */
 /* ScaledImage(min_pt, max_pt, data, sizes, nb_coeffs) -- ScaledImage.h:26; sizes x fastest.
  * nb_coeffs = 1: constant per pixel.  nb_coeffs = 6 (2D only, no ball): the density in
  * pixel p is c0 + c1 x + c2 y + c3 x^2 + c4 x y + c5 y^2 in domain coordinates, with

@@ -22,7 +22,8 @@ _p = ctypes.c_void_p
 
 class Domain(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("p0", _d * 3), ("p1", _d * 3), ("coeff", _d),
-                ("img", _p), ("img_sizes", _i64 * 3), ("img_nc", _i64)]
+                ("img", _p), ("img_sizes", _i64 * 3), ("img_nc", _i64),
+                ("nitems", _i64), ("it_box", _p), ("it_coeff", _p), ("it_planes", _p), ("it_off", _p)]
 
 
 class SolveStats(ctypes.Structure):
@@ -108,16 +109,16 @@ def cp2_centroid(box, cuts, ball=None):
     return dict(status=st, centroid=out, area=area.value)
 
 
-def centroids(pos, w, dim=None, ball=False, nthreads=None, domain=None, img=None):
+def centroids(pos, w, dim=None, ball=False, nthreads=None, domain=None, img=None, items=None):
     """get_centroids: (status, centroids[N, dim], masses[N])."""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     dim = dim or pos.shape[1]
     N = pos.shape[0]
     w = np.ascontiguousarray(w, dtype=np.float64)
     if domain is None:
-        dom, keep = make_domain(dim, img=img)
+        dom, keep = make_domain(dim, img=img, items=items)
     else:
-        dom, keep = make_domain(dim, *domain, img=img)
+        dom, keep = make_domain(dim, *domain, img=img, items=items)
     c = np.zeros(N * dim)
     m = np.zeros(N)
     st = lib().or_centroids(dim, N, _ptr(pos), _ptr(w), ctypes.byref(dom), 1 if ball else 0,
@@ -141,7 +142,44 @@ def cp3_eval(p0, p1, cuts, cap=256):
     return dict(status=st, volume=vol.value, nb_faces=n, areas=areas[:n].copy(), ids=ids[:n].copy())
 
 
-def make_domain(dim, p0=None, p1=None, coeff=1.0, img=None):
+def convex_polyhedron_item(positions, normals, coeff=1.0):
+    """ConvexPolyhedronAssembly::add_convex_polyhedron (ConvexPolyhedronAssembly.tcc:7-33):
+    the box of the points widened by its extent on each side, cut by the planes
+    (x - p_k) . n_k <= 0."""
+    p = np.asarray(positions, dtype=np.float64)
+    n = np.asarray(normals, dtype=np.float64)
+    lo, hi = p.min(axis=0), p.max(axis=0)
+    e = hi - lo
+    h = (p * n).sum(axis=1)
+    return dict(lo=lo - e, hi=hi + e, coeff=coeff, planes=np.column_stack([n, h]))
+
+
+def simplex_item(points, coeff=1.0):
+    """ConvexPolyhedronAssembly::add_simplex (:36-43): dim+1 points; one plane per facet,
+    oriented away from the opposite vertex."""
+    q = np.asarray(points, dtype=np.float64)
+    dim = q.shape[1]
+    planes = []
+    for k in range(dim + 1):
+        f = np.delete(q, k, axis=0)
+        if dim == 2:
+            t = f[1] - f[0]
+            nrm = np.array([t[1], -t[0]])
+        else:
+            nrm = np.cross(f[1] - f[0], f[2] - f[0])
+        if np.dot(q[k] - f[0], nrm) > 0:
+            nrm = -nrm
+        planes.append(np.concatenate([nrm, [np.dot(nrm, f[0])]]))
+    return dict(lo=q.min(axis=0), hi=q.max(axis=0), coeff=coeff, planes=np.array(planes))
+
+
+def box_item(p0, p1, coeff=1.0):
+    dim = len(p0)
+    return dict(lo=np.asarray(p0, dtype=np.float64), hi=np.asarray(p1, dtype=np.float64), coeff=coeff,
+                planes=np.zeros((0, dim + 1)))
+
+
+def make_domain(dim, p0=None, p1=None, coeff=1.0, img=None, items=None):
     d = Domain()
     d.dim = dim
     p0 = p0 if p0 is not None else [0.0] * dim
@@ -163,10 +201,34 @@ def make_domain(dim, p0=None, p1=None, coeff=1.0, img=None):
     else:
         d.img = None
         d.img_nc = 1
+    d.nitems = 0
+    if items:
+        box = np.zeros((len(items), 6))
+        off = np.zeros(len(items) + 1, dtype=np.int64)
+        planes = []
+        for k, it in enumerate(items):
+            box[k, :dim] = it["lo"]
+            box[k, 3:3 + dim] = it["hi"]
+            for pl in np.atleast_2d(it["planes"]):
+                if len(pl) == 0:
+                    continue
+                row = np.zeros(4)
+                row[:dim] = pl[:dim]
+                row[3] = pl[dim]
+                planes.append(row)
+            off[k + 1] = len(planes)
+        coeffs = np.array([it["coeff"] for it in items], dtype=np.float64)
+        pl = np.ascontiguousarray(np.array(planes).reshape(-1, 4) if planes else np.zeros((1, 4)))
+        keep = (keep, box, off, coeffs, pl)
+        d.nitems = len(items)
+        d.it_box = box.ctypes.data
+        d.it_coeff = coeffs.ctypes.data
+        d.it_planes = pl.ctypes.data
+        d.it_off = off.ctypes.data
     return d, keep
 
 
-def cells(pos, w, dim=None, ball=False, brute=False, nthreads=None, domain=None, img=None, cap=None):
+def cells(pos, w, dim=None, ball=False, brute=False, nthreads=None, domain=None, img=None, cap=None, items=None):
     """Per-cell masses, ball arc, perimeter and raw rows.  Returns dict."""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     dim = dim or pos.shape[1]
@@ -174,9 +236,9 @@ def cells(pos, w, dim=None, ball=False, brute=False, nthreads=None, domain=None,
     w = np.ascontiguousarray(w, dtype=np.float64)
     cap = cap or (64 if dim == 2 else 160)
     if domain is None:
-        dom, keep = make_domain(dim, img=img)
+        dom, keep = make_domain(dim, img=img, items=items)
     else:
-        dom, keep = make_domain(dim, *domain, img=img)
+        dom, keep = make_domain(dim, *domain, img=img, items=items)
     nthreads = nthreads or os.cpu_count() or 1
     masses = np.zeros(N)
     arc = np.zeros(N)
@@ -240,15 +302,15 @@ def pcg(off, cols, vals, b, rtol=1e-12, max_iter=100000):
 
 
 def ot_solve(pos, w0, targets, ball=False, tol=1e-6, stop_rule=0, max_iter=150, nthreads=None,
-             domain=None, img=None):
+             domain=None, img=None, items=None):
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     dim = pos.shape[1]
     w = np.array(w0, dtype=np.float64, copy=True)
     t = np.ascontiguousarray(targets, dtype=np.float64)
     if domain is None:
-        dom, keep = make_domain(dim, img=img)
+        dom, keep = make_domain(dim, img=img, items=items)
     else:
-        dom, keep = make_domain(dim, *domain, img=img)
+        dom, keep = make_domain(dim, *domain, img=img, items=items)
     stats = SolveStats()
     st = lib().or_ot_solve(dim, pos.shape[0], _ptr(pos), _ptr(w), _ptr(t), ctypes.byref(dom),
                            1 if ball else 0, nthreads or os.cpu_count() or 1, tol, stop_rule, max_iter,

@@ -667,6 +667,10 @@ static double dist_between(const double *pos, int dim, int64_t i, int64_t j) {
 
 /* ---- 2D ---- */
 
+/* an image, or several assembly items (for_each_intersection,
+ * ConvexPolyhedronAssembly.tcc:132-142): each cell is integrated piece by piece */
+static int pieces_mode(const or_domain *dom) { return dom->img != NULL || dom->nitems >= 2; }
+
 /* the per-piece part of the get_der_integrals_wrt_weights callback
  * (get_der_integrals_wrt_weights.h:52-75) for one 2D piece of density coeff */
 static void piece2_accumulate(const cells_job *J, int64_t i, const or_cp2 *cp, double coeff, const double *poly,
@@ -692,7 +696,7 @@ static void piece2_accumulate(const cells_job *J, int64_t i, const or_cp2 *cp, d
     }
     double wi = J->w[i];
     for (int k = 0; k < m; ++k) {
-        if (!J->dom->img || ids[k] != -1) *perim += meas[k];   /* image density: pixel sides are not boundary */
+        if (!pieces_mode(J->dom) || ids[k] != -1) *perim += meas[k];   /* pieces: their sides are not boundary */
         if (ids[k] == -1) continue;
         if (ids[k] == i) {
             *arcm += meas[k];
@@ -729,10 +733,75 @@ static int evict2(const or_cp2 *cp, double c0x, double c0y, double w0, double bx
     return 1;
 }
 
+/* ConvexPolyhedronAssembly (ConvexPolyhedronAssembly.tcc:7-142): the polygon of item
+ * k is its box cut by its planes (add_convex_polyhedron :7-33 cuts a box the same way) */
+static void item_poly2(const or_domain *dom, int64_t k, or_cp2 *cp) {
+    const double *b = dom->it_box + 6 * k;
+    or_cp2_box(cp, b[0], b[1], b[3], b[4], -1);
+    for (int64_t q = dom->it_off[k]; q < dom->it_off[k + 1] && cp->n; ++q) {
+        const double *pl = dom->it_planes + 4 * q;
+        double n2 = pl[0] * pl[0] + pl[1] * pl[1];
+        or_cp2_plane_cut(cp, pl[3] * pl[0] / n2, pl[3] * pl[1] / n2, pl[0], pl[1], -1, 0);
+    }
+}
+
+static void item_poly3(const or_domain *dom, int64_t k, or_cp3 *cp) {
+    const double *b = dom->it_box + 6 * k;
+    or_cp3_box(cp, b, b + 3, -1);
+    for (int64_t q = dom->it_off[k]; q < dom->it_off[k + 1] && cp->nf; ++q) {
+        const double *pl = dom->it_planes + 4 * q;
+        double n2 = pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2], nn = sqrt(n2);
+        double o[3] = {pl[3] * pl[0] / n2, pl[3] * pl[1] / n2, pl[3] * pl[2] / n2};
+        double nu[3] = {pl[0] / nn, pl[1] / nn, pl[2] / nn};
+        or_cp3_plane_cut(cp, o, nu, -1);
+    }
+}
+
+/* englobing_convex_polyhedron (:60-92): one item is the item itself, several give a box
+ * around them all, widened by its extent on each side */
+static void englobing_box(const or_domain *dom, int dim, double *lo, double *hi) {
+    for (int d = 0; d < dim; ++d) { lo[d] = INFINITY; hi[d] = -INFINITY; }
+    for (int64_t k = 0; k < dom->nitems; ++k)
+        for (int d = 0; d < dim; ++d) {
+            lo[d] = fmin(lo[d], dom->it_box[6 * k + d]);
+            hi[d] = fmax(hi[d], dom->it_box[6 * k + 3 + d]);
+        }
+    for (int d = 0; d < dim; ++d) {
+        double e = hi[d] - lo[d];
+        lo[d] -= e;
+        hi[d] += e;
+    }
+}
+
+
+static void start_cell2(const or_domain *dom, or_cp2 *cp) {
+    if (dom->nitems == 1) {
+        item_poly2(dom, 0, cp);
+    } else if (dom->nitems >= 2) {
+        double lo[3], hi[3];
+        englobing_box(dom, 2, lo, hi);
+        or_cp2_box(cp, lo[0], lo[1], hi[0], hi[1], -1);
+    } else {
+        or_cp2_box(cp, dom->p0[0], dom->p0[1], dom->p1[0], dom->p1[1], -1);
+    }
+}
+
+static void start_cell3(const or_domain *dom, or_cp3 *cp) {
+    if (dom->nitems == 1) {
+        item_poly3(dom, 0, cp);
+    } else if (dom->nitems >= 2) {
+        double lo[3], hi[3];
+        englobing_box(dom, 3, lo, hi);
+        or_cp3_box(cp, lo, hi, -1);
+    } else {
+        or_cp3_box(cp, dom->p0, dom->p1, -1);
+    }
+}
+
 static int build_cell2(const cells_job *J, int64_t i, or_cp2 *cp) {
     const or_domain *dom = J->dom;
     const double *pos = J->pos, *w = J->w;
-    or_cp2_box(cp, dom->p0[0], dom->p0[1], dom->p1[0], dom->p1[1], -1);
+    start_cell2(dom, cp);
     double c0x = pos[2 * i], c0y = pos[2 * i + 1], w0 = w[i];
     if (J->flags & 2) {
         for (int64_t j = 0; j < J->N; ++j)
@@ -789,8 +858,20 @@ static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
     double mass = 0, arcm = 0, perim = 0, mom[2] = {0, 0};
     int st = build_cell2(J, i, cp);
     const or_domain *dom = J->dom;
-    if (!dom->img) {
-        piece2_accumulate(J, i, cp, dom->coeff, NULL, &mass, &arcm, &perim, &row, mom);
+    if (dom->nitems >= 2) {
+        if (!or_cp2_empty(cp))
+            for (int64_t k = 0; k < dom->nitems; ++k) {
+                item_poly2(dom, k, piece);
+                /* intersect_with, ConvexPolyhedron2.tcc:2475-2497 */
+                for (int q = 0; q < cp->n && piece->n; ++q)
+                    or_cp2_plane_cut(piece, cp->x[q], cp->y[q], cp->nx[q], cp->ny[q], cp->id[q], 1);
+                if (cp->sr > 0) or_cp2_ball_cut(piece, cp->scx, cp->scy, cp->sr, cp->sid);
+                else { piece->scx = cp->scx; piece->scy = cp->scy; piece->sr = cp->sr; piece->sid = cp->sid; }
+                piece2_accumulate(J, i, piece, dom->it_coeff[k], NULL, &mass, &arcm, &perim, &row, mom);
+            }
+    } else if (!dom->img) {
+        piece2_accumulate(J, i, cp, dom->nitems == 1 ? dom->it_coeff[0] : dom->coeff, NULL, &mass, &arcm, &perim, &row,
+                          mom);
     } else {
         /* ScaledImage::for_each_intersection, ScaledImage.tcc:98-149 */
         double mn[2], mx[2];
@@ -881,7 +962,7 @@ static int evict3(const or_cp3 *cp, const double *c0, double w0, const double *b
 static int build_cell3(const cells_job *J, int64_t i, or_cp3 *cp) {
     const or_domain *dom = J->dom;
     const double *pos = J->pos, *w = J->w;
-    or_cp3_box(cp, dom->p0, dom->p1, -1);
+    start_cell3(dom, cp);
     const double *c0 = pos + 3 * i;
     double w0 = w[i];
     if (J->flags & 2) {
@@ -949,7 +1030,7 @@ static void piece3_accumulate(const cells_job *J, int64_t i, const or_cp3 *cp, d
         if (cp->nv[f] < 3) continue;
         double a = coeff * or_cp3_face_area(cp, f);
         int64_t j = cp->id[f];
-        if (!J->dom->img || j >= 0) *perim += a;   /* image density: faces shared with neighbours only */
+        if (!pieces_mode(J->dom) || j >= 0) *perim += a;   /* pieces: faces shared with neighbours only */
         if (j == -1) continue;
         double dd = dist_between(J->pos, 3, i, j);
         if (dd != 0) {
@@ -966,8 +1047,18 @@ static void eval_cell3(cells_job *J, int64_t i, or_cp3 *cp, or_cp3 *piece) {
     double mass = 0, perim = 0, mom[3] = {0, 0, 0};
     int st = build_cell3(J, i, cp);
     const or_domain *dom = J->dom;
-    if (!dom->img) {
-        piece3_accumulate(J, i, cp, dom->coeff, &mass, &perim, &row, mom);
+    if (dom->nitems >= 2) {
+        if (cp->nf)
+            for (int64_t k = 0; k < dom->nitems; ++k) {
+                item_poly3(dom, k, piece);
+                for (int f = 0; f < cp->nf && piece->nf; ++f) {
+                    if (cp->nv[f] < 3) continue;
+                    or_cp3_plane_cut(piece, cp->v[f][0], cp->n[f], cp->id[f]);
+                }
+                if (piece->nf) piece3_accumulate(J, i, piece, dom->it_coeff[k], &mass, &perim, &row, mom);
+            }
+    } else if (!dom->img) {
+        piece3_accumulate(J, i, cp, dom->nitems == 1 ? dom->it_coeff[0] : dom->coeff, &mass, &perim, &row, mom);
     } else if (cp->nf) {
         double mn[3], mx[3];
         for (int d = 0; d < 3; ++d) { mn[d] = INFINITY; mx[d] = -INFINITY; }

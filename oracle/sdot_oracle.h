@@ -115,6 +115,13 @@ typedef struct or_domain {
     int64_t img_nc;          /* coefficients per pixel: 1 (constant) or, in 2D, 6:
                               * c0 + c1 x + c2 y + c3 x^2 + c4 x y + c5 y^2 in domain
                               * coordinates, coefficient k of pixel p at img[k * npix + p] */
+    /* ConvexPolyhedronAssembly items (nitems = 0: the box p0, p1 with coeff above).
+     * Item k: the box it_box[6k..6k+2] .. it_box[6k+3..6k+5] cut by the planes
+     * n . x <= h stored as it_planes[4q..4q+3] = (nx, ny, nz, h) for q in
+     * [it_off[k], it_off[k+1]), density it_coeff[k]. */
+    int64_t nitems;
+    const double *it_box, *it_coeff, *it_planes;
+    const int64_t *it_off;
 } or_domain;
 
 /* Per-cell evaluation (get_der_integrals_wrt_weights.h:16-122 restated).

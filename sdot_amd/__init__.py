@@ -108,6 +108,8 @@ SIGNATURES = {
     "sdot_pd_set_ball_clip": (_int, [_p, _int]),
     "sdot_pd_compute": (_int, [_p]),
     "sdot_pd_integrals": (_int, [_p, _p]),
+    "sdot_pd_add_convex_polyhedron_shape": (_int, [_p, _p, _p, _i64, _d, _i64]),
+    "sdot_pd_add_simplex_shape": (_int, [_p, _p, _d, _i64]),
     "sdot_pd_centroids": (_int, [_p, _p, _p]),
     "sdot_pd_boundary_measures": (_int, [_p, _p, _p]),
     "sdot_pd_der_integrals_wrt_weights": (_int, [_p, _p, _p, _p, _p, _i64, ctypes.POINTER(_i64)]),
@@ -232,6 +234,19 @@ class PowerDiagram:
         a = np.ascontiguousarray(p0, dtype=np.float64)
         b = np.ascontiguousarray(p1, dtype=np.float64)
         self._check(lib().sdot_pd_add_box_shape(self._h, _ptr(a), _ptr(b), float(coeff), int(cut_id)), "add_box_shape")
+
+    def add_convex_polyhedron_shape(self, positions, normals, coeff=1.0, cut_id=-1):
+        """ConvexPolyhedronAssembly::add_convex_polyhedron: (x - p_i) . n_i <= 0 for each i."""
+        p = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, self.dim)
+        n = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, self.dim)
+        self._check(lib().sdot_pd_add_convex_polyhedron_shape(self._h, _ptr(p), _ptr(n), len(p), float(coeff),
+                                                              int(cut_id)), "add_convex_polyhedron_shape")
+
+    def add_simplex_shape(self, points, coeff=1.0, cut_id=-1):
+        """ConvexPolyhedronAssembly::add_simplex: dim+1 points."""
+        q = np.ascontiguousarray(points, dtype=np.float64).reshape(self.dim + 1, self.dim)
+        self._check(lib().sdot_pd_add_simplex_shape(self._h, _ptr(q), float(coeff), int(cut_id)),
+                    "add_simplex_shape")
 
     def set_image_density(self, min_pt, max_pt, data):
         """data indexed [z][y][x] / [y][x] (x fastest), constant per pixel; in 2D an

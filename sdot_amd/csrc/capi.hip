@@ -6,6 +6,8 @@
 #include <cstdio>
 #include <new>
 
+#include <algorithm>
+#include <cfloat>
 #include "internal.h"
 #include "pcg.h"
 
@@ -50,6 +52,76 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// The box the cells of an item are clipped to. A face of the item's box that its
+// planes make redundant (their polytope reaches it at most) moves out by an eighth
+// of the box's extent: the item is the same set, and no cell starts from a corner
+// where a box face meets several planes (the corners of a simplex lie on its
+// bounding box; a regular octahedron's can lie on the widened box), a vertex of
+// four or more planes whose exact in/out tests round inconsistently in a plane cut.
+// The vertices of planes + box widened by a quarter are enumerated (all pairs in
+// 2D, triples in 3D); items with more than 160 planes keep their box.
+void item_clip_box(Shape &sh, int dim) {
+    for (int d = 0; d < 3; ++d) {
+        sh.q0[d] = sh.p0[d];
+        sh.q1[d] = sh.p1[d];
+    }
+    if (sh.planes.empty() || sh.planes.size() > 160) return;
+    double ext[3] = {0, 0, 0}, L = 0;
+    for (int d = 0; d < dim; ++d) {
+        ext[d] = sh.p1[d] - sh.p0[d];
+        L = std::max(L, ext[d]);
+    }
+    std::vector<std::array<double, 4>> c = sh.planes;
+    for (int d = 0; d < dim; ++d) {
+        std::array<double, 4> lo{0, 0, 0, -(sh.p0[d] - 0.25 * ext[d])}, hi{0, 0, 0, sh.p1[d] + 0.25 * ext[d]};
+        lo[d] = -1;
+        hi[d] = 1;
+        c.push_back(lo);
+        c.push_back(hi);
+    }
+    const int m = (int)c.size();
+    std::vector<double> nrm(m);
+    for (int i = 0; i < m; ++i) nrm[i] = std::sqrt(c[i][0] * c[i][0] + c[i][1] * c[i][1] + c[i][2] * c[i][2]);
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    bool any = false;
+    auto take = [&](const double *x) {
+        for (int i = 0; i < m; ++i)
+            if (c[i][0] * x[0] + c[i][1] * x[1] + c[i][2] * x[2] - c[i][3] > 1e-11 * nrm[i] * L) return;
+        any = true;
+        for (int d = 0; d < dim; ++d) {
+            mn[d] = std::min(mn[d], x[d]);
+            mx[d] = std::max(mx[d], x[d]);
+        }
+    };
+    for (int i = 0; i < m; ++i)
+        for (int j = i + 1; j < m; ++j) {
+            if (dim == 2) {
+                const double det = c[i][0] * c[j][1] - c[i][1] * c[j][0];
+                if (!(std::fabs(det) > 1e-12 * nrm[i] * nrm[j])) continue;
+                const double x[3] = {(c[i][3] * c[j][1] - c[j][3] * c[i][1]) / det,
+                                     (c[i][0] * c[j][3] - c[j][0] * c[i][3]) / det, 0};
+                take(x);
+                continue;
+            }
+            for (int k = j + 1; k < m; ++k) {
+                const auto &a = c[i], &b = c[j], &e = c[k];
+                const double bx = b[1] * e[2] - b[2] * e[1], by = b[2] * e[0] - b[0] * e[2], bz = b[0] * e[1] - b[1] * e[0];
+                const double ex = e[1] * a[2] - e[2] * a[1], ey = e[2] * a[0] - e[0] * a[2], ez = e[0] * a[1] - e[1] * a[0];
+                const double ax = a[1] * b[2] - a[2] * b[1], ay = a[2] * b[0] - a[0] * b[2], az = a[0] * b[1] - a[1] * b[0];
+                const double det = a[0] * bx + a[1] * by + a[2] * bz;
+                if (!(std::fabs(det) > 1e-12 * nrm[i] * nrm[j] * nrm[k])) continue;
+                const double x[3] = {(a[3] * bx + b[3] * ex + e[3] * ax) / det, (a[3] * by + b[3] * ey + e[3] * ay) / det,
+                                     (a[3] * bz + b[3] * ez + e[3] * az) / det};
+                take(x);
+            }
+        }
+    if (!any) return;   // an empty item: its cells come out empty with any box
+    for (int d = 0; d < dim; ++d) {
+        if (mn[d] >= sh.p0[d] - 1e-11 * L) sh.q0[d] = sh.p0[d] - 0.125 * ext[d];
+        if (mx[d] <= sh.p1[d] + 1e-11 * L) sh.q1[d] = sh.p1[d] + 0.125 * ext[d];
+    }
+}
+
 // the domain as seen by the kernels: one box (with density coefficient) or an image
 void domain_args(sdot_pd *pd, CellArgs &a) {
     a.img = nullptr;
@@ -66,16 +138,57 @@ void domain_args(sdot_pd *pd, CellArgs &a) {
         a.img_nc = pd->img_nc;
         return;
     }
-    if (pd->shapes.size() != 1)
-        throw StatusError{SDOT_NOT_SUPPORTED, pd->shapes.empty() ? "no domain shape (add_box_shape)" :
-                                                                   "several domain shapes are not supported on the device path"};
+    if (pd->shapes.empty()) throw StatusError{SDOT_NOT_SUPPORTED, "no domain shape (add_box_shape)"};
+    a.nitems = 0;
     const Shape &sh = pd->shapes[0];
     for (int d = 0; d < 3; ++d) {
-        a.dom_lo[d] = d < pd->dim ? sh.p0[d] : 0;
-        a.dom_hi[d] = d < pd->dim ? sh.p1[d] : 1;
+        a.dom_lo[d] = d < pd->dim ? sh.q0[d] : 0;
+        a.dom_hi[d] = d < pd->dim ? sh.q1[d] : 1;
         a.img_n[d] = 1;
     }
     a.coeff = sh.coeff;
+    if (pd->shapes.size() == 1 && sh.planes.empty()) return;   // one box: the starting cell
+    // items (ConvexPolyhedronAssembly.tcc:7-142): tables on the device
+    const int K = (int)pd->shapes.size();
+    if (K > 1 && pd->ball) throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping with several domain shapes"};
+    std::vector<double> box(6 * K), coeff(K), planes;
+    std::vector<int> off(K + 1, 0);
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int k = 0; k < K; ++k) {
+        const Shape &it = pd->shapes[k];
+        for (int d = 0; d < 3; ++d) {
+            box[6 * k + d] = it.q0[d];
+            box[6 * k + 3 + d] = it.q1[d];
+            lo[d] = std::min(lo[d], it.p0[d]);
+            hi[d] = std::max(hi[d], it.p1[d]);
+        }
+        coeff[k] = it.coeff;
+        for (const auto &pl : it.planes) planes.insert(planes.end(), pl.begin(), pl.end());
+        off[k + 1] = (int)(planes.size() / 4);
+    }
+    if (planes.empty()) planes.assign(4, 0.0);
+    pd->it_box.ensure(box.size());
+    pd->it_coeff.ensure(coeff.size());
+    pd->it_planes.ensure(planes.size());
+    pd->it_off.ensure(off.size());
+    HIPCK(hipMemcpyAsync(pd->it_box.p, box.data(), box.size() * sizeof(double), hipMemcpyHostToDevice, pd->stream));
+    HIPCK(hipMemcpyAsync(pd->it_coeff.p, coeff.data(), coeff.size() * sizeof(double), hipMemcpyHostToDevice, pd->stream));
+    HIPCK(hipMemcpyAsync(pd->it_planes.p, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice, pd->stream));
+    HIPCK(hipMemcpyAsync(pd->it_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, pd->stream));
+    HIPCK(hipStreamSynchronize(pd->stream));
+    a.nitems = K;
+    a.it_box = pd->it_box.p;
+    a.it_coeff = pd->it_coeff.p;
+    a.it_planes = pd->it_planes.p;
+    a.it_off = pd->it_off.p;
+    if (K > 1) {   // englobing_convex_polyhedron (:60-92): a box around the items, widened by its extent
+        for (int d = 0; d < 3; ++d) {
+            const double e = d < pd->dim ? hi[d] - lo[d] : 0.0;
+            a.dom_lo[d] = d < pd->dim ? lo[d] - e : 0;
+            a.dom_hi[d] = d < pd->dim ? hi[d] + e : 1;
+        }
+        a.coeff = 1;
+    }
 }
 
 }  // namespace
@@ -355,6 +468,110 @@ sdot_status sdot_pd_add_box_shape(sdot_pd *pd, const double *p0, const double *p
     }
     sh.coeff = coeff;
     sh.cut_id = cut_id;
+    item_clip_box(sh, pd->dim);
+    pd->shapes.push_back(sh);
+    pd->pass_valid = false;
+    return SDOT_OK;
+}
+
+sdot_status sdot_pd_add_convex_polyhedron_shape(sdot_pd *pd, const double *positions, const double *normals, int64_t n,
+                                                double coeff, int64_t cut_id) {
+    if (!pd || !positions || !normals || n < 1) return SDOT_INVALID_ARG;
+    if (cut_id != -1) {
+        pd->err = "only the domain cut id -1 is supported";
+        return SDOT_NOT_SUPPORTED;
+    }
+    const int dim = pd->dim;
+    Shape sh{};
+    // ConvexPolyhedronAssembly.tcc:7-33: the box of the points widened by its extent on
+    // each side, cut by the planes (x - p_i) . n_i <= 0
+    for (int d = 0; d < dim; ++d) {
+        double lo = DBL_MAX, hi = -DBL_MAX;
+        for (int64_t i = 0; i < n; ++i) {
+            if (!std::isfinite(positions[dim * i + d]) || !std::isfinite(normals[dim * i + d])) return SDOT_INVALID_ARG;
+            lo = std::min(lo, positions[dim * i + d]);
+            hi = std::max(hi, positions[dim * i + d]);
+        }
+        const double e = hi - lo;
+        sh.p0[d] = lo - e;
+        sh.p1[d] = hi + e;
+        if (!(sh.p1[d] > sh.p0[d])) { sh.p0[d] = lo - 1; sh.p1[d] = hi + 1; }
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        std::array<double, 4> pl{0, 0, 0, 0};
+        double nn = 0;
+        for (int d = 0; d < dim; ++d) {
+            pl[d] = normals[dim * i + d];
+            nn += pl[d] * pl[d];
+            pl[3] += normals[dim * i + d] * positions[dim * i + d];
+        }
+        if (!(nn > 0)) return SDOT_INVALID_ARG;
+        sh.planes.push_back(pl);
+    }
+    sh.coeff = coeff;
+    sh.cut_id = cut_id;
+    item_clip_box(sh, pd->dim);
+    pd->shapes.push_back(sh);
+    pd->pass_valid = false;
+    return SDOT_OK;
+}
+
+sdot_status sdot_pd_add_simplex_shape(sdot_pd *pd, const double *points, double coeff, int64_t cut_id) {
+    if (!pd || !points) return SDOT_INVALID_ARG;
+    if (cut_id != -1) {
+        pd->err = "only the domain cut id -1 is supported";
+        return SDOT_NOT_SUPPORTED;
+    }
+    const int dim = pd->dim;
+    Shape sh{};
+    for (int d = 0; d < dim; ++d) {
+        sh.p0[d] = DBL_MAX;
+        sh.p1[d] = -DBL_MAX;
+        for (int k = 0; k <= dim; ++k) {
+            if (!std::isfinite(points[dim * k + d])) return SDOT_INVALID_ARG;
+            sh.p0[d] = std::min(sh.p0[d], points[dim * k + d]);
+            sh.p1[d] = std::max(sh.p1[d], points[dim * k + d]);
+        }
+        if (!(sh.p1[d] > sh.p0[d])) return SDOT_INVALID_ARG;
+    }
+    // one plane per facet (the points but vertex k), oriented away from vertex k
+    for (int k = 0; k <= dim; ++k) {
+        double f[3][3] = {}, q[3] = {};
+        int m = 0;
+        for (int j = 0; j <= dim; ++j) {
+            if (j == k) {
+                for (int d = 0; d < dim; ++d) q[d] = points[dim * j + d];
+            } else {
+                for (int d = 0; d < dim; ++d) f[m][d] = points[dim * j + d];
+                ++m;
+            }
+        }
+        double nv[3] = {0, 0, 0};
+        if (dim == 2) {
+            nv[0] = f[1][1] - f[0][1];
+            nv[1] = -(f[1][0] - f[0][0]);
+        } else {
+            const double u[3] = {f[1][0] - f[0][0], f[1][1] - f[0][1], f[1][2] - f[0][2]};
+            const double v[3] = {f[2][0] - f[0][0], f[2][1] - f[0][1], f[2][2] - f[0][2]};
+            nv[0] = u[1] * v[2] - u[2] * v[1];
+            nv[1] = u[2] * v[0] - u[0] * v[2];
+            nv[2] = u[0] * v[1] - u[1] * v[0];
+        }
+        double s = 0, h = 0;
+        for (int d = 0; d < dim; ++d) {
+            s += (q[d] - f[0][d]) * nv[d];
+            h += nv[d] * f[0][d];
+        }
+        if (s == 0) return SDOT_INVALID_ARG;   // degenerate simplex
+        if (s > 0) {
+            for (int d = 0; d < dim; ++d) nv[d] = -nv[d];
+            h = -h;
+        }
+        sh.planes.push_back({nv[0], nv[1], nv[2], h});
+    }
+    sh.coeff = coeff;
+    sh.cut_id = cut_id;
+    item_clip_box(sh, pd->dim);
     pd->shapes.push_back(sh);
     pd->pass_valid = false;
     return SDOT_OK;

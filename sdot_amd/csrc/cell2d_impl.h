@@ -177,6 +177,13 @@ SDOT_HD inline void line_of(const CellArgs &a, const ProbView &pv, int id, doubl
         lc = 0.5 * (la * la + lb * lb + w0 - a.sw[id]);
         return;
     }
+    if (id <= -100) {   // plane -100 - id of the domain item (n . x <= h)
+        const double *pl = a.it_planes + 4 * (-100 - id);
+        la = pl[0];
+        lb = pl[1];
+        lc = pl[3] - pl[0] * c0x - pl[1] * c0y;
+        return;
+    }
     const double sq = ball_square(w0);
     switch (-1 - id) {
         case 0: la = 0; lb = 1; lc = pv.lo[1] - c0y; break;
@@ -202,10 +209,10 @@ struct IdPoly2 {
 };
 
 // one side of a convex polygon clip (Sutherland-Hodgman keeping edge ids): keeps
-// s * (bound - coord) >= 0, the new edge along the clipping line gets `id`;
-// returns false if the result would not fit
+// la x + lb y <= lc, the new edge along the clipping line gets `id`; returns false if
+// the result would not fit
 template <int PC>
-SDOT_HD inline bool clip_axis(const IdPoly2<PC> &in, IdPoly2<PC> &out, int axis, double bound, double s, int id) {
+SDOT_HD inline bool clip_line(const IdPoly2<PC> &in, IdPoly2<PC> &out, double la, double lb, double lc, int id) {
     int m = 0;
     bool fits = true;
     auto put = [&](double x, double y, int e) {
@@ -213,8 +220,8 @@ SDOT_HD inline bool clip_axis(const IdPoly2<PC> &in, IdPoly2<PC> &out, int axis,
     };
     for (int k = 0; k < in.n; ++k) {
         const int k1 = k + 1 == in.n ? 0 : k + 1;
-        const double dk = s * (bound - (axis == 0 ? in.x[k] : in.y[k]));
-        const double d1 = s * (bound - (axis == 0 ? in.x[k1] : in.y[k1]));
+        const double dk = lc - (la * in.x[k] + lb * in.y[k]);
+        const double d1 = lc - (la * in.x[k1] + lb * in.y[k1]);
         if (dk >= 0) {
             put(in.x[k], in.y[k], in.e[k]);
             if (d1 < 0) {
@@ -353,9 +360,15 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         n = nn;
         return 1;
     };
-    double R2 = radius2();
     bool overflow = false;
     bool bad = false;
+    if (!BATCH && a.nitems == 1)   // one domain item: the starting cell is its box cut by its planes
+        for (int q = a.it_off[0]; q < a.it_off[1] && n > 0 && !overflow; ++q) {
+            const double *pl = a.it_planes + 4 * q;
+            const double hl = pl[3] - pl[0] * c0x - pl[1] * c0y;
+            if (cut(T(pl[0]), T(pl[1]), T(hl), -100 - q, pl[0], pl[1], hl) == 2) overflow = true;
+        }
+    double R2 = radius2();
 
     const int hx = grid_coord(g, 0, c0x), hy = grid_coord(g, 1, c0y);
     const int maxr = g.ncell[0] > g.ncell[1] ? g.ncell[0] : g.ncell[1];
@@ -436,9 +449,11 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     };
     bool full_disc = false;
     if (IMG) {
-        // pixel pieces (ScaledImage::for_each_intersection, ScaledImage.tcc:98-149): the cell
-        // clipped by every pixel that overlaps its bounding box, each piece integrated with
-        // the pixel's density; a neighbour's entry sums density x length over its edge's pieces
+        // Pieces, each integrated with its own density; a neighbour's entry sums density
+        // x length over its edge's pieces.  Pixels (ScaledImage::for_each_intersection,
+        // ScaledImage.tcc:98-149): the cell clipped by every pixel under its bounding box.
+        // Several domain items (ConvexPolyhedronAssembly::for_each_intersection,
+        // ConvexPolyhedronAssembly.tcc:132-142): the cell clipped by each item.
         constexpr int PC = CAP + 8;
         IdPoly2<PC> cell, pa, pb;
         double edge_rho_len[CAP];
@@ -451,85 +466,111 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             mn[0] = fmin(mn[0], cell.x[k]); mx[0] = fmax(mx[0], cell.x[k]);
             mn[1] = fmin(mn[1], cell.y[k]); mx[1] = fmax(mx[1], cell.y[k]);
         }
-        const double c0[2] = {c0x, c0y};
-        int i0[2] = {0, 0}, i1[2] = {-1, -1};
-        double ps[2] = {0, 0};
-        for (int d = 0; d < 2 && n > 0; ++d) {
-            ps[d] = (a.dom_hi[d] - a.dom_lo[d]) / a.img_n[d];
-            const int u0 = (int)floor((mn[d] + c0[d] - a.dom_lo[d]) / ps[d]);
-            const int u1 = (int)floor((mx[d] + c0[d] - a.dom_lo[d]) / ps[d]);
-            i0[d] = u0 < 0 ? 0 : (u0 >= a.img_n[d] ? a.img_n[d] - 1 : u0);
-            i1[d] = u1 < 0 ? 0 : (u1 >= a.img_n[d] ? a.img_n[d] - 1 : u1);
-        }
         double mass_sum = 0, mom_x = 0, mom_y = 0;
         bool fits = true;
-        const bool poly = a.img_nc == 6;
-        const int64_t npix = (int64_t)a.img_n[0] * a.img_n[1];
-        for (int py = i0[1]; py <= i1[1]; ++py)
-            for (int px = i0[0]; px <= i1[0]; ++px) {
-                const int64_t pix = (int64_t)py * a.img_n[0] + px;
-                const double rho = a.img[pix];
-                // quadratic pixel (Polynomial<6>, Polynomial.h:37-50): the coefficients are
-                // given in domain coordinates; q re-expands them about the dirac
-                double q[6] = {rho, 0, 0, 0, 0, 0};
-                if (poly) {
-                    double c[6];
-                    for (int k = 0; k < 6; ++k) c[k] = a.img[k * npix + pix];
-                    q[0] = c[0] + c0x * (c[1] + c[3] * c0x + c[4] * c0y) + c0y * (c[2] + c[5] * c0y);
-                    q[1] = c[1] + 2 * c[3] * c0x + c[4] * c0y;
-                    q[2] = c[2] + c[4] * c0x + 2 * c[5] * c0y;
-                    q[3] = c[3];
-                    q[4] = c[4];
-                    q[5] = c[5];
+        const bool poly = a.img && a.img_nc == 6;
+        const IdPoly2<PC> *src = &cell;
+        IdPoly2<PC> *dst = &pa;
+        auto clip = [&](double la, double lb, double lc) {
+            fits &= clip_line(*src, *dst, la, lb, lc, -9);
+            src = dst;
+            dst = dst == &pa ? &pb : &pa;
+        };
+        // integrates the piece in *src with density rho (or the quadratic q)
+        auto add_piece = [&](double rho, const double *q) {
+            auto q_at = [&](double x, double y) {
+                return q[0] + x * (q[1] + q[3] * x + q[4] * y) + y * (q[2] + q[5] * y);
+            };
+            const IdPoly2<PC> &pc = *src;
+            // area moments of the piece (Green's theorem over its edges)
+            double m0 = 0, sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0;
+            for (int k = 0; k < pc.n; ++k) {
+                const int k1 = k + 1 == pc.n ? 0 : k + 1;
+                const double x0 = pc.x[k], y0 = pc.y[k], x1 = pc.x[k1], y1 = pc.y[k1];
+                const double cr = x0 * y1 - x1 * y0;
+                m0 += cr;
+                if (q || a.mom) {
+                    sx += (x0 + x1) * cr;
+                    sy += (y0 + y1) * cr;
                 }
-                auto q_at = [&](double x, double y) {
-                    return q[0] + x * (q[1] + q[3] * x + q[4] * y) + y * (q[2] + q[5] * y);
-                };
-                // sides of the pixel that lie inside the bounding box cut the cell; the
-                // others cannot, so the piece of an edge pixel keeps the cell's own edges
-                const IdPoly2<PC> *src = &cell;
-                IdPoly2<PC> *dst = &pa;
-                auto clip = [&](int axis, double bound, double sgn) {
-                    fits &= clip_axis(*src, *dst, axis, bound, sgn, -9);
-                    src = dst;
-                    dst = dst == &pa ? &pb : &pa;
-                };
-                if (px > i0[0]) clip(0, a.dom_lo[0] + ps[0] * px - c0x, -1.0);
-                if (px < i1[0]) clip(0, a.dom_lo[0] + ps[0] * (px + 1) - c0x, 1.0);
-                if (py > i0[1]) clip(1, a.dom_lo[1] + ps[1] * py - c0y, -1.0);
-                if (py < i1[1]) clip(1, a.dom_lo[1] + ps[1] * (py + 1) - c0y, 1.0);
-                const IdPoly2<PC> &pc = *src;
-                // area moments of the piece (Green's theorem over its edges)
-                double m0 = 0, sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0;
-                for (int k = 0; k < pc.n; ++k) {
-                    const int k1 = k + 1 == pc.n ? 0 : k + 1;
-                    const double x0 = pc.x[k], y0 = pc.y[k], x1 = pc.x[k1], y1 = pc.y[k1];
-                    const double cr = x0 * y1 - x1 * y0;
-                    m0 += cr;
-                    if (poly || a.mom) {
-                        sx += (x0 + x1) * cr;
-                        sy += (y0 + y1) * cr;
-                        sxx += (x0 * x0 + x0 * x1 + x1 * x1) * cr;
-                        syy += (y0 * y0 + y0 * y1 + y1 * y1) * cr;
-                        sxy += (x0 * y1 + 2 * x0 * y0 + 2 * x1 * y1 + x1 * y0) * cr;
-                    }
-                    if (pc.e[k] < 0) continue;
-                    const double ex = x1 - x0, ey = y1 - y0;
-                    const double len = sqrt(ex * ex + ey * ey);
-                    // density integrated along the edge piece (Simpson, exact for a quadratic)
-                    const double dens = poly ? (q_at(x0, y0) + 4 * q_at(0.5 * (x0 + x1), 0.5 * (y0 + y1)) + q_at(x1, y1)) / 6
-                                             : rho;
-                    for (int j = 0; j < n; ++j)
-                        if (cell.e[j] == pc.e[k]) { edge_rho_len[j] += dens * len; break; }
+                if (q) {
+                    sxx += (x0 * x0 + x0 * x1 + x1 * x1) * cr;
+                    syy += (y0 * y0 + y0 * y1 + y1 * y1) * cr;
+                    sxy += (x0 * y1 + 2 * x0 * y0 + 2 * x1 * y1 + x1 * y0) * cr;
                 }
-                mass_sum += poly ? q[0] * m0 / 2 + q[1] * sx / 6 + q[2] * sy / 6 + q[3] * sxx / 12 + q[4] * sxy / 24 +
-                                       q[5] * syy / 12
-                                 : rho * 0.5 * m0;
-                if (!poly) {   // first moments of a constant piece (quadratic pixels: not offered)
-                    mom_x += rho * sx / 6;
-                    mom_y += rho * sy / 6;
-                }
+                if (pc.e[k] < 0) continue;
+                const double ex = x1 - x0, ey = y1 - y0;
+                const double len = sqrt(ex * ex + ey * ey);
+                // density integrated along the edge piece (Simpson, exact for a quadratic)
+                const double dens = q ? (q_at(x0, y0) + 4 * q_at(0.5 * (x0 + x1), 0.5 * (y0 + y1)) + q_at(x1, y1)) / 6 : rho;
+                for (int j = 0; j < n; ++j)
+                    if (cell.e[j] == pc.e[k]) { edge_rho_len[j] += dens * len; break; }
             }
+            mass_sum += q ? q[0] * m0 / 2 + q[1] * sx / 6 + q[2] * sy / 6 + q[3] * sxx / 12 + q[4] * sxy / 24 +
+                                q[5] * syy / 12
+                          : rho * 0.5 * m0;
+            if (!q) {   // first moments of a constant piece (quadratic pixels: not offered)
+                mom_x += rho * sx / 6;
+                mom_y += rho * sy / 6;
+            }
+        };
+        if (a.img) {
+            const double c0[2] = {c0x, c0y};
+            int i0[2] = {0, 0}, i1[2] = {-1, -1};
+            double ps[2] = {0, 0};
+            for (int d = 0; d < 2 && n > 0; ++d) {
+                ps[d] = (a.dom_hi[d] - a.dom_lo[d]) / a.img_n[d];
+                const int u0 = (int)floor((mn[d] + c0[d] - a.dom_lo[d]) / ps[d]);
+                const int u1 = (int)floor((mx[d] + c0[d] - a.dom_lo[d]) / ps[d]);
+                i0[d] = u0 < 0 ? 0 : (u0 >= a.img_n[d] ? a.img_n[d] - 1 : u0);
+                i1[d] = u1 < 0 ? 0 : (u1 >= a.img_n[d] ? a.img_n[d] - 1 : u1);
+            }
+            const int64_t npix = (int64_t)a.img_n[0] * a.img_n[1];
+            for (int py = i0[1]; py <= i1[1]; ++py)
+                for (int px = i0[0]; px <= i1[0]; ++px) {
+                    const int64_t pix = (int64_t)py * a.img_n[0] + px;
+                    const double rho = a.img[pix];
+                    // quadratic pixel (Polynomial<6>, Polynomial.h:37-50): the coefficients are
+                    // given in domain coordinates; q re-expands them about the dirac
+                    double q[6] = {rho, 0, 0, 0, 0, 0};
+                    if (poly) {
+                        double c[6];
+                        for (int k = 0; k < 6; ++k) c[k] = a.img[k * npix + pix];
+                        q[0] = c[0] + c0x * (c[1] + c[3] * c0x + c[4] * c0y) + c0y * (c[2] + c[5] * c0y);
+                        q[1] = c[1] + 2 * c[3] * c0x + c[4] * c0y;
+                        q[2] = c[2] + c[4] * c0x + 2 * c[5] * c0y;
+                        q[3] = c[3];
+                        q[4] = c[4];
+                        q[5] = c[5];
+                    }
+                    // sides of the pixel that lie inside the bounding box cut the cell; the
+                    // others cannot, so the piece of an edge pixel keeps the cell's own edges
+                    src = &cell;
+                    dst = &pa;
+                    if (px > i0[0]) clip(-1, 0, -(a.dom_lo[0] + ps[0] * px - c0x));
+                    if (px < i1[0]) clip(1, 0, a.dom_lo[0] + ps[0] * (px + 1) - c0x);
+                    if (py > i0[1]) clip(0, -1, -(a.dom_lo[1] + ps[1] * py - c0y));
+                    if (py < i1[1]) clip(0, 1, a.dom_lo[1] + ps[1] * (py + 1) - c0y);
+                    add_piece(rho, poly ? q : nullptr);
+                }
+        } else {
+            for (int k = 0; k < a.nitems && n > 0; ++k) {
+                const double *b = a.it_box + 6 * k;
+                const double lx = b[0] - c0x, ly = b[1] - c0y, hx = b[3] - c0x, hy = b[4] - c0y;
+                if (lx > mx[0] || hx < mn[0] || ly > mx[1] || hy < mn[1]) continue;   // item box misses the cell
+                src = &cell;
+                dst = &pa;
+                if (lx > mn[0]) clip(-1, 0, -lx);
+                if (hx < mx[0]) clip(1, 0, hx);
+                if (ly > mn[1]) clip(0, -1, -ly);
+                if (hy < mx[1]) clip(0, 1, hy);
+                for (int q = a.it_off[k]; q < a.it_off[k + 1]; ++q) {
+                    const double *pl = a.it_planes + 4 * q;
+                    clip(pl[0], pl[1], pl[3] - pl[0] * c0x - pl[1] * c0y);
+                }
+                add_piece(a.it_coeff[k], nullptr);
+            }
+        }
         if (!fits) flag_or(a.flags, FLAG_CELL_OVERFLOW);
         double der_line = 0, perim = 0;
         int nrow = 0;

@@ -85,6 +85,14 @@ struct Cell3 {
             h = 0.5 * (nx * nx + ny * ny + nz * nz + w0 - a.sw[id]);
             return;
         }
+        if (id <= -100) {   // plane -100 - id of the domain item (n . x <= h)
+            const double *pl = a.it_planes + 4 * (-100 - id);
+            nx = pl[0];
+            ny = pl[1];
+            nz = pl[2];
+            h = pl[3] - (pl[0] * c0x + pl[1] * c0y + pl[2] * c0z);
+            return;
+        }
         const int s = id >= -6 ? -1 - id : -7 - id;
         const int axis = s >> 1, up = s & 1;
         const double c0[3] = {c0x, c0y, c0z};
@@ -415,8 +423,18 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
     const GridInfo &g = a.g;
     const double W = dbl_unorder(*a.wglob);
     C.init_box();
-    double R2 = C.radius2();
     bool overflow = false, bad = false;
+    if (a.nitems == 1) {   // one domain item: the starting cell is its box cut by its planes
+        const double r0 = sqrt(C.radius2());
+        for (int q = a.it_off[0]; q < a.it_off[1] && C.nv > 0 && !overflow; ++q) {
+            const double *pl = a.it_planes + 4 * q;
+            const double hl = pl[3] - (pl[0] * c0x + pl[1] * c0y + pl[2] * c0z);
+            const double nn = sqrt(pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2]);
+            if (C.cut(T(pl[0]), T(pl[1]), T(pl[2]), T(hl), -100 - q, pl[0], pl[1], pl[2], hl, Eps3<T>::tie * nn * r0) >= 2)
+                overflow = true;
+        }
+    }
+    double R2 = C.radius2();
     const int h0 = grid_coord(g, 0, c0x), h1 = grid_coord(g, 1, c0y), h2 = grid_coord(g, 2, c0z);
     int maxr = g.ncell[0];
     if (g.ncell[1] > maxr) maxr = g.ncell[1];
@@ -484,12 +502,65 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
     for (int k = 0; k < PCAP; ++k) C.m.hacc[k * C.m.S] = 0;
     const int npc = C.np;
     if (C.nv > 0) {
-        if (!a.img) {
+        if (!a.img && a.nitems < 2) {
             const double vol = C.integrate(want);
             mass = a.coeff * vol;
             if (want)
                 for (int d = 0; d < 3; ++d) mom[d] = mass * off[d];
             for (int k = 0; k < npc; ++k) C.m.hacc[k * C.m.S] = a.coeff * C.m.acc[k * C.m.S];
+        } else if (!a.img) {
+            // several domain items (ConvexPolyhedronAssembly.tcc:132-142): the cell cut by
+            // each item's box sides and planes, integrated with its density
+            double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+            for (int k = 0; k < C.nv; ++k) {
+                double x, y, z;
+                C.vertex64(k, x, y, z);
+                mn[0] = fmin(mn[0], x); mx[0] = fmax(mx[0], x);
+                mn[1] = fmin(mn[1], y); mx[1] = fmax(mx[1], y);
+                mn[2] = fmin(mn[2], z); mx[2] = fmax(mx[2], z);
+            }
+            const int nv0 = C.nv, np0 = C.np;
+            C.save();
+            for (int it = 0; it < a.nitems && !overflow; ++it) {
+                const double *b = a.it_box + 6 * it;
+                bool empty = false;
+                for (int d = 0; d < 3; ++d) {
+                    C.vlo[d] = b[d] - c0[d];
+                    C.vhi[d] = b[3 + d] - c0[d];
+                    if (C.vlo[d] > mx[d] || C.vhi[d] < mn[d]) empty = true;   // the item's box misses the cell
+                }
+                for (int sd = 0; sd < 6 && !empty; ++sd) {
+                    const int axis = sd >> 1, up = sd & 1;
+                    const double bound = up ? sel3(C.vhi, axis) : sel3(C.vlo, axis);
+                    if (up ? !(bound < sel3(mx, axis)) : !(bound > sel3(mn, axis))) continue;   // cannot cut
+                    const double nxx = axis == 0 ? (up ? 1.0 : -1.0) : 0.0, nyy = axis == 1 ? (up ? 1.0 : -1.0) : 0.0,
+                                 nzz = axis == 2 ? (up ? 1.0 : -1.0) : 0.0, hq = up ? bound : -bound;
+                    const int rc = C.cut(T(nxx), T(nyy), T(nzz), T(hq), -7 - sd, nxx, nyy, nzz, hq,
+                                         Eps3<T>::tie * sqrt(R2), false);
+                    if (rc >= 2) { overflow = true; break; }
+                    if (C.nv == 0) empty = true;
+                }
+                for (int q = a.it_off[it]; q < a.it_off[it + 1] && !empty && !overflow; ++q) {
+                    const double *pl = a.it_planes + 4 * q;
+                    const double hl = pl[3] - (pl[0] * c0[0] + pl[1] * c0[1] + pl[2] * c0[2]);
+                    const double nn = sqrt(pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2]);
+                    const int rc = C.cut(T(pl[0]), T(pl[1]), T(pl[2]), T(hl), -100 - q, pl[0], pl[1], pl[2], hl,
+                                         Eps3<T>::tie * nn * sqrt(R2), false);
+                    if (rc >= 2) { overflow = true; break; }
+                    if (C.nv == 0) empty = true;
+                }
+                if (overflow) break;
+                if (!empty) {
+                    const double rho = a.it_coeff[it];
+                    const double vol = C.integrate(want);
+                    mass += rho * vol;
+                    if (want)
+                        for (int d = 0; d < 3; ++d) mom[d] += rho * vol * off[d];
+                    for (int k = 0; k < npc; ++k) C.m.hacc[k * C.m.S] += rho * C.m.acc[k * C.m.S];
+                }
+                C.restore(nv0, np0);
+            }
+            if (overflow) return 2;
         } else {
             // voxel pieces: the cell clipped by each overlapping voxel (ScaledImage.tcc:98-149)
             double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
@@ -564,7 +635,7 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
         if (C.nv == 0) continue;
         ++nfaces;
         // boundary measure: all faces (box domain) / the faces shared with neighbours (image density)
-        if (!a.img || is_neighbour(id)) perim += ar;
+        if ((!a.img && a.nitems < 2) || is_neighbour(id)) perim += ar;
         if (!is_neighbour(id)) continue;
         double nx, ny, nz, hv;
         C.plane64(k, nx, ny, nz, hv);

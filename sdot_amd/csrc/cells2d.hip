@@ -159,7 +159,7 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
     hipStream_t s = pd->stream;
     const bool ball = pd->ball != 0;
     const bool mixed = pd->precision == SDOT_MIXED;
-    const bool img = a.img != nullptr;
+    const bool img = a.img != nullptr || a.nitems >= 2;   // pieces: pixels or several domain items
     const int64_t N = a.N;
     const unsigned nb_all = (unsigned)((N + 63) / 64);
     int *lists[2] = {pd->ovf_list.p, pd->ovf_list.p + N + 1};

@@ -416,7 +416,7 @@ struct WaveCell {
 
     __device__ void plane64(int k, double &nx, double &ny, double &nz, double &h) const {
         const int id = L.pid[k];
-        if (id <= -7) {   // voxel side
+        if (id <= -7 && id >= -12) {   // voxel side
             const int s = -7 - id, axis = s >> 1, up = s & 1;
             const double off = up ? vhi(axis) : vlo(axis);
             nx = axis == 0 ? (up ? 1.0 : -1.0) : 0.0;
@@ -542,8 +542,19 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
         const double c0x = C.c0x, c0y = C.c0y, c0z = C.c0z, w0 = C.w0;
         const double c0[3] = {c0x, c0y, c0z};
         C.init_box();
-        double R2 = C.radius2();
         bool overflow = false, bad = false;
+        if (a.nitems == 1) {   // one domain item: the starting cell is its box cut by its planes
+            const double r0 = sqrt(C.radius2());
+            for (int q = a.it_off[0]; q < a.it_off[1] && C.nv > 0 && !overflow; ++q) {
+                const double *pl = a.it_planes + 4 * q;
+                const double hl = pl[3] - (pl[0] * c0x + pl[1] * c0y + pl[2] * c0z);
+                const double nn = sqrt(pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2]);
+                if (C.cut(T(pl[0]), T(pl[1]), T(pl[2]), T(hl), -100 - q, pl[0], pl[1], pl[2], hl,
+                          c3::Eps3<T>::tie * nn * r0, true) == 2)
+                    overflow = true;
+            }
+        }
+        double R2 = C.radius2();
         const int hh[3] = {grid_coord(g, 0, c0x), grid_coord(g, 1, c0y), grid_coord(g, 2, c0z)};
         // The diracs of the grid cells held by the lanes (ok, [cs, ce)) are staged 64 at a
         // time (one per lane, loaded in parallel), tested against the current radius,
@@ -735,12 +746,84 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
         if (lane < npc) L.hacc[lane] = 0;
         wave_sync();
         if (C.nv > 0) {
-            if (!a.img) {
+            if (!a.img && a.nitems < 2) {
                 const double vol = C.integrate(want);
                 mass = a.coeff * vol;
                 if (want)
                     for (int d = 0; d < 3; ++d) mom[d] = mass * off[d];
                 if (lane < npc) L.hacc[lane] = a.coeff * L.acc[lane];
+            } else if (!a.img) {
+                // several domain items (for_each_intersection, ConvexPolyhedronAssembly.tcc:132-142):
+                // the cell cut by each item's box sides and planes, integrated with its density
+                double X, Y, Z;
+                C.vertex64(X, Y, Z);
+                const bool valid = lane < C.nv;
+                double mn[3], mx[3];
+                mn[0] = -wave_max_dpp(valid ? -X : -DBL_MAX); mx[0] = wave_max_dpp(valid ? X : -DBL_MAX);
+                mn[1] = -wave_max_dpp(valid ? -Y : -DBL_MAX); mx[1] = wave_max_dpp(valid ? Y : -DBL_MAX);
+                mn[2] = -wave_max_dpp(valid ? -Z : -DBL_MAX); mx[2] = wave_max_dpp(valid ? Z : -DBL_MAX);
+                const int nv0 = C.nv, np0 = C.np;
+                const T sx = C.x, sy = C.y, sz = C.z;
+                const int stri = C.tri;
+                for (int k = 0; k < a.nitems && !overflow; ++k) {
+                    const double *b = a.it_box + 6 * k;
+                    C.vlo0 = b[0] - c0[0];
+                    C.vlo1 = b[1] - c0[1];
+                    C.vlo2 = b[2] - c0[2];
+                    C.vhi0 = b[3] - c0[0];
+                    C.vhi1 = b[4] - c0[1];
+                    C.vhi2 = b[5] - c0[2];
+                    if (C.vlo0 > mx[0] || C.vhi0 < mn[0] || C.vlo1 > mx[1] || C.vhi1 < mn[1] || C.vlo2 > mx[2] ||
+                        C.vhi2 < mn[2])
+                        continue;   // the item's box misses the cell
+                    bool empty = false;
+#pragma unroll
+                    for (int sd = 0; sd < 6; ++sd) {
+                        const int axis = sd >> 1, up = sd & 1;
+                        if (empty || overflow) continue;
+                        const double bound = up ? C.vhi(axis) : C.vlo(axis);
+                        if (up ? !(bound < mx[axis]) : !(bound > mn[axis])) continue;   // cannot cut
+                        const double nxx = axis == 0 ? (up ? 1.0 : -1.0) : 0.0, nyy = axis == 1 ? (up ? 1.0 : -1.0) : 0.0,
+                                     nzz = axis == 2 ? (up ? 1.0 : -1.0) : 0.0, hq = up ? bound : -bound;
+                        const int rc = C.cut(T(nxx), T(nyy), T(nzz), T(hq), -7 - sd, nxx, nyy, nzz, hq,
+                                             c3::Eps3<T>::tie * sqrt(R2), false);
+                        if (rc == 2) { overflow = true; break; }
+                        if (C.nv == 0) empty = true;
+                    }
+                    for (int q = a.it_off[k]; q < a.it_off[k + 1] && !empty && !overflow; ++q) {
+                        const double *pl = a.it_planes + 4 * q;
+                        const double hl = pl[3] - (pl[0] * c0[0] + pl[1] * c0[1] + pl[2] * c0[2]);
+                        const double nn = sqrt(pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2]);
+                        const int rc = C.cut(T(pl[0]), T(pl[1]), T(pl[2]), T(hl), -100 - q, pl[0], pl[1], pl[2], hl,
+                                             c3::Eps3<T>::tie * nn * sqrt(R2), false);
+                        if (rc == 2) { overflow = true; break; }
+                        if (C.nv == 0) empty = true;
+                    }
+                    if (!overflow && !empty) {
+                        const double rho = a.it_coeff[k];
+                        const double vol = C.integrate(want);
+                        mass += rho * vol;
+                        if (want)
+                            for (int d = 0; d < 3; ++d) mom[d] += rho * vol * off[d];
+                        if (lane < npc) L.hacc[lane] += rho * L.acc[lane];
+                    }
+                    // restore the cell
+                    C.nv = nv0;
+                    C.np = np0;
+                    C.x = sx;
+                    C.y = sy;
+                    C.z = sz;
+                    C.tri = stri;
+                    C.write_edges();
+                }
+                if (overflow) {
+                    if (lane == 0) {
+                        unsigned kq = atomicAdd(a.ovf_count, 1u);
+                        a.ovf_list[kq] = (int)t;
+                    }
+                    wave_sync();
+                    continue;
+                }
             } else {
                 double X, Y, Z;
                 C.vertex64(X, Y, Z);
@@ -833,7 +916,7 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
             a.ell_val[slot * a.ell_stride + t] = -b;
         }
         const double der = wave_sum_dpp(b);
-        const double perim = wave_sum_dpp(mine && (!a.img || id >= 0) ? ar : 0.0);
+        const double perim = wave_sum_dpp(mine && ((!a.img && a.nitems < 2) || id >= 0) ? ar : 0.0);
         if (lane == 0) {
             a.mass[t] = mass;
             if (a.mom)

@@ -9,6 +9,7 @@
 #include <type_traits>
 #include <string>
 #include <vector>
+#include <array>
 
 #include "../../include/sdot_mi355x.h"
 
@@ -180,6 +181,14 @@ struct CellArgs {
     double dom_lo[3], dom_hi[3];
     double coeff;
     // piecewise-constant image density (3D voxel / 2D pixel), null when constant
+    // ConvexPolyhedronAssembly items when the domain is not a single box (nitems = 0):
+    // item k is the box it_box[6k..6k+5] (lo, hi) cut by the planes it_planes[4q..4q+3]
+    // = (nx, ny, nz, h), n . x <= h, q in [it_off[k], it_off[k+1]), density it_coeff[k].
+    // One item is the starting cell (planes with cut ids -100 - q); several items are
+    // integrated piece by piece from a box around them (dom_lo, dom_hi).
+    int nitems;
+    const double *it_box, *it_coeff, *it_planes;
+    const int *it_off;
     const double *img;
     int img_n[3];
     int img_nc;   // coefficients per pixel: 1, or 6 (2D quadratic, coefficient-major)
@@ -278,10 +287,14 @@ constexpr int WCTR = 1024;       // counters are spread over WCTR slots to avoid
 // ---------------------------------------------------------------------------
 // the handle
 // ---------------------------------------------------------------------------
+// one ConvexPolyhedronAssembly item: the box p0..p1 cut by the planes n . x <= h
+// ({nx, ny, nz, h}), with a constant density
 struct Shape {
     double p0[3], p1[3];
+    double q0[3], q1[3];   // the box the cells are clipped to: p0, p1 with redundant faces moved out
     double coeff;
     int64_t cut_id;
+    std::vector<std::array<double, 4>> planes;
 };
 
 class PcgSolver;
@@ -321,6 +334,8 @@ struct sdot_pd {
     // pass outputs (sorted order)
     sdot_amd::DevBuf<double> mass, arc, perim, diag, ell_val;
     sdot_amd::DevBuf<double> mom;   // first moments (centroids), dim per cell
+    sdot_amd::DevBuf<double> it_box, it_coeff, it_planes;   // domain items on the device
+    sdot_amd::DevBuf<int> it_off;
     bool want_mom = false;          // the next pass computes the first moments
     bool pass_has_mom = false;      // the current pass did
     sdot_amd::DevBuf<int> row_len, ell_col, ovf_list;

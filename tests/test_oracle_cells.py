@@ -222,3 +222,72 @@ def test_centroids_first_moment_of_the_domain(dim, ball, img):
         expected = np.array([(image * np.expand_dims(ctr, tuple(a for a in range(dim) if a != dim - 1 - d))).mean()
                              for d in range(dim)])
     assert np.abs(total - expected).max() < 1e-12
+
+
+def test_assembly_simplex_and_polygon_domains_2d():
+    # one dirac in a triangle (add_simplex): mass = coeff x area, centroid = vertex mean
+    tri = [[0.1, 0.2], [0.9, 0.3], [0.4, 0.8]]
+    it = O.simplex_item(tri, coeff=2.0)
+    area = 0.5 * abs((0.9 - 0.1) * (0.8 - 0.2) - (0.4 - 0.1) * (0.3 - 0.2))
+    st, c, m = O.centroids(np.array([[0.45, 0.4]]), np.zeros(1), items=[it])
+    assert st == 0 and abs(m[0] - 2 * area) < 1e-14
+    assert np.abs(c[0] - np.mean(tri, axis=0)).max() < 1e-14
+    # a regular hexagon (add_convex_polyhedron) holding many cells: the masses sum to its area
+    ang = np.arange(6) * np.pi / 3
+    pts = 0.5 + 0.4 * np.column_stack([np.cos(ang), np.sin(ang)])
+    nrm = np.column_stack([np.cos(ang), np.sin(ang)])
+    hexa = O.convex_polyhedron_item(pts, nrm)      # faces through the corners, normals radial
+    pos, w = inputs(2, 500, 61, wscale=1e-6)
+    pos = 0.5 + 0.5 * (pos - 0.5)                   # all inside the hexagon's inner disc
+    r = O.cells(pos, w, items=[hexa])
+    apothem = 0.4
+    area_hex = 6 * apothem ** 2 / np.sqrt(3)
+    assert r["status"] == 0 and abs(r["masses"].sum() - area_hex) < 1e-12
+    # the same box as add_box and as add_convex_polyhedron: identical cells
+    planes = np.array([[-1, 0, 0], [1, 0, 1], [0, -1, 0], [0, 1, 1]], dtype=float)
+    bp = dict(lo=np.array([-1.0, -1.0]), hi=np.array([2.0, 2.0]), coeff=1.0, planes=planes)
+    r1 = O.cells(pos, w, items=[bp])
+    r0 = O.cells(pos, w)
+    assert np.abs(r1["masses"] - r0["masses"]).max() < 1e-15
+    M0, M1 = csr(*O.to_csr(r0), 500), csr(*O.to_csr(r1), 500)
+    assert abs(M0 - M1).max() < 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_assembly_several_items(dim):
+    # two overlapping boxes with densities 1 and 3: the density adds up where they overlap
+    a = O.box_item(np.zeros(dim), np.full(dim, 0.6), coeff=1.0)
+    b = O.box_item(np.full(dim, 0.4), np.ones(dim), coeff=3.0)
+    total = 0.6 ** dim * 1.0 + 0.6 ** dim * 3.0
+    n = 300
+    pos, w = inputs(dim, n, 62, wscale=0.05 * n ** (-2 / dim))
+    r = O.cells(pos, w, items=[a, b])
+    assert r["status"] in (0, 1)
+    assert abs(r["masses"].sum() - total) < 1e-12
+    # the Hessian is the derivative of the masses (pieces with their own densities)
+    M = csr(*O.to_csr(r), n).toarray()
+    assert np.abs(M - M.T).max() < 1e-10 * np.abs(M).max()
+    eps = 1e-7 * n ** (-2 / dim)
+    # columns of cells that meet both boxes or one of them (diracs outside the union
+    # can have empty cells)
+    cols = [j for j in range(n) if r["masses"][j] > 0.2 / n][:3]
+    assert len(cols) == 3
+    for j in cols:
+        wp, wm = w.copy(), w.copy()
+        wp[j] += eps
+        wm[j] -= eps
+        fd = (O.cells(pos, wp, items=[a, b])["masses"] - O.cells(pos, wm, items=[a, b])["masses"]) / (2 * eps)
+        assert np.abs(fd - M[:, j]).max() < 1e-5 * np.abs(M[:, j]).max()
+    # one dirac: its cell is the union, centroid the density-weighted mean of the boxes
+    st, c, m = O.centroids(np.full((1, dim), 0.5), np.zeros(1), items=[a, b])
+    expect = (0.6 ** dim * 1.0 * 0.3 + 0.6 ** dim * 3.0 * 0.7) / total
+    assert abs(m[0] - total) < 1e-13 and np.abs(c[0] - expect).max() < 1e-13
+
+
+def test_assembly_tetrahedron_3d():
+    tet = O.simplex_item([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], coeff=1.5)
+    n = 200
+    pos, w = inputs(3, n, 63)
+    pos = pos / pos.sum(axis=1, keepdims=True).clip(1, None) * 0.9   # inside the tetrahedron
+    r = O.cells(pos, np.zeros(n), items=[tet])
+    assert abs(r["masses"].sum() - 1.5 / 6) < 1e-13
