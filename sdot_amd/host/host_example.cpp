@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "sdot_mi355x/PowerDiagram.h"
@@ -16,8 +17,58 @@ struct Pc {
     using CI = std::size_t;
 };
 
+// A domain of two items (ConvexPolyhedronAssembly.tcc:7-43): a regular hexagon of
+// apothem 0.4 about (0.5, 0.5) from its face points and normals, density 1, and a
+// triangle of density 2 inside it (density 3 there).  `measure` checks the assembly's measure
+// against the closed forms on the host only; `shapes` also solves on the device.
+static int shapes(bool device) {
+    using namespace sdot_mi355x;
+    ConvexPolyhedronAssembly<Pc> bounds;
+    const double pi = 3.14159265358979323846, a = 0.4;
+    std::vector<Point2> fp, fn;
+    for (int k = 0; k < 6; ++k) {
+        const double nx = std::cos(k * pi / 3), ny = std::sin(k * pi / 3);
+        fp.push_back(Point2{{0.5 + a * nx, 0.5 + a * ny}});
+        fn.push_back(Point2{{nx, ny}});
+    }
+    bounds.add_convex_polyhedron(fp, fn, 1.0);
+    const double tri[6] = {0.3, 0.3, 0.7, 0.35, 0.5, 0.7};
+    bounds.add_simplex(tri, 2.0);
+    const double exact = 2 * std::sqrt(3.0) * a * a + 2.0 * 0.5 * (0.4 * 0.4 - 0.05 * 0.2);
+    const double meas = bounds.measure();
+    double mass_sum = 0, max_rel = 0;
+    int st = 0, iters = 0;
+    if (device) {
+        const std::size_t n = 4000;
+        std::vector<double> u(2 * n);
+        sdot_rng_uniform(3, 2 * n, u.data());
+        std::vector<Point2> positions(n);
+        for (std::size_t i = 0; i < n; ++i) positions[i] = Point2{{0.22 + 0.56 * u[2 * i], 0.22 + 0.56 * u[2 * i + 1]}};
+        std::vector<double> weights(n, 0.0), masses(n, meas / n), v(n);
+        SpZGrid<Pc> grid(11);
+        grid.update(positions.data(), weights.data(), n);
+        get_integrals(v.data(), grid, bounds, positions.data(), weights.data(), n, FunctionEnum::Unit());
+        for (double m : v) mass_sum += m;
+        OptimalTransportSolver<SpZGrid<Pc>, ConvexPolyhedronAssembly<Pc>> solver(&grid, &bounds);
+        solver.stop_rule = 0;
+        solver.tol = 1e-6;
+        st = solver.solve(positions.data(), weights.data(), n, masses.data());
+        iters = solver.stats.newton_iters;
+        get_integrals(v.data(), grid, bounds, positions.data(), weights.data(), n, FunctionEnum::Unit());
+        for (std::size_t i = 0; i < n; ++i) max_rel = std::fmax(max_rel, std::fabs(v[i] / masses[i] - 1));
+    }
+    std::printf("{\"measure\": %.15f, \"exact\": %.15f, \"mass_sum\": %.15f, \"status\": %d, \"newton_iters\": %d, "
+                "\"max_rel_mass_err\": %.3e}\n",
+                meas, exact, mass_sum, st, iters, max_rel);
+    bool ok = std::fabs(meas - exact) < 1e-13;
+    if (device) ok = ok && std::fabs(mass_sum - exact) < 1e-12 && st == 0 && max_rel < 1e-6;
+    return ok ? 0 : 1;
+}
+
 int main(int argc, char **argv) {
     using namespace sdot_mi355x;
+    if (argc > 1 && std::string(argv[1]) == "measure") return shapes(false);
+    if (argc > 1 && std::string(argv[1]) == "shapes") return shapes(true);
     const std::size_t n = argc > 1 ? std::strtoul(argv[1], nullptr, 10) : 10000;
     std::vector<double> u(2 * n);
     sdot_rng_uniform(0, 2 * n, u.data());

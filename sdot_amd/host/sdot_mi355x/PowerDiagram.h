@@ -5,7 +5,8 @@
 //
 // Reference interfaces mirrored (paths relative to the sdot tree):
 //   SpZGrid<Pc>(max_diracs_per_cell), update(...)   src/sdot/PowerDiagram/Visitors/SpZGrid.h:37-39
-//   ConvexPolyhedronAssembly<Pc>::add_box(...)      src/sdot/Domains/ConvexPolyhedronAssembly.h:26-28
+//   ConvexPolyhedronAssembly<Pc>::add_convex_polyhedron / add_simplex / add_box / measure / normalize
+//                                                   src/sdot/Domains/ConvexPolyhedronAssembly.h:26-40
 //   ScaledImage<Pc>(min, max, data, sizes, nb_coeffs) src/sdot/Domains/ScaledImage.h:26
 //   FunctionEnum::Unit, FunctionEnum::InBallW05     src/sdot/Integration/Unit.h, InBallW05.h
 //   get_integrals(res, grid, bounds, pos, w, N, rf) src/sdot/PowerDiagram/get_integrals.h:13
@@ -18,7 +19,9 @@
 // stop rule can be the reference's max|dw| or the max relative mass error.
 #pragma once
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -96,41 +99,211 @@ private:
     int64_t n = 0;
 };
 
-/// ConvexPolyhedronAssembly<Pc> restricted to what the device path evaluates: one box.
+namespace detail {
+// Measure of {x in [lo, hi] : n_q . x <= h_q} (dim 2 or 3): the vertices of the
+// arrangement (every pair / triple of constraints, kept when feasible), then in
+// 2D the shoelace over the vertices in angular order, in 3D the sum over the
+// faces of area x distance to an inner point / 3, each face's vertices in
+// angular order in its plane.  Coincident vertices only add empty edges.
+inline double polytope_measure(int dim, const double *lo, const double *hi,
+                               const std::vector<std::array<double, 4>> &planes) {
+    std::vector<std::array<double, 4>> c = planes;
+    double L = 0;
+    for (int d = 0; d < dim; ++d) {
+        std::array<double, 4> a{0, 0, 0, -lo[d]}, b{0, 0, 0, hi[d]};
+        a[d] = -1;
+        b[d] = 1;
+        c.push_back(a);
+        c.push_back(b);
+        L = std::max(L, hi[d] - lo[d]);
+    }
+    const int m = (int)c.size();
+    std::vector<double> nrm(m);
+    for (int i = 0; i < m; ++i) nrm[i] = std::sqrt(c[i][0] * c[i][0] + c[i][1] * c[i][1] + c[i][2] * c[i][2]);
+    const double tol = 1e-11 * L;
+    std::vector<std::array<double, 3>> vs;
+    auto feasible = [&](const std::array<double, 3> &x) {
+        for (int i = 0; i < m; ++i)
+            if (c[i][0] * x[0] + c[i][1] * x[1] + c[i][2] * x[2] - c[i][3] > tol * nrm[i]) return false;
+        return true;
+    };
+    for (int i = 0; i < m; ++i)
+        for (int j = i + 1; j < m; ++j) {
+            if (dim == 2) {
+                const double det = c[i][0] * c[j][1] - c[i][1] * c[j][0];
+                if (!(std::fabs(det) > 1e-12 * nrm[i] * nrm[j])) continue;
+                const std::array<double, 3> x{(c[i][3] * c[j][1] - c[j][3] * c[i][1]) / det,
+                                              (c[i][0] * c[j][3] - c[j][0] * c[i][3]) / det, 0};
+                if (feasible(x)) vs.push_back(x);
+                continue;
+            }
+            for (int k = j + 1; k < m; ++k) {
+                const auto &a = c[i], &b = c[j], &e = c[k];
+                const double bx = b[1] * e[2] - b[2] * e[1], by = b[2] * e[0] - b[0] * e[2], bz = b[0] * e[1] - b[1] * e[0];
+                const double ex = e[1] * a[2] - e[2] * a[1], ey = e[2] * a[0] - e[0] * a[2], ez = e[0] * a[1] - e[1] * a[0];
+                const double ax = a[1] * b[2] - a[2] * b[1], ay = a[2] * b[0] - a[0] * b[2], az = a[0] * b[1] - a[1] * b[0];
+                const double det = a[0] * bx + a[1] * by + a[2] * bz;
+                if (!(std::fabs(det) > 1e-12 * nrm[i] * nrm[j] * nrm[k])) continue;
+                const std::array<double, 3> x{(a[3] * bx + b[3] * ex + e[3] * ax) / det,
+                                              (a[3] * by + b[3] * ey + e[3] * ay) / det,
+                                              (a[3] * bz + b[3] * ez + e[3] * az) / det};
+                if (feasible(x)) vs.push_back(x);
+            }
+        }
+    if ((int)vs.size() <= dim) return 0;
+    std::array<double, 3> g{0, 0, 0};
+    for (const auto &v : vs)
+        for (int d = 0; d < 3; ++d) g[d] += v[d] / vs.size();
+    // polygon area of the points `ids` about their mean, in the plane of unit normal n
+    auto polygon_area = [&](std::vector<std::array<double, 3>> pts, const double *n) {
+        std::array<double, 3> f{0, 0, 0};
+        for (const auto &v : pts)
+            for (int d = 0; d < 3; ++d) f[d] += v[d] / pts.size();
+        double u[3] = {0, 0, 0};
+        const int ax = std::fabs(n[0]) < 0.6 ? 0 : (std::fabs(n[1]) < 0.6 ? 1 : 2);
+        u[ax] = 1;
+        const double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+        for (int d = 0; d < 3; ++d) u[d] -= un * n[d];
+        const double ul = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        for (int d = 0; d < 3; ++d) u[d] /= ul;
+        const double w[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
+        std::vector<std::pair<double, std::array<double, 2>>> q;
+        for (const auto &v : pts) {
+            const double r[3] = {v[0] - f[0], v[1] - f[1], v[2] - f[2]};
+            const double x = r[0] * u[0] + r[1] * u[1] + r[2] * u[2], y = r[0] * w[0] + r[1] * w[1] + r[2] * w[2];
+            q.push_back({std::atan2(y, x), {x, y}});
+        }
+        std::sort(q.begin(), q.end(), [](const auto &A, const auto &B) { return A.first < B.first; });
+        double a2 = 0;
+        for (std::size_t k = 0; k < q.size(); ++k) {
+            const auto &P = q[k].second, &Q = q[(k + 1) % q.size()].second;
+            a2 += P[0] * Q[1] - Q[0] * P[1];
+        }
+        return 0.5 * std::fabs(a2);
+    };
+    if (dim == 2) {
+        const double n[3] = {0, 0, 1};
+        return polygon_area(vs, n);
+    }
+    double vol = 0;
+    for (int i = 0; i < m; ++i) {
+        std::vector<std::array<double, 3>> face;
+        for (const auto &v : vs)
+            if (std::fabs(c[i][0] * v[0] + c[i][1] * v[1] + c[i][2] * v[2] - c[i][3]) <= tol * nrm[i]) face.push_back(v);
+        if (face.size() < 3) continue;
+        const double n[3] = {c[i][0] / nrm[i], c[i][1] / nrm[i], c[i][2] / nrm[i]};
+        const double dist = (c[i][3] - (c[i][0] * g[0] + c[i][1] * g[1] + c[i][2] * g[2])) / nrm[i];
+        vol += polygon_area(face, n) * dist / 3;
+    }
+    return vol;
+}
+}  // namespace detail
+
+/// ConvexPolyhedronAssembly<Pc>: items added with add_box, add_convex_polyhedron or
+/// add_simplex, each with its density coefficient, passed to the device handle as
+/// they were added (one item is the starting cell; several are integrated piece by
+/// piece).  cut_id must be -1.
 template <class Pc>
 class ConvexPolyhedronAssembly {
 public:
     static constexpr int dim = Pc::dim;
     using Pt = Point<dim>;
     using TF = double;
+    using TI = std::int64_t;
 
-    void add_box(Pt p0, Pt p1, TF coeff = 1.0, std::int64_t cut_id = -1) { boxes.push_back({p0, p1, coeff, cut_id}); }
+    void add_convex_polyhedron(const std::vector<Pt> &positions, const std::vector<Pt> &normals, TF coeff = 1.0,
+                               TI cut_id = -1) {
+        if (positions.size() != normals.size() || positions.empty())
+            throw Error(SDOT_INVALID_ARG, "add_convex_polyhedron: one normal per position");
+        Item it{Item::POLY, coeff, cut_id, {}, {}};
+        for (std::size_t i = 0; i < positions.size(); ++i)
+            for (int d = 0; d < dim; ++d) {
+                it.a.push_back(positions[i][d]);
+                it.b.push_back(normals[i][d]);
+            }
+        items.push_back(it);
+    }
+    void add_simplex(const TF *points, TF coeff = 1.0, TI cut_id = -1) {
+        items.push_back({Item::SIMPLEX, coeff, cut_id, std::vector<double>(points, points + dim * (dim + 1)), {}});
+    }
+    void add_box(Pt p0, Pt p1, TF coeff = 1.0, TI cut_id = -1) {
+        items.push_back({Item::BOX, coeff, cut_id, std::vector<double>(p0.v, p0.v + dim),
+                         std::vector<double>(p1.v, p1.v + dim)});
+    }
+    /// sum of coeff x measure over the items (ConvexPolyhedronAssembly.tcc:115-120)
     TF measure() const {
         TF r = 0;
-        for (const auto &b : boxes) {
-            TF v = b.coeff;
-            for (int d = 0; d < dim; ++d) v *= b.p1[d] - b.p0[d];
-            r += v;
-        }
+        for (const auto &it : items) r += it.coeff * item_measure(it);
         return r;
     }
     void normalize() {
         TF m = measure();
         if (m)
-            for (auto &b : boxes) b.coeff /= m;
+            for (auto &it : items) it.coeff /= m;
     }
     void apply(sdot_pd *pd) const {
         check(pd, sdot_pd_clear_shapes(pd), "clear_shapes");
-        for (const auto &b : boxes) check(pd, sdot_pd_add_box_shape(pd, b.p0.v, b.p1.v, b.coeff, b.cut_id), "add_box");
+        for (const auto &it : items) {
+            if (it.kind == Item::BOX)
+                check(pd, sdot_pd_add_box_shape(pd, it.a.data(), it.b.data(), it.coeff, it.cut_id), "add_box");
+            else if (it.kind == Item::POLY)
+                check(pd, sdot_pd_add_convex_polyhedron_shape(pd, it.a.data(), it.b.data(), (int64_t)(it.a.size() / dim),
+                                                              it.coeff, it.cut_id),
+                      "add_convex_polyhedron");
+            else
+                check(pd, sdot_pd_add_simplex_shape(pd, it.a.data(), it.coeff, it.cut_id), "add_simplex");
+        }
     }
 
 private:
-    struct Box {
-        Pt p0, p1;
+    struct Item {
+        enum Kind { BOX, POLY, SIMPLEX } kind;
         TF coeff;
-        std::int64_t cut_id;
+        TI cut_id;
+        std::vector<double> a, b;   // box: p0, p1; polyhedron: positions, normals; simplex: points
     };
-    std::vector<Box> boxes;
+    static TF item_measure(const Item &it) {
+        double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        std::vector<std::array<double, 4>> planes;
+        if (it.kind == Item::BOX) {
+            TF v = 1;
+            for (int d = 0; d < dim; ++d) v *= it.b[d] - it.a[d];
+            return v;
+        }
+        const std::size_t np = it.a.size() / dim;
+        for (int d = 0; d < dim; ++d) {
+            lo[d] = hi[d] = it.a[d];
+            for (std::size_t i = 1; i < np; ++i) {
+                lo[d] = std::min(lo[d], it.a[dim * i + d]);
+                hi[d] = std::max(hi[d], it.a[dim * i + d]);
+            }
+        }
+        if (it.kind == Item::POLY) {   // the box widened by its extent, cut by (x - p_i) . n_i <= 0
+            for (int d = 0; d < dim; ++d) {
+                const double e = hi[d] - lo[d];
+                lo[d] -= e;
+                hi[d] += e;
+            }
+            for (std::size_t i = 0; i < np; ++i) {
+                std::array<double, 4> pl{0, 0, 0, 0};
+                for (int d = 0; d < dim; ++d) {
+                    pl[d] = it.b[dim * i + d];
+                    pl[3] += it.b[dim * i + d] * it.a[dim * i + d];
+                }
+                planes.push_back(pl);
+            }
+            return detail::polytope_measure(dim, lo, hi, planes);
+        }
+        // simplex: |det(p_k - p_0)| / dim!
+        const double *q = it.a.data();
+        if (dim == 2)
+            return 0.5 * std::fabs((q[2] - q[0]) * (q[5] - q[1]) - (q[3] - q[1]) * (q[4] - q[0]));
+        const double u[3] = {q[3] - q[0], q[4] - q[1], q[5] - q[2]}, v[3] = {q[6] - q[0], q[7] - q[1], q[8] - q[2]},
+                     w[3] = {q[9] - q[0], q[10] - q[1], q[11] - q[2]};
+        return std::fabs(u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0]) +
+                         u[2] * (v[0] * w[1] - v[1] * w[0])) / 6;
+    }
+    std::vector<Item> items;
 };
 
 /// ScaledImage<Pc>: piecewise-constant density on a box, x fastest.

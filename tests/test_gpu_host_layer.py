@@ -23,3 +23,14 @@ def test_host_example_solves_and_exports_csr():
     assert abs(r["mass_sum"] - 1) < 1e-12
     # get_centroids: the mass-weighted mean centroid is the centre of the unit square
     assert r["centroid_mean_err"] < 1e-12
+
+
+def test_host_example_assembly_of_two_shapes():
+    """add_convex_polyhedron + add_simplex through the host layer: the masses sum to
+    the assembly's closed-form measure and the solve reaches the targets."""
+    exe = os.path.join(ROOT, "sdot_amd", "host", "host_example")
+    out = subprocess.run([exe, "shapes"], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    assert out.returncode == 0, r
+    assert abs(r["mass_sum"] - r["exact"]) < 1e-12 and r["status"] == 0 and r["max_rel_mass_err"] < 1e-6
