@@ -71,8 +71,7 @@ sdot_status sdot_pd_dirac_buffers(sdot_pd *pd, double **d_positions, double **d_
 sdot_status sdot_pd_mark_diracs_changed(sdot_pd *pd, int positions_changed);
 
 /* Domain ("bounds").  ConvexPolyhedronAssembly::add_box(p0, p1, coeff, cut_id) -- .tcc:46.
- * One box item is supported on the device path (the configuration every
- * workload uses); several items return SDOT_NOT_SUPPORTED at evaluation. */
+ * Shapes accumulate (see "Several shapes" below); cut_id = -1. */
 sdot_status sdot_pd_add_box_shape(sdot_pd *pd, const double *p0, const double *p1, double coeff, int64_t cut_id);
 /* add_convex_polyhedron(positions, normals, coeff, cut_id) -- ConvexPolyhedronAssembly.tcc:7:
  * the region (x - positions[i]) . normals[i] <= 0 for i < n (n*dim values each), inside
@@ -95,6 +94,14 @@ sdot_status sdot_pd_clear_shapes(sdot_pd *pd);
 /* Radial function: 0 = FunctionEnum::Unit, 1 = FunctionEnum::InBallW05 (clip by the
  * ball of radius sqrt(w_i)) -- src/sdot/Integration/Unit.h, InBallW05.h */
 sdot_status sdot_pd_set_ball_clip(sdot_pd *pd, int on);
+/* SpZGrid::translations (SpZGrid.h:47-53, traversal SpZGrid.tcc:558-561,610): every
+ * cell is also cut by the diracs translated by each of the n vectors
+ * (translations[dim s .. dim s + dim)); a face with the image of dirac j adds to
+ * M_ij with the distance to that image, a face with the cell's own image adds
+ * nothing (get_der_integrals_wrt_weights.h:58-70).  n = 0 clears them.  At most 64
+ * vectors, non-zero and finite; N * (n + 1) < 2^31.  Single problems only (not
+ * batches or the serving loop). */
+sdot_status sdot_pd_set_translations(sdot_pd *pd, const double *translations, int64_t n);
 
 /* Run the cell pass (grid build + cells + Hessian rows) on the device; results
  * stay resident.  Returns SDOT_EMPTY_CELL when a cell is empty or has zero mass

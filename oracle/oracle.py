@@ -23,7 +23,8 @@ _p = ctypes.c_void_p
 class Domain(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("p0", _d * 3), ("p1", _d * 3), ("coeff", _d),
                 ("img", _p), ("img_sizes", _i64 * 3), ("img_nc", _i64),
-                ("nitems", _i64), ("it_box", _p), ("it_coeff", _p), ("it_planes", _p), ("it_off", _p)]
+                ("nitems", _i64), ("it_box", _p), ("it_coeff", _p), ("it_planes", _p), ("it_off", _p),
+                ("ntrans", _i64), ("trans", _p)]
 
 
 class SolveStats(ctypes.Structure):
@@ -109,16 +110,16 @@ def cp2_centroid(box, cuts, ball=None):
     return dict(status=st, centroid=out, area=area.value)
 
 
-def centroids(pos, w, dim=None, ball=False, nthreads=None, domain=None, img=None, items=None):
+def centroids(pos, w, dim=None, ball=False, nthreads=None, domain=None, img=None, items=None, translations=None):
     """get_centroids: (status, centroids[N, dim], masses[N])."""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     dim = dim or pos.shape[1]
     N = pos.shape[0]
     w = np.ascontiguousarray(w, dtype=np.float64)
     if domain is None:
-        dom, keep = make_domain(dim, img=img, items=items)
+        dom, keep = make_domain(dim, img=img, items=items, translations=translations)
     else:
-        dom, keep = make_domain(dim, *domain, img=img, items=items)
+        dom, keep = make_domain(dim, *domain, img=img, items=items, translations=translations)
     c = np.zeros(N * dim)
     m = np.zeros(N)
     st = lib().or_centroids(dim, N, _ptr(pos), _ptr(w), ctypes.byref(dom), 1 if ball else 0,
@@ -179,7 +180,8 @@ def box_item(p0, p1, coeff=1.0):
                 planes=np.zeros((0, dim + 1)))
 
 
-def make_domain(dim, p0=None, p1=None, coeff=1.0, img=None, items=None):
+def make_domain(dim, p0=None, p1=None, coeff=1.0, img=None, items=None, translations=None):
+    """translations: SpZGrid::translations, a (T, dim) array of vectors."""
     d = Domain()
     d.dim = dim
     p0 = p0 if p0 is not None else [0.0] * dim
@@ -225,10 +227,17 @@ def make_domain(dim, p0=None, p1=None, coeff=1.0, img=None, items=None):
         d.it_coeff = coeffs.ctypes.data
         d.it_planes = pl.ctypes.data
         d.it_off = off.ctypes.data
+    d.ntrans = 0
+    if translations is not None and len(translations):
+        tr = np.ascontiguousarray(np.asarray(translations, dtype=np.float64).reshape(-1, dim))
+        keep = (keep, tr)
+        d.ntrans = len(tr)
+        d.trans = tr.ctypes.data
     return d, keep
 
 
-def cells(pos, w, dim=None, ball=False, brute=False, nthreads=None, domain=None, img=None, cap=None, items=None):
+def cells(pos, w, dim=None, ball=False, brute=False, nthreads=None, domain=None, img=None, cap=None, items=None,
+          translations=None):
     """Per-cell masses, ball arc, perimeter and raw rows.  Returns dict."""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     dim = dim or pos.shape[1]
@@ -236,9 +245,9 @@ def cells(pos, w, dim=None, ball=False, brute=False, nthreads=None, domain=None,
     w = np.ascontiguousarray(w, dtype=np.float64)
     cap = cap or (64 if dim == 2 else 160)
     if domain is None:
-        dom, keep = make_domain(dim, img=img, items=items)
+        dom, keep = make_domain(dim, img=img, items=items, translations=translations)
     else:
-        dom, keep = make_domain(dim, *domain, img=img, items=items)
+        dom, keep = make_domain(dim, *domain, img=img, items=items, translations=translations)
     nthreads = nthreads or os.cpu_count() or 1
     masses = np.zeros(N)
     arc = np.zeros(N)
@@ -302,15 +311,15 @@ def pcg(off, cols, vals, b, rtol=1e-12, max_iter=100000):
 
 
 def ot_solve(pos, w0, targets, ball=False, tol=1e-6, stop_rule=0, max_iter=150, nthreads=None,
-             domain=None, img=None, items=None):
+             domain=None, img=None, items=None, translations=None):
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     dim = pos.shape[1]
     w = np.array(w0, dtype=np.float64, copy=True)
     t = np.ascontiguousarray(targets, dtype=np.float64)
     if domain is None:
-        dom, keep = make_domain(dim, img=img, items=items)
+        dom, keep = make_domain(dim, img=img, items=items, translations=translations)
     else:
-        dom, keep = make_domain(dim, *domain, img=img, items=items)
+        dom, keep = make_domain(dim, *domain, img=img, items=items, translations=translations)
     stats = SolveStats()
     st = lib().or_ot_solve(dim, pos.shape[0], _ptr(pos), _ptr(w), _ptr(t), ctypes.byref(dom),
                            1 if ball else 0, nthreads or os.cpu_count() or 1, tol, stop_rule, max_iter,

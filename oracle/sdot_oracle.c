@@ -665,6 +665,50 @@ static double dist_between(const double *pos, int dim, int64_t i, int64_t j) {
     return sqrt(s);
 }
 
+/* the radical cut of cell i by dirac j translated by t (t = NULL: no translation),
+ * cut id `id` (SpZGrid.tcc:506-522, translated positions :610) */
+static void power_cut2_t(or_cp2 *cp, const double *pos, const double *w, int64_t i, int64_t j, const double *t,
+                         int64_t id) {
+    double c0x = pos[2 * i], c0y = pos[2 * i + 1];
+    double vx = pos[2 * j] + (t ? t[0] : 0.0) - c0x, vy = pos[2 * j + 1] + (t ? t[1] : 0.0) - c0y;
+    double n = vx * vx + vy * vy;
+    double x = 0.5 + 0.5 * (w[i] - w[j]) / n;
+    or_cp2_plane_cut(cp, c0x + x * vx, c0y + x * vy, vx, vy, id, 0);
+}
+
+static void power_cut2(or_cp2 *cp, const double *pos, const double *w, int64_t i, int64_t j) {
+    power_cut2_t(cp, pos, w, i, j, NULL, j);
+}
+
+/* a cut id's dirac and its position: j, or the image of j = id mod N under translation
+ * id / N - 1 (get_der_integrals_wrt_weights.h:61-62) */
+static int64_t cut_dirac(const cells_job *J, int64_t id, double *p) {
+    const int dim = J->dim;
+    int64_t j = id % J->N, s = id / J->N - 1;
+    for (int d = 0; d < dim; ++d) p[d] = J->pos[dim * j + d] + (s >= 0 ? J->dom->trans[dim * s + d] : 0.0);
+    return j;
+}
+
+/* Hessian terms of the face of measure `meas` between cell i and cut id `id`
+ * (get_der_integrals_wrt_weights.h:56-70): the image of i itself adds -b to M_ii and
+ * +b to the diagonal, i.e. nothing */
+static void face_term(const cells_job *J, int64_t i, int64_t id, double meas, row_acc *row) {
+    double p[3];
+    int64_t j = cut_dirac(J, id, p);
+    if (j == i) return;
+    double dd = 0;
+    for (int d = 0; d < J->dim; ++d) {
+        double e = J->pos[J->dim * i + d] - p[d];
+        dd += e * e;
+    }
+    dd = sqrt(dd);
+    if (dd != 0) {
+        double b = 0.5 * meas / dd;
+        row_add(row, j, -b);
+        row->der0 += b;
+    }
+}
+
 /* ---- 2D ---- */
 
 /* an image, or several assembly items (for_each_intersection,
@@ -702,22 +746,9 @@ static void piece2_accumulate(const cells_job *J, int64_t i, const or_cp2 *cp, d
             *arcm += meas[k];
             row->der0 += 0.5 * meas[k] / sqrt(wi);
         } else {
-            double dd = dist_between(J->pos, 2, i, ids[k]);
-            if (dd != 0) {
-                double b = 0.5 * meas[k] / dd;
-                row_add(row, ids[k], -b);
-                row->der0 += b;
-            }
+            face_term(J, i, ids[k], meas[k], row);
         }
     }
-}
-
-static void power_cut2(or_cp2 *cp, const double *pos, const double *w, int64_t i, int64_t j) {
-    double c0x = pos[2 * i], c0y = pos[2 * i + 1];
-    double vx = pos[2 * j] - c0x, vy = pos[2 * j + 1] - c0y;
-    double n = vx * vx + vy * vy;
-    double x = 0.5 + 0.5 * (w[i] - w[j]) / n;                /* SpZGrid.tcc:506-522 */
-    or_cp2_plane_cut(cp, c0x + x * vx, c0y + x * vy, vx, vy, j, 0);
 }
 
 /* can_be_evicted, SpZGrid.tcc:475-480: the box cannot cut if, for every vertex p,
@@ -798,6 +829,54 @@ static void start_cell3(const or_domain *dom, or_cp3 *cp) {
     }
 }
 
+/* Ring traversal of the grid around the cell of q = c0 - t: the diracs translated by
+ * t (none for t = NULL) as candidates, cut ids j + idbase; the stop rule and the box
+ * eviction are those of the untranslated traversal taken in the grid's frame. */
+static void traverse2(const cells_job *J, int64_t i, or_cp2 *cp, const double *t, int64_t idbase) {
+    const double *pos = J->pos, *w = J->w;
+    const double tx = t ? t[0] : 0.0, ty = t ? t[1] : 0.0;
+    double c0x = pos[2 * i], c0y = pos[2 * i + 1], w0 = w[i];
+    const or_grid *g = &J->grid;
+    int64_t hx = grid_coord(g, 0, c0x - tx), hy = grid_coord(g, 1, c0y - ty);
+    int64_t maxr = g->ncell[0] > g->ncell[1] ? g->ncell[0] : g->ncell[1];
+    for (int64_t r = 0; r <= maxr && cp->n; ++r) {
+        if (r > 0) {
+            /* every dirac in rings >= r lies outside the box of rings < r */
+            double rx0 = g->lo[0] + (hx - r + 1) * g->h[0] + tx, rx1 = g->lo[0] + (hx + r) * g->h[0] + tx;
+            double ry0 = g->lo[1] + (hy - r + 1) * g->h[1] + ty, ry1 = g->lo[1] + (hy + r) * g->h[1] + ty;
+            int covers = hx - r + 1 <= 0 && hy - r + 1 <= 0 && hx + r >= g->ncell[0] && hy + r >= g->ncell[1];
+            if (covers) break;
+            int stop = 1;
+            for (int k = 0; k < cp->n && stop; ++k) {
+                double px = cp->x[k], py = cp->y[k];
+                double dm = INFINITY;
+                if (hx - r + 1 > 0) dm = fmin(dm, px - rx0);
+                if (hx + r < g->ncell[0]) dm = fmin(dm, rx1 - px);
+                if (hy - r + 1 > 0) dm = fmin(dm, py - ry0);
+                if (hy + r < g->ncell[1]) dm = fmin(dm, ry1 - py);
+                if (dm < 0) dm = 0;
+                double ax = c0x - px, ay = c0y - py;
+                if (!(ax * ax + ay * ay - w0 <= dm * dm - g->wglob)) stop = 0;
+            }
+            if (stop) break;
+        }
+        for (int64_t gy = hy - r < 0 ? 0 : hy - r; gy <= hy + r && gy < g->ncell[1]; ++gy) {
+            int full = llabs(gy - hy) == r;
+            for (int64_t gx = hx - r; gx <= hx + r; gx += (full || r == 0) ? 1 : 2 * r) {
+                if (gx < 0 || gx >= g->ncell[0]) continue;
+                int64_t c = gy * g->ncell[0] + gx;
+                if (g->start[c] == g->start[c + 1]) continue;
+                double bx0 = g->lo[0] + gx * g->h[0] + tx, by0 = g->lo[1] + gy * g->h[1] + ty;
+                if (evict2(cp, c0x, c0y, w0, bx0, by0, bx0 + g->h[0], by0 + g->h[1], g->wmax[c])) continue;
+                for (int64_t k = g->start[c]; k < g->start[c + 1]; ++k) {
+                    int64_t j = g->items[k];
+                    if (j != i || idbase) power_cut2_t(cp, pos, w, i, j, t, j + idbase);
+                }
+            }
+        }
+    }
+}
+
 static int build_cell2(const cells_job *J, int64_t i, or_cp2 *cp) {
     const or_domain *dom = J->dom;
     const double *pos = J->pos, *w = J->w;
@@ -806,46 +885,11 @@ static int build_cell2(const cells_job *J, int64_t i, or_cp2 *cp) {
     if (J->flags & 2) {
         for (int64_t j = 0; j < J->N; ++j)
             if (j != i) power_cut2(cp, pos, w, i, j);
+        for (int64_t s = 0; s < dom->ntrans; ++s)
+            for (int64_t j = 0; j < J->N; ++j) power_cut2_t(cp, pos, w, i, j, dom->trans + 2 * s, j + (s + 1) * J->N);
     } else {
-        const or_grid *g = &J->grid;
-        int64_t hx = grid_coord(g, 0, c0x), hy = grid_coord(g, 1, c0y);
-        int64_t maxr = g->ncell[0] > g->ncell[1] ? g->ncell[0] : g->ncell[1];
-        for (int64_t r = 0; r <= maxr && cp->n; ++r) {
-            if (r > 0) {
-                /* every dirac in rings >= r lies outside the box of rings < r */
-                double rx0 = g->lo[0] + (hx - r + 1) * g->h[0], rx1 = g->lo[0] + (hx + r) * g->h[0];
-                double ry0 = g->lo[1] + (hy - r + 1) * g->h[1], ry1 = g->lo[1] + (hy + r) * g->h[1];
-                int covers = hx - r + 1 <= 0 && hy - r + 1 <= 0 && hx + r >= g->ncell[0] && hy + r >= g->ncell[1];
-                if (covers) break;
-                int stop = 1;
-                for (int k = 0; k < cp->n && stop; ++k) {
-                    double px = cp->x[k], py = cp->y[k];
-                    double dm = INFINITY;
-                    if (hx - r + 1 > 0) dm = fmin(dm, px - rx0);
-                    if (hx + r < g->ncell[0]) dm = fmin(dm, rx1 - px);
-                    if (hy - r + 1 > 0) dm = fmin(dm, py - ry0);
-                    if (hy + r < g->ncell[1]) dm = fmin(dm, ry1 - py);
-                    if (dm < 0) dm = 0;
-                    double ax = c0x - px, ay = c0y - py;
-                    if (!(ax * ax + ay * ay - w0 <= dm * dm - g->wglob)) stop = 0;
-                }
-                if (stop) break;
-            }
-            for (int64_t gy = hy - r < 0 ? 0 : hy - r; gy <= hy + r && gy < g->ncell[1]; ++gy) {
-                int full = llabs(gy - hy) == r;
-                for (int64_t gx = hx - r; gx <= hx + r; gx += (full || r == 0) ? 1 : 2 * r) {
-                    if (gx < 0 || gx >= g->ncell[0]) continue;
-                    int64_t c = gy * g->ncell[0] + gx;
-                    if (g->start[c] == g->start[c + 1]) continue;
-                    double bx0 = g->lo[0] + gx * g->h[0], by0 = g->lo[1] + gy * g->h[1];
-                    if (evict2(cp, c0x, c0y, w0, bx0, by0, bx0 + g->h[0], by0 + g->h[1], g->wmax[c])) continue;
-                    for (int64_t k = g->start[c]; k < g->start[c + 1]; ++k) {
-                        int64_t j = g->items[k];
-                        if (j != i) power_cut2(cp, pos, w, i, j);
-                    }
-                }
-            }
-        }
+        traverse2(J, i, cp, NULL, 0);
+        for (int64_t s = 0; s < dom->ntrans && cp->n; ++s) traverse2(J, i, cp, dom->trans + 2 * s, (s + 1) * J->N);
     }
     if (J->flags & 1) or_cp2_ball_cut(cp, c0x, c0y, sqrt(w0), i);
     else { cp->scx = c0x; cp->scy = c0y; }
@@ -934,13 +978,19 @@ static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
 
 /* ---- 3D ---- */
 
-static void power_cut3(or_cp3 *cp, const double *pos, const double *w, int64_t i, int64_t j) {
+static void power_cut3_t(or_cp3 *cp, const double *pos, const double *w, int64_t i, int64_t j, const double *t,
+                         int64_t id) {
     const double *c0 = pos + 3 * i, *c1 = pos + 3 * j;
-    double V[3] = {c1[0] - c0[0], c1[1] - c0[1], c1[2] - c0[2]};
+    double V[3];
+    for (int d = 0; d < 3; ++d) V[d] = c1[d] + (t ? t[d] : 0.0) - c0[d];
     double n = V[0] * V[0] + V[1] * V[1] + V[2] * V[2];
     double x = 0.5 + 0.5 * (w[i] - w[j]) / n;
     double o[3] = {c0[0] + x * V[0], c0[1] + x * V[1], c0[2] + x * V[2]};
-    or_cp3_plane_cut(cp, o, V, j);
+    or_cp3_plane_cut(cp, o, V, id);
+}
+
+static void power_cut3(or_cp3 *cp, const double *pos, const double *w, int64_t i, int64_t j) {
+    power_cut3_t(cp, pos, w, i, j, NULL, j);
 }
 
 static int evict3(const or_cp3 *cp, const double *c0, double w0, const double *b0, const double *b1, double wmax) {
@@ -959,61 +1009,73 @@ static int evict3(const or_cp3 *cp, const double *c0, double w0, const double *b
     return 1;
 }
 
-static int build_cell3(const cells_job *J, int64_t i, or_cp3 *cp) {
-    const or_domain *dom = J->dom;
+/* the 3D counterpart of traverse2 */
+static void traverse3(const cells_job *J, int64_t i, or_cp3 *cp, const double *t, int64_t idbase) {
     const double *pos = J->pos, *w = J->w;
-    start_cell3(dom, cp);
     const double *c0 = pos + 3 * i;
+    const double sh[3] = {t ? t[0] : 0.0, t ? t[1] : 0.0, t ? t[2] : 0.0};
     double w0 = w[i];
-    if (J->flags & 2) {
-        for (int64_t j = 0; j < J->N && cp->nf; ++j)
-            if (j != i) power_cut3(cp, pos, w, i, j);
-    } else {
-        const or_grid *g = &J->grid;
-        int64_t h[3];
-        for (int d = 0; d < 3; ++d) h[d] = grid_coord(g, d, c0[d]);
-        int64_t maxr = g->ncell[0];
-        if (g->ncell[1] > maxr) maxr = g->ncell[1];
-        if (g->ncell[2] > maxr) maxr = g->ncell[2];
-        for (int64_t r = 0; r <= maxr && cp->nf; ++r) {
-            if (r > 0) {
-                int covers = 1;
-                for (int d = 0; d < 3; ++d) covers &= h[d] - r + 1 <= 0 && h[d] + r >= g->ncell[d];
-                if (covers) break;
-                int stop = 1;
-                for (int f = 0; f < cp->nf && stop; ++f)
-                    for (int k = 0; k < cp->nv[f] && stop; ++k) {
-                        const double *p = cp->v[f][k];
-                        double dm = INFINITY, a = 0;
-                        for (int d = 0; d < 3; ++d) {
-                            if (h[d] - r + 1 > 0) dm = fmin(dm, p[d] - (g->lo[d] + (h[d] - r + 1) * g->h[d]));
-                            if (h[d] + r < g->ncell[d]) dm = fmin(dm, g->lo[d] + (h[d] + r) * g->h[d] - p[d]);
-                            double e = c0[d] - p[d];
-                            a += e * e;
-                        }
-                        if (dm < 0) dm = 0;
-                        if (!(a - w0 <= dm * dm - g->wglob)) stop = 0;
+    const or_grid *g = &J->grid;
+    int64_t h[3];
+    for (int d = 0; d < 3; ++d) h[d] = grid_coord(g, d, c0[d] - sh[d]);
+    int64_t maxr = g->ncell[0];
+    if (g->ncell[1] > maxr) maxr = g->ncell[1];
+    if (g->ncell[2] > maxr) maxr = g->ncell[2];
+    for (int64_t r = 0; r <= maxr && cp->nf; ++r) {
+        if (r > 0) {
+            int covers = 1;
+            for (int d = 0; d < 3; ++d) covers &= h[d] - r + 1 <= 0 && h[d] + r >= g->ncell[d];
+            if (covers) break;
+            int stop = 1;
+            for (int f = 0; f < cp->nf && stop; ++f)
+                for (int k = 0; k < cp->nv[f] && stop; ++k) {
+                    const double *p = cp->v[f][k];
+                    double dm = INFINITY, a = 0;
+                    for (int d = 0; d < 3; ++d) {
+                        if (h[d] - r + 1 > 0) dm = fmin(dm, p[d] - (g->lo[d] + (h[d] - r + 1) * g->h[d] + sh[d]));
+                        if (h[d] + r < g->ncell[d]) dm = fmin(dm, g->lo[d] + (h[d] + r) * g->h[d] + sh[d] - p[d]);
+                        double e = c0[d] - p[d];
+                        a += e * e;
                     }
-                if (stop) break;
-            }
-            for (int64_t gz = h[2] - r < 0 ? 0 : h[2] - r; gz <= h[2] + r && gz < g->ncell[2]; ++gz) {
-                for (int64_t gy = h[1] - r < 0 ? 0 : h[1] - r; gy <= h[1] + r && gy < g->ncell[1]; ++gy) {
-                    int full = llabs(gz - h[2]) == r || llabs(gy - h[1]) == r;
-                    for (int64_t gx = h[0] - r; gx <= h[0] + r; gx += (full || r == 0) ? 1 : 2 * r) {
-                        if (gx < 0 || gx >= g->ncell[0]) continue;
-                        int64_t c = (gz * g->ncell[1] + gy) * g->ncell[0] + gx;
-                        if (g->start[c] == g->start[c + 1]) continue;
-                        double b0[3] = {g->lo[0] + gx * g->h[0], g->lo[1] + gy * g->h[1], g->lo[2] + gz * g->h[2]};
-                        double b1[3] = {b0[0] + g->h[0], b0[1] + g->h[1], b0[2] + g->h[2]};
-                        if (evict3(cp, c0, w0, b0, b1, g->wmax[c])) continue;
-                        for (int64_t k = g->start[c]; k < g->start[c + 1]; ++k) {
-                            int64_t j = g->items[k];
-                            if (j != i) power_cut3(cp, pos, w, i, j);
-                        }
+                    if (dm < 0) dm = 0;
+                    if (!(a - w0 <= dm * dm - g->wglob)) stop = 0;
+                }
+            if (stop) break;
+        }
+        for (int64_t gz = h[2] - r < 0 ? 0 : h[2] - r; gz <= h[2] + r && gz < g->ncell[2]; ++gz) {
+            for (int64_t gy = h[1] - r < 0 ? 0 : h[1] - r; gy <= h[1] + r && gy < g->ncell[1]; ++gy) {
+                int full = llabs(gz - h[2]) == r || llabs(gy - h[1]) == r;
+                for (int64_t gx = h[0] - r; gx <= h[0] + r; gx += (full || r == 0) ? 1 : 2 * r) {
+                    if (gx < 0 || gx >= g->ncell[0]) continue;
+                    int64_t c = (gz * g->ncell[1] + gy) * g->ncell[0] + gx;
+                    if (g->start[c] == g->start[c + 1]) continue;
+                    double b0[3] = {g->lo[0] + gx * g->h[0] + sh[0], g->lo[1] + gy * g->h[1] + sh[1],
+                                    g->lo[2] + gz * g->h[2] + sh[2]};
+                    double b1[3] = {b0[0] + g->h[0], b0[1] + g->h[1], b0[2] + g->h[2]};
+                    if (evict3(cp, c0, w0, b0, b1, g->wmax[c])) continue;
+                    for (int64_t k = g->start[c]; k < g->start[c + 1]; ++k) {
+                        int64_t j = g->items[k];
+                        if (j != i || idbase) power_cut3_t(cp, pos, w, i, j, t, j + idbase);
                     }
                 }
             }
         }
+    }
+}
+
+static int build_cell3(const cells_job *J, int64_t i, or_cp3 *cp) {
+    const or_domain *dom = J->dom;
+    const double *pos = J->pos, *w = J->w;
+    start_cell3(dom, cp);
+    if (J->flags & 2) {
+        for (int64_t j = 0; j < J->N && cp->nf; ++j)
+            if (j != i) power_cut3(cp, pos, w, i, j);
+        for (int64_t s = 0; s < dom->ntrans; ++s)
+            for (int64_t j = 0; j < J->N && cp->nf; ++j)
+                power_cut3_t(cp, pos, w, i, j, dom->trans + 3 * s, j + (s + 1) * J->N);
+    } else {
+        traverse3(J, i, cp, NULL, 0);
+        for (int64_t s = 0; s < dom->ntrans && cp->nf; ++s) traverse3(J, i, cp, dom->trans + 3 * s, (s + 1) * J->N);
     }
     if (cp->overflow) return OR_CAPACITY;
     return cp->nf == 0 ? OR_EMPTY_CELL : OR_OK;             /* ConvexPolyhedron3.h:142 */
@@ -1032,12 +1094,7 @@ static void piece3_accumulate(const cells_job *J, int64_t i, const or_cp3 *cp, d
         int64_t j = cp->id[f];
         if (!pieces_mode(J->dom) || j >= 0) *perim += a;   /* pieces: faces shared with neighbours only */
         if (j == -1) continue;
-        double dd = dist_between(J->pos, 3, i, j);
-        if (dd != 0) {
-            double b = 0.5 * a / dd;
-            row_add(row, j, -b);
-            row->der0 += b;
-        }
+        face_term(J, i, j, a, row);
     }
 }
 

@@ -122,6 +122,11 @@ typedef struct or_domain {
     int64_t nitems;
     const double *it_box, *it_coeff, *it_planes;
     const int64_t *it_off;
+    /* SpZGrid::translations (SpZGrid.h:47-53): every cell is also cut by the diracs
+     * translated by trans[dim s .. dim s + dim) for s < ntrans, the image of dirac j
+     * under translation s carrying the cut id j + (s + 1) N (SpZGrid.tcc:558-561,610) */
+    int64_t ntrans;
+    const double *trans;
 } or_domain;
 
 /* Per-cell evaluation (get_der_integrals_wrt_weights.h:16-122 restated).

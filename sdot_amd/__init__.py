@@ -106,6 +106,7 @@ SIGNATURES = {
     "sdot_pd_set_image_density": (_int, [_p, _p, _p, _p, _p, _i64]),
     "sdot_pd_clear_shapes": (_int, [_p]),
     "sdot_pd_set_ball_clip": (_int, [_p, _int]),
+    "sdot_pd_set_translations": (_int, [_p, _p, _i64]),
     "sdot_pd_compute": (_int, [_p]),
     "sdot_pd_integrals": (_int, [_p, _p]),
     "sdot_pd_add_convex_polyhedron_shape": (_int, [_p, _p, _p, _i64, _d, _i64]),
@@ -262,6 +263,12 @@ class PowerDiagram:
 
     def clear_shapes(self):
         self._check(lib().sdot_pd_clear_shapes(self._h), "clear_shapes")
+
+    def set_translations(self, translations):
+        """SpZGrid::translations: every cell is also cut by the diracs translated by each
+        vector (periodic images); an empty list clears them."""
+        t = np.ascontiguousarray(np.asarray(translations, dtype=np.float64).reshape(-1, self.dim))
+        self._check(lib().sdot_pd_set_translations(self._h, _ptr(t), len(t)), "set_translations")
 
     def set_ball_clip(self, on=True):
         self._check(lib().sdot_pd_set_ball_clip(self._h, int(bool(on))), "set_ball_clip")

@@ -126,6 +126,10 @@ void item_clip_box(Shape &sh, int dim) {
 void domain_args(sdot_pd *pd, CellArgs &a) {
     a.img = nullptr;
     a.img_nc = 1;
+    a.ntrans = (int)(pd->htrans.size() / pd->dim);
+    a.trans = a.ntrans > 0 ? pd->trans.p : nullptr;
+    if (a.ntrans > 0 && (double)pd->N * (a.ntrans + 1) >= 2147483647.0)
+        throw StatusError{SDOT_NOT_SUPPORTED, "translations: N * (translations + 1) must stay below 2^31"};
     if (pd->has_image) {
         if (!pd->shapes.empty()) throw StatusError{SDOT_NOT_SUPPORTED, "image density together with shapes"};
         for (int d = 0; d < 3; ++d) {
@@ -612,6 +616,34 @@ sdot_status sdot_pd_clear_shapes(sdot_pd *pd) {
     pd->has_image = false;
     pd->pass_valid = false;
     return SDOT_OK;
+}
+
+sdot_status sdot_pd_set_translations(sdot_pd *pd, const double *translations, int64_t n) {
+    if (!pd || n < 0 || (n > 0 && !translations)) return SDOT_INVALID_ARG;
+    if (n > 64) {
+        pd->err = "at most 64 translations";
+        return SDOT_NOT_SUPPORTED;
+    }
+    const int dim = pd->dim;
+    for (int64_t s = 0; s < n; ++s) {
+        double nn = 0;
+        for (int d = 0; d < dim; ++d) {
+            if (!std::isfinite(translations[dim * s + d])) return SDOT_INVALID_ARG;
+            nn += translations[dim * s + d] * translations[dim * s + d];
+        }
+        if (!(nn > 0)) return SDOT_INVALID_ARG;
+    }
+    return guard(pd, [&] {
+        pd->htrans.assign(translations, translations + n * dim);
+        if (n > 0) {
+            pd->trans.ensure(pd->htrans.size());
+            HIPCK(hipMemcpyAsync(pd->trans.p, pd->htrans.data(), pd->htrans.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, pd->stream));
+            HIPCK(hipStreamSynchronize(pd->stream));
+        }
+        pd->pass_valid = false;
+        return SDOT_OK;
+    });
 }
 
 sdot_status sdot_pd_set_ball_clip(sdot_pd *pd, int on) {

@@ -75,7 +75,7 @@ constexpr double PI_D = 3.1415926535897932384626433832795;
 
 // Accumulates the boundary items of the final cell (line pieces with a cut id,
 // arcs of the ball) into the integrals and the Hessian row.
-template <class T>
+template <class T, bool PER = false>
 struct Integrator {
     const CellArgs *a;
     int64_t t;
@@ -105,12 +105,14 @@ struct Integrator {
             my += cr * ((double)y0 + y1) / 6;
         }
         if (id >= 0) {
-            double vx = a->spos[2 * (int64_t)id] - c0x, vy = a->spos[2 * (int64_t)id + 1] - c0y;
-            double dist = sqrt(vx * vx + vy * vy);
-            if (dist != 0) {
+            double q[2];
+            const int64_t j = cut_dirac<2, PER>(*a, id, q);
+            const double vx = q[0] - c0x, vy = q[1] - c0y;
+            const double dist = sqrt(vx * vx + vy * vy);
+            if (dist != 0 && (!PER || j != t)) {   // a face with the cell's own image adds nothing
                 double b = 0.5 * a->coeff * len / dist;
                 if (nrow < a->ell_cap) {
-                    a->ell_col[nrow * a->ell_stride + t] = id;
+                    a->ell_col[nrow * a->ell_stride + t] = (int)j;
                     a->ell_val[nrow * a->ell_stride + t] = -b;
                     ++nrow;
                 } else {
@@ -169,12 +171,15 @@ SDOT_HD inline double ball_square(double w0) { return sqrt(fmax(w0, 0.0)) * (1 +
 // The line a*x + b*y = c (coordinates local to the dirac) that carries an edge:
 // the radical line of neighbour id >= 0, box side -1 - id (bottom, right, top,
 // left) for ids -1..-4, or a side of the ball's enclosing square for -5..-8.
+template <bool PER = false>
 SDOT_HD inline void line_of(const CellArgs &a, const ProbView &pv, int id, double c0x, double c0y, double w0,
                             double &la, double &lb, double &lc) {
     if (id >= 0) {
-        la = a.spos[2 * (int64_t)id] - c0x;
-        lb = a.spos[2 * (int64_t)id + 1] - c0y;
-        lc = 0.5 * (la * la + lb * lb + w0 - a.sw[id]);
+        double q[2];
+        const int64_t j = cut_dirac<2, PER>(a, id, q);
+        la = q[0] - c0x;
+        lb = q[1] - c0y;
+        lc = 0.5 * (la * la + lb * lb + w0 - a.sw[j]);
         return;
     }
     if (id <= -100) {   // plane -100 - id of the domain item (n . x <= h)
@@ -239,7 +244,8 @@ SDOT_HD inline bool clip_line(const IdPoly2<PC> &in, IdPoly2<PC> &out, double la
 
 // BATCH = false: a single problem (CellArgs::prob is null), whose grid and domain
 // then stay uniform values (scalar registers) instead of per-thread copies.
-template <class T, int CAP, bool BALL, bool IMG = false, bool BATCH = true>
+// PER: the cell is also cut by the translated diracs (a.ntrans translations).
+template <class T, int CAP, bool BALL, bool IMG = false, bool BATCH = true, bool PER = false>
 SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, int *I, const int S) {
 #define PX(k) X[(k) * S]
 #define PY(k) Y[(k) * S]
@@ -288,8 +294,8 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     // exact-ish side of vertex k (FP64, from its two lines) for the line q.v <= h
     auto side64 = [&](int k, double vx, double vy, double hh) -> double {
         double a1, b1, e1, a2, b2, e2;
-        line_of(a, pv, PI_(k == 0 ? n - 1 : k - 1), c0x, c0y, w0, a1, b1, e1);
-        line_of(a, pv, PI_(k), c0x, c0y, w0, a2, b2, e2);
+        line_of<PER>(a, pv, PI_(k == 0 ? n - 1 : k - 1), c0x, c0y, w0, a1, b1, e1);
+        line_of<PER>(a, pv, PI_(k), c0x, c0y, w0, a2, b2, e2);
         double x = (double)PX(k), y = (double)PY(k);
         const double det = a1 * b2 - a2 * b1;
         if (fabs(det) > 1e-12 * (fabs(a1 * b2) + fabs(a2 * b1))) {
@@ -370,17 +376,22 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         }
     double R2 = radius2();
 
-    const int hx = grid_coord(g, 0, c0x), hy = grid_coord(g, 1, c0y);
     const int maxr = g.ncell[0] > g.ncell[1] ? g.ncell[0] : g.ncell[1];
+    // s = -1: the diracs themselves; s >= 0 (PER): the diracs translated by trans[s],
+    // found around q = c0 - trans[s] in the grid's frame (SpZGrid.tcc:558-561,610)
+    for (int s = -1; s < (PER ? a.ntrans : 0) && n > 0 && !overflow; ++s) {
+    const double qx = s < 0 ? c0x : c0x - a.trans[2 * s], qy = s < 0 ? c0y : c0y - a.trans[2 * s + 1];
+    const int idbase = s < 0 ? 0 : (s + 1) * (int)a.N;
+    const int hx = grid_coord(g, 0, qx), hy = grid_coord(g, 1, qy);
     for (int r = 0; r <= maxr && n > 0 && !overflow; ++r) {
         if (r > 0) {
             bool lf = hx - r + 1 > 0, rt = hx + r < g.ncell[0], bt = hy - r + 1 > 0, tp = hy + r < g.ncell[1];
             if (!(lf || rt || bt || tp)) break;
             double dm = DBL_MAX;
-            if (lf) dm = fmin(dm, c0x - (g.lo[0] + (hx - r + 1) * g.h[0]));
-            if (rt) dm = fmin(dm, (g.lo[0] + (hx + r) * g.h[0]) - c0x);
-            if (bt) dm = fmin(dm, c0y - (g.lo[1] + (hy - r + 1) * g.h[1]));
-            if (tp) dm = fmin(dm, (g.lo[1] + (hy + r) * g.h[1]) - c0y);
+            if (lf) dm = fmin(dm, qx - (g.lo[0] + (hx - r + 1) * g.h[0]));
+            if (rt) dm = fmin(dm, (g.lo[0] + (hx + r) * g.h[0]) - qx);
+            if (bt) dm = fmin(dm, qy - (g.lo[1] + (hy - r + 1) * g.h[1]));
+            if (tp) dm = fmin(dm, (g.lo[1] + (hy + r) * g.h[1]) - qy);
             dm = fmax(dm, 0.0);
             if (region_skippable<T>(dm, R2, w0, W)) break;
         }
@@ -398,19 +409,19 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             if (cs == ce) continue;
             {
                 double bx0 = g.lo[0] + gx * g.h[0], by0 = g.lo[1] + gy * g.h[1];
-                double ex = fmax(0.0, fmax(bx0 - c0x, c0x - (bx0 + g.h[0])));
-                double ey = fmax(0.0, fmax(by0 - c0y, c0y - (by0 + g.h[1])));
+                double ex = fmax(0.0, fmax(bx0 - qx, qx - (bx0 + g.h[0])));
+                double ey = fmax(0.0, fmax(by0 - qy, qy - (by0 + g.h[1])));
                 if (region_skippable<T>(sqrt(ex * ex + ey * ey), R2, w0, pv.cell_wmax[code])) continue;
             }
             for (uint32_t j = cs; j < ce; ++j) {
-                if ((int64_t)j == t) continue;
-                const double vx = a.spos[2 * (int64_t)j] - c0x, vy = a.spos[2 * (int64_t)j + 1] - c0y;
+                if ((int64_t)j == t && s < 0) continue;
+                const double vx = a.spos[2 * (int64_t)j] - qx, vy = a.spos[2 * (int64_t)j + 1] - qy;
                 const double vv = vx * vx + vy * vy;
                 const double hh = 0.5 * (vv + w0 - a.sw[j]);
                 if (hh > 0 && R2 * vv <= hh * hh * (1 - Eps<T>::cand)) continue;   // cannot cut
                 if (vv == 0) { bad = true; continue; }                             // coincident diracs
                 const T Vx = T(vx), Vy = T(vy), H = T(hh);
-                const int rc = cut(Vx, Vy, H, (int)j, vx, vy, hh);
+                const int rc = cut(Vx, Vy, H, (int)j + idbase, vx, vy, hh);
                 if (rc == 2) { overflow = true; break; }
                 if (rc == 0) continue;
                 if (n == 0) break;
@@ -419,6 +430,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
           }
         }
     }
+    }
     if (bad) flag_or(a.flags, FLAG_BAD_INPUT);
     if (overflow) return 1;
 
@@ -426,7 +438,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     // FP64 path: the vertices as clipped.  Mixed path: the FP32 clip fixed
     // which lines bound the cell; each vertex is rebuilt in FP64 as the
     // intersection of its two lines, so the integrals carry no FP32 error.
-    Integrator<double> it;
+    Integrator<double, PER> it;
     it.a = &a;
     it.t = t;
     it.c0x = c0x;
@@ -438,8 +450,8 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         y = (double)PY(k);
         if (sizeof(T) < sizeof(double)) {
             double a1, b1, e1, a2, b2, e2;
-            line_of(a, pv, PI_(k == 0 ? n - 1 : k - 1), c0x, c0y, w0, a1, b1, e1);
-            line_of(a, pv, PI_(k), c0x, c0y, w0, a2, b2, e2);
+            line_of<PER>(a, pv, PI_(k == 0 ? n - 1 : k - 1), c0x, c0y, w0, a1, b1, e1);
+            line_of<PER>(a, pv, PI_(k), c0x, c0y, w0, a2, b2, e2);
             double det = a1 * b2 - a2 * b1;
             if (fabs(det) > 1e-12 * (fabs(a1 * b2) + fabs(a2 * b1))) {
                 x = (e1 * b2 - e2 * b1) / det;
@@ -579,12 +591,14 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             const int id = cell.e[k];
             if (id < 0) continue;
             perim += edge_rho_len[k];   // image density: faces shared with neighbours only
-            const double vx = a.spos[2 * (int64_t)id] - c0x, vy = a.spos[2 * (int64_t)id + 1] - c0y;
+            double q[2];
+            const int64_t j = cut_dirac<2, PER>(a, id, q);
+            const double vx = q[0] - c0x, vy = q[1] - c0y;
             const double dist = sqrt(vx * vx + vy * vy);
-            if (dist == 0) continue;
+            if (dist == 0 || (PER && j == t)) continue;
             const double b = 0.5 * edge_rho_len[k] / dist;
             if (nrow < a.ell_cap) {
-                a.ell_col[nrow * a.ell_stride + t] = id;
+                a.ell_col[nrow * a.ell_stride + t] = (int)j;
                 a.ell_val[nrow * a.ell_stride + t] = -b;
                 ++nrow;
             } else {

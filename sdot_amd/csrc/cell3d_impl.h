@@ -79,10 +79,12 @@ struct Cell3 {
     SDOT_HD void plane64(int k, double &nx, double &ny, double &nz, double &h) const {
         const int id = P_I(k);
         if (id >= 0) {
-            nx = a.spos[3 * (int64_t)id] - c0x;
-            ny = a.spos[3 * (int64_t)id + 1] - c0y;
-            nz = a.spos[3 * (int64_t)id + 2] - c0z;
-            h = 0.5 * (nx * nx + ny * ny + nz * nz + w0 - a.sw[id]);
+            double q[3];
+            const int64_t j = cut_dirac<3>(a, id, q);
+            nx = q[0] - c0x;
+            ny = q[1] - c0y;
+            nz = q[2] - c0z;
+            h = 0.5 * (nx * nx + ny * ny + nz * nz + w0 - a.sw[j]);
             return;
         }
         if (id <= -100) {   // plane -100 - id of the domain item (n . x <= h)
@@ -435,19 +437,25 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
         }
     }
     double R2 = C.radius2();
-    const int h0 = grid_coord(g, 0, c0x), h1 = grid_coord(g, 1, c0y), h2 = grid_coord(g, 2, c0z);
     int maxr = g.ncell[0];
     if (g.ncell[1] > maxr) maxr = g.ncell[1];
     if (g.ncell[2] > maxr) maxr = g.ncell[2];
-    const int hh[3] = {h0, h1, h2};
     const double c0[3] = {c0x, c0y, c0z};
+    // s = -1: the diracs themselves; s >= 0: the diracs translated by trans[s], found
+    // around q = c0 - trans[s] in the grid's frame (SpZGrid.tcc:558-561,610)
+    for (int s = -1; s < a.ntrans && C.nv > 0 && !overflow; ++s) {
+    const double q[3] = {s < 0 ? c0x : c0x - a.trans[3 * s], s < 0 ? c0y : c0y - a.trans[3 * s + 1],
+                         s < 0 ? c0z : c0z - a.trans[3 * s + 2]};
+    const int idbase = s < 0 ? 0 : (s + 1) * (int)a.N;
+    const int h0 = grid_coord(g, 0, q[0]), h1 = grid_coord(g, 1, q[1]), h2 = grid_coord(g, 2, q[2]);
+    const int hh[3] = {h0, h1, h2};
     for (int r = 0; r <= maxr && C.nv > 0 && !overflow; ++r) {
         if (r > 0) {
             double dm = DBL_MAX;
             bool any = false;
             for (int d = 0; d < 3; ++d) {
-                if (hh[d] - r + 1 > 0) { dm = fmin(dm, c0[d] - (g.lo[d] + (hh[d] - r + 1) * g.h[d])); any = true; }
-                if (hh[d] + r < g.ncell[d]) { dm = fmin(dm, (g.lo[d] + (hh[d] + r) * g.h[d]) - c0[d]); any = true; }
+                if (hh[d] - r + 1 > 0) { dm = fmin(dm, q[d] - (g.lo[d] + (hh[d] - r + 1) * g.h[d])); any = true; }
+                if (hh[d] + r < g.ncell[d]) { dm = fmin(dm, (g.lo[d] + (hh[d] + r) * g.h[d]) - q[d]); any = true; }
             }
             if (!any) break;
             if (region_skippable3<T>(fmax(dm, 0.0), R2, w0, W)) break;
@@ -468,20 +476,21 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
                         double e2 = 0;
                         for (int d = 0; d < 3; ++d) {
                             double b0 = g.lo[d] + gg[d] * g.h[d];
-                            double e = fmax(0.0, fmax(b0 - c0[d], c0[d] - (b0 + g.h[d])));
+                            double e = fmax(0.0, fmax(b0 - q[d], q[d] - (b0 + g.h[d])));
                             e2 += e * e;
                         }
                         if (region_skippable3<T>(sqrt(e2), R2, w0, a.cell_wmax[code])) continue;
                     }
                     for (uint32_t j = cs; j < ce; ++j) {
-                        if ((int64_t)j == t) continue;
-                        const double vx = a.spos[3 * (int64_t)j] - c0x, vy = a.spos[3 * (int64_t)j + 1] - c0y,
-                                     vz = a.spos[3 * (int64_t)j + 2] - c0z;
+                        if ((int64_t)j == t && s < 0) continue;
+                        const double vx = a.spos[3 * (int64_t)j] - q[0], vy = a.spos[3 * (int64_t)j + 1] - q[1],
+                                     vz = a.spos[3 * (int64_t)j + 2] - q[2];
                         const double vv = vx * vx + vy * vy + vz * vz;
                         const double hv = 0.5 * (vv + w0 - a.sw[j]);
                         if (hv > 0 && R2 * vv <= hv * hv * (1 - Eps3<T>::cand)) continue;
                         if (vv == 0) { bad = true; continue; }
-                        const int rc = C.cut(T(vx), T(vy), T(vz), T(hv), (int)j, vx, vy, vz, hv, Eps3<T>::tie * sqrt(vv * R2));
+                        const int rc = C.cut(T(vx), T(vy), T(vz), T(hv), (int)j + idbase, vx, vy, vz, hv,
+                                             Eps3<T>::tie * sqrt(vv * R2));
                         if (rc >= 2) { overflow = true; break; }
                         if (rc == 1) {
                             if (C.nv == 0) break;
@@ -491,6 +500,7 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
                 }
             }
         }
+    }
     }
     if (bad) flag_or(a.flags, FLAG_BAD_INPUT);
     if (overflow) return 2;
@@ -637,12 +647,15 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
         // boundary measure: all faces (box domain) / the faces shared with neighbours (image density)
         if ((!a.img && a.nitems < 2) || is_neighbour(id)) perim += ar;
         if (!is_neighbour(id)) continue;
+        double qn[3];
+        const int64_t jn = cut_dirac<3>(a, id, qn);
+        if (jn == t) continue;   // a face with the cell's own image adds nothing
         double nx, ny, nz, hv;
         C.plane64(k, nx, ny, nz, hv);
         const double dist = sqrt(nx * nx + ny * ny + nz * nz);
         const double b = 0.5 * ar / dist;
         if (nrow < a.ell_cap) {
-            a.ell_col[nrow * a.ell_stride + t] = id;
+            a.ell_col[nrow * a.ell_stride + t] = (int)jn;
             a.ell_val[nrow * a.ell_stride + t] = -b;
             ++nrow;
         } else {

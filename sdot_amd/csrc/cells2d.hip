@@ -35,7 +35,7 @@ using namespace c2;
 // device memory, grid-stride.  A cell that outgrows CAP goes to ovf_list for the
 // next tier; in the last tier (no ovf_list) it raises FLAG_CELL_OVERFLOW.
 // WAVES: the wavefronts per SIMD the register allocation must allow.
-template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH>
+template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER>
 __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
     __shared__ T sx[CAP * BS];
     __shared__ T sy[CAP * BS];
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
     }
     for (; k < count; k += step) {
         const int64_t t = a.work ? a.work[k] : k;
-        if (cell2_body<T, CAP, BALL, IMG, BATCH>(a, t, sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x, BS)) {
+        if (cell2_body<T, CAP, BALL, IMG, BATCH, PER>(a, t, sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x, BS)) {
             if (a.ovf_list) {
                 const unsigned q = atomicAdd(a.ovf_count, 1u);
                 a.ovf_list[q] = (int)t;
@@ -125,20 +125,24 @@ __global__ __launch_bounds__(BS) void k_csr_fill(const int *row_len, const int *
     }
 }
 
-template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH>
+template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER = false>
 void launch2(const CellArgs &a, unsigned nb, hipStream_t s) {
     if (nb == 0) return;
-    hipLaunchKernelGGL((k_cells2<T, CAP, BS, BALL, IMG, WAVES, BATCH>), dim3(nb), dim3(BS), 0, s, a);
+    hipLaunchKernelGGL((k_cells2<T, CAP, BS, BALL, IMG, WAVES, BATCH, PER>), dim3(nb), dim3(BS), 0, s, a);
     HIPCK(hipGetLastError());
 }
 
 // ball and image density exclude each other (rejected before the pass); batches
-// carry no image
+// carry no image and no translations
 template <class T, int CAP, int WAVES>
 void dispatch2(bool ball, bool img, const CellArgs &a, unsigned nb, hipStream_t s) {
     if (a.prob) {
         if (ball) launch2<T, CAP, 64, true, false, WAVES, true>(a, nb, s);
         else launch2<T, CAP, 64, false, false, WAVES, true>(a, nb, s);
+    } else if (a.ntrans > 0) {
+        if (ball) launch2<T, CAP, 64, true, false, WAVES, false, true>(a, nb, s);
+        else if (img) launch2<T, CAP, 64, false, true, WAVES, false, true>(a, nb, s);
+        else launch2<T, CAP, 64, false, false, WAVES, false, true>(a, nb, s);
     } else {
         if (ball) launch2<T, CAP, 64, true, false, WAVES, false>(a, nb, s);
         else if (img) launch2<T, CAP, 64, false, true, WAVES, false>(a, nb, s);

@@ -960,8 +960,15 @@ void run_cells3(sdot_pd *pd, CellArgs &a) {
     a.work = nullptr;
     a.work_n = 0;
     const bool mixed = pd->precision == SDOT_MIXED;
-    if (mixed) launch_wave3<float>(pd, a);
-    else launch_wave3<double>(pd, a);
+    // translated diracs (periodic images): every cell on the thread kernel, whose
+    // traversal visits the grid once per translation; otherwise the wavefront kernel
+    if (a.ntrans > 0) {
+        if (mixed) launch_thread3<float, VCAP3, PCAP3, false>(pd, a);
+        else launch_thread3<double, VCAP3, PCAP3, false>(pd, a);
+    } else {
+        if (mixed) launch_wave3<float>(pd, a);
+        else launch_wave3<double>(pd, a);
+    }
     HIPCK(hipEventRecord(pd->ev[3], s));
     unsigned cnt[2] = {0, 0};
     HIPCK(hipMemcpyAsync(cnt, pd->flags.p, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
@@ -970,7 +977,7 @@ void run_cells3(sdot_pd *pd, CellArgs &a) {
     // cells beyond 64 vertices / planes: the thread kernel, tier 1 then tier 2; each
     // tier works on a copy of the list while the next tier's list refills ovf_list
     int *list2 = pd->ovf_list.p + a.N + 1;
-    for (int tier = 1; tier <= 2 && cnt[1] > 0; ++tier) {
+    for (int tier = a.ntrans > 0 ? 2 : 1; tier <= 2 && cnt[1] > 0; ++tier) {
         HIPCK(hipMemcpyAsync(list2, pd->ovf_list.p, cnt[1] * sizeof(int), hipMemcpyDeviceToDevice, s));
         fill_u32(pd->flags.p + 1, 1, 0u, s);
         a.work = list2;

@@ -106,11 +106,12 @@ __host__ __device__ inline uint32_t morton3(int x, int y, int z) {
     return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
 }
 __host__ __device__ inline int grid_coord(const GridInfo &g, int d, double x) {
-    double f = floor((x - g.lo[d]) * g.inv_h[d]);
-    int c = (int)f;
-    if (f < 0) c = 0;
-    if (c >= g.ncell[d]) c = g.ncell[d] - 1;
-    return c;
+    // clamped before the conversion: points far outside the grid (translated
+    // centres) stay in int range, NaN goes to cell 0
+    const double f = floor((x - g.lo[d]) * g.inv_h[d]);
+    if (!(f >= 0)) return 0;
+    if (f >= g.ncell[d]) return g.ncell[d] - 1;
+    return (int)f;
 }
 
 // blockIdx -> work tile mapping that gives each of the 8 XCDs (dispatched
@@ -189,6 +190,11 @@ struct CellArgs {
     int nitems;
     const double *it_box, *it_coeff, *it_planes;
     const int *it_off;
+    // SpZGrid::translations: each cell is also cut by the diracs translated by
+    // trans[dim s .. dim s + dim), s < ntrans; the image of sorted dirac j under s has
+    // the cut id j + (s + 1) N (single problems only; N (ntrans + 1) < 2^31)
+    int ntrans;
+    const double *trans;
     const double *img;
     int img_n[3];
     int img_nc;   // coefficients per pixel: 1, or 6 (2D quadratic, coefficient-major)
@@ -230,6 +236,21 @@ SDOT_HD inline bool void_seen(const CellArgs &a, int p) {
     if (a.pflags && !(a.pstop && a.pstop[p])) return false;
     const unsigned *f = a.pflags ? a.pflags + p : a.flags;
     return (__atomic_load_n(f, __ATOMIC_RELAXED) & FLAG_EMPTY) != 0;
+}
+
+// The dirac of cut id `id` >= 0 and its position p: sorted dirac id, or for id >= N
+// the image of dirac id mod N under translation id / N - 1 (get_der_integrals_wrt_weights.h:61-62)
+template <int DIM, bool PER = true>
+SDOT_HD inline int64_t cut_dirac(const CellArgs &a, int64_t id, double *p) {
+    int64_t j = id;
+    if (PER && j >= a.N) {
+        const int64_t s = j / a.N - 1;
+        j -= (s + 1) * a.N;
+        for (int d = 0; d < DIM; ++d) p[d] = a.spos[DIM * j + d] + a.trans[DIM * s + d];
+    } else {
+        for (int d = 0; d < DIM; ++d) p[d] = a.spos[DIM * j + d];
+    }
+    return j;
 }
 
 // the per-problem view of CellArgs for sorted index t
@@ -335,6 +356,8 @@ struct sdot_pd {
     sdot_amd::DevBuf<double> mass, arc, perim, diag, ell_val;
     sdot_amd::DevBuf<double> mom;   // first moments (centroids), dim per cell
     sdot_amd::DevBuf<double> it_box, it_coeff, it_planes;   // domain items on the device
+    std::vector<double> htrans;                             // translations (dim per vector)
+    sdot_amd::DevBuf<double> trans;
     sdot_amd::DevBuf<int> it_off;
     bool want_mom = false;          // the next pass computes the first moments
     bool pass_has_mom = false;      // the current pass did

@@ -4,7 +4,8 @@
 // OptimalTransportSolver compiles against this header with the namespace changed.
 //
 // Reference interfaces mirrored (paths relative to the sdot tree):
-//   SpZGrid<Pc>(max_diracs_per_cell), update(...)   src/sdot/PowerDiagram/Visitors/SpZGrid.h:37-39
+//   SpZGrid<Pc>(max_diracs_per_cell), update(...), translations
+//                                                   src/sdot/PowerDiagram/Visitors/SpZGrid.h:37-53
 //   ConvexPolyhedronAssembly<Pc>::add_convex_polyhedron / add_simplex / add_box / measure / normalize
 //                                                   src/sdot/Domains/ConvexPolyhedronAssembly.h:26-40
 //   ScaledImage<Pc>(min, max, data, sizes, nb_coeffs) src/sdot/Domains/ScaledImage.h:26
@@ -78,9 +79,16 @@ public:
     SpZGrid(const SpZGrid &) = delete;
     SpZGrid &operator=(const SpZGrid &) = delete;
 
-    /// SpZGrid::update (SpZGrid.tcc:24): uploads diracs; the device grid is rebuilt on the next pass.
+    /// SpZGrid::update (SpZGrid.tcc:24): uploads diracs (and `translations` when they
+    /// changed); the device grid is rebuilt on the next pass.
     void update(const Pt *positions, const TF *weights, TI nb_diracs, bool positions_have_changed = true,
                 bool weights_have_changed = true, bool = false) {
+        std::vector<double> tr;
+        for (const Pt &t : translations) tr.insert(tr.end(), t.v, t.v + dim);
+        if (tr != applied_translations) {
+            check(pd, sdot_pd_set_translations(pd, tr.data(), (int64_t)translations.size()), "translations");
+            applied_translations = tr;
+        }
         if (positions_have_changed || n != (int64_t)nb_diracs) {
             check(pd, sdot_pd_set_diracs(pd, reinterpret_cast<const double *>(positions), weights, (int64_t)nb_diracs),
                   "update");
@@ -93,10 +101,13 @@ public:
     sdot_pd *handle() const { return pd; }
     int64_t size() const { return n; }
     TI max_diracs_per_cell;
+    /// SpZGrid::translations (SpZGrid.h:53): periodic images cut every cell
+    std::vector<Pt> translations;
 
 private:
     sdot_pd *pd = nullptr;
     int64_t n = 0;
+    std::vector<double> applied_translations;
 };
 
 namespace detail {
