@@ -110,6 +110,18 @@ sdot_status sdot_pd_compute(sdot_pd *pd);
 
 /* get_integrals(res, ...) -- get_integrals.h:13.  Runs the pass if needed. */
 sdot_status sdot_pd_integrals(sdot_pd *pd, double *masses);
+/* get_der_centroids_and_integrals_wrt_weight_and_positions(m_offsets, m_columns, m_values,
+ * v_values, ...) -- get_der_centroids_and_integrals_wrt_weight_and_positions.h:16 (2D, one
+ * domain shape, with or without the ball and translations).  Rows 3 n + d (d = 0, 1:
+ * centroid x, y; d = 2: mass of dirac n), columns 3 m + e (e = 0, 1: position x, y;
+ * e = 2: weight of dirac m), sorted; offsets[3N + 1]; v_values[3N] = centroids and
+ * masses.  As in the reference, the centroid rows are divided by mass + (mass == 1), and
+ * the first moments of the faces are not scaled by the density coefficient (exact for
+ * density 1).  Two-phase like sdot_pd_der_integrals_wrt_weights (cols = NULL: nnz only).
+ * Runs its own pass (FP64 geometry in either precision). */
+sdot_status sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions(sdot_pd *pd, int64_t *offsets, int64_t *cols,
+                                                                         double *vals, double *v_values, int64_t cap,
+                                                                         int64_t *nnz);
 /* get_centroids(grid, bounds, positions, weights, N, radial_func, cb) -- get_centroids.h:14
  * (rand_ratio = 0): centroids[N*dim] (dirac order) of the density over each cell,
  * 0 for an empty cell; masses may be NULL.  Runs the pass (with first moments) if

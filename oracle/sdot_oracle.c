@@ -636,6 +636,10 @@ typedef struct {
     int64_t *row_len, *cols;
     double *vals;
     or_grid grid;
+    /* get_der_centroids_and_integrals_wrt_weight_and_positions (2D, one shape): per
+     * dirac up to cap (column, 3x3 block) items and v = (centroid, mass) */
+    int64_t *jlen, *jcols;
+    double *jblocks, *jv;
     int64_t next;          /* shared chunk counter */
     int64_t end;           /* cells [next0, end) are evaluated */
     pthread_mutex_t mtx;
@@ -896,6 +900,147 @@ static int build_cell2(const cells_job *J, int64_t i, or_cp2 *cp) {
     return or_cp2_empty(cp) ? OR_EMPTY_CELL : OR_OK;
 }
 
+/* get_der_centroids_and_integrals_wrt_weight_and_positions.h:46-190 for one 2D cell
+ * integrated as one piece of density coeff: the boundary items of for_each_boundary_item
+ * (ConvexPolyhedron2.tcc:437-477: straight edges carry pos_integral = l (p0 + p1) / 2 and
+ * momentum[a][b] = l / 6 (2 (p0a p0b + p1a p1b) + p0a p1b + p1a p0b), not scaled by the
+ * density, measure = coeff l; arcs carry their angles), block entries [3 d + e] for
+ * output d (centroid x, y, mass) and unknown e (position x, y, weight) */
+static void jac2_cell(cells_job *J, int64_t i, const or_cp2 *cp, double coeff, double mass, const double *cent) {
+    const int64_t cap = J->cap;
+    int64_t *cols = J->jcols + i * cap;
+    double *blk = J->jblocks + i * cap * 9;
+    int64_t m = 0;
+    double der0[9] = {0};
+    const double c0[2] = {J->pos[2 * i], J->pos[2 * i + 1]};
+    const double R = sqrt(J->w[i]);   /* the ball's radius (:70), also for a void part whose sphere radius is 0 */
+    int overflow = 0;
+    int nitem = cp->n ? cp->n : (cp->sr >= 0 ? 1 : 0);
+    for (int k = 0; k < nitem; ++k) {
+        int64_t id;
+        double measure, a0 = 1, a1 = 0, C[2] = {0, 0}, M[2][2] = {{0, 0}, {0, 0}};
+        int is_arc;
+        if (cp->n == 0) {                       /* the whole circle (measure 0 for a void part, :444-452) */
+            id = cp->sid;
+            measure = coeff * 2 * M_PI * cp->sr;
+            is_arc = 1;
+        } else {
+            int i0 = k, i1 = k + 1 == cp->n ? 0 : k + 1;
+            const double p0[2] = {cp->x[i0], cp->y[i0]}, p1[2] = {cp->x[i1], cp->y[i1]};
+            id = cp->id[i0];
+            is_arc = cp->arc[i0];
+            if (is_arc) {
+                a0 = atan2(p0[1] - cp->scy, p0[0] - cp->scx);
+                a1 = atan2(p1[1] - cp->scy, p1[0] - cp->scx);
+                if (a1 < a0) a1 += 2 * M_PI;
+                measure = coeff * (a1 - a0) * R;
+            } else {
+                double l = sqrt((p1[0] - p0[0]) * (p1[0] - p0[0]) + (p1[1] - p0[1]) * (p1[1] - p0[1]));
+                for (int a = 0; a < 2; ++a) C[a] = 0.5 * l * (p0[a] + p1[a]);
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b)
+                        M[a][b] = l / 6 * (2 * (p0[a] * p0[b] + p1[a] * p1[b]) + p0[a] * p1[b] + p1[a] * p0[b]);
+                measure = coeff * l;
+            }
+        }
+        if (id == -1) continue;
+        if (id == i) {                          /* the ball (:66-122) */
+            double mm = 0.5 * measure / R;
+            der0[8] += mm;
+            double mx = 0, my = 0;
+            if (a0 < a1) {
+                mx = R * (sin(a1) - sin(a0));
+                my = R * (cos(a0) - cos(a1));
+                der0[6] += mx;
+                der0[7] += my;
+            }
+            double T[2] = {mm * c0[0], mm * c0[1]};
+            if (a0 < a1) {
+                T[0] += 0.5 * R * (sin(a1) - sin(a0));
+                T[1] += 0.5 * R * (cos(a0) - cos(a1));
+            }
+            der0[2] += T[0];
+            der0[5] += T[1];
+            if (a0 < a1) {
+                der0[0] += mx * c0[0] + R * R / 2 * ((a1 + sin(a1) * cos(a1)) - (a0 + sin(a0) * cos(a0)));
+                der0[1] += my * c0[0] + R * R / 4 * (cos(2 * a0) - cos(2 * a1));
+                der0[3] += mx * c0[1] + R * R / 4 * (cos(2 * a0) - cos(2 * a1));
+                der0[4] += my * c0[1] + R * R / 2 * ((a1 - sin(a1) * cos(a1)) - (a0 - sin(a0) * cos(a0)));
+            } else {
+                der0[0] += M_PI * R * R;
+                der0[4] += M_PI * R * R;
+            }
+            continue;
+        }
+        /* a neighbour or the image of one (:124-166) */
+        double q[2];
+        int64_t j = cut_dirac(J, id, q);
+        double dist = sqrt((q[0] - c0[0]) * (q[0] - c0[0]) + (q[1] - c0[1]) * (q[1] - c0[1]));
+        double b = 0.5 * measure / dist, d1[9] = {0};
+        der0[8] += b;
+        d1[8] = -b;
+        for (int e = 0; e < 2; ++e) {
+            double v = 0.5 * C[e] / dist;
+            der0[3 * e + 2] += v;
+            d1[3 * e + 2] = -v;
+        }
+        for (int d = 0; d < 2; ++d) {
+            der0[6 + d] += (C[d] - measure * c0[d]) / dist;
+            d1[6 + d] = (measure * q[d] - C[d]) / dist;
+        }
+        for (int d = 0; d < 2; ++d)
+            for (int e = 0; e < 2; ++e) {
+                der0[3 * e + d] += (M[d][e] - C[e] * c0[d]) / dist;
+                d1[3 * e + d] = (C[e] * q[d] - M[d][e]) / dist;
+            }
+        if (m >= cap) { overflow = 1; continue; }
+        cols[m] = j;
+        memcpy(blk + 9 * m, d1, sizeof d1);
+        ++m;
+    }
+    if (m >= cap) overflow = 1;
+    else {
+        cols[m] = i;
+        memcpy(blk + 9 * m, der0, sizeof der0);
+        ++m;
+    }
+    /* sort by column, merge duplicates (:170-186) */
+    for (int64_t a = 1; a < m; ++a)
+        for (int64_t b2 = a; b2 > 0 && cols[b2 - 1] > cols[b2]; --b2) {
+            int64_t tc = cols[b2]; cols[b2] = cols[b2 - 1]; cols[b2 - 1] = tc;
+            double tb[9];
+            memcpy(tb, blk + 9 * b2, sizeof tb);
+            memcpy(blk + 9 * b2, blk + 9 * (b2 - 1), sizeof tb);
+            memcpy(blk + 9 * (b2 - 1), tb, sizeof tb);
+        }
+    int64_t w = 0;
+    for (int64_t a = 0; a < m; ++a) {
+        if (w > 0 && cols[w - 1] == cols[a]) {
+            for (int k = 0; k < 9; ++k) blk[9 * (w - 1) + k] += blk[9 * a + k];
+            continue;
+        }
+        if (w != a) { cols[w] = cols[a]; memcpy(blk + 9 * w, blk + 9 * a, 9 * sizeof(double)); }
+        ++w;
+    }
+    m = w;
+    /* v and the mass / centroid correction of the centroid rows (:172-177; the
+     * reference divides by mass + (mass == 1)) */
+    double *v = J->jv + 3 * i;
+    v[0] = cent[0];
+    v[1] = cent[1];
+    v[2] = mass;
+    for (int64_t a = 0; a < m; ++a)
+        for (int r = 0; r < 2; ++r)
+            for (int c = 0; c < 3; ++c)
+                blk[9 * a + 3 * r + c] = (blk[9 * a + 3 * r + c] - blk[9 * a + 6 + c] * v[r]) / (mass + (mass == 1));
+    J->jlen[i] = m;
+    if (overflow) {
+        pthread_mutex_lock(&J->mtx);
+        J->status = OR_CAPACITY;
+        pthread_mutex_unlock(&J->mtx);
+    }
+}
+
 static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
     row_acc row = { J->cols + i * J->cap, J->vals + i * J->cap, 1, J->cap, 0.0, 0 };
     row.cols[0] = i;
@@ -963,6 +1108,11 @@ static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
     J->masses[i] = mass;
     if (J->centroids) {   /* get_centroids.h:53-58: centroid / (mass + (mass == 0)) */
         for (int d = 0; d < 2; ++d) J->centroids[2 * i + d] = mass != 0 ? J->pos[2 * i + d] + mom[d] / mass : 0.0;
+    }
+    if (J->jblocks) {     /* one shape only (checked by or_der_pos2) */
+        double cent[2];
+        for (int d = 0; d < 2; ++d) cent[d] = mass != 0 ? J->pos[2 * i + d] + mom[d] / mass : 0.0;
+        jac2_cell(J, i, cp, dom->nitems == 1 ? dom->it_coeff[0] : dom->coeff, mass, cent);
     }
     if (J->ball_arc) J->ball_arc[i] = arcm;
     if (J->perimeter) J->perimeter[i] = perim;
@@ -1223,10 +1373,39 @@ int or_centroids(int dim, int64_t N, const double *pos, const double *w, const o
     return st;
 }
 
+static int cells_run_j(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom,
+                       int flags, int nthreads, int64_t i_begin, int64_t i_end,
+                       double *masses, double *ball_arc, double *perimeter, double *centroids,
+                       int64_t cap, int64_t *row_len, int64_t *cols, double *vals,
+                       int64_t *jlen, int64_t *jcols, double *jblocks, double *jv);
+
 static int cells_run(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom,
                      int flags, int nthreads, int64_t i_begin, int64_t i_end,
                      double *masses, double *ball_arc, double *perimeter, double *centroids,
                      int64_t cap, int64_t *row_len, int64_t *cols, double *vals) {
+    return cells_run_j(dim, N, pos, w, dom, flags, nthreads, i_begin, i_end, masses, ball_arc, perimeter, centroids,
+                       cap, row_len, cols, vals, NULL, NULL, NULL, NULL);
+}
+
+int or_der_pos2(int64_t N, const double *pos, const double *w, const or_domain *dom, int flags, int nthreads,
+                int64_t cap, int64_t *jlen, int64_t *jcols, double *jblocks, double *jv) {
+    if (!dom || dom->dim != 2 || dom->img || dom->nitems >= 2 || !jlen || !jcols || !jblocks || !jv) return OR_INVALID;
+    double *masses = (double *)malloc(sizeof(double) * (N + 1));
+    double *cent = (double *)malloc(sizeof(double) * (2 * N + 1));
+    int64_t *row_len = (int64_t *)malloc(sizeof(int64_t) * (N + 1));
+    int64_t *cols = (int64_t *)malloc(sizeof(int64_t) * (N * cap + 1));
+    double *vals = (double *)malloc(sizeof(double) * (N * cap + 1));
+    int st = cells_run_j(2, N, pos, w, dom, flags, nthreads, 0, N, masses, NULL, NULL, cent, cap, row_len, cols, vals,
+                         jlen, jcols, jblocks, jv);
+    free(masses); free(cent); free(row_len); free(cols); free(vals);
+    return st;
+}
+
+static int cells_run_j(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom,
+                       int flags, int nthreads, int64_t i_begin, int64_t i_end,
+                       double *masses, double *ball_arc, double *perimeter, double *centroids,
+                       int64_t cap, int64_t *row_len, int64_t *cols, double *vals,
+                       int64_t *jlen, int64_t *jcols, double *jblocks, double *jv) {
     if (i_begin < 0 || i_end > N || i_begin > i_end) return OR_INVALID;
     if ((dim != 2 && dim != 3) || N < 0 || cap < 1 || !dom || dom->dim != dim) return OR_INVALID;
     if (dim == 3 && (flags & 1)) return OR_INVALID;  /* ConvexPolyhedron3::ball_cut is TODO (:292-293) */
@@ -1238,6 +1417,7 @@ static int cells_run(int dim, int64_t N, const double *pos, const double *w, con
     J.dim = dim; J.N = N; J.pos = pos; J.w = w; J.dom = dom; J.flags = flags;
     J.masses = masses; J.ball_arc = ball_arc; J.perimeter = perimeter; J.centroids = centroids;
     J.cap = cap; J.row_len = row_len; J.cols = cols; J.vals = vals;
+    J.jlen = jlen; J.jcols = jcols; J.jblocks = jblocks; J.jv = jv;
     J.next = i_begin;
     J.end = i_end;
     J.status = OR_OK;

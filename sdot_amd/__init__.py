@@ -112,6 +112,8 @@ SIGNATURES = {
     "sdot_pd_add_convex_polyhedron_shape": (_int, [_p, _p, _p, _i64, _d, _i64]),
     "sdot_pd_add_simplex_shape": (_int, [_p, _p, _d, _i64]),
     "sdot_pd_centroids": (_int, [_p, _p, _p]),
+    "sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions": (_int, [_p, _p, _p, _p, _p, _i64,
+                                                                           ctypes.POINTER(_i64)]),
     "sdot_pd_boundary_measures": (_int, [_p, _p, _p]),
     "sdot_pd_der_integrals_wrt_weights": (_int, [_p, _p, _p, _p, _p, _i64, ctypes.POINTER(_i64)]),
     "sdot_ot_solve": (_int, [_p, _p, _d, _int, _int, _p, ctypes.POINTER(SolveStats)]),
@@ -296,6 +298,21 @@ class PowerDiagram:
         m = np.empty(self.N)
         self._check(lib().sdot_pd_centroids(self._h, _ptr(c), _ptr(m)), "centroids", allow_empty)
         return c, m
+
+    def der_centroids_and_integrals_wrt_weight_and_positions(self, allow_empty=True):
+        """get_der_centroids_and_integrals_wrt_weight_and_positions (2D): (offsets, cols, vals, v)
+        with rows 3 n + d (centroid x, y, mass) and columns 3 m + e (position x, y, weight)."""
+        nnz = _i64(0)
+        f = lib().sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions
+        self._check(f(self._h, None, None, None, None, 0, ctypes.byref(nnz)), "der_centroids_and_integrals",
+                    allow_empty)
+        off = np.zeros(3 * self.N + 1, dtype=np.int64)
+        cols = np.zeros(max(nnz.value, 1), dtype=np.int64)
+        vals = np.zeros(max(nnz.value, 1))
+        v = np.zeros(3 * self.N)
+        self._check(f(self._h, _ptr(off), _ptr(cols), _ptr(vals), _ptr(v), nnz.value, ctypes.byref(nnz)),
+                    "der_centroids_and_integrals", allow_empty)
+        return off, cols[:nnz.value], vals[:nnz.value], v
 
     def boundary_measures(self, allow_empty=True):
         a = np.empty(self.N)

@@ -259,6 +259,13 @@ sdot_status compute_pass(sdot_pd *pd) {
         a.mom = pd->mom.p;
     }
     pd->pass_has_mom = pd->want_mom;
+    a.jblk = a.jdiag = nullptr;
+    if (pd->want_jac) {
+        pd->jblk.ensure((size_t)9 * pd->ell_cap * N);
+        pd->jdiag.ensure((size_t)9 * N);
+        a.jblk = pd->jblk.p;
+        a.jdiag = pd->jdiag.p;
+    }
     a.ovf_list = pd->ovf_list.p;
     a.ovf_count = pd->flags.p + 1;
     pd->wctr.ensure(3 * WCTR);
@@ -718,6 +725,37 @@ sdot_status sdot_pd_der_integrals_wrt_weights(sdot_pd *pd, int64_t *offsets, int
         sdot_status st = ensure_pass(pd);
         export_csr(pd, offsets, cols, vals, cap, nnz);
         if (masses) export_vec(pd, pd->mass.p, masses);
+        return st;
+    });
+}
+
+sdot_status sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions(sdot_pd *pd, int64_t *offsets, int64_t *cols,
+                                                                         double *vals, double *v_values, int64_t cap,
+                                                                         int64_t *nnz) {
+    if (!pd || !nnz || (cols && (!offsets || !vals))) return SDOT_INVALID_ARG;
+    if (pd->dim != 2) {
+        pd->err = "position derivatives: 2D only";
+        return SDOT_NOT_SUPPORTED;
+    }
+    if (pd->has_image || pd->shapes.size() != 1) {
+        pd->err = "position derivatives: one domain shape (no image, no assembly of several shapes)";
+        return SDOT_NOT_SUPPORTED;
+    }
+    return guard(pd, [&] {
+        // a pass that also writes the derivative blocks (and the first moments); it is a
+        // full pass, kept for the other getters
+        struct Restore {
+            sdot_pd *pd;
+            bool mom;
+            ~Restore() {
+                pd->want_jac = false;
+                pd->want_mom = mom;
+            }
+        } restore{pd, pd->want_mom};
+        pd->want_jac = true;
+        pd->want_mom = true;
+        sdot_status st = compute_pass(pd);
+        export_jac(pd, offsets, cols, vals, v_values, cap, nnz);
         return st;
     });
 }

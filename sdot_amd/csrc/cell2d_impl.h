@@ -75,11 +75,17 @@ constexpr double PI_D = 3.1415926535897932384626433832795;
 
 // Accumulates the boundary items of the final cell (line pieces with a cut id,
 // arcs of the ball) into the integrals and the Hessian row.
-template <class T, bool PER = false>
+// JAC: also the blocks of get_der_centroids_and_integrals_wrt_weight_and_positions
+// (get_der_centroids_and_integrals_wrt_weight_and_positions.h:46-166): one 3x3 block
+// per neighbour slot (a->jblk, beside the Hessian entry of that slot) and the
+// diagonal block jd (a->jdiag); entry [3 d + e] for output d (centroid x, y, mass)
+// and unknown e (position x, y, weight), before the mass / centroid correction.
+template <class T, bool PER = false, bool JAC = false>
 struct Integrator {
     const CellArgs *a;
     int64_t t;
     double c0x, c0y;
+    double jd[JAC ? 9 : 1] = {};
     T r, r2;
     CSum<T> area2;       // twice the polygonal area (shoelace about the dirac)
     double arcs = 0;     // circular segment areas
@@ -109,6 +115,33 @@ struct Integrator {
             const int64_t j = cut_dirac<2, PER>(*a, id, q);
             const double vx = q[0] - c0x, vy = q[1] - c0y;
             const double dist = sqrt(vx * vx + vy * vy);
+            if (JAC && dist != 0) {
+                // straight boundary item (ConvexPolyhedron2.tcc:437-477) in absolute
+                // coordinates: pos_integral C and momentum M carry no density, measure does
+                const double P0[2] = {c0x + (double)x0, c0y + (double)y0}, P1[2] = {c0x + (double)x1, c0y + (double)y1};
+                const double c0[2] = {c0x, c0y};
+                double C[2], M[2][2], d1[9];
+                for (int u = 0; u < 2; ++u) C[u] = 0.5 * len * (P0[u] + P1[u]);
+                for (int u = 0; u < 2; ++u)
+                    for (int v = 0; v < 2; ++v)
+                        M[u][v] = len / 6 * (2 * (P0[u] * P0[v] + P1[u] * P1[v]) + P0[u] * P1[v] + P1[u] * P0[v]);
+                const double meas = a->coeff * len, bw = 0.5 * meas / dist;
+                d1[8] = -bw;
+                for (int e = 0; e < 2; ++e) d1[3 * e + 2] = -0.5 * C[e] / dist;
+                for (int d = 0; d < 2; ++d) d1[6 + d] = (meas * q[d] - C[d]) / dist;
+                for (int d = 0; d < 2; ++d)
+                    for (int e = 0; e < 2; ++e) d1[3 * e + d] = (C[e] * q[d] - M[d][e]) / dist;
+                jd[8] += bw;
+                for (int e = 0; e < 2; ++e) jd[3 * e + 2] += 0.5 * C[e] / dist;
+                for (int d = 0; d < 2; ++d) jd[6 + d] += (C[d] - meas * c0[d]) / dist;
+                for (int d = 0; d < 2; ++d)
+                    for (int e = 0; e < 2; ++e) jd[3 * e + d] += (M[d][e] - C[e] * c0[d]) / dist;
+                if (PER && j == t) {   // the cell's own image: its block merges into the diagonal
+                    for (int k = 0; k < 9; ++k) jd[k] += d1[k];
+                } else if (nrow < a->ell_cap) {
+                    for (int k = 0; k < 9; ++k) a->jblk[((int64_t)nrow * 9 + k) * a->ell_stride + t] = d1[k];
+                }
+            }
             if (dist != 0 && (!PER || j != t)) {   // a face with the cell's own image adds nothing
                 double b = 0.5 * a->coeff * len / dist;
                 if (nrow < a->ell_cap) {
@@ -130,6 +163,27 @@ struct Integrator {
         arcm += l;
         perim += l;
         der_arc += 0.5 * (double)da;
+        if (JAC) {   // ball item (get_der_centroids_and_integrals_wrt_weight_and_positions.h:66-114)
+            const double R = (double)r, A0 = (double)a0, A1 = (double)a0 + (double)da;
+            const double mm = 0.5 * a->coeff * (double)da;
+            const double mxx = R * (sin(A1) - sin(A0)), myy = R * (cos(A0) - cos(A1));
+            jd[8] += mm;
+            if (A0 < A1) {
+                jd[6] += mxx;
+                jd[7] += myy;
+                jd[2] += mm * c0x + 0.5 * R * (sin(A1) - sin(A0));
+                jd[5] += mm * c0y + 0.5 * R * (cos(A0) - cos(A1));
+                jd[0] += mxx * c0x + R * R / 2 * ((A1 + sin(A1) * cos(A1)) - (A0 + sin(A0) * cos(A0)));
+                jd[1] += myy * c0x + R * R / 4 * (cos(2 * A0) - cos(2 * A1));
+                jd[3] += mxx * c0y + R * R / 4 * (cos(2 * A0) - cos(2 * A1));
+                jd[4] += myy * c0y + R * R / 2 * ((A1 - sin(A1) * cos(A1)) - (A0 - sin(A0) * cos(A0)));
+            } else {
+                jd[2] += mm * c0x;
+                jd[5] += mm * c0y;
+                jd[0] += PI_D * R * R;
+                jd[4] += PI_D * R * R;
+            }
+        }
         if (a->mom) {   // ball sector (_centroid_arc, ConvexPolyhedron2.tcc:2296-2330)
             const double mid = (double)a0 + 0.5 * (double)da;
             const double k = 2.0 / 3.0 * (double)r * r * r * sin(0.5 * (double)da);
@@ -245,7 +299,8 @@ SDOT_HD inline bool clip_line(const IdPoly2<PC> &in, IdPoly2<PC> &out, double la
 // BATCH = false: a single problem (CellArgs::prob is null), whose grid and domain
 // then stay uniform values (scalar registers) instead of per-thread copies.
 // PER: the cell is also cut by the translated diracs (a.ntrans translations).
-template <class T, int CAP, bool BALL, bool IMG = false, bool BATCH = true, bool PER = false>
+// JAC: the position / weight derivative blocks (one domain shape, no pieces).
+template <class T, int CAP, bool BALL, bool IMG = false, bool BATCH = true, bool PER = false, bool JAC = false>
 SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, int *I, const int S) {
 #define PX(k) X[(k) * S]
 #define PY(k) Y[(k) * S]
@@ -438,7 +493,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     // FP64 path: the vertices as clipped.  Mixed path: the FP32 clip fixed
     // which lines bound the cell; each vertex is rebuilt in FP64 as the
     // intersection of its two lines, so the integrals carry no FP32 error.
-    Integrator<double, PER> it;
+    Integrator<double, PER, JAC> it;
     it.a = &a;
     it.t = t;
     it.c0x = c0x;
@@ -727,7 +782,24 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         it.arcm = l;
         it.perim = l;
         it.der_arc = PI_D;
+        if (JAC) {   // the whole circle: a0 = 1 > a1 = 0 in the reference's boundary item
+            const double mm = PI_D * a.coeff;
+            it.jd[8] += mm;
+            it.jd[2] += mm * c0x;
+            it.jd[5] += mm * c0y;
+            it.jd[0] += PI_D * (double)w0;
+            it.jd[4] += PI_D * (double)w0;
+        }
     }
+    if (JAC && BALL && !full_disc && it.m == 0) {
+        // a void cell still has the reference's whole-circle item (sphere radius 0,
+        // ConvexPolyhedron2.tcc:548-551 and :440-448): measure 0, and pi w in the
+        // centroid / position entries (get_der_centroids_and_integrals_wrt_weight_and_positions.h:113-117)
+        it.jd[0] += PI_D * (double)w0;
+        it.jd[4] += PI_D * (double)w0;
+    }
+    if (JAC)
+        for (int k = 0; k < 9; ++k) a.jdiag[(int64_t)k * a.ell_stride + t] = it.jd[k];
     const double coeff = a.coeff;
     const double mass = coeff * area;
     a.mass[t] = mass;

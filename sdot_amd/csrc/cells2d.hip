@@ -35,7 +35,7 @@ using namespace c2;
 // device memory, grid-stride.  A cell that outgrows CAP goes to ovf_list for the
 // next tier; in the last tier (no ovf_list) it raises FLAG_CELL_OVERFLOW.
 // WAVES: the wavefronts per SIMD the register allocation must allow.
-template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER>
+template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER, bool JAC = false>
 __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
     __shared__ T sx[CAP * BS];
     __shared__ T sy[CAP * BS];
@@ -54,7 +54,8 @@ __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
     }
     for (; k < count; k += step) {
         const int64_t t = a.work ? a.work[k] : k;
-        if (cell2_body<T, CAP, BALL, IMG, BATCH, PER>(a, t, sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x, BS)) {
+        if (cell2_body<T, CAP, BALL, IMG, BATCH, PER, JAC>(a, t, sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x,
+                                                          BS)) {
             if (a.ovf_list) {
                 const unsigned q = atomicAdd(a.ovf_count, 1u);
                 a.ovf_list[q] = (int)t;
@@ -125,10 +126,10 @@ __global__ __launch_bounds__(BS) void k_csr_fill(const int *row_len, const int *
     }
 }
 
-template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER = false>
+template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER = false, bool JAC = false>
 void launch2(const CellArgs &a, unsigned nb, hipStream_t s) {
     if (nb == 0) return;
-    hipLaunchKernelGGL((k_cells2<T, CAP, BS, BALL, IMG, WAVES, BATCH, PER>), dim3(nb), dim3(BS), 0, s, a);
+    hipLaunchKernelGGL((k_cells2<T, CAP, BS, BALL, IMG, WAVES, BATCH, PER, JAC>), dim3(nb), dim3(BS), 0, s, a);
     HIPCK(hipGetLastError());
 }
 
@@ -178,7 +179,17 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
         ++tier;
         return tier == 1 ? nb_all : std::min(nb_all, 4096u);
     };
-    if (mixed) {
+    if (a.jdiag) {
+        // position / weight derivatives: one FP64 tier of 64 vertices, one domain shape
+        unsigned nb = next(true);
+        if (a.ntrans > 0) {
+            if (ball) launch2<double, 64, 64, true, false, 1, false, true, true>(a, nb, s);
+            else launch2<double, 64, 64, false, false, 1, false, true, true>(a, nb, s);
+        } else {
+            if (ball) launch2<double, 64, 64, true, false, 1, false, false, true>(a, nb, s);
+            else launch2<double, 64, 64, false, false, 1, false, false, true>(a, nb, s);
+        }
+    } else if (mixed) {
         unsigned nb = next(false);
         dispatch2<float, 16, 3>(ball, img, a, nb, s);
         nb = next(true);
@@ -207,6 +218,127 @@ __global__ void k_export_centroids(const double *spos, const double *mass, const
     for (int d = 0; d < dim; ++d) out[dim * i + d] = m != 0 ? spos[dim * t + d] + mom[dim * t + d] / m : 0.0;
 }
 }  // namespace
+
+namespace {
+// Export of get_der_centroids_and_integrals_wrt_weight_and_positions
+// (get_der_centroids_and_integrals_wrt_weight_and_positions.h:167-226): per sorted row t
+// (dirac i = perm[t]) the items -- neighbour slots and the diagonal -- sorted by dirac
+// column, duplicates (images of one dirac) merged.
+constexpr int JMAX = 65;   // 64 ELL slots + the diagonal
+
+__device__ int jac_items(const int *row_len, const int *ell_col, int64_t stride, const int *perm, int64_t t, int *c,
+                         int *slot) {
+    const int L = row_len[t];
+    int m = 0;
+    c[m] = perm[t];
+    slot[m++] = -1;
+    for (int e = 0; e < L && m < JMAX; ++e) {
+        c[m] = perm[ell_col[e * stride + t]];
+        slot[m++] = e;
+    }
+    for (int a = 1; a < m; ++a)
+        for (int b = a; b > 0 && c[b - 1] > c[b]; --b) {
+            const int tc = c[b], ts = slot[b];
+            c[b] = c[b - 1];
+            slot[b] = slot[b - 1];
+            c[b - 1] = tc;
+            slot[b - 1] = ts;
+        }
+    return m;
+}
+
+// cnt[i] = 9 x the number of distinct columns of dirac i
+__global__ void k_jac_count(const int *row_len, const int *ell_col, int64_t stride, const int *perm, int64_t n,
+                            int64_t *cnt) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int c[JMAX], slot[JMAX];
+    const int m = jac_items(row_len, ell_col, stride, perm, t, c, slot);
+    int u = 0;
+    for (int a = 0; a < m; ++a) u += a == 0 || c[a] != c[a - 1];
+    cnt[perm[t]] = 9 * (int64_t)u;
+}
+
+// rows 3 i + d (centroid x, y, mass), columns 3 m + e (position x, y, weight); the
+// centroid rows get the mass / centroid correction (:172-177: (b - b_mass * centroid) /
+// (mass + (mass == 1)), the reference's own divisor)
+__global__ void k_jac_fill(const int *row_len, const int *ell_col, const double *jblk, const double *jdiag, int64_t stride,
+                           const double *spos, const double *mass, const double *mom, const int *perm, int64_t n,
+                           const int64_t *base, int64_t *cols, double *vals) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int c[JMAX], slot[JMAX];
+    const int m = jac_items(row_len, ell_col, stride, perm, t, c, slot);
+    int u = 0;
+    for (int a = 0; a < m; ++a) u += a == 0 || c[a] != c[a - 1];
+    const double ms = mass[t];
+    const double v[2] = {ms != 0 ? spos[2 * t] + mom[2 * t] / ms : 0.0, ms != 0 ? spos[2 * t + 1] + mom[2 * t + 1] / ms : 0.0};
+    const double den = ms + (ms == 1 ? 1.0 : 0.0);
+    const int64_t b0 = base[perm[t]];
+    int k = 0;
+    for (int a = 0; a < m;) {
+        const int col = c[a];
+        double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (; a < m && c[a] == col; ++a)
+            for (int q = 0; q < 9; ++q)
+                B[q] += slot[a] < 0 ? jdiag[q * stride + t] : jblk[((int64_t)slot[a] * 9 + q) * stride + t];
+        for (int r = 0; r < 2; ++r)
+            for (int cc = 0; cc < 3; ++cc) B[3 * r + cc] = (B[3 * r + cc] - B[6 + cc] * v[r]) / den;
+        for (int d = 0; d < 3; ++d) {
+            const int64_t p = b0 + (int64_t)d * 3 * u + 3 * k;
+            for (int e = 0; e < 3; ++e) {
+                cols[p + e] = 3 * (int64_t)col + e;
+                vals[p + e] = B[3 * d + e];
+            }
+        }
+        ++k;
+    }
+}
+}  // namespace
+
+void export_jac(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, double *h_v, int64_t cap, int64_t *nnz) {
+    const int64_t N = pd->N;
+    hipStream_t s = pd->stream;
+    pd->csr_cnt.ensure(N);
+    pd->csr_off.ensure(N + 1);
+    pd->csr_tmp.ensure(scan_tmp_size(N));
+    const unsigned nb = (unsigned)((N + 127) / 128);
+    hipLaunchKernelGGL(k_jac_count, dim3(nb), dim3(128), 0, s, pd->row_len.p, pd->ell_col.p, (int64_t)N, pd->perm.p, N,
+                       pd->csr_cnt.p);
+    HIPCK(hipGetLastError());
+    exclusive_scan<int64_t>(pd->csr_cnt.p, pd->csr_off.p, N, pd->csr_tmp.p, s);
+    std::vector<int64_t> base(N + 1);
+    HIPCK(hipMemcpyAsync(base.data(), pd->csr_off.p, (N + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    const int64_t total = base[N];
+    *nnz = total;
+    if (h_v) {
+        std::vector<double> m(N), c(2 * N);
+        export_vec(pd, pd->mass.p, m.data());
+        export_centroids(pd, c.data());
+        for (int64_t i = 0; i < N; ++i) {
+            h_v[3 * i] = c[2 * i];
+            h_v[3 * i + 1] = c[2 * i + 1];
+            h_v[3 * i + 2] = m[i];
+        }
+    }
+    if (!h_col) return;
+    if (total > cap) throw std::string("Jacobian capacity too small");
+    for (int64_t i = 0; i < N; ++i) {
+        const int64_t row = (base[i + 1] - base[i]) / 3;
+        for (int d = 0; d < 3; ++d) h_off[3 * i + d] = base[i] + d * row;
+    }
+    h_off[3 * N] = total;
+    pd->csr_col.ensure(total > 0 ? total : 1);
+    pd->csr_val.ensure(total > 0 ? total : 1);
+    hipLaunchKernelGGL(k_jac_fill, dim3(nb), dim3(128), 0, s, pd->row_len.p, pd->ell_col.p, pd->jblk.p, pd->jdiag.p,
+                       (int64_t)N, pd->spos.p, pd->mass.p, pd->mom.p, pd->perm.p, N, pd->csr_off.p, pd->csr_col.p,
+                       pd->csr_val.p);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(h_col, pd->csr_col.p, total * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(h_val, pd->csr_val.p, total * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+}
 
 void export_centroids(sdot_pd *pd, double *h_out) {
     const int64_t N = pd->N;

@@ -195,6 +195,9 @@ struct CellArgs {
     // the cut id j + (s + 1) N (single problems only; N (ntrans + 1) < 2^31)
     int ntrans;
     const double *trans;
+    // position / weight derivative blocks (9 doubles): per neighbour slot e at
+    // jblk[(e * 9 + k) * ell_stride + t], the diagonal at jdiag[k * ell_stride + t]
+    double *jblk, *jdiag;
     const double *img;
     int img_n[3];
     int img_nc;   // coefficients per pixel: 1, or 6 (2D quadratic, coefficient-major)
@@ -358,6 +361,8 @@ struct sdot_pd {
     sdot_amd::DevBuf<double> it_box, it_coeff, it_planes;   // domain items on the device
     std::vector<double> htrans;                             // translations (dim per vector)
     sdot_amd::DevBuf<double> trans;
+    bool want_jac = false;                                  // next pass writes the derivative blocks
+    sdot_amd::DevBuf<double> jblk, jdiag;
     sdot_amd::DevBuf<int> it_off;
     bool want_mom = false;          // the next pass computes the first moments
     bool pass_has_mom = false;      // the current pass did
@@ -404,6 +409,8 @@ void run_cells2(sdot_pd *pd, CellArgs &a);
 void run_cells3(sdot_pd *pd, CellArgs &a);
 void fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
 void export_csr(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, int64_t cap, int64_t *nnz);
+// the position / weight derivative CSR of the last (Jacobian) pass, 2D
+void export_jac(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, double *h_v, int64_t cap, int64_t *nnz);
 void export_vec(sdot_pd *pd, const double *d_sorted, double *h_out);  // sorted -> dirac order, to host
 void export_centroids(sdot_pd *pd, double *h_out);   // position + moment / mass (0 when empty), dirac order
 // capi.hip

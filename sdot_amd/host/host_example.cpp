@@ -113,8 +113,25 @@ int main(int argc, char **argv) {
         csum += m;
     });
     const double cerr = std::fmax(std::fabs(cm[0] / csum - 0.5), std::fabs(cm[1] / csum - 0.5));
+    // get_der_centroids_and_integrals_wrt_weight_and_positions: the mass rows of the
+    // weight columns are the Hessian of the masses (same offsets up to the 3x layout)
+    std::vector<std::size_t> joff, jcol;
+    std::vector<double> jval, jv;
+    const int jerr = get_der_centroids_and_integrals_wrt_weight_and_positions(joff, jcol, jval, jv, grid, bounds,
+                                                                              positions.data(), weights.data(), n,
+                                                                              FunctionEnum::Unit());
+    double jdiff = 0;
+    for (std::size_t i = 0; i < n && jerr == 0; ++i) {
+        std::size_t k2 = joff[3 * i + 2];
+        for (std::size_t k = offsets[i]; k < offsets[i + 1]; ++k, k2 += 3) {   // (mass i, weight col) entries
+            jdiff = std::fmax(jdiff, std::fabs(jval[k2 + 2] - values[k]));
+            if (jcol[k2 + 2] != 3 * columns[k] + 2) jdiff = 1e300;
+        }
+    }
     std::printf("{\"status\": %d, \"der_status\": %d, \"newton_iters\": %d, \"max_rel_mass_err\": %.3e, \"mass_sum\": %.15f, "
-                "\"nnz\": %zu, \"max_row_sum\": %.3e, \"ball_mass_0\": %.6e, \"centroid_mean_err\": %.3e}\n",
-                st, err, solver.stats.newton_iters, max_rel, sum, columns.size(), max_rowsum, ball[0], cerr);
-    return (st == 0 && err == 0 && max_rel < 1e-6 && max_rowsum < 1e-9 && cerr < 1e-12) ? 0 : 1;
+                "\"nnz\": %zu, \"max_row_sum\": %.3e, \"ball_mass_0\": %.6e, \"centroid_mean_err\": %.3e, "
+                "\"der_pos_status\": %d, \"der_pos_vs_hessian\": %.3e}\n",
+                st, err, solver.stats.newton_iters, max_rel, sum, columns.size(), max_rowsum, ball[0], cerr, jerr, jdiff);
+    return (st == 0 && err == 0 && max_rel < 1e-6 && max_rowsum < 1e-9 && cerr < 1e-12 && jerr == 0 && jdiff < 1e-9) ? 0
+                                                                                                                 : 1;
 }

@@ -13,6 +13,8 @@
 //   get_integrals(res, grid, bounds, pos, w, N, rf) src/sdot/PowerDiagram/get_integrals.h:13
 //   get_centroids(grid, bounds, pos, w, N, rf, cb)  src/sdot/PowerDiagram/get_centroids.h:14
 //   get_der_integrals_wrt_weights(...) -> int       src/sdot/PowerDiagram/get_der_integrals_wrt_weights.h:16
+//   get_der_centroids_and_integrals_wrt_weight_and_positions(...) -> int
+//                         src/sdot/PowerDiagram/get_der_centroids_and_integrals_wrt_weight_and_positions.h:16
 //   OptimalTransportSolver<Grid,Bounds>::solve(...) src/sdot/Solvers/OptimalTransportSolver.h:18-26
 //
 // Differences: the grid owns a device handle (one HIP stream); failures other
@@ -423,6 +425,35 @@ int get_der_integrals_wrt_weights(std::vector<TI> &m_offsets, std::vector<TI> &m
                                   const TF *weights, std::size_t nb_diracs) {
     return get_der_integrals_wrt_weights(m_offsets, m_columns, m_values, v_values, grid, bounds, positions, weights,
                                          nb_diracs, FunctionEnum::Unit());
+}
+
+/// get_der_centroids_and_integrals_wrt_weight_and_positions
+/// (get_der_centroids_and_integrals_wrt_weight_and_positions.h:16, 2D, one domain shape):
+/// rows 3 n + d (centroid x, y, mass of dirac n), columns 3 m + e (position x, y, weight
+/// of dirac m); v_values = centroids and masses; returns 1 on an empty cell, 0 otherwise.
+template <class TI, class TF, class Grid, class Bounds, class Pt, class Func>
+int get_der_centroids_and_integrals_wrt_weight_and_positions(std::vector<TI> &m_offsets, std::vector<TI> &m_columns,
+                                                             std::vector<TF> &m_values, std::vector<TF> &v_values,
+                                                             Grid &grid, Bounds &bounds, const Pt *positions,
+                                                             const TF *weights, std::size_t nb_diracs, const Func &,
+                                                             bool stop_if_void = true) {
+    grid.update(positions, weights, nb_diracs, false, true);
+    sdot_pd *pd = grid.handle();
+    bounds.apply(pd);
+    check(pd, sdot_pd_set_ball_clip(pd, Func::ball), "set_ball_clip");
+    std::int64_t nnz = 0;
+    int st = sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions(pd, nullptr, nullptr, nullptr, nullptr, 0, &nnz);
+    check(pd, st, "get_der_centroids_and_integrals_wrt_weight_and_positions", true);
+    std::vector<std::int64_t> off(3 * nb_diracs + 1), col(nnz > 0 ? nnz : 1);
+    m_values.resize(nnz > 0 ? nnz : 1);
+    v_values.resize(3 * nb_diracs);
+    st = sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions(pd, off.data(), col.data(), m_values.data(),
+                                                                      v_values.data(), nnz, &nnz);
+    check(pd, st, "get_der_centroids_and_integrals_wrt_weight_and_positions", true);
+    m_offsets.assign(off.begin(), off.end());
+    m_columns.assign(col.begin(), col.begin() + nnz);
+    m_values.resize(nnz);
+    return (st == SDOT_EMPTY_CELL && stop_if_void) ? 1 : 0;
 }
 
 /// OptimalTransportSolver (OptimalTransportSolver.h:18-26): Newton on the weights with

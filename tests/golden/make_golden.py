@@ -111,7 +111,20 @@ def known_values():
              ball=[0, 0, 10], centroid=[0.0, 0.0], tol=1e-6),
         dict(src=kv + ":322-336 box", box=[0, 0, 2, 1], cuts=[], ball=None, centroid=[1.0, 0.5], tol=1e-5),
     ]
-    return dict(cp2=cases, cp3=cp3, scaled_image=img, cp2_centroid=cen)
+    # tests/test_der_pos.cpp:118-135: the 36 diracs (6 x 6 lattice of pitch 0.08 from
+    # 0.05) and their weights on which the reference checks
+    # get_der_centroids_and_integrals_wrt_weight_and_positions against finite
+    # differences, with InBallW05 on the unit box
+    g = [0.05, 0.13, 0.21, 0.29, 0.37, 0.45]
+    der_pos = dict(src="tests/test_der_pos.cpp:118-135", box=[0, 0, 1, 1], ball=True,
+                   positions=[[x, y] for y in g for x in g],
+                   weights=[0.00314773, 0.00393608, 0.00428933, 0.0042071, 0.00371394, 0.00284918,
+                            0.00393608, 0.0048636, 0.00525743, 0.00514763, 0.00452475, 0.00332225,
+                            0.00428933, 0.00525743, 0.00566234, 0.0055339, 0.00484392, 0.0034862,
+                            0.0042071, 0.00514763, 0.0055339, 0.00541476, 0.00476184, 0.00345417,
+                            0.00371394, 0.00452475, 0.00484392, 0.00476184, 0.00426395, 0.00322731,
+                            0.00284918, 0.00332225, 0.0034862, 0.00345417, 0.00322731, 0.00270565])
+    return dict(cp2=cases, cp3=cp3, scaled_image=img, cp2_centroid=cen, der_pos=der_pos)
 
 
 def rng_first8():

@@ -23,6 +23,8 @@ def test_host_example_solves_and_exports_csr():
     assert abs(r["mass_sum"] - 1) < 1e-12
     # get_centroids: the mass-weighted mean centroid is the centre of the unit square
     assert r["centroid_mean_err"] < 1e-12
+    # the weight columns of the mass rows of the position Jacobian equal the Hessian
+    assert r["der_pos_status"] == 0 and r["der_pos_vs_hessian"] < 1e-9
 
 
 def test_host_example_assembly_of_two_shapes():

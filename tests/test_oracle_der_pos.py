@@ -1,0 +1,104 @@
+"""The oracle's get_der_centroids_and_integrals_wrt_weight_and_positions (2D), pinned by
+finite differences of its own centroids and masses (the check the reference's
+tests/test_der_pos.cpp makes, on its 36-dirac ball configuration from the golden
+fixtures and on random diracs, with and without the ball and with periodic images) and
+by the closed form of a lone disc (d centroid / d position = identity, d mass / d w = pi,
+everything else 0)."""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+T2 = [(x, y) for x in (-1, 0, 1) for y in (-1, 0, 1) if x or y]
+
+
+def dense(off, cols, vals, n):
+    return sp.csr_matrix((vals, cols, off), shape=(3 * n, 3 * n)).toarray()
+
+
+def finite_differences(pos, w, h, **kw):
+    n = len(w)
+
+    def state(p, ww):
+        st, c, m = O.centroids(p, ww, **kw)
+        return np.column_stack([c, m]).ravel()
+
+    J = np.zeros((3 * n, 3 * n))
+    for m in range(n):
+        for e in range(3):
+            pp, wp, pm, wm = pos.copy(), w.copy(), pos.copy(), w.copy()
+            if e < 2:
+                pp[m, e] += h
+                pm[m, e] -= h
+            else:
+                wp[m] += h
+                wm[m] -= h
+            J[:, 3 * m + e] = (state(pp, wp) - state(pm, wm)) / (2 * h)
+    return J
+
+
+def test_lone_disc_closed_form():
+    st, off, cols, vals, v = O.der_pos(np.array([[0.4, 0.55]]), np.array([0.01]), ball=True)
+    assert st == O.OR_OK
+    J = dense(off, cols, vals, 1)
+    assert np.abs(J - np.diag([1.0, 1.0, np.pi])).max() < 1e-14
+    assert np.abs(v - [0.4, 0.55, 0.01 * np.pi]).max() < 1e-15
+
+
+def test_reference_der_pos_configuration():
+    ref = json.load(open(os.path.join(HERE, "golden", "reference_known_values.json")))["der_pos"]
+    pos = np.array(ref["positions"])
+    w = np.array(ref["weights"])
+    st, off, cols, vals, v = O.der_pos(pos, w, ball=True)
+    assert st == O.OR_OK
+    J = dense(off, cols, vals, len(w))
+    F = finite_differences(pos, w, 1e-7, ball=True)
+    assert np.abs(J - F).max() < 1e-6 * np.abs(F).max()
+    st2, c, m = O.centroids(pos, w, ball=True)
+    assert np.abs(v - np.column_stack([c, m]).ravel()).max() < 1e-15
+
+
+@pytest.mark.parametrize("case", ["box", "ball", "periodic"])
+def test_random_diracs_against_finite_differences(case):
+    n = 14
+    u = O.rng_uniform(111, 3 * n)
+    pos = u[: 2 * n].reshape(n, 2).copy()
+    kw = {}
+    if case == "ball":
+        w = 0.02 + 0.01 * u[2 * n:]
+        kw["ball"] = True
+    else:
+        w = 0.01 * u[2 * n:]
+    if case == "periodic":
+        kw["translations"] = T2
+    st, off, cols, vals, v = O.der_pos(pos, w, **kw)
+    assert st == O.OR_OK
+    J = dense(off, cols, vals, n)
+    F = finite_differences(pos, w, 1e-7, **kw)
+    assert np.abs(J - F).max() < 1e-6 * np.abs(F).max()
+    # the layout: rows 3 n + d sorted, diagonal block present
+    for r in range(3 * n):
+        c = cols[off[r]:off[r + 1]]
+        assert (np.diff(c) > 0).all() and 3 * (r // 3) in c
+
+
+def test_reference_divisor_for_a_cell_of_mass_one():
+    """The reference divides the centroid rows by mass + (mass == 1)
+    (get_der_centroids_and_integrals_wrt_weight_and_positions.h:177): two diracs splitting
+    the box [0, 2] x [0, 1] into cells of mass exactly 1 get half the finite-difference
+    centroid rows, the mass rows unchanged."""
+    pos = np.array([[0.5, 0.5], [1.5, 0.5]])
+    w = np.zeros(2)
+    st, off, cols, vals, v = O.der_pos(pos, w, domain=([0, 0], [2, 1]))
+    assert st == O.OR_OK and np.array_equal(v[2::3], [1.0, 1.0])
+    J = dense(off, cols, vals, 2)
+    F = finite_differences(pos, w, 1e-7, domain=([0, 0], [2, 1]))
+    rows_c = [0, 1, 3, 4]
+    assert np.abs(J[rows_c] - 0.5 * F[rows_c]).max() < 1e-7
+    assert np.abs(J[[2, 5]] - F[[2, 5]]).max() < 1e-7
+    assert np.abs(F[rows_c]).max() > 0.1
