@@ -111,11 +111,11 @@ sdot_status sdot_pd_compute(sdot_pd *pd);
 /* get_integrals(res, ...) -- get_integrals.h:13.  Runs the pass if needed. */
 sdot_status sdot_pd_integrals(sdot_pd *pd, double *masses);
 /* get_der_centroids_and_integrals_wrt_weight_and_positions(m_offsets, m_columns, m_values,
- * v_values, ...) -- get_der_centroids_and_integrals_wrt_weight_and_positions.h:16 (2D, one
- * domain shape, with or without the ball and translations).  Rows 3 n + d (d = 0, 1:
- * centroid x, y; d = 2: mass of dirac n), columns 3 m + e (e = 0, 1: position x, y;
- * e = 2: weight of dirac m), sorted; offsets[3N + 1]; v_values[3N] = centroids and
- * masses.  As in the reference, the centroid rows are divided by mass + (mass == 1), and
+ * v_values, ...) -- get_der_centroids_and_integrals_wrt_weight_and_positions.h:16 (one
+ * domain shape, with translations, with the ball in 2D).  With nu = dim + 1: rows
+ * nu n + d (d < dim: centroid coordinate d; d = dim: mass of dirac n), columns nu m + e
+ * (e < dim: position coordinate e; e = dim: weight of dirac m), sorted;
+ * offsets[nu N + 1]; v_values[nu N] = centroids and masses.  As in the reference, the centroid rows are divided by mass + (mass == 1), and
  * the first moments of the faces are not scaled by the density coefficient (exact for
  * density 1).  Two-phase like sdot_pd_der_integrals_wrt_weights (cols = NULL: nnz only).
  * Runs its own pass (FP64 geometry in either precision). */

@@ -68,9 +68,9 @@ def lib():
         L.or_centroids.restype = ctypes.c_int
         L.or_centroids.argtypes = [ctypes.c_int, _i64, _p, _p, ctypes.POINTER(Domain), ctypes.c_int, ctypes.c_int,
                                    _p, _p]
-        L.or_der_pos2.restype = ctypes.c_int
-        L.or_der_pos2.argtypes = [_i64, _p, _p, ctypes.POINTER(Domain), ctypes.c_int, ctypes.c_int, _i64, _p, _p, _p,
-                                  _p]
+        L.or_der_pos.restype = ctypes.c_int
+        L.or_der_pos.argtypes = [ctypes.c_int, _i64, _p, _p, ctypes.POINTER(Domain), ctypes.c_int, ctypes.c_int, _i64,
+                                 _p, _p, _p, _p]
         L.or_rng_uniform.restype = None
         L.or_rng_uniform.argtypes = [ctypes.c_uint64, _i64, _p]
         _LIB = L
@@ -287,33 +287,36 @@ def cells_sample_seconds(pos, w, i_begin, i_end, ball=False, nthreads=None, img=
     return dt, st, masses
 
 
-def der_pos(pos, w, ball=False, nthreads=None, domain=None, items=None, translations=None, cap=64):
-    """get_der_centroids_and_integrals_wrt_weight_and_positions (2D): returns (status,
-    offsets, columns, values, v) in the reference's layout -- row 3 n + d (centroid x, y,
-    mass of dirac n), column 3 m + e (position x, y, weight of dirac m); v[3 n + d]."""
+def der_pos(pos, w, ball=False, nthreads=None, domain=None, items=None, translations=None, cap=None):
+    """get_der_centroids_and_integrals_wrt_weight_and_positions: returns (status, offsets,
+    columns, values, v) in the reference's layout -- row nu n + d (centroid coordinates,
+    then the mass of dirac n), column nu m + e (position coordinates, then the weight of
+    dirac m), nu = dim + 1; v[nu n + d]."""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
-    N = pos.shape[0]
+    N, dim = pos.shape
+    nu = dim + 1
+    cap = cap or (64 if dim == 2 else 160)
     w = np.ascontiguousarray(w, dtype=np.float64)
     if domain is None:
-        dom, keep = make_domain(2, items=items, translations=translations)
+        dom, keep = make_domain(dim, items=items, translations=translations)
     else:
-        dom, keep = make_domain(2, *domain, items=items, translations=translations)
+        dom, keep = make_domain(dim, *domain, items=items, translations=translations)
     jlen = np.zeros(N, dtype=np.int64)
     jcols = np.zeros(N * cap, dtype=np.int64)
-    jblk = np.zeros(N * cap * 9)
-    jv = np.zeros(3 * N)
-    st = lib().or_der_pos2(N, _ptr(pos), _ptr(w), ctypes.byref(dom), 1 if ball else 0, nthreads or os.cpu_count() or 1,
-                           cap, _ptr(jlen), _ptr(jcols), _ptr(jblk), _ptr(jv))
+    jblk = np.zeros(N * cap * 16)
+    jv = np.zeros(nu * N)
+    st = lib().or_der_pos(dim, N, _ptr(pos), _ptr(w), ctypes.byref(dom), 1 if ball else 0,
+                          nthreads or os.cpu_count() or 1, cap, _ptr(jlen), _ptr(jcols), _ptr(jblk), _ptr(jv))
     del keep
     offsets = [0]
     cols, vals = [], []
     for n in range(N):
         L = int(jlen[n])
         c = jcols[n * cap:n * cap + L]
-        b = jblk[n * cap * 9:(n * cap + L) * 9].reshape(L, 3, 3)
-        for d in range(3):
+        b = jblk[n * cap * 16:(n * cap + L) * 16].reshape(L, 16)[:, :nu * nu].reshape(L, nu, nu)
+        for d in range(nu):
             for k in range(L):
-                cols.extend(3 * int(c[k]) + e for e in range(3))
+                cols.extend(nu * int(c[k]) + e for e in range(nu))
                 vals.extend(b[k, d, :])
             offsets.append(len(cols))
     return st, np.array(offsets, dtype=np.int64), np.array(cols, dtype=np.int64), np.array(vals), jv

@@ -900,6 +900,45 @@ static int build_cell2(const cells_job *J, int64_t i, or_cp2 *cp) {
     return or_cp2_empty(cp) ? OR_EMPTY_CELL : OR_OK;
 }
 
+/* sort a cell's (column, block) items by column, merge duplicates, apply the mass /
+ * centroid correction to the centroid rows (dim rows of nu = dim + 1 entries), store */
+static void jac_finish(cells_job *J, int64_t i, int64_t m, int dim, double mass, const double *cent, int overflow) {
+    const int nu = dim + 1, B = nu * nu;
+    const int64_t cap = J->cap;
+    int64_t *cols = J->jcols + i * cap;
+    double *blk = J->jblocks + i * cap * 16;
+    double tb[16];
+    for (int64_t a = 1; a < m; ++a)
+        for (int64_t b2 = a; b2 > 0 && cols[b2 - 1] > cols[b2]; --b2) {
+            int64_t tc = cols[b2]; cols[b2] = cols[b2 - 1]; cols[b2 - 1] = tc;
+            memcpy(tb, blk + 16 * b2, sizeof tb);
+            memcpy(blk + 16 * b2, blk + 16 * (b2 - 1), sizeof tb);
+            memcpy(blk + 16 * (b2 - 1), tb, sizeof tb);
+        }
+    int64_t w = 0;
+    for (int64_t a = 0; a < m; ++a) {
+        if (w > 0 && cols[w - 1] == cols[a]) {
+            for (int k = 0; k < B; ++k) blk[16 * (w - 1) + k] += blk[16 * a + k];
+            continue;
+        }
+        if (w != a) { cols[w] = cols[a]; memcpy(blk + 16 * w, blk + 16 * a, 16 * sizeof(double)); }
+        ++w;
+    }
+    double *v = J->jv + nu * i;
+    for (int d = 0; d < dim; ++d) v[d] = cent[d];
+    v[dim] = mass;
+    for (int64_t a = 0; a < w; ++a)
+        for (int r = 0; r < dim; ++r)
+            for (int c = 0; c < nu; ++c)
+                blk[16 * a + nu * r + c] = (blk[16 * a + nu * r + c] - blk[16 * a + nu * dim + c] * v[r]) / (mass + (mass == 1));
+    J->jlen[i] = w;
+    if (overflow) {
+        pthread_mutex_lock(&J->mtx);
+        J->status = OR_CAPACITY;
+        pthread_mutex_unlock(&J->mtx);
+    }
+}
+
 /* get_der_centroids_and_integrals_wrt_weight_and_positions.h:46-190 for one 2D cell
  * integrated as one piece of density coeff: the boundary items of for_each_boundary_item
  * (ConvexPolyhedron2.tcc:437-477: straight edges carry pos_integral = l (p0 + p1) / 2 and
@@ -909,7 +948,7 @@ static int build_cell2(const cells_job *J, int64_t i, or_cp2 *cp) {
 static void jac2_cell(cells_job *J, int64_t i, const or_cp2 *cp, double coeff, double mass, const double *cent) {
     const int64_t cap = J->cap;
     int64_t *cols = J->jcols + i * cap;
-    double *blk = J->jblocks + i * cap * 9;
+    double *blk = J->jblocks + i * cap * 16;
     int64_t m = 0;
     double der0[9] = {0};
     const double c0[2] = {J->pos[2 * i], J->pos[2 * i + 1]};
@@ -995,50 +1034,16 @@ static void jac2_cell(cells_job *J, int64_t i, const or_cp2 *cp, double coeff, d
             }
         if (m >= cap) { overflow = 1; continue; }
         cols[m] = j;
-        memcpy(blk + 9 * m, d1, sizeof d1);
+        memcpy(blk + 16 * m, d1, sizeof d1);
         ++m;
     }
     if (m >= cap) overflow = 1;
     else {
         cols[m] = i;
-        memcpy(blk + 9 * m, der0, sizeof der0);
+        memcpy(blk + 16 * m, der0, sizeof der0);
         ++m;
     }
-    /* sort by column, merge duplicates (:170-186) */
-    for (int64_t a = 1; a < m; ++a)
-        for (int64_t b2 = a; b2 > 0 && cols[b2 - 1] > cols[b2]; --b2) {
-            int64_t tc = cols[b2]; cols[b2] = cols[b2 - 1]; cols[b2 - 1] = tc;
-            double tb[9];
-            memcpy(tb, blk + 9 * b2, sizeof tb);
-            memcpy(blk + 9 * b2, blk + 9 * (b2 - 1), sizeof tb);
-            memcpy(blk + 9 * (b2 - 1), tb, sizeof tb);
-        }
-    int64_t w = 0;
-    for (int64_t a = 0; a < m; ++a) {
-        if (w > 0 && cols[w - 1] == cols[a]) {
-            for (int k = 0; k < 9; ++k) blk[9 * (w - 1) + k] += blk[9 * a + k];
-            continue;
-        }
-        if (w != a) { cols[w] = cols[a]; memcpy(blk + 9 * w, blk + 9 * a, 9 * sizeof(double)); }
-        ++w;
-    }
-    m = w;
-    /* v and the mass / centroid correction of the centroid rows (:172-177; the
-     * reference divides by mass + (mass == 1)) */
-    double *v = J->jv + 3 * i;
-    v[0] = cent[0];
-    v[1] = cent[1];
-    v[2] = mass;
-    for (int64_t a = 0; a < m; ++a)
-        for (int r = 0; r < 2; ++r)
-            for (int c = 0; c < 3; ++c)
-                blk[9 * a + 3 * r + c] = (blk[9 * a + 3 * r + c] - blk[9 * a + 6 + c] * v[r]) / (mass + (mass == 1));
-    J->jlen[i] = m;
-    if (overflow) {
-        pthread_mutex_lock(&J->mtx);
-        J->status = OR_CAPACITY;
-        pthread_mutex_unlock(&J->mtx);
-    }
+    jac_finish(J, i, m, 2, mass, cent, overflow);
 }
 
 static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
@@ -1248,6 +1253,70 @@ static void piece3_accumulate(const cells_job *J, int64_t i, const or_cp3 *cp, d
     }
 }
 
+/* get_der_centroids_and_integrals_wrt_weight_and_positions.h:124-166 for a 3D cell of
+ * one piece (no ball: TODO in the reference): the faces' boundary items of
+ * ConvexPolyhedron3.tcc:226-265 -- pos_integral and momentum over the fan of triangles
+ * (v0, vk, vk+1) of each face, not scaled by the density, measure = coeff * area */
+static void jac3_cell(cells_job *J, int64_t i, const or_cp3 *cp, double coeff, double mass, const double *cent) {
+    const int64_t cap = J->cap;
+    int64_t *cols = J->jcols + i * cap;
+    double *blk = J->jblocks + i * cap * 16;
+    int64_t m = 0;
+    double der0[16] = {0};
+    const double *c0 = J->pos + 3 * i;
+    int overflow = 0;
+    for (int f = 0; f < cp->nf; ++f) {
+        if (cp->nv[f] < 3) continue;
+        int64_t id = cp->id[f];
+        if (id < 0) continue;
+        const double (*v)[3] = cp->v[f];
+        double C[3] = {0, 0, 0}, M[3][3] = {{0}}, area = 0;
+        for (int k = 2; k < cp->nv[f]; ++k) {
+            const double *A = v[0], *Bp = v[k - 1], *Cp = v[k];
+            double e1[3] = {Bp[0] - A[0], Bp[1] - A[1], Bp[2] - A[2]}, e2[3] = {Cp[0] - A[0], Cp[1] - A[1], Cp[2] - A[2]};
+            double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+            double ta = 0.5 * (cr[0] * cp->n[f][0] + cr[1] * cp->n[f][1] + cr[2] * cp->n[f][2]);
+            area += ta;
+            for (int a = 0; a < 3; ++a) C[a] += ta * (A[a] + Bp[a] + Cp[a]) / 3;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    M[a][b] += ta / 12 * (A[a] * (2 * A[b] + Bp[b] + Cp[b]) + Bp[a] * (A[b] + 2 * Bp[b] + Cp[b]) +
+                                          Cp[a] * (A[b] + Bp[b] + 2 * Cp[b]));
+        }
+        double q[3];
+        int64_t j = cut_dirac(J, id, q);
+        double dist = sqrt((q[0] - c0[0]) * (q[0] - c0[0]) + (q[1] - c0[1]) * (q[1] - c0[1]) + (q[2] - c0[2]) * (q[2] - c0[2]));
+        double measure = coeff * area, b = 0.5 * measure / dist, d1[16] = {0};
+        der0[15] += b;
+        d1[15] = -b;
+        for (int e = 0; e < 3; ++e) {
+            double val = 0.5 * C[e] / dist;
+            der0[4 * e + 3] += val;
+            d1[4 * e + 3] = -val;
+        }
+        for (int d = 0; d < 3; ++d) {
+            der0[12 + d] += (C[d] - measure * c0[d]) / dist;
+            d1[12 + d] = (measure * q[d] - C[d]) / dist;
+        }
+        for (int d = 0; d < 3; ++d)
+            for (int e = 0; e < 3; ++e) {
+                der0[4 * e + d] += (M[d][e] - C[e] * c0[d]) / dist;
+                d1[4 * e + d] = (C[e] * q[d] - M[d][e]) / dist;
+            }
+        if (m >= cap) { overflow = 1; continue; }
+        cols[m] = j;
+        memcpy(blk + 16 * m, d1, sizeof d1);
+        ++m;
+    }
+    if (m >= cap) overflow = 1;
+    else {
+        cols[m] = i;
+        memcpy(blk + 16 * m, der0, sizeof der0);
+        ++m;
+    }
+    jac_finish(J, i, m, 3, mass, cent, overflow);
+}
+
 static void eval_cell3(cells_job *J, int64_t i, or_cp3 *cp, or_cp3 *piece) {
     row_acc row = { J->cols + i * J->cap, J->vals + i * J->cap, 1, J->cap, 0.0, 0 };
     row.cols[0] = i;
@@ -1304,6 +1373,11 @@ static void eval_cell3(cells_job *J, int64_t i, or_cp3 *cp, or_cp3 *piece) {
     J->masses[i] = mass;
     if (J->centroids) {
         for (int d = 0; d < 3; ++d) J->centroids[3 * i + d] = mass != 0 ? J->pos[3 * i + d] + mom[d] / mass : 0.0;
+    }
+    if (J->jblocks) {     /* one shape only (checked by or_der_pos) */
+        double cent[3];
+        for (int d = 0; d < 3; ++d) cent[d] = mass != 0 ? J->pos[3 * i + d] + mom[d] / mass : 0.0;
+        jac3_cell(J, i, cp, dom->nitems == 1 ? dom->it_coeff[0] : dom->coeff, mass, cent);
     }
     if (J->ball_arc) J->ball_arc[i] = 0;
     if (J->perimeter) J->perimeter[i] = perim;
@@ -1387,15 +1461,15 @@ static int cells_run(int dim, int64_t N, const double *pos, const double *w, con
                        cap, row_len, cols, vals, NULL, NULL, NULL, NULL);
 }
 
-int or_der_pos2(int64_t N, const double *pos, const double *w, const or_domain *dom, int flags, int nthreads,
-                int64_t cap, int64_t *jlen, int64_t *jcols, double *jblocks, double *jv) {
-    if (!dom || dom->dim != 2 || dom->img || dom->nitems >= 2 || !jlen || !jcols || !jblocks || !jv) return OR_INVALID;
+int or_der_pos(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom, int flags, int nthreads,
+               int64_t cap, int64_t *jlen, int64_t *jcols, double *jblocks, double *jv) {
+    if (!dom || dom->dim != dim || dom->img || dom->nitems >= 2 || !jlen || !jcols || !jblocks || !jv) return OR_INVALID;
     double *masses = (double *)malloc(sizeof(double) * (N + 1));
-    double *cent = (double *)malloc(sizeof(double) * (2 * N + 1));
+    double *cent = (double *)malloc(sizeof(double) * (dim * N + 1));
     int64_t *row_len = (int64_t *)malloc(sizeof(int64_t) * (N + 1));
     int64_t *cols = (int64_t *)malloc(sizeof(int64_t) * (N * cap + 1));
     double *vals = (double *)malloc(sizeof(double) * (N * cap + 1));
-    int st = cells_run_j(2, N, pos, w, dom, flags, nthreads, 0, N, masses, NULL, NULL, cent, cap, row_len, cols, vals,
+    int st = cells_run_j(dim, N, pos, w, dom, flags, nthreads, 0, N, masses, NULL, NULL, cent, cap, row_len, cols, vals,
                          jlen, jcols, jblocks, jv);
     free(masses); free(cent); free(row_len); free(cols); free(vals);
     return st;

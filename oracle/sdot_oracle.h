@@ -167,13 +167,14 @@ int64_t or_rows_to_csr(int64_t N, int64_t cap, const int64_t *row_len, const int
 int or_pcg(int64_t N, const int64_t *offsets, const int64_t *cols, const double *vals,
            const double *b, double *x, double rtol, int max_iter, int *iters);
 
-/* get_der_centroids_and_integrals_wrt_weight_and_positions (2D, one domain shape: a box
- * or one item; optional ball (flags & 1) and translations).  Per dirac i: jlen[i] items,
- * columns jcols[i*cap ..] ascending (diagonal included, duplicates merged), 3x3 blocks
- * jblocks[(i*cap + k)*9 + 3d + e] for output d (centroid x, y, mass) and unknown e
- * (position x, y, weight); jv[3i..3i+2] = centroid, mass. */
-int or_der_pos2(int64_t N, const double *pos, const double *w, const or_domain *dom, int flags, int nthreads,
-                int64_t cap, int64_t *jlen, int64_t *jcols, double *jblocks, double *jv);
+/* get_der_centroids_and_integrals_wrt_weight_and_positions (one domain shape: a box or
+ * one item; translations; the ball (flags & 1) in 2D only, TODO in the reference in
+ * 3D).  Per dirac i: jlen[i] items, columns jcols[i*cap ..] ascending (diagonal
+ * included, duplicates merged), blocks jblocks[(i*cap + k)*16 + nu*d + e], nu = dim + 1,
+ * for output d (centroid coordinates, mass at d = dim) and unknown e (position
+ * coordinates, weight at e = dim); jv[nu*i ..] = centroid, mass. */
+int or_der_pos(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom, int flags, int nthreads,
+               int64_t cap, int64_t *jlen, int64_t *jcols, double *jblocks, double *jv);
 
 typedef struct or_solve_stats {
     int newton_iters;

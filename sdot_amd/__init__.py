@@ -300,16 +300,18 @@ class PowerDiagram:
         return c, m
 
     def der_centroids_and_integrals_wrt_weight_and_positions(self, allow_empty=True):
-        """get_der_centroids_and_integrals_wrt_weight_and_positions (2D): (offsets, cols, vals, v)
-        with rows 3 n + d (centroid x, y, mass) and columns 3 m + e (position x, y, weight)."""
+        """get_der_centroids_and_integrals_wrt_weight_and_positions: (offsets, cols, vals, v) with
+        rows nu n + d (centroid coordinates, mass) and columns nu m + e (position
+        coordinates, weight), nu = dim + 1."""
+        nu = self.dim + 1
         nnz = _i64(0)
         f = lib().sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions
         self._check(f(self._h, None, None, None, None, 0, ctypes.byref(nnz)), "der_centroids_and_integrals",
                     allow_empty)
-        off = np.zeros(3 * self.N + 1, dtype=np.int64)
+        off = np.zeros(nu * self.N + 1, dtype=np.int64)
         cols = np.zeros(max(nnz.value, 1), dtype=np.int64)
         vals = np.zeros(max(nnz.value, 1))
-        v = np.zeros(3 * self.N)
+        v = np.zeros(nu * self.N)
         self._check(f(self._h, _ptr(off), _ptr(cols), _ptr(vals), _ptr(v), nnz.value, ctypes.byref(nnz)),
                     "der_centroids_and_integrals", allow_empty)
         return off, cols[:nnz.value], vals[:nnz.value], v

@@ -261,8 +261,9 @@ sdot_status compute_pass(sdot_pd *pd) {
     pd->pass_has_mom = pd->want_mom;
     a.jblk = a.jdiag = nullptr;
     if (pd->want_jac) {
-        pd->jblk.ensure((size_t)9 * pd->ell_cap * N);
-        pd->jdiag.ensure((size_t)9 * N);
+        const size_t nb = (size_t)(pd->dim + 1) * (pd->dim + 1);
+        pd->jblk.ensure(nb * pd->ell_cap * N);
+        pd->jdiag.ensure(nb * N);
         a.jblk = pd->jblk.p;
         a.jdiag = pd->jdiag.p;
     }
@@ -733,10 +734,6 @@ sdot_status sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions(sdot_pd
                                                                          double *vals, double *v_values, int64_t cap,
                                                                          int64_t *nnz) {
     if (!pd || !nnz || (cols && (!offsets || !vals))) return SDOT_INVALID_ARG;
-    if (pd->dim != 2) {
-        pd->err = "position derivatives: 2D only";
-        return SDOT_NOT_SUPPORTED;
-    }
     if (pd->has_image || pd->shapes.size() != 1) {
         pd->err = "position derivatives: one domain shape (no image, no assembly of several shapes)";
         return SDOT_NOT_SUPPORTED;

@@ -46,6 +46,9 @@ struct Poly3Mem {
     // dual-edge map: vertex holding the directed edge (x -> y) at [(x * PCAP + y) * S];
     // null: edges are found by a scan of the vertices
     unsigned short *emap = nullptr;
+    // per-plane first moments (3) and second moments (xx, xy, xz, yy, yz, zz) of the
+    // face, local coordinates, for the position derivatives; null: not wanted
+    double *jmom = nullptr;   // [(k * 9 + c) * S]
 };
 
 // voxel / box plane ids: box side k -> -1 - k (k = 0..5: -x,+x,-y,+y,-z,+z);
@@ -333,6 +336,9 @@ struct Cell3 {
     SDOT_HD double integrate(double *off = nullptr) {
         double tv = 0, tm[3] = {0, 0, 0};
         for (int k = 0; k < np; ++k) m.acc[k * m.S] = 0;
+        if (m.jmom)
+            for (int k = 0; k < np; ++k)
+                for (int c = 0; c < 9; ++c) m.jmom[(k * 9 + c) * m.S] = 0;
         for (int k = 0; k < nv; ++k) {
             const int t = V_T(k);
             const int p[3] = {t & 255, (t >> 8) & 255, (t >> 16) & 255};
@@ -349,6 +355,16 @@ struct Cell3 {
                 const double cx = y0 * z1 - z0 * y1, cy = z0 * x1 - x0 * z1, cz = x0 * y1 - y0 * x1;
                 const double cn = cx * fx + cy * fy + cz * fz, n2 = fx * fx + fy * fy + fz * fz;
                 m.acc[f * m.S] += cn / sqrt(n2);
+                if (m.jmom) {   // triangle (foot of the plane F, P, Q), signed area cn / (2 |n|)
+                    const double sf = fh / n2, sa = 0.5 * cn / sqrt(n2);
+                    const double F[3] = {sf * fx, sf * fy, sf * fz}, P[3] = {x0, y0, z0}, Q[3] = {x1, y1, z1};
+                    const double S3[3] = {F[0] + P[0] + Q[0], F[1] + P[1] + Q[1], F[2] + P[2] + Q[2]};
+                    for (int c = 0; c < 3; ++c) m.jmom[(f * 9 + c) * m.S] += sa * S3[c] / 3;
+                    int c = 3;
+                    for (int u = 0; u < 3; ++u)
+                        for (int v = u; v < 3; ++v, ++c)
+                            m.jmom[(f * 9 + c) * m.S] += sa / 12 * (F[u] * F[v] + P[u] * P[v] + Q[u] * Q[v] + S3[u] * S3[v]);
+                }
                 if (off) {
                     const double s = fh / n2, v = s * cn / 6;
                     tv += v;
@@ -362,6 +378,8 @@ struct Cell3 {
             for (int d = 0; d < 3; ++d) off[d] = tv != 0 ? tm[d] / tv : 0.0;
         double vol = 0;
         for (int k = 0; k < np; ++k) {
+            if (m.jmom && m.acc[k * m.S] < 0)   // the face's edges ran clockwise: flip its moments
+                for (int c = 0; c < 9; ++c) m.jmom[(k * 9 + c) * m.S] = -m.jmom[(k * 9 + c) * m.S];
             const double ar = 0.5 * fabs(m.acc[k * m.S]);
             m.acc[k * m.S] = ar;
             double fx, fy, fz, fh;
@@ -629,6 +647,7 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
     // Hessian row and boundary measure
     double der = 0;
     int nrow = 0, nfaces = 0;
+    double jd[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool row_ovf = false;
     for (int k = 0; k < npc; ++k) {
         const double ar = C.m.hacc[k * C.m.S];
@@ -649,6 +668,48 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
         if (!is_neighbour(id)) continue;
         double qn[3];
         const int64_t jn = cut_dirac<3>(a, id, qn);
+        if (C.m.jmom && C.nv > 0) {
+            // get_der_centroids_and_integrals_wrt_weight_and_positions.h:124-166 with the
+            // face's moments in absolute coordinates (ConvexPolyhedron3.tcc:226-265: not
+            // scaled by the density; measure = the rho-weighted area)
+            const double c0a[3] = {c0x, c0y, c0z};
+            const double ag = C.m.acc[k * C.m.S];
+            double Cl[3], Ml[3][3], Ca[3], Ma[3][3];
+            for (int u = 0; u < 3; ++u) Cl[u] = C.m.jmom[(k * 9 + u) * C.m.S];
+            {
+                int c = 3;
+                for (int u = 0; u < 3; ++u)
+                    for (int v = u; v < 3; ++v, ++c) Ml[u][v] = Ml[v][u] = C.m.jmom[(k * 9 + c) * C.m.S];
+            }
+            for (int u = 0; u < 3; ++u) Ca[u] = ag * c0a[u] + Cl[u];
+            for (int u = 0; u < 3; ++u)
+                for (int v = 0; v < 3; ++v) Ma[u][v] = ag * c0a[u] * c0a[v] + c0a[u] * Cl[v] + Cl[u] * c0a[v] + Ml[u][v];
+            const double dist = sqrt((qn[0] - c0x) * (qn[0] - c0x) + (qn[1] - c0y) * (qn[1] - c0y) + (qn[2] - c0z) * (qn[2] - c0z));
+            if (dist != 0) {
+                const double bw = 0.5 * ar / dist;
+                double d1[16];
+                d1[15] = -bw;
+                jd[15] += bw;
+                for (int e = 0; e < 3; ++e) {
+                    d1[4 * e + 3] = -0.5 * Ca[e] / dist;
+                    jd[4 * e + 3] += 0.5 * Ca[e] / dist;
+                }
+                for (int d = 0; d < 3; ++d) {
+                    d1[12 + d] = (ar * qn[d] - Ca[d]) / dist;
+                    jd[12 + d] += (Ca[d] - ar * c0a[d]) / dist;
+                }
+                for (int d = 0; d < 3; ++d)
+                    for (int e = 0; e < 3; ++e) {
+                        d1[4 * e + d] = (Ca[e] * qn[d] - Ma[d][e]) / dist;
+                        jd[4 * e + d] += (Ma[d][e] - Ca[e] * c0a[d]) / dist;
+                    }
+                if (jn == t) {   // the cell's own image: its block merges into the diagonal
+                    for (int q = 0; q < 16; ++q) jd[q] += d1[q];
+                } else if (nrow < a.ell_cap) {
+                    for (int q = 0; q < 16; ++q) a.jblk[((int64_t)nrow * 16 + q) * a.ell_stride + t] = d1[q];
+                }
+            }
+        }
         if (jn == t) continue;   // a face with the cell's own image adds nothing
         double nx, ny, nz, hv;
         C.plane64(k, nx, ny, nz, hv);
@@ -666,6 +727,8 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
     a.mass[t] = mass;
     if (a.mom)
         for (int d = 0; d < 3; ++d) a.mom[3 * t + d] = mom[d];
+    if (C.m.jmom)
+        for (int q = 0; q < 16; ++q) a.jdiag[(int64_t)q * a.ell_stride + t] = jd[q];
     a.arc[t] = 0;
     a.perim[t] = perim;
     a.diag[t] = der;

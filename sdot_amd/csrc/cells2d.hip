@@ -224,7 +224,7 @@ namespace {
 // (get_der_centroids_and_integrals_wrt_weight_and_positions.h:167-226): per sorted row t
 // (dirac i = perm[t]) the items -- neighbour slots and the diagonal -- sorted by dirac
 // column, duplicates (images of one dirac) merged.
-constexpr int JMAX = 65;   // 64 ELL slots + the diagonal
+constexpr int JMAX = 256;   // up to 255 ELL slots + the diagonal
 
 __device__ int jac_items(const int *row_len, const int *ell_col, int64_t stride, const int *perm, int64_t t, int *c,
                          int *slot) {
@@ -247,8 +247,8 @@ __device__ int jac_items(const int *row_len, const int *ell_col, int64_t stride,
     return m;
 }
 
-// cnt[i] = 9 x the number of distinct columns of dirac i
-__global__ void k_jac_count(const int *row_len, const int *ell_col, int64_t stride, const int *perm, int64_t n,
+// cnt[i] = nb x the number of distinct columns of dirac i (nb = (dim + 1)^2)
+__global__ void k_jac_count(const int *row_len, const int *ell_col, int64_t stride, const int *perm, int64_t n, int nb,
                             int64_t *cnt) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -256,15 +256,18 @@ __global__ void k_jac_count(const int *row_len, const int *ell_col, int64_t stri
     const int m = jac_items(row_len, ell_col, stride, perm, t, c, slot);
     int u = 0;
     for (int a = 0; a < m; ++a) u += a == 0 || c[a] != c[a - 1];
-    cnt[perm[t]] = 9 * (int64_t)u;
+    cnt[perm[t]] = nb * (int64_t)u;
 }
 
-// rows 3 i + d (centroid x, y, mass), columns 3 m + e (position x, y, weight); the
-// centroid rows get the mass / centroid correction (:172-177: (b - b_mass * centroid) /
-// (mass + (mass == 1)), the reference's own divisor)
+// rows NU i + d (centroid coordinates, then the mass), columns NU m + e (position
+// coordinates, then the weight), NU = DIM + 1; the centroid rows get the mass /
+// centroid correction (:172-177: (b - b_mass * centroid) / (mass + (mass == 1)), the
+// reference's own divisor).  Blocks are NU x NU at NU * NU doubles per slot.
+template <int DIM>
 __global__ void k_jac_fill(const int *row_len, const int *ell_col, const double *jblk, const double *jdiag, int64_t stride,
                            const double *spos, const double *mass, const double *mom, const int *perm, int64_t n,
                            const int64_t *base, int64_t *cols, double *vals) {
+    constexpr int NU = DIM + 1, NB = NU * NU;
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= n) return;
     int c[JMAX], slot[JMAX];
@@ -272,23 +275,25 @@ __global__ void k_jac_fill(const int *row_len, const int *ell_col, const double 
     int u = 0;
     for (int a = 0; a < m; ++a) u += a == 0 || c[a] != c[a - 1];
     const double ms = mass[t];
-    const double v[2] = {ms != 0 ? spos[2 * t] + mom[2 * t] / ms : 0.0, ms != 0 ? spos[2 * t + 1] + mom[2 * t + 1] / ms : 0.0};
+    double v[DIM];
+    for (int d = 0; d < DIM; ++d) v[d] = ms != 0 ? spos[DIM * t + d] + mom[DIM * t + d] / ms : 0.0;
     const double den = ms + (ms == 1 ? 1.0 : 0.0);
     const int64_t b0 = base[perm[t]];
     int k = 0;
     for (int a = 0; a < m;) {
         const int col = c[a];
-        double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double B[NB];
+        for (int q = 0; q < NB; ++q) B[q] = 0;
         for (; a < m && c[a] == col; ++a)
-            for (int q = 0; q < 9; ++q)
-                B[q] += slot[a] < 0 ? jdiag[q * stride + t] : jblk[((int64_t)slot[a] * 9 + q) * stride + t];
-        for (int r = 0; r < 2; ++r)
-            for (int cc = 0; cc < 3; ++cc) B[3 * r + cc] = (B[3 * r + cc] - B[6 + cc] * v[r]) / den;
-        for (int d = 0; d < 3; ++d) {
-            const int64_t p = b0 + (int64_t)d * 3 * u + 3 * k;
-            for (int e = 0; e < 3; ++e) {
-                cols[p + e] = 3 * (int64_t)col + e;
-                vals[p + e] = B[3 * d + e];
+            for (int q = 0; q < NB; ++q)
+                B[q] += slot[a] < 0 ? jdiag[q * stride + t] : jblk[((int64_t)slot[a] * NB + q) * stride + t];
+        for (int r = 0; r < DIM; ++r)
+            for (int cc = 0; cc < NU; ++cc) B[NU * r + cc] = (B[NU * r + cc] - B[NU * DIM + cc] * v[r]) / den;
+        for (int d = 0; d < NU; ++d) {
+            const int64_t p = b0 + (int64_t)d * NU * u + NU * k;
+            for (int e = 0; e < NU; ++e) {
+                cols[p + e] = NU * (int64_t)col + e;
+                vals[p + e] = B[NU * d + e];
             }
         }
         ++k;
@@ -298,13 +303,14 @@ __global__ void k_jac_fill(const int *row_len, const int *ell_col, const double 
 
 void export_jac(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, double *h_v, int64_t cap, int64_t *nnz) {
     const int64_t N = pd->N;
+    const int dim = pd->dim, nu = dim + 1;
     hipStream_t s = pd->stream;
     pd->csr_cnt.ensure(N);
     pd->csr_off.ensure(N + 1);
     pd->csr_tmp.ensure(scan_tmp_size(N));
     const unsigned nb = (unsigned)((N + 127) / 128);
     hipLaunchKernelGGL(k_jac_count, dim3(nb), dim3(128), 0, s, pd->row_len.p, pd->ell_col.p, (int64_t)N, pd->perm.p, N,
-                       pd->csr_cnt.p);
+                       nu * nu, pd->csr_cnt.p);
     HIPCK(hipGetLastError());
     exclusive_scan<int64_t>(pd->csr_cnt.p, pd->csr_off.p, N, pd->csr_tmp.p, s);
     std::vector<int64_t> base(N + 1);
@@ -313,27 +319,31 @@ void export_jac(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, doub
     const int64_t total = base[N];
     *nnz = total;
     if (h_v) {
-        std::vector<double> m(N), c(2 * N);
+        std::vector<double> m(N), c(dim * N);
         export_vec(pd, pd->mass.p, m.data());
         export_centroids(pd, c.data());
         for (int64_t i = 0; i < N; ++i) {
-            h_v[3 * i] = c[2 * i];
-            h_v[3 * i + 1] = c[2 * i + 1];
-            h_v[3 * i + 2] = m[i];
+            for (int d = 0; d < dim; ++d) h_v[nu * i + d] = c[dim * i + d];
+            h_v[nu * i + dim] = m[i];
         }
     }
     if (!h_col) return;
     if (total > cap) throw std::string("Jacobian capacity too small");
     for (int64_t i = 0; i < N; ++i) {
-        const int64_t row = (base[i + 1] - base[i]) / 3;
-        for (int d = 0; d < 3; ++d) h_off[3 * i + d] = base[i] + d * row;
+        const int64_t row = (base[i + 1] - base[i]) / nu;
+        for (int d = 0; d < nu; ++d) h_off[nu * i + d] = base[i] + d * row;
     }
-    h_off[3 * N] = total;
+    h_off[nu * N] = total;
     pd->csr_col.ensure(total > 0 ? total : 1);
     pd->csr_val.ensure(total > 0 ? total : 1);
-    hipLaunchKernelGGL(k_jac_fill, dim3(nb), dim3(128), 0, s, pd->row_len.p, pd->ell_col.p, pd->jblk.p, pd->jdiag.p,
-                       (int64_t)N, pd->spos.p, pd->mass.p, pd->mom.p, pd->perm.p, N, pd->csr_off.p, pd->csr_col.p,
-                       pd->csr_val.p);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_jac_fill<2>, dim3(nb), dim3(128), 0, s, pd->row_len.p, pd->ell_col.p, pd->jblk.p,
+                           pd->jdiag.p, (int64_t)N, pd->spos.p, pd->mass.p, pd->mom.p, pd->perm.p, N, pd->csr_off.p,
+                           pd->csr_col.p, pd->csr_val.p);
+    else
+        hipLaunchKernelGGL(k_jac_fill<3>, dim3(nb), dim3(128), 0, s, pd->row_len.p, pd->ell_col.p, pd->jblk.p,
+                           pd->jdiag.p, (int64_t)N, pd->spos.p, pd->mass.p, pd->mom.p, pd->perm.p, N, pd->csr_off.p,
+                           pd->csr_col.p, pd->csr_val.p);
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(h_col, pd->csr_col.p, total * sizeof(int64_t), hipMemcpyDeviceToHost, s));
     HIPCK(hipMemcpyAsync(h_val, pd->csr_val.p, total * sizeof(double), hipMemcpyDeviceToHost, s));

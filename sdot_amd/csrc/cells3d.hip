@@ -16,13 +16,13 @@ constexpr int VCAP3W = 508, PCAP3W = 255;   // tier 2
 constexpr int BS3 = 256;
 
 template <class T>
-size_t slot_bytes(int vcap, int pcap) {
+size_t slot_bytes(int vcap, int pcap, bool jac = false) {
     return (size_t)vcap * (7 * sizeof(T) + 2 * sizeof(int)) + (size_t)pcap * (4 * sizeof(T) + sizeof(int) + 2 * sizeof(double)) +
-           (size_t)pcap * pcap * sizeof(unsigned short);
+           (size_t)pcap * pcap * sizeof(unsigned short) + (jac ? (size_t)pcap * 9 * sizeof(double) : 0);
 }
 
 template <class T>
-__host__ __device__ c3::Poly3Mem<T> slot(char *base, int64_t S, int64_t s, int vcap, int pcap) {
+__host__ __device__ c3::Poly3Mem<T> slot(char *base, int64_t S, int64_t s, int vcap, int pcap, bool jac = false) {
     // field-major layout: [field][k][slot]
     c3::Poly3Mem<T> m;
     char *p = base;
@@ -48,18 +48,19 @@ __host__ __device__ c3::Poly3Mem<T> slot(char *base, int64_t S, int64_t s, int v
     take(m.acc, sizeof(double), pcap);
     take(m.hacc, sizeof(double), pcap);
     take(m.emap, sizeof(unsigned short), pcap * pcap);
+    if (jac) take(m.jmom, sizeof(double), pcap * 9);
     m.S = (int)S;
     return m;
 }
 
 // LAST: overflowing cells raise FLAG_CELL_OVERFLOW; otherwise they are appended
 // to ovf_list (counter ovf_count) for the next tier
-template <class T, int VC, int PC, bool LAST>
+template <class T, int VC, int PC, bool LAST, bool JAC = false>
 __global__ __launch_bounds__(BS3) void k_cells3_thread(const CellArgs *__restrict__ ap, char *scratch, int64_t nslots) {
     const CellArgs &a = *ap;
     const int64_t s = blockIdx.x * (int64_t)BS3 + threadIdx.x;
     if (s >= nslots) return;
-    const c3::Poly3Mem<T> m = slot<T>(scratch, nslots, s, VC, PC);
+    const c3::Poly3Mem<T> m = slot<T>(scratch, nslots, s, VC, PC, JAC);
     const int64_t count = a.work ? a.work_n : a.N;
     for (int64_t k = s; k < count; k += nslots) {
         const int64_t t = a.work ? a.work[k] : k;
@@ -84,7 +85,7 @@ const CellArgs *upload_args(sdot_pd *pd, const CellArgs &a, int slot) {
     return pd->dargs.p + slot;
 }
 
-template <class T, int VC, int PC, bool LAST>
+template <class T, int VC, int PC, bool LAST, bool JAC = false>
 void launch_thread3(sdot_pd *pd, const CellArgs &a) {
     int dev = 0, ncu = 0;
     HIPCK(hipGetDevice(&dev));
@@ -93,10 +94,11 @@ void launch_thread3(sdot_pd *pd, const CellArgs &a) {
     int64_t nslots = (int64_t)ncu * 4 * 64;     // four wavefronts per CU
     if (nslots > count) nslots = count;
     if (nslots < 1) return;
-    pd->scratch3.ensure(slot_bytes<T>(VC, PC) * nslots);
+    pd->scratch3.ensure(slot_bytes<T>(VC, PC, JAC) * nslots);
     const unsigned nb = (unsigned)((nslots + BS3 - 1) / BS3);
     const CellArgs *da = upload_args(pd, a, 1);
-    hipLaunchKernelGGL((k_cells3_thread<T, VC, PC, LAST>), dim3(nb), dim3(BS3), 0, pd->stream, da, pd->scratch3.p, nslots);
+    hipLaunchKernelGGL((k_cells3_thread<T, VC, PC, LAST, JAC>), dim3(nb), dim3(BS3), 0, pd->stream, da, pd->scratch3.p,
+                       nslots);
     HIPCK(hipGetLastError());
 }
 
@@ -960,6 +962,24 @@ void run_cells3(sdot_pd *pd, CellArgs &a) {
     a.work = nullptr;
     a.work_n = 0;
     const bool mixed = pd->precision == SDOT_MIXED;
+    // position derivatives: the thread kernel in FP64, with per-face moments
+    if (a.jdiag) {
+        launch_thread3<double, VCAP3, PCAP3, false, true>(pd, a);
+        HIPCK(hipEventRecord(pd->ev[3], s));
+        unsigned c1 = 0;
+        HIPCK(hipMemcpyAsync(&c1, pd->flags.p + 1, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        pd->last.nb_cells_overflow = c1;
+        if (c1 > 0) {
+            int *list2 = pd->ovf_list.p + a.N + 1;
+            HIPCK(hipMemcpyAsync(list2, pd->ovf_list.p, c1 * sizeof(int), hipMemcpyDeviceToDevice, s));
+            a.work = list2;
+            a.work_n = c1;
+            launch_thread3<double, VCAP3W, PCAP3W, true, true>(pd, a);
+            a.work = nullptr;
+        }
+        return;
+    }
     // translated diracs (periodic images): every cell on the thread kernel, whose
     // traversal visits the grid once per translation; otherwise the wavefront kernel
     if (a.ntrans > 0) {

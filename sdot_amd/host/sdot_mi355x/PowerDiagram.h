@@ -428,9 +428,10 @@ int get_der_integrals_wrt_weights(std::vector<TI> &m_offsets, std::vector<TI> &m
 }
 
 /// get_der_centroids_and_integrals_wrt_weight_and_positions
-/// (get_der_centroids_and_integrals_wrt_weight_and_positions.h:16, 2D, one domain shape):
-/// rows 3 n + d (centroid x, y, mass of dirac n), columns 3 m + e (position x, y, weight
-/// of dirac m); v_values = centroids and masses; returns 1 on an empty cell, 0 otherwise.
+/// (get_der_centroids_and_integrals_wrt_weight_and_positions.h:16, one domain shape):
+/// rows nu n + d (centroid coordinates, then the mass of dirac n), columns nu m + e
+/// (position coordinates, then the weight of dirac m), nu = dim + 1; v_values = centroids
+/// and masses; returns 1 on an empty cell, 0 otherwise.
 template <class TI, class TF, class Grid, class Bounds, class Pt, class Func>
 int get_der_centroids_and_integrals_wrt_weight_and_positions(std::vector<TI> &m_offsets, std::vector<TI> &m_columns,
                                                              std::vector<TF> &m_values, std::vector<TF> &v_values,
@@ -444,9 +445,10 @@ int get_der_centroids_and_integrals_wrt_weight_and_positions(std::vector<TI> &m_
     std::int64_t nnz = 0;
     int st = sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions(pd, nullptr, nullptr, nullptr, nullptr, 0, &nnz);
     check(pd, st, "get_der_centroids_and_integrals_wrt_weight_and_positions", true);
-    std::vector<std::int64_t> off(3 * nb_diracs + 1), col(nnz > 0 ? nnz : 1);
+    constexpr int nu = sizeof(Pt) / sizeof(double) + 1;
+    std::vector<std::int64_t> off(nu * nb_diracs + 1), col(nnz > 0 ? nnz : 1);
     m_values.resize(nnz > 0 ? nnz : 1);
-    v_values.resize(3 * nb_diracs);
+    v_values.resize(nu * nb_diracs);
     st = sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions(pd, off.data(), col.data(), m_values.data(),
                                                                       v_values.data(), nnz, &nnz);
     check(pd, st, "get_der_centroids_and_integrals_wrt_weight_and_positions", true);

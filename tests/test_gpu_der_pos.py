@@ -1,5 +1,5 @@
 """get_der_centroids_and_integrals_wrt_weight_and_positions on the device
-(sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions, 2D) against the oracle's
+(sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions, 2D and 3D) against the oracle's
 restatement: box, ball, periodic images and a polygon domain, FP64 and mixed handles, the
 reference's 36-dirac configuration and a 1e5-dirac run checked by finite differences
 of the device's own centroids on a sample of columns."""
@@ -31,7 +31,7 @@ def device(pos, w, precision="fp64", ball=False, translations=None, shape=None):
     return pd.der_centroids_and_integrals_wrt_weight_and_positions(), pd
 
 
-def check(g, o, n):
+def check(g, o, n, nu=3):
     off, cols, vals, v = g
     st, ooff, ocols, ovals, ov = o
     assert st in (O.OR_OK, O.OR_EMPTY_CELL)
@@ -43,8 +43,8 @@ def check(g, o, n):
     # the centroid rows are divided by the cell's mass: rounding in slivers of mass near
     # 0 is amplified on both sides, so those rows are compared for cells of mass
     # above 1e-3 / N only (the mass rows everywhere)
-    row = np.repeat(np.arange(3 * n), np.diff(off))
-    keep = fin & ((row % 3 == 2) | (ov[3 * (row // 3) + 2] > 1e-3 / n))
+    row = np.repeat(np.arange(nu * n), np.diff(off))
+    keep = fin & ((row % nu == nu - 1) | (ov[nu * (row // nu) + nu - 1] > 1e-3 / n))
     scale = np.abs(ovals[keep]).max()
     err = np.abs(vals[keep] - ovals[keep]).max() / scale
     assert err < 1e-9, err
@@ -113,13 +113,25 @@ def test_der_pos_large_against_device_finite_differences():
             assert np.abs(col - fd).max() < 1e-5 * max(np.abs(fd).max(), 1e-12), (m, e)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_der_pos_3d_parity(periodic, precision):
+    import sdot_amd
+    n = 2000
+    pos, w = inputs(3, n, 84, wscale=0.05 * n ** (-2 / 3))
+    T = [(x, y, z) for x in (-1, 0, 1) for y in (-1, 0, 1) for z in (-1, 0, 1) if x or y or z] if periodic else None
+    pd = sdot_amd.PowerDiagram(3, precision)
+    pd.add_box_shape([0, 0, 0], [1, 1, 1])
+    if T:
+        pd.set_translations(T)
+    pd.set_diracs(pos, w)
+    g = pd.der_centroids_and_integrals_wrt_weight_and_positions()
+    err = check(g, O.der_pos(pos, w, translations=T), n, 4)
+    print(f"[der_pos 3D periodic={periodic}] {precision}: rel err {err:.2e}, nnz {len(g[1])}")
+
+
 def test_der_pos_rejected_cases():
     import sdot_amd
-    pd = sdot_amd.PowerDiagram(3)
-    pd.add_box_shape([0, 0, 0], [1, 1, 1])
-    pd.set_diracs(np.full((2, 3), 0.5) + [[0.1, 0, 0], [-0.1, 0, 0]], np.zeros(2))
-    with pytest.raises(sdot_amd.SdotError):
-        pd.der_centroids_and_integrals_wrt_weight_and_positions()
     pd = sdot_amd.PowerDiagram(2)
     pd.set_image_density([0, 0], [1, 1], np.ones((3, 3)))
     pd.set_diracs(np.array([[0.3, 0.3], [0.6, 0.6]]), np.zeros(2))

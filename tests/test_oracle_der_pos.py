@@ -1,4 +1,4 @@
-"""The oracle's get_der_centroids_and_integrals_wrt_weight_and_positions (2D), pinned by
+"""The oracle's get_der_centroids_and_integrals_wrt_weight_and_positions (2D and 3D), pinned by
 finite differences of its own centroids and masses (the check the reference's
 tests/test_der_pos.cpp makes, on its 36-dirac ball configuration from the golden
 fixtures and on random diracs, with and without the ball and with periodic images) and
@@ -17,28 +17,29 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 T2 = [(x, y) for x in (-1, 0, 1) for y in (-1, 0, 1) if x or y]
 
 
-def dense(off, cols, vals, n):
-    return sp.csr_matrix((vals, cols, off), shape=(3 * n, 3 * n)).toarray()
+def dense(off, cols, vals, n, nu=3):
+    return sp.csr_matrix((vals, cols, off), shape=(nu * n, nu * n)).toarray()
 
 
 def finite_differences(pos, w, h, **kw):
-    n = len(w)
+    n, dim = pos.shape
+    nu = dim + 1
 
     def state(p, ww):
         st, c, m = O.centroids(p, ww, **kw)
         return np.column_stack([c, m]).ravel()
 
-    J = np.zeros((3 * n, 3 * n))
+    J = np.zeros((nu * n, nu * n))
     for m in range(n):
-        for e in range(3):
+        for e in range(nu):
             pp, wp, pm, wm = pos.copy(), w.copy(), pos.copy(), w.copy()
-            if e < 2:
+            if e < dim:
                 pp[m, e] += h
                 pm[m, e] -= h
             else:
                 wp[m] += h
                 wm[m] -= h
-            J[:, 3 * m + e] = (state(pp, wp) - state(pm, wm)) / (2 * h)
+            J[:, nu * m + e] = (state(pp, wp) - state(pm, wm)) / (2 * h)
     return J
 
 
@@ -102,3 +103,21 @@ def test_reference_divisor_for_a_cell_of_mass_one():
     assert np.abs(J[rows_c] - 0.5 * F[rows_c]).max() < 1e-7
     assert np.abs(J[[2, 5]] - F[[2, 5]]).max() < 1e-7
     assert np.abs(F[rows_c]).max() > 0.1
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_3d_against_finite_differences(periodic):
+    """3D faces: pos_integral and momentum over the triangles of each face
+    (ConvexPolyhedron3.tcc:226-265), no ball (TODO in the reference)."""
+    n = 10
+    u = O.rng_uniform(112, 4 * n)
+    pos = u[: 3 * n].reshape(n, 3).copy()
+    w = 0.01 * u[3 * n:]
+    kw = {}
+    if periodic:
+        kw["translations"] = [(x, y, z) for x in (-1, 0, 1) for y in (-1, 0, 1) for z in (-1, 0, 1) if x or y or z]
+    st, off, cols, vals, v = O.der_pos(pos, w, **kw)
+    assert st == O.OR_OK
+    J = dense(off, cols, vals, n, 4)
+    F = finite_differences(pos, w, 1e-7, **kw)
+    assert np.abs(J - F).max() < 1e-6 * np.abs(F).max()
