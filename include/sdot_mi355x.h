@@ -94,6 +94,18 @@ sdot_status sdot_pd_clear_shapes(sdot_pd *pd);
 /* Radial function: 0 = FunctionEnum::Unit, 1 = FunctionEnum::InBallW05 (clip by the
  * ball of radius sqrt(w_i)) -- src/sdot/Integration/Unit.h, InBallW05.h */
 sdot_status sdot_pd_set_ball_clip(sdot_pd *pd, int on);
+/* Radial function (the Func argument of get_integrals / get_der_integrals_wrt_weights /
+ * OptimalTransportSolver::solve), 2D with one domain shape for the last four:
+ *   SDOT_FUNC_UNIT, SDOT_FUNC_IN_BALL_W05 -- Unit.h, InBallW05.h (as sdot_pd_set_ball_clip);
+ *   SDOT_FUNC_R2, SDOT_FUNC_R2_IN_BALL_W05 -- |x - x_i|^2 (R2.h, R2InBallW05.h): integrals only
+ *     (the reference has no R2 boundary items: ConvexPolyhedron2.tcc:432-435);
+ *   SDOT_FUNC_WMR2, SDOT_FUNC_PP_WMR2 -- w_i - |x - x_i|^2 (WmR2.h, PpWmR2.h: its positive part,
+ *     i.e. with the ball cut): integrals, Hessian (ConvexPolyhedron2.tcc:391-430, 2044-2088;
+ *     d/dw_i adds the cell's measure, :135-138) and the Newton solve.
+ * Centroids and position derivatives take Unit / InBallW05 only. */
+enum { SDOT_FUNC_UNIT = 0, SDOT_FUNC_IN_BALL_W05 = 1, SDOT_FUNC_R2 = 2, SDOT_FUNC_R2_IN_BALL_W05 = 3,
+       SDOT_FUNC_WMR2 = 4, SDOT_FUNC_PP_WMR2 = 5 };
+sdot_status sdot_pd_set_radial_func(sdot_pd *pd, int func);
 /* SpZGrid::translations (SpZGrid.h:47-53, traversal SpZGrid.tcc:558-561,610): every
  * cell is also cut by the diracs translated by each of the n vectors
  * (translations[dim s .. dim s + dim)); a face with the image of dirac j adds to

@@ -239,8 +239,13 @@ def make_domain(dim, p0=None, p1=None, coeff=1.0, img=None, items=None, translat
     return d, keep
 
 
+RADIAL = {"unit": 0, "r2": 1, "wmr2": 2}
+
+
 def cells(pos, w, dim=None, ball=False, brute=False, nthreads=None, domain=None, img=None, cap=None, items=None,
-          translations=None):
+          translations=None, radial="unit"):
+    """radial: the final integration function -- "unit" (Unit / InBallW05), "r2" (R2 /
+    R2InBallW05: masses only) or "wmr2" (WmR2 / PpWmR2 with ball=True)."""
     """Per-cell masses, ball arc, perimeter and raw rows.  Returns dict."""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     dim = dim or pos.shape[1]
@@ -258,7 +263,7 @@ def cells(pos, w, dim=None, ball=False, brute=False, nthreads=None, domain=None,
     rl = np.zeros(N, dtype=np.int64)
     rc = np.zeros(N * cap, dtype=np.int64)
     rv = np.zeros(N * cap)
-    flags = (1 if ball else 0) | (2 if brute else 0)
+    flags = (1 if ball else 0) | (2 if brute else 0) | (RADIAL[radial] << 4)
     st = lib().or_cells(dim, N, _ptr(pos), _ptr(w), ctypes.byref(dom), flags, nthreads,
                         _ptr(masses), _ptr(arc), _ptr(per), cap, _ptr(rl), _ptr(rc), _ptr(rv))
     del keep
@@ -349,7 +354,7 @@ def pcg(off, cols, vals, b, rtol=1e-12, max_iter=100000):
 
 
 def ot_solve(pos, w0, targets, ball=False, tol=1e-6, stop_rule=0, max_iter=150, nthreads=None,
-             domain=None, img=None, items=None, translations=None):
+             domain=None, img=None, items=None, translations=None, radial="unit"):
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     dim = pos.shape[1]
     w = np.array(w0, dtype=np.float64, copy=True)
@@ -360,7 +365,7 @@ def ot_solve(pos, w0, targets, ball=False, tol=1e-6, stop_rule=0, max_iter=150, 
         dom, keep = make_domain(dim, *domain, img=img, items=items, translations=translations)
     stats = SolveStats()
     st = lib().or_ot_solve(dim, pos.shape[0], _ptr(pos), _ptr(w), _ptr(t), ctypes.byref(dom),
-                           1 if ball else 0, nthreads or os.cpu_count() or 1, tol, stop_rule, max_iter,
-                           ctypes.byref(stats))
+                           (1 if ball else 0) | (RADIAL[radial] << 4), nthreads or os.cpu_count() or 1, tol,
+                           stop_rule, max_iter, ctypes.byref(stats))
     del keep
     return st, w, stats

@@ -220,6 +220,64 @@ double or_cp2_area(const or_cp2 *cp, double coeff) {
     return coeff * res;
 }
 
+/* integration(Constant, R2) and (Constant, WmR2), ConvexPolyhedron2.tcc:2044-2132: the
+ * integral of |x - c|^2 (rf = 1) or of w - |x - c|^2 (rf = 2) over the polygon, c the
+ * sphere centre (the dirac), as the fan of boundary items seen from c: a straight edge
+ * (P0, P1) gives the triangle (c, P0, P1), whose integral of |x - c|^2 is
+ * A (|P0|^2 + |P1|^2 + P0.P1) / 6 with A its signed area; an arc of angle da gives the
+ * sector, R^4 da / 4; the whole disc pi R^4 / 2. */
+double or_cp2_radial_integral(const or_cp2 *cp, double coeff, int rf, double w) {
+    const double R = cp->sr;
+    if (cp->n == 0) {
+        if (!(R > 0)) return 0.0;
+        return coeff * (rf == 1 ? 0.5 * M_PI * R * R * R * R : M_PI * R * R * (w - R * R / 2));
+    }
+    double area = 0, r2 = 0;
+    for (int i0 = cp->n - 1, i1 = 0; i1 < cp->n; i0 = i1++) {
+        if (cp->arc[i0]) {
+            double da = arc_angle(cp, i0, i1);
+            area += R * R * da / 2;
+            r2 += R * R * R * R * da / 4;
+        } else {
+            double px = cp->x[i0] - cp->scx, py = cp->y[i0] - cp->scy, qx = cp->x[i1] - cp->scx, qy = cp->y[i1] - cp->scy;
+            double a = 0.5 * (px * qy - py * qx);
+            area += a;
+            r2 += a * (px * px + py * py + qx * qx + qy * qy + px * qx + py * qy) / 6;
+        }
+    }
+    return coeff * (rf == 1 ? r2 : w * area - r2);
+}
+
+/* for_each_boundary_item(Constant, WmR2), ConvexPolyhedron2.tcc:391-430: measure = coeff
+ * x the integral of w - |x - c|^2 along the item (edge: length x (w - (|P0|^2 + P0.P1 +
+ * |P1|^2) / 3); arc: length x (w - R^2)); zero-length edges are skipped */
+int or_cp2_boundary_items_wmr2(const or_cp2 *cp, double coeff, double w, double *measures, int64_t *ids, int cap) {
+    const double R = cp->sr;
+    if (cp->n == 0) {
+        if (R >= 0 && cap > 0) {
+            measures[0] = coeff * 2 * M_PI * R * (w - R * R);
+            ids[0] = cp->sid;
+            return 1;
+        }
+        return 0;
+    }
+    int m = 0;
+    for (int i1 = 0, i0 = cp->n - 1; i1 < cp->n; i0 = i1++) {
+        if (m >= cap) return -1;
+        if (cp->arc[i0]) {
+            measures[m] = coeff * arc_angle(cp, i0, i1) * R * (w - R * R);
+            ids[m++] = cp->id[i0];
+        } else {
+            double px = cp->x[i0] - cp->scx, py = cp->y[i0] - cp->scy, qx = cp->x[i1] - cp->scx, qy = cp->y[i1] - cp->scy;
+            double d2 = (qx - px) * (qx - px) + (qy - py) * (qy - py);
+            if (d2 == 0) continue;
+            measures[m] = coeff * sqrt(d2) * (w - (px * (px + qx) + qx * qx + py * (py + qy) + qy * qy) / 3);
+            ids[m++] = cp->id[i0];
+        }
+    }
+    return m;
+}
+
 /* add_centroid_contrib(Constant, Unit), ConvexPolyhedron2.tcc:1047-1073, _centroid_arc
  * :2296-2330: area and first moments of the polygon, as signed triangles and
  * circular sectors seen from (ox, oy) (the ball centre for arcs), relative to it */
@@ -730,7 +788,13 @@ static void piece2_accumulate(const cells_job *J, int64_t i, const or_cp2 *cp, d
     static __thread double meas[2 * OR_CP2_CAP + 2];
     static __thread int64_t ids[2 * OR_CP2_CAP + 2];
     int m;
-    if (!poly) {
+    const int rf = (J->flags >> 4) & 3;   /* 0 Unit, 1 R2, 2 WmR2 (the final integration function) */
+    if (!poly && rf) {
+        *mass += or_cp2_radial_integral(cp, coeff, rf, J->w[i]);
+        if (rf == 1) return;   /* R2 has no boundary items in the reference (TODO there) */
+        m = or_cp2_boundary_items_wmr2(cp, coeff, J->w[i], meas, ids, 2 * OR_CP2_CAP + 2);
+        row->der0 += or_cp2_area(cp, coeff);   /* integration_der_wrt_weight(WmR2) = the measure (:135-138) */
+    } else if (!poly) {
         *mass += or_cp2_area(cp, coeff);
         m = or_cp2_boundary_items(cp, coeff, meas, ids, 2 * OR_CP2_CAP + 2);
     } else {                                     /* straight edges only (no ball with a polynomial) */

@@ -107,6 +107,7 @@ SIGNATURES = {
     "sdot_pd_clear_shapes": (_int, [_p]),
     "sdot_pd_set_ball_clip": (_int, [_p, _int]),
     "sdot_pd_set_translations": (_int, [_p, _p, _i64]),
+    "sdot_pd_set_radial_func": (_int, [_p, _int]),
     "sdot_pd_compute": (_int, [_p]),
     "sdot_pd_integrals": (_int, [_p, _p]),
     "sdot_pd_add_convex_polyhedron_shape": (_int, [_p, _p, _p, _i64, _d, _i64]),
@@ -271,6 +272,12 @@ class PowerDiagram:
         vector (periodic images); an empty list clears them."""
         t = np.ascontiguousarray(np.asarray(translations, dtype=np.float64).reshape(-1, self.dim))
         self._check(lib().sdot_pd_set_translations(self._h, _ptr(t), len(t)), "set_translations")
+
+    RADIAL_FUNCS = {"unit": 0, "in_ball_w05": 1, "r2": 2, "r2_in_ball_w05": 3, "wmr2": 4, "pp_wmr2": 5}
+
+    def set_radial_func(self, name):
+        """FunctionEnum of the integrals: unit, in_ball_w05, r2, r2_in_ball_w05, wmr2, pp_wmr2."""
+        self._check(lib().sdot_pd_set_radial_func(self._h, self.RADIAL_FUNCS[name]), "set_radial_func")
 
     def set_ball_clip(self, on=True):
         self._check(lib().sdot_pd_set_ball_clip(self._h, int(bool(on))), "set_ball_clip")

@@ -209,6 +209,10 @@ sdot_status compute_pass(sdot_pd *pd) {
     if (pd->ball && pd->dim == 3)
         throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping in 3D (ConvexPolyhedron3::ball_cut is TODO in the reference)"};
     if (pd->ball && pd->has_image) throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping with an image density"};
+    if (pd->rf && (pd->dim != 2 || pd->has_image || pd->shapes.size() > 1))
+        throw StatusError{SDOT_NOT_SUPPORTED, "R2 / WmR2 integration: 2D, one domain shape"};
+    if (pd->rf && (pd->want_mom || pd->want_jac))
+        throw StatusError{SDOT_NOT_SUPPORTED, "centroids and position derivatives: Unit / InBallW05 only"};
     if (pd->N > (int64_t)0x7FFFFFFF) throw StatusError{SDOT_INVALID_ARG, "more than 2^31-1 diracs"};
     hipStream_t s = pd->stream;
     CellArgs a{};
@@ -259,6 +263,7 @@ sdot_status compute_pass(sdot_pd *pd) {
         a.mom = pd->mom.p;
     }
     pd->pass_has_mom = pd->want_mom;
+    a.rf = pd->rf;
     a.jblk = a.jdiag = nullptr;
     if (pd->want_jac) {
         const size_t nb = (size_t)(pd->dim + 1) * (pd->dim + 1);
@@ -657,6 +662,19 @@ sdot_status sdot_pd_set_translations(sdot_pd *pd, const double *translations, in
 sdot_status sdot_pd_set_ball_clip(sdot_pd *pd, int on) {
     if (!pd) return SDOT_INVALID_ARG;
     pd->ball = on ? 1 : 0;
+    pd->rf = 0;
+    pd->pass_valid = false;
+    return SDOT_OK;
+}
+
+sdot_status sdot_pd_set_radial_func(sdot_pd *pd, int func) {
+    if (!pd) return SDOT_INVALID_ARG;
+    // FunctionEnum: the ball cut (need_ball_cut) and the final integration function
+    // (func_for_final_cp_integration) of each
+    static const int ball[6] = {0, 1, 0, 1, 0, 1}, rf[6] = {0, 0, 1, 1, 2, 2};
+    if (func < 0 || func > 5) return SDOT_INVALID_ARG;
+    pd->ball = ball[func];
+    pd->rf = rf[func];
     pd->pass_valid = false;
     return SDOT_OK;
 }
@@ -722,6 +740,10 @@ sdot_status sdot_pd_boundary_measures(sdot_pd *pd, double *ball_arc, double *per
 sdot_status sdot_pd_der_integrals_wrt_weights(sdot_pd *pd, int64_t *offsets, int64_t *cols, double *vals,
                                               double *masses, int64_t cap, int64_t *nnz) {
     if (!pd || !nnz || (cols && (!offsets || !vals))) return SDOT_INVALID_ARG;
+    if (pd->rf == 1) {
+        pd->err = "R2 has no boundary items (TODO in the reference): integrals only";
+        return SDOT_NOT_SUPPORTED;
+    }
     return guard(pd, [&] {
         sdot_status st = ensure_pass(pd);
         export_csr(pd, offsets, cols, vals, cap, nnz);
@@ -768,6 +790,10 @@ sdot_status sdot_ot_solve(sdot_pd *pd, const double *targets, double tol, int st
     if (!pd || !targets || !weights_inout || !(tol > 0) || max_iter < 1 || (stop_rule != 0 && stop_rule != 1))
         return SDOT_INVALID_ARG;
     if (pd->N <= 0) return SDOT_INVALID_ARG;
+    if (pd->rf == 1) {
+        pd->err = "R2 has no boundary items (TODO in the reference): no Newton solve";
+        return SDOT_NOT_SUPPORTED;
+    }
     return guard(pd, [&] { return ot_solve(pd, targets, tol, stop_rule, max_iter, weights_inout, stats); });
 }
 

@@ -80,11 +80,16 @@ constexpr double PI_D = 3.1415926535897932384626433832795;
 // per neighbour slot (a->jblk, beside the Hessian entry of that slot) and the
 // diagonal block jd (a->jdiag); entry [3 d + e] for output d (centroid x, y, mass)
 // and unknown e (position x, y, weight), before the mass / centroid correction.
-template <class T, bool PER = false, bool JAC = false>
+// RF: the final integration function -- 0 Unit, 1 R2 (|x - c|^2, integrals only), 2 WmR2
+// (w - |x - c|^2): integration(Constant, R2 / WmR2) and for_each_boundary_item(Constant,
+// WmR2), ConvexPolyhedron2.tcc:2044-2132 and :391-430.
+template <class T, bool PER = false, bool JAC = false, int RF = 0>
 struct Integrator {
     const CellArgs *a;
     int64_t t;
     double c0x, c0y;
+    double w0v = 0;      // the dirac's weight (RF)
+    double rint = 0;     // integral of the radial function over the fan of items (RF)
     double jd[JAC ? 9 : 1] = {};
     T r, r2;
     CSum<T> area2;       // twice the polygonal area (shoelace about the dirac)
@@ -104,6 +109,16 @@ struct Integrator {
     SDOT_HD void line(T x0, T y0, T x1, T y1, int id) {
         T ex = x1 - x0, ey = y1 - y0;
         double len = (double)sqrt(ex * ex + ey * ey);
+        if (RF) {
+            // triangle (dirac, p0, p1): integral of |x - c|^2 = A (|p0|^2 + |p1|^2 + p0.p1) / 6
+            const double X0 = x0, Y0 = y0, X1 = x1, Y1 = y1;
+            const double at = 0.5 * (X0 * Y1 - Y0 * X1);
+            const double r2 = at * (X0 * X0 + Y0 * Y0 + X1 * X1 + Y1 * Y1 + X0 * X1 + Y0 * Y1) / 6;
+            rint += RF == 1 ? r2 : w0v * at - r2;
+            if (RF == 1) return;   // R2: no boundary items (TODO in the reference)
+            if (len == 0) return;  // WmR2 skips zero-length edges
+            len *= w0v - (X0 * (X0 + X1) + X1 * X1 + Y0 * (Y0 + Y1) + Y1 * Y1) / 3;
+        }
         perim += len;
         if (a->mom) {   // triangle (dirac, p0, p1): area cr/2, centroid (p0 + p1)/3
             const double cr = (double)x0 * y1 - (double)x1 * y0;
@@ -160,9 +175,15 @@ struct Integrator {
         T da = a1 - a0;
         if (a1 < a0) da = (a1 + T(TWO_PI)) - a0;
         double l = (double)da * (double)r;
-        arcm += l;
-        perim += l;
-        der_arc += 0.5 * (double)da;
+        double fac = 1;
+        if (RF) {   // sector: integral of |x - c|^2 = R^4 da / 4
+            const double R2 = (double)r2, r2s = R2 * R2 * (double)da / 4;
+            rint += RF == 1 ? r2s : w0v * R2 * (double)da / 2 - r2s;
+            fac = RF == 1 ? 0.0 : w0v - R2;   // WmR2 arc item: length x (w - R^2)
+        }
+        arcm += l * fac;
+        perim += l * fac;
+        der_arc += 0.5 * (double)da * fac;
         if (JAC) {   // ball item (get_der_centroids_and_integrals_wrt_weight_and_positions.h:66-114)
             const double R = (double)r, A0 = (double)a0, A1 = (double)a0 + (double)da;
             const double mm = 0.5 * a->coeff * (double)da;
@@ -300,7 +321,9 @@ SDOT_HD inline bool clip_line(const IdPoly2<PC> &in, IdPoly2<PC> &out, double la
 // then stay uniform values (scalar registers) instead of per-thread copies.
 // PER: the cell is also cut by the translated diracs (a.ntrans translations).
 // JAC: the position / weight derivative blocks (one domain shape, no pieces).
-template <class T, int CAP, bool BALL, bool IMG = false, bool BATCH = true, bool PER = false, bool JAC = false>
+// RF: the final integration function (Integrator).
+template <class T, int CAP, bool BALL, bool IMG = false, bool BATCH = true, bool PER = false, bool JAC = false,
+          int RF = 0>
 SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, int *I, const int S) {
 #define PX(k) X[(k) * S]
 #define PY(k) Y[(k) * S]
@@ -493,7 +516,8 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     // FP64 path: the vertices as clipped.  Mixed path: the FP32 clip fixed
     // which lines bound the cell; each vertex is rebuilt in FP64 as the
     // intersection of its two lines, so the integrals carry no FP32 error.
-    Integrator<double, PER, JAC> it;
+    Integrator<double, PER, JAC, RF> it;
+    it.w0v = w0;
     it.a = &a;
     it.t = t;
     it.c0x = c0x;
@@ -782,6 +806,10 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         it.arcm = l;
         it.perim = l;
         it.der_arc = PI_D;
+        if (RF) {   // the whole disc: pi R^4 / 2 (R2), pi R^2 (w - R^2 / 2) (WmR2), arcs of measure 0 (R = sqrt(w))
+            it.rint = RF == 1 ? 0.5 * PI_D * (double)w0 * (double)w0 : PI_D * (double)w0 * ((double)w0 - (double)w0 / 2);
+            it.arcm = it.perim = it.der_arc = 0;
+        }
         if (JAC) {   // the whole circle: a0 = 1 > a1 = 0 in the reference's boundary item
             const double mm = PI_D * a.coeff;
             it.jd[8] += mm;
@@ -801,7 +829,8 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     if (JAC)
         for (int k = 0; k < 9; ++k) a.jdiag[(int64_t)k * a.ell_stride + t] = it.jd[k];
     const double coeff = a.coeff;
-    const double mass = coeff * area;
+    const double mass = coeff * (RF ? it.rint : area);
+    if (RF == 2) it.der_line += coeff * area;   // integration_der_wrt_weight(WmR2) = the measure (:135-138)
     a.mass[t] = mass;
     if (a.mom) {
         a.mom[2 * t] = coeff * it.mx;

@@ -21,6 +21,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 
 #include "cell2d_impl.h"
 
@@ -35,7 +36,7 @@ using namespace c2;
 // device memory, grid-stride.  A cell that outgrows CAP goes to ovf_list for the
 // next tier; in the last tier (no ovf_list) it raises FLAG_CELL_OVERFLOW.
 // WAVES: the wavefronts per SIMD the register allocation must allow.
-template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER, bool JAC = false>
+template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER, bool JAC = false, int RF = 0>
 __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
     __shared__ T sx[CAP * BS];
     __shared__ T sy[CAP * BS];
@@ -54,8 +55,8 @@ __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
     }
     for (; k < count; k += step) {
         const int64_t t = a.work ? a.work[k] : k;
-        if (cell2_body<T, CAP, BALL, IMG, BATCH, PER, JAC>(a, t, sx + threadIdx.x, sy + threadIdx.x, sid + threadIdx.x,
-                                                          BS)) {
+        if (cell2_body<T, CAP, BALL, IMG, BATCH, PER, JAC, RF>(a, t, sx + threadIdx.x, sy + threadIdx.x,
+                                                              sid + threadIdx.x, BS)) {
             if (a.ovf_list) {
                 const unsigned q = atomicAdd(a.ovf_count, 1u);
                 a.ovf_list[q] = (int)t;
@@ -126,10 +127,11 @@ __global__ __launch_bounds__(BS) void k_csr_fill(const int *row_len, const int *
     }
 }
 
-template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER = false, bool JAC = false>
+template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER = false, bool JAC = false,
+          int RF = 0>
 void launch2(const CellArgs &a, unsigned nb, hipStream_t s) {
     if (nb == 0) return;
-    hipLaunchKernelGGL((k_cells2<T, CAP, BS, BALL, IMG, WAVES, BATCH, PER, JAC>), dim3(nb), dim3(BS), 0, s, a);
+    hipLaunchKernelGGL((k_cells2<T, CAP, BS, BALL, IMG, WAVES, BATCH, PER, JAC, RF>), dim3(nb), dim3(BS), 0, s, a);
     HIPCK(hipGetLastError());
 }
 
@@ -179,7 +181,22 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
         ++tier;
         return tier == 1 ? nb_all : std::min(nb_all, 4096u);
     };
-    if (a.jdiag) {
+    if (a.rf) {
+        // R2 / WmR2 integration functions: one FP64 tier of 64 vertices, one domain shape
+        unsigned nb = next(true);
+        auto go = [&](auto rfc) {
+            constexpr int RF = decltype(rfc)::value;
+            if (a.ntrans > 0) {
+                if (ball) launch2<double, 64, 64, true, false, 1, false, true, false, RF>(a, nb, s);
+                else launch2<double, 64, 64, false, false, 1, false, true, false, RF>(a, nb, s);
+            } else {
+                if (ball) launch2<double, 64, 64, true, false, 1, false, false, false, RF>(a, nb, s);
+                else launch2<double, 64, 64, false, false, 1, false, false, false, RF>(a, nb, s);
+            }
+        };
+        if (a.rf == 1) go(std::integral_constant<int, 1>());
+        else go(std::integral_constant<int, 2>());
+    } else if (a.jdiag) {
         // position / weight derivatives: one FP64 tier of 64 vertices, one domain shape
         unsigned nb = next(true);
         if (a.ntrans > 0) {

@@ -195,6 +195,8 @@ struct CellArgs {
     // the cut id j + (s + 1) N (single problems only; N (ntrans + 1) < 2^31)
     int ntrans;
     const double *trans;
+    // final integration function (2D, one domain shape): 0 Unit, 1 R2, 2 WmR2
+    int rf;
     // position / weight derivative blocks (9 doubles): per neighbour slot e at
     // jblk[(e * 9 + k) * ell_stride + t], the diagonal at jdiag[k * ell_stride + t]
     double *jblk, *jdiag;
@@ -362,6 +364,7 @@ struct sdot_pd {
     std::vector<double> htrans;                             // translations (dim per vector)
     sdot_amd::DevBuf<double> trans;
     bool want_jac = false;                                  // next pass writes the derivative blocks
+    int rf = 0;                                             // final integration function: 0 Unit, 1 R2, 2 WmR2
     sdot_amd::DevBuf<double> jblk, jdiag;
     sdot_amd::DevBuf<int> it_off;
     bool want_mom = false;          // the next pass computes the first moments

@@ -9,7 +9,8 @@
 //   ConvexPolyhedronAssembly<Pc>::add_convex_polyhedron / add_simplex / add_box / measure / normalize
 //                                                   src/sdot/Domains/ConvexPolyhedronAssembly.h:26-40
 //   ScaledImage<Pc>(min, max, data, sizes, nb_coeffs) src/sdot/Domains/ScaledImage.h:26
-//   FunctionEnum::Unit, FunctionEnum::InBallW05     src/sdot/Integration/Unit.h, InBallW05.h
+//   FunctionEnum::Unit, InBallW05, R2, R2InBallW05, WmR2, PpWmR2
+//                                                   src/sdot/Integration/*.h
 //   get_integrals(res, grid, bounds, pos, w, N, rf) src/sdot/PowerDiagram/get_integrals.h:13
 //   get_centroids(grid, bounds, pos, w, N, rf, cb)  src/sdot/PowerDiagram/get_centroids.h:14
 //   get_der_integrals_wrt_weights(...) -> int       src/sdot/PowerDiagram/get_der_integrals_wrt_weights.h:16
@@ -55,11 +56,24 @@ using Point2 = Point<2>;
 using Point3 = Point<3>;
 
 namespace FunctionEnum {
+// `code`: the sdot_pd_set_radial_func value; `ball`: need_ball_cut
 struct Unit {
-    static constexpr int ball = 0;
+    static constexpr int ball = 0, code = SDOT_FUNC_UNIT;
 };
 struct InBallW05 {   // clip each cell by the ball of radius sqrt(w_i)
-    static constexpr int ball = 1;
+    static constexpr int ball = 1, code = SDOT_FUNC_IN_BALL_W05;
+};
+struct R2 {          // |x - x_i|^2 (integrals only)
+    static constexpr int ball = 0, code = SDOT_FUNC_R2;
+};
+struct R2InBallW05 {
+    static constexpr int ball = 1, code = SDOT_FUNC_R2_IN_BALL_W05;
+};
+struct WmR2 {        // w_i - |x - x_i|^2
+    static constexpr int ball = 0, code = SDOT_FUNC_WMR2;
+};
+struct PpWmR2 {      // its positive part (ball cut)
+    static constexpr int ball = 1, code = SDOT_FUNC_PP_WMR2;
 };
 }  // namespace FunctionEnum
 
@@ -361,7 +375,7 @@ void get_integrals(TF *res, Grid &grid, Bounds &bounds, const Pt *positions, con
                    const Func &) {
     grid.update(positions, weights, nb_diracs, false, true);
     bounds.apply(grid.handle());
-    check(grid.handle(), sdot_pd_set_ball_clip(grid.handle(), Func::ball), "set_ball_clip");
+    check(grid.handle(), sdot_pd_set_radial_func(grid.handle(), Func::code), "radial function");
     check(grid.handle(), sdot_pd_integrals(grid.handle(), res), "get_integrals", true);
 }
 
@@ -379,7 +393,7 @@ void get_centroids(Grid &grid, Bounds &bounds, const Pt *positions, const TF *we
     constexpr int dim = sizeof(Pt) / sizeof(double);
     grid.update(positions, weights, nb_diracs, false, true);
     bounds.apply(grid.handle());
-    check(grid.handle(), sdot_pd_set_ball_clip(grid.handle(), Func::ball), "set_ball_clip");
+    check(grid.handle(), sdot_pd_set_radial_func(grid.handle(), Func::code), "radial function");
     std::vector<double> c((std::size_t)nb_diracs * dim), m(nb_diracs);
     check(grid.handle(), sdot_pd_centroids(grid.handle(), c.data(), m.data()), "get_centroids", true);
     for (std::size_t i = 0; i < (std::size_t)nb_diracs; ++i) {
@@ -404,7 +418,7 @@ int get_der_integrals_wrt_weights(std::vector<TI> &m_offsets, std::vector<TI> &m
     grid.update(positions, weights, nb_diracs, false, true);
     sdot_pd *pd = grid.handle();
     bounds.apply(pd);
-    check(pd, sdot_pd_set_ball_clip(pd, Func::ball), "set_ball_clip");
+    check(pd, sdot_pd_set_radial_func(pd, Func::code), "radial function");
     std::int64_t nnz = 0;
     int st = sdot_pd_der_integrals_wrt_weights(pd, nullptr, nullptr, nullptr, nullptr, 0, &nnz);
     check(pd, st, "get_der_integrals_wrt_weights", true);
@@ -441,7 +455,7 @@ int get_der_centroids_and_integrals_wrt_weight_and_positions(std::vector<TI> &m_
     grid.update(positions, weights, nb_diracs, false, true);
     sdot_pd *pd = grid.handle();
     bounds.apply(pd);
-    check(pd, sdot_pd_set_ball_clip(pd, Func::ball), "set_ball_clip");
+    check(pd, sdot_pd_set_radial_func(pd, Func::code), "radial function");
     std::int64_t nnz = 0;
     int st = sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions(pd, nullptr, nullptr, nullptr, nullptr, 0, &nnz);
     check(pd, st, "get_der_centroids_and_integrals_wrt_weight_and_positions", true);
@@ -472,7 +486,7 @@ public:
         grid.update(positions, weights, nb_diracs, true, true);
         sdot_pd *pd = grid.handle();
         bounds.apply(pd);
-        check(pd, sdot_pd_set_ball_clip(pd, RadialFunc::ball), "set_ball_clip");
+        check(pd, sdot_pd_set_radial_func(pd, RadialFunc::code), "radial function");
         int st = sdot_ot_solve(pd, masses, tol, stop_rule, (int)max_nb_iter, weights, &stats);
         if (st != SDOT_OK && st != SDOT_NOT_CONVERGED) check(pd, st, "OptimalTransportSolver::solve");
         return st;

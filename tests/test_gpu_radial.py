@@ -1,0 +1,99 @@
+"""R2 / R2InBallW05 / WmR2 / PpWmR2 on the device (sdot_pd_set_radial_func) against the
+oracle's restatement (ConvexPolyhedron2.tcc:391-430, 2044-2132): integrals for all four,
+Hessian for WmR2 / PpWmR2, periodic images, a Newton solve with WmR2, the closed forms of
+a centred dirac, and the rejected combinations."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_parity import compare, inputs
+
+pytestmark = pytest.mark.gpu
+T2 = [(x, y) for x in (-1, 0, 1) for y in (-1, 0, 1) if x or y]
+
+
+def make(name, precision="fp64", translations=None):
+    import sdot_amd
+    pd = sdot_amd.PowerDiagram(2, precision)
+    pd.add_box_shape([0, 0], [1, 1])
+    pd.set_radial_func(name)
+    if translations:
+        pd.set_translations(translations)
+    return pd
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", ["r2", "r2_in_ball_w05", "wmr2", "pp_wmr2"])
+def test_radial_parity(name, precision):
+    n = 4000
+    ball = name in ("r2_in_ball_w05", "pp_wmr2")
+    if ball:
+        pos, w = inputs(2, n, 91, wscale=1.0 / n, wshift=1.0 / n)
+    else:
+        pos, w = inputs(2, n, 92, wscale=0.3 / n)
+    radial = "r2" if name.startswith("r2") else "wmr2"
+    pd = make(name, precision)
+    pd.set_diracs(pos, w)
+    o = O.cells(pos, w, ball=ball, radial=radial)
+    m = pd.integrals()
+    scale = np.abs(o["masses"]).max()
+    assert np.abs(m - o["masses"]).max() < 1e-10 * scale
+    if radial == "wmr2":
+        off, cols, vals, m2 = pd.der_integrals_wrt_weights()
+        arc, per = pd.boundary_measures()
+        # boundary measures of w - r^2 change sign: compared against their largest value
+        for a, b in ((per, o["perimeter"]), (arc, o["ball_arc"])):
+            assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max() + 1e-18   # w - R^2 rounds to ~1e-21 on arcs
+        ones, zeros = np.ones(n), np.zeros(n)
+        g = dict(status=pd.last_status, masses=m2, ball_arc=zeros, perimeter=ones, csr=(off, cols, vals))
+        o2 = dict(status=o["status"], masses=o["masses"], ball_arc=zeros, perimeter=ones, csr=O.to_csr(o))
+        err = compare(g, o2, n, scale, hess_rtol=1e-6)
+        print(f"[radial {name}] {precision}: {err}")
+
+
+def test_radial_periodic_and_closed_forms():
+    n = 3000
+    pos, w = inputs(2, n, 93, wscale=0.3 / n)
+    pd = make("wmr2", translations=T2)
+    pd.set_diracs(pos, w)
+    o = O.cells(pos, w, radial="wmr2", translations=T2)
+    assert np.abs(pd.integrals() - o["masses"]).max() < 1e-10 * np.abs(o["masses"]).max()
+    pd = make("r2")
+    pd.set_diracs(np.array([[0.5, 0.5]]), np.array([0.3]))
+    assert abs(pd.integrals()[0] - 1 / 6) < 1e-15
+    pd = make("pp_wmr2")
+    pd.set_diracs(np.array([[0.5, 0.5]]), np.array([0.04]))
+    assert abs(pd.integrals()[0] - math.pi * 0.04 ** 2 / 2) < 1e-16
+
+
+def test_wmr2_newton_matches_the_oracle():
+    n = 2000
+    pos, _ = inputs(2, n, 94)
+    targets = np.full(n, 0.5 / n)   # integrals of w - r^2 over each cell
+    pd = make("wmr2")
+    pd.set_diracs(pos, np.zeros(n))
+    st, w, stats = pd.ot_solve(targets, np.full(n, 0.5), tol=1e-8)
+    assert st == 0, stats
+    o = O.cells(pos, w, radial="wmr2")
+    assert np.abs(o["masses"] / targets - 1).max() < 1e-6
+    so_st, _, so = O.ot_solve(pos, np.full(n, 0.5), targets, radial="wmr2", tol=1e-8)
+    assert so_st == O.OR_OK
+    print(f"[radial newton wmr2] iters {stats['newton_iters']} (oracle {so.newton_iters})")
+
+
+def test_radial_rejected_combinations():
+    import sdot_amd
+    pd = make("r2")
+    pd.set_diracs(np.array([[0.3, 0.5], [0.7, 0.5]]), np.zeros(2))
+    with pytest.raises(sdot_amd.SdotError):
+        pd.der_integrals_wrt_weights()
+    with pytest.raises(sdot_amd.SdotError):
+        pd.centroids()
+    pd = sdot_amd.PowerDiagram(3)
+    pd.add_box_shape([0, 0, 0], [1, 1, 1])
+    pd.set_radial_func("wmr2")
+    pd.set_diracs(np.array([[0.3, 0.5, 0.5], [0.7, 0.5, 0.5]]), np.zeros(2))
+    with pytest.raises(sdot_amd.SdotError):
+        pd.integrals()

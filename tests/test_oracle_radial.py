@@ -1,0 +1,71 @@
+"""The oracle's R2 / WmR2 integration functions (integration(Constant, R2 / WmR2),
+ConvexPolyhedron2.tcc:2044-2132; for_each_boundary_item(Constant, WmR2), :391-430;
+integration_der_wrt_weight(WmR2) = the measure, :135-138), pinned by closed forms (the
+second moment of a square about its centre, of a disc; w - r^2 over the disc of radius
+sqrt(w)), the sum over cells against the domain's own moment about each dirac computed by
+brute-force quadrature, and the Hessian against finite differences of the integrals."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import oracle as O
+
+
+def test_closed_forms():
+    # one dirac at the centre of the unit square: integral of |x - c|^2 = 1/6
+    r = O.cells(np.array([[0.5, 0.5]]), np.array([0.3]), radial="r2")
+    assert abs(r["masses"][0] - 1 / 6) < 1e-15
+    r = O.cells(np.array([[0.5, 0.5]]), np.array([0.3]), radial="wmr2")
+    assert abs(r["masses"][0] - (0.3 - 1 / 6)) < 1e-15
+    # off-centre dirac: (x0^3 + (1 - x0)^3) / 3 + (y0^3 + (1 - y0)^3) / 3
+    x0, y0 = 0.2, 0.7
+    r = O.cells(np.array([[x0, y0]]), np.array([0.0]), radial="r2")
+    assert abs(r["masses"][0] - ((x0 ** 3 + (1 - x0) ** 3) + (y0 ** 3 + (1 - y0) ** 3)) / 3) < 1e-15
+    # a disc inside the square: pi R^4 / 2 and pi R^2 (w - R^2 / 2) = pi w^2 / 2
+    w = 0.04
+    r = O.cells(np.array([[0.5, 0.5]]), np.array([w]), radial="r2", ball=True)
+    assert abs(r["masses"][0] - math.pi * w * w / 2) < 1e-16
+    r = O.cells(np.array([[0.5, 0.5]]), np.array([w]), radial="wmr2", ball=True)
+    assert abs(r["masses"][0] - math.pi * w * w / 2) < 1e-16
+    # d/dw of pi w^2 / 2 is the disc's area (the ball arcs carry w - R^2 = 0)
+    off, cols, vals = O.to_csr(r)
+    assert abs(vals[0] - math.pi * w) < 1e-15
+
+
+def test_sum_against_quadrature():
+    """sum_i integral over cell i of |x - x_i|^2 = integral over the square of the squared
+    distance to the nearest dirac (Voronoi, w = 0): midpoint quadrature on a fine grid."""
+    n = 12
+    u = O.rng_uniform(21, 2 * n)
+    pos = u.reshape(n, 2).copy()
+    r = O.cells(pos, np.zeros(n), radial="r2")
+    k = 1500
+    g = (np.arange(k) + 0.5) / k
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    d2 = np.min((X[..., None] - pos[:, 0]) ** 2 + (Y[..., None] - pos[:, 1]) ** 2, axis=2)
+    assert abs(r["masses"].sum() - d2.mean()) < 1e-6
+
+
+@pytest.mark.parametrize("ball", [False, True])
+def test_wmr2_hessian_against_finite_differences(ball):
+    n = 40
+    u = O.rng_uniform(9, 3 * n)
+    pos = u[: 2 * n].reshape(n, 2).copy()
+    w = 0.02 * u[2 * n:] + (0.01 if ball else 0.0)
+    r = O.cells(pos, w, radial="wmr2", ball=ball)
+    off, cols, vals = O.to_csr(r)
+    M = sp.csr_matrix((vals, cols, off), shape=(n, n)).toarray()
+    h = 1e-7
+    for j in range(n):
+        wp, wm = w.copy(), w.copy()
+        wp[j] += h
+        wm[j] -= h
+        fd = (O.cells(pos, wp, radial="wmr2", ball=ball)["masses"] - O.cells(pos, wm, radial="wmr2", ball=ball)["masses"]) / (2 * h)
+        assert np.abs(fd - M[:, j]).max() < 1e-7 * max(np.abs(M).max(), 1e-12)
+
+
+def test_r2_has_no_rows():
+    r = O.cells(np.array([[0.3, 0.5], [0.7, 0.5]]), np.zeros(2), radial="r2")
+    assert (r["row_len"] == 1).all() and (r["rvals"][[0, r["cap"]]] == 0).all()
