@@ -1300,8 +1300,36 @@ static int build_cell3(const cells_job *J, int64_t i, or_cp3 *cp) {
     return cp->nf == 0 ? OR_EMPTY_CELL : OR_OK;             /* ConvexPolyhedron3.h:142 */
 }
 
+/* integration(Constant, R2), ConvexPolyhedron3.tcc:1120-1123 (Gauss quadrature, exact for
+ * a quadratic): the integral of |x - c|^2 over the polyhedron as the tetrahedra (c, v0,
+ * vk, vk+1) of each face, each V (sum |a|^2 + sum a.b) / 10 over its corners a, b seen
+ * from c, V signed by the face's outward normal */
+static double cp3_r2(const or_cp3 *cp, const double *c) {
+    double res = 0;
+    for (int f = 0; f < cp->nf; ++f) {
+        const double (*v)[3] = cp->v[f];
+        for (int k = 2; k < cp->nv[f]; ++k) {
+            double a[3], b[3], e[3];
+            for (int d = 0; d < 3; ++d) { a[d] = v[0][d] - c[d]; b[d] = v[k - 1][d] - c[d]; e[d] = v[k][d] - c[d]; }
+            double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, q[3] = {e[0] - a[0], e[1] - a[1], e[2] - a[2]};
+            double cr[3] = {u[1] * q[2] - u[2] * q[1], u[2] * q[0] - u[0] * q[2], u[0] * q[1] - u[1] * q[0]};
+            double sg = cr[0] * cp->n[f][0] + cr[1] * cp->n[f][1] + cr[2] * cp->n[f][2] < 0 ? -1.0 : 1.0;
+            double V = sg * (a[0] * cr[0] + a[1] * cr[1] + a[2] * cr[2]) / 6;
+            double s2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + b[0] * b[0] + b[1] * b[1] + b[2] * b[2] + e[0] * e[0] +
+                        e[1] * e[1] + e[2] * e[2] + a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[0] * e[0] + a[1] * e[1] +
+                        a[2] * e[2] + b[0] * e[0] + b[1] * e[1] + b[2] * e[2];
+            res += V * s2 / 10;
+        }
+    }
+    return res;
+}
+
 static void piece3_accumulate(const cells_job *J, int64_t i, const or_cp3 *cp, double coeff, double *mass, double *perim,
                               row_acc *row, double *mom) {
+    if (((J->flags >> 4) & 3) == 1) {   /* R2: integrals only (no boundary items in the reference) */
+        *mass += coeff * cp3_r2(cp, J->pos + 3 * i);
+        return;
+    }
     *mass += coeff * or_cp3_volume(cp, J->pos + 3 * i);
     if (J->centroids) {
         double m0 = 0;

@@ -333,8 +333,8 @@ struct Cell3 {
     // face areas of the current polyhedron into m.acc[0..np), returns the volume; with
     // `off`, also the centroid offset from the dirac (signed tetrahedra: dirac, foot of
     // the face plane, directed edge; add_centroid_contrib, ConvexPolyhedron3.tcc:884-924)
-    SDOT_HD double integrate(double *off = nullptr) {
-        double tv = 0, tm[3] = {0, 0, 0};
+    SDOT_HD double integrate(double *off = nullptr, double *r2int = nullptr) {
+        double tv = 0, tm[3] = {0, 0, 0}, tr2 = 0;
         for (int k = 0; k < np; ++k) m.acc[k * m.S] = 0;
         if (m.jmom)
             for (int k = 0; k < np; ++k)
@@ -365,6 +365,14 @@ struct Cell3 {
                         for (int v = u; v < 3; ++v, ++c)
                             m.jmom[(f * 9 + c) * m.S] += sa / 12 * (F[u] * F[v] + P[u] * P[v] + Q[u] * Q[v] + S3[u] * S3[v]);
                 }
+                if (r2int) {   // tetrahedron (dirac, F, P, Q): V (|F|^2 + |P|^2 + |Q|^2 + F.P + F.Q + P.Q) / 10
+                    const double s = fh / n2, v = s * cn / 6;
+                    if (!off) tv += v;
+                    const double F[3] = {s * fx, s * fy, s * fz};
+                    tr2 += v * (F[0] * F[0] + F[1] * F[1] + F[2] * F[2] + x0 * x0 + y0 * y0 + z0 * z0 + x1 * x1 + y1 * y1 +
+                                z1 * z1 + F[0] * x0 + F[1] * y0 + F[2] * z0 + F[0] * x1 + F[1] * y1 + F[2] * z1 + x0 * x1 +
+                                y0 * y1 + z0 * z1) / 10;
+                }
                 if (off) {
                     const double s = fh / n2, v = s * cn / 6;
                     tv += v;
@@ -376,6 +384,7 @@ struct Cell3 {
         }
         if (off)
             for (int d = 0; d < 3; ++d) off[d] = tv != 0 ? tm[d] / tv : 0.0;
+        if (r2int) *r2int = tv < 0 ? -tr2 : tr2;   // the edges ran clockwise about the faces: every term flips
         double vol = 0;
         for (int k = 0; k < np; ++k) {
             if (m.jmom && m.acc[k * m.S] < 0)   // the face's edges ran clockwise: flip its moments
@@ -530,7 +539,12 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
     for (int k = 0; k < PCAP; ++k) C.m.hacc[k * C.m.S] = 0;
     const int npc = C.np;
     if (C.nv > 0) {
-        if (!a.img && a.nitems < 2) {
+        if (!a.img && a.nitems < 2 && a.rf == 1) {
+            // R2 (integration(Constant, R2), ConvexPolyhedron3.tcc:1120-1123): integrals only
+            double r2 = 0;
+            C.integrate(nullptr, &r2);
+            mass = a.coeff * r2;
+        } else if (!a.img && a.nitems < 2) {
             const double vol = C.integrate(want);
             mass = a.coeff * vol;
             if (want)
@@ -649,7 +663,7 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
     int nrow = 0, nfaces = 0;
     double jd[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     bool row_ovf = false;
-    for (int k = 0; k < npc; ++k) {
+    for (int k = 0; k < npc && a.rf != 1; ++k) {   // R2: no boundary items
         const double ar = C.m.hacc[k * C.m.S];
         const int id = C.m.pid[k * C.m.S];
         if (ar == 0 && C.nv > 0) {

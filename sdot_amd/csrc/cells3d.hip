@@ -980,9 +980,10 @@ void run_cells3(sdot_pd *pd, CellArgs &a) {
         }
         return;
     }
-    // translated diracs (periodic images): every cell on the thread kernel, whose
-    // traversal visits the grid once per translation; otherwise the wavefront kernel
-    if (a.ntrans > 0) {
+    // translated diracs (periodic images) or the R2 integrals: every cell on the thread
+    // kernel, whose traversal visits the grid once per translation; otherwise the
+    // wavefront kernel
+    if (a.ntrans > 0 || a.rf) {
         if (mixed) launch_thread3<float, VCAP3, PCAP3, false>(pd, a);
         else launch_thread3<double, VCAP3, PCAP3, false>(pd, a);
     } else {
@@ -997,7 +998,7 @@ void run_cells3(sdot_pd *pd, CellArgs &a) {
     // cells beyond 64 vertices / planes: the thread kernel, tier 1 then tier 2; each
     // tier works on a copy of the list while the next tier's list refills ovf_list
     int *list2 = pd->ovf_list.p + a.N + 1;
-    for (int tier = a.ntrans > 0 ? 2 : 1; tier <= 2 && cnt[1] > 0; ++tier) {
+    for (int tier = a.ntrans > 0 || a.rf ? 2 : 1; tier <= 2 && cnt[1] > 0; ++tier) {
         HIPCK(hipMemcpyAsync(list2, pd->ovf_list.p, cnt[1] * sizeof(int), hipMemcpyDeviceToDevice, s));
         fill_u32(pd->flags.p + 1, 1, 0u, s);
         a.work = list2;

@@ -97,3 +97,20 @@ def test_radial_rejected_combinations():
     pd.set_diracs(np.array([[0.3, 0.5, 0.5], [0.7, 0.5, 0.5]]), np.zeros(2))
     with pytest.raises(sdot_amd.SdotError):
         pd.integrals()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_r2_3d_parity(precision):
+    import sdot_amd
+    n = 3000
+    pos, w = inputs(3, n, 95, wscale=0.05 * n ** (-2 / 3))
+    pd = sdot_amd.PowerDiagram(3, precision)
+    pd.add_box_shape([0, 0, 0], [1, 1, 1])
+    pd.set_radial_func("r2")
+    pd.set_diracs(pos, w)
+    o = O.cells(pos, w, radial="r2")
+    m = pd.integrals()
+    assert np.abs(m - o["masses"]).max() < 1e-10 * np.abs(o["masses"]).max()
+    pd.set_diracs(np.array([[0.2, 0.7, 0.4]]), np.zeros(1))
+    x = np.array([0.2, 0.7, 0.4])
+    assert abs(pd.integrals()[0] - ((x ** 3 + (1 - x) ** 3) / 3).sum()) < 1e-15

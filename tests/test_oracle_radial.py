@@ -69,3 +69,20 @@ def test_wmr2_hessian_against_finite_differences(ball):
 def test_r2_has_no_rows():
     r = O.cells(np.array([[0.3, 0.5], [0.7, 0.5]]), np.zeros(2), radial="r2")
     assert (r["row_len"] == 1).all() and (r["rvals"][[0, r["cap"]]] == 0).all()
+
+
+def test_r2_3d_closed_forms_and_quadrature():
+    """3D R2 (integration(Constant, R2), ConvexPolyhedron3.tcc:1120-1123): a lone dirac in
+    the unit cube gives sum_d (x_d^3 + (1 - x_d)^3) / 3; Voronoi cells sum to the mean squared
+    distance to the nearest dirac (midpoint quadrature)."""
+    x = np.array([0.2, 0.7, 0.4])
+    r = O.cells(x[None, :], np.zeros(1), radial="r2")
+    assert abs(r["masses"][0] - ((x ** 3 + (1 - x) ** 3) / 3).sum()) < 1e-15
+    n = 8
+    pos = O.rng_uniform(31, 3 * n).reshape(n, 3).copy()
+    r = O.cells(pos, np.zeros(n), radial="r2")
+    k = 120
+    g = (np.arange(k) + 0.5) / k
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    d2 = np.min((X[..., None] - pos[:, 0]) ** 2 + (Y[..., None] - pos[:, 1]) ** 2 + (Z[..., None] - pos[:, 2]) ** 2, axis=3)
+    assert abs(r["masses"].sum() - d2.mean()) < 5e-5
