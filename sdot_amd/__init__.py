@@ -377,6 +377,25 @@ class PowerDiagram:
         return st, w, stats.as_dict()
 
 
+def read_xyw(path):
+    """The benchmarks' `.xyw` dirac files (benchmarks/set_up_diracs_2D.h:171-181): returns
+    (positions (n, 2), weights (n,)) from whitespace-separated "x y w" triples."""
+    v = np.loadtxt(path, dtype=np.float64, ndmin=2)
+    if v.size == 0:
+        return np.zeros((0, 2)), np.zeros(0)
+    if v.shape[1] != 3:
+        raise ValueError(f"{path}: expected 3 values per line")
+    return v[:, :2].copy(), v[:, 2].copy()
+
+
+def write_xyw(path, positions, weights):
+    p = np.asarray(positions, dtype=np.float64).reshape(-1, 2)
+    w = np.asarray(weights, dtype=np.float64).reshape(-1)
+    if len(p) != len(w):
+        raise ValueError("one weight per position")
+    np.savetxt(path, np.column_stack([p, w]), fmt="%.17g")
+
+
 class OptimalTransport:
     """pysdot-style front door (`OptimalTransport(positions).adjust_weights()`)."""
 

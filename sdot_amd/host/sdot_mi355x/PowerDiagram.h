@@ -27,6 +27,7 @@
 #include <array>
 #include <cmath>
 #include <cstddef>
+#include <cstdio>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -470,6 +471,33 @@ int get_der_centroids_and_integrals_wrt_weight_and_positions(std::vector<TI> &m_
     m_columns.assign(col.begin(), col.begin() + nnz);
     m_values.resize(nnz);
     return (st == SDOT_EMPTY_CELL && stop_if_void) ? 1 : 0;
+}
+
+/// The benchmarks' `.xyw` dirac files (benchmarks/set_up_diracs_2D.h:171-181): whitespace
+/// separated "x y w" triples, one dirac each.  Returns false when the file cannot be read.
+inline bool read_xyw(const std::string &path, std::vector<Point<2>> &positions, std::vector<double> &weights) {
+    std::FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return false;
+    positions.clear();
+    weights.clear();
+    double x, y, w;
+    while (std::fscanf(f, "%lf %lf %lf", &x, &y, &w) == 3) {
+        positions.push_back(Point<2>{{x, y}});
+        weights.push_back(w);
+    }
+    std::fclose(f);
+    return true;
+}
+
+inline bool write_xyw(const std::string &path, const std::vector<Point<2>> &positions, const std::vector<double> &weights) {
+    std::FILE *f = std::fopen(path.c_str(), "w");
+    if (!f || positions.size() != weights.size()) {
+        if (f) std::fclose(f);
+        return false;
+    }
+    for (std::size_t i = 0; i < positions.size(); ++i)
+        std::fprintf(f, "%.17g %.17g %.17g\n", positions[i][0], positions[i][1], weights[i]);
+    return std::fclose(f) == 0;
 }
 
 /// OptimalTransportSolver (OptimalTransportSolver.h:18-26): Newton on the weights with
