@@ -124,8 +124,9 @@ sdot_status sdot_pd_compute(sdot_pd *pd);
 /* get_integrals(res, ...) -- get_integrals.h:13.  Runs the pass if needed. */
 sdot_status sdot_pd_integrals(sdot_pd *pd, double *masses);
 /* get_der_centroids_and_integrals_wrt_weight_and_positions(m_offsets, m_columns, m_values,
- * v_values, ...) -- get_der_centroids_and_integrals_wrt_weight_and_positions.h:16 (one
- * domain shape, with translations, with the ball in 2D).  With nu = dim + 1: rows
+ * v_values, ...) -- get_der_centroids_and_integrals_wrt_weight_and_positions.h:16 (2D:
+ * one domain shape with or without the ball, several shapes or constant pixels; 3D: one
+ * shape; translations in both; quadratic pixels: SDOT_NOT_SUPPORTED).  With nu = dim + 1: rows
  * nu n + d (d < dim: centroid coordinate d; d = dim: mass of dirac n), columns nu m + e
  * (e < dim: position coordinate e; e = dim: weight of dirac m), sorted;
  * offsets[nu N + 1]; v_values[nu N] = centroids and masses.  As in the reference, the centroid rows are divided by mass + (mass == 1), and

@@ -292,7 +292,7 @@ def cells_sample_seconds(pos, w, i_begin, i_end, ball=False, nthreads=None, img=
     return dt, st, masses
 
 
-def der_pos(pos, w, ball=False, nthreads=None, domain=None, items=None, translations=None, cap=None):
+def der_pos(pos, w, ball=False, nthreads=None, domain=None, img=None, items=None, translations=None, cap=None):
     """get_der_centroids_and_integrals_wrt_weight_and_positions: returns (status, offsets,
     columns, values, v) in the reference's layout -- row nu n + d (centroid coordinates,
     then the mass of dirac n), column nu m + e (position coordinates, then the weight of
@@ -303,9 +303,9 @@ def der_pos(pos, w, ball=False, nthreads=None, domain=None, items=None, translat
     cap = cap or (64 if dim == 2 else 160)
     w = np.ascontiguousarray(w, dtype=np.float64)
     if domain is None:
-        dom, keep = make_domain(dim, items=items, translations=translations)
+        dom, keep = make_domain(dim, img=img, items=items, translations=translations)
     else:
-        dom, keep = make_domain(dim, *domain, items=items, translations=translations)
+        dom, keep = make_domain(dim, *domain, img=img, items=items, translations=translations)
     jlen = np.zeros(N, dtype=np.int64)
     jcols = np.zeros(N * cap, dtype=np.int64)
     jblk = np.zeros(N * cap * 16)

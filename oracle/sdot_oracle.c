@@ -1009,15 +1009,14 @@ static void jac_finish(cells_job *J, int64_t i, int64_t m, int dim, double mass,
  * momentum[a][b] = l / 6 (2 (p0a p0b + p1a p1b) + p0a p1b + p1a p0b), not scaled by the
  * density, measure = coeff l; arcs carry their angles), block entries [3 d + e] for
  * output d (centroid x, y, mass) and unknown e (position x, y, weight) */
-static void jac2_cell(cells_job *J, int64_t i, const or_cp2 *cp, double coeff, double mass, const double *cent) {
+static void jac2_items(cells_job *J, int64_t i, const or_cp2 *cp, double coeff, double *der0, int64_t *mp,
+                       int *overflow) {
     const int64_t cap = J->cap;
     int64_t *cols = J->jcols + i * cap;
     double *blk = J->jblocks + i * cap * 16;
-    int64_t m = 0;
-    double der0[9] = {0};
+    int64_t m = *mp;
     const double c0[2] = {J->pos[2 * i], J->pos[2 * i + 1]};
     const double R = sqrt(J->w[i]);   /* the ball's radius (:70), also for a void part whose sphere radius is 0 */
-    int overflow = 0;
     int nitem = cp->n ? cp->n : (cp->sr >= 0 ? 1 : 0);
     for (int k = 0; k < nitem; ++k) {
         int64_t id;
@@ -1096,15 +1095,24 @@ static void jac2_cell(cells_job *J, int64_t i, const or_cp2 *cp, double coeff, d
                 der0[3 * e + d] += (M[d][e] - C[e] * c0[d]) / dist;
                 d1[3 * e + d] = (C[e] * q[d] - M[d][e]) / dist;
             }
-        if (m >= cap) { overflow = 1; continue; }
+        if (m >= cap) { *overflow = 1; continue; }
         cols[m] = j;
         memcpy(blk + 16 * m, d1, sizeof d1);
         ++m;
     }
+    *mp = m;
+}
+
+/* one cell: the items of its pieces (one per domain shape or pixel under the cell, each
+ * with its density: for_each_intersection, get_der_centroids_and_integrals_wrt_weight_and_positions.h:63),
+ * then the diagonal block, sorting, merging and the correction */
+static void jac2_close(cells_job *J, int64_t i, const double *der0, int64_t m, int overflow, double mass,
+                       const double *cent) {
+    const int64_t cap = J->cap;
     if (m >= cap) overflow = 1;
     else {
-        cols[m] = i;
-        memcpy(blk + 16 * m, der0, sizeof der0);
+        J->jcols[i * cap + m] = i;
+        memcpy(J->jblocks + (i * cap + m) * 16, der0, 9 * sizeof(double));
         ++m;
     }
     jac_finish(J, i, m, 2, mass, cent, overflow);
@@ -1114,6 +1122,9 @@ static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
     row_acc row = { J->cols + i * J->cap, J->vals + i * J->cap, 1, J->cap, 0.0, 0 };
     row.cols[0] = i;
     double mass = 0, arcm = 0, perim = 0, mom[2] = {0, 0};
+    double jder0[9] = {0};
+    int64_t jm = 0;
+    int jovf = 0;
     int st = build_cell2(J, i, cp);
     const or_domain *dom = J->dom;
     if (dom->nitems >= 2) {
@@ -1126,6 +1137,7 @@ static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
                 if (cp->sr > 0) or_cp2_ball_cut(piece, cp->scx, cp->scy, cp->sr, cp->sid);
                 else { piece->scx = cp->scx; piece->scy = cp->scy; piece->sr = cp->sr; piece->sid = cp->sid; }
                 piece2_accumulate(J, i, piece, dom->it_coeff[k], NULL, &mass, &arcm, &perim, &row, mom);
+                if (J->jblocks) jac2_items(J, i, piece, dom->it_coeff[k], jder0, &jm, &jovf);
             }
     } else if (!dom->img) {
         piece2_accumulate(J, i, cp, dom->nitems == 1 ? dom->it_coeff[0] : dom->coeff, NULL, &mass, &arcm, &perim, &row,
@@ -1170,6 +1182,7 @@ static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
                     if (cp->sr > 0) or_cp2_ball_cut(piece, cp->scx, cp->scy, cp->sr, cp->sid);
                     else { piece->scx = cp->scx; piece->scy = cp->scy; piece->sr = cp->sr; piece->sid = cp->sid; }
                     piece2_accumulate(J, i, piece, rho, dom->img_nc == 6 ? c6 : NULL, &mass, &arcm, &perim, &row, mom);
+                    if (J->jblocks) jac2_items(J, i, piece, rho, jder0, &jm, &jovf);
                 }
         }
     }
@@ -1178,10 +1191,11 @@ static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
     if (J->centroids) {   /* get_centroids.h:53-58: centroid / (mass + (mass == 0)) */
         for (int d = 0; d < 2; ++d) J->centroids[2 * i + d] = mass != 0 ? J->pos[2 * i + d] + mom[d] / mass : 0.0;
     }
-    if (J->jblocks) {     /* one shape only (checked by or_der_pos2) */
+    if (J->jblocks) {     /* constant pieces only (checked by or_der_pos) */
         double cent[2];
         for (int d = 0; d < 2; ++d) cent[d] = mass != 0 ? J->pos[2 * i + d] + mom[d] / mass : 0.0;
-        jac2_cell(J, i, cp, dom->nitems == 1 ? dom->it_coeff[0] : dom->coeff, mass, cent);
+        if (!pieces_mode(dom)) jac2_items(J, i, cp, dom->nitems == 1 ? dom->it_coeff[0] : dom->coeff, jder0, &jm, &jovf);
+        jac2_close(J, i, jder0, jm, jovf, mass, cent);
     }
     if (J->ball_arc) J->ball_arc[i] = arcm;
     if (J->perimeter) J->perimeter[i] = perim;
@@ -1555,7 +1569,9 @@ static int cells_run(int dim, int64_t N, const double *pos, const double *w, con
 
 int or_der_pos(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom, int flags, int nthreads,
                int64_t cap, int64_t *jlen, int64_t *jcols, double *jblocks, double *jv) {
-    if (!dom || dom->dim != dim || dom->img || dom->nitems >= 2 || !jlen || !jcols || !jblocks || !jv) return OR_INVALID;
+    if (!dom || dom->dim != dim || !jlen || !jcols || !jblocks || !jv) return OR_INVALID;
+    if (dim == 3 && (dom->img || dom->nitems >= 2)) return OR_INVALID;   /* 3D: one shape */
+    if (dom->img && dom->img_nc != 1) return OR_INVALID;                /* constant pixels */
     double *masses = (double *)malloc(sizeof(double) * (N + 1));
     double *cent = (double *)malloc(sizeof(double) * (dim * N + 1));
     int64_t *row_len = (int64_t *)malloc(sizeof(int64_t) * (N + 1));

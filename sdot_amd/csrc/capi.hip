@@ -758,8 +758,12 @@ sdot_status sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions(sdot_pd
                                                                          double *vals, double *v_values, int64_t cap,
                                                                          int64_t *nnz) {
     if (!pd || !nnz || (cols && (!offsets || !vals))) return SDOT_INVALID_ARG;
-    if (pd->has_image || pd->shapes.size() != 1) {
-        pd->err = "position derivatives: one domain shape (no image, no assembly of several shapes)";
+    if (pd->dim == 3 && (pd->has_image || pd->shapes.size() > 1)) {
+        pd->err = "position derivatives in 3D: one domain shape";
+        return SDOT_NOT_SUPPORTED;
+    }
+    if (pd->has_image && pd->img_nc != 1) {
+        pd->err = "position derivatives of quadratic pixels are not offered";
         return SDOT_NOT_SUPPORTED;
     }
     return guard(pd, [&] {

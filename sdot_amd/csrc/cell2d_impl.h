@@ -75,6 +75,44 @@ constexpr double PI_D = 3.1415926535897932384626433832795;
 
 // Accumulates the boundary items of the final cell (line pieces with a cut id,
 // arcs of the ball) into the integrals and the Hessian row.
+// pos_integral C and momentum M of a straight boundary item p0 -> p1 (local coordinates
+// from the dirac at (c0x, c0y)) in absolute coordinates, added to C and M: the reference
+// does not scale them by the density (ConvexPolyhedron2.tcc:462-472), only the measure.
+SDOT_HD inline void edge_moments(double c0x, double c0y, double x0, double y0, double x1, double y1, double len,
+                                 double *C, double *M) {
+    const double P0[2] = {c0x + x0, c0y + y0}, P1[2] = {c0x + x1, c0y + y1};
+    for (int u = 0; u < 2; ++u) C[u] += 0.5 * len * (P0[u] + P1[u]);
+    for (int u = 0; u < 2; ++u)
+        for (int v = 0; v < 2; ++v) M[2 * u + v] += len / 6 * (2 * (P0[u] * P0[v] + P1[u] * P1[v]) + P0[u] * P1[v] + P1[u] * P0[v]);
+}
+
+// One straight boundary item of get_der_centroids_and_integrals_wrt_weight_and_positions
+// (:124-166) between the dirac (c0x, c0y) and the neighbour (or image) at q, with its
+// pos_integral C, momentum M (row-major) and measure.  The neighbour's block goes to ELL
+// slot `slot` (if within the row capacity), the cell's own image merges into jd, and jd
+// gets the diagonal terms.
+SDOT_HD inline void jac_edge(const CellArgs &a, int64_t t, double c0x, double c0y, const double *C, const double *M,
+                             double meas, const double *q, double dist, bool own, int slot, double *jd) {
+    const double c0[2] = {c0x, c0y};
+    double d1[9];
+    const double bw = 0.5 * meas / dist;
+    d1[8] = -bw;
+    for (int e = 0; e < 2; ++e) d1[3 * e + 2] = -0.5 * C[e] / dist;
+    for (int d = 0; d < 2; ++d) d1[6 + d] = (meas * q[d] - C[d]) / dist;
+    for (int d = 0; d < 2; ++d)
+        for (int e = 0; e < 2; ++e) d1[3 * e + d] = (C[e] * q[d] - M[2 * d + e]) / dist;
+    jd[8] += bw;
+    for (int e = 0; e < 2; ++e) jd[3 * e + 2] += 0.5 * C[e] / dist;
+    for (int d = 0; d < 2; ++d) jd[6 + d] += (C[d] - meas * c0[d]) / dist;
+    for (int d = 0; d < 2; ++d)
+        for (int e = 0; e < 2; ++e) jd[3 * e + d] += (M[2 * d + e] - C[e] * c0[d]) / dist;
+    if (own) {
+        for (int k = 0; k < 9; ++k) jd[k] += d1[k];
+    } else if (slot < a.ell_cap) {
+        for (int k = 0; k < 9; ++k) a.jblk[((int64_t)slot * 9 + k) * a.ell_stride + t] = d1[k];
+    }
+}
+
 // JAC: also the blocks of get_der_centroids_and_integrals_wrt_weight_and_positions
 // (get_der_centroids_and_integrals_wrt_weight_and_positions.h:46-166): one 3x3 block
 // per neighbour slot (a->jblk, beside the Hessian entry of that slot) and the
@@ -131,31 +169,9 @@ struct Integrator {
             const double vx = q[0] - c0x, vy = q[1] - c0y;
             const double dist = sqrt(vx * vx + vy * vy);
             if (JAC && dist != 0) {
-                // straight boundary item (ConvexPolyhedron2.tcc:437-477) in absolute
-                // coordinates: pos_integral C and momentum M carry no density, measure does
-                const double P0[2] = {c0x + (double)x0, c0y + (double)y0}, P1[2] = {c0x + (double)x1, c0y + (double)y1};
-                const double c0[2] = {c0x, c0y};
-                double C[2], M[2][2], d1[9];
-                for (int u = 0; u < 2; ++u) C[u] = 0.5 * len * (P0[u] + P1[u]);
-                for (int u = 0; u < 2; ++u)
-                    for (int v = 0; v < 2; ++v)
-                        M[u][v] = len / 6 * (2 * (P0[u] * P0[v] + P1[u] * P1[v]) + P0[u] * P1[v] + P1[u] * P0[v]);
-                const double meas = a->coeff * len, bw = 0.5 * meas / dist;
-                d1[8] = -bw;
-                for (int e = 0; e < 2; ++e) d1[3 * e + 2] = -0.5 * C[e] / dist;
-                for (int d = 0; d < 2; ++d) d1[6 + d] = (meas * q[d] - C[d]) / dist;
-                for (int d = 0; d < 2; ++d)
-                    for (int e = 0; e < 2; ++e) d1[3 * e + d] = (C[e] * q[d] - M[d][e]) / dist;
-                jd[8] += bw;
-                for (int e = 0; e < 2; ++e) jd[3 * e + 2] += 0.5 * C[e] / dist;
-                for (int d = 0; d < 2; ++d) jd[6 + d] += (C[d] - meas * c0[d]) / dist;
-                for (int d = 0; d < 2; ++d)
-                    for (int e = 0; e < 2; ++e) jd[3 * e + d] += (M[d][e] - C[e] * c0[d]) / dist;
-                if (PER && j == t) {   // the cell's own image: its block merges into the diagonal
-                    for (int k = 0; k < 9; ++k) jd[k] += d1[k];
-                } else if (nrow < a->ell_cap) {
-                    for (int k = 0; k < 9; ++k) a->jblk[((int64_t)nrow * 9 + k) * a->ell_stride + t] = d1[k];
-                }
+                double C[2] = {0, 0}, M[4] = {0, 0, 0, 0};
+                edge_moments(c0x, c0y, x0, y0, x1, y1, len, C, M);
+                jac_edge(*a, t, c0x, c0y, C, M, a->coeff * len, q, dist, PER && j == t, nrow, jd);
             }
             if (dist != 0 && (!PER || j != t)) {   // a face with the cell's own image adds nothing
                 double b = 0.5 * a->coeff * len / dist;
@@ -548,12 +564,19 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         constexpr int PC = CAP + 8;
         IdPoly2<PC> cell, pa, pb;
         double edge_rho_len[CAP];
+        bool edge_hit[CAP];   // crossed by a piece: only such edges are boundary items of the cell
+        double edge_c[JAC ? CAP : 1][2], edge_m[JAC ? CAP : 1][4];   // JAC: unweighted moments, summed over pieces
         double mn[2] = {DBL_MAX, DBL_MAX}, mx[2] = {-DBL_MAX, -DBL_MAX};
         cell.n = n;
         for (int k = 0; k < n; ++k) {
             vtx(k, cell.x[k], cell.y[k]);
             cell.e[k] = PI_(k);
             edge_rho_len[k] = 0;
+            edge_hit[k] = false;
+            if (JAC) {
+                edge_c[k][0] = edge_c[k][1] = 0;
+                edge_m[k][0] = edge_m[k][1] = edge_m[k][2] = edge_m[k][3] = 0;
+            }
             mn[0] = fmin(mn[0], cell.x[k]); mx[0] = fmax(mx[0], cell.x[k]);
             mn[1] = fmin(mn[1], cell.y[k]); mx[1] = fmax(mx[1], cell.y[k]);
         }
@@ -595,7 +618,12 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                 // density integrated along the edge piece (Simpson, exact for a quadratic)
                 const double dens = q ? (q_at(x0, y0) + 4 * q_at(0.5 * (x0 + x1), 0.5 * (y0 + y1)) + q_at(x1, y1)) / 6 : rho;
                 for (int j = 0; j < n; ++j)
-                    if (cell.e[j] == pc.e[k]) { edge_rho_len[j] += dens * len; break; }
+                    if (cell.e[j] == pc.e[k]) {
+                        edge_rho_len[j] += dens * len;
+                        edge_hit[j] = true;
+                        if (JAC) edge_moments(c0x, c0y, x0, y0, x1, y1, len, edge_c[j], edge_m[j]);
+                        break;
+                    }
             }
             mass_sum += q ? q[0] * m0 / 2 + q[1] * sx / 6 + q[2] * sy / 6 + q[3] * sxx / 12 + q[4] * sxy / 24 +
                                 q[5] * syy / 12
@@ -666,14 +694,17 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         double der_line = 0, perim = 0;
         int nrow = 0;
         bool row_ovf = false;
+        double jdp[JAC ? 9 : 1] = {};
         for (int k = 0; k < n; ++k) {
             const int id = cell.e[k];
-            if (id < 0) continue;
+            if (id < 0 || !edge_hit[k]) continue;
             perim += edge_rho_len[k];   // image density: faces shared with neighbours only
             double q[2];
             const int64_t j = cut_dirac<2, PER>(a, id, q);
             const double vx = q[0] - c0x, vy = q[1] - c0y;
             const double dist = sqrt(vx * vx + vy * vy);
+            if (JAC && dist != 0)   // the items of this edge's pieces together (the blocks are linear in them)
+                jac_edge(a, t, c0x, c0y, edge_c[k], edge_m[k], edge_rho_len[k], q, dist, PER && j == t, nrow, jdp);
             if (dist == 0 || (PER && j == t)) continue;
             const double b = 0.5 * edge_rho_len[k] / dist;
             if (nrow < a.ell_cap) {
@@ -690,6 +721,8 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             a.mom[2 * t] = mom_x;
             a.mom[2 * t + 1] = mom_y;
         }
+        if (JAC)
+            for (int k = 0; k < 9; ++k) a.jdiag[(int64_t)k * a.ell_stride + t] = jdp[k];
         a.arc[t] = 0;
         a.perim[t] = perim;
         a.diag[t] = der_line;

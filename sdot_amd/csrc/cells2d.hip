@@ -197,13 +197,16 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
         if (a.rf == 1) go(std::integral_constant<int, 1>());
         else go(std::integral_constant<int, 2>());
     } else if (a.jdiag) {
-        // position / weight derivatives: one FP64 tier of 64 vertices, one domain shape
+        // position / weight derivatives: one FP64 tier of 64 vertices (pieces: constant
+        // pixels or several shapes, no ball)
         unsigned nb = next(true);
         if (a.ntrans > 0) {
             if (ball) launch2<double, 64, 64, true, false, 1, false, true, true>(a, nb, s);
+            else if (img) launch2<double, 64, 64, false, true, 1, false, true, true>(a, nb, s);
             else launch2<double, 64, 64, false, false, 1, false, true, true>(a, nb, s);
         } else {
             if (ball) launch2<double, 64, 64, true, false, 1, false, false, true>(a, nb, s);
+            else if (img) launch2<double, 64, 64, false, true, 1, false, false, true>(a, nb, s);
             else launch2<double, 64, 64, false, false, 1, false, false, true>(a, nb, s);
         }
     } else if (mixed) {

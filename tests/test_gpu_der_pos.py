@@ -1,6 +1,7 @@
 """get_der_centroids_and_integrals_wrt_weight_and_positions on the device
 (sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions, 2D and 3D) against the oracle's
-restatement: box, ball, periodic images and a polygon domain, FP64 and mixed handles, the
+restatement: box, ball, periodic images, a polygon domain and domains of pieces (pixels,
+overlapping boxes), FP64 and mixed handles, the
 reference's 36-dirac configuration and a 1e5-dirac run checked by finite differences
 of the device's own centroids on a sample of columns."""
 import json
@@ -130,10 +131,44 @@ def test_der_pos_3d_parity(periodic, precision):
     print(f"[der_pos 3D periodic={periodic}] {precision}: rel err {err:.2e}, nnz {len(g[1])}")
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", ["image", "two_boxes", "periodic_image"])
+def test_der_pos_pieces_parity(case, precision):
+    """Pieces in 2D: the constant pixels of an image (densities in [1, 3]) or two
+    overlapping boxes of densities 1 and 3; each edge's items are summed over the pieces
+    it crosses, the moments unscaled by the density as in the reference."""
+    import sdot_amd
+    n = 3000
+    pos, w = inputs(2, n, 85, wscale=0.2 / n)
+    pd = sdot_amd.PowerDiagram(2, precision)
+    T = T2 if case == "periodic_image" else None
+    if case == "two_boxes":
+        lohi = [(np.array([0.0, 0.0]), np.array([0.6, 0.7]), 1.0), (np.array([0.3, 0.4]), np.array([1.0, 1.0]), 3.0)]
+        for lo, hi, c in lohi:
+            pd.add_box_shape(lo, hi, c)
+        kw = dict(items=[O.box_item(*x) for x in lohi])
+    else:
+        img = 1.0 + 2.0 * O.rng_uniform(86, 24 * 20).reshape(20, 24)
+        pd.set_image_density([0, 0], [1, 1], img)
+        kw = dict(img=img)
+    if T:
+        pd.set_translations(T)
+    pd.set_diracs(pos, w)
+    g = pd.der_centroids_and_integrals_wrt_weight_and_positions()
+    err = check(g, O.der_pos(pos, w, translations=T, **kw), n)
+    print(f"[der_pos pieces {case}] {precision}: rel err {err:.2e}, nnz {len(g[1])}")
+
+
 def test_der_pos_rejected_cases():
     import sdot_amd
     pd = sdot_amd.PowerDiagram(2)
-    pd.set_image_density([0, 0], [1, 1], np.ones((3, 3)))
+    pd.set_image_density([0, 0], [1, 1], np.ones((6, 3, 3)))
     pd.set_diracs(np.array([[0.3, 0.3], [0.6, 0.6]]), np.zeros(2))
+    with pytest.raises(sdot_amd.SdotError):
+        pd.der_centroids_and_integrals_wrt_weight_and_positions()
+    pd = sdot_amd.PowerDiagram(3)
+    pd.add_box_shape([0, 0, 0], [0.6, 1, 1])
+    pd.add_box_shape([0.4, 0, 0], [1, 1, 1])
+    pd.set_diracs(np.array([[0.3, 0.3, 0.3], [0.6, 0.6, 0.6]]), np.zeros(2))
     with pytest.raises(sdot_amd.SdotError):
         pd.der_centroids_and_integrals_wrt_weight_and_positions()

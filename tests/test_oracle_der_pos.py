@@ -121,3 +121,46 @@ def test_3d_against_finite_differences(periodic):
     J = dense(off, cols, vals, n, 4)
     F = finite_differences(pos, w, 1e-7, **kw)
     assert np.abs(J - F).max() < 1e-6 * np.abs(F).max()
+
+
+@pytest.mark.parametrize("case", ["image", "two_boxes"])
+def test_pieces_against_finite_differences(case):
+    """A domain of pieces (pixels of an image; two overlapping boxes, density 2 where
+    they overlap) of density 1 each: every piece adds its own items.  The reference
+    scales only the items' measure by the piece's density, not their pos_integral and
+    momentum (ConvexPolyhedron2.tcc:462-472), so finite differences pin the blocks where
+    each piece has density 1; other densities are pinned by the next test."""
+    n = 14
+    u = O.rng_uniform(112, 3 * n)
+    pos = u[: 2 * n].reshape(n, 2).copy()
+    w = 0.01 * u[2 * n:]
+    if case == "image":
+        kw = dict(img=np.ones((3, 4)))
+    else:
+        kw = dict(items=[O.box_item(np.array([0.0, 0.0]), np.array([0.6, 0.7]), 1.0),
+                         O.box_item(np.array([0.3, 0.4]), np.array([1.0, 1.0]), 1.0)])
+    st, off, cols, vals, v = O.der_pos(pos, w, **kw)
+    assert st == O.OR_OK
+    J = dense(off, cols, vals, n)
+    F = finite_differences(pos, w, 1e-7, **kw)
+    assert np.abs(J - F).max() < 1e-6 * np.abs(F).max()
+
+
+def test_pieces_of_one_density_match_the_whole_box():
+    """An image of 3 x 4 pixels of one density c and a box of coefficient c give the same
+    blocks: the pieces' items of an edge sum to the whole edge's.  With c != 1 the
+    reference's blocks differ from finite differences (measure scaled, moments not), which
+    this also shows, so a piece path that scaled the moments would fail here."""
+    n = 14
+    u = O.rng_uniform(114, 3 * n)
+    pos = u[: 2 * n].reshape(n, 2).copy()
+    w = 0.01 * u[2 * n:]
+    c = 2.5
+    st, off, cols, vals, v = O.der_pos(pos, w, img=np.full((3, 4), c))
+    st2, off2, cols2, vals2, v2 = O.der_pos(pos, w, items=[O.box_item(np.zeros(2), np.ones(2), c)])
+    assert st == st2 == O.OR_OK
+    J, J2 = dense(off, cols, vals, n), dense(off2, cols2, vals2, n)
+    assert np.abs(J - J2).max() < 1e-11 * np.abs(J2).max()
+    assert np.abs(v - v2).max() < 1e-13
+    F = finite_differences(pos, w, 1e-7, img=np.full((3, 4), c))
+    assert np.abs(J - F).max() > 0.1 * np.abs(F).max()
