@@ -19,6 +19,7 @@
 // offsets, distances and the mass/Hessian outputs in FP64).
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <type_traits>
@@ -79,51 +80,142 @@ __global__ void k_csr_count(const int *row_len, const int *inv_perm, int64_t n, 
 }
 
 // one row per thread, threads in grid order (the ELL slots [e * stride + t] of a
-// wavefront are contiguous): gather the row (columns mapped to dirac indices) and
-// the diagonal into this thread's LDS slot, insertion-sort by column, write the
-// dirac-order row i = perm[t] at its CSR offset
-template <int MAXE, int BS>
-__global__ __launch_bounds__(BS) void k_csr_fill(const int *row_len, const int *ell_col, const double *ell_val,
-                                                 int64_t stride, const double *diag, const int *perm,
-                                                 const int *inv_perm, int64_t n, const int64_t *off, int64_t *cols,
-                                                 double *vals) {
-    __shared__ int sc[MAXE * BS];
-    __shared__ double sv[MAXE * BS];
-    const int64_t t = blockIdx.x * (int64_t)BS + threadIdx.x;
+// wavefront are contiguous): the row (columns mapped to dirac indices) and the diagonal,
+// sorted by column with images of one dirac merged, written as the dirac-order row
+// i = perm[t] at its CSR offset (row_len + 1 places; merged duplicates leave padding,
+// marked by column -1).  Rows of up to R entries are sorted in registers (insertion with
+// static indices, no LDS: high occupancy for the gathers), as (column, value) pairs for
+// R <= 16, as packed (column, slot) keys above (values read when written); longer rows
+// are written unsorted and sorted in place in global memory.
+template <int R>
+__global__ __launch_bounds__(256) void k_csr_fill(const int *row_len, const int *ell_col, const double *ell_val,
+                                                  int64_t stride, const double *diag, const int *perm, int64_t n,
+                                                  const int64_t *off, int64_t *cols, double *vals) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= n) return;
-    (void)inv_perm;
-    int *c = sc + threadIdx.x;
-    double *v = sv + threadIdx.x;
     const int64_t i = perm[t];
     const int L = row_len[t];
-    int m = 0;
-    auto insert = [&](int cc, double vv) {
-        int k = m;
-        while (k > 0 && c[(k - 1) * BS] > cc) {
-            c[k * BS] = c[(k - 1) * BS];
-            v[k * BS] = v[(k - 1) * BS];
+    const int64_t o = off[i];
+    const int m = L + 1;
+    if (R <= 16 && m <= R) {
+        // columns and values in registers: the value loads stay coalesced over the wavefront
+        int c[R];
+        double v[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            c[k] = INT_MAX;   // dirac indices are below 2^31 - 1
+            v[k] = 0;
+        }
+        auto insert = [&](int cc, double vv) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const int oc = c[k];
+                const double ov = v[k];
+                const bool sw = oc > cc;
+                c[k] = sw ? cc : oc;
+                v[k] = sw ? vv : ov;
+                cc = sw ? oc : cc;
+                vv = sw ? ov : vv;
+            }
+        };
+        insert((int)i, diag[t]);
+        for (int e = 0; e < L; ++e) insert(perm[ell_col[e * stride + t]], ell_val[e * stride + t]);
+        int w = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (k < m) {
+                if (k + 1 < R && k + 1 < m && c[k] == c[k + 1]) {
+                    v[k + 1] += v[k];
+                } else {
+                    cols[o + w] = c[k];
+                    vals[o + w] = v[k];
+                    ++w;
+                }
+            }
+        }
+        for (int k = w; k < m; ++k) {
+            cols[o + k] = -1;
+            vals[o + k] = 0;
+        }
+        return;
+    }
+    if (R > 16 && m <= R) {
+        // keys column << 8 | ELL slot (slot 255: the diagonal; ELL rows hold at most 255)
+        uint64_t c[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) c[k] = ~0ull;
+        auto insert = [&](uint64_t key) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const uint64_t ok = c[k];
+                const bool sw = ok > key;
+                c[k] = sw ? key : ok;
+                key = sw ? ok : key;
+            }
+        };
+        insert((uint64_t)i << 8 | 255u);
+        for (int e = 0; e < L; ++e) insert((uint64_t)perm[ell_col[e * stride + t]] << 8 | (unsigned)e);
+        int w = 0;
+        int64_t pc = -1;
+        double acc = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (k < m) {
+                const int64_t col = (int64_t)(c[k] >> 8);
+                const int e = (int)(c[k] & 255u);
+                const double val = e == 255 ? diag[t] : ell_val[e * stride + t];
+                if (col == pc) {
+                    acc += val;
+                } else {
+                    if (pc >= 0) {
+                        cols[o + w] = pc;
+                        vals[o + w] = acc;
+                        ++w;
+                    }
+                    pc = col;
+                    acc = val;
+                }
+            }
+        }
+        cols[o + w] = pc;
+        vals[o + w] = acc;
+        ++w;
+        for (int k = w; k < m; ++k) {
+            cols[o + k] = -1;
+            vals[o + k] = 0;
+        }
+        return;
+    }
+    // long row: in place
+    int64_t *c = cols + o;
+    double *v = vals + o;
+    c[0] = i;
+    v[0] = diag[t];
+    for (int e = 0; e < L; ++e) {
+        const int64_t cc = perm[ell_col[e * stride + t]];
+        const double vv = ell_val[e * stride + t];
+        int k = e + 1;
+        while (k > 0 && c[k - 1] > cc) {
+            c[k] = c[k - 1];
+            v[k] = v[k - 1];
             --k;
         }
-        c[k * BS] = cc;
-        v[k * BS] = vv;
-        ++m;
-    };
-    insert((int)i, diag[t]);
-    for (int e = 0; e < L && m < MAXE; ++e) insert(perm[ell_col[e * stride + t]], ell_val[e * stride + t]);
-    const int64_t o = off[i];
+        c[k] = cc;
+        v[k] = vv;
+    }
     int w = 0;
     for (int k = 0; k < m; ++k) {
-        if (k + 1 < m && c[k * BS] == c[(k + 1) * BS]) {
-            v[(k + 1) * BS] += v[k * BS];
+        if (w > 0 && c[w - 1] == c[k]) {
+            v[w - 1] += v[k];
             continue;
         }
-        cols[o + w] = c[k * BS];
-        vals[o + w] = v[k * BS];
+        c[w] = c[k];
+        v[w] = v[k];
         ++w;
     }
-    for (int k = w; k < m; ++k) {   // merged duplicates leave padding: mark it
-        cols[o + k] = -1;
-        vals[o + k] = 0;
+    for (int k = w; k < m; ++k) {
+        c[k] = -1;
+        v[k] = 0;
     }
 }
 
@@ -406,18 +498,13 @@ int64_t build_csr_device(sdot_pd *pd) {
     const int64_t bound = (int64_t)N * (pd->ell_cap + 1);
     pd->csr_col.ensure(bound);
     pd->csr_val.ensure(bound);
-    if (pd->ell_cap + 1 <= 33)
-        hipLaunchKernelGGL((k_csr_fill<33, 64>), dim3((unsigned)((N + 63) / 64)), dim3(64), 0, s, pd->row_len.p,
-                           pd->ell_col.p, pd->ell_val.p, (int64_t)N, pd->diag.p, pd->perm.p, pd->inv_perm.p, N,
-                           pd->csr_off.p, pd->csr_col.p, pd->csr_val.p);
-    else if (pd->ell_cap + 1 <= 97)
-        hipLaunchKernelGGL((k_csr_fill<97, 32>), dim3((unsigned)((N + 31) / 32)), dim3(32), 0, s, pd->row_len.p,
-                           pd->ell_col.p, pd->ell_val.p, (int64_t)N, pd->diag.p, pd->perm.p, pd->inv_perm.p, N,
-                           pd->csr_off.p, pd->csr_col.p, pd->csr_val.p);
+    // register rows of 16 entries in 2D (about 7 on average), 32 in 3D (about 16)
+    if (pd->dim == 2)
+        hipLaunchKernelGGL((k_csr_fill<16>), dim3(nb), dim3(256), 0, s, pd->row_len.p, pd->ell_col.p, pd->ell_val.p,
+                           (int64_t)N, pd->diag.p, pd->perm.p, N, pd->csr_off.p, pd->csr_col.p, pd->csr_val.p);
     else
-        hipLaunchKernelGGL((k_csr_fill<256, 16>), dim3((unsigned)((N + 15) / 16)), dim3(16), 0, s, pd->row_len.p,
-                           pd->ell_col.p, pd->ell_val.p, (int64_t)N, pd->diag.p, pd->perm.p, pd->inv_perm.p, N,
-                           pd->csr_off.p, pd->csr_col.p, pd->csr_val.p);
+        hipLaunchKernelGGL((k_csr_fill<32>), dim3(nb), dim3(256), 0, s, pd->row_len.p, pd->ell_col.p, pd->ell_val.p,
+                           (int64_t)N, pd->diag.p, pd->perm.p, N, pd->csr_off.p, pd->csr_col.p, pd->csr_val.p);
     HIPCK(hipGetLastError());
     return bound;
 }
