@@ -95,8 +95,9 @@ sdot_status sdot_pd_clear_shapes(sdot_pd *pd);
  * ball of radius sqrt(w_i)) -- src/sdot/Integration/Unit.h, InBallW05.h */
 sdot_status sdot_pd_set_ball_clip(sdot_pd *pd, int on);
 /* Radial function (the Func argument of get_integrals / get_der_integrals_wrt_weights /
- * OptimalTransportSolver::solve), with one domain shape for the last four (R2 also in 3D,
- * ConvexPolyhedron3.tcc:1120-1123; WmR2 2D only, TODO in 3D there):
+ * OptimalTransportSolver::solve).  The last four in 2D: one domain shape; R2 / WmR2 also
+ * over constant pixels or several shapes.  R2 also in 3D on one shape
+ * (ConvexPolyhedron3.tcc:1120-1123); WmR2 2D only (TODO in 3D there):
  *   SDOT_FUNC_UNIT, SDOT_FUNC_IN_BALL_W05 -- Unit.h, InBallW05.h (as sdot_pd_set_ball_clip);
  *   SDOT_FUNC_R2, SDOT_FUNC_R2_IN_BALL_W05 -- |x - x_i|^2 (R2.h, R2InBallW05.h): integrals only
  *     (the reference has no R2 boundary items: ConvexPolyhedron2.tcc:432-435);

@@ -209,8 +209,8 @@ sdot_status compute_pass(sdot_pd *pd) {
     if (pd->ball && pd->dim == 3)
         throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping in 3D (ConvexPolyhedron3::ball_cut is TODO in the reference)"};
     if (pd->ball && pd->has_image) throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping with an image density"};
-    if (pd->rf && (pd->has_image || pd->shapes.size() > 1))
-        throw StatusError{SDOT_NOT_SUPPORTED, "R2 / WmR2 integration: one domain shape"};
+    if (pd->rf && ((pd->has_image && pd->img_nc != 1) || (pd->dim == 3 && (pd->has_image || pd->shapes.size() > 1))))
+        throw StatusError{SDOT_NOT_SUPPORTED, "R2 / WmR2 integration: constant pieces in 2D, one domain shape in 3D"};
     if (pd->rf == 2 && pd->dim == 3)
         throw StatusError{SDOT_NOT_SUPPORTED, "WmR2 in 3D (ConvexPolyhedron3 integration is TODO in the reference)"};
     if (pd->rf && (pd->want_mom || pd->want_jac))

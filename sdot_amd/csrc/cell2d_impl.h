@@ -580,7 +580,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             mn[0] = fmin(mn[0], cell.x[k]); mx[0] = fmax(mx[0], cell.x[k]);
             mn[1] = fmin(mn[1], cell.y[k]); mx[1] = fmax(mx[1], cell.y[k]);
         }
-        double mass_sum = 0, mom_x = 0, mom_y = 0;
+        double mass_sum = 0, mom_x = 0, mom_y = 0, rf_area = 0;
         bool fits = true;
         const bool poly = a.img && a.img_nc == 6;
         const IdPoly2<PC> *src = &cell;
@@ -607,16 +607,19 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                     sx += (x0 + x1) * cr;
                     sy += (y0 + y1) * cr;
                 }
-                if (q) {
+                if (q || RF) {
                     sxx += (x0 * x0 + x0 * x1 + x1 * x1) * cr;
                     syy += (y0 * y0 + y0 * y1 + y1 * y1) * cr;
                     sxy += (x0 * y1 + 2 * x0 * y0 + 2 * x1 * y1 + x1 * y0) * cr;
                 }
-                if (pc.e[k] < 0) continue;
+                if (pc.e[k] < 0 || RF == 1) continue;   // R2: no boundary items (TODO in the reference)
                 const double ex = x1 - x0, ey = y1 - y0;
                 const double len = sqrt(ex * ex + ey * ey);
-                // density integrated along the edge piece (Simpson, exact for a quadratic)
-                const double dens = q ? (q_at(x0, y0) + 4 * q_at(0.5 * (x0 + x1), 0.5 * (y0 + y1)) + q_at(x1, y1)) / 6 : rho;
+                // density integrated along the edge piece (Simpson, exact for a quadratic);
+                // WmR2: times the mean of w - |x - c|^2 over the edge
+                const double dens =
+                    (q ? (q_at(x0, y0) + 4 * q_at(0.5 * (x0 + x1), 0.5 * (y0 + y1)) + q_at(x1, y1)) / 6 : rho) *
+                    (RF == 2 ? w0 - (x0 * x0 + y0 * y0 + x0 * x1 + y0 * y1 + x1 * x1 + y1 * y1) / 3 : 1.0);
                 for (int j = 0; j < n; ++j)
                     if (cell.e[j] == pc.e[k]) {
                         edge_rho_len[j] += dens * len;
@@ -625,9 +628,15 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                         break;
                     }
             }
-            mass_sum += q ? q[0] * m0 / 2 + q[1] * sx / 6 + q[2] * sy / 6 + q[3] * sxx / 12 + q[4] * sxy / 24 +
-                                q[5] * syy / 12
-                          : rho * 0.5 * m0;
+            if (RF) {   // constant pieces: the second moment about the dirac, (w - |x - c|^2) for WmR2
+                const double r2 = (sxx + syy) / 12;
+                mass_sum += rho * (RF == 1 ? r2 : w0 * 0.5 * m0 - r2);
+                rf_area += rho * 0.5 * m0;
+            } else {
+                mass_sum += q ? q[0] * m0 / 2 + q[1] * sx / 6 + q[2] * sy / 6 + q[3] * sxx / 12 + q[4] * sxy / 24 +
+                                    q[5] * syy / 12
+                              : rho * 0.5 * m0;
+            }
             if (!q) {   // first moments of a constant piece (quadratic pixels: not offered)
                 mom_x += rho * sx / 6;
                 mom_y += rho * sy / 6;
@@ -725,7 +734,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             for (int k = 0; k < 9; ++k) a.jdiag[(int64_t)k * a.ell_stride + t] = jdp[k];
         a.arc[t] = 0;
         a.perim[t] = perim;
-        a.diag[t] = der_line;
+        a.diag[t] = der_line + (RF == 2 ? rf_area : 0.0);   // integration_der_wrt_weight(WmR2) = the measure
         a.row_len[t] = nrow;
         if (a.ctr) {
             const unsigned long long K = n;

@@ -274,15 +274,18 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
         return tier == 1 ? nb_all : std::min(nb_all, 4096u);
     };
     if (a.rf) {
-        // R2 / WmR2 integration functions: one FP64 tier of 64 vertices, one domain shape
+        // R2 / WmR2 integration functions: one FP64 tier of 64 vertices (pieces: constant
+        // pixels or several shapes, no ball)
         unsigned nb = next(true);
         auto go = [&](auto rfc) {
             constexpr int RF = decltype(rfc)::value;
             if (a.ntrans > 0) {
                 if (ball) launch2<double, 64, 64, true, false, 1, false, true, false, RF>(a, nb, s);
+                else if (img) launch2<double, 64, 64, false, true, 1, false, true, false, RF>(a, nb, s);
                 else launch2<double, 64, 64, false, false, 1, false, true, false, RF>(a, nb, s);
             } else {
                 if (ball) launch2<double, 64, 64, true, false, 1, false, false, false, RF>(a, nb, s);
+                else if (img) launch2<double, 64, 64, false, true, 1, false, false, false, RF>(a, nb, s);
                 else launch2<double, 64, 64, false, false, 1, false, false, false, RF>(a, nb, s);
             }
         };

@@ -114,3 +114,40 @@ def test_r2_3d_parity(precision):
     pd.set_diracs(np.array([[0.2, 0.7, 0.4]]), np.zeros(1))
     x = np.array([0.2, 0.7, 0.4])
     assert abs(pd.integrals()[0] - ((x ** 3 + (1 - x) ** 3) / 3).sum()) < 1e-15
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", ["pixels", "boxes"])
+@pytest.mark.parametrize("name", ["r2", "wmr2"])
+def test_radial_over_pieces_parity(name, case, precision):
+    """R2 / WmR2 over constant pixels (24 x 20, densities in [1, 3]) or two overlapping
+    boxes of densities 1 and 3: integrals, and for WmR2 the Hessian and the boundary
+    measures."""
+    import sdot_amd
+    n = 3000
+    pos, w = inputs(2, n, 93, wscale=0.3 / n)
+    pd = sdot_amd.PowerDiagram(2, precision)
+    if case == "pixels":
+        img = 1.0 + 2.0 * O.rng_uniform(94, 24 * 20).reshape(20, 24)
+        pd.set_image_density([0, 0], [1, 1], img)
+        kw = dict(img=img)
+    else:
+        lohi = [(np.array([0.0, 0.0]), np.array([0.6, 0.7]), 1.0), (np.array([0.3, 0.4]), np.array([1.0, 1.0]), 3.0)]
+        for lo, hi, c in lohi:
+            pd.add_box_shape(lo, hi, c)
+        kw = dict(items=[O.box_item(*x) for x in lohi])
+    pd.set_radial_func(name)
+    pd.set_diracs(pos, w)
+    o = O.cells(pos, w, radial=name, **kw)
+    m = pd.integrals()
+    scale = np.abs(o["masses"]).max()
+    assert np.abs(m - o["masses"]).max() < 1e-10 * scale
+    if name == "wmr2":
+        off, cols, vals, m2 = pd.der_integrals_wrt_weights(allow_empty=True)
+        arc, per = pd.boundary_measures()
+        assert np.abs(per - o["perimeter"]).max() <= 1e-10 * np.abs(o["perimeter"]).max()
+        ones, zeros = np.ones(n), np.zeros(n)
+        g = dict(status=pd.last_status, masses=m2, ball_arc=zeros, perimeter=ones, csr=(off, cols, vals))
+        o2 = dict(status=o["status"], masses=o["masses"], ball_arc=zeros, perimeter=ones, csr=O.to_csr(o))
+        err = compare(g, o2, n, scale, hess_rtol=1e-6)
+        print(f"[radial pieces {name} {case}] {precision}: {err}")

@@ -86,3 +86,62 @@ def test_r2_3d_closed_forms_and_quadrature():
     X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
     d2 = np.min((X[..., None] - pos[:, 0]) ** 2 + (Y[..., None] - pos[:, 1]) ** 2 + (Z[..., None] - pos[:, 2]) ** 2, axis=3)
     assert abs(r["masses"].sum() - d2.mean()) < 5e-5
+
+
+def _pieces(case):
+    if case == "pixels":
+        return dict(img=1.0 + 2.0 * O.rng_uniform(122, 12).reshape(3, 4))
+    return dict(items=[O.box_item(np.array([0.0, 0.0]), np.array([0.6, 0.7]), 1.0),
+                       O.box_item(np.array([0.3, 0.4]), np.array([1.0, 1.0]), 3.0)])
+
+
+def _density(case, P):
+    if case == "pixels":
+        img = _pieces(case)["img"]
+        return img[np.minimum((P[:, 1] * 3).astype(int), 2), np.minimum((P[:, 0] * 4).astype(int), 3)]
+    a = ((P >= [0.0, 0.0]) & (P <= [0.6, 0.7])).all(1)
+    b = ((P >= [0.3, 0.4]) & (P <= [1.0, 1.0])).all(1)
+    return 1.0 * a + 3.0 * b
+
+
+@pytest.mark.parametrize("case", ["pixels", "boxes"])
+@pytest.mark.parametrize("radial", ["r2", "wmr2"])
+def test_radial_over_pieces_against_quadrature(radial, case):
+    """R2 / WmR2 over pieces of constant density (ConvexPolyhedron2 integration per piece
+    of for_each_intersection): the masses against a midpoint rule on a 1000 x 1000 grid
+    of the power-cell assignment."""
+    n = 12
+    u = O.rng_uniform(121, 3 * n)
+    pos = u[: 2 * n].reshape(n, 2).copy()
+    w = 0.01 * u[2 * n:]
+    r = O.cells(pos, w, radial=radial, **_pieces(case))
+    M = 1000
+    xs = (np.arange(M) + 0.5) / M
+    X, Y = np.meshgrid(xs, xs)
+    P = np.stack([X.ravel(), Y.ravel()], 1)
+    d2 = ((P[:, None, :] - pos[None]) ** 2).sum(-1)
+    own = np.argmin(d2 - w[None], 1)
+    f = d2[np.arange(len(P)), own]
+    if radial == "wmr2":
+        f = w[own] - f
+    q = np.bincount(own, weights=_density(case, P) * f, minlength=n) / M ** 2
+    assert np.abs(r["masses"] - q).max() < 3e-3 * np.abs(q).max()
+
+
+@pytest.mark.parametrize("case", ["pixels", "boxes"])
+def test_wmr2_hessian_over_pieces_against_finite_differences(case):
+    n = 16
+    u = O.rng_uniform(123, 3 * n)
+    pos = u[: 2 * n].reshape(n, 2).copy()
+    w = 0.02 * u[2 * n:]
+    kw = _pieces(case)
+    r = O.cells(pos, w, radial="wmr2", **kw)
+    off, cols, vals = O.to_csr(r)
+    H = sp.csr_matrix((vals, cols, off), shape=(n, n)).toarray()
+    h = 1e-7
+    for j in range(n):
+        wp, wm = w.copy(), w.copy()
+        wp[j] += h
+        wm[j] -= h
+        fd = (O.cells(pos, wp, radial="wmr2", **kw)["masses"] - O.cells(pos, wm, radial="wmr2", **kw)["masses"]) / (2 * h)
+        assert np.abs(fd - H[:, j]).max() < 1e-7 * max(np.abs(H).max(), 1e-12)
