@@ -151,3 +151,26 @@ def test_radial_over_pieces_parity(name, case, precision):
         o2 = dict(status=o["status"], masses=o["masses"], ball_arc=zeros, perimeter=ones, csr=O.to_csr(o))
         err = compare(g, o2, n, scale, hess_rtol=1e-6)
         print(f"[radial pieces {name} {case}] {precision}: {err}")
+
+
+def test_wmr2_newton_over_pixels_matches_the_oracle():
+    """The WmR2 Newton solve on an image of constant pixels (densities in [1, 3]): the
+    converged weights give every cell its target integral (checked by the oracle's own
+    pass).  The WmR2 Hessian can be indefinite (edge measures w - r^2 of either sign), so
+    the solve is delicate; this configuration converges in the oracle too."""
+    import sdot_amd
+    n = 2000
+    pos, _ = inputs(2, n, 94)
+    img = 1.0 + 2.0 * O.rng_uniform(96, 16 * 12).reshape(12, 16)
+    targets = np.full(n, 1.0 / n)
+    pd = sdot_amd.PowerDiagram(2, "fp64")
+    pd.set_image_density([0, 0], [1, 1], img)
+    pd.set_radial_func("wmr2")
+    pd.set_diracs(pos, np.zeros(n))
+    st, w, stats = pd.ot_solve(targets, np.full(n, 0.5), tol=1e-8)
+    assert st == 0, stats
+    o = O.cells(pos, w, radial="wmr2", img=img)
+    assert np.abs(o["masses"] / targets - 1).max() < 1e-6
+    so_st, _, so = O.ot_solve(pos, np.full(n, 0.5), targets, radial="wmr2", img=img, tol=1e-8)
+    assert so_st == O.OR_OK
+    print(f"[radial newton wmr2 pixels] iters {stats['newton_iters']} (oracle {so.newton_iters})")
