@@ -124,6 +124,21 @@ sdot_status sdot_pd_compute(sdot_pd *pd);
 
 /* get_integrals(res, ...) -- get_integrals.h:13.  Runs the pass if needed. */
 sdot_status sdot_pd_integrals(sdot_pd *pd, double *masses);
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid allocation after set-up): the masses of
+ * sdot_pd_integrals for n diracs given as device arrays (d_positions 2n interleaved,
+ * d_weights n), written to the device array d_masses (n, dirac order), with no device
+ * allocation and no host allocation.  The handle gives the settings (2D, precision, one
+ * box shape, ball clip on or off; no image, translations or radial function) and its
+ * stream, and keeps none of the results (its own pass is left as it was).  Every
+ * intermediate array lives in one fixed 40 MiB device workspace per device, held
+ * until the process ends: n <= 262144 and a grid of at most 2^20 Morton codes
+ * (a dirac bounding box of aspect ratio up to about 4 at the largest n), else
+ * SDOT_CAPACITY_OVERFLOW.  Calls on one device share the workspace and are
+ * serialised by a lock inside the library; the call returns when the masses are
+ * written.  Status as sdot_pd_integrals. */
+sdot_status sdot_pd_integrals_noalloc(sdot_pd *pd, const double *d_positions, const double *d_weights,
+                                      int64_t nb_diracs, double *d_masses);
 /* get_der_centroids_and_integrals_wrt_weight_and_positions(m_offsets, m_columns, m_values,
  * v_values, ...) -- get_der_centroids_and_integrals_wrt_weight_and_positions.h:16 (2D:
  * one domain shape with or without the ball, several shapes or constant pixels; 3D: one

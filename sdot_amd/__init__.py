@@ -110,6 +110,7 @@ SIGNATURES = {
     "sdot_pd_set_radial_func": (_int, [_p, _int]),
     "sdot_pd_compute": (_int, [_p]),
     "sdot_pd_integrals": (_int, [_p, _p]),
+    "sdot_pd_integrals_noalloc": (_int, [_p, _p, _p, _i64, _p]),
     "sdot_pd_add_convex_polyhedron_shape": (_int, [_p, _p, _p, _i64, _d, _i64]),
     "sdot_pd_add_simplex_shape": (_int, [_p, _p, _d, _i64]),
     "sdot_pd_centroids": (_int, [_p, _p, _p]),
@@ -298,6 +299,13 @@ class PowerDiagram:
         m = np.empty(self.N)
         self._check(lib().sdot_pd_integrals(self._h, _ptr(m)), "integrals", allow_empty)
         return m
+
+    def integrals_noalloc(self, pos_ptr, w_ptr, n, masses_ptr, allow_empty=True):
+        """Masses of n diracs given as device pointers (positions 2n interleaved, weights n)
+        into the device array masses_ptr (n), with this handle's settings and no device
+        allocation (2D, one box shape, n <= 262144); the handle's own pass is untouched."""
+        return self._check(lib().sdot_pd_integrals_noalloc(self._h, _p(pos_ptr), _p(w_ptr), int(n), _p(masses_ptr)),
+                           "integrals_noalloc", allow_empty)
 
     def centroids(self, allow_empty=True):
         """get_centroids (rand_ratio 0): (centroids[N, dim], masses[N]); 0 for empty cells."""

@@ -15,34 +15,6 @@ using namespace sdot_amd;
 
 namespace {
 
-
-template <class F>
-sdot_status guard(sdot_pd *pd, F &&f) {
-    try {
-        return f();
-    } catch (const HipError &e) {
-        if (pd) {
-            char buf[512];
-            snprintf(buf, sizeof buf, "HIP error %d (%s) in %s (line %d)", (int)e.err, hipGetErrorString(e.err), e.what,
-                     e.line);
-            pd->err = buf;
-        }
-        return SDOT_HIP_ERROR;
-    } catch (const OomError &) {
-        if (pd) pd->err = "device allocation failed";
-        return SDOT_OOM;
-    } catch (const StatusError &e) {
-        if (pd) pd->err = e.msg;
-        return e.st;
-    } catch (const std::string &e) {
-        if (pd) pd->err = e;
-        return SDOT_INVALID_ARG;
-    } catch (const std::bad_alloc &) {
-        if (pd) pd->err = "host allocation failed";
-        return SDOT_OOM;
-    }
-}
-
 double elapsed_ms(hipEvent_t a, hipEvent_t b) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
@@ -368,6 +340,7 @@ sdot_status sdot_pd_create(sdot_pd **out, int dim, int precision) {
         int ndev = 0;
         HIPCK(hipGetDeviceCount(&ndev));
         if (ndev < 1) throw StatusError{SDOT_HIP_ERROR, "no HIP device"};
+        HIPCK(hipGetDevice(&pd->device));
         HIPCK(hipStreamCreateWithFlags(&pd->stream, hipStreamNonBlocking));
         return SDOT_OK;
     });

@@ -889,7 +889,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         ctr_add(a.ctr + WCTR + slot, K);
         ctr_add(a.ctr + 2 * WCTR + slot, A);
     }
-    if (it.row_ovf) flag_or(a.flags, FLAG_ROW_OVERFLOW);
+    if (it.row_ovf && a.ell_cap > 0) flag_or(a.flags, FLAG_ROW_OVERFLOW);   // ell_cap 0: a pass without rows
     if (mass == 0) {
         flag_or(a.flags, FLAG_EMPTY);
         if (a.pflags) flag_or(a.pflags + pv.p, FLAG_EMPTY);

@@ -317,7 +317,7 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
         nb = next(true);
         dispatch2<double, 64, 1>(ball, img, a, nb, s);
     }
-    HIPCK(hipEventRecord(pd->ev[3], s));
+    if (pd->ev[3]) HIPCK(hipEventRecord(pd->ev[3], s));
     a.work = nullptr;
     a.work_count = nullptr;
 }
@@ -475,6 +475,13 @@ void export_centroids(sdot_pd *pd, double *h_out) {
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(h_out, pd->out_tmp.p, (size_t)dim * N * sizeof(double), hipMemcpyDeviceToHost, pd->stream));
     HIPCK(hipStreamSynchronize(pd->stream));
+}
+
+void export_vec_device(sdot_pd *pd, const double *d_sorted, double *d_out) {
+    const int64_t N = pd->N;
+    hipLaunchKernelGGL(k_export_vec, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, pd->stream, d_sorted, pd->inv_perm.p,
+                       N, d_out);
+    HIPCK(hipGetLastError());
 }
 
 void export_vec(sdot_pd *pd, const double *d_sorted, double *h_out) {

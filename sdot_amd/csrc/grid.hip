@@ -12,7 +12,7 @@ namespace sdot_amd {
 
 namespace {
 
-constexpr int RED_BLOCKS = 256;
+constexpr int RED_BLOCKS = BBOX_BLOCKS;
 constexpr int BS = 256;
 
 template <int DIM>
@@ -238,9 +238,9 @@ void build_grid(sdot_pd *pd) {
     else
         hipLaunchKernelGGL(k_bbox<3>, dim3(RED_BLOCKS), dim3(BS), 0, s, pd->pos.p, N, pd->bbox_part.p, pd->flags.p);
     HIPCK(hipGetLastError());
-    std::vector<double> part(RED_BLOCKS * 2 * dim);
+    double part[RED_BLOCKS * 2 * 3];   // on the stack: the grid build allocates nothing on the host
     unsigned fl = 0;
-    HIPCK(hipMemcpyAsync(part.data(), pd->bbox_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(part, pd->bbox_part.p, RED_BLOCKS * 2 * dim * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCK(hipMemcpyAsync(&fl, pd->flags.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
     if (fl & FLAG_BAD_INPUT) throw std::string("non-finite dirac position");

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
+#include <new>
 #include <mutex>
 #include <type_traits>
 #include <string>
@@ -43,13 +45,21 @@ struct StatusError {
 // records a graph (the capture holds the stream the synchronisation waits on).
 std::recursive_mutex &device_alloc_mutex();
 
-// grow-only device buffer
+// grow-only device buffer; a borrowed one points into memory it does not own
+// (a caller's array or a slice of a fixed workspace) and never allocates or frees
 template <class T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    bool borrowed = false;
+    void borrow(T *q, size_t m) {
+        p = q;
+        n = m;
+        borrowed = true;
+    }
     void ensure(size_t m) {
         if (m <= n && p) return;
+        if (borrowed) throw StatusError{SDOT_CAPACITY_OVERFLOW, "fixed workspace too small for this input"};
         std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
         if (p) HIPCK(hipFree(p));
         p = nullptr;
@@ -63,12 +73,13 @@ struct DevBuf {
         n = want;
     }
     void release() {
-        if (p) {
+        if (p && !borrowed) {
             std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
             (void)hipFree(p);
         }
         p = nullptr;
         n = 0;
+        borrowed = false;
     }
 };
 
@@ -309,6 +320,7 @@ SDOT_HD inline uint32_t compact3(uint32_t x) {
     return x;
 }
 constexpr int WCTR = 1024;       // counters are spread over WCTR slots to avoid contention
+constexpr int BBOX_BLOCKS = 256; // blocks of the bounding-box reduction (2 dim partial values each)
 
 // ---------------------------------------------------------------------------
 // the handle
@@ -333,6 +345,7 @@ struct sdot_pd {
     int dpc_x100 = 0;  // tuning: diracs per grid cell x100 (0 = auto)
     int block = 0;     // tuning: threads per block (0 = auto)
     hipStream_t stream = nullptr;
+    int device = 0;    // the device current at creation (the stream's)
     std::string err;
 
     // domain
@@ -397,6 +410,35 @@ struct sdot_pd {
 };
 
 namespace sdot_amd {
+// runs an entry point's body: HIP failures, invalid input and capacity errors
+// become status codes, with the text on the handle
+template <class F>
+sdot_status guard(sdot_pd *pd, F &&f) {
+    try {
+        return f();
+    } catch (const HipError &e) {
+        if (pd) {
+            char buf[512];
+            snprintf(buf, sizeof buf, "HIP error %d (%s) in %s (line %d)", (int)e.err, hipGetErrorString(e.err), e.what,
+                     e.line);
+            pd->err = buf;
+        }
+        return SDOT_HIP_ERROR;
+    } catch (const OomError &) {
+        if (pd) pd->err = "device allocation failed";
+        return SDOT_OOM;
+    } catch (const StatusError &e) {
+        if (pd) pd->err = e.msg;
+        return e.st;
+    } catch (const std::string &e) {
+        if (pd) pd->err = e;
+        return SDOT_INVALID_ARG;
+    } catch (const std::bad_alloc &) {
+        if (pd) pd->err = "host allocation failed";
+        return SDOT_OOM;
+    }
+}
+
 // grid.hip
 void build_grid(sdot_pd *pd);         // bbox, keys, sort, sorted positions (positions changed)
 void refresh_weights(sdot_pd *pd);    // sorted weights, per-cell and global max weight
@@ -415,6 +457,7 @@ void export_csr(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, int6
 // the position / weight derivative CSR of the last (Jacobian) pass, 2D
 void export_jac(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, double *h_v, int64_t cap, int64_t *nnz);
 void export_vec(sdot_pd *pd, const double *d_sorted, double *h_out);  // sorted -> dirac order, to host
+void export_vec_device(sdot_pd *pd, const double *d_sorted, double *d_out);  // sorted -> dirac order, in HBM
 void export_centroids(sdot_pd *pd, double *h_out);   // position + moment / mass (0 when empty), dirac order
 // capi.hip
 sdot_status compute_pass(sdot_pd *pd);  // grid (if needed) + weights + cell pass; flags -> status
