@@ -256,29 +256,30 @@ def c5_run(rate, duration, deadline, precision, seed=7, label=""):
 
     th = threading.Thread(target=submitter, daemon=True)
     th.start()
+    # collect every finished request, in whatever order they finish: each sweep polls
+    # all outstanding ids (wait with a zero timeout returns at once when one is pending)
     res = {}
+    outstanding = set()
+    seen = 0
     last = time.perf_counter()
-    k = 0
-    while k < len(t_arr):
-        if ids[k] is None:
-            if done_submit.is_set():
-                break
-            time.sleep(0.005)
-            continue
-        left = deadline - (time.perf_counter() - t0)
-        if left <= 0:
+    while True:
+        while seen < len(ids) and ids[seen] is not None:
+            outstanding.add(seen)
+            seen += 1
+        for k in list(outstanding):
+            r = srv.wait(ids[k], timeout=0.0)
+            if r is not None:
+                res[k] = r[1]
+                outstanding.discard(k)
+        if done_submit.is_set() and seen >= sum(1 for i in ids if i is not None) and not outstanding:
             break
-        r = srv.wait(ids[k], timeout=min(left, 5.0))
-        if r is None:
-            if time.perf_counter() - last > 20:
-                print(f"# c5{label}: waiting on request {k}, stats {srv.stats()}", file=sys.stderr, flush=True)
-                last = time.perf_counter()
-            continue
-        res[k] = r[1]
-        k += 1
+        if time.perf_counter() - t0 > deadline:
+            break
         if time.perf_counter() - last > 20:
-            print(f"# c5{label}: {len(res)} done, stats {srv.stats()}", file=sys.stderr, flush=True)
+            print(f"# c5{label}: {len(res)} done, {len(outstanding)} outstanding, stats {srv.stats()}",
+                  file=sys.stderr, flush=True)
             last = time.perf_counter()
+        time.sleep(0.005)
     wall = time.perf_counter() - t0
     th.join(timeout=1.0)
     st = srv.stats()

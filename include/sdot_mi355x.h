@@ -314,7 +314,10 @@ typedef struct sdot_srv_stats {
 
 sdot_status sdot_srv_create(sdot_srv **srv, const sdot_srv_config *cfg);
 /* Copies the inputs and returns at once.  positions n*dim interleaved; weights
- * (start, may be NULL = 0); targets (n, > 0); box: lo[dim], hi[dim] or NULL = unit box. */
+ * (start, may be NULL = 0); targets (n, > 0); box: lo[dim], hi[dim] or NULL = unit box.
+ * SDOT_OOM when the request's estimated footprint (with a 3D worker's handle) exceeds
+ * the HBM budget, so that it could not run even on an idle device.  A queued request
+ * starts when its footprint fits beside the resident ones, or when nothing is running. */
 sdot_status sdot_srv_submit(sdot_srv *srv, int dim, int64_t n, const double *positions, const double *weights,
                             const double *targets, const double *box, int64_t *request_id);
 /* state: 0 queued, 1 running, 2 finished */

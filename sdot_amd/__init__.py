@@ -231,7 +231,9 @@ class PowerDiagram:
         self._check(lib().sdot_pd_set_weights(self._h, _ptr(w)), "set_weights")
 
     def set_diracs_device(self, pos_ptr, w_ptr, n):
-        """Device pointers (e.g. torch cuda tensors' data_ptr()) already in HBM."""
+        """Device pointers already in HBM, allocated through the HIP runtime this library
+        links (ROCm's libamdhip64.so.7; a second runtime in the process, such as the one
+        torch bundles, does not share the GPU with it)."""
         self._check(lib().sdot_pd_set_diracs_device(self._h, _p(pos_ptr), _p(w_ptr), int(n)), "set_diracs_device")
         self.N = int(n)
 
@@ -376,6 +378,8 @@ class PowerDiagram:
         """OptimalTransportSolver::solve; returns (status, weights, stats dict)."""
         t = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1)
         w = np.array(weights, dtype=np.float64, copy=True).reshape(-1)
+        if len(t) != self.N or len(w) != self.N:
+            raise ValueError(f"ot_solve: {len(t)} targets and {len(w)} weights for {self.N} diracs")
         stats = SolveStats()
         st = lib().sdot_ot_solve(self._h, _ptr(t), float(tol), int(stop_rule), int(max_iter), _ptr(w),
                                  ctypes.byref(stats))

@@ -278,7 +278,9 @@ sdot_status compute_pass(sdot_pd *pd) {
     pd->last.ms_total = elapsed_ms(pd->ev[0], pd->ev[2]);
     pd->last.grid_cells = (int64_t)pd->g.ncell[0] * pd->g.ncell[1] * pd->g.ncell[2];
     pd->last_flags = fl;
-    pd->pass_valid = true;
+    // a pass that ends in an error is never reused: the next getter runs it again and
+    // reports the same error instead of returning the partial results
+    pd->pass_valid = !(fl & (FLAG_BAD_INPUT | FLAG_CELL_OVERFLOW | FLAG_ROW_OVERFLOW));
     if (fl & FLAG_BAD_INPUT) throw StatusError{SDOT_INVALID_ARG, "non-finite weight or coincident diracs"};
     if (pd->stop_if_void && (fl & FLAG_EMPTY)) {   // stopped early: the per-cell results are partial
         pd->pass_valid = false;
@@ -302,7 +304,7 @@ namespace {
 sdot_status ensure_pass(sdot_pd *pd) {
     if (pd->pass_valid && pd->want_mom && !pd->pass_has_mom) pd->pass_valid = false;
     if (pd->pass_valid) {
-        if (pd->last_flags & (FLAG_BAD_INPUT | FLAG_ROW_OVERFLOW)) return compute_pass(pd);
+        if (pd->last_flags & (FLAG_BAD_INPUT | FLAG_ROW_OVERFLOW | FLAG_CELL_OVERFLOW)) return compute_pass(pd);
         return (pd->last_flags & FLAG_EMPTY) ? SDOT_EMPTY_CELL : SDOT_OK;
     }
     return compute_pass(pd);

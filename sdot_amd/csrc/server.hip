@@ -25,6 +25,7 @@
 #include <new>
 #include <thread>
 
+#include "admission.h"
 #include "batch_engine.h"
 
 using namespace sdot_amd;
@@ -99,11 +100,10 @@ void lane2_main(sdot_srv *S) {
             std::unique_lock<std::mutex> lk(S->mu);
             for (;;) {
                 if (S->stop) return;
-                // admission: FIFO while the footprint fits (one request always fits an empty device)
+                // admission: FIFO while the footprint fits (a request always starts on an idle device)
                 while (!S->q2.empty() && (int)(resident.size() + add.size()) < S->cfg.max_batch_problems) {
                     Request *r = S->q2.front();
-                    if (S->resident + r->bytes > S->budget && !(resident.empty() && add.empty() && S->resident == 0))
-                        break;
+                    if (!srv_admit(S->resident, r->bytes, S->budget, S->running)) break;
                     S->q2.pop_front();
                     S->resident += r->bytes;
                     S->peak = std::max(S->peak, S->resident);
@@ -208,7 +208,7 @@ void lane3_main(sdot_srv *S) {
                 if (!S->q3.empty()) {
                     Request *h = S->q3.front();
                     const double need = h->bytes + (reserved ? 0.0 : HANDLE_3D_BYTES);
-                    if (S->resident + need <= S->budget || S->resident == 0) {
+                    if (srv_admit(S->resident, need, S->budget, S->running)) {
                         S->q3.pop_front();
                         S->resident += need;
                         if (!reserved) reserved = true;
@@ -331,6 +331,10 @@ sdot_status sdot_srv_submit(sdot_srv *S, int dim, int64_t n, const double *posit
     }
     r->bytes = (double)n * (dim == 2 ? BatchEngine::bytes_per_dirac() : BYTES_PER_DIRAC_3D);
     std::lock_guard<std::mutex> lk(S->mu);
+    if (srv_never_fits(r->bytes, dim == 3 ? HANDLE_3D_BYTES : 0.0, S->budget)) {
+        S->err = "request footprint exceeds the HBM budget";
+        return SDOT_OOM;
+    }
     r->id = S->next_id++;
     r->res.t_submit = S->now();
     *request_id = r->id;

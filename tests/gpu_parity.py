@@ -40,7 +40,15 @@ def csr_matrix(c, n):
     return sp.csr_matrix((vals, cols, off), shape=(n, n))
 
 
-def compare(g, o, n, mass_scale, rtol=1e-6, hess_tie=1e-9, hess_rtol=1e-6):
+def hess_rtol(precision):
+    """Hessian entries above the tie level agree with the oracle's within this
+    fraction of the largest entry.  Both precisions integrate in FP64 (the mixed
+    path only decides in FP32 which planes bound the cell, then rebuilds the
+    vertices in FP64), so one bar holds for both."""
+    return 1e-8
+
+
+def compare(g, o, n, mass_scale, rtol=1e-6, hess_tie=1e-9, hess_rtol=1e-8):
     """Asserts masses / boundary measures / Hessian agree; returns the max errors."""
     assert g["status"] == o["status"], (g["status"], o["status"])
     dm = np.abs(g["masses"] - o["masses"])
@@ -65,8 +73,7 @@ def compare(g, o, n, mass_scale, rtol=1e-6, hess_tie=1e-9, hess_rtol=1e-6):
     So = (abs(Mo) > 1e-6 * scale).astype(np.int8)
     diff = abs(Sg - So)
     if diff.nnz:
-        # a differing entry must be within the tie band on the other side
-        r, c = diff.nonzero()
-        worst = max(abs(Mg[i, j] - Mo[i, j]) for i, j in zip(r[:50], c[:50]))
-        assert worst <= 2e-6 * scale, f"neighbour lists differ beyond ties ({worst:.3e})"
+        # every differing entry must be within the tie band on the other side
+        worst = abs(Mg - Mo).multiply(diff).max()
+        assert worst <= 2e-6 * scale, f"neighbour lists differ beyond ties ({worst:.3e}, {diff.nnz} entries)"
     return dict(mass=float((dm / mass_scale).max()), hess=float(D.max() / scale), sparsity_diffs=int(diff.nnz))

@@ -5,7 +5,7 @@ import math
 import numpy as np
 import pytest
 
-from tests.gpu_parity import compare, csr_matrix, gpu_pass, inputs, oracle_pass
+from tests.gpu_parity import compare, csr_matrix, gpu_pass, hess_rtol, inputs, oracle_pass
 
 pytestmark = pytest.mark.gpu
 
@@ -20,8 +20,7 @@ def test_cells2d_parity(precision, ball, n, seed):
         pos, w = inputs(2, n, seed, wscale=0.3 / n)
     g = gpu_pass(pos, w, 2, precision, ball)
     o = oracle_pass(pos, w, 2, ball)
-    hess_rtol = 1e-6 if precision == "fp64" else 1e-4
-    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
     print(f"[parity2d] {precision} ball={ball} n={n}: {err}")
 
 
@@ -32,7 +31,7 @@ def test_cells2d_c2_full_size_fp64_and_mixed():
     o = oracle_pass(pos, w, 2, ball=True)
     for prec in ("fp64", "mixed"):
         g = gpu_pass(pos, w, 2, prec, ball=True)
-        err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if prec == "fp64" else 1e-4)
+        err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(prec))
         print(f"[parity2d C2] {prec}: {err} timings {g['timings']}")
         # size-independent properties: symmetry, row sums = ball terms, disc bound
         M = csr_matrix(g["csr"], n)
@@ -68,7 +67,7 @@ def test_cells2d_edge_cases(precision):
     w = np.zeros(len(pos))
     g = gpu_pass(pos, w, 2, precision)
     o = oracle_pass(pos, w, 2)
-    compare(g, o, len(pos), 1.0 / len(pos), hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    compare(g, o, len(pos), 1.0 / len(pos), hess_rtol=hess_rtol(precision))
 
 
 def test_cells2d_empty_cell_and_bad_input():
@@ -82,6 +81,33 @@ def test_cells2d_empty_cell_and_bad_input():
     pd.set_diracs(np.array([[0.5, 0.5], [0.5, 0.5]]), np.zeros(2))
     with pytest.raises(sdot_amd.SdotError):
         pd.compute()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cells2d_capacity_overflow_is_reported_on_every_call(precision):
+    # a dirac ringed by 100 others: its cell is a 100-gon, beyond the 64-vertex capacity
+    import sdot_amd
+    k = 100
+    a = 2 * np.pi * np.arange(k) / k
+    pos = np.concatenate([[[0.5, 0.5]], 0.5 + 0.3 * np.c_[np.cos(a), np.sin(a)]])
+    pd = sdot_amd.PowerDiagram(2, precision)
+    pd.add_box_shape([0, 0], [1, 1])
+    pd.set_diracs(pos, np.zeros(k + 1))
+    for call in range(2):   # the failed pass is not reused: every getter reports it
+        with pytest.raises(sdot_amd.SdotError) as e:
+            pd.integrals()
+        assert e.value.status == sdot_amd.SDOT_CAPACITY_OVERFLOW, call
+    with pytest.raises(sdot_amd.SdotError) as e:
+        pd.boundary_measures()
+    assert e.value.status == sdot_amd.SDOT_CAPACITY_OVERFLOW
+    # a 50-gon fits: the handle recovers with the next input
+    k = 50
+    a = 2 * np.pi * np.arange(k) / k
+    pos = np.concatenate([[[0.5, 0.5]], 0.5 + 0.3 * np.c_[np.cos(a), np.sin(a)]])
+    pd.set_diracs(pos, np.zeros(k + 1))
+    m = pd.integrals()
+    o = __import__("oracle.oracle", fromlist=["x"]).cells(pos, np.zeros(k + 1))
+    assert np.abs(m - o["masses"]).max() < 1e-12
 
 
 def test_cells2d_density_coefficient_and_domain():
@@ -112,7 +138,7 @@ def test_cells2d_parity_image(precision, shape):
     img = lognormal_image_2d(*shape)
     g = gpu_pass(pos, w, 2, precision, img=img)
     o = oracle_pass(pos, w, 2, img=img)
-    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
     assert abs(g["masses"].sum() - img.mean()) < 1e-10
     print(f"[parity2d img {shape}] {precision}: {err}")
 
@@ -186,7 +212,7 @@ def test_cells2d_parity_quadratic_image(precision, shape):
     img[0] += 2.0
     g = gpu_pass(pos, w, 2, precision, img=img)
     o = oracle_pass(pos, w, 2, img=img)
-    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
     print(f"[parity2d quad img {shape}] {precision}: {err}")
 
 

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.gpu_parity import compare, csr_matrix, gpu_pass, inputs, oracle_pass
+from tests.gpu_parity import compare, csr_matrix, gpu_pass, hess_rtol, inputs, oracle_pass
 
 pytestmark = pytest.mark.gpu
 
@@ -24,7 +24,7 @@ def test_cells3d_parity_box(precision, n, seed):
     pos, w = inputs(3, n, seed, wscale=0.1 * n ** (-2 / 3))
     g = gpu_pass(pos, w, 3, precision)
     o = oracle_pass(pos, w, 3)
-    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
     print(f"[parity3d] {precision} n={n}: {err} {g['timings']}")
 
 
@@ -41,7 +41,7 @@ def test_cells3d_parity_spread_weights(precision):
     w[u > 0.99] += 1.0 * s2      # cells of up to ~9 times the mean volume, a few empty cells
     g = gpu_pass(pos, w, 3, precision)
     o = oracle_pass(pos, w, 3)
-    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
     print(f"[parity3d spread] {precision}: {err} {g['timings']}")
 
 
@@ -53,7 +53,7 @@ def test_cells3d_parity_image(precision, img_n):
     img = lognormal_image(img_n)
     g = gpu_pass(pos, w, 3, precision, img=img)
     o = oracle_pass(pos, w, 3, img=img)
-    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
     assert abs(g["masses"].sum() - img.mean()) < 1e-10
     print(f"[parity3d img {img_n}] {precision}: {err}")
 

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.gpu_parity import compare, csr_matrix, inputs
+from tests.gpu_parity import compare, csr_matrix, hess_rtol, inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -43,7 +43,7 @@ def test_periodic_2d_parity(case, precision):
     T = [(1, 0), (-1, 0)] if case == "unit_x" else T2
     dom = (np.array([-1.0, -1.0]), np.array([2.0, 2.0])) if case == "padded8" else None
     g, o, _ = run_pair(pos, w, 2, precision, T, dom, ball)
-    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
     if case == "padded8":
         assert abs(g["masses"].sum() - 1) < 1e-11   # the torus
     print(f"[periodic 2D {case}] {precision}: {err}")
@@ -56,7 +56,7 @@ def test_periodic_3d_parity(padded, precision):
     pos, w = inputs(3, n, 73, wscale=0.05 * n ** (-2 / 3))
     dom = (-np.ones(3), 2 * np.ones(3)) if padded else None
     g, o, _ = run_pair(pos, w, 3, precision, T3, dom)
-    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
     if padded:
         assert abs(g["masses"].sum() - 1) < 1e-11
     print(f"[periodic 3D padded={padded}] {precision}: {err}")

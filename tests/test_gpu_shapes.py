@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.gpu_parity import compare, inputs
+from tests.gpu_parity import compare, hess_rtol, inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -93,7 +93,7 @@ def test_shapes_2d_parity(case, precision):
     pos, w = inputs(2, n, 61, wscale=0.3 / n)
     shapes = CASES_2D[case]
     g, o, _ = run_pair(shapes, pos, w, 2, precision)
-    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
     assert abs(g["masses"].sum() - total_mass(shapes, 2)) < 1e-11
     print(f"[shapes 2D {case}] {precision}: {err}")
 
@@ -103,7 +103,7 @@ def test_shapes_2d_hexagon_with_ball(precision):
     n = 6000
     pos, w = inputs(2, n, 62, wscale=1.0 / n, wshift=1.0 / n)
     g, o, _ = run_pair(CASES_2D["hexagon"], pos, w, 2, precision, ball=True)
-    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
     assert (g["ball_arc"] > 0).any()
     print(f"[shapes 2D hexagon+ball] {precision}: {err}")
 
@@ -115,7 +115,7 @@ def test_shapes_3d_parity(case, precision):
     pos, w = inputs(3, n, 63, wscale=0.1 * n ** (-2 / 3))
     shapes = CASES_3D[case]
     g, o, _ = run_pair(shapes, pos, w, 3, precision)
-    err = compare(g, o, n, 1.0 / n, hess_rtol=1e-6 if precision == "fp64" else 1e-4)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
     assert abs(g["masses"].sum() - total_mass(shapes, 3)) < 1e-11
     print(f"[shapes 3D {case}] {precision}: {err}")
 
