@@ -1,20 +1,29 @@
 #!/usr/bin/env python3
-"""Headline benchmark: Laguerre cells/s of the ball-clipped 2D cell pass.
+"""Headline benchmark of the Laguerre-cell path on one MI355X.
 
-Workload: N = 1,000,000 diracs uniform in [0,1]^2 (seed 0), weights
-w_i ~ U[0, 1e-6] shifted by r0^2 = 1e-6, so each cell is clipped by the disc of
-radius sqrt(w_i + r0^2); domain the unit square with density 1.  One step = one
-full pass on resident HBM inputs: grid build (bounding box, Morton keys,
-counting sort, per-cell max weight), cell kernel (power cell, disc clip, cell
-area, boundary measures and Hessian rows) and the dirac-order CSR build in
-HBM.  Both precisions are measured; `value` is the faster one whose masses
-agree with the FP64 CPU oracle within 1e-6 relative.
+Default workload (the line's `value`): the ball-clipped 2D cell pass at N = 1,000,000
+diracs uniform in [0,1]^2 (seed 0), weights w_i ~ U[0, 1e-6] shifted by r0^2 = 1e-6,
+so each cell is clipped by the disc of radius sqrt(w_i + r0^2); domain the unit square
+with density 1.  One step = one full pass on resident HBM inputs: grid build (bounding
+box, Morton keys, counting sort, per-cell max weight), cell kernel (power cell, disc
+clip, cell area, boundary measures, Hessian rows) and the dirac-order CSR in HBM.
+`value` is the FP64 pass (the reference's arithmetic); the mixed FP32-clip pass is
+reported beside it with its error against the FP64 oracle.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+The same run adds two Newton-solve legs as extra keys:
+  c1: 2D Newton, N = 10^4 uniform, w0 = 0, uniform targets, to 1e-6 max relative mass
+      error, FP64 and mixed, beside the oracle's own full solve on the host cores;
+  c3: 3D Newton, N = 10^6 uniform in [0,1]^3, 64^3 lognormal(0, 0.5) voxel density
+      (mass 1), to 1e-6, FP64 and mixed, beside the oracle's cell pass at the converged
+      weights timed on a bounded sample of the cells.
+
+Other workloads: --workload c3 | c4 | c5 (one line each).
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2] [--no-legs]
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -29,6 +38,18 @@ PEAK = {"fp64": 78.6, "mixed": 157.3}        # MI355X vector FP64 / FP32, TFLOP/
 # FETCH_SIZE and WRITE_SIZE of k_cells2, each scaled by the same counter on a 1 GiB device copy
 # (bench.py --copy-calibration 1024: FETCH reads half the copied bytes, WRITE all of them)
 TRAFFIC = {"mixed": 184.9e6, "fp64": 636.1e6}
+CPU_CORES = min(16, os.cpu_count() or 1)     # the host share of one GPU on the box
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
 
 
 def make_inputs(n, seed=0):
@@ -82,19 +103,59 @@ def copy_calibration(mib, reps=3):
 
 def run_cpu_baseline(pos, w, budget_s=10.0, max_reps=40):
     from oracle import oracle as O
-    cores = min(16, os.cpu_count() or 1)
     reps = 0
     t0 = time.perf_counter()
     while True:
-        r = O.cells(pos, w, ball=True, nthreads=cores)
+        r = O.cells(pos, w, ball=True, nthreads=CPU_CORES)
         O.to_csr(r)
         reps += 1
         if time.perf_counter() - t0 >= budget_s or reps >= max_reps:
             break
     dt = (time.perf_counter() - t0) / reps
-    return dict(value=len(w) / dt, unit="cells/s", cores=cores, kind="port",
+    return dict(value=len(w) / dt, unit="cells/s", cores=CPU_CORES, kind="port", cpu_model=cpu_model(),
                 sample=f"{reps} full passes of the same 1M-dirac ball workload (grid + cells + CSR) "
-                       f"by the C oracle with {cores} threads, {dt:.3f} s each"), r["masses"]
+                       f"by the C oracle with {CPU_CORES} threads, {dt:.3f} s each"), r["masses"]
+
+
+def run_c1(reps=3):
+    """2D Newton, N = 10^4 (seed 0), w0 = 0, targets 1/N, to 1e-6: device solves (median
+    of reps) in both precisions and the oracle's complete solve on the host cores."""
+    import sdot_amd
+    from sdot_amd import workloads
+    from oracle import oracle as O
+    n = 10_000
+    pos = workloads.uniform_diracs(n, 2, seed=0)
+    t = np.full(n, 1.0 / n)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        pd = sdot_amd.PowerDiagram(2, prec)
+        pd.add_box_shape([0, 0], [1, 1])
+        pd.set_diracs(pos, np.zeros(n))
+        runs, stats = [], None
+        for _ in range(reps + 1):   # the first solve builds the solver's buffers
+            t0 = time.perf_counter()
+            st, w, stats = pd.ot_solve(t, np.zeros(n), tol=1e-6)
+            runs.append(time.perf_counter() - t0)
+        out[prec] = dict(seconds=float(np.median(runs[1:])), status=st, newton_iters=stats["newton_iters"],
+                         pcg_iters=stats["pcg_iters_total"], max_rel_mass_err=stats["max_rel_mass_err"],
+                         nnz=stats["nnz"])
+        out[prec + "_w"] = w
+    t0 = time.perf_counter()
+    sto, wo, so = O.ot_solve(pos, np.zeros(n), t, tol=1e-6, nthreads=CPU_CORES)
+    cpu_s = time.perf_counter() - t0
+    d = wo - wo.mean()
+    werr = {p: float(np.abs((out[p + "_w"] - out[p + "_w"].mean()) - d).max() / np.abs(d).max())
+            for p in ("fp64", "mixed")}
+    res = {"workload": "2D Newton OT solve, N=10000 uniform in [0,1]^2 (seed 0), w0=0, targets 1/N, unit box, "
+                       "stop at max relative mass error 1e-6",
+           "fp64": out["fp64"], "mixed": out["mixed"],
+           "weights_rel_diff_vs_oracle": werr,
+           "cpu_baseline": {"value": cpu_s, "unit": "s", "cores": CPU_CORES, "kind": "port",
+                            "sample": f"the oracle's complete Newton solve of the same problem (Jacobi-PCG), "
+                                      f"{so.newton_iters} iterations, status {sto}"},
+           "speedup_vs_cpu": {p: cpu_s / out[p]["seconds"] for p in ("fp64", "mixed")}}
+    print(f"# c1: {res}", file=sys.stderr, flush=True)
+    return res
 
 
 def run_c3(n, img_size, steps, precisions=("fp64", "mixed")):
@@ -124,30 +185,53 @@ def run_c3(n, img_size, steps, precisions=("fp64", "mixed")):
     return out, pos, img
 
 
-def main_c3(args):
+def c3_cpu_baseline(pos, w, img, budget_s=10.0):
+    """The oracle's cell pass (voxel pieces, Hessian rows) at the converged weights, on the
+    first S cells in dirac order (a spatially uniform sample), S sized to about budget_s."""
     from oracle import oracle as O
+    n = len(pos)
+    s0 = min(n, 2000)
+    dt0, _, _ = O.cells_sample_seconds(pos, w, 0, s0, img=img, nthreads=CPU_CORES)
+    s = int(min(n, max(s0, s0 * budget_s / max(dt0, 1e-6))))
+    dt, st, _ = O.cells_sample_seconds(pos, w, 0, s, img=img, nthreads=CPU_CORES)
+    return dict(value=s / dt, unit="cells/s (3D image pass at the converged weights)", cores=CPU_CORES, kind="port",
+                cpu_model=cpu_model(),
+                sample=f"oracle cell pass (voxel pieces, Hessian rows) on cells [0, {s}) of the {n} at the "
+                       f"converged FP64 weights: {dt:.2f} s with {CPU_CORES} threads")
+
+
+def c3_leg(n=1_000_000, img=64):
+    res, pos, image = run_c3(n, img, 1)
+    cpu = c3_cpu_baseline(pos, res["fp64_weights"], image)
+    f, m = res["fp64"], res["mixed"]
+    # a lower bound of the CPU solve: its Newton passes alone (every pass a full pass at the
+    # measured rate), without its linear solves
+    cpu_passes = f["newton_iters"] * n / cpu["value"]
+    out = {"workload": f"3D Newton OT solve, N={n} uniform in [0,1]^3, {img}^3 lognormal(0,0.5) voxel density "
+                       "(mass 1), w0=0, uniform targets, stop at max relative mass error 1e-6",
+           "fp64": {k: f[k] for k in ("seconds", "status", "newton_iters", "pcg_iters_total", "backtracks",
+                                      "max_rel_mass_err", "seconds_cells", "seconds_solve", "cells_per_s_pass")},
+           "mixed": {k: m[k] for k in ("seconds", "status", "newton_iters", "pcg_iters_total", "backtracks",
+                                       "max_rel_mass_err", "seconds_cells", "seconds_solve", "cells_per_s_pass")},
+           "mixed_speedup_vs_fp64": f["seconds"] / m["seconds"],
+           "cpu_baseline": cpu,
+           "gpu_pass_vs_cpu_pass": {p: res[p]["cells_per_s_pass"] / cpu["value"] for p in ("fp64", "mixed")},
+           "cpu_solve_lower_bound_s": cpu_passes,
+           "solve_speedup_lower_bound": {p: cpu_passes / res[p]["seconds"] for p in ("fp64", "mixed")}}
+    print(f"# c3 leg: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def main_c3(args):
     precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
     res, pos, img = run_c3(args.n, args.img, args.steps, precs)
-    best = min(precs, key=lambda p: res[p]["seconds"] if res[p]["status"] == 0 else 1e30)
-    cpu = None
-    if not args.no_cpu_baseline:
-        w = res[best + "_weights"]
-        sample = min(args.n, 4000)
-        dt, st, _ = O.cells_sample_seconds(pos, w, 0, sample, img=img, nthreads=min(16, os.cpu_count() or 1))
-        cpu_cells = sample / dt
-        iters = res[best]["newton_iters"]
-        cpu_solve = iters * args.n / cpu_cells
-        cpu = {"value": cpu_cells, "unit": "cells/s (one image pass)", "cores": min(16, os.cpu_count() or 1),
-               "kind": "port",
-               "sample": f"oracle cell pass (voxel pieces, Hessian rows) on {sample} of the {args.n} cells at "
-                         f"the converged weights: {dt:.2f} s",
-               "solve_seconds_estimate": cpu_solve,
-               "estimate": f"{iters} Newton passes x N / (cells/s), CPU linear solves not counted"}
+    best = "fp64" if "fp64" in precs else precs[0]
+    cpu = None if args.no_cpu_baseline else c3_cpu_baseline(pos, res[best + "_weights"], img)
     line = {
         "metric": "solve s @1e-6 mass err (3D Newton, 1M diracs, 64^3 lognormal density)",
         "value": res[best]["seconds"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
         "ms_per_step": res[best]["seconds"] * 1e3, "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32-clip/f64-solve" if best == "mixed" else "f64",
+        "vs_baseline": None, "dtype": "f64" if best == "fp64" else "f32-clip/f64-solve",
         "data": "synthetic (xoshiro256** seed 0 positions, seed 1 density)",
         "config": {"workload": f"3D Newton OT solve, N={args.n} uniform in [0,1]^3, {args.img}^3 lognormal(0,0.5) "
                                "voxel density (mass 1), w0=0, uniform targets, stop at max relative mass error 1e-6",
@@ -155,7 +239,6 @@ def main_c3(args):
         "solves": {p: {k: v for k, v in res[p].items()} for p in precs},
         "mixed_speedup_vs_fp64": (res["fp64"]["seconds"] / res["mixed"]["seconds"]) if len(precs) == 2 else None,
         "cpu_baseline": cpu,
-        "speedup_vs_cpu_estimate": (cpu["solve_seconds_estimate"] / res[best]["seconds"]) if cpu else None,
     }
     print(json.dumps(line))
 
@@ -222,14 +305,8 @@ def main_c4(args):
 
 def c5_schedule(rate, duration, seed=7):
     """Poisson arrivals (rate/s over `duration` s), 70 % 2D / 30 % 3D, N = floor(exp(U[ln 1e3, ln 2e6]))."""
-    import sdot_amd
-    m = int(rate * duration * 2 + 64)
-    u = sdot_amd.rng_uniform(seed, 3 * m).reshape(m, 3)
-    t = np.cumsum(-np.log1p(-u[:, 0]) / rate)
-    keep = t < duration
-    dims = np.where(u[:, 1] < 0.7, 2, 3)
-    sizes = np.floor(np.exp(np.log(1e3) + u[:, 2] * (np.log(2e6) - np.log(1e3)))).astype(np.int64)
-    return t[keep], dims[keep], sizes[keep]
+    from sdot_amd import workloads
+    return workloads.serving_schedule(rate, duration, seed)
 
 
 def c5_run(rate, duration, deadline, precision, seed=7, label=""):
@@ -324,6 +401,8 @@ def main_c5(args):
     print(json.dumps(line))
 
 
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -339,6 +418,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="c2 only: skip the C1 / C3 solve legs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run device-to-device copies of MIB MiB (known HBM bytes) so hardware "
                          "counters of the cell kernel can be scaled by the copy's counters")
@@ -375,13 +455,12 @@ def main():
             err = np.abs(masses[prec] - ref)
             res[prec]["max_mass_err_vs_oracle"] = float((err / (np.abs(ref) + 1e-9 * scale)).max())
             res[prec]["max_mass_err_x_N"] = float(err.max() / scale)
+    legs = {}
+    if not args.no_legs:
+        legs["c1"] = run_c1()
+        legs["c3"] = c3_leg()
 
-    def ok(p):
-        return "max_mass_err_vs_oracle" not in res[p] or res[p]["max_mass_err_vs_oracle"] < 1e-6
-
-    cands = [p for p in res if ok(p)] or ["fp64"]
-    best = max(cands, key=lambda p: res[p]["cells_per_s"])
-    b = res[best]
+    b = res["fp64"]
     achieved = b["achieved_tflops"]
     line = {
         "metric": METRIC,
@@ -394,18 +473,27 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-clip/f64-integrate" if best == "mixed" else "f64",
+        "dtype": "f64",
         "data": "synthetic (xoshiro256** seed 0)",
         "config": {"workload": "2D ball-clipped cell pass, N=1e6 uniform in [0,1]^2, w~U[0,1e-6]+r0^2, r0=1e-3, "
                                "unit-square domain; grid + cells + Hessian CSR per step",
-                   "n_diracs": args.n, "dim": 2, "precision": best},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK[best], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[best], "traffic": TRAFFIC.get(best) if args.n == 1_000_000 else None,
-                     "kernel": "k_cells2", "flops_per_launch": b["cell_flops"],
+                   "n_diracs": args.n, "dim": 2, "precision": "fp64"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK["fp64"], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK["fp64"], "traffic": TRAFFIC["fp64"] if args.n == 1_000_000 else None,
+                     "kernel": "k_cells2 (FP64 capacity tiers)", "flops_per_launch": b["cell_flops"],
                      "kernel_ms": b["ms_cell_kernel"]},
         "cpu_baseline": cpu,
+        "mixed": {"cells_per_s": res["mixed"]["cells_per_s"],
+                  "speedup_vs_fp64": res["mixed"]["cells_per_s"] / b["cells_per_s"],
+                  "dtype": "f32-clip/f64-integrate",
+                  "max_mass_err_vs_oracle": res["mixed"].get("max_mass_err_vs_oracle"),
+                  "roofline": {"achieved": res["mixed"]["achieved_tflops"], "peak": PEAK["mixed"],
+                               "frac": res["mixed"]["achieved_tflops"] / PEAK["mixed"],
+                               "traffic": TRAFFIC["mixed"] if args.n == 1_000_000 else None}},
         "precisions": res,
+        "host": {"cpu_model": cpu_model(), "cores_used": CPU_CORES},
     }
+    line.update(legs)
     print(json.dumps(line))
 
 
