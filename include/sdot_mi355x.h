@@ -181,7 +181,7 @@ typedef struct sdot_solve_stats {
     int precond_levels;          /* multigrid levels of the last linear solve (0: Jacobi) */
     double max_rel_mass_err;     /* at the returned weights */
     double max_abs_dw;           /* last Newton step */
-    int64_t nnz;
+    int64_t nnz;                 /* Hessian entries of the final pass (row slots + diagonals) */
     double seconds_cells;        /* device time in the cell passes */
     double seconds_solve;        /* device time in the PCG solves */
     double seconds_total;        /* wall time of the call */
@@ -209,7 +209,8 @@ typedef struct sdot_timings {
     double ms_rows;              /* CSR export (0 unless requested) */
     double ms_total;
     int64_t nb_cells_overflow;   /* cells re-run with a larger polygon / polyhedron capacity */
-    int64_t nnz_offdiag;         /* off-diagonal Hessian entries in the last pass */
+    int64_t nnz_offdiag;         /* not counted here (0): a timing read stays free of device work;
+                                    sdot_solve_stats.nnz gives the entries of a solve's final pass */
     int64_t grid_cells;          /* grid cells of the last grid build */
     double ms_cell_kernel;       /* the cell kernels (2D: all capacity tiers; 3D: the main
                                     kernel alone), HIP events around the launches */

@@ -399,44 +399,49 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         return hh - (x * vx + y * vy);
     };
     auto cut = [&](const T Vx, const T Vy, const T H, const int cid, double vx64, double vy64, double h64) -> int {
-        // classify vertices: d = H - q.V, outside when d < 0
+        // classify vertices: d = H - q.V, outside when d < 0.  The loop only builds the
+        // bitmask of outside vertices (no per-vertex branches, which cost the threads of a
+        // wavefront most of the cut); the runs are read from the mask afterwards.
+        auto dist = [&](int k) { return fma(-PX(k), Vx, fma(-PY(k), Vy, H)); };
         uint64_t mask = 0;
-        T dfirst = 0, dprev = 0, dAin = 0, dAout = 0, dBout = 0, dBin = 0;
-        int i1 = -1, i2 = -1, ntr = 0;
-        auto classify = [&](auto &&dist) {
-            mask = 0;
-            ntr = 0;
-            for (int k = 0; k < n; ++k) {
-                const T d = dist(k);
-                const bool o = d < 0;
-                if (o) mask |= 1ull << k;
-                if (k == 0) {
-                    dfirst = d;
-                } else {
-                    const bool po = dprev < 0;
-                    if (o && !po) { i1 = k; dAin = dprev; dAout = d; ++ntr; }
-                    if (!o && po) { i2 = k - 1; dBout = dprev; dBin = d; }
-                }
-                dprev = d;
-            }
-            if (n > 0 && dfirst < 0 && !(dprev < 0)) ++ntr;
-        };
-        // the fast loop stays free of the exact path (a separate, rarely run loop)
-        classify([&](int k) { return fma(-PX(k), Vx, fma(-PY(k), Vy, H)); });
-        if (ntr > 1)   // a convex polygon crosses a line once: rounding noise, redo in FP64
-            classify([&](int k) { return T(side64(k, vx64, vy64, h64)); });
+        if (CAP <= 8) {
+#pragma unroll
+            for (int k = 0; k < CAP; ++k) mask |= (k < n && dist(k) < 0) ? (1ull << k) : 0ull;
+        } else {
+            for (int k = 0; k < n; ++k) mask |= dist(k) < 0 ? (1ull << k) : 0ull;
+        }
         if (!mask) return 0;
+        const uint64_t full = n >= 64 ? ~0ull : (1ull << n) - 1;
+        // starts: outside vertices whose predecessor is inside; ends: whose successor is inside
+        auto runs = [&](uint64_t m, uint64_t &starts, uint64_t &ends) {
+            const uint64_t prev = ((m << 1) | (m >> (n - 1))) & full, next = ((m >> 1) | (m << (n - 1))) & full;
+            starts = m & ~prev;
+            ends = m & ~next;
+        };
+        uint64_t starts, ends;
+        runs(mask, starts, ends);
+        bool exact = false;
+        if (__builtin_popcountll(starts) > 1) {
+            // a convex polygon crosses a line once: rounding noise, redo in FP64 (off the fast path)
+            exact = true;
+            mask = 0;
+            for (int k = 0; k < n; ++k) mask |= T(side64(k, vx64, vy64, h64)) < 0 ? (1ull << k) : 0ull;
+            if (!mask) return 0;
+            runs(mask, starts, ends);
+        }
         const int nout = __builtin_popcountll(mask);
         if (nout == n) { n = 0; return 1; }
-        {
-            bool o0 = dfirst < 0, ol = dprev < 0;
-            if (o0 && !ol) { i1 = 0; dAin = dprev; dAout = dfirst; }
-            if (!o0 && ol) { i2 = n - 1; dBout = dprev; dBin = dfirst; }
-        }
+        // the entering vertex i1 and the leaving vertex i2 (with several runs, as a scan
+        // keeping the last transitions would choose: index 0 / n - 1 first)
+        const int i1 = (starts & 1) ? 0 : 63 - __builtin_clzll(starts);
+        const uint64_t ends_lo = ends & (full >> 1);
+        const int i2 = ((ends >> (n - 1)) & 1) ? n - 1 : 63 - __builtin_clzll(ends_lo);
         const int nn = n - nout + 2;
         if (nn > CAP) return 2;
         const int i0 = i1 == 0 ? n - 1 : i1 - 1;
         const int i3 = i2 == n - 1 ? 0 : i2 + 1;
+        auto dval = [&](int k) { return exact ? T(side64(k, vx64, vy64, h64)) : dist(k); };
+        const T dAin = dval(i0), dAout = dval(i1), dBout = dval(i2), dBin = dval(i3);
         const T m0 = dAin / (dAout - dAin);
         const T Ax = PX(i0) - m0 * (PX(i1) - PX(i0)), Ay = PY(i0) - m0 * (PY(i1) - PY(i0));
         const T m2 = dBin / (dBout - dBin);

@@ -69,6 +69,20 @@ __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
 }
 
 // ---- dirac-order export ----
+// sum of the row lengths (off-diagonal Hessian entries of the pass)
+__global__ __launch_bounds__(256) void k_sum_rows(const int *row_len, int64_t n, unsigned long long *out) {
+    __shared__ unsigned long long sm[256];
+    unsigned long long acc = 0;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) acc += row_len[i];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, sm[0]);
+}
+
 __global__ void k_export_vec(const double *v, const int *inv_perm, int64_t n, double *out) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n) out[i] = v[inv_perm[i]];
@@ -475,6 +489,21 @@ void export_centroids(sdot_pd *pd, double *h_out) {
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(h_out, pd->out_tmp.p, (size_t)dim * N * sizeof(double), hipMemcpyDeviceToHost, pd->stream));
     HIPCK(hipStreamSynchronize(pd->stream));
+}
+
+int64_t count_offdiag(sdot_pd *pd) {
+    const int64_t N = pd->N;
+    if (N <= 0 || !pd->row_len.p) return 0;
+    pd->wctr.ensure(3 * WCTR);   // its first slot is free between passes
+    unsigned long long *acc = pd->wctr.p;
+    fill_u32(reinterpret_cast<uint32_t *>(acc), 2, 0u, pd->stream);
+    const unsigned nb = (unsigned)std::min<int64_t>(1024, (N + 255) / 256);
+    hipLaunchKernelGGL(k_sum_rows, dim3(nb), dim3(256), 0, pd->stream, pd->row_len.p, N, acc);
+    HIPCK(hipGetLastError());
+    unsigned long long h = 0;
+    HIPCK(hipMemcpyAsync(&h, acc, sizeof h, hipMemcpyDeviceToHost, pd->stream));
+    HIPCK(hipStreamSynchronize(pd->stream));
+    return (int64_t)h;
 }
 
 void export_vec_device(sdot_pd *pd, const double *d_sorted, double *d_out) {

@@ -236,7 +236,7 @@ sdot_status ot_solve(sdot_pd *pd, const double *targets, double tol, int stop_ru
     }
     HIPCK(hipMemcpyAsync(w_inout, pd->w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
-    st.nnz = 0;
+    st.nnz = pd->pass_valid ? n + count_offdiag(pd) : 0;   // Hessian entries of the final pass (rows + diagonal)
     st.precond_levels = pd->last_precond_levels;
     st.seconds_cells = ms_cells * 1e-3;
     st.seconds_solve = ms_solve * 1e-3;
