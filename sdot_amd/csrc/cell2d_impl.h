@@ -40,15 +40,20 @@ struct Eps<float> {
     static constexpr double region = 2e-5, cand = 2e-5;
 };
 
-// Can a region whose points are all at distance >= d from c0 and whose diracs
-// have weight <= wmax hold a dirac that cuts a polygon contained in the disc of
-// radius sqrt(R2) about c0?  Dirac j cuts only if |V|*R > (|V|^2 + w0 - wj)/2.
+// Can a region whose points are all at distance >= d = sqrt(D2) from c0 and whose
+// diracs have weight <= wmax hold a dirac that cuts a polygon contained in the disc
+// of radius R = sqrt(R2) about c0?  Dirac j cuts only if |V|*R > (|V|^2 + w0 - wj)/2,
+// so the region is skippable when d >= R and d^2 - 2 R d + w0 - wmax >= tol, or when
+// d < R and w0 - wmax - R^2 >= tol.  The first test is taken in squares (both sides
+// of d^2 + w0 - wmax - tol >= 2 R d are then non-negative): no square root.
 template <class T>
-SDOT_HD inline bool region_skippable(double d, double R2, double w0, double wmax) {
-    double R = sqrt(R2);
-    double dw = w0 - wmax;
-    double tol = Eps<T>::region * (d * d + R2 + fabs(w0) + fabs(wmax));
-    if (d >= R) return d * d - 2 * R * d + dw >= tol;
+SDOT_HD inline bool region_skippable(double D2, double R2, double w0, double wmax) {
+    const double dw = w0 - wmax;
+    const double tol = Eps<T>::region * (D2 + R2 + fabs(w0) + fabs(wmax));
+    if (D2 >= R2) {
+        const double l = D2 + dw - tol;
+        return l >= 0 && l * l >= 4 * R2 * D2;
+    }
     return dw - R2 >= tol;
 }
 
@@ -492,7 +497,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             if (bt) dm = fmin(dm, qy - (g.lo[1] + (hy - r + 1) * g.h[1]));
             if (tp) dm = fmin(dm, (g.lo[1] + (hy + r) * g.h[1]) - qy);
             dm = fmax(dm, 0.0);
-            if (region_skippable<T>(dm, R2, w0, W)) break;
+            if (region_skippable<T>(dm * dm, R2, w0, W)) break;
         }
         // the in-range cells of ring r: full rows at dy = +-r, the two side columns in between
         const int gy0 = hy - r < 0 ? 0 : hy - r, gy1 = hy + r >= g.ncell[1] ? g.ncell[1] - 1 : hy + r;
@@ -510,13 +515,22 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                 double bx0 = g.lo[0] + gx * g.h[0], by0 = g.lo[1] + gy * g.h[1];
                 double ex = fmax(0.0, fmax(bx0 - qx, qx - (bx0 + g.h[0])));
                 double ey = fmax(0.0, fmax(by0 - qy, qy - (by0 + g.h[1])));
-                if (region_skippable<T>(sqrt(ex * ex + ey * ey), R2, w0, pv.cell_wmax[code])) continue;
+                if (region_skippable<T>(ex * ex + ey * ey, R2, w0, pv.cell_wmax[code])) continue;
             }
+            // the next candidate's position and weight are loaded while the current one is
+            // tested and cut (software pipelining: the loads' latency overlaps the cut)
+            double nx = a.spos[2 * (int64_t)cs], ny = a.spos[2 * (int64_t)cs + 1], nw = a.sw[cs];
             for (uint32_t j = cs; j < ce; ++j) {
+                const double px = nx, py = ny, pw = nw;
+                if (j + 1 < ce) {
+                    nx = a.spos[2 * (int64_t)(j + 1)];
+                    ny = a.spos[2 * (int64_t)(j + 1) + 1];
+                    nw = a.sw[j + 1];
+                }
                 if ((int64_t)j == t && s < 0) continue;
-                const double vx = a.spos[2 * (int64_t)j] - qx, vy = a.spos[2 * (int64_t)j + 1] - qy;
+                const double vx = px - qx, vy = py - qy;
                 const double vv = vx * vx + vy * vy;
-                const double hh = 0.5 * (vv + w0 - a.sw[j]);
+                const double hh = 0.5 * (vv + w0 - pw);
                 if (hh > 0 && R2 * vv <= hh * hh * (1 - Eps<T>::cand)) continue;   // cannot cut
                 if (vv == 0) { bad = true; continue; }                             // coincident diracs
                 const T Vx = T(vx), Vy = T(vy), H = T(hh);
