@@ -44,8 +44,10 @@ typedef enum sdot_status {
 
 typedef enum sdot_precision {
     SDOT_FP64 = 0,               /* clipping, integrals and solve in FP64 */
-    SDOT_MIXED = 1               /* FP32 cell-local clipping and integrals; FP64 plane offsets,
-                                    mass sums, residuals and the Newton/CG solve */
+    SDOT_MIXED = 1               /* FP32 cell-local clipping (which planes bound each cell), FP64
+                                    plane offsets; the vertices are then rebuilt in FP64 from their
+                                    planes and every integral, Hessian entry, residual and the
+                                    Newton/CG solve run in FP64 */
 } sdot_precision;
 
 typedef struct sdot_pd sdot_pd;
@@ -106,8 +108,15 @@ sdot_status sdot_pd_set_ball_clip(sdot_pd *pd, int on);
  *     d/dw_i adds the cell's measure, :135-138) and the Newton solve.
  * Centroids and position derivatives take Unit / InBallW05 only. */
 enum { SDOT_FUNC_UNIT = 0, SDOT_FUNC_IN_BALL_W05 = 1, SDOT_FUNC_R2 = 2, SDOT_FUNC_R2_IN_BALL_W05 = 3,
-       SDOT_FUNC_WMR2 = 4, SDOT_FUNC_PP_WMR2 = 5 };
+       SDOT_FUNC_WMR2 = 4, SDOT_FUNC_PP_WMR2 = 5, SDOT_FUNC_EXP_WMR2_DB = 6 };
 sdot_status sdot_pd_set_radial_func(sdot_pd *pd, int func);
+/* FunctionEnum::ExpWmR2db<TF>{eps} (ExpWmR2db.h) -- func = SDOT_FUNC_EXP_WMR2_DB: the
+ * integrand exp((w_i - |x - x_i|^2) / eps), no ball cut, 2D (its 3D integration is TODO in
+ * the reference, ConvexPolyhedron3.tcc:1010-1013); one domain shape or constant pieces.
+ * Masses are the exact integrals (the reference fits a table to precision 1e-10,
+ * ConvexPolyhedron2.tcc:1747-1787), edge items the erf closed form (:346-389), and
+ * d m_i / d w_i adds m_i / eps (:124-127).  eps > 0; other funcs ignore eps. */
+sdot_status sdot_pd_set_radial_func_eps(sdot_pd *pd, int func, double eps);
 /* SpZGrid::translations (SpZGrid.h:47-53, traversal SpZGrid.tcc:558-561,610): every
  * cell is also cut by the diracs translated by each of the n vectors
  * (translations[dim s .. dim s + dim)); a face with the image of dirac j adds to

@@ -24,7 +24,7 @@ class Domain(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("p0", _d * 3), ("p1", _d * 3), ("coeff", _d),
                 ("img", _p), ("img_sizes", _i64 * 3), ("img_nc", _i64),
                 ("nitems", _i64), ("it_box", _p), ("it_coeff", _p), ("it_planes", _p), ("it_off", _p),
-                ("ntrans", _i64), ("trans", _p)]
+                ("ntrans", _i64), ("trans", _p), ("rf_eps", _d)]
 
 
 class SolveStats(ctypes.Structure):
@@ -239,13 +239,14 @@ def make_domain(dim, p0=None, p1=None, coeff=1.0, img=None, items=None, translat
     return d, keep
 
 
-RADIAL = {"unit": 0, "r2": 1, "wmr2": 2}
+RADIAL = {"unit": 0, "r2": 1, "wmr2": 2, "expwmr2db": 3}
 
 
 def cells(pos, w, dim=None, ball=False, brute=False, nthreads=None, domain=None, img=None, cap=None, items=None,
-          translations=None, radial="unit"):
+          translations=None, radial="unit", eps=1.0):
     """radial: the final integration function -- "unit" (Unit / InBallW05), "r2" (R2 /
-    R2InBallW05: masses only) or "wmr2" (WmR2 / PpWmR2 with ball=True)."""
+    R2InBallW05: masses only), "wmr2" (WmR2 / PpWmR2 with ball=True) or "expwmr2db"
+    (ExpWmR2db{eps}: exp((w - |x - c|^2) / eps), no ball)."""
     """Per-cell masses, ball arc, perimeter and raw rows.  Returns dict."""
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     dim = dim or pos.shape[1]
@@ -256,6 +257,7 @@ def cells(pos, w, dim=None, ball=False, brute=False, nthreads=None, domain=None,
         dom, keep = make_domain(dim, img=img, items=items, translations=translations)
     else:
         dom, keep = make_domain(dim, *domain, img=img, items=items, translations=translations)
+    dom.rf_eps = eps
     nthreads = nthreads or os.cpu_count() or 1
     masses = np.zeros(N)
     arc = np.zeros(N)
@@ -354,7 +356,7 @@ def pcg(off, cols, vals, b, rtol=1e-12, max_iter=100000):
 
 
 def ot_solve(pos, w0, targets, ball=False, tol=1e-6, stop_rule=0, max_iter=150, nthreads=None,
-             domain=None, img=None, items=None, translations=None, radial="unit"):
+             domain=None, img=None, items=None, translations=None, radial="unit", eps=1.0):
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     dim = pos.shape[1]
     w = np.array(w0, dtype=np.float64, copy=True)
@@ -363,6 +365,7 @@ def ot_solve(pos, w0, targets, ball=False, tol=1e-6, stop_rule=0, max_iter=150, 
         dom, keep = make_domain(dim, img=img, items=items, translations=translations)
     else:
         dom, keep = make_domain(dim, *domain, img=img, items=items, translations=translations)
+    dom.rf_eps = eps
     stats = SolveStats()
     st = lib().or_ot_solve(dim, pos.shape[0], _ptr(pos), _ptr(w), _ptr(t), ctypes.byref(dom),
                            (1 if ball else 0) | (RADIAL[radial] << 4), nthreads or os.cpu_count() or 1, tol,

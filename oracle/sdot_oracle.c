@@ -278,6 +278,75 @@ int or_cp2_boundary_items_wmr2(const or_cp2 *cp, double coeff, double w, double 
     return m;
 }
 
+/* ---- ExpWmR2db (ExpWmR2db.h): f(x) = exp((w - |x - c|^2) / eps), no ball cut ----
+ * integration(Constant, ExpWmR2db), ConvexPolyhedron2.tcc:1747-1787: the reference integrates
+ * a tabulated piecewise-polynomial fit of the Gaussian (precision 1e-10).  Restated here as
+ * the exact integral over the fan of triangles (c, P0, P1) of the edges, each triangle
+ * written along its edge in polar coordinates about c with the radial part done exactly:
+ *   T = (P0 x P1) int_0^1 (eps / 2) (1 - exp(-q(u) / eps)) / q(u) du,  q(u) = |P0 + u (P1 - P0)|^2,
+ * integrated by adaptive Gauss-Legendre (8 points, bisection).  Boundary items (:346-389):
+ * coeff int_edge exp((w - |x - c|^2) / eps) ds, by the same adaptive rule. */
+static const double GL8X[4] = {0.1834346424956498, 0.5255324099163290, 0.7966664774136267, 0.9602898564975363};
+static const double GL8W[4] = {0.3626837833783620, 0.3137066458778873, 0.2223810344533745, 0.1012285362903763};
+
+typedef struct edge_fn {
+    double px, py, dx, dy, eps;
+    int kind;   /* 0: (1 - exp(-q/eps)) / q (mass), 1: exp(-q/eps) (edge item) */
+} edge_fn;
+
+static double edge_fn_eval(const edge_fn *f, double u) {
+    const double x = f->px + u * f->dx, y = f->py + u * f->dy, q = x * x + y * y;
+    if (f->kind == 1) return exp(-q / f->eps);
+    return q > 1e-300 * f->eps ? -expm1(-q / f->eps) / q : 1.0 / f->eps;
+}
+
+static double gl8(const edge_fn *f, double a, double b) {
+    const double m = 0.5 * (a + b), h = 0.5 * (b - a);
+    double s = 0;
+    for (int k = 0; k < 4; ++k) s += GL8W[k] * (edge_fn_eval(f, m - h * GL8X[k]) + edge_fn_eval(f, m + h * GL8X[k]));
+    return h * s;
+}
+
+/* bisection until the two halves agree with the whole to tol (an absolute bound fixed
+ * from the first estimate, so rounding noise cannot drive the recursion), at most 30 levels */
+static double gl8_adaptive(const edge_fn *f, double a, double b, double whole, double tol, int depth) {
+    const double m = 0.5 * (a + b), l = gl8(f, a, m), r = gl8(f, m, b);
+    if (depth >= 30 || fabs(l + r - whole) <= tol) return l + r;
+    return gl8_adaptive(f, a, m, l, tol, depth + 1) + gl8_adaptive(f, m, b, r, tol, depth + 1);
+}
+
+static double edge_integral(edge_fn *f) {
+    const double g = gl8(f, 0, 1);
+    return gl8_adaptive(f, 0, 1, g, 1e-14 * fabs(g) + 1e-300, 0);
+}
+
+double or_cp2_expwmr2db_integral(const or_cp2 *cp, double coeff, double w, double eps) {
+    double res = 0;
+    for (int i0 = cp->n - 1, i1 = 0; i1 < cp->n; i0 = i1++) {
+        const double px = cp->x[i0] - cp->scx, py = cp->y[i0] - cp->scy, qx = cp->x[i1] - cp->scx, qy = cp->y[i1] - cp->scy;
+        const double cr = px * qy - py * qx;
+        if (cr == 0) continue;
+        edge_fn f = {px, py, qx - px, qy - py, eps, 0};
+        res += cr * 0.5 * eps * edge_integral(&f);
+    }
+    return coeff * exp(w / eps) * res;
+}
+
+int or_cp2_boundary_items_expwmr2db(const or_cp2 *cp, double coeff, double w, double eps, double *measures, int64_t *ids,
+                                    int cap) {
+    int m = 0;
+    for (int i1 = 0, i0 = cp->n - 1; i1 < cp->n; i0 = i1++) {
+        if (m >= cap) return -1;
+        const double px = cp->x[i0] - cp->scx, py = cp->y[i0] - cp->scy, qx = cp->x[i1] - cp->scx, qy = cp->y[i1] - cp->scy;
+        const double d2 = (qx - px) * (qx - px) + (qy - py) * (qy - py);
+        if (d2 == 0) continue;
+        edge_fn f = {px, py, qx - px, qy - py, eps, 1};
+        measures[m] = coeff * exp(w / eps) * sqrt(d2) * edge_integral(&f);
+        ids[m++] = cp->id[i0];
+    }
+    return m;
+}
+
 /* add_centroid_contrib(Constant, Unit), ConvexPolyhedron2.tcc:1047-1073, _centroid_arc
  * :2296-2330: area and first moments of the polygon, as signed triangles and
  * circular sectors seen from (ox, oy) (the ball centre for arcs), relative to it */
@@ -788,8 +857,13 @@ static void piece2_accumulate(const cells_job *J, int64_t i, const or_cp2 *cp, d
     static __thread double meas[2 * OR_CP2_CAP + 2];
     static __thread int64_t ids[2 * OR_CP2_CAP + 2];
     int m;
-    const int rf = (J->flags >> 4) & 3;   /* 0 Unit, 1 R2, 2 WmR2 (the final integration function) */
-    if (!poly && rf) {
+    const int rf = (J->flags >> 4) & 3;   /* 0 Unit, 1 R2, 2 WmR2, 3 ExpWmR2db (the final integration function) */
+    if (!poly && rf == 3) {
+        const double eps = J->dom->rf_eps, mi = or_cp2_expwmr2db_integral(cp, coeff, J->w[i], eps);
+        *mass += mi;
+        m = or_cp2_boundary_items_expwmr2db(cp, coeff, J->w[i], eps, meas, ids, 2 * OR_CP2_CAP + 2);
+        row->der0 += mi / eps;   /* integration_der_wrt_weight(ExpWmR2db) = integration / eps (:124-127) */
+    } else if (!poly && rf) {
         *mass += or_cp2_radial_integral(cp, coeff, rf, J->w[i]);
         if (rf == 1) return;   /* R2 has no boundary items in the reference (TODO there) */
         m = or_cp2_boundary_items_wmr2(cp, coeff, J->w[i], meas, ids, 2 * OR_CP2_CAP + 2);

@@ -127,6 +127,9 @@ typedef struct or_domain {
      * under translation s carrying the cut id j + (s + 1) N (SpZGrid.tcc:558-561,610) */
     int64_t ntrans;
     const double *trans;
+    /* eps of FunctionEnum::ExpWmR2db (ExpWmR2db.h: exp((w - |x - c|^2) / eps)), the final
+     * integration function when the cells flags carry radial function 3 */
+    double rf_eps;
 } or_domain;
 
 /* Per-cell evaluation (get_der_integrals_wrt_weights.h:16-122 restated).

@@ -108,6 +108,7 @@ SIGNATURES = {
     "sdot_pd_set_ball_clip": (_int, [_p, _int]),
     "sdot_pd_set_translations": (_int, [_p, _p, _i64]),
     "sdot_pd_set_radial_func": (_int, [_p, _int]),
+    "sdot_pd_set_radial_func_eps": (_int, [_p, _int, _d]),
     "sdot_pd_compute": (_int, [_p]),
     "sdot_pd_integrals": (_int, [_p, _p]),
     "sdot_pd_integrals_noalloc": (_int, [_p, _p, _p, _i64, _p]),
@@ -276,11 +277,17 @@ class PowerDiagram:
         t = np.ascontiguousarray(np.asarray(translations, dtype=np.float64).reshape(-1, self.dim))
         self._check(lib().sdot_pd_set_translations(self._h, _ptr(t), len(t)), "set_translations")
 
-    RADIAL_FUNCS = {"unit": 0, "in_ball_w05": 1, "r2": 2, "r2_in_ball_w05": 3, "wmr2": 4, "pp_wmr2": 5}
+    RADIAL_FUNCS = {"unit": 0, "in_ball_w05": 1, "r2": 2, "r2_in_ball_w05": 3, "wmr2": 4, "pp_wmr2": 5,
+                    "exp_wmr2db": 6}
 
-    def set_radial_func(self, name):
-        """FunctionEnum of the integrals: unit, in_ball_w05, r2, r2_in_ball_w05, wmr2, pp_wmr2."""
-        self._check(lib().sdot_pd_set_radial_func(self._h, self.RADIAL_FUNCS[name]), "set_radial_func")
+    def set_radial_func(self, name, eps=None):
+        """FunctionEnum of the integrals: unit, in_ball_w05, r2, r2_in_ball_w05, wmr2, pp_wmr2,
+        exp_wmr2db (ExpWmR2db{eps}: exp((w - |x - c|^2) / eps))."""
+        if eps is None:
+            self._check(lib().sdot_pd_set_radial_func(self._h, self.RADIAL_FUNCS[name]), "set_radial_func")
+        else:
+            self._check(lib().sdot_pd_set_radial_func_eps(self._h, self.RADIAL_FUNCS[name], float(eps)),
+                        "set_radial_func_eps")
 
     def set_ball_clip(self, on=True):
         self._check(lib().sdot_pd_set_ball_clip(self._h, int(bool(on))), "set_ball_clip")

@@ -182,9 +182,9 @@ sdot_status compute_pass(sdot_pd *pd) {
         throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping in 3D (ConvexPolyhedron3::ball_cut is TODO in the reference)"};
     if (pd->ball && pd->has_image) throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping with an image density"};
     if (pd->rf && ((pd->has_image && pd->img_nc != 1) || (pd->dim == 3 && (pd->has_image || pd->shapes.size() > 1))))
-        throw StatusError{SDOT_NOT_SUPPORTED, "R2 / WmR2 integration: constant pieces in 2D, one domain shape in 3D"};
-    if (pd->rf == 2 && pd->dim == 3)
-        throw StatusError{SDOT_NOT_SUPPORTED, "WmR2 in 3D (ConvexPolyhedron3 integration is TODO in the reference)"};
+        throw StatusError{SDOT_NOT_SUPPORTED, "R2 / WmR2 / ExpWmR2db integration: constant pieces in 2D, one domain shape in 3D"};
+    if (pd->rf >= 2 && pd->dim == 3)
+        throw StatusError{SDOT_NOT_SUPPORTED, "WmR2 / ExpWmR2db in 3D (ConvexPolyhedron3 integration is TODO in the reference)"};
     if (pd->rf && (pd->want_mom || pd->want_jac))
         throw StatusError{SDOT_NOT_SUPPORTED, "centroids and position derivatives: Unit / InBallW05 only"};
     if (pd->N > (int64_t)0x7FFFFFFF) throw StatusError{SDOT_INVALID_ARG, "more than 2^31-1 diracs"};
@@ -238,6 +238,7 @@ sdot_status compute_pass(sdot_pd *pd) {
     }
     pd->pass_has_mom = pd->want_mom;
     a.rf = pd->rf;
+    a.rf_eps = pd->rf_eps;
     a.jblk = a.jdiag = nullptr;
     if (pd->want_jac) {
         const size_t nb = (size_t)(pd->dim + 1) * (pd->dim + 1);
@@ -648,12 +649,19 @@ sdot_status sdot_pd_set_radial_func(sdot_pd *pd, int func) {
     if (!pd) return SDOT_INVALID_ARG;
     // FunctionEnum: the ball cut (need_ball_cut) and the final integration function
     // (func_for_final_cp_integration) of each
-    static const int ball[6] = {0, 1, 0, 1, 0, 1}, rf[6] = {0, 0, 1, 1, 2, 2};
-    if (func < 0 || func > 5) return SDOT_INVALID_ARG;
+    static const int ball[7] = {0, 1, 0, 1, 0, 1, 0}, rf[7] = {0, 0, 1, 1, 2, 2, 3};
+    if (func < 0 || func > 6) return SDOT_INVALID_ARG;
     pd->ball = ball[func];
     pd->rf = rf[func];
     pd->pass_valid = false;
     return SDOT_OK;
+}
+
+sdot_status sdot_pd_set_radial_func_eps(sdot_pd *pd, int func, double eps) {
+    if (!pd || !(eps > 0) || !std::isfinite(eps)) return SDOT_INVALID_ARG;
+    const sdot_status st = sdot_pd_set_radial_func(pd, func);
+    if (st == SDOT_OK) pd->rf_eps = eps;
+    return st;
 }
 
 sdot_status sdot_pd_set_tuning(sdot_pd *pd, int dpc_x100, int block) {

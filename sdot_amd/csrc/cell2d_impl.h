@@ -78,6 +78,82 @@ SDOT_HD inline float my_atan2<float>(float y, float x) { return atan2f(y, x); }
 constexpr double TWO_PI = 6.283185307179586476925286766559;
 constexpr double PI_D = 3.1415926535897932384626433832795;
 
+// ---- ExpWmR2db (ExpWmR2db.h: f = exp((w - |x - c|^2) / eps)), coordinates about c ----
+// The reference integrates f over a polygon with a generated table of piecewise radial
+// polynomials (ConvexPolyhedron2.tcc:1747-1787, precision 1e-10).  Here: the exact
+// integral over the fan of triangles (c, P0, P1).  In polar coordinates about c the
+// radial part of a triangle is exact, (eps / 2)(1 - exp(-R(theta)^2 / eps)); with rho the
+// distance from c to the edge's line, a = rho / sqrt(eps) and s the offset along the line
+// from the foot of the perpendicular over sqrt(eps), the angular part becomes
+//   T = (eps / 2) [dtheta - sign(P0 x P1) exp(-a^2) (Lam(s1) - Lam(s0))],
+//   Lam(s) = a int_0^s exp(-t^2) / (a^2 + t^2) dt,
+// and Lam(s) = atan(s / a) + a int_0^s expm1(-t^2) / (a^2 + t^2) dt splits off the Lorentzian
+// exactly; the remaining integrand is bounded and smooth away from t ~ a, integrated by
+// 16-point Gauss-Legendre on [0, a], [a, 1] (in log t) and [1, s], s capped at 6.5 (the
+// tail is below exp(-42)).  Agreement with adaptive 30-digit quadrature: 5e-13 of the
+// bracket.
+constexpr double GL16X[8] = {0.095012509837637454, 0.28160355077925892, 0.45801677765722737, 0.61787624440264377,
+                             0.755404408355003,    0.86563120238783176, 0.9445750230732326,  0.98940093499164994};
+constexpr double GL16W[8] = {0.18945061045506859, 0.18260341504492361, 0.16915651939500262, 0.14959598881657676,
+                             0.12462897125553403, 0.095158511682492591, 0.062253523938647706, 0.027152459411754037};
+
+// Lam(s) for s >= 0
+SDOT_HD inline double gauss_lorentz(double a, double s) {
+    const double S = fmin(s, 6.5);
+    if (!(S > 0)) return 0.0;
+    if (!(a > 1e-300)) return 0.5 * PI_D;
+    const double a2 = a * a;
+    auto f = [&](double t) { return expm1(-t * t) / (a2 + t * t); };
+    auto gl = [&](double lo, double hi, bool logt) {
+        const double m = 0.5 * (lo + hi), h = 0.5 * (hi - lo);
+        double r = 0;
+        for (int k = 0; k < 8; ++k) {
+            double u = m - h * GL16X[k], v = m + h * GL16X[k];
+            if (logt) {
+                u = exp(u);
+                v = exp(v);
+                r += GL16W[k] * (f(u) * u + f(v) * v);
+            } else {
+                r += GL16W[k] * (f(u) + f(v));
+            }
+        }
+        return h * r;
+    };
+    const double b1 = fmin(a, S), b2 = fmin(fmax(a, 1.0), S);
+    double tot = gl(0.0, b1, false);
+    if (b2 > b1) tot += gl(log(b1), log(b2), true);
+    if (S > b2) tot += gl(b2, S, false);
+    return atan(S / a) + a * tot;
+}
+
+// int over the triangle (0, P0, P1) of exp(-|x|^2 / eps), signed like P0 x P1
+SDOT_HD inline double exp_triangle(double x0, double y0, double x1, double y1, double eps) {
+    const double cr = x0 * y1 - y0 * x1;
+    if (cr == 0) return 0.0;
+    const double dx = x1 - x0, dy = y1 - y0, d = sqrt(dx * dx + dy * dy), se = sqrt(eps);
+    const double a = fabs(cr) / (d * se);
+    const double s0 = (x0 * dx + y0 * dy) / (d * se), s1 = (x1 * dx + y1 * dy) / (d * se);
+    const double dth = atan2(cr, x0 * x1 + y0 * y1);
+    double e = 0;
+    if (a * a < 45) {
+        const double l1 = s1 >= 0 ? gauss_lorentz(a, s1) : -gauss_lorentz(a, -s1);
+        const double l0 = s0 >= 0 ? gauss_lorentz(a, s0) : -gauss_lorentz(a, -s0);
+        e = (cr > 0 ? 1.0 : -1.0) * exp(-a * a) * (l1 - l0);
+    }
+    return 0.5 * eps * (dth - e);
+}
+
+// int along the segment P0 -> P1 of exp((w - |x|^2) / eps) ds (erf closed form; erfc
+// differences when both ends lie on one side of the foot); 0 for a zero-length edge
+SDOT_HD inline double exp_edge(double x0, double y0, double x1, double y1, double w, double eps) {
+    const double dx = x1 - x0, dy = y1 - y0, d2 = dx * dx + dy * dy;
+    if (d2 == 0) return 0.0;
+    const double d = sqrt(d2), se = sqrt(eps), cr = x0 * y1 - y0 * x1;
+    const double s0 = (x0 * dx + y0 * dy) / (d * se), s1 = (x1 * dx + y1 * dy) / (d * se);
+    const double diff = s0 >= 0 ? erfc(s0) - erfc(s1) : (s1 <= 0 ? erfc(-s1) - erfc(-s0) : erf(s1) - erf(s0));
+    return 0.5 * sqrt(PI_D * eps) * exp((w - cr * cr / d2) / eps) * diff;
+}
+
 // Accumulates the boundary items of the final cell (line pieces with a cut id,
 // arcs of the ball) into the integrals and the Hessian row.
 // pos_integral C and momentum M of a straight boundary item p0 -> p1 (local coordinates
@@ -125,13 +201,15 @@ SDOT_HD inline void jac_edge(const CellArgs &a, int64_t t, double c0x, double c0
 // and unknown e (position x, y, weight), before the mass / centroid correction.
 // RF: the final integration function -- 0 Unit, 1 R2 (|x - c|^2, integrals only), 2 WmR2
 // (w - |x - c|^2): integration(Constant, R2 / WmR2) and for_each_boundary_item(Constant,
-// WmR2), ConvexPolyhedron2.tcc:2044-2132 and :391-430.
+// WmR2), ConvexPolyhedron2.tcc:2044-2132 and :391-430; 3 ExpWmR2db (exp((w - |x - c|^2) /
+// eps)): integration :1747-1787 and boundary items :346-389.
 template <class T, bool PER = false, bool JAC = false, int RF = 0>
 struct Integrator {
     const CellArgs *a;
     int64_t t;
     double c0x, c0y;
     double w0v = 0;      // the dirac's weight (RF)
+    double ew = 1;       // exp(w / eps) (RF = 3, ExpWmR2db)
     double rint = 0;     // integral of the radial function over the fan of items (RF)
     double jd[JAC ? 9 : 1] = {};
     T r, r2;
@@ -152,7 +230,12 @@ struct Integrator {
     SDOT_HD void line(T x0, T y0, T x1, T y1, int id) {
         T ex = x1 - x0, ey = y1 - y0;
         double len = (double)sqrt(ex * ex + ey * ey);
-        if (RF) {
+        if (RF == 3) {   // ExpWmR2db: the fan triangle, and the edge item (ConvexPolyhedron2.tcc:346-389)
+            const double X0 = x0, Y0 = y0, X1 = x1, Y1 = y1;
+            rint += ew * exp_triangle(X0, Y0, X1, Y1, a->rf_eps);
+            if (len == 0) return;   // a zero-length edge carries no item
+            len = exp_edge(X0, Y0, X1, Y1, w0v, a->rf_eps);
+        } else if (RF) {
             // triangle (dirac, p0, p1): integral of |x - c|^2 = A (|p0|^2 + |p1|^2 + p0.p1) / 6
             const double X0 = x0, Y0 = y0, X1 = x1, Y1 = y1;
             const double at = 0.5 * (X0 * Y1 - Y0 * X1);
@@ -553,6 +636,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     // intersection of its two lines, so the integrals carry no FP32 error.
     Integrator<double, PER, JAC, RF> it;
     it.w0v = w0;
+    if (RF == 3) it.ew = exp(w0 / a.rf_eps);
     it.a = &a;
     it.t = t;
     it.c0x = c0x;
@@ -616,12 +700,13 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             };
             const IdPoly2<PC> &pc = *src;
             // area moments of the piece (Green's theorem over its edges)
-            double m0 = 0, sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0;
+            double m0 = 0, sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0, tri = 0;
             for (int k = 0; k < pc.n; ++k) {
                 const int k1 = k + 1 == pc.n ? 0 : k + 1;
                 const double x0 = pc.x[k], y0 = pc.y[k], x1 = pc.x[k1], y1 = pc.y[k1];
                 const double cr = x0 * y1 - x1 * y0;
                 m0 += cr;
+                if (RF == 3) tri += exp_triangle(x0, y0, x1, y1, a.rf_eps);   // the piece's fan about the dirac
                 if (q || a.mom) {
                     sx += (x0 + x1) * cr;
                     sy += (y0 + y1) * cr;
@@ -633,7 +718,8 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                 }
                 if (pc.e[k] < 0 || RF == 1) continue;   // R2: no boundary items (TODO in the reference)
                 const double ex = x1 - x0, ey = y1 - y0;
-                const double len = sqrt(ex * ex + ey * ey);
+                // ExpWmR2db: the edge piece's integral of exp((w - |x - c|^2) / eps)
+                const double len = RF == 3 ? exp_edge(x0, y0, x1, y1, w0, a.rf_eps) : sqrt(ex * ex + ey * ey);
                 // density integrated along the edge piece (Simpson, exact for a quadratic);
                 // WmR2: times the mean of w - |x - c|^2 over the edge
                 const double dens =
@@ -647,7 +733,9 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                         break;
                     }
             }
-            if (RF) {   // constant pieces: the second moment about the dirac, (w - |x - c|^2) for WmR2
+            if (RF == 3) {
+                mass_sum += rho * exp(w0 / a.rf_eps) * tri;
+            } else if (RF) {   // constant pieces: the second moment about the dirac, (w - |x - c|^2) for WmR2
                 const double r2 = (sxx + syy) / 12;
                 mass_sum += rho * (RF == 1 ? r2 : w0 * 0.5 * m0 - r2);
                 rf_area += rho * 0.5 * m0;
@@ -753,7 +841,8 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             for (int k = 0; k < 9; ++k) a.jdiag[(int64_t)k * a.ell_stride + t] = jdp[k];
         a.arc[t] = 0;
         a.perim[t] = perim;
-        a.diag[t] = der_line + (RF == 2 ? rf_area : 0.0);   // integration_der_wrt_weight(WmR2) = the measure
+        // integration_der_wrt_weight: WmR2 the measure, ExpWmR2db the integral / eps (:124-138)
+        a.diag[t] = der_line + (RF == 2 ? rf_area : RF == 3 ? mass_sum / a.rf_eps : 0.0);
         a.row_len[t] = nrow;
         if (a.ctr) {
             const unsigned long long K = n;
@@ -892,6 +981,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     const double coeff = a.coeff;
     const double mass = coeff * (RF ? it.rint : area);
     if (RF == 2) it.der_line += coeff * area;   // integration_der_wrt_weight(WmR2) = the measure (:135-138)
+    if (RF == 3) it.der_line += mass / a.rf_eps;   // integration_der_wrt_weight(ExpWmR2db) = integral / eps (:124-127)
     a.mass[t] = mass;
     if (a.mom) {
         a.mom[2 * t] = coeff * it.mx;

@@ -288,23 +288,25 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
         return tier == 1 ? nb_all : std::min(nb_all, 4096u);
     };
     if (a.rf) {
-        // R2 / WmR2 integration functions: one FP64 tier of 64 vertices (pieces: constant
-        // pixels or several shapes, no ball)
+        // R2 / WmR2 / ExpWmR2db integration functions: one FP64 tier of 64 vertices (pieces:
+        // constant pixels or several shapes; ExpWmR2db has no ball cut)
         unsigned nb = next(true);
         auto go = [&](auto rfc) {
             constexpr int RF = decltype(rfc)::value;
+            constexpr bool BL = RF != 3;   // instantiations with the ball clip
             if (a.ntrans > 0) {
-                if (ball) launch2<double, 64, 64, true, false, 1, false, true, false, RF>(a, nb, s);
+                if (BL && ball) launch2<double, 64, 64, BL, false, 1, false, true, false, RF>(a, nb, s);
                 else if (img) launch2<double, 64, 64, false, true, 1, false, true, false, RF>(a, nb, s);
                 else launch2<double, 64, 64, false, false, 1, false, true, false, RF>(a, nb, s);
             } else {
-                if (ball) launch2<double, 64, 64, true, false, 1, false, false, false, RF>(a, nb, s);
+                if (BL && ball) launch2<double, 64, 64, BL, false, 1, false, false, false, RF>(a, nb, s);
                 else if (img) launch2<double, 64, 64, false, true, 1, false, false, false, RF>(a, nb, s);
                 else launch2<double, 64, 64, false, false, 1, false, false, false, RF>(a, nb, s);
             }
         };
         if (a.rf == 1) go(std::integral_constant<int, 1>());
-        else go(std::integral_constant<int, 2>());
+        else if (a.rf == 2) go(std::integral_constant<int, 2>());
+        else go(std::integral_constant<int, 3>());
     } else if (a.jdiag) {
         // position / weight derivatives: one FP64 tier of 64 vertices (pieces: constant
         // pixels or several shapes, no ball)

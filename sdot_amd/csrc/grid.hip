@@ -305,6 +305,13 @@ GridInfo grid_params(int dim, const double *lo_in, const double *hi_in, int64_t 
     // or a plane still get cells of comparable size along every axis
     double lmax = 0;
     for (int d = 0; d < dim; ++d) lmax = fmax(lmax, hi[d] - lo[d]);
+    if (!(lmax > 0)) {   // every dirac at one point (one dirac): a unit extent around it
+        for (int d = 0; d < dim; ++d) {
+            lo[d] -= 0.5;
+            hi[d] += 0.5;
+        }
+        lmax = 1;
+    }
     for (int d = 0; d < dim; ++d) {
         double l = hi[d] - lo[d];
         double scale = fmax(fabs(lo[d]), fabs(hi[d]));
@@ -316,6 +323,11 @@ GridInfo grid_params(int dim, const double *lo_in, const double *hi_in, int64_t 
         vol *= L[d];
     }
     double h = pow(vol * dpc / (double)(N > 0 ? N : 1), 1.0 / dim);
+    if (!(vol > 1e-290)) {   // extents so small that their product underflows: the same cell size from logarithms
+        double lh = log(dpc / (double)(N > 0 ? N : 1));
+        for (int d = 0; d < dim; ++d) lh += log(L[d]);
+        h = exp(lh / dim);
+    }
     int maxc = dim == 2 ? 32768 : 1024;
     int mx = 1;
     for (int d = 0; d < dim; ++d) {
@@ -348,7 +360,7 @@ void refresh_weights(sdot_pd *pd) {
     for (int d = 0; d < pd->dim; ++d) vol *= g.h[d] * g.ncell[d];
     const double spacing2 = pow(vol / (double)(N > 0 ? N : 1), 2.0 / pd->dim);
     hipLaunchKernelGGL(k_gather_w, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->w.p, pd->perm.p, N, pd->sw.p, pd->flags.p,
-                       1e-12 * spacing2);
+                       N > 1 ? 1e-12 * spacing2 : 0.0);   // a lone dirac has no ties to break
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, pd->wglob.p, dbl_order(-DBL_MAX));
     hipLaunchKernelGGL(k_cell_wmax, dim3(nblk(g.ncodes, BS)), dim3(BS), 0, s, pd->cell_start.p, g.ncodes, pd->sw.p,
                        pd->cell_wmax.p, pd->wglob.p);

@@ -206,8 +206,9 @@ struct CellArgs {
     // the cut id j + (s + 1) N (single problems only; N (ntrans + 1) < 2^31)
     int ntrans;
     const double *trans;
-    // final integration function (2D, one domain shape): 0 Unit, 1 R2, 2 WmR2
+    // final integration function (2D): 0 Unit, 1 R2, 2 WmR2, 3 ExpWmR2db (exp((w - |x - c|^2) / rf_eps))
     int rf;
+    double rf_eps;
     // position / weight derivative blocks (9 doubles): per neighbour slot e at
     // jblk[(e * 9 + k) * ell_stride + t], the diagonal at jdiag[k * ell_stride + t]
     double *jblk, *jdiag;
@@ -377,7 +378,8 @@ struct sdot_pd {
     std::vector<double> htrans;                             // translations (dim per vector)
     sdot_amd::DevBuf<double> trans;
     bool want_jac = false;                                  // next pass writes the derivative blocks
-    int rf = 0;                                             // final integration function: 0 Unit, 1 R2, 2 WmR2
+    int rf = 0;                                             // final integration function: 0 Unit, 1 R2, 2 WmR2, 3 ExpWmR2db
+    double rf_eps = 1.0;                                    // eps of ExpWmR2db
     sdot_amd::DevBuf<double> jblk, jdiag;
     sdot_amd::DevBuf<int> it_off;
     bool want_mom = false;          // the next pass computes the first moments
@@ -458,6 +460,7 @@ void export_csr(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, int6
 void export_jac(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, double *h_v, int64_t cap, int64_t *nnz);
 void export_vec(sdot_pd *pd, const double *d_sorted, double *h_out);  // sorted -> dirac order, to host
 void export_vec_device(sdot_pd *pd, const double *d_sorted, double *d_out);  // sorted -> dirac order, in HBM
+int64_t count_offdiag(sdot_pd *pd);   // off-diagonal Hessian entries (ELL slots) of the last pass
 void export_centroids(sdot_pd *pd, double *h_out);   // position + moment / mass (0 when empty), dirac order
 // capi.hip
 sdot_status compute_pass(sdot_pd *pd);  // grid (if needed) + weights + cell pass; flags -> status

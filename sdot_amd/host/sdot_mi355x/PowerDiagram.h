@@ -9,7 +9,7 @@
 //   ConvexPolyhedronAssembly<Pc>::add_convex_polyhedron / add_simplex / add_box / measure / normalize
 //                                                   src/sdot/Domains/ConvexPolyhedronAssembly.h:26-40
 //   ScaledImage<Pc>(min, max, data, sizes, nb_coeffs) src/sdot/Domains/ScaledImage.h:26
-//   FunctionEnum::Unit, InBallW05, R2, R2InBallW05, WmR2, PpWmR2
+//   FunctionEnum::Unit, InBallW05, R2, R2InBallW05, WmR2, PpWmR2, ExpWmR2db<TF>{eps}
 //                                                   src/sdot/Integration/*.h
 //   get_integrals(res, grid, bounds, pos, w, N, rf) src/sdot/PowerDiagram/get_integrals.h:13
 //   get_centroids(grid, bounds, pos, w, N, rf, cb)  src/sdot/PowerDiagram/get_centroids.h:14
@@ -76,7 +76,22 @@ struct WmR2 {        // w_i - |x - x_i|^2
 struct PpWmR2 {      // its positive part (ball cut)
     static constexpr int ball = 1, code = SDOT_FUNC_PP_WMR2;
 };
+template <class TS>
+struct ExpWmR2db {   // exp((w_i - |x - x_i|^2) / eps), no ball cut (ExpWmR2db.h), 2D
+    static constexpr int ball = 0, code = SDOT_FUNC_EXP_WMR2_DB;
+    TS eps;
+};
 }  // namespace FunctionEnum
+
+/// the radial function on the handle (with its parameter for ExpWmR2db)
+template <class Func>
+inline int set_radial_func(sdot_pd *pd, const Func &) {
+    return sdot_pd_set_radial_func(pd, Func::code);
+}
+template <class TS>
+inline int set_radial_func(sdot_pd *pd, const FunctionEnum::ExpWmR2db<TS> &f) {
+    return sdot_pd_set_radial_func_eps(pd, f.code, (double)f.eps);
+}
 
 /// Pc: the reference's traits struct (dim; TF = double; TI = index type).
 template <class Pc>
@@ -373,10 +388,10 @@ private:
 /// get_integrals (get_integrals.h:13): res[i] = integral of the density over cell i.
 template <class TF, class Grid, class Bounds, class Pt, class Func>
 void get_integrals(TF *res, Grid &grid, Bounds &bounds, const Pt *positions, const TF *weights, std::size_t nb_diracs,
-                   const Func &) {
+                   const Func &func) {
     grid.update(positions, weights, nb_diracs, false, true);
     bounds.apply(grid.handle());
-    check(grid.handle(), sdot_pd_set_radial_func(grid.handle(), Func::code), "radial function");
+    check(grid.handle(), set_radial_func(grid.handle(), func), "radial function");
     check(grid.handle(), sdot_pd_integrals(grid.handle(), res), "get_integrals", true);
 }
 
@@ -388,13 +403,13 @@ void get_integrals(TF *res, Grid &grid, Bounds &bounds, const Pt *positions, con
 /// get_centroids (get_centroids.h:14, rand_ratio = 0): cb(centroid, mass, num_dirac) for
 /// every cell, in dirac order; the centroid of an empty cell is 0 (get_centroids.h:58).
 template <class Grid, class Bounds, class Pt, class TF, class TI, class Func, class CB>
-void get_centroids(Grid &grid, Bounds &bounds, const Pt *positions, const TF *weights, TI nb_diracs, const Func &,
+void get_centroids(Grid &grid, Bounds &bounds, const Pt *positions, const TF *weights, TI nb_diracs, const Func &func,
                    const CB &cb, TF rand_ratio = TF(0)) {
     if (rand_ratio != TF(0)) throw Error{SDOT_NOT_SUPPORTED, "get_centroids: rand_ratio (random points) is not offered"};
     constexpr int dim = sizeof(Pt) / sizeof(double);
     grid.update(positions, weights, nb_diracs, false, true);
     bounds.apply(grid.handle());
-    check(grid.handle(), sdot_pd_set_radial_func(grid.handle(), Func::code), "radial function");
+    check(grid.handle(), set_radial_func(grid.handle(), func), "radial function");
     std::vector<double> c((std::size_t)nb_diracs * dim), m(nb_diracs);
     check(grid.handle(), sdot_pd_centroids(grid.handle(), c.data(), m.data()), "get_centroids", true);
     for (std::size_t i = 0; i < (std::size_t)nb_diracs; ++i) {
@@ -415,11 +430,11 @@ void get_centroids(Grid &grid, Bounds &bounds, const Pt *positions, const TF *we
 template <class TI, class TF, class Grid, class Bounds, class Pt, class Func>
 int get_der_integrals_wrt_weights(std::vector<TI> &m_offsets, std::vector<TI> &m_columns, std::vector<TF> &m_values,
                                   std::vector<TF> &v_values, Grid &grid, Bounds &bounds, const Pt *positions,
-                                  const TF *weights, std::size_t nb_diracs, const Func &, bool stop_if_void = true) {
+                                  const TF *weights, std::size_t nb_diracs, const Func &func, bool stop_if_void = true) {
     grid.update(positions, weights, nb_diracs, false, true);
     sdot_pd *pd = grid.handle();
     bounds.apply(pd);
-    check(pd, sdot_pd_set_radial_func(pd, Func::code), "radial function");
+    check(pd, set_radial_func(pd, func), "radial function");
     std::int64_t nnz = 0;
     int st = sdot_pd_der_integrals_wrt_weights(pd, nullptr, nullptr, nullptr, nullptr, 0, &nnz);
     check(pd, st, "get_der_integrals_wrt_weights", true);
@@ -452,12 +467,12 @@ template <class TI, class TF, class Grid, class Bounds, class Pt, class Func>
 int get_der_centroids_and_integrals_wrt_weight_and_positions(std::vector<TI> &m_offsets, std::vector<TI> &m_columns,
                                                              std::vector<TF> &m_values, std::vector<TF> &v_values,
                                                              Grid &grid, Bounds &bounds, const Pt *positions,
-                                                             const TF *weights, std::size_t nb_diracs, const Func &,
+                                                             const TF *weights, std::size_t nb_diracs, const Func &func,
                                                              bool stop_if_void = true) {
     grid.update(positions, weights, nb_diracs, false, true);
     sdot_pd *pd = grid.handle();
     bounds.apply(pd);
-    check(pd, sdot_pd_set_radial_func(pd, Func::code), "radial function");
+    check(pd, set_radial_func(pd, func), "radial function");
     std::int64_t nnz = 0;
     int st = sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions(pd, nullptr, nullptr, nullptr, nullptr, 0, &nnz);
     check(pd, st, "get_der_centroids_and_integrals_wrt_weight_and_positions", true);
@@ -511,11 +526,11 @@ public:
 
     /// stop_rule 1 (default, as the reference): max |dw| < tol; 0: max relative mass error < tol
     template <class Pt, class TI, class RadialFunc>
-    int solve(const Pt *positions, TF *weights, TI nb_diracs, TF *masses, RadialFunc) {
+    int solve(const Pt *positions, TF *weights, TI nb_diracs, TF *masses, RadialFunc func) {
         grid.update(positions, weights, nb_diracs, true, true);
         sdot_pd *pd = grid.handle();
         bounds.apply(pd);
-        check(pd, sdot_pd_set_radial_func(pd, RadialFunc::code), "radial function");
+        check(pd, set_radial_func(pd, func), "radial function");
         int st = sdot_ot_solve(pd, masses, tol, stop_rule, (int)max_nb_iter, weights, &stats);
         if (st != SDOT_OK && st != SDOT_NOT_CONVERGED) check(pd, st, "OptimalTransportSolver::solve");
         return st;

@@ -46,6 +46,9 @@ def test_cells2d_edge_cases(precision):
     # a single dirac: the cell is the whole box
     g = gpu_pass(np.array([[0.3, 0.6]]), np.array([0.0]), 2, precision)
     assert abs(g["masses"][0] - 1) < 1e-14 and len(g["csr"][1]) == 1
+    # a lone dirac at the origin (a bounding box of zero extent at 0: one grid cell)
+    g = gpu_pass(np.array([[0.0, 0.0]]), np.array([0.0]), 2, precision)
+    assert abs(g["masses"][0] - 1) < 1e-14 and g["timings"]["grid_cells"] == 1
     # two diracs, unequal weights: exact split x = 0.5 + (w0 - w1)/(2*0.5) shifted
     pos = np.array([[0.25, 0.5], [0.75, 0.5]])
     g = gpu_pass(pos, np.array([0.05, 0.0]), 2, precision)

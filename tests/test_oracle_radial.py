@@ -145,3 +145,69 @@ def test_wmr2_hessian_over_pieces_against_finite_differences(case):
         wm[j] -= h
         fd = (O.cells(pos, wp, radial="wmr2", **kw)["masses"] - O.cells(pos, wm, radial="wmr2", **kw)["masses"]) / (2 * h)
         assert np.abs(fd - H[:, j]).max() < 1e-7 * max(np.abs(H).max(), 1e-12)
+
+
+# ---- ExpWmR2db{eps}: exp((w - |x - c|^2) / eps) (ExpWmR2db.h) ----
+def _box_exp(x0, y0, eps, w, lo=(0.0, 0.0), hi=(1.0, 1.0)):
+    """closed form over a box: exp(w / eps) (pi eps / 4) prod_d (erf((hi - c) / se) - erf((lo - c) / se))"""
+    from scipy.special import erf
+    se = math.sqrt(eps)
+    f = 1.0
+    for c, a, b in ((x0, lo[0], hi[0]), (y0, lo[1], hi[1])):
+        f *= erf((b - c) / se) - erf((a - c) / se)
+    return math.exp(w / eps) * math.pi * eps / 4 * f
+
+
+def test_expwmr2db_reference_known_value_and_box_closed_forms():
+    # the reference's own test (tests/test_ConvexPolyhedron2.cpp:340-343): the box [0,2]x[0,1]
+    # seen from its corner, eps = 1: 0.6587596697261 (the reference checks 1e-5)
+    r = O.cells(np.array([[0.0, 0.0]]), np.zeros(1), domain=([0, 0], [2, 1], 1.0), radial="expwmr2db", eps=1.0)
+    assert abs(r["masses"][0] - 0.6587596697261) < 1e-12
+    # the whole box from any dirac, several eps and weights, and a density coefficient
+    for (x0, y0, eps, w) in [(0.3, 0.6, 0.05, 0.0), (0.5, 0.5, 1e-3, 1e-3), (0.9, 0.1, 0.3, -0.2),
+                             (0.001, 0.999, 0.01, 0.0), (-0.3, 1.4, 0.5, 0.1)]:
+        r = O.cells(np.array([[x0, y0]]), np.array([w]), radial="expwmr2db", eps=eps)
+        assert abs(r["masses"][0] / _box_exp(x0, y0, eps, w) - 1) < 1e-13, (x0, y0, eps)
+    r = O.cells(np.array([[0.2, 0.3]]), np.zeros(1), domain=([-1, 0], [2, 0.5], 2.5), radial="expwmr2db", eps=0.2)
+    assert abs(r["masses"][0] / (2.5 * _box_exp(0.2, 0.3, 0.2, 0.0, (-1, 0), (2, 0.5))) - 1) < 1e-13
+
+
+def test_expwmr2db_two_cells_split_the_box_and_their_shared_edge():
+    # two diracs: the cells are the two half boxes of the radical line x = xe; the masses are
+    # box closed forms and the Hessian entry is -0.5 * (edge integral) / |x0 - x1|, with the
+    # edge integral exp((w0 - (xe - x0)^2) / eps) sqrt(pi eps) / 2 (erf((1 - y)/se) + erf(y/se))
+    from scipy.special import erf
+    eps = 0.04
+    pos = np.array([[0.3, 0.45], [0.7, 0.45]])
+    w = np.array([0.01, 0.0])
+    xe = 0.5 + 0.5 * (w[0] - w[1]) / 0.4
+    r = O.cells(pos, w, radial="expwmr2db", eps=eps)
+    m0 = _box_exp(0.3, 0.45, eps, w[0], (0, 0), (xe, 1))
+    m1 = _box_exp(0.7, 0.45, eps, w[1], (xe, 0), (1, 1))
+    assert np.allclose(r["masses"], [m0, m1], rtol=1e-13, atol=0)
+    se = math.sqrt(eps)
+    g = math.exp((w[0] - (xe - 0.3) ** 2) / eps) * math.sqrt(math.pi * eps) / 2 * (erf(0.55 / se) + erf(0.45 / se))
+    off, cols, vals = O.to_csr(r)
+    M = sp.csr_matrix((vals, cols, off), shape=(2, 2)).toarray()
+    assert abs(M[0, 1] + 0.5 * g / 0.4) < 1e-13 * abs(M[0, 1])
+    assert abs(M[1, 0] - M[0, 1]) < 1e-13 * abs(M[0, 1])     # the edge integral is the same from both sides
+    assert abs(M[0, 0] - (0.5 * g / 0.4 + m0 / eps)) < 1e-12 * M[0, 0]   # + integration / eps (:124-127)
+
+
+def test_expwmr2db_hessian_against_finite_differences():
+    n = 40
+    u = O.rng_uniform(19, 3 * n)
+    pos = u[: 2 * n].reshape(n, 2).copy()
+    eps = 0.01
+    w = 0.02 * u[2 * n:]
+    r = O.cells(pos, w, radial="expwmr2db", eps=eps)
+    off, cols, vals = O.to_csr(r)
+    M = sp.csr_matrix((vals, cols, off), shape=(n, n)).toarray()
+    h = 1e-7
+    for j in range(n):
+        wp, wm = w.copy(), w.copy()
+        wp[j] += h
+        wm[j] -= h
+        fd = (O.cells(pos, wp, radial="expwmr2db", eps=eps)["masses"] -
+              O.cells(pos, wm, radial="expwmr2db", eps=eps)["masses"]) / (2 * h)
+        assert np.abs(fd - M[:, j]).max() < 1e-6 * np.abs(M).max()
