@@ -85,9 +85,11 @@ sdot_status sdot_pd_add_simplex_shape(sdot_pd *pd, const double *points, double 
 /* Several shapes: each cell is integrated piece by piece over the shapes
  * (for_each_intersection, ConvexPolyhedronAssembly.tcc:132-142), densities adding up
  * where shapes overlap; the perimeter then counts the faces shared with other cells
- * only.  Not with the ball clip. */
+ * (and the ball arcs) only.  With the ball clip each piece is also clipped by the
+ * cell's disc (intersect_with, ConvexPolyhedron2.tcc:2474-2496); position derivatives
+ * of a ball-clipped cell over several pieces return SDOT_NOT_SUPPORTED. */
 /* ScaledImage(min_pt, max_pt, data, sizes, nb_coeffs) -- ScaledImage.h:26; sizes x fastest.
- * nb_coeffs = 1: constant per pixel.  nb_coeffs = 6 (2D only, no ball): the density in
+ * nb_coeffs = 1: constant per pixel (ball clip allowed: pieces as for several shapes).  nb_coeffs = 6 (2D only, no ball): the density in
  * pixel p is c0 + c1 x + c2 y + c3 x^2 + c4 x y + c5 y^2 in domain coordinates, with
  * c_k = data[k * nb_pixels + p] (apply_poly, Polynomial.h:37-50). */
 sdot_status sdot_pd_set_image_density(sdot_pd *pd, const double *min_pt, const double *max_pt, const double *data,
@@ -150,7 +152,7 @@ sdot_status sdot_pd_integrals_noalloc(sdot_pd *pd, const double *d_positions, co
                                       int64_t nb_diracs, double *d_masses);
 /* get_der_centroids_and_integrals_wrt_weight_and_positions(m_offsets, m_columns, m_values,
  * v_values, ...) -- get_der_centroids_and_integrals_wrt_weight_and_positions.h:16 (2D:
- * one domain shape with or without the ball, several shapes or constant pixels; 3D: one
+ * one domain shape with or without the ball, several shapes or constant pixels without it; 3D: one
  * shape; translations in both; quadratic pixels: SDOT_NOT_SUPPORTED).  With nu = dim + 1: rows
  * nu n + d (d < dim: centroid coordinate d; d = dim: mass of dirac n), columns nu m + e
  * (e < dim: position coordinate e; e = dim: weight of dirac m), sorted;
@@ -167,9 +169,9 @@ sdot_status sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions(sdot_pd
  * needed.  SDOT_NOT_SUPPORTED with quadratic pixels. */
 sdot_status sdot_pd_centroids(sdot_pd *pd, double *centroids, double *masses);
 /* per-cell boundary measures: ball part (arcs with cut id i) and whole perimeter
- * (area of the boundary in 3D); either may be NULL.  With an image density the
- * perimeter is the density-weighted measure of the faces shared with other
- * cells only (the domain boundary is not counted). */
+ * (area of the boundary in 3D); either may be NULL.  With an image density or
+ * several shapes both are density-weighted and the perimeter counts the faces
+ * shared with other cells and the ball arcs only (the domain boundary is not counted). */
 sdot_status sdot_pd_boundary_measures(sdot_pd *pd, double *ball_arc, double *perimeter);
 /* get_der_integrals_wrt_weights(offsets, cols, vals, masses, ...) -- get_der_integrals_wrt_weights.h:16.
  * Two-phase: with cols == NULL only *nnz is written.  offsets has N+1 entries;

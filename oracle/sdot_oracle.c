@@ -8,6 +8,7 @@
 #define _GNU_SOURCE
 #include "sdot_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -1223,10 +1224,14 @@ static void eval_cell2(cells_job *J, int64_t i, or_cp2 *cp, or_cp2 *piece) {
             double r = cp->sr > 0 ? cp->sr : 0;
             mn[0] = cp->scx - r; mx[0] = cp->scx + r; mn[1] = cp->scy - r; mx[1] = cp->scy + r;
         } else {
-            mn[0] = mx[0] = cp->x[0]; mn[1] = mx[1] = cp->y[0];
-            for (int k = 1; k < cp->n; ++k) {
-                mn[0] = fmin(mn[0], cp->x[k]); mx[0] = fmax(mx[0], cp->x[k]);
-                mn[1] = fmin(mn[1], cp->y[k]); mx[1] = fmax(mx[1], cp->y[k]);
+            /* min_position / max_position (ConvexPolyhedron2.tcc:502-523): an arc
+             * contributes the box of its whole disc */
+            mn[0] = mn[1] = DBL_MAX; mx[0] = mx[1] = -DBL_MAX;
+            for (int k = 0; k < cp->n; ++k) {
+                double lx = cp->x[k], ly = cp->y[k], hx = lx, hy = ly;
+                if (cp->arc[k]) { lx = cp->scx - cp->sr; ly = cp->scy - cp->sr; hx = cp->scx + cp->sr; hy = cp->scy + cp->sr; }
+                mn[0] = fmin(mn[0], lx); mx[0] = fmax(mx[0], hx);
+                mn[1] = fmin(mn[1], ly); mx[1] = fmax(mx[1], hy);
             }
         }
         int64_t mi[2], ma[2];

@@ -126,7 +126,6 @@ void domain_args(sdot_pd *pd, CellArgs &a) {
     if (pd->shapes.size() == 1 && sh.planes.empty()) return;   // one box: the starting cell
     // items (ConvexPolyhedronAssembly.tcc:7-142): tables on the device
     const int K = (int)pd->shapes.size();
-    if (K > 1 && pd->ball) throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping with several domain shapes"};
     std::vector<double> box(6 * K), coeff(K), planes;
     std::vector<int> off(K + 1, 0);
     double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
@@ -180,7 +179,11 @@ sdot_status compute_pass(sdot_pd *pd) {
     if (pd->N <= 0) throw StatusError{SDOT_INVALID_ARG, "no diracs"};
     if (pd->ball && pd->dim == 3)
         throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping in 3D (ConvexPolyhedron3::ball_cut is TODO in the reference)"};
-    if (pd->ball && pd->has_image) throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping with an image density"};
+    const bool pieces = pd->has_image || pd->shapes.size() > 1;
+    if (pd->ball && pd->has_image && pd->img_nc != 1)
+        throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping with a quadratic image (apply_poly has no ball in the reference)"};
+    if (pd->ball && pieces && pd->want_jac)
+        throw StatusError{SDOT_NOT_SUPPORTED, "position derivatives with ball clipping over several pieces"};
     if (pd->rf && ((pd->has_image && pd->img_nc != 1) || (pd->dim == 3 && (pd->has_image || pd->shapes.size() > 1))))
         throw StatusError{SDOT_NOT_SUPPORTED, "R2 / WmR2 / ExpWmR2db integration: constant pieces in 2D, one domain shape in 3D"};
     if (pd->rf >= 2 && pd->dim == 3)

@@ -394,11 +394,13 @@ struct IdPoly2 {
 
 // one side of a convex polygon clip (Sutherland-Hodgman keeping edge ids): keeps
 // la x + lb y <= lc, the new edge along the clipping line gets `id`; returns false if
-// the result would not fit
+// the result would not fit.  A polygon with no vertex strictly inside meets the kept
+// side in a segment at most and gives an empty piece: an edge lying on the line then
+// belongs to the piece on its other side only (not to both)
 template <int PC>
 SDOT_HD inline bool clip_line(const IdPoly2<PC> &in, IdPoly2<PC> &out, double la, double lb, double lc, int id) {
     int m = 0;
-    bool fits = true;
+    bool fits = true, any_in = false;
     auto put = [&](double x, double y, int e) {
         if (m < PC) { out.x[m] = x; out.y[m] = y; out.e[m] = e; ++m; } else fits = false;
     };
@@ -406,6 +408,7 @@ SDOT_HD inline bool clip_line(const IdPoly2<PC> &in, IdPoly2<PC> &out, double la
         const int k1 = k + 1 == in.n ? 0 : k + 1;
         const double dk = lc - (la * in.x[k] + lb * in.y[k]);
         const double d1 = lc - (la * in.x[k1] + lb * in.y[k1]);
+        any_in |= dk > 0;
         if (dk >= 0) {
             put(in.x[k], in.y[k], in.e[k]);
             if (d1 < 0) {
@@ -417,7 +420,7 @@ SDOT_HD inline bool clip_line(const IdPoly2<PC> &in, IdPoly2<PC> &out, double la
             put(in.x[k1] + u * (in.x[k] - in.x[k1]), in.y[k1] + u * (in.y[k] - in.y[k1]), in.e[k]);
         }
     }
-    out.n = m;
+    out.n = any_in ? m : 0;
     return fits;
 }
 
@@ -749,6 +752,120 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                 mom_y += rho * sy / 6;
             }
         };
+        // ball mode: each piece is also clipped by the disc of radius sqrt(w0) about the dirac
+        // (intersect_with re-applies the cell's ball cut, ConvexPolyhedron2.tcc:2474-2496, with the
+        // LINE / ARC rules of ball_cut :534-665); arcs add rho x length to the ball measure and
+        // 0.5 rho x angle to the diagonal (get_der_integrals_wrt_weights.h:58-60)
+        double arc_sum = 0, der_arc_sum = 0;
+        auto add_ball_piece = [&](double rho) {
+            const IdPoly2<PC> &pc = *src;
+            const double r2 = w0, r = sqrt(fmax(w0, 0.0));
+            double area2 = 0, segs = 0, pmx = 0, pmy = 0;
+            int m = 0, ptype = 0, pid = 0, ftype = 0, fid = 0;
+            double px = 0, py = 0, fx = 0, fy = 0;
+            auto item = [&](int type, int id, double x0, double y0, double x1, double y1) {
+                area2 += x0 * y1 - x1 * y0;
+                if (type == 0) {
+                    if (a.mom) {   // triangle (dirac, p0, p1)
+                        const double cr = x0 * y1 - x1 * y0;
+                        pmx += cr * (x0 + x1) / 6;
+                        pmy += cr * (y0 + y1) / 6;
+                    }
+                    if (id < 0) return;   // domain, pixel and enclosing-square sides carry no row
+                    const double ex = x1 - x0, ey = y1 - y0, len = sqrt(ex * ex + ey * ey);
+                    for (int j = 0; j < n; ++j)
+                        if (cell.e[j] == id) {
+                            edge_rho_len[j] += rho * len;
+                            edge_hit[j] = true;
+                            break;
+                        }
+                    return;
+                }
+                const double a0 = atan2(y0, x0), a1 = atan2(y1, x1);
+                const double da = a1 < a0 ? (a1 + TWO_PI) - a0 : a1 - a0;
+                arc_sum += rho * da * r;
+                der_arc_sum += rho * 0.5 * da;
+                const double ex = x1 - x0, ey = y1 - y0, h0 = sqrt(ex * ex + ey * ey) / 2, h1 = sqrt(fmax(0.0, r2 - h0 * h0));
+                segs += r2 * da / 2 + (da > PI_D ? h1 * h0 : -h1 * h0);
+                if (a.mom) {   // ball sector (_centroid_arc, ConvexPolyhedron2.tcc:2296-2330)
+                    const double mid = a0 + 0.5 * da, k = 2.0 / 3.0 * r * r * r * sin(0.5 * da);
+                    pmx += k * cos(mid);
+                    pmy += k * sin(mid);
+                }
+            };
+            auto push = [&](int type, int id, double x, double y) {
+                if (m == 0) { fx = x; fy = y; ftype = type; fid = id; }
+                else item(ptype, pid, px, py, x, y);
+                px = x; py = y; ptype = type; pid = id;
+                ++m;
+            };
+            bool all_inside = true;
+            for (int k = 0; k < pc.n; ++k) all_inside &= pc.x[k] * pc.x[k] + pc.y[k] * pc.y[k] - r2 <= 0;
+            for (int k0 = 0; k0 < pc.n; ++k0) {
+                const int k1 = k0 + 1 == pc.n ? 0 : k0 + 1, id = pc.e[k0];
+                const double p0x = pc.x[k0], p0y = pc.y[k0], p1x = pc.x[k1], p1y = pc.y[k1];
+                if (all_inside) {
+                    push(0, id, p0x, p0y);
+                    continue;
+                }
+                const double d0 = p0x * p0x + p0y * p0y - r2, d1 = p1x * p1x + p1y * p1y - r2;
+                auto root = [&](double ax, double ay, double ex, double ey, double c) {   // |A + t E|^2 = r2 nearest 1/2
+                    const double aa = ex * ex + ey * ey, b = ax * ex + ay * ey, dd = sqrt(fmax(0.0, b * b - aa * c));
+                    const double u = (-b + dd) / aa, v = (-b - dd) / aa;
+                    return fmin(1.0, fmax(0.0, fabs(u - 0.5) <= fabs(v - 0.5) ? u : v));
+                };
+                if (d0 <= 0) {
+                    push(0, id, p0x, p0y);
+                    if (d1 > 0) {
+                        const double tt = root(p0x, p0y, p1x - p0x, p1y - p0y, d0);
+                        push(1, 0, p0x + tt * (p1x - p0x), p0y + tt * (p1y - p0y));
+                    }
+                } else if (d1 <= 0) {
+                    const double tt = root(p1x, p1y, p0x - p1x, p0y - p1y, d1);
+                    push(0, id, p1x + tt * (p0x - p1x), p1y + tt * (p0y - p1y));
+                } else {   // both outside: 0 or 2 crossings
+                    const double ex = p1x - p0x, ey = p1y - p0y, aa = ex * ex + ey * ey;
+                    if (aa == 0) continue;
+                    const double b = p0x * ex + p0y * ey, s = b * b - aa * d0;
+                    if (s <= 0) continue;
+                    const double dd = sqrt(s);
+                    double u = (-b - dd) / aa, v = (-b + dd) / aa;
+                    if (u > 0 && u < 1) v = fmax(0.0, fmin(1.0, v));
+                    else if (v > 0 && v < 1) u = fmax(0.0, fmin(1.0, u));
+                    else continue;
+                    const double q0x = p0x + u * ex, q0y = p0y + u * ey, q1x = p0x + v * ex, q1y = p0y + v * ey;
+                    if (atan2(q0y, q0x) == atan2(q1y, q1x)) continue;
+                    push(0, id, q0x, q0y);
+                    push(1, 0, q1x, q1y);
+                }
+            }
+            double area = 0;
+            if (m > 0) {
+                item(ptype, pid, px, py, fx, fy);
+                area = 0.5 * area2 + segs;
+                (void)ftype;
+                (void)fid;
+            } else if (pc.n > 0) {   // no item: the disc lies inside the piece, or outside it
+                bool outside = false;
+                for (int k0 = 0; k0 < pc.n; ++k0) {
+                    const int k1 = k0 + 1 == pc.n ? 0 : k0 + 1;
+                    const double ex = pc.x[k1] - pc.x[k0], ey = pc.y[k1] - pc.y[k0];
+                    if (ex * pc.y[k0] - ey * pc.x[k0] > 0) outside = true;   // the dirac lies right of the edge
+                }
+                if (!outside && r > 0) {   // the whole circle (moment 0 about the dirac)
+                    area = PI_D * r2;
+                    arc_sum += rho * TWO_PI * r;
+                    der_arc_sum += rho * PI_D;
+                }
+            }
+            mass_sum += rho * area;
+            mom_x += rho * pmx;
+            mom_y += rho * pmy;
+        };
+        auto add = [&](double rho, const double *q) {
+            if (BALL) add_ball_piece(rho);
+            else add_piece(rho, q);
+        };
         if (a.img) {
             const double c0[2] = {c0x, c0y};
             int i0[2] = {0, 0}, i1[2] = {-1, -1};
@@ -786,7 +903,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                     if (px < i1[0]) clip(1, 0, a.dom_lo[0] + ps[0] * (px + 1) - c0x);
                     if (py > i0[1]) clip(0, -1, -(a.dom_lo[1] + ps[1] * py - c0y));
                     if (py < i1[1]) clip(0, 1, a.dom_lo[1] + ps[1] * (py + 1) - c0y);
-                    add_piece(rho, poly ? q : nullptr);
+                    add(rho, poly ? q : nullptr);
                 }
         } else {
             for (int k = 0; k < a.nitems && n > 0; ++k) {
@@ -803,7 +920,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                     const double *pl = a.it_planes + 4 * q;
                     clip(pl[0], pl[1], pl[3] - pl[0] * c0x - pl[1] * c0y);
                 }
-                add_piece(a.it_coeff[k], nullptr);
+                add(a.it_coeff[k], nullptr);
             }
         }
         if (!fits) flag_or(a.flags, FLAG_CELL_OVERFLOW);
@@ -839,10 +956,10 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         }
         if (JAC)
             for (int k = 0; k < 9; ++k) a.jdiag[(int64_t)k * a.ell_stride + t] = jdp[k];
-        a.arc[t] = 0;
-        a.perim[t] = perim;
+        a.arc[t] = arc_sum;
+        a.perim[t] = perim + arc_sum;   // pieces: the faces shared with neighbours and the ball arcs
         // integration_der_wrt_weight: WmR2 the measure, ExpWmR2db the integral / eps (:124-138)
-        a.diag[t] = der_line + (RF == 2 ? rf_area : RF == 3 ? mass_sum / a.rf_eps : 0.0);
+        a.diag[t] = der_line + der_arc_sum + (RF == 2 ? rf_area : RF == 3 ? mass_sum / a.rf_eps : 0.0);
         a.row_len[t] = nrow;
         if (a.ctr) {
             const unsigned long long K = n;

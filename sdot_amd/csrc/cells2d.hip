@@ -241,19 +241,21 @@ void launch2(const CellArgs &a, unsigned nb, hipStream_t s) {
     HIPCK(hipGetLastError());
 }
 
-// ball and image density exclude each other (rejected before the pass); batches
-// carry no image and no translations
+// batches carry no image and no translations; ball clipping over pieces (pixels or
+// several domain items) clips each piece by the disc
 template <class T, int CAP, int WAVES>
 void dispatch2(bool ball, bool img, const CellArgs &a, unsigned nb, hipStream_t s) {
     if (a.prob) {
         if (ball) launch2<T, CAP, 64, true, false, WAVES, true>(a, nb, s);
         else launch2<T, CAP, 64, false, false, WAVES, true>(a, nb, s);
     } else if (a.ntrans > 0) {
-        if (ball) launch2<T, CAP, 64, true, false, WAVES, false, true>(a, nb, s);
+        if (ball && img) launch2<T, CAP, 64, true, true, WAVES, false, true>(a, nb, s);
+        else if (ball) launch2<T, CAP, 64, true, false, WAVES, false, true>(a, nb, s);
         else if (img) launch2<T, CAP, 64, false, true, WAVES, false, true>(a, nb, s);
         else launch2<T, CAP, 64, false, false, WAVES, false, true>(a, nb, s);
     } else {
-        if (ball) launch2<T, CAP, 64, true, false, WAVES, false>(a, nb, s);
+        if (ball && img) launch2<T, CAP, 64, true, true, WAVES, false>(a, nb, s);
+        else if (ball) launch2<T, CAP, 64, true, false, WAVES, false>(a, nb, s);
         else if (img) launch2<T, CAP, 64, false, true, WAVES, false>(a, nb, s);
         else launch2<T, CAP, 64, false, false, WAVES, false>(a, nb, s);
     }

@@ -162,13 +162,68 @@ def test_cells2d_image_strip_closed_form(precision):
         assert np.allclose(g["perimeter"], [b, b], atol=1e-13)
 
 
-def test_cells2d_image_with_ball_is_rejected():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("shape", [(5, 7), (64, 48), (300, 300)])
+def test_cells2d_parity_image_with_ball(precision, shape):
+    """ball clipping over pixels: each pixel piece clipped by the cell's disc
+    (intersect_with re-applies ball_cut, ConvexPolyhedron2.tcc:2474-2496)"""
+    n = 20000
+    pos, w = inputs(2, n, 12, wscale=0.3 / n, wshift=0.4 / n)   # most cells touch their disc
+    img = lognormal_image_2d(*shape)
+    g = gpu_pass(pos, w, 2, precision, ball=True, img=img)
+    o = oracle_pass(pos, w, 2, ball=True, img=img)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
+    assert (g["masses"] <= math.pi * w * img.max() * (1 + 1e-12)).all()
+    assert (g["ball_arc"] > 0).mean() > 0.5
+    # size-independent: the row sums are the arc terms 0.5 * arc / sqrt(w)
+    rs = np.asarray(csr_matrix(g["csr"], n).sum(axis=1)).ravel()
+    assert np.abs(rs - 0.5 * g["ball_arc"] / np.sqrt(w)).max() < 1e-9 * np.abs(rs).max()
+    print(f"[parity2d img+ball {shape}] {precision}: {err}")
+
+
+def _disc_left_of(r, d):
+    th = math.acos(-d / r)
+    return r * r * (th - math.sin(th) * math.cos(th)), 2 * r * th
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cells2d_ball_over_pixels_closed_form(precision):
     import sdot_amd
-    pd = sdot_amd.PowerDiagram(2, ball=True)
-    pd.set_image_density([0, 0], [1, 1], np.ones((4, 4)))
+    # one disc of radius 0.3 over two pixel columns a | b split at x = 0.5
+    a, b, r = 1.5, 0.25, 0.3
+    img = np.array([[a, b]] * 3)
+    for delta in (0.0, 0.07, -0.13):
+        g = gpu_pass(np.array([[0.5 + delta, 0.5]]), np.array([r * r]), 2, precision, ball=True, img=img)
+        al, ll = _disc_left_of(r, -delta)
+        area, arc = math.pi * r * r, 2 * math.pi * r
+        assert abs(g["masses"][0] - (a * al + b * (area - al))) < 1e-14
+        assert abs(g["ball_arc"][0] - (a * ll + b * (arc - ll))) < 1e-14
+        assert abs(g["perimeter"][0] - (a * ll + b * (arc - ll))) < 1e-14
+        assert abs(g["csr"][2][0] - 0.5 * (a * ll + b * (arc - ll)) / r) < 1e-13
+    # two discs cut at x = 0.5: the arcs reach pixels far from the cell's vertices
+    # (r = 0.29: no pixel line or domain side is tangent to a disc)
+    c, r = 2.0, 0.29
+    g = gpu_pass(np.array([[0.3, 0.5], [0.7, 0.5]]), np.full(2, r * r), 2, precision, ball=True,
+                 img=np.full((20, 20), c))
+    al, ll = _disc_left_of(r, 0.2)
+    chord = 2 * math.sqrt(r * r - 0.04)
+    assert np.abs(g["masses"] - c * al).max() < 1e-14
+    assert np.abs(g["ball_arc"] - c * ll).max() < 1e-14
+    assert np.abs(g["perimeter"] - c * (ll + chord)).max() < 1e-14
+    M = csr_matrix(g["csr"], 2).toarray()
+    assert abs(M[0, 1] + 0.5 * c * chord / 0.4) < 1e-13
+    # without the ball: the shared edge x = 0.5 lies on a pixel line and is counted once
+    g = gpu_pass(np.array([[0.3, 0.5], [0.7, 0.5]]), np.zeros(2), 2, precision, img=np.full((20, 20), c))
+    assert np.abs(g["masses"] - 0.5 * c).max() < 1e-14
+    assert np.abs(g["perimeter"] - c).max() < 1e-14
+    M = csr_matrix(g["csr"], 2).toarray()
+    assert np.abs(M - c * 1.25 * np.array([[1, -1], [-1, 1]])).max() < 1e-13
+    # a quadratic image has no ball clipping in the reference (apply_poly)
+    pd = sdot_amd.PowerDiagram(2, precision, ball=True)
+    pd.set_image_density([0, 0], [1, 1], np.ones((6, 2, 2)))
     pd.set_diracs(np.array([[0.3, 0.4], [0.6, 0.7]]), np.full(2, 0.01))
     with pytest.raises(sdot_amd.SdotError):
-        pd.compute()
+        pd.integrals()
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

@@ -30,15 +30,15 @@ def check(c, m, oc, om, n, tol):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("case", ["box", "ball", "image"])
+@pytest.mark.parametrize("case", ["box", "ball", "image", "ball_image"])
 def test_centroids_2d(precision, case):
     n = 20000
-    ball = case == "ball"
+    ball = case.startswith("ball")
     if ball:
         pos, w = inputs(2, n, 51, wscale=1.0 / n, wshift=1.0 / n)
     else:
         pos, w = inputs(2, n, 52, wscale=0.3 / n)
-    img = np.exp(0.5 * np.random.default_rng(7).normal(size=(37, 29))) if case == "image" else None
+    img = np.exp(0.5 * np.random.default_rng(7).normal(size=(37, 29))) if case.endswith("image") else None
     c, m, _ = gpu_centroids(pos, w, 2, precision, ball=ball, img=img)
     st, oc, om = O.centroids(pos, w, ball=ball, img=img)
     check(c, m, oc, om, n, 1e-9)

@@ -177,12 +177,36 @@ def test_shapes_known_values():
         assert np.abs(c[0] - q.mean(axis=0)).max() < 1e-13
 
 
-def test_shapes_ball_with_several_items_rejected():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", ["two_boxes", "box_and_triangle"])
+def test_shapes_2d_several_items_with_ball(case, precision):
+    """ball clipping over several items: each item's piece clipped by the disc
+    (ConvexPolyhedronAssembly::for_each_intersection, :132-142, then intersect_with)"""
+    n = 6000
+    pos, w = inputs(2, n, 65, wscale=0.3 / n, wshift=0.4 / n)   # most cells touch their disc
+    g, o, _ = run_pair(CASES_2D[case], pos, w, 2, precision, ball=True)
+    err = compare(g, o, n, 1.0 / n, hess_rtol=hess_rtol(precision))
+    assert (g["ball_arc"] > 0).mean() > 0.5
+    print(f"[shapes 2D {case}+ball] {precision}: {err}")
+
+
+def test_shapes_ball_with_several_items_closed_form_and_limits():
     import sdot_amd
-    pd = make_pd(CASES_2D["two_boxes"], 2, "fp64", ball=True)
-    pd.set_diracs(*inputs(2, 100, 65, wscale=0.01, wshift=0.01))
-    with pytest.raises(sdot_amd.SdotError):
-        pd.integrals()
+    # two items a | b split at x = 0.5 under one disc of radius 0.3
+    a, b, r = 1.5, 0.25, 0.3
+    shapes = [("box", (np.array([0.0, 0.0]), np.array([0.5, 1.0])), a),
+              ("box", (np.array([0.5, 0.0]), np.array([1.0, 1.0])), b)]
+    for prec in ("fp64", "mixed"):
+        pd = make_pd(shapes, 2, prec, ball=True)
+        pd.set_diracs(np.array([[0.57, 0.5]]), np.array([r * r]))
+        m = pd.integrals()
+        th = math.acos(0.07 / r)
+        al = r * r * (th - math.sin(th) * math.cos(th))
+        assert abs(m[0] - (a * al + b * (math.pi * r * r - al))) < 1e-14
+        # position derivatives with the ball over several pieces are not provided
+        pd.set_diracs(*inputs(2, 100, 66, wscale=0.01, wshift=0.01))
+        with pytest.raises(sdot_amd.SdotError):
+            pd.der_centroids_and_integrals_wrt_weight_and_positions()
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

@@ -291,3 +291,100 @@ def test_assembly_tetrahedron_3d():
     pos = pos / pos.sum(axis=1, keepdims=True).clip(1, None) * 0.9   # inside the tetrahedron
     r = O.cells(pos, np.zeros(n), items=[tet])
     assert abs(r["masses"].sum() - 1.5 / 6) < 1e-13
+
+
+def _disc_left_of(r, d):
+    """area and arc length of a disc of radius r left of a line at signed distance d
+    right of its centre (|d| <= r): the circular segment formulas"""
+    th = math.acos(-d / r)          # half angle of the part left of the line
+    return r * r * (th - math.sin(th) * math.cos(th)), 2 * r * th
+
+
+@pytest.mark.parametrize("delta", [0.0, 0.07, -0.13])
+def test_ball_over_two_pixel_columns_closed_form(delta):
+    # one dirac at (0.5 + delta, 0.5), disc radius 0.3, densities a | b left and
+    # right of x = 0.5: every piece is a circular segment
+    a, b, r = 1.5, 0.25, 0.3
+    img = np.array([[a, b]] * 3)                       # [y][x]: 3 rows, 2 columns
+    res = O.cells(np.array([[0.5 + delta, 0.5]]), np.array([r * r]), ball=True, img=img)
+    al, ll = _disc_left_of(r, -delta)                   # the line x = 0.5 is at -delta from the centre
+    area, arc = math.pi * r * r, 2 * math.pi * r
+    assert abs(res["masses"][0] - (a * al + b * (area - al))) < 1e-14
+    assert abs(res["ball_arc"][0] - (a * ll + b * (arc - ll))) < 1e-14
+    assert abs(res["perimeter"][0] - (a * ll + b * (arc - ll))) < 1e-14   # pieces: arcs only
+    # d mass / d w = 0.5 * sum rho * arc / r
+    assert abs(csr(*O.to_csr(res), 1).toarray()[0, 0] - 0.5 * (a * ll + b * (arc - ll)) / r) < 1e-13
+    # the same split from two domain items of densities a and b
+    items = [O.box_item([0, 0], [0.5, 1], a), O.box_item([0.5, 0], [1, 1], b)]
+    ri = O.cells(np.array([[0.5 + delta, 0.5]]), np.array([r * r]), ball=True, items=items)
+    for k in ("masses", "ball_arc", "perimeter"):
+        assert abs(ri[k][0] - res[k][0]) < 1e-14, k
+
+
+def test_ball_over_pixels_follows_the_arcs():
+    # two diracs at x = 0.3, 0.7 with discs of radius 0.3: each cell is the disc
+    # cut at x = 0.5, whose vertices span no width; the pixels under the arcs
+    # (min_position / max_position widen to the disc's box) carry the mass
+    r, c = 0.29, 2.0                                    # no pixel line tangent to a disc
+    img = np.full((20, 20), c)
+    res = O.cells(np.array([[0.3, 0.5], [0.7, 0.5]]), np.full(2, r * r), ball=True, img=img)
+    al, ll = _disc_left_of(r, 0.2)
+    assert np.abs(res["masses"] - c * al).max() < 1e-14
+    assert np.abs(res["ball_arc"] - c * ll).max() < 1e-14
+    chord = 2 * math.sqrt(r * r - 0.2 * 0.2)
+    assert np.abs(res["perimeter"] - c * (ll + chord)).max() < 1e-14
+    M = csr(*O.to_csr(res), 2).toarray()
+    assert abs(M[0, 1] + 0.5 * c * chord / 0.4) < 1e-14
+    # without the ball the shared edge x = 0.5 lies on a pixel line: counted once
+    res = O.cells(np.array([[0.3, 0.5], [0.7, 0.5]]), np.zeros(2), img=img)
+    assert np.abs(res["perimeter"] - c).max() < 1e-14
+    assert np.abs(csr(*O.to_csr(res), 2).toarray() - c * 1.25 * np.array([[1, -1], [-1, 1]])).max() < 1e-13
+
+
+def test_ball_over_pixels_centroids():
+    # a full disc over a 2x2 image centred on the shared corner: the centroid
+    # moves towards the dense quadrant by the quadrant moments 4 r / (3 pi)
+    r = 0.3
+    img = np.array([[1.0, 2.0], [3.0, 4.0]])            # [y][x]
+    st, c, m = O.centroids(np.array([[0.5, 0.5]]), np.array([r * r]), ball=True, img=img)
+    q = math.pi * r * r / 4
+    assert abs(m[0] - 10 * q) < 1e-14
+    k = 4 * r / (3 * math.pi)                             # quadrant centroid offset
+    ex = (-1 * k + 2 * k - 3 * k + 4 * k) * q / (10 * q)
+    ey = (-1 * k - 2 * k + 3 * k + 4 * k) * q / (10 * q)
+    assert abs(c[0, 0] - (0.5 + ex)) < 1e-14 and abs(c[0, 1] - (0.5 + ey)) < 1e-14
+
+
+def test_ball_over_constant_pixels_is_rho_times_one_box():
+    # pixels that all carry rho: the pieces add up to rho x the one-box cell
+    n, rho = 300, 1.7
+    pos, w = inputs(2, n, 13, wscale=1.0 / n, wshift=1.0 / n)
+    one = O.cells(pos, w, ball=True)
+    for shape in ((1, 1), (7, 5), (40, 40)):
+        r = O.cells(pos, w, ball=True, img=np.full(shape, rho))
+        assert np.abs(r["masses"] - rho * one["masses"]).max() < 1e-14
+        assert np.abs(r["ball_arc"] - rho * one["ball_arc"]).max() < 1e-13
+        d = csr(*O.to_csr(r), n) - rho * csr(*O.to_csr(one), n)
+        assert abs(d).max() < 1e-11 * rho * abs(csr(*O.to_csr(one), n)).max()
+    # items of density rho tiling the box give the same pieces
+    items = [O.box_item([0, 0], [0.35, 1], rho), O.box_item([0.35, 0], [1, 0.6], rho),
+             O.box_item([0.35, 0.6], [1, 1], rho)]
+    r = O.cells(pos, w, ball=True, items=items)
+    assert np.abs(r["masses"] - rho * one["masses"]).max() < 1e-14
+
+
+@pytest.mark.parametrize("pieces", ["img", "items"])
+def test_ball_over_pieces_finite_differences(pieces):
+    n = 150
+    pos, w = inputs(2, n, 14, wscale=0.3 / n, wshift=0.6 / n)
+    kw = dict(img=np.random.default_rng(2).uniform(0.5, 2.0, size=(9, 11))) if pieces == "img" else \
+        dict(items=[O.box_item([0, 0], [0.6, 0.7], 1.0), O.box_item([0.3, 0.4], [1, 1], 3.0)])
+    r = O.cells(pos, w, ball=True, **kw)
+    M = csr(*O.to_csr(r), n).toarray()
+    eps = 1e-7 / n
+    for j in (3, 57, 101):
+        wp, wm = w.copy(), w.copy()
+        wp[j] += eps
+        wm[j] -= eps
+        fd = (O.cells(pos, wp, ball=True, **kw)["masses"] - O.cells(pos, wm, ball=True, **kw)["masses"]) / (2 * eps)
+        assert np.abs(fd - M[:, j]).max() < 1e-5 * np.abs(M[:, j]).max()
