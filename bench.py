@@ -89,15 +89,31 @@ def run_gpu(pos, w, precision, steps, warmup):
 
 
 def copy_calibration(mib, reps=3):
-    """Plain device copies of a known size (reads and writes mib MiB each) for
-    scaling FETCH_SIZE / WRITE_SIZE counters; torch provides the buffers."""
-    import torch
-    n = mib * (1 << 20) // 8
-    a = torch.empty(n, dtype=torch.float64, device="cuda").fill_(1.0)
-    b = torch.empty_like(a)
-    for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
+    """Plain device-to-device copies of a known size (reads and writes mib MiB each)
+    for scaling FETCH_SIZE / WRITE_SIZE counters: hipMemcpy of the HIP runtime the
+    library links (one runtime in the process; the copy runs as a blit kernel)."""
+    import ctypes
+    import sdot_amd  # noqa: F401  (loads the library and its runtime first)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    nbytes = mib * (1 << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for buf in (a, b):
+        if hip.hipMalloc(ctypes.byref(buf), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed in the copy calibration")
+    try:
+        if hip.hipMemset(a, 1, nbytes) != 0:
+            raise RuntimeError("hipMemset failed in the copy calibration")
+        for _ in range(reps):
+            if hip.hipMemcpy(b, a, nbytes, 3) != 0:   # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed in the copy calibration")
+        hip.hipDeviceSynchronize()
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
     print(f"# copy calibration: {reps} copies of {mib} MiB", file=sys.stderr, flush=True)
 
 
@@ -254,7 +270,7 @@ def main_c4(args):
     K = args.problems
     sizes = workloads.batch_sizes(K, 1e3, 1e5, seed=0)
     pos = [workloads.uniform_diracs(int(n), 2, seed=k + 1) for k, n in enumerate(sizes)]
-    b = sdot_amd.Batch("mixed", diracs_per_cell=0.0, use_graph=True)
+    b = sdot_amd.Batch("mixed", diracs_per_cell=0.0, use_graph=not args.no_graph)
     b.set_problems(pos)
     for _ in range(args.warmup):   # the first solve also runs the autotuner
         r = b.solve(tol=1e-6)
@@ -419,6 +435,7 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="c2 only: skip the C1 / C3 solve legs")
+    ap.add_argument("--no-graph", action="store_true", help="c4 only: launch the PCG kernels one by one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run device-to-device copies of MIB MiB (known HBM bytes) so hardware "
                          "counters of the cell kernel can be scaled by the copy's counters")

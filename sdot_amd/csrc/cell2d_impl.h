@@ -23,6 +23,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 #include "internal.h"
 
@@ -39,6 +40,16 @@ template <>
 struct Eps<float> {
     static constexpr double region = 2e-5, cand = 2e-5;
 };
+
+// vertex bitmasks of a polygon of at most CAP vertices, and their bit counts
+template <int CAP>
+struct Mask {
+    using type = typename std::conditional<(CAP <= 32), uint32_t, uint64_t>::type;
+};
+SDOT_HD inline int popc(uint32_t x) { return __builtin_popcount(x); }
+SDOT_HD inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+SDOT_HD inline int hibit(uint32_t x) { return 31 - __builtin_clz(x); }
+SDOT_HD inline int hibit(uint64_t x) { return 63 - __builtin_clzll(x); }
 
 // Can a region whose points are all at distance >= d = sqrt(D2) from c0 and whose
 // diracs have weight <= wmax hold a dirac that cuts a polygon contained in the disc
@@ -431,10 +442,10 @@ SDOT_HD inline bool clip_line(const IdPoly2<PC> &in, IdPoly2<PC> &out, double la
 // RF: the final integration function (Integrator).
 template <class T, int CAP, bool BALL, bool IMG = false, bool BATCH = true, bool PER = false, bool JAC = false,
           int RF = 0>
-SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, int *I, const int S) {
+SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, int *I, const int S, const int SI) {
 #define PX(k) X[(k) * S]
 #define PY(k) Y[(k) * S]
-#define PI_(k) I[(k) * S]
+#define PI_(k) I[(k) * SI]
 
     const double c0x = a.spos[2 * t], c0y = a.spos[2 * t + 1], w0 = a.sw[t];
     ProbView pv;
@@ -494,39 +505,41 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         // bitmask of outside vertices (no per-vertex branches, which cost the threads of a
         // wavefront most of the cut); the runs are read from the mask afterwards.
         auto dist = [&](int k) { return fma(-PX(k), Vx, fma(-PY(k), Vy, H)); };
-        uint64_t mask = 0;
+        using M = typename Mask<CAP>::type;   // one bit per vertex (32-bit words up to 32 vertices)
+        constexpr int WB = 8 * sizeof(M);
+        M mask = 0;
         if (CAP <= 8) {
 #pragma unroll
-            for (int k = 0; k < CAP; ++k) mask |= (k < n && dist(k) < 0) ? (1ull << k) : 0ull;
+            for (int k = 0; k < CAP; ++k) mask |= (k < n && dist(k) < 0) ? (M(1) << k) : M(0);
         } else {
-            for (int k = 0; k < n; ++k) mask |= dist(k) < 0 ? (1ull << k) : 0ull;
+            for (int k = 0; k < n; ++k) mask |= dist(k) < 0 ? (M(1) << k) : M(0);
         }
         if (!mask) return 0;
-        const uint64_t full = n >= 64 ? ~0ull : (1ull << n) - 1;
+        const M full = n >= WB ? ~M(0) : (M(1) << n) - 1;
         // starts: outside vertices whose predecessor is inside; ends: whose successor is inside
-        auto runs = [&](uint64_t m, uint64_t &starts, uint64_t &ends) {
-            const uint64_t prev = ((m << 1) | (m >> (n - 1))) & full, next = ((m >> 1) | (m << (n - 1))) & full;
+        auto runs = [&](M m, M &starts, M &ends) {
+            const M prev = ((m << 1) | (m >> (n - 1))) & full, next = ((m >> 1) | (m << (n - 1))) & full;
             starts = m & ~prev;
             ends = m & ~next;
         };
-        uint64_t starts, ends;
+        M starts, ends;
         runs(mask, starts, ends);
         bool exact = false;
-        if (__builtin_popcountll(starts) > 1) {
+        if (popc(starts) > 1) {
             // a convex polygon crosses a line once: rounding noise, redo in FP64 (off the fast path)
             exact = true;
             mask = 0;
-            for (int k = 0; k < n; ++k) mask |= T(side64(k, vx64, vy64, h64)) < 0 ? (1ull << k) : 0ull;
+            for (int k = 0; k < n; ++k) mask |= T(side64(k, vx64, vy64, h64)) < 0 ? (M(1) << k) : M(0);
             if (!mask) return 0;
             runs(mask, starts, ends);
         }
-        const int nout = __builtin_popcountll(mask);
+        const int nout = popc(mask);
         if (nout == n) { n = 0; return 1; }
         // the entering vertex i1 and the leaving vertex i2 (with several runs, as a scan
         // keeping the last transitions would choose: index 0 / n - 1 first)
-        const int i1 = (starts & 1) ? 0 : 63 - __builtin_clzll(starts);
-        const uint64_t ends_lo = ends & (full >> 1);
-        const int i2 = ((ends >> (n - 1)) & 1) ? n - 1 : 63 - __builtin_clzll(ends_lo);
+        const int i1 = (starts & 1) ? 0 : hibit(starts);
+        const M ends_lo = ends & (full >> 1);
+        const int i2 = ((ends >> (n - 1)) & 1) ? n - 1 : hibit(ends_lo);
         const int nn = n - nout + 2;
         if (nn > CAP) return 2;
         const int i0 = i1 == 0 ? n - 1 : i1 - 1;
@@ -982,87 +995,92 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         }
         it.finish();
     } else if (n > 0) {
+        // one item stream with a single push site: edge k0 gives its start (inside the
+        // disc) and, where it leaves the disc, the exit point that starts an arc
+        // (ball_cut's LINE / ARC rules, ConvexPolyhedron2.tcc:534-665)
         const double r2 = it.r2;
-        bool all_inside = true;
         double x0v, y0v;
         vtx(0, x0v, y0v);
-        {
-            double px = x0v, py = y0v;
-            for (int k = 0; k < n; ++k) {
-                double x = px, y = py;
-                if (k + 1 < n) vtx(k + 1, px, py);
-                all_inside &= x * x + y * y - r2 <= 0;
-            }
-        }
-        if (all_inside) {
-            for (int k = 0; k < n; ++k) {
-                double x, y;
-                vtx(k, x, y);
-                it.push(0, PI_(k), x, y);
-            }
-        } else {
-            double p1x = x0v, p1y = y0v;
-            for (int k0 = 0; k0 < n; ++k0) {
+        double p1x = x0v, p1y = y0v;
+        int k0 = 0;
+        bool pend = false;
+        double qx1 = 0, qy1 = 0;
+        while (true) {
+            int ty = 0, idd = 0;
+            double x, y;
+            if (pend) {
+                ty = 1;
+                x = qx1;
+                y = qy1;
+                pend = false;
+            } else {
+                if (k0 >= n) break;
                 const int k1 = k0 + 1 == n ? 0 : k0 + 1;
                 const double p0x = p1x, p0y = p1y;
                 if (k1 == 0) { p1x = x0v; p1y = y0v; }
                 else vtx(k1, p1x, p1y);
                 const double d0 = p0x * p0x + p0y * p0y - r2, d1 = p1x * p1x + p1y * p1y - r2;
-                const int id = PI_(k0);
-                if (d0 <= 0) {
-                    if (d1 <= 0) {
-                        it.push(0, id, p0x, p0y);
+                idd = PI_(k0);
+                ++k0;
+                if (d0 <= 0 && d1 <= 0) {
+                    x = p0x;
+                    y = p0y;
+                } else if (d0 <= 0 || d1 <= 0) {
+                    // find_unique_intersection: the root nearest t = 1/2 from the inside end
+                    const bool in0 = d0 <= 0;
+                    const double ax = in0 ? p0x : p1x, ay = in0 ? p0y : p1y;
+                    const double ex = in0 ? p1x - p0x : p0x - p1x, ey = in0 ? p1y - p0y : p0y - p1y;
+                    const double aa = ex * ex + ey * ey, b = ax * ex + ay * ey, c = in0 ? d0 : d1;
+                    const double dd = sqrt(fmax(0.0, b * b - aa * c));
+                    const double u = (-b + dd) / aa, v = (-b - dd) / aa;
+                    double tt = fabs(u - 0.5) <= fabs(v - 0.5) ? u : v;
+                    tt = fmin(1.0, fmax(0.0, tt));
+                    const double qx = ax + tt * ex, qy = ay + tt * ey;
+                    if (in0) {
+                        x = p0x;
+                        y = p0y;
+                        qx1 = qx;
+                        qy1 = qy;
+                        pend = true;
                     } else {
-                        double ex = p1x - p0x, ey = p1y - p0y;
-                        double aa = ex * ex + ey * ey, b = p0x * ex + p0y * ey, c = d0;
-                        double dd = sqrt(fmax(0.0, b * b - aa * c));
-                        double u = (-b + dd) / aa, v = (-b - dd) / aa;
-                        double tt = fabs(u - 0.5) <= fabs(v - 0.5) ? u : v;
-                        tt = fmin(1.0, fmax(0.0, tt));
-                        it.push(0, id, p0x, p0y);
-                        it.push(1, 0, p0x + tt * ex, p0y + tt * ey);
+                        x = qx;
+                        y = qy;
                     }
                 } else {
-                    if (d1 <= 0) {
-                        double ex = p0x - p1x, ey = p0y - p1y;
-                        double aa = ex * ex + ey * ey, b = p1x * ex + p1y * ey, c = d1;
-                        double dd = sqrt(fmax(0.0, b * b - aa * c));
-                        double u = (-b + dd) / aa, v = (-b - dd) / aa;
-                        double tt = fabs(u - 0.5) <= fabs(v - 0.5) ? u : v;
-                        tt = fmin(1.0, fmax(0.0, tt));
-                        it.push(0, id, p1x + tt * ex, p1y + tt * ey);
-                    } else {
-                        double ex = p1x - p0x, ey = p1y - p0y;
-                        double aa = ex * ex + ey * ey;
-                        if (aa == 0) continue;
-                        double b = p0x * ex + p0y * ey, c = d0;
-                        double s = b * b - aa * c;
-                        if (s <= 0) continue;
-                        double dd = sqrt(s);
-                        double u = (-b - dd) / aa, v = (-b + dd) / aa;
-                        if (u > 0 && u < 1) v = fmax(0.0, fmin(1.0, v));
-                        else if (v > 0 && v < 1) u = fmax(0.0, fmin(1.0, u));
-                        else continue;
-                        double q0x = p0x + u * ex, q0y = p0y + u * ey, q1x = p0x + v * ex, q1y = p0y + v * ey;
-                        if (atan2(q0y, q0x) == atan2(q1y, q1x)) continue;
-                        it.push(0, id, q0x, q0y);
-                        it.push(1, 0, q1x, q1y);
-                    }
+                    // find_two_cuts: both ends outside, the chord through the disc if any
+                    const double ex = p1x - p0x, ey = p1y - p0y;
+                    const double aa = ex * ex + ey * ey;
+                    if (aa == 0) continue;
+                    const double b = p0x * ex + p0y * ey, s2 = b * b - aa * d0;
+                    if (s2 <= 0) continue;
+                    const double dd = sqrt(s2);
+                    double u = (-b - dd) / aa, v = (-b + dd) / aa;
+                    if (u > 0 && u < 1) v = fmax(0.0, fmin(1.0, v));
+                    else if (v > 0 && v < 1) u = fmax(0.0, fmin(1.0, u));
+                    else continue;
+                    const double q0x = p0x + u * ex, q0y = p0y + u * ey, q1x = p0x + v * ex, q1y = p0y + v * ey;
+                    if (atan2(q0y, q0x) == atan2(q1y, q1x)) continue;
+                    x = q0x;
+                    y = q0y;
+                    qx1 = q1x;
+                    qy1 = q1y;
+                    pend = true;
                 }
             }
-            if (it.m == 0) {
-                // no item left: the disc is either inside the polygon or disjoint from it
-                bool outside = false;
-                double qx = x0v, qy = y0v;
-                for (int k0 = 0; k0 < n; ++k0) {
-                    double px = qx, py = qy;
-                    if (k0 + 1 < n) vtx(k0 + 1, qx, qy);
-                    else { qx = x0v; qy = y0v; }
-                    double ex = qx - px, ey = qy - py;
-                    if (ex * py - ey * px > 0) outside = true;   // the dirac lies right of the edge
-                }
-                full_disc = !outside;
+            it.push(ty, idd, x, y);
+        }
+        if (it.m == 0) {
+            // no item left: the disc is either inside the polygon or disjoint from it
+            bool outside = false;
+            double qx = x0v, qy = y0v;
+            for (int j0 = 0; j0 < n; ++j0) {
+                double px = qx, py = qy;
+                if (j0 + 1 < n) vtx(j0 + 1, qx, qy);
+                else { qx = x0v; qy = y0v; }
+                double ex = qx - px, ey = qy - py;
+                if (ex * py - ey * px > 0) outside = true;   // the dirac lies right of the edge
             }
+            full_disc = !outside;
         }
         it.finish();
     }

@@ -39,8 +39,7 @@ using namespace c2;
 // WAVES: the wavefronts per SIMD the register allocation must allow.
 template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER, bool JAC = false, int RF = 0>
 __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
-    __shared__ T sx[CAP * BS];
-    __shared__ T sy[CAP * BS];
+    __shared__ T sxy[2 * CAP * BS];   // x and y of a vertex side by side (one paired LDS access)
     __shared__ int sid[CAP * BS];
     // one call site of the cell body (so it is inlined): without a work list the
     // loop runs once
@@ -56,8 +55,8 @@ __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
     }
     for (; k < count; k += step) {
         const int64_t t = a.work ? a.work[k] : k;
-        if (cell2_body<T, CAP, BALL, IMG, BATCH, PER, JAC, RF>(a, t, sx + threadIdx.x, sy + threadIdx.x,
-                                                              sid + threadIdx.x, BS)) {
+        if (cell2_body<T, CAP, BALL, IMG, BATCH, PER, JAC, RF>(a, t, sxy + 2 * threadIdx.x, sxy + 2 * threadIdx.x + 1,
+                                                              sid + threadIdx.x, 2 * BS, BS)) {
             if (a.ovf_list) {
                 const unsigned q = atomicAdd(a.ovf_count, 1u);
                 a.ovf_list[q] = (int)t;
