@@ -272,16 +272,22 @@ def main_c4(args):
     pos = [workloads.uniform_diracs(int(n), 2, seed=k + 1) for k, n in enumerate(sizes)]
     b = sdot_amd.Batch("mixed", diracs_per_cell=0.0, use_graph=not args.no_graph)
     b.set_problems(pos)
-    for _ in range(args.warmup):   # the first solve also runs the autotuner
-        r = b.solve(tol=1e-6)
-        print(f"# c4 warmup: status {r[0]} stats {r[5]}", file=sys.stderr, flush=True)
-    runs = []
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        st, ws, ms, pst, its, stats = b.solve(tol=1e-6)
-        runs.append(time.perf_counter() - t0)
-        print(f"# c4 step {len(runs)}: {runs[-1]:.3f} s status {st} stats {stats}", file=sys.stderr, flush=True)
-    sec = float(np.sum(runs) / len(runs))
+    modes = ["device", "host"] if args.newton == "both" else [args.newton]
+    timed = {}
+    for mode in modes:   # the first mode's warmup also runs the autotuner
+        b.set_schedule(device_newton=mode == "device")
+        for _ in range(args.warmup):
+            r = b.solve(tol=1e-6)
+            print(f"# c4 {mode} warmup: status {r[0]} stats {r[5]}", file=sys.stderr, flush=True)
+        runs = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            res = b.solve(tol=1e-6)
+            runs.append(time.perf_counter() - t0)
+            print(f"# c4 {mode} step {len(runs)}: {runs[-1]:.3f} s status {res[0]} stats {res[5]}", file=sys.stderr,
+                  flush=True)
+        timed[mode] = (float(np.sum(runs) / len(runs)), res)
+    sec, (st, ws, ms, pst, its, stats) = timed[modes[0]]
     # CPU: the oracle's own solve on the first problems, about 10 s, scaled by dirac count
     cpu = None
     if not args.no_cpu_baseline:
@@ -307,8 +313,12 @@ def main_c4(args):
         "dtype": "f32-clip/f64-solve", "data": "synthetic (xoshiro256** seeds 0..512)",
         "config": {"workload": f"{K} independent 2D OT Newton solves, N_k log-uniform in [1e3,1e5] "
                                f"(sum {int(sizes.sum())}), unit box, w0=0, targets 1/N_k, per-problem stop at max "
-                               "relative mass error 1e-6, one batched launch per kernel, PCG from a captured hipGraph",
-                   "problems": K, "total_diracs": int(sizes.sum()), "precision": "mixed"},
+                               "relative mass error 1e-6, " + (
+                                   "Newton driven from the device (one captured graph per iteration, replayed)"
+                                   if modes[0] == "device" else "host-driven Newton iterations, PCG from a captured "
+                                   "hipGraph"),
+                   "problems": K, "total_diracs": int(sizes.sum()), "precision": "mixed", "newton": modes[0]},
+        "newton_modes_s": {m: timed[m][0] for m in modes},
         "newton_iters_per_problem": {"min": int(its.min()), "median": float(np.median(its)), "max": int(its.max())},
         "ms_pcg_iteration_graph": stats["ms_pcg_iteration"],
         "max_rel_mass_err": err,
@@ -435,7 +445,9 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="c2 only: skip the C1 / C3 solve legs")
-    ap.add_argument("--no-graph", action="store_true", help="c4 only: launch the PCG kernels one by one")
+    ap.add_argument("--no-graph", action="store_true", help="c4 only: launch the kernels one by one")
+    ap.add_argument("--newton", choices=["device", "host", "both"], default="both",
+                    help="c4 only: device-driven Newton (the line's value), host-driven, or both")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run device-to-device copies of MIB MiB (known HBM bytes) so hardware "
                          "counters of the cell kernel can be scaled by the copy's counters")

@@ -181,8 +181,9 @@ sdot_status sdot_pd_der_integrals_wrt_weights(sdot_pd *pd, int64_t *offsets, int
 
 /* Builds the dirac-order CSR of the last pass in HBM (no host copy) and returns
  * handle-owned device pointers: offsets[N+1]; row i occupies
- * [offsets[i], offsets[i+1]) with ascending columns (a merged duplicate leaves a
- * trailing column -1, value 0).  Valid until the next call on the handle. */
+ * [offsets[i], offsets[i+1]) with strictly ascending columns, one entry per dirac
+ * (faces with a dirac and with its periodic images merged), the layout of
+ * get_der_integrals_wrt_weights.h:80-88.  Valid until the next call on the handle. */
 sdot_status sdot_pd_csr_device(sdot_pd *pd, const int64_t **d_offsets, const int64_t **d_cols, const double **d_vals);
 
 typedef struct sdot_solve_stats {
@@ -258,6 +259,13 @@ typedef struct sdot_batch_stats {
     double ms_pcg_iteration;     /* device time of one captured PCG iteration (all problems) */
     int precond_levels;          /* deepest multigrid hierarchy (0: Jacobi) */
     int precond_fallbacks;       /* problem solves whose hierarchy could not be built (Jacobi used) */
+    int tuned_cells_per_wave;    /* cells a wavefront grabs at a time in the persistent pass */
+    int tuned_first_tier;        /* capacity tier set of the pass (0 / 1) */
+    int tuning_passes;           /* timed passes of the autotuner in this call (0: schedule given or cached) */
+    int device_newton;           /* 1: Newton driven from the device (newton_replays graph replays) */
+    int newton_replays;
+    double seconds_device;       /* device-driven Newton: device time of all replays (passes and
+                                    solves are not timed apart; seconds_cells / seconds_solve are 0) */
 } sdot_batch_stats;
 
 /* dim must be 2 (3 returns SDOT_NOT_SUPPORTED). */
@@ -274,6 +282,16 @@ sdot_status sdot_batch_set_problems(sdot_batch *b, int64_t nb_problems, const in
 sdot_status sdot_batch_set_tuning(sdot_batch *b, int diracs_per_cell_x100, int use_graph);
 /* as sdot_pd_set_linear_solver */
 sdot_status sdot_batch_set_linear_solver(sdot_batch *b, int precond);
+/* Schedule of the batched passes and of Newton (no counterpart in the reference):
+ * cells_per_wave = cells a wavefront grabs at a time in the persistent first-tier
+ * pass (a multiple of 64 up to 4096; 0 = autotune); first_tier = capacity tier set
+ * (0: FP64 starts at 8-vertex LDS slots, mixed at 16; 1: FP64 at 16, mixed at 8;
+ * -1 = autotune); device_newton = 1: the Newton iteration of every problem (pass,
+ * residual, decision, accept / backtrack, multigrid setup, cg_per_replay CG
+ * iterations, update) is one captured hipGraph replayed until a device-side count of
+ * unfinished problems reaches 0; 0: one host-driven Newton iteration at a time. */
+sdot_status sdot_batch_set_schedule(sdot_batch *b, int cells_per_wave, int first_tier, int device_newton,
+                                    int cg_per_replay);
 /* Newton on every problem, starting from the weights given to set_problems; stop_rule
  * as sdot_ot_solve; the autotuning passes (first solve of a size class) use the start
  * weights.  weights_out (N), masses_out

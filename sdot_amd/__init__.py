@@ -65,7 +65,8 @@ class BatchStats(ctypes.Structure):
     _fields_ = [("newton_iters_max", _int), ("nb_converged", _int), ("pcg_iters_total", _int), ("backtracks", _int),
                 ("graph_replays", _int), ("tuned_dpc_x100", _int), ("max_rel_mass_err", _d), ("seconds_cells", _d),
                 ("seconds_solve", _d), ("seconds_total", _d), ("ms_pcg_iteration", _d), ("precond_levels", _int),
-                ("precond_fallbacks", _int)]
+                ("precond_fallbacks", _int), ("tuned_cells_per_wave", _int), ("tuned_first_tier", _int),
+                ("tuning_passes", _int), ("device_newton", _int), ("newton_replays", _int), ("seconds_device", _d)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -134,6 +135,7 @@ SIGNATURES = {
     "sdot_batch_set_problems": (_int, [_p, _i64, _p, _p, _p, _p, _p, _int]),
     "sdot_batch_set_tuning": (_int, [_p, _int, _int]),
     "sdot_batch_solve": (_int, [_p, _d, _int, _int, _p, _p, _p, _p, ctypes.POINTER(BatchStats)]),
+    "sdot_batch_set_schedule": (_int, [_p, _int, _int, _int, _int]),
     "sdot_srv_create": (_int, [ctypes.POINTER(_p), ctypes.POINTER(ServerConfig)]),
     "sdot_srv_submit": (_int, [_p, _int, _i64, _p, _p, _p, _p, ctypes.POINTER(_i64)]),
     "sdot_srv_poll": (_int, [_p, _i64, ctypes.POINTER(_int)]),
@@ -447,7 +449,8 @@ class Batch:
     (Solvers/OptimalTransportSolver.tcc:27-119).  A problem's result does not
     depend on the other problems of the batch."""
 
-    def __init__(self, precision="mixed", diracs_per_cell=0.0, use_graph=True, precond="amg"):
+    def __init__(self, precision="mixed", diracs_per_cell=0.0, use_graph=True, precond="amg", device_newton=True,
+                 cells_per_wave=0, first_tier=-1, cg_per_replay=16):
         h = _p()
         st = lib().sdot_batch_create(ctypes.byref(h), 2, FP64 if precision == "fp64" else MIXED)
         if st != SDOT_OK:
@@ -455,6 +458,7 @@ class Batch:
         self._h = h
         self.precision = precision
         self.set_tuning(diracs_per_cell, use_graph)
+        self.set_schedule(cells_per_wave, first_tier, device_newton, cg_per_replay)
         code = PRECOND_AMG if precond == "amg" else PRECOND_JACOBI
         self._check(lib().sdot_batch_set_linear_solver(self._h, code), "set_linear_solver")
         self.offsets = None
@@ -473,6 +477,12 @@ class Batch:
     def set_tuning(self, diracs_per_cell=0.0, use_graph=True):
         self._check(lib().sdot_batch_set_tuning(self._h, int(round(diracs_per_cell * 100)), int(bool(use_graph))),
                     "set_tuning")
+
+    def set_schedule(self, cells_per_wave=0, first_tier=-1, device_newton=True, cg_per_replay=16):
+        """Pass schedule (cells per wavefront grab, capacity tier set; 0 / -1 = autotune)
+        and whether Newton runs from one replayed device graph."""
+        self._check(lib().sdot_batch_set_schedule(self._h, int(cells_per_wave), int(first_tier), int(bool(device_newton)),
+                                                  int(cg_per_replay)), "set_schedule")
 
     def set_problems(self, positions, weights=None, targets=None, boxes=None, ball=False):
         """positions: list of (N_k, 2) arrays; weights/targets: lists of (N_k,) arrays

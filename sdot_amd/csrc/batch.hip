@@ -30,11 +30,15 @@
 #include "pcg.h"
 
 using namespace sdot_amd;
+using namespace sdot_amd::pcg_state;
 
 namespace {
 
 constexpr int BB = 256;         // rows per block
 enum { ACT_NONE = 0, ACT_ACCEPT = 1, ACT_BACKTRACK = 2 };
+// device-driven Newton: where each problem stands (0 must be "needs a pass": the
+// pass and residual kernels take this array as their skip mask, like the done marks)
+enum { ST_PASS = 0, ST_CG = 1, ST_DONE = 2 };
 
 __device__ inline double bsum(double v, double *sm) {
     sm[threadIdx.x] = v;
@@ -131,13 +135,14 @@ __global__ void kb_gather(const double *src, const int *perm, int64_t n, double 
 }
 
 // ---- Newton control ------------------------------------------------------------
-// b = m - target (grid order) and the per-block max of |b| / target
+// b = m - target (grid order) and the per-block max of |b| / target, for the problems
+// whose skip mark is 0 (done marks, or the device Newton state ST_PASS)
 __global__ __launch_bounds__(BB) void kb_residual(Blocks B, const double *m, const double *tsort, double *b,
-                                                  double *part, const int *pdone) {
+                                                  double *part, const int *pskip) {
     __shared__ double sm[BB];
     const BlockRows br = block_rows(B, blockIdx.x);
     double mx = 0;
-    if (!pdone[br.p])
+    if (!pskip[br.p])
         for (int64_t t = br.r0 + threadIdx.x; t < br.r1; t += BB) {
             const double v = m[t] - tsort[t];
             b[t] = v;
@@ -193,6 +198,152 @@ __global__ void kb_export(const double *sorted, const int *inv_perm, int64_t n, 
     if (i < n) out[i] = sorted[inv_perm[i]];
 }
 
+// ---- persistent pass work list ----------------------------------------------------
+// The problems whose skip mark is 0, in problem order, with their cells numbered
+// consecutively (wl_off[j]: first number of the j-th listed problem; wl_off[count] =
+// cells in all; wl_meta = {count, cells}); clears the 8 grab counters.  One block:
+// block-wide inclusive scans over chunks of BB problems.
+__global__ __launch_bounds__(BB) void kb_worklist(const int *pskip, const int64_t *off, int64_t K, int64_t *wl_off,
+                                                  int *wl_prob, int64_t *wl_meta, unsigned long long *grab) {
+    __shared__ int sn[BB];
+    __shared__ int64_t sc[BB];
+    if (threadIdx.x < 8) grab[threadIdx.x] = 0;
+    int carry_n = 0;
+    int64_t carry_c = 0;
+    for (int64_t b = 0; b < K; b += BB) {
+        const int64_t p = b + threadIdx.x;
+        const int need = p < K && pskip[p] == 0;
+        const int64_t cells = need ? off[p + 1] - off[p] : 0;
+        sn[threadIdx.x] = need;
+        sc[threadIdx.x] = cells;
+        __syncthreads();
+        for (int d = 1; d < BB; d <<= 1) {
+            const int vn = threadIdx.x >= d ? sn[threadIdx.x - d] : 0;
+            const int64_t vc = threadIdx.x >= d ? sc[threadIdx.x - d] : 0;
+            __syncthreads();
+            sn[threadIdx.x] += vn;
+            sc[threadIdx.x] += vc;
+            __syncthreads();
+        }
+        if (need) {
+            const int j = carry_n + sn[threadIdx.x] - 1;
+            wl_prob[j] = (int)p;
+            wl_off[j] = carry_c + sc[threadIdx.x] - cells;
+        }
+        carry_n += sn[BB - 1];
+        carry_c += sc[BB - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        wl_off[carry_n] = carry_c;
+        wl_meta[0] = carry_n;
+        wl_meta[1] = carry_c;
+    }
+}
+
+// ---- device-driven Newton (the per-problem rules of OptimalTransportSolver.tcc:60-112) ----
+struct NewtonDev {
+    int *state, *iters, *backtracks, *pcg_iters, *dw_ok, *status, *passes;
+    int *fallbacks;       // solves whose multigrid hierarchy could not be built (Jacobi used)
+    double *err;          // relative mass error of the last pass with no empty cell
+    double *mdw;          // max |dw| of the last update
+    int *stat;            // [0] problems not done, [1] sticky pass failure flags
+};
+
+__global__ __launch_bounds__(BB) void kb_newton_init(NewtonDev nd, int64_t K, int *pstop, double *S) {
+    const int64_t p = blockIdx.x * (int64_t)BB + threadIdx.x;
+    if (p == 0) {
+        nd.stat[0] = (int)K;
+        nd.stat[1] = 0;
+    }
+    if (p >= K) return;
+    S[p * S_NS + S_DONE] = 1;   // no solve running
+    S[p * S_NS + S_START] = 0;
+    nd.state[p] = ST_PASS;
+    nd.iters[p] = nd.backtracks[p] = nd.pcg_iters[p] = nd.dw_ok[p] = nd.passes[p] = nd.fallbacks[p] = 0;
+    nd.status[p] = SDOT_NOT_CONVERGED;
+    nd.err[p] = INFINITY;
+    nd.mdw[p] = INFINITY;
+    pstop[p] = 0;   // complete passes until the first decision
+}
+
+// after a pass: the decision of every problem that had one (tick() on the host makes
+// the same one); a pass that failed (flags) decides nothing and is recorded in stat[1]
+__global__ __launch_bounds__(BB) void kb_decide(NewtonDev nd, int64_t K, const unsigned *pflags, const double *perr,
+                                                const unsigned *flags, int *act, double *S, int *amg_off, double tol,
+                                                int stop_rule, int max_iter) {
+    const int64_t p = blockIdx.x * (int64_t)BB + threadIdx.x;
+    const unsigned bad = flags[0] & (FLAG_BAD_INPUT | FLAG_CELL_OVERFLOW | FLAG_ROW_OVERFLOW);
+    if (p == 0 && bad) nd.stat[1] |= (int)bad;
+    if (p >= K) return;
+    double *sp = S + p * S_NS;
+    int a = ACT_NONE;
+    sp[S_START] = 0;
+    if (nd.state[p] == ST_PASS && !bad) {
+        const bool empty = (pflags[p] & FLAG_EMPTY) != 0;
+        const double e = perr[p];
+        if (!empty) nd.err[p] = e;
+        nd.passes[p] += 1;
+        if (nd.dw_ok[p] || (stop_rule == 0 && !empty && e < tol)) {   // converged at these weights
+            nd.state[p] = ST_DONE;
+            nd.status[p] = SDOT_OK;
+        } else if (nd.iters[p] >= max_iter) {   // this pass only evaluated the last update
+            nd.state[p] = ST_DONE;
+            nd.status[p] = SDOT_NOT_CONVERGED;
+        } else {
+            nd.iters[p] += 1;
+            if (empty) {   // :72-79
+                a = ACT_BACKTRACK;
+                nd.backtracks[p] += 1;
+            } else {       // the solve starts in this replay
+                a = ACT_ACCEPT;
+                nd.state[p] = ST_CG;
+                sp[S_START] = 1;
+                sp[S_DONE] = 0;
+                sp[S_IT] = 0;
+                amg_off[p] = 0;
+            }
+        }
+    }
+    act[p] = a;
+}
+
+// w[perm[t]] -= x[t] for the problems whose solve ended (state ST_CG, S_DONE); per-block max |x|
+__global__ __launch_bounds__(BB) void kb_update_dev(Blocks B, const double *x, const int *perm, double *w,
+                                                    const int *state, const double *S, double *part) {
+    __shared__ double sm[BB];
+    const BlockRows br = block_rows(B, blockIdx.x);
+    double mx = 0;
+    if (state[br.p] == ST_CG && S[br.p * S_NS + S_DONE] != 0)
+        for (int64_t t = br.r0 + threadIdx.x; t < br.r1; t += BB) {
+            w[perm[t]] -= x[t];
+            mx = fmax(mx, fabs(x[t]));
+        }
+    mx = bmax(mx, sm);
+    if (threadIdx.x == 0) part[blockIdx.x] = mx;
+}
+
+// the problems whose solve ended go back to a pass; stop_if_void marks of the next
+// pass; stat[0] = problems not done.  One block.
+__global__ __launch_bounds__(BB) void kb_finish(NewtonDev nd, int64_t K, const double *S, const int *amg_off,
+                                                int *pstop, double tol, int stop_rule, int max_iter) {
+    __shared__ double sm[BB];
+    double cnt = 0;
+    for (int64_t p = threadIdx.x; p < K; p += BB) {
+        if (nd.state[p] == ST_CG && S[p * S_NS + S_DONE] != 0) {
+            nd.pcg_iters[p] += (int)S[p * S_NS + S_IT];
+            nd.fallbacks[p] += amg_off[p] != 0;
+            if (stop_rule == 1 && nd.mdw[p] < tol) nd.dw_ok[p] = 1;   // done once the next pass evaluates it
+            nd.state[p] = ST_PASS;
+        }
+        // an empty cell in the next pass means a backtrack, whose pass results are not read
+        pstop[p] = nd.state[p] == ST_PASS && !nd.dw_ok[p] && nd.iters[p] < max_iter;
+        cnt += nd.state[p] != ST_DONE ? 1.0 : 0.0;
+    }
+    cnt = bsum(cnt, sm);
+    if (threadIdx.x == 0) nd.stat[0] = (int)cnt;
+}
+
 // ---- PCG on problem-aligned blocks --------------------------------------------
 // relayout: kept problems move from their old offset to the new one (dirac order)
 __global__ __launch_bounds__(BB) void kb_move(const int *newprob, int64_t n, const int64_t *src_off,
@@ -227,8 +378,13 @@ __global__ __launch_bounds__(BB) void kb_owner(const int64_t *off, int64_t K, in
 }
 
 std::mutex tune_mutex;
-std::map<int, int> tune_cache;  // size class -> diracs per grid cell x100
-constexpr int TUNE_CANDIDATES[] = {150, 250, 400};
+struct Tuned {
+    int dpc_x100, cells_per_wave, tierset;
+};
+std::map<int, Tuned> tune_cache;  // size class -> pass schedule
+constexpr int TUNE_DPC[] = {150, 250, 400};
+constexpr int TUNE_CPW[] = {64, 128, 256};
+constexpr int TUNE_TIERS[] = {0, 1};
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -261,6 +417,39 @@ struct BatchEngine::Impl {
     std::vector<int64_t> blk_off;
     int64_t nblk = 0, ncodes = 0;
     PcgSolver pcg;
+    // persistent pass work list
+    DevBuf<int64_t> wl_off, wl_meta;
+    DevBuf<int> wl_prob;
+    DevBuf<unsigned long long> grab;
+    int cpw = 128;                       // cells a wavefront grabs at a time
+    // device-driven Newton: per-problem state, the captured replay and its key
+    DevBuf<int> nst, nit, nbt, npcg, ndw, nstatus, npass, nfb, nstat;
+    DevBuf<double> nerr, nmdw;
+    hipGraphExec_t rgraph = nullptr;
+    struct Key {
+        const void *ell_col = nullptr, *ell_val = nullptr, *w = nullptr, *pcg_state = nullptr;
+        int64_t N = -1, K = -1;
+        int ell_cap = 0, cpw = 0, tierset = -1, cg_per_replay = 0, amg = -1, stop_rule = -1, max_iter = -1;
+        double tol = 0;
+        bool operator==(const Key &o) const {
+            return ell_col == o.ell_col && ell_val == o.ell_val && w == o.w && pcg_state == o.pcg_state && N == o.N &&
+                   K == o.K && ell_cap == o.ell_cap && cpw == o.cpw && tierset == o.tierset &&
+                   cg_per_replay == o.cg_per_replay && amg == o.amg && stop_rule == o.stop_rule &&
+                   max_iter == o.max_iter && tol == o.tol;
+        }
+    } rkey;
+    int *hstat = nullptr;                // pinned: two slots of {not done, failure flags}
+    hipEvent_t sev[2] = {};
+    NewtonDev newton() {
+        return NewtonDev{nst.p, nit.p, nbt.p, npcg.p, ndw.p, nstatus.p, npass.p, nfb.p, nerr.p, nmdw.p, nstat.p};
+    }
+    void drop_replay() {
+        if (!rgraph) return;
+        std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());
+        (void)hipGraphExecDestroy(rgraph);
+        rgraph = nullptr;
+        rkey = Key{};
+    }
 };
 
 BatchEngine::BatchEngine(int precision, int ball, int dpc_x100, bool use_graph)
@@ -278,6 +467,16 @@ BatchEngine::BatchEngine(int precision, int ball, int dpc_x100, bool use_graph)
 }
 
 BatchEngine::~BatchEngine() {
+    im->drop_replay();
+    for (auto &e : im->sev)
+        if (e) (void)hipEventDestroy(e);
+    if (im->hstat) (void)hipHostFree(im->hstat);
+    for (auto *b : {&im->nst, &im->nit, &im->nbt, &im->npcg, &im->ndw, &im->nstatus, &im->npass, &im->nfb, &im->nstat,
+                    &im->wl_prob})
+        b->release();
+    for (auto *b : {&im->nerr, &im->nmdw}) b->release();
+    for (auto *b : {&im->wl_off, &im->wl_meta}) b->release();
+    im->grab.release();
     for (auto *b : {&im->prob, &im->blk_prob, &im->pdone, &im->pstop, &im->act}) b->release();
     for (auto *b : {&im->dcode_base, &im->doff, &im->dblk, &im->src_off}) b->release();
     for (auto *b : {&im->pdom, &im->tie, &im->w0, &im->tgt, &im->tsort, &im->old, &im->b, &im->red, &im->perr,
@@ -401,6 +600,16 @@ void BatchEngine::relayout(const std::vector<int> &keep, const std::vector<NewPr
     im->pstop.ensure(K);
     im->act.ensure(K);
     im->wglob.ensure(K);
+    im->wl_off.ensure(K + 1);
+    im->wl_prob.ensure(K);
+    im->wl_meta.ensure(2);
+    im->grab.ensure(8);
+    for (auto *b : {&im->nst, &im->nit, &im->nbt, &im->npcg, &im->ndw, &im->nstatus, &im->npass, &im->nfb})
+        b->ensure(K);
+    im->nstat.ensure(2);
+    im->nerr.ensure(K);
+    im->nmdw.ensure(K);
+    im->drop_replay();
     if (pd->ell_cap < ELL_CAP2) pd->ell_cap = ELL_CAP2;
     pd->N = N;
     pd->mass.ensure(N);
@@ -430,6 +639,7 @@ void BatchEngine::set_resolution(int dpc_x100) {
 // per-problem grids in one code space, the counting sort, sorted positions, targets in grid order
 void BatchEngine::rebuild_grid() {
     hipStream_t s = pd->stream;
+    im->drop_replay();   // the grid arrays may move
     const int64_t K = (int64_t)probs.size(), N = N_;
     std::vector<GridInfo> grids(K);
     std::vector<int64_t> cb(K + 1, 0);
@@ -489,12 +699,12 @@ void BatchEngine::restart() {
 
 namespace {
 
-// one cell pass over the problems not marked done; device ms
-double engine_pass(sdot_pd *pd, BatchEngine::Impl *im, int64_t K, int64_t N, bool &regrow) {
+// One cell pass, enqueued only (capturable): sorted weights and bounds, the work list
+// of the problems whose skip mark is 0, and the capacity tiers of the 2D kernel, the
+// first one persistent over the work list.
+void enqueue_pass(sdot_pd *pd, BatchEngine::Impl *im, int64_t K, int64_t N, const int *pskip) {
     hipStream_t s = pd->stream;
     const int64_t nc = im->ncodes;
-    regrow = false;
-    HIPCK(hipEventRecord(pd->ev[0], s));
     fill_u32(pd->flags.p, 3, 0u, s);
     fill_u32(im->pflags.p, K, 0u, s);
     hipLaunchKernelGGL(kb_gather_w, dim3(nblocks(N, BB)), dim3(BB), 0, s, pd->w.p, pd->perm.p, im->prob.p, im->doff.p,
@@ -502,8 +712,9 @@ double engine_pass(sdot_pd *pd, BatchEngine::Impl *im, int64_t K, int64_t N, boo
     hipLaunchKernelGGL(kb_cell_wmax, dim3(nblocks(nc, BB)), dim3(BB), 0, s, pd->cell_start.p, nc, pd->sw.p,
                        pd->cell_wmax.p);
     hipLaunchKernelGGL(kb_wmax, dim3((unsigned)K), dim3(BB), 0, s, pd->sw.p, im->doff.p, im->wglob.p);
+    hipLaunchKernelGGL(kb_worklist, dim3(1), dim3(BB), 0, s, pskip, im->doff.p, K, im->wl_off.p, im->wl_prob.p,
+                       im->wl_meta.p, im->grab.p);
     HIPCK(hipGetLastError());
-    HIPCK(hipEventRecord(pd->ev[1], s));
     CellArgs a{};
     a.N = N;
     a.spos = pd->spos.p;
@@ -530,10 +741,24 @@ double engine_pass(sdot_pd *pd, BatchEngine::Impl *im, int64_t K, int64_t N, boo
     a.code_base = im->dcode_base.p;
     a.pdom = im->pdom.p;
     a.pflags = im->pflags.p;
-    a.pdone = im->pdone.p;
+    a.pdone = nullptr;   // the work list holds only the problems that need the pass
     a.pstop = im->pstop.p;
     a.stop_if_void = 1;
+    a.wl_off = im->wl_off.p;
+    a.wl_prob = im->wl_prob.p;
+    a.wl_meta = im->wl_meta.p;
+    a.pb_off = im->doff.p;
+    a.grab = im->grab.p;
+    a.cpw = im->cpw;
     run_cells2(pd, a);
+}
+
+// one cell pass over the problems not marked done; device ms
+double engine_pass(sdot_pd *pd, BatchEngine::Impl *im, int64_t K, int64_t N, bool &regrow) {
+    hipStream_t s = pd->stream;
+    regrow = false;
+    HIPCK(hipEventRecord(pd->ev[0], s));
+    enqueue_pass(pd, im, K, N, im->pdone.p);
     HIPCK(hipEventRecord(pd->ev[2], s));
     unsigned fl = 0;
     HIPCK(hipMemcpyAsync(&fl, pd->flags.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
@@ -547,6 +772,7 @@ double engine_pass(sdot_pd *pd, BatchEngine::Impl *im, int64_t K, int64_t N, boo
         pd->ell_col.ensure((size_t)pd->ell_cap * N);
         pd->ell_val.ensure((size_t)pd->ell_cap * N);
         im->pcg.invalidate();
+        im->drop_replay();
         regrow = true;
         return 0;
     }
@@ -663,6 +889,201 @@ int BatchEngine::tick(const TickParams &tp, std::vector<int> &finished, TickStat
     return running;
 }
 
+// ---- device-driven Newton ------------------------------------------------------------
+// One replay advances every problem by what its state asks for: a problem that needs a
+// pass gets it, then its decision (kb_decide: converged, out of iterations, backtrack,
+// or accept, which starts its linear solve); every problem in a solve runs
+// cg_per_replay more CG iterations; a problem whose solve ended takes its update and
+// needs a pass again.  The replay is one captured graph; the host only replays it and
+// reads two counters (problems not done, pass failures), one replay behind the device.
+// Each problem follows the same steps as under tick(), so its iterates are the same.
+
+void BatchEngine::set_schedule(int cells_per_wave, int tierset) {
+    if (cells_per_wave < 64 || cells_per_wave % 64 || cells_per_wave > 4096 || (tierset != 0 && tierset != 1))
+        throw StatusError{SDOT_INVALID_ARG, "cells per wavefront: a multiple of 64 up to 4096; tier set 0 or 1"};
+    im->cpw = cells_per_wave;
+    pd->tierset = tierset;
+}
+
+int BatchEngine::cells_per_wave() const { return im->cpw; }
+int BatchEngine::precond_levels() const { return im->pcg.levels(); }
+int BatchEngine::tierset() const { return pd->tierset; }
+
+// the kernels of one replay (enqueued only)
+void BatchEngine::enqueue_replay(const TickParams &tp, int cg_per_replay) {
+    hipStream_t s = pd->stream;
+    const int64_t K = (int64_t)probs.size(), N = N_;
+    const NewtonDev nd = im->newton();
+    const PcgMat A{pd->diag.p, pd->row_len.p, pd->ell_col.p, pd->ell_val.p, N};
+    enqueue_pass(pd, im, K, N, nd.state);
+    hipLaunchKernelGGL(kb_residual, dim3((unsigned)im->nblk), dim3(BB), 0, s, blocks_of(im), pd->mass.p, im->tsort.p,
+                       im->b.p, im->red.p, nd.state);
+    hipLaunchKernelGGL(kb_pmax, dim3((unsigned)K), dim3(BB), 0, s, im->red.p, im->dblk.p, im->perr.p);
+    hipLaunchKernelGGL(kb_decide, dim3(nblocks(K, BB)), dim3(BB), 0, s, nd, K, im->pflags.p, im->perr.p, pd->flags.p,
+                       im->act.p, im->pcg.state(), im->pcg.fallback(), tp.tol, tp.stop_rule, tp.max_iter);
+    hipLaunchKernelGGL(kb_accept, dim3(nblocks(N, BB)), dim3(BB), 0, s, im->prob.p, N, im->act.p, pd->w.p, im->old.p);
+    if (!ball_)   // :62-63
+        hipLaunchKernelGGL(kb_gauge, dim3(nblocks(K, 256)), dim3(256), 0, s, pd->diag.p, pd->inv_perm.p, im->doff.p,
+                           im->act.p, K);
+    im->pcg.enqueue_start(A, im->b.p, im->x.p, im->perr.p);
+    im->pcg.enqueue_iterations(A, im->x.p, cg_per_replay, 100000);
+    hipLaunchKernelGGL(kb_update_dev, dim3((unsigned)im->nblk), dim3(BB), 0, s, blocks_of(im), im->x.p, pd->perm.p,
+                       pd->w.p, nd.state, im->pcg.state(), im->red.p);
+    hipLaunchKernelGGL(kb_pmax, dim3((unsigned)K), dim3(BB), 0, s, im->red.p, im->dblk.p, nd.mdw);
+    hipLaunchKernelGGL(kb_finish, dim3(1), dim3(BB), 0, s, nd, K, im->pcg.state(), im->pcg.fallback(), im->pstop.p,
+                       tp.tol, tp.stop_rule, tp.max_iter);
+    HIPCK(hipGetLastError());
+}
+
+void BatchEngine::capture_replay(const TickParams &tp, int cg_per_replay) {
+    hipStream_t s = pd->stream;
+    const int64_t K = (int64_t)probs.size(), N = N_;
+    Impl::Key key;
+    key.ell_col = pd->ell_col.p;
+    key.ell_val = pd->ell_val.p;
+    key.w = pd->w.p;
+    key.pcg_state = im->pcg.state();
+    key.N = N;
+    key.K = K;
+    key.ell_cap = pd->ell_cap;
+    key.cpw = im->cpw;
+    key.tierset = pd->tierset;
+    key.cg_per_replay = cg_per_replay;
+    key.amg = im->pcg.amg() ? 1 : 0;
+    key.stop_rule = tp.stop_rule;
+    key.max_iter = tp.max_iter;
+    key.tol = tp.tol;
+    if (im->rgraph && im->rkey == key) return;
+    std::lock_guard<std::recursive_mutex> lk(device_alloc_mutex());   // no device free while recording
+    im->drop_replay();
+    hipGraph_t g = nullptr;
+    pd->capturing = true;
+    HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue_replay(tp, cg_per_replay);
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        pd->capturing = false;
+        throw;
+    }
+    pd->capturing = false;
+    HIPCK(hipStreamEndCapture(s, &g));
+    const hipError_t e = hipGraphInstantiate(&im->rgraph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) im->rgraph = nullptr;
+    HIPCK(e);
+    im->rkey = key;
+}
+
+// Hessian rows widened in place: the layout [e * N + t] keeps the entries of the
+// problems that are inside a solve at the same places
+void BatchEngine::widen_rows() {
+    hipStream_t s = pd->stream;
+    if (pd->ell_cap >= ELL_CAP2_WIDE) throw StatusError{SDOT_CAPACITY_OVERFLOW, "Hessian row longer than the row capacity"};
+    const size_t old_n = (size_t)pd->ell_cap * N_, new_n = (size_t)ELL_CAP2_WIDE * N_;
+    DevBuf<int> c2;
+    DevBuf<double> v2;
+    c2.ensure(new_n);
+    v2.ensure(new_n);
+    HIPCK(hipMemcpyAsync(c2.p, pd->ell_col.p, old_n * sizeof(int), hipMemcpyDeviceToDevice, s));
+    HIPCK(hipMemcpyAsync(v2.p, pd->ell_val.p, old_n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCK(hipStreamSynchronize(s));
+    std::swap(pd->ell_col, c2);
+    std::swap(pd->ell_val, v2);
+    c2.release();
+    v2.release();
+    pd->ell_cap = ELL_CAP2_WIDE;
+    im->pcg.invalidate();
+    im->drop_replay();
+}
+
+int BatchEngine::solve_device(const TickParams &tp, int cg_per_replay, DeviceSolveStats &ds) {
+    hipStream_t s = pd->stream;
+    const int64_t K = (int64_t)probs.size();
+    if (K == 0) return 0;
+    if (!im->hstat) {
+        void *h = nullptr;
+        HIPCK(hipHostMalloc(&h, 4 * sizeof(int), hipHostMallocDefault));
+        im->hstat = static_cast<int *>(h);
+        for (auto &e : im->sev) HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    const NewtonDev nd = im->newton();
+    hipLaunchKernelGGL(kb_newton_init, dim3(nblocks(K, BB)), dim3(BB), 0, s, nd, K, im->pstop.p, im->pcg.state());
+    HIPCK(hipGetLastError());
+    HIPCK(hipEventRecord(pd->ev[0], s));
+    // every problem ends: at most max_iter + 1 passes, each solve at most 100000 CG iterations
+    const int64_t bound = (int64_t)(tp.max_iter + 2) * (100000 / cg_per_replay + 2);
+    int64_t r = 0;
+    int running = (int)K;
+    bool pending = false;   // a status copy in flight for the previous replay
+    auto launch = [&](int64_t k) {
+        if (use_graph_) HIPCK(hipGraphLaunch(im->rgraph, s));
+        else enqueue_replay(tp, cg_per_replay);
+        HIPCK(hipMemcpyAsync(im->hstat + 2 * (k & 1), nd.stat, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCK(hipEventRecord(im->sev[k & 1], s));
+    };
+    if (use_graph_) capture_replay(tp, cg_per_replay);
+    while (running > 0) {
+        if (r >= bound) throw StatusError{SDOT_NOT_CONVERGED, "device Newton did not end"};
+        launch(r);
+        ++r;
+        if (!pending) {
+            pending = true;
+            continue;
+        }
+        // read the replay before the one just launched
+        HIPCK(hipEventSynchronize(im->sev[(r - 2) & 1]));
+        const int *st = im->hstat + 2 * ((r - 2) & 1);
+        const int fail = st[1];
+        running = st[0];
+        if (fail) {
+            HIPCK(hipStreamSynchronize(s));
+            if (fail & FLAG_BAD_INPUT) throw StatusError{SDOT_INVALID_ARG, "non-finite weight or coincident diracs in the batch"};
+            if (fail & FLAG_CELL_OVERFLOW) throw StatusError{SDOT_CAPACITY_OVERFLOW, "cell polygon exceeds 64 vertices"};
+            // a Hessian row outgrew its capacity: the problems of that pass decided
+            // nothing, the others went on; widen and go on
+            widen_rows();
+            ++ds.widened;
+            fill_u32(reinterpret_cast<uint32_t *>(nd.stat + 1), 1, 0u, s);
+            if (use_graph_) capture_replay(tp, cg_per_replay);
+            pending = false;
+            running = 1;
+        }
+    }
+    HIPCK(hipEventRecord(pd->ev[1], s));
+    HIPCK(hipStreamSynchronize(s));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, pd->ev[0], pd->ev[1]) != hipSuccess) (void)hipGetLastError();
+    ds.replays += r;
+    ds.ms_device += ms;
+    // per-problem results
+    std::vector<int> it(K), bt(K), pc(K), stt(K), np(K), fb(K);
+    std::vector<double> er(K);
+    HIPCK(hipMemcpyAsync(it.data(), nd.iters, K * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(bt.data(), nd.backtracks, K * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(pc.data(), nd.pcg_iters, K * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(stt.data(), nd.status, K * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(np.data(), nd.passes, K * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(fb.data(), nd.fallbacks, K * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(er.data(), nd.err, K * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    for (int64_t k = 0; k < K; ++k) {
+        BatchProblem &b = probs[k];
+        b.iters = it[k];
+        b.backtracks = bt[k];
+        b.pcg_iters = pc[k];
+        b.status = stt[k];
+        b.err = er[k];
+        b.done = 1;
+        ds.backtracks += bt[k];
+        ds.pcg_iters += pc[k];
+        ds.cells += (int64_t)np[k] * b.n;
+        ds.fallbacks += fb[k];
+    }
+    return 0;
+}
+
 void BatchEngine::read_weights(int k, double *out) {
     const BatchProblem &b = probs.at(k);
     HIPCK(hipMemcpyAsync(out, pd->w.p + b.off, b.n * sizeof(double), hipMemcpyDeviceToHost, pd->stream));
@@ -701,6 +1122,10 @@ struct sdot_batch {
     int precision = SDOT_FP64;
     int dpc_x100 = 0;             // 0: autotune on the first solve of a size class
     int use_graph = 1;
+    int cells_per_wave = 0;       // 0: autotune
+    int tierset = -1;             // -1: autotune
+    int device_newton = 1;        // Newton driven from the device (one replayed graph)
+    int cg_per_replay = 16;
     int precond = SDOT_PRECOND_AMG;
     std::vector<double> hpos;     // host copy of the positions (grids are built from it)
     std::string err;
@@ -813,6 +1238,18 @@ sdot_status sdot_batch_set_problems(sdot_batch *B, int64_t K, const int64_t *off
     });
 }
 
+sdot_status sdot_batch_set_schedule(sdot_batch *B, int cells_per_wave, int first_tier, int device_newton,
+                                    int cg_per_replay) {
+    if (!B || (cells_per_wave != 0 && (cells_per_wave < 64 || cells_per_wave % 64 || cells_per_wave > 4096)) ||
+        first_tier < -1 || first_tier > 1 || cg_per_replay < 1 || cg_per_replay > 256)
+        return SDOT_INVALID_ARG;
+    B->cells_per_wave = cells_per_wave;
+    B->tierset = first_tier;
+    B->device_newton = device_newton ? 1 : 0;
+    B->cg_per_replay = cg_per_replay;
+    return SDOT_OK;
+}
+
 sdot_status sdot_batch_solve(sdot_batch *B, double tol, int stop_rule, int max_iter, double *weights_out,
                              double *masses_out, int *status_out, int *iters_out, sdot_batch_stats *stats) {
     if (!B || !weights_out || !(tol > 0) || max_iter < 1 || (stop_rule != 0 && stop_rule != 1)) return SDOT_INVALID_ARG;
@@ -825,40 +1262,90 @@ sdot_status sdot_batch_solve(sdot_batch *B, double tol, int stop_rule, int max_i
         const int64_t K = (int64_t)E.probs.size(), N = E.total();
         const auto t0 = std::chrono::steady_clock::now();
         sdot_batch_stats st{};
-        // grid resolution: given, cached for this size class, or tuned now on the start weights
-        int dpc = B->dpc_x100;
+        // pass schedule (grid resolution, cells per wavefront grab, capacity tiers): given,
+        // cached for this size class, or tuned now on the start weights by coordinate
+        // descent over the candidates (one pass per candidate after a first touch)
+        Tuned tu{B->dpc_x100, B->cells_per_wave, B->tierset};
+        const bool autotune = !tu.dpc_x100 || !tu.cells_per_wave || tu.tierset < 0;
         const int size_class = ((int)std::lround(std::log2((double)N / (double)K)) * 2 + E.pd->ball) * 2 +
                                (B->precision == SDOT_MIXED);
-        if (!dpc) {
+        bool cached = false;
+        if (autotune) {
             std::lock_guard<std::mutex> lk(tune_mutex);
             auto it = tune_cache.find(size_class);
-            if (it != tune_cache.end()) dpc = it->second;
+            if (it != tune_cache.end()) {
+                if (!tu.dpc_x100) tu.dpc_x100 = it->second.dpc_x100;
+                if (!tu.cells_per_wave) tu.cells_per_wave = it->second.cells_per_wave;
+                if (tu.tierset < 0) tu.tierset = it->second.tierset;
+                cached = true;
+            }
         }
         E.restart();
-        if (!dpc) {
-            double best = DBL_MAX;
-            for (int cand : TUNE_CANDIDATES) {
-                E.set_resolution(cand);
+        if (autotune && !cached) {
+            Tuned best{tu.dpc_x100 ? tu.dpc_x100 : 250, tu.cells_per_wave ? tu.cells_per_wave : 128,
+                       tu.tierset >= 0 ? tu.tierset : 0};
+            auto time_of = [&](const Tuned &c) {
+                E.set_resolution(c.dpc_x100);
+                E.set_schedule(c.cells_per_wave, c.tierset);
                 (void)E.pass_only();   // first touch
-                const double ms = E.pass_only();
-                if (ms < best) {
-                    best = ms;
-                    dpc = cand;
+                return E.pass_only();
+            };
+            double best_ms = DBL_MAX;
+            auto sweep = [&](auto field, const int *cand, int nc, bool fixed) {
+                if (fixed) return;
+                Tuned keep = best;
+                for (int q = 0; q < nc; ++q) {
+                    Tuned c = keep;
+                    c.*field = cand[q];
+                    const double ms = time_of(c);
+                    ++st.tuning_passes;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = c;
+                    }
                 }
-            }
+            };
+            sweep(&Tuned::dpc_x100, TUNE_DPC, 3, tu.dpc_x100 != 0);
+            sweep(&Tuned::cells_per_wave, TUNE_CPW, 3, tu.cells_per_wave != 0);
+            sweep(&Tuned::tierset, TUNE_TIERS, 2, tu.tierset >= 0);
+            tu = best;
             std::lock_guard<std::mutex> lk(tune_mutex);
-            tune_cache[size_class] = dpc;
+            tune_cache[size_class] = tu;
         }
-        st.tuned_dpc_x100 = dpc;
-        E.set_resolution(dpc);
+        st.tuned_dpc_x100 = tu.dpc_x100;
+        st.tuned_cells_per_wave = tu.cells_per_wave;
+        st.tuned_first_tier = tu.tierset;
+        E.set_resolution(tu.dpc_x100);
+        E.set_schedule(tu.cells_per_wave, tu.tierset);
+        E.restart();
         TickParams tp;
         tp.tol = tol;
         tp.stop_rule = stop_rule;
         tp.max_iter = max_iter;
-        TickStats ts;
-        std::vector<int> fin;
-        for (int guard = 0; guard <= max_iter + 1; ++guard)
-            if (E.tick(tp, fin, ts) == 0) break;
+        if (B->device_newton) {
+            DeviceSolveStats ds;
+            E.solve_device(tp, B->cg_per_replay, ds);
+            st.device_newton = 1;
+            st.newton_replays = (int)ds.replays;
+            st.pcg_iters_total = ds.pcg_iters;
+            st.backtracks = ds.backtracks;
+            st.seconds_device = ds.ms_device * 1e-3;
+            st.precond_levels = E.precond_levels();
+            st.precond_fallbacks = ds.fallbacks;
+        } else {
+            TickStats ts;
+            std::vector<int> fin;
+            for (int guard = 0; guard <= max_iter + 1; ++guard)
+                if (E.tick(tp, fin, ts) == 0) break;
+            st.pcg_iters_total = ts.pcg_iters;
+            st.backtracks = ts.backtracks;
+            st.graph_replays = ts.graph_replays;
+            st.seconds_cells = ts.ms_pass * 1e-3;
+            st.seconds_solve = ts.ms_pcg * 1e-3;
+            st.ms_pcg_iteration = ts.graph_iterations ? ts.ms_pcg_graph / (double)ts.graph_iterations : 0.0;
+            st.precond_levels = ts.amg_levels;
+            st.precond_fallbacks = ts.amg_fallbacks;
+        }
         E.read_weights_all(weights_out);
         if (masses_out) E.read_masses_all(masses_out);
         int nconv = 0, maxit = 0;
@@ -873,15 +1360,7 @@ sdot_status sdot_batch_solve(sdot_batch *B, double tol, int stop_rule, int max_i
         }
         st.newton_iters_max = maxit;
         st.nb_converged = nconv;
-        st.pcg_iters_total = ts.pcg_iters;
-        st.backtracks = ts.backtracks;
-        st.graph_replays = ts.graph_replays;
         st.max_rel_mass_err = maxerr;
-        st.seconds_cells = ts.ms_pass * 1e-3;
-        st.seconds_solve = ts.ms_pcg * 1e-3;
-        st.ms_pcg_iteration = ts.graph_iterations ? ts.ms_pcg_graph / (double)ts.graph_iterations : 0.0;
-        st.precond_levels = ts.amg_levels;
-        st.precond_fallbacks = ts.amg_fallbacks;
         st.seconds_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (stats) *stats = st;
         return nconv == K ? SDOT_OK : SDOT_NOT_CONVERGED;

@@ -39,6 +39,13 @@ struct TickParams {
     int max_iter = 150;
 };
 
+struct DeviceSolveStats {
+    int64_t replays = 0;            // graph replays
+    double ms_device = 0;           // device time from the first replay to the last
+    int pcg_iters = 0, backtracks = 0, widened = 0, fallbacks = 0;
+    int64_t cells = 0;              // cells evaluated (problem size x passes)
+};
+
 struct TickStats {
     double ms_pass = 0, ms_pcg = 0, ms_pcg_graph = 0;
     int graph_replays = 0, pcg_iters = 0, backtracks = 0, solved = 0, amg_levels = 0, amg_fallbacks = 0;
@@ -67,6 +74,16 @@ public:
     int tick(const TickParams &tp, std::vector<int> &finished, TickStats &ts);
     // one cell pass at the current weights (autotuning); device ms
     double pass_only();
+    // Newton on every problem from its start weights, driven from the device: one
+    // captured graph replayed until every problem is done (cg_per_replay CG
+    // iterations per replay); fills probs[k]'s results
+    int solve_device(const TickParams &tp, int cg_per_replay, DeviceSolveStats &ds);
+    // scheduling of the cell passes: cells a wavefront grabs at a time (multiple of
+    // 64) and the capacity tier set (0 default, 1 alternative first slot size)
+    void set_schedule(int cells_per_wave, int tierset);
+    int cells_per_wave() const;
+    int tierset() const;
+    int precond_levels() const;                      // multigrid levels (0: Jacobi)
 
     void read_weights(int k, double *out);          // dirac order of problem k
     void read_weights_all(double *out);
@@ -87,6 +104,9 @@ private:
     int dpc_x100_;
     bool use_graph_ = true;
     void rebuild_grid();
+    void enqueue_replay(const TickParams &tp, int cg_per_replay);
+    void capture_replay(const TickParams &tp, int cg_per_replay);
+    void widen_rows();
 };
 
 GridInfo problem_grid(const double *pos, int64_t n, int dpc_x100, double *tie);

@@ -372,6 +372,7 @@ sdot_status sdot_pd_destroy(sdot_pd *pd) {
     pd->row_len.release(); pd->mom.release(); pd->ell_col.release(); pd->ovf_list.release(); pd->flags.release(); pd->wctr.release();
     pd->csr_off.release(); pd->csr_col.release(); pd->csr_cnt.release(); pd->csr_tmp.release();
     pd->csr_val.release(); pd->out_tmp.release(); pd->scratch3.release();
+    pd->csr_off2.release(); pd->csr_col2.release(); pd->csr_val2.release();
     for (auto &b : pd->sv) b.release();
     pd->red.release();
     delete pd->pcg;

@@ -32,10 +32,53 @@ namespace {
 
 using namespace c2;
 
+// Persistent work-list traversal of a batch pass (CellArgs::wl_prob): the state of
+// one wavefront, uniform over its lanes.  next() moves to the next 64 cells of the
+// current grab, grabbing cpw more cells when it is used up (first from the
+// wavefront's own segment, then from the others); false when every segment is done.
+struct WaveGrab {
+    int64_t cb = 0, ce = 0, j = -64, seg = 0, total = 0;
+    int sx = 0, x0 = 0;
+    __device__ bool next(const CellArgs &a, int lane) {
+        j += 64;
+        while (cb + j >= ce) {
+            if (sx >= 8) return false;
+            const int x = (x0 + sx) & 7;
+            const int64_t s0 = x * seg, s1 = s0 + seg < total ? s0 + seg : total;
+            unsigned long long got = 0;
+            if (lane == 0) got = atomicAdd(a.grab + x, (unsigned long long)a.cpw);
+            got = __shfl(got, 0);
+            const int64_t base = s0 + (int64_t)got;
+            if (base >= s1) {
+                ++sx;
+                continue;
+            }
+            cb = base;
+            ce = base + a.cpw < s1 ? base + a.cpw : s1;
+            j = 0;
+        }
+        return true;
+    }
+    // sorted index of the lane's cell in the current 64 (-1: none)
+    __device__ int64_t cell(const CellArgs &a, int lane) const {
+        const int64_t k = cb + j + lane;
+        if (k >= ce) return -1;
+        int64_t lo = 0, hi = a.wl_meta[0];   // wl_off[lo] <= k < wl_off[hi]
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (a.wl_off[mid] <= k) lo = mid;
+            else hi = mid;
+        }
+        return a.pb_off[a.wl_prob[lo]] + (k - a.wl_off[lo]);
+    }
+};
+
 // One thread per cell, polygon in the thread's LDS slot (CAP vertices).  Without
 // a work list: every cell, in grid order.  With one: the cells listed, count in
-// device memory, grid-stride.  A cell that outgrows CAP goes to ovf_list for the
-// next tier; in the last tier (no ovf_list) it raises FLAG_CELL_OVERFLOW.
+// device memory, grid-stride.  With the persistent list of a batch (BATCH,
+// wl_prob): a grid of resident wavefronts that grab runs of cells.  A cell that
+// outgrows CAP goes to ovf_list for the next tier; in the last tier (no ovf_list)
+// it raises FLAG_CELL_OVERFLOW.
 // WAVES: the wavefronts per SIMD the register allocation must allow.
 template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER, bool JAC = false, int RF = 0>
 __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
@@ -44,7 +87,17 @@ __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
     // one call site of the cell body (so it is inlined): without a work list the
     // loop runs once
     int64_t k, count, step;
-    if (a.work) {
+    const bool grab = BATCH && a.wl_prob != nullptr && a.work == nullptr;   // first tier of a batch pass
+    WaveGrab wg;
+    const int lane = threadIdx.x & 63;
+    if (grab) {
+        k = 0;
+        count = 1;
+        step = 0;
+        wg.total = a.wl_meta[1];
+        wg.seg = (wg.total + 7) / 8;
+        wg.x0 = blockIdx.x & 7;
+    } else if (a.work) {
         k = blockIdx.x * (int64_t)BS + threadIdx.x;
         count = *a.work_count;
         step = (int64_t)gridDim.x * BS;
@@ -54,9 +107,16 @@ __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
         step = a.N;
     }
     for (; k < count; k += step) {
-        const int64_t t = a.work ? a.work[k] : k;
-        if (cell2_body<T, CAP, BALL, IMG, BATCH, PER, JAC, RF>(a, t, sxy + 2 * threadIdx.x, sxy + 2 * threadIdx.x + 1,
-                                                              sid + threadIdx.x, 2 * BS, BS)) {
+        int64_t t;
+        if (grab) {
+            if (!wg.next(a, lane)) break;   // uniform over the wavefront
+            t = wg.cell(a, lane);
+        } else {
+            t = a.work ? a.work[k] : k;
+        }
+        if (t >= 0 && cell2_body<T, CAP, BALL, IMG, BATCH, PER, JAC, RF>(a, t, sxy + 2 * threadIdx.x,
+                                                                        sxy + 2 * threadIdx.x + 1, sid + threadIdx.x,
+                                                                        2 * BS, BS)) {
             if (a.ovf_list) {
                 const unsigned q = atomicAdd(a.ovf_count, 1u);
                 a.ovf_list[q] = (int)t;
@@ -103,7 +163,7 @@ __global__ void k_csr_count(const int *row_len, const int *inv_perm, int64_t n, 
 template <int R>
 __global__ __launch_bounds__(256) void k_csr_fill(const int *row_len, const int *ell_col, const double *ell_val,
                                                   int64_t stride, const double *diag, const int *perm, int64_t n,
-                                                  const int64_t *off, int64_t *cols, double *vals) {
+                                                  const int64_t *off, int64_t *cols, double *vals, int64_t *mlen) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int64_t i = perm[t];
@@ -150,6 +210,7 @@ __global__ __launch_bounds__(256) void k_csr_fill(const int *row_len, const int 
             cols[o + k] = -1;
             vals[o + k] = 0;
         }
+        if (mlen) mlen[i] = w;
         return;
     }
     if (R > 16 && m <= R) {
@@ -197,6 +258,7 @@ __global__ __launch_bounds__(256) void k_csr_fill(const int *row_len, const int 
             cols[o + k] = -1;
             vals[o + k] = 0;
         }
+        if (mlen) mlen[i] = w;
         return;
     }
     // long row: in place
@@ -230,13 +292,41 @@ __global__ __launch_bounds__(256) void k_csr_fill(const int *row_len, const int 
         c[k] = -1;
         v[k] = 0;
     }
+    if (mlen) mlen[i] = w;
+}
+
+// merged rows moved to their final offsets (rows whose duplicates merged leave
+// trailing padding in the fill's layout), one row per thread
+__global__ __launch_bounds__(256) void k_csr_squeeze(const int64_t *off, const int64_t *cols, const double *vals,
+                                                     const int64_t *off2, int64_t n, int64_t *cols2, double *vals2) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = off[i], o2 = off2[i], m = off2[i + 1] - o2;
+    for (int64_t k = 0; k < m; ++k) {
+        cols2[o2 + k] = cols[o + k];
+        vals2[o2 + k] = vals[o + k];
+    }
+}
+
+// workgroups of one kernel that the device holds at once (the grid of a persistent launch)
+int resident_grid(const void *kernel, int block) {
+    int dev = 0, cus = 0, per_cu = 0;
+    HIPCK(hipGetDevice(&dev));
+    HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
+    return std::max(1, cus * std::max(1, per_cu));
 }
 
 template <class T, int CAP, int BS, bool BALL, bool IMG, int WAVES, bool BATCH, bool PER = false, bool JAC = false,
           int RF = 0>
 void launch2(const CellArgs &a, unsigned nb, hipStream_t s) {
     if (nb == 0) return;
-    hipLaunchKernelGGL((k_cells2<T, CAP, BS, BALL, IMG, WAVES, BATCH, PER, JAC, RF>), dim3(nb), dim3(BS), 0, s, a);
+    auto kern = k_cells2<T, CAP, BS, BALL, IMG, WAVES, BATCH, PER, JAC, RF>;
+    if (BATCH && a.wl_prob && !a.work) {
+        static const int resident = resident_grid(reinterpret_cast<const void *>(kern), BS);   // per instantiation
+        nb = (unsigned)resident;
+    }
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(BS), 0, s, a);
     HIPCK(hipGetLastError());
 }
 
@@ -244,6 +334,11 @@ void launch2(const CellArgs &a, unsigned nb, hipStream_t s) {
 // several domain items) clips each piece by the disc
 template <class T, int CAP, int WAVES>
 void dispatch2(bool ball, bool img, const CellArgs &a, unsigned nb, hipStream_t s) {
+    if constexpr (std::is_same<T, float>::value && CAP == 8) {   // a batch-only tier
+        if (ball) launch2<T, CAP, 64, true, false, WAVES, true>(a, nb, s);
+        else launch2<T, CAP, 64, false, false, WAVES, true>(a, nb, s);
+        return;
+    }
     if (a.prob) {
         if (ball) launch2<T, CAP, 64, true, false, WAVES, true>(a, nb, s);
         else launch2<T, CAP, 64, false, false, WAVES, true>(a, nb, s);
@@ -323,18 +418,24 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
         }
     } else if (mixed) {
         unsigned nb = next(false);
+        if (a.prob && pd->tierset == 1) {   // batch option: 8-vertex FP32 slots first
+            dispatch2<float, 8, 4>(ball, img, a, nb, s);
+            nb = next(false);
+        }
         dispatch2<float, 16, 3>(ball, img, a, nb, s);
         nb = next(true);
         dispatch2<float, 64, 1>(ball, img, a, nb, s);
     } else {
         unsigned nb = next(false);
-        dispatch2<double, 8, 4>(ball, img, a, nb, s);
-        nb = next(false);
+        if (!(a.prob && pd->tierset == 1)) {   // batch option: start at 16-vertex slots
+            dispatch2<double, 8, 4>(ball, img, a, nb, s);
+            nb = next(false);
+        }
         dispatch2<double, 16, 2>(ball, img, a, nb, s);
         nb = next(true);
         dispatch2<double, 64, 1>(ball, img, a, nb, s);
     }
-    if (pd->ev[3]) HIPCK(hipEventRecord(pd->ev[3], s));
+    if (pd->ev[3] && !pd->capturing) HIPCK(hipEventRecord(pd->ev[3], s));
     a.work = nullptr;
     a.work_count = nullptr;
 }
@@ -540,14 +641,32 @@ int64_t build_csr_device(sdot_pd *pd) {
     const int64_t bound = (int64_t)N * (pd->ell_cap + 1);
     pd->csr_col.ensure(bound);
     pd->csr_val.ensure(bound);
+    // Only periodic images put one dirac twice in a row (a face with j and one with an
+    // image of j; without translations each neighbour bounds at most one face): then
+    // the fill records each row's merged length and the rows are squeezed to their
+    // final offsets, so the returned rows never carry padding.
+    const bool dup = pd->htrans.size() > 0;
+    int64_t *mlen = dup ? pd->csr_cnt.p : nullptr;   // the counts are consumed by the scan above
     // register rows of 16 entries in 2D (about 7 on average), 32 in 3D (about 16)
     if (pd->dim == 2)
         hipLaunchKernelGGL((k_csr_fill<16>), dim3(nb), dim3(256), 0, s, pd->row_len.p, pd->ell_col.p, pd->ell_val.p,
-                           (int64_t)N, pd->diag.p, pd->perm.p, N, pd->csr_off.p, pd->csr_col.p, pd->csr_val.p);
+                           (int64_t)N, pd->diag.p, pd->perm.p, N, pd->csr_off.p, pd->csr_col.p, pd->csr_val.p, mlen);
     else
         hipLaunchKernelGGL((k_csr_fill<32>), dim3(nb), dim3(256), 0, s, pd->row_len.p, pd->ell_col.p, pd->ell_val.p,
-                           (int64_t)N, pd->diag.p, pd->perm.p, N, pd->csr_off.p, pd->csr_col.p, pd->csr_val.p);
+                           (int64_t)N, pd->diag.p, pd->perm.p, N, pd->csr_off.p, pd->csr_col.p, pd->csr_val.p, mlen);
     HIPCK(hipGetLastError());
+    if (dup) {
+        pd->csr_off2.ensure(N + 1);
+        exclusive_scan<int64_t>(pd->csr_cnt.p, pd->csr_off2.p, N, pd->csr_tmp.p, s);
+        pd->csr_col2.ensure(bound);
+        pd->csr_val2.ensure(bound);
+        hipLaunchKernelGGL(k_csr_squeeze, dim3(nb), dim3(256), 0, s, pd->csr_off.p, pd->csr_col.p, pd->csr_val.p,
+                           pd->csr_off2.p, N, pd->csr_col2.p, pd->csr_val2.p);
+        HIPCK(hipGetLastError());
+        std::swap(pd->csr_off, pd->csr_off2);
+        std::swap(pd->csr_col, pd->csr_col2);
+        std::swap(pd->csr_val, pd->csr_val2);
+    }
     return bound;
 }
 
@@ -558,31 +677,13 @@ void export_csr(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, int6
     int64_t total = 0;
     HIPCK(hipMemcpyAsync(&total, pd->csr_off.p + N, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
-    if (!h_col) {
-        *nnz = total;
-        return;
-    }
-    std::vector<int64_t> off(N + 1);
-    std::vector<int64_t> col(total);
-    std::vector<double> val(total);
-    HIPCK(hipMemcpyAsync(off.data(), pd->csr_off.p, (N + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    HIPCK(hipMemcpyAsync(col.data(), pd->csr_col.p, total * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    HIPCK(hipMemcpyAsync(val.data(), pd->csr_val.p, total * sizeof(double), hipMemcpyDeviceToHost, s));
+    *nnz = total;
+    if (!h_col) return;
+    if (total > cap) throw std::string("CSR capacity too small");
+    HIPCK(hipMemcpyAsync(h_off, pd->csr_off.p, (N + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(h_col, pd->csr_col.p, total * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(h_val, pd->csr_val.p, total * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
-    // rows whose duplicate columns were merged carry -1 padding: squeeze it out
-    int64_t w = 0;
-    for (int64_t i = 0; i < N; ++i) {
-        h_off[i] = w;
-        for (int64_t k = off[i]; k < off[i + 1]; ++k) {
-            if (col[k] < 0) continue;
-            if (w >= cap) throw std::string("CSR capacity too small");
-            h_col[w] = col[k];
-            h_val[w] = val[k];
-            ++w;
-        }
-    }
-    h_off[N] = w;
-    *nnz = w;
 }
 
 }  // namespace sdot_amd

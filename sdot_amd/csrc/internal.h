@@ -244,6 +244,19 @@ struct CellArgs {
     unsigned *pflags;
     const int *pdone;
     const int *pstop;   // batch: problems whose pass may stop at an empty cell (null: none)
+    // persistent work list of a batch pass (first capacity tier; null: every cell,
+    // one thread each): the cells of the problems wl_prob[j], j < wl_meta[0], problem
+    // wl_prob[j]'s cells numbered from wl_off[j] (wl_meta[1] in all, wl_off has
+    // wl_meta[0] + 1 entries); pb_off: the problems' row offsets.  The numbering is
+    // cut into 8 segments; a wavefront grabs cpw consecutive cells at a time with
+    // grab[x] (cells taken from segment x), starting in the segment of its XCD
+    // (workgroups go round-robin over the 8 XCDs) and then helping the others.
+    const int64_t *wl_off;
+    const int *wl_prob;
+    const int64_t *wl_meta;
+    const int64_t *pb_off;
+    unsigned long long *grab;
+    int cpw;
 };
 
 // true when the pass may stop: stop_if_void is set and an empty cell has been
@@ -394,8 +407,8 @@ struct sdot_pd {
     sdot_amd::CellArgs hargs{};
 
     // CSR export scratch
-    sdot_amd::DevBuf<int64_t> csr_off, csr_col, csr_cnt, csr_tmp;
-    sdot_amd::DevBuf<double> csr_val, out_tmp;
+    sdot_amd::DevBuf<int64_t> csr_off, csr_col, csr_cnt, csr_tmp, csr_off2, csr_col2;
+    sdot_amd::DevBuf<double> csr_val, out_tmp, csr_val2;
 
     // solver scratch
     sdot_amd::DevBuf<double> sv[10];
@@ -405,6 +418,12 @@ struct sdot_pd {
     int precond = SDOT_PRECOND_AMG;
     int use_graph = 1;                    // replay the PCG iterations from a captured graph
     int last_precond_levels = 0;          // multigrid levels of the last solve (0: Jacobi)
+
+    // batches: capacity tiers of the 2D pass (0: the default sequence; 1: FP64 starts at
+    // 16-vertex slots, mixed adds an 8-vertex first tier); the stream is being captured
+    // into a graph (no event records)
+    int tierset = 0;
+    bool capturing = false;
 
     // timing
     hipEvent_t ev[8] = {};

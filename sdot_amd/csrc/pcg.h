@@ -21,6 +21,14 @@
 
 namespace sdot_amd {
 
+// per-problem solver state, S[p * S_NS + field] (FP64): the CG scalars, the done
+// mark and iteration count, the squared tolerance, and the start mark (1: the
+// problem's solve starts at the next setup; set by the host in solve(), by the
+// caller's own kernels when the solve is driven from the device)
+namespace pcg_state {
+enum { S_RZ = 0, S_BB, S_ALPHA, S_BETA, S_RR, S_DONE, S_IT, S_RTOL2, S_START, S_NS };
+}
+
 struct PcgMat {                 // rows in grid order
     const double *diag;
     const int *len;
@@ -56,7 +64,20 @@ public:
     void solve(const PcgMat &A, const double *b, double *x, const double *d_err, const std::vector<int> &active,
                int max_iter, std::vector<int> &iters, PcgStats &st);
 
+    // Device-driven use (every call only enqueues kernels on the layout's stream, so a
+    // caller may capture it in a graph): the caller's kernels mark in state() the
+    // problems that start a solve (S_START = 1, S_DONE = 0, S_IT = 0, their
+    // fallback() entry 0) and clear S_START for every other problem.
+    // enqueue_start builds the hierarchy and initialises the iterates of the
+    // starting problems (the others keep theirs); enqueue_iterations runs n CG
+    // iterations of every problem whose S_DONE is 0.
+    void enqueue_start(const PcgMat &A, const double *b, double *x, const double *d_err);
+    void enqueue_iterations(const PcgMat &A, double *x, int n, int max_iter);
+    double *state();
+    int *fallback();
+
     bool amg() const { return use_amg_; }
+    int levels() const;         // multigrid levels of the layout (0: Jacobi)
     struct Impl;
 
 private:

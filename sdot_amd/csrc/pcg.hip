@@ -10,6 +10,8 @@
 
 namespace sdot_amd {
 
+using namespace pcg_state;
+
 namespace {
 
 constexpr int BB = 256;          // rows per block
@@ -20,7 +22,6 @@ constexpr int CCAP = 64;         // off-diagonal entries per coarse row
 constexpr int CBS = 64;          // threads per block of the coarsening kernel
 constexpr double OMEGA = 0.8;    // damped-Jacobi smoothing
 constexpr double CSCALE = 1.5;   // coarse-correction scaling
-enum { S_RZ = 0, S_BB, S_ALPHA, S_BETA, S_RR, S_DONE, S_IT, S_RTOL2, S_NS };
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(BB) void kp_owner(const int64_t *off, int64_t K, in
 // mapped to aggregates, duplicates merged in a fixed order (one thread per row).
 __global__ __launch_bounds__(CBS) void kp_coarsen(PcgMat A, const int64_t *offl, const int64_t *offc,
                                                   const int *cprob, int64_t nc, double *cdiag, int *clen, int *ccol,
-                                                  double *cval, int *amg_off) {
+                                                  double *cval, int *amg_off, const double *S) {
     __shared__ int sc[CCAP * CBS];
     __shared__ double sv[CCAP * CBS];
     const int64_t I = blockIdx.x * (int64_t)CBS + threadIdx.x;
@@ -78,6 +79,7 @@ __global__ __launch_bounds__(CBS) void kp_coarsen(PcgMat A, const int64_t *offl,
     int *uc = sc + threadIdx.x;
     double *uv = sv + threadIdx.x;
     const int p = cprob[I];
+    if (S[p * S_NS + S_START] == 0) return;   // only the problems that start a solve
     const int64_t li = I - offc[p];
     const int64_t f0 = offl[p] + AGG * li;
     const int64_t f1 = f0 + AGG < offl[p + 1] ? f0 + AGG : offl[p + 1];
@@ -120,11 +122,11 @@ __global__ __launch_bounds__(CBS) void kp_coarsen(PcgMat A, const int64_t *offl,
 // dense inverse of each problem's coarsest matrix (at most CMIN rows), in-place
 // Gauss-Jordan without pivoting (the matrix is SPD); one block per problem
 __global__ __launch_bounds__(CMIN) void kp_coarse_inv(PcgMat A, const int64_t *coff, const int *cn, const int *plev,
-                                                      int lev, double *inv, int *amg_off) {
+                                                      int lev, double *inv, int *amg_off, const double *S) {
     __shared__ double M[CMIN][CMIN + 1];
     __shared__ int bad;
     const int p = blockIdx.x, i = threadIdx.x, n = cn[p];
-    if (plev[p] != lev || amg_off[p]) return;
+    if (plev[p] != lev || amg_off[p] || S[p * S_NS + S_START] == 0) return;
     const int64_t o = coff[p];
     for (int j = 0; j < CMIN; ++j) M[i][j] = 0;
     if (i == 0) bad = 0;
@@ -180,18 +182,23 @@ struct LevelView {
     int lev;
 };
 
-__device__ inline bool skip(const double *S, const int *amg_off, int p) {
-    return S[p * S_NS + S_DONE] != 0 || amg_off[p];
+// gate 0: the iterations (problems still iterating); gate 1: the start of a solve
+// (problems whose solve starts now; the others keep their iterates)
+__device__ inline bool gated_out(const double *S, int p, int gate) {
+    return gate ? S[p * S_NS + S_START] == 0 : S[p * S_NS + S_DONE] != 0;
+}
+__device__ inline bool skip(const double *S, const int *amg_off, int p, int gate) {
+    return gated_out(S, p, gate) || amg_off[p];
 }
 
 // pre-smooth level l from zero (x = w r / D) and its residual r - A x, one thread per
 // row (rows of problems that have a coarser level)
 __global__ __launch_bounds__(BB) void kp_smooth_res(LevelView F, const int *plev, double *res, const double *S,
-                                                    const int *amg_off) {
+                                                    const int *amg_off, int gate) {
     const int64_t t = blockIdx.x * (int64_t)BB + threadIdx.x;
     if (t >= F.n) return;
     const int p = F.prob[t];
-    if (skip(S, amg_off, p) || plev[p] <= F.lev) return;
+    if (skip(S, amg_off, p, gate) || plev[p] <= F.lev) return;
     const double rt = F.r[t], dt = F.A.diag[t];
     const double xt = OMEGA * rt / dt;
     F.xpre[t] = xt;
@@ -206,11 +213,11 @@ __global__ __launch_bounds__(BB) void kp_smooth_res(LevelView F, const int *plev
 
 // restriction: coarse row I = sum of the residuals of its aggregate, in row order
 __global__ __launch_bounds__(BB) void kp_restrict(LevelView F, LevelView C, const double *res, const double *S,
-                                                  const int *amg_off) {
+                                                  const int *amg_off, int gate) {
     const int64_t I = blockIdx.x * (int64_t)BB + threadIdx.x;
     if (I >= C.n) return;
     const int p = C.prob[I];
-    if (skip(S, amg_off, p)) return;
+    if (skip(S, amg_off, p, gate)) return;
     const int64_t f0 = F.off[p] + AGG * (I - C.off[p]);
     const int64_t f1 = f0 + AGG < F.off[p + 1] ? f0 + AGG : F.off[p + 1];
     double sum = 0;
@@ -222,10 +229,10 @@ __global__ __launch_bounds__(BB) void kp_restrict(LevelView F, LevelView C, cons
 __global__ __launch_bounds__(CMIN) void kp_coarse_solve(const int64_t *cbase, const int *cn, const int *clev,
                                                         const double *inv, const double *vr, double *vout,
                                                         const double *r0, double *z0, const double *S,
-                                                        const int *amg_off) {
+                                                        const int *amg_off, int gate) {
     __shared__ double rs[CMIN];
     const int p = blockIdx.x, i = threadIdx.x, n = cn[p];
-    if (skip(S, amg_off, p)) return;
+    if (skip(S, amg_off, p, gate)) return;
     const bool top = clev[p] == 0;
     const double *r = top ? r0 : vr;
     double *out = top ? z0 : vout;
@@ -242,11 +249,11 @@ __global__ __launch_bounds__(CMIN) void kp_coarse_solve(const int64_t *cbase, co
 
 // prolong the coarse correction, post-smooth (levels >= 1)
 __global__ __launch_bounds__(BB) void kp_up(LevelView F, LevelView C, const int *plev, const double *S,
-                                            const int *amg_off) {
+                                            const int *amg_off, int gate) {
     const int64_t t = blockIdx.x * (int64_t)BB + threadIdx.x;
     if (t >= F.n) return;
     const int p = F.prob[t];
-    if (skip(S, amg_off, p) || plev[p] == F.lev) return;   // coarsest of p: set by the coarse solve
+    if (skip(S, amg_off, p, gate) || plev[p] == F.lev) return;   // coarsest of p: set by the coarse solve
     const int64_t fo = F.off[p], co = C.off[p];
     const double y = F.xpre[t] + CSCALE * C.out[co + (t - fo) / AGG];
     double ay = F.A.diag[t] * y;
@@ -265,7 +272,7 @@ __global__ __launch_bounds__(BB) void kp_up0(LevelView F, LevelView C, const int
     __shared__ double sm[BB];
     const BlockRows br = block_rows(B, blockIdx.x);
     const int p = br.p;
-    if (S[p * S_NS + S_DONE] != 0) return;   // uniform per block
+    if (gated_out(S, p, init)) return;   // uniform per block
     const bool jac = jacobi || amg_off[p];
     const bool coarse = !jac && plev[p] == 0;
     double rz = 0;
@@ -301,7 +308,7 @@ __global__ __launch_bounds__(BB) void kp_init(Blocks B, const double *b, double 
                                               const double *S) {
     __shared__ double sm[BB];
     const BlockRows br = block_rows(B, blockIdx.x);
-    if (S[br.p * S_NS + S_DONE] != 0) return;
+    if (gated_out(S, br.p, 1)) return;
     double bb = 0;
     for (int64_t t = br.r0 + threadIdx.x; t < br.r1; t += BB) {
         const double rv = b[t];
@@ -318,7 +325,7 @@ __global__ __launch_bounds__(BB) void kp_init_red(const double *part, const int6
     __shared__ double sm[BB];
     const int p = blockIdx.x;
     double *sp = S + p * S_NS;
-    if (sp[S_DONE] != 0) return;
+    if (gated_out(S, p, 1)) return;
     double a = 0, c = 0;
     for (int64_t k = blk_off[p] + threadIdx.x; k < blk_off[p + 1]; k += BB) {
         a += part[2 * k];
@@ -601,20 +608,23 @@ LevelView level_view(PcgSolver::Impl &I, const PcgMat &A0, int l, double *r0, do
 void apply_prec(PcgSolver::Impl &I, const PcgMat &A0, bool amg, bool init) {
     hipStream_t s = I.s;
     const int L = amg ? I.L : 0;
+    const int gate = init ? 1 : 0;   // the first application runs for the problems that start
     for (int l = 0; l < L; ++l) {
         LevelView F = level_view(I, A0, l, I.r.p, I.z.p), C = level_view(I, A0, l + 1, I.r.p, I.z.p);
         if (C.n == 0) continue;
         hipLaunchKernelGGL(kp_smooth_res, dim3(nblocks(F.n, BB)), dim3(BB), 0, s, F, I.dplev.p, I.res.p, I.S.p,
-                           I.amg_off.p);
-        hipLaunchKernelGGL(kp_restrict, dim3(nblocks(C.n, BB)), dim3(BB), 0, s, F, C, I.res.p, I.S.p, I.amg_off.p);
+                           I.amg_off.p, gate);
+        hipLaunchKernelGGL(kp_restrict, dim3(nblocks(C.n, BB)), dim3(BB), 0, s, F, C, I.res.p, I.S.p, I.amg_off.p,
+                           gate);
     }
     if (amg)
         hipLaunchKernelGGL(kp_coarse_solve, dim3((unsigned)I.K), dim3(CMIN), 0, s, I.dcbase.p, I.dcn.p, I.dplev.p,
-                           I.inv.p, I.vr.p, I.vout.p, I.r.p, I.z.p, I.S.p, I.amg_off.p);
+                           I.inv.p, I.vr.p, I.vout.p, I.r.p, I.z.p, I.S.p, I.amg_off.p, gate);
     for (int l = L - 1; l >= 1; --l) {
         LevelView F = level_view(I, A0, l, I.r.p, I.z.p), C = level_view(I, A0, l + 1, I.r.p, I.z.p);
         if (F.n > 0)
-            hipLaunchKernelGGL(kp_up, dim3(nblocks(F.n, BB)), dim3(BB), 0, s, F, C, I.dplev.p, I.S.p, I.amg_off.p);
+            hipLaunchKernelGGL(kp_up, dim3(nblocks(F.n, BB)), dim3(BB), 0, s, F, C, I.dplev.p, I.S.p, I.amg_off.p,
+                               gate);
     }
     LevelView F = level_view(I, A0, 0, I.r.p, I.z.p);
     LevelView C = L >= 1 ? level_view(I, A0, 1, I.r.p, I.z.p) : F;
@@ -636,43 +646,60 @@ void cg_iteration(PcgSolver::Impl &I, const PcgMat &A, double *x, bool amg, int 
 
 }  // namespace
 
-void PcgSolver::solve(const PcgMat &A, const double *b, double *x, const double *d_err, const std::vector<int> &active,
-                      int max_iter, std::vector<int> &iters, PcgStats &st) {
+// hierarchy of the starting problems' matrices, then their first iterates
+void PcgSolver::enqueue_start(const PcgMat &A, const double *b, double *x, const double *d_err) {
     Impl &I = *im;
     hipStream_t s = I.s;
     const int64_t K = I.K;
-    std::vector<double> S(K * S_NS, 0.0);
-    for (int64_t k = 0; k < K; ++k) S[k * S_NS + S_DONE] = active[k] ? 0.0 : 1.0;
-    HIPCK(hipMemcpyAsync(I.S.p, S.data(), S.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    fill_u32(reinterpret_cast<uint32_t *>(I.amg_off.p), K, 0u, s);
     const bool amg = use_amg_ && I.L > 0;
-    // hierarchy of this matrix
-    const auto t0 = std::chrono::steady_clock::now();
     if (amg) {
         for (int l = 0; l < I.L; ++l) {
             LevelView F = level_view(I, A, l, nullptr, nullptr);
             Impl::Level &V = I.lv[l + 1];
             if (V.n > 0)
                 hipLaunchKernelGGL(kp_coarsen, dim3(nblocks(V.n, CBS)), dim3(CBS), 0, s, F.A, F.off, V.doff.p, V.prob.p,
-                                   V.n, V.diag.p, V.len.p, V.col.p, V.val.p, I.amg_off.p);
+                                   V.n, V.diag.p, V.len.p, V.col.p, V.val.p, I.amg_off.p, I.S.p);
         }
         // dense inverses: each problem's coarsest matrix lives at its own level
-        std::vector<int> any(I.L + 1, 0);
-        for (int64_t k = 0; k < K; ++k) any[I.plev[k]] = 1;
-        for (int l = 0; l <= I.L; ++l)
-            if (any[l]) {
-                LevelView F = level_view(I, A, l, nullptr, nullptr);
-                hipLaunchKernelGGL(kp_coarse_inv, dim3((unsigned)K), dim3(CMIN), 0, s, F.A, I.dcoff.p, I.dcn.p, I.dplev.p,
-                                   l, I.inv.p, I.amg_off.p);
-            }
-        HIPCK(hipGetLastError());
+        for (int l = 0; l <= I.L; ++l) {
+            LevelView F = level_view(I, A, l, nullptr, nullptr);
+            hipLaunchKernelGGL(kp_coarse_inv, dim3((unsigned)K), dim3(CMIN), 0, s, F.A, I.dcoff.p, I.dcn.p, I.dplev.p, l,
+                               I.inv.p, I.amg_off.p, I.S.p);
+        }
     }
-    st.ms_setup += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    st.amg_levels = std::max(st.amg_levels, amg ? I.L + 1 : 0);
     hipLaunchKernelGGL(kp_init, dim3((unsigned)I.nblk), dim3(BB), 0, s, I.blocks(), b, x, I.r.p, I.red.p, I.S.p);
     apply_prec(I, A, amg, true);
     hipLaunchKernelGGL(kp_init_red, dim3((unsigned)K), dim3(BB), 0, s, I.red.p, I.dblk.p, d_err, I.S.p);
     HIPCK(hipGetLastError());
+}
+
+void PcgSolver::enqueue_iterations(const PcgMat &A, double *x, int n, int max_iter) {
+    const bool amg = use_amg_ && im->L > 0;
+    for (int k = 0; k < n; ++k) cg_iteration(*im, A, x, amg, max_iter);
+    HIPCK(hipGetLastError());
+}
+
+double *PcgSolver::state() { return im->S.p; }
+int PcgSolver::levels() const { return use_amg_ && im->L > 0 ? im->L + 1 : 0; }
+int *PcgSolver::fallback() { return im->amg_off.p; }
+
+void PcgSolver::solve(const PcgMat &A, const double *b, double *x, const double *d_err, const std::vector<int> &active,
+                      int max_iter, std::vector<int> &iters, PcgStats &st) {
+    Impl &I = *im;
+    hipStream_t s = I.s;
+    const int64_t K = I.K;
+    std::vector<double> S(K * S_NS, 0.0);
+    for (int64_t k = 0; k < K; ++k) {
+        S[k * S_NS + S_DONE] = active[k] ? 0.0 : 1.0;
+        S[k * S_NS + S_START] = active[k] ? 1.0 : 0.0;
+    }
+    HIPCK(hipMemcpyAsync(I.S.p, S.data(), S.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    fill_u32(reinterpret_cast<uint32_t *>(I.amg_off.p), K, 0u, s);
+    const bool amg = use_amg_ && I.L > 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    enqueue_start(A, b, x, d_err);
+    st.ms_setup += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    st.amg_levels = std::max(st.amg_levels, amg ? I.L + 1 : 0);
     const bool same_args = I.graph_A.diag == A.diag && I.graph_A.len == A.len && I.graph_A.col == A.col &&
                            I.graph_A.val == A.val && I.graph_A.stride == A.stride && I.graph_x == x &&
                            I.graph_max_iter == max_iter && I.graph_amg == amg;

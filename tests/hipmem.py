@@ -65,3 +65,11 @@ def free_device_memory():
     fr, tot = ctypes.c_size_t(), ctypes.c_size_t()
     _ck(_hip.hipMemGetInfo(ctypes.byref(fr), ctypes.byref(tot)), "hipMemGetInfo")
     return fr.value
+
+
+def read_device(ptr, n, dtype=np.float64):
+    """Copy n elements of dtype from a device pointer the library returned."""
+    out = np.empty(int(n), dtype=dtype)
+    if out.nbytes:
+        _ck(_hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptr), out.nbytes, D2H), "hipMemcpy")
+    return out

@@ -93,6 +93,8 @@ def test_batch_stop_rule_dw_and_autotune():
     st, ws, ms, pst, its, stats = solve(pos, "fp64", dpc=0.0, tol=1e-6, stop_rule=1)
     assert st == 0 and (pst == 0).all()
     assert stats["tuned_dpc_x100"] in (150, 250, 400)
+    assert stats["tuned_cells_per_wave"] in (64, 128, 256) and stats["tuned_first_tier"] in (0, 1)
+    assert stats["tuning_passes"] in (0, 8) and stats["device_newton"] == 1   # 0: this size class was tuned before
     # same stop rule on the single-problem path: same number of Newton passes
     for k in (0, 5):
         n = len(pos[k])
@@ -139,3 +141,83 @@ def test_batch_multigrid_matches_jacobi_and_is_independent_of_the_mix():
     r1 = b.solve(tol=1e-9)
     assert np.array_equal(r1[1][0], wa[5]) and r1[4][0] == ia[5]
     print(f"[batch precond] amg {sa['pcg_iters_total']} ({sa['precond_levels']} levels) jacobi {sj['pcg_iters_total']}")
+
+
+@pytest.mark.parametrize("precision,ball", [("fp64", False), ("mixed", False), ("mixed", True)])
+def test_batch_device_newton_equals_host_iterations(precision, ball):
+    """The device-driven Newton (one captured graph replayed, a per-problem state on
+    the device) takes the steps of the host-driven one: bitwise the same weights,
+    masses, iteration and backtrack counts, eager or from the graph."""
+    import sdot_amd
+    pos = make_problems(9, 700, 300, 6000)
+    kw = {}
+    if ball:
+        kw = dict(targets=[np.full(len(p), 0.5 / len(p)) for p in pos],
+                  weights=[np.full(len(p), 0.6 / len(p) / np.pi) for p in pos], ball=True)
+    out = {}
+    for mode, dev, graph in (("host", False, True), ("device", True, True), ("device_eager", True, False)):
+        b = sdot_amd.Batch(precision, diracs_per_cell=2.5, use_graph=graph, device_newton=dev, cells_per_wave=128,
+                           first_tier=0, cg_per_replay=16)
+        b.set_problems(pos, **kw)
+        out[mode] = b.solve(tol=1e-8 if ball else 1e-6)
+    sh, wh, mh, ph, ih, th = out["host"]
+    assert sh == 0 and (ph == 0).all()
+    for mode in ("device", "device_eager"):
+        sd, wd, md, pdv, idv, td = out[mode]
+        assert sd == 0 and (pdv == 0).all() and td["device_newton"] == 1
+        assert td["newton_replays"] > 0 and td["pcg_iters_total"] == th["pcg_iters_total"], (mode, td, th)
+        assert td["backtracks"] == th["backtracks"]
+        for k in range(len(pos)):
+            assert np.array_equal(wd[k], wh[k]) and np.array_equal(md[k], mh[k]) and idv[k] == ih[k], (mode, k)
+    print(f"[batch device newton] {precision} ball={ball}: replays {out['device'][5]['newton_replays']} "
+          f"host {th['seconds_total']:.3f} s device {out['device'][5]['seconds_total']:.3f} s")
+
+
+def test_batch_schedule_knobs_do_not_change_results():
+    """Cells per wavefront grab and the capacity tier set change only the schedule:
+    every problem's weights and masses are bitwise the same."""
+    import sdot_amd
+    pos = make_problems(7, 800, 500, 8000)
+    ref = None
+    for precision in ("fp64", "mixed"):
+        ref = None
+        for cpw, tier in ((128, 0), (64, 1), (256, 0), (192, 1)):
+            b = sdot_amd.Batch(precision, diracs_per_cell=2.5, cells_per_wave=cpw, first_tier=tier)
+            b.set_problems(pos)
+            r = b.solve(tol=1e-6)
+            assert r[0] == 0 and r[5]["tuned_cells_per_wave"] == cpw and r[5]["tuned_first_tier"] == tier
+            if ref is None:
+                ref = r
+                continue
+            for k in range(len(pos)):
+                assert np.array_equal(r[1][k], ref[1][k]) and np.array_equal(r[2][k], ref[2][k]), (precision, cpw, k)
+
+
+def test_batch_device_newton_widens_rows():
+    """A cell with more faces than the row capacity (32) in the first pass: the
+    device-driven solve widens the rows in place and ends like the host-driven one."""
+    import sdot_amd
+    rng = np.random.default_rng(5)
+    ring = 48
+    ang = 2 * np.pi * np.arange(ring) / ring
+    u = rng.uniform(0, 1, (600, 2))
+    u = u[np.hypot(u[:, 0] - 0.5, u[:, 1] - 0.5) > 0.26]
+    star = np.concatenate([[[0.5, 0.5]], 0.5 + 0.2 * np.c_[np.cos(ang), np.sin(ang)], u])
+    pos = [star, make_problems(1, 900, 500, 600)[0]]
+    n0 = len(star)
+    w0 = [np.zeros(len(p)) for p in pos]
+    w0[0][0] = 0.03   # the centre's cell is a 48-gon: one face per ring dirac
+    t0 = np.full(n0, 0.91 / (n0 - 1))
+    t0[0] = 0.09      # and keeps that size
+    tg = [t0, np.full(len(pos[1]), 1.0 / len(pos[1]))]
+    assert O.cells(star, w0[0])["row_len"][0] > 32
+    out = {}
+    for dev in (False, True):
+        b = sdot_amd.Batch("fp64", diracs_per_cell=2.5, device_newton=dev, cells_per_wave=128, first_tier=0)
+        b.set_problems(pos, weights=w0, targets=tg)
+        out[dev] = b.solve(tol=1e-6)
+    assert out[False][0] == 0 and out[True][0] == 0
+    for k in range(2):
+        assert np.array_equal(out[True][1][k], out[False][1][k]) and out[True][4][k] == out[False][4][k], k
+    m = O.cells(star, out[True][1][0])["masses"]
+    assert np.abs(m / t0 - 1).max() < 1.5e-6
