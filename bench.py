@@ -35,9 +35,15 @@ sys.path.insert(0, ROOT)
 METRIC = "Laguerre cells/s (1M diracs 2D/3D); solve s @1e-6 mass err; p99 latency"
 PEAK = {"fp64": 78.6, "mixed": 157.3}        # MI355X vector FP64 / FP32, TFLOP/s
 # HBM bytes per pass of the cell kernels on the 1M ball workload (all capacity tiers): rocprofv3
-# FETCH_SIZE and WRITE_SIZE of k_cells2, each scaled by the same counter on a 1 GiB device copy
-# (bench.py --copy-calibration 1024: FETCH reads half the copied bytes, WRITE all of them)
-TRAFFIC = {"mixed": 184.9e6, "fp64": 636.1e6}
+# FETCH_SIZE and WRITE_SIZE of k_cells2, each scaled by the same counter on the library's copy
+# kernel moving 1 GiB (bench.py --copy-calibration 1024: FETCH reads half the copied bytes,
+# WRITE all of them, so FETCH x 2, WRITE x 1)
+TRAFFIC = {"mixed": 184.2e6, "fp64": 576.1e6}
+# VALU instructions (wavefront instructions, SQ_INSTS_VALU) per pass of the same kernels, and
+# the issue rate they are compared with: one wave64 VALU instruction per SIMD every 4 cycles,
+# 1024 SIMDs at 2.4 GHz
+VALU_INSTS = {"mixed": 4.286e8, "fp64": 4.203e8}
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 CPU_CORES = min(16, os.cpu_count() or 1)     # the host share of one GPU on the box
 
 
@@ -90,8 +96,9 @@ def run_gpu(pos, w, precision, steps, warmup):
 
 def copy_calibration(mib, reps=3):
     """Plain device-to-device copies of a known size (reads and writes mib MiB each)
-    for scaling FETCH_SIZE / WRITE_SIZE counters: hipMemcpy of the HIP runtime the
-    library links (one runtime in the process; the copy runs as a blit kernel)."""
+    for scaling FETCH_SIZE / WRITE_SIZE counters: the library's copy kernel
+    (k_copy16) on buffers of the HIP runtime the library links (one runtime in the
+    process)."""
     import ctypes
     import sdot_amd  # noqa: F401  (loads the library and its runtime first)
     hip = ctypes.CDLL("libamdhip64.so.7")
@@ -107,9 +114,10 @@ def copy_calibration(mib, reps=3):
     try:
         if hip.hipMemset(a, 1, nbytes) != 0:
             raise RuntimeError("hipMemset failed in the copy calibration")
+        lib = sdot_amd.lib()
         for _ in range(reps):
-            if hip.hipMemcpy(b, a, nbytes, 3) != 0:   # hipMemcpyDeviceToDevice
-                raise RuntimeError("hipMemcpy failed in the copy calibration")
+            if lib.sdot_calibration_copy(b, a, nbytes) != 0:
+                raise RuntimeError("copy kernel failed in the copy calibration")
         hip.hipDeviceSynchronize()
     finally:
         hip.hipFree(a)
@@ -195,6 +203,15 @@ def run_c3(n, img_size, steps, precisions=("fp64", "mixed")):
         stats["seconds"] = float(np.median(runs))
         stats["status"] = st
         stats["cells_per_s_pass"] = n * stats["newton_iters"] / stats["seconds_cells"] if stats["seconds_cells"] else None
+        stats["ms_pcg_iteration"] = 1e3 * stats["seconds_solve"] / max(1, stats["pcg_iters_total"])
+        # one more pass at the converged weights: the wave kernel against the vector peak
+        pd.set_weights(w)
+        pd.integrals()
+        tm = pd.last_timings()
+        ach = tm["cell_flops"] / (tm["ms_cell_kernel"] * 1e-3) / 1e12
+        stats["roofline_pass"] = {"bound": "valu", "kernel": "k_cells3_wave", "flops_per_launch": tm["cell_flops"],
+                                  "kernel_ms": tm["ms_cell_kernel"], "achieved": ach, "peak": PEAK[prec],
+                                  "unit": "TFLOP/s", "frac": ach / PEAK[prec]}
         out[prec] = stats
         out[prec + "_weights"] = w
         print(f"# c3 {prec}: {stats}", file=sys.stderr, flush=True)
@@ -226,9 +243,11 @@ def c3_leg(n=1_000_000, img=64):
     out = {"workload": f"3D Newton OT solve, N={n} uniform in [0,1]^3, {img}^3 lognormal(0,0.5) voxel density "
                        "(mass 1), w0=0, uniform targets, stop at max relative mass error 1e-6",
            "fp64": {k: f[k] for k in ("seconds", "status", "newton_iters", "pcg_iters_total", "backtracks",
-                                      "max_rel_mass_err", "seconds_cells", "seconds_solve", "cells_per_s_pass")},
+                                      "max_rel_mass_err", "seconds_cells", "seconds_solve", "cells_per_s_pass",
+                                      "ms_pcg_iteration", "roofline_pass")},
            "mixed": {k: m[k] for k in ("seconds", "status", "newton_iters", "pcg_iters_total", "backtracks",
-                                       "max_rel_mass_err", "seconds_cells", "seconds_solve", "cells_per_s_pass")},
+                                       "max_rel_mass_err", "seconds_cells", "seconds_solve", "cells_per_s_pass",
+                                       "ms_pcg_iteration", "roofline_pass")},
            "mixed_speedup_vs_fp64": f["seconds"] / m["seconds"],
            "cpu_baseline": cpu,
            "gpu_pass_vs_cpu_pass": {p: res[p]["cells_per_s_pass"] / cpu["value"] for p in ("fp64", "mixed")},
@@ -457,6 +476,8 @@ def main():
               "is not run: multi-GPU sharding is a design (ghost-layer exchange), not part of this build.",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     if args.workload == "c3":
         args.steps = min(args.steps, 3)
         return main_c3(args)
@@ -467,8 +488,6 @@ def main():
     if args.workload == "c5":
         return main_c5(args)
 
-    if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
     pos, w = make_inputs(args.n)
     res = {}
     masses = {}
@@ -510,7 +529,9 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK["fp64"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK["fp64"], "traffic": TRAFFIC["fp64"] if args.n == 1_000_000 else None,
                      "kernel": "k_cells2 (FP64 capacity tiers)", "flops_per_launch": b["cell_flops"],
-                     "kernel_ms": b["ms_cell_kernel"]},
+                     "kernel_ms": b["ms_cell_kernel"],
+                     "valu_issue_frac": VALU_INSTS["fp64"] / (b["ms_cell_kernel"] * 1e-3 * VALU_ISSUE_PER_S)
+                     if args.n == 1_000_000 else None},
         "cpu_baseline": cpu,
         "mixed": {"cells_per_s": res["mixed"]["cells_per_s"],
                   "speedup_vs_fp64": res["mixed"]["cells_per_s"] / b["cells_per_s"],
@@ -518,7 +539,10 @@ def main():
                   "max_mass_err_vs_oracle": res["mixed"].get("max_mass_err_vs_oracle"),
                   "roofline": {"achieved": res["mixed"]["achieved_tflops"], "peak": PEAK["mixed"],
                                "frac": res["mixed"]["achieved_tflops"] / PEAK["mixed"],
-                               "traffic": TRAFFIC["mixed"] if args.n == 1_000_000 else None}},
+                               "traffic": TRAFFIC["mixed"] if args.n == 1_000_000 else None,
+                               "valu_issue_frac": VALU_INSTS["mixed"] / (res["mixed"]["ms_cell_kernel"] * 1e-3 *
+                                                                         VALU_ISSUE_PER_S)
+                               if args.n == 1_000_000 else None}},
         "precisions": res,
         "host": {"cpu_model": cpu_model(), "cores_used": CPU_CORES},
     }

@@ -366,6 +366,12 @@ sdot_status sdot_srv_destroy(sdot_srv *srv);
 /* splitmix64 -> xoshiro256** uniform doubles (x >> 11) * 2^-53: the synthetic-input generator. */
 void sdot_rng_uniform(uint64_t seed, int64_t n, double *out);
 
+/* Measurement helper, no counterpart in the reference: copies nbytes (a positive
+ * multiple of 16) between two device buffers of the current device with a plain
+ * copy kernel (k_copy16) on the null stream and waits.  Its hardware counters
+ * (FETCH_SIZE, WRITE_SIZE) for a known size scale the counters of the other kernels. */
+sdot_status sdot_calibration_copy(void *dst, const void *src, int64_t nbytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,6 +136,7 @@ SIGNATURES = {
     "sdot_batch_set_tuning": (_int, [_p, _int, _int]),
     "sdot_batch_solve": (_int, [_p, _d, _int, _int, _p, _p, _p, _p, ctypes.POINTER(BatchStats)]),
     "sdot_batch_set_schedule": (_int, [_p, _int, _int, _int, _int]),
+    "sdot_calibration_copy": (_int, [_p, _p, _i64]),
     "sdot_srv_create": (_int, [ctypes.POINTER(_p), ctypes.POINTER(ServerConfig)]),
     "sdot_srv_submit": (_int, [_p, _int, _i64, _p, _p, _p, _p, ctypes.POINTER(_i64)]),
     "sdot_srv_poll": (_int, [_p, _i64, ctypes.POINTER(_int)]),

@@ -316,6 +316,13 @@ sdot_status ensure_pass(sdot_pd *pd) {
 
 }  // namespace
 
+// a plain copy of n 16-byte words (grid-stride): the reference kernel for scaling
+// the hardware HBM counters
+__global__ void k_copy16(uint4 *dst, const uint4 *src, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 extern "C" {
 
 const char *sdot_status_string(int st) {
@@ -788,6 +795,16 @@ sdot_status sdot_ot_solve(sdot_pd *pd, const double *targets, double tol, int st
         return SDOT_NOT_SUPPORTED;
     }
     return guard(pd, [&] { return ot_solve(pd, targets, tol, stop_rule, max_iter, weights_inout, stats); });
+}
+
+sdot_status sdot_calibration_copy(void *dst, const void *src, int64_t nbytes) {
+    if (!dst || !src || nbytes <= 0 || nbytes % 16) return SDOT_INVALID_ARG;
+    const int64_t n = nbytes / 16;
+    const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, nullptr, static_cast<uint4 *>(dst),
+                       static_cast<const uint4 *>(src), n);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return SDOT_HIP_ERROR;
+    return SDOT_OK;
 }
 
 // splitmix64 seeding xoshiro256** (Blackman & Vigna), u = (x >> 11) * 2^-53

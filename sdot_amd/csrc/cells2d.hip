@@ -114,15 +114,22 @@ __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
         } else {
             t = a.work ? a.work[k] : k;
         }
-        if (t >= 0 && cell2_body<T, CAP, BALL, IMG, BATCH, PER, JAC, RF>(a, t, sxy + 2 * threadIdx.x,
-                                                                        sxy + 2 * threadIdx.x + 1, sid + threadIdx.x,
-                                                                        2 * BS, BS)) {
-            if (a.ovf_list) {
-                const unsigned q = atomicAdd(a.ovf_count, 1u);
-                a.ovf_list[q] = (int)t;
-            } else {
-                flag_or(a.flags, FLAG_CELL_OVERFLOW);
+        const bool ovf = t >= 0 && cell2_body<T, CAP, BALL, IMG, BATCH, PER, JAC, RF>(
+                                       a, t, sxy + 2 * threadIdx.x, sxy + 2 * threadIdx.x + 1, sid + threadIdx.x,
+                                       2 * BS, BS);
+        if (a.ovf_list) {
+            // the wavefront's overflowing cells go to the list together, in lane (hence
+            // grid) order: the next tier's wavefronts then read runs of nearby cells
+            const unsigned long long m = __ballot(ovf);
+            if (m) {
+                const int leader = __ffsll((long long)m) - 1;
+                unsigned base = 0;
+                if (lane == leader) base = atomicAdd(a.ovf_count, (unsigned)__popcll(m));
+                base = __shfl(base, leader);
+                if (ovf) a.ovf_list[base + __popcll(m & ((1ull << lane) - 1))] = (int)t;
             }
+        } else if (ovf) {
+            flag_or(a.flags, FLAG_CELL_OVERFLOW);
         }
     }
 }
