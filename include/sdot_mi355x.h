@@ -363,6 +363,17 @@ const char *sdot_srv_last_error(const sdot_srv *srv);
 /* Stops the lanes after their current step; unfinished requests are dropped. */
 sdot_status sdot_srv_destroy(sdot_srv *srv);
 
+/* display_vtk(VtkOutput<1, TF> &, grid, bounds, positions, weights, nb_diracs[, radial_func])
+ * -- PowerDiagram/display_vtk.h:13-25, written as VtkOutput::save does
+ * (Display/VtkOutput.tcc:17-79): the cells of the current diracs and weights to `filename`
+ * as a legacy ASCII VTK unstructured grid of polygons (type 7) with one cell field,
+ * `field_name` (NULL: "num_dirac"), holding the dirac index.  2D: one polygon per piece of
+ * the domain (shape, pixel) a cell meets, ball arcs cut into 20 segments, a whole disc as 21
+ * points, every polygon closed by repeating its first point; 3D: one polygon per face of
+ * each piece.  Cells are listed in dirac order; a cell of zero mass draws nothing.  Runs
+ * the pass if needed; SDOT_NOT_SUPPORTED with periodic images. */
+sdot_status sdot_pd_display_vtk(sdot_pd *pd, const char *filename, const char *field_name);
+
 /* splitmix64 -> xoshiro256** uniform doubles (x >> 11) * 2^-53: the synthetic-input generator. */
 void sdot_rng_uniform(uint64_t seed, int64_t n, double *out);
 

@@ -137,6 +137,7 @@ SIGNATURES = {
     "sdot_batch_solve": (_int, [_p, _d, _int, _int, _p, _p, _p, _p, ctypes.POINTER(BatchStats)]),
     "sdot_batch_set_schedule": (_int, [_p, _int, _int, _int, _int]),
     "sdot_calibration_copy": (_int, [_p, _p, _i64]),
+    "sdot_pd_display_vtk": (_int, [_p, ctypes.c_char_p, ctypes.c_char_p]),
     "sdot_srv_create": (_int, [ctypes.POINTER(_p), ctypes.POINTER(ServerConfig)]),
     "sdot_srv_submit": (_int, [_p, _int, _i64, _p, _p, _p, _p, ctypes.POINTER(_i64)]),
     "sdot_srv_poll": (_int, [_p, _i64, ctypes.POINTER(_int)]),
@@ -371,6 +372,11 @@ class PowerDiagram:
         a, b = _p(), _p()
         self._check(lib().sdot_pd_dirac_buffers(self._h, ctypes.byref(a), ctypes.byref(b)), "dirac_buffers")
         return a.value, b.value
+
+    def display_vtk(self, filename, field_name="num_dirac", allow_empty=True):
+        """display_vtk (PowerDiagram/display_vtk.h:13-25): the cells as a legacy ASCII VTK file."""
+        self._check(lib().sdot_pd_display_vtk(self._h, str(filename).encode(), field_name.encode()), "display_vtk",
+                    allow_empty)
 
     def csr_device(self, allow_empty=True):
         a, b, c = _p(), _p(), _p()

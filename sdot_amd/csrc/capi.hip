@@ -797,6 +797,16 @@ sdot_status sdot_ot_solve(sdot_pd *pd, const double *targets, double tol, int st
     return guard(pd, [&] { return ot_solve(pd, targets, tol, stop_rule, max_iter, weights_inout, stats); });
 }
 
+sdot_status sdot_pd_display_vtk(sdot_pd *pd, const char *filename, const char *field_name) {
+    if (!pd || !filename || !filename[0]) return SDOT_INVALID_ARG;
+    return guard(pd, [&] {
+        const sdot_status st = ensure_pass(pd);
+        if (st != SDOT_OK && st != SDOT_EMPTY_CELL) return st;
+        write_vtk(pd, filename, field_name && field_name[0] ? field_name : "num_dirac");
+        return st;
+    });
+}
+
 sdot_status sdot_calibration_copy(void *dst, const void *src, int64_t nbytes) {
     if (!dst || !src || nbytes <= 0 || nbytes % 16) return SDOT_INVALID_ARG;
     const int64_t n = nbytes / 16;

@@ -470,7 +470,9 @@ void exclusive_scan(const T *in, T *out, int64_t n, T *tmp, hipStream_t s);  // 
 size_t scan_tmp_size(int64_t n);
 // cells.hip
 void run_cells(sdot_pd *pd);          // cell kernels for the current dim / precision / ball
-int64_t build_csr_device(sdot_pd *pd);  // dirac-order CSR in HBM (csr_off/csr_col/csr_val); returns slot bound
+int64_t build_csr_device(sdot_pd *pd);
+// display.hip: the cells of the last pass as a legacy ASCII VTK file (pass valid)
+void write_vtk(sdot_pd *pd, const char *filename, const char *field_name);  // dirac-order CSR in HBM (csr_off/csr_col/csr_val); returns slot bound
 void run_cells2(sdot_pd *pd, CellArgs &a);
 void run_cells3(sdot_pd *pd, CellArgs &a);
 void fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);

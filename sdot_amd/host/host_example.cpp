@@ -128,10 +128,24 @@ int main(int argc, char **argv) {
             if (jcol[k2 + 2] != 3 * columns[k] + 2) jdiff = 1e300;
         }
     }
+    // display_vtk (display_vtk.h:13-25): one polygon per cell, the count read back from the file
+    long vtk_polygons = -1;
+    if (argc > 2) {
+        VtkOutput<1> vo(VtkOutput<1>::CN{{"num_dirac"}});
+        display_vtk(vo, grid, bounds, positions.data(), weights.data(), n);
+        vo.save(argv[2]);
+        if (std::FILE *f = std::fopen(argv[2], "r")) {
+            char line[256];
+            while (std::fgets(line, sizeof line, f))
+                if (std::sscanf(line, "CELL_TYPES %ld", &vtk_polygons) == 1) break;
+            std::fclose(f);
+        }
+    }
     std::printf("{\"status\": %d, \"der_status\": %d, \"newton_iters\": %d, \"max_rel_mass_err\": %.3e, \"mass_sum\": %.15f, "
                 "\"nnz\": %zu, \"max_row_sum\": %.3e, \"ball_mass_0\": %.6e, \"centroid_mean_err\": %.3e, "
-                "\"der_pos_status\": %d, \"der_pos_vs_hessian\": %.3e}\n",
-                st, err, solver.stats.newton_iters, max_rel, sum, columns.size(), max_rowsum, ball[0], cerr, jerr, jdiff);
+                "\"der_pos_status\": %d, \"der_pos_vs_hessian\": %.3e, \"vtk_polygons\": %ld}\n",
+                st, err, solver.stats.newton_iters, max_rel, sum, columns.size(), max_rowsum, ball[0], cerr, jerr, jdiff,
+                vtk_polygons);
     return (st == 0 && err == 0 && max_rel < 1e-6 && max_rowsum < 1e-9 && cerr < 1e-12 && jerr == 0 && jdiff < 1e-9) ? 0
                                                                                                                  : 1;
 }

@@ -400,6 +400,40 @@ void get_integrals(TF *res, Grid &grid, Bounds &bounds, const Pt *positions, con
     get_integrals(res, grid, bounds, positions, weights, nb_diracs, FunctionEnum::Unit());
 }
 
+/// VtkOutput<nb_cell_fields, TF> (Display/VtkOutput.h:17-72) as display_vtk uses it: one
+/// cell field; display_vtk records the diagram to draw and save(filename) writes it (the
+/// cells are built on the device from the diagram's last pass).
+template <int nb_cell_fields = 1, class TF = double>
+class VtkOutput {
+    static_assert(nb_cell_fields == 1, "display_vtk writes one cell field (the dirac index)");
+
+public:
+    using CN = std::array<std::string, nb_cell_fields>;
+    explicit VtkOutput(const CN &names = {{"num_dirac"}}) : names(names) {}
+    void save(const std::string &filename) const {
+        if (!pd) throw Error{SDOT_INVALID_ARG, "VtkOutput::save: nothing displayed"};
+        check(pd, sdot_pd_display_vtk(pd, filename.c_str(), names[0].c_str()), "display_vtk", true);
+    }
+    sdot_pd *pd = nullptr;   // the diagram drawn by the last display_vtk
+    CN names;
+};
+
+/// display_vtk (PowerDiagram/display_vtk.h:13-25): the cells of the diagram, drawn into res.
+template <class TF, class Grid, class Bounds, class Pt, class Func>
+void display_vtk(VtkOutput<1, TF> &res, Grid &grid, Bounds &bounds, const Pt *positions, const TF *weights,
+                 std::size_t nb_diracs, const Func &func) {
+    grid.update(positions, weights, nb_diracs, false, true);
+    bounds.apply(grid.handle());
+    check(grid.handle(), set_radial_func(grid.handle(), func), "radial function");
+    res.pd = grid.handle();
+}
+
+template <class TF, class Grid, class Bounds, class Pt>
+void display_vtk(VtkOutput<1, TF> &res, Grid &grid, Bounds &bounds, const Pt *positions, const TF *weights,
+                 std::size_t nb_diracs) {
+    display_vtk(res, grid, bounds, positions, weights, nb_diracs, FunctionEnum::Unit());
+}
+
 /// get_centroids (get_centroids.h:14, rand_ratio = 0): cb(centroid, mass, num_dirac) for
 /// every cell, in dirac order; the centroid of an empty cell is 0 (get_centroids.h:58).
 template <class Grid, class Bounds, class Pt, class TF, class TI, class Func, class CB>

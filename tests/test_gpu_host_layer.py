@@ -11,10 +11,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 pytestmark = pytest.mark.gpu
 
 
-def test_host_example_solves_and_exports_csr():
+def test_host_example_solves_and_exports_csr(tmp_path):
     exe = os.path.join(ROOT, "sdot_amd", "host", "host_example")
     assert os.path.exists(exe), "build first (make -C sdot_amd)"
-    out = subprocess.run([exe, "10000"], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([exe, "10000", str(tmp_path / "cells.vtk")], capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
     r = json.loads(out.stdout.strip().splitlines()[-1])
     assert out.returncode == 0, r
@@ -25,6 +25,8 @@ def test_host_example_solves_and_exports_csr():
     assert r["centroid_mean_err"] < 1e-12
     # the weight columns of the mass rows of the position Jacobian equal the Hessian
     assert r["der_pos_status"] == 0 and r["der_pos_vs_hessian"] < 1e-9
+    # display_vtk through VtkOutput::save: one polygon per cell
+    assert r["vtk_polygons"] == 10000
 
 
 def test_host_example_assembly_of_two_shapes():
