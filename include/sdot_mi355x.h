@@ -100,8 +100,8 @@ sdot_status sdot_pd_clear_shapes(sdot_pd *pd);
 sdot_status sdot_pd_set_ball_clip(sdot_pd *pd, int on);
 /* Radial function (the Func argument of get_integrals / get_der_integrals_wrt_weights /
  * OptimalTransportSolver::solve).  The last four in 2D: one domain shape; R2 / WmR2 also
- * over constant pixels or several shapes.  R2 also in 3D on one shape
- * (ConvexPolyhedron3.tcc:1120-1123); WmR2 2D only (TODO in 3D there):
+ * over constant pixels or several shapes.  R2 also in 3D, over one shape, several shapes or
+ * voxels (ConvexPolyhedron3.tcc:1120-1123); WmR2 2D only (TODO in 3D there):
  *   SDOT_FUNC_UNIT, SDOT_FUNC_IN_BALL_W05 -- Unit.h, InBallW05.h (as sdot_pd_set_ball_clip);
  *   SDOT_FUNC_R2, SDOT_FUNC_R2_IN_BALL_W05 -- |x - x_i|^2 (R2.h, R2InBallW05.h): integrals only
  *     (the reference has no R2 boundary items: ConvexPolyhedron2.tcc:432-435);
@@ -153,7 +153,7 @@ sdot_status sdot_pd_integrals_noalloc(sdot_pd *pd, const double *d_positions, co
 /* get_der_centroids_and_integrals_wrt_weight_and_positions(m_offsets, m_columns, m_values,
  * v_values, ...) -- get_der_centroids_and_integrals_wrt_weight_and_positions.h:16 (2D:
  * one domain shape with or without the ball, several shapes or constant pixels without it; 3D: one
- * shape; translations in both; quadratic pixels: SDOT_NOT_SUPPORTED).  With nu = dim + 1: rows
+ * shape, several shapes or voxels; translations in both; quadratic pixels: SDOT_NOT_SUPPORTED).  With nu = dim + 1: rows
  * nu n + d (d < dim: centroid coordinate d; d = dim: mass of dirac n), columns nu m + e
  * (e < dim: position coordinate e; e = dim: weight of dirac m), sorted;
  * offsets[nu N + 1]; v_values[nu N] = centroids and masses.  As in the reference, the centroid rows are divided by mass + (mass == 1), and

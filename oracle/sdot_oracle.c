@@ -1438,18 +1438,20 @@ static void piece3_accumulate(const cells_job *J, int64_t i, const or_cp3 *cp, d
     }
 }
 
-/* get_der_centroids_and_integrals_wrt_weight_and_positions.h:124-166 for a 3D cell of
- * one piece (no ball: TODO in the reference): the faces' boundary items of
+/* get_der_centroids_and_integrals_wrt_weight_and_positions.h:124-166 for one piece of a
+ * 3D cell (no ball: TODO in the reference), density coeff: the faces' boundary items of
  * ConvexPolyhedron3.tcc:226-265 -- pos_integral and momentum over the fan of triangles
- * (v0, vk, vk+1) of each face, not scaled by the density, measure = coeff * area */
-static void jac3_cell(cells_job *J, int64_t i, const or_cp3 *cp, double coeff, double mass, const double *cent) {
+ * (v0, vk, vk+1) of each face, not scaled by the density, measure = coeff * area.  The
+ * pieces of a cell (for_each_intersection, :63) append their items in turn; jac3_close adds
+ * the diagonal block and merges the items of one dirac. */
+static void jac3_items(cells_job *J, int64_t i, const or_cp3 *cp, double coeff, double *der0, int64_t *mp,
+                       int *overflowp) {
     const int64_t cap = J->cap;
     int64_t *cols = J->jcols + i * cap;
     double *blk = J->jblocks + i * cap * 16;
-    int64_t m = 0;
-    double der0[16] = {0};
+    int64_t m = *mp;
     const double *c0 = J->pos + 3 * i;
-    int overflow = 0;
+    int overflow = *overflowp;
     for (int f = 0; f < cp->nf; ++f) {
         if (cp->nv[f] < 3) continue;
         int64_t id = cp->id[f];
@@ -1493,10 +1495,17 @@ static void jac3_cell(cells_job *J, int64_t i, const or_cp3 *cp, double coeff, d
         memcpy(blk + 16 * m, d1, sizeof d1);
         ++m;
     }
+    *mp = m;
+    *overflowp = overflow;
+}
+
+static void jac3_close(cells_job *J, int64_t i, const double *der0, int64_t m, int overflow, double mass,
+                       const double *cent) {
+    const int64_t cap = J->cap;
     if (m >= cap) overflow = 1;
     else {
-        cols[m] = i;
-        memcpy(blk + 16 * m, der0, sizeof der0);
+        J->jcols[i * cap + m] = i;
+        memcpy(J->jblocks + (i * cap + m) * 16, der0, 16 * sizeof(double));
         ++m;
     }
     jac_finish(J, i, m, 3, mass, cent, overflow);
@@ -1506,6 +1515,9 @@ static void eval_cell3(cells_job *J, int64_t i, or_cp3 *cp, or_cp3 *piece) {
     row_acc row = { J->cols + i * J->cap, J->vals + i * J->cap, 1, J->cap, 0.0, 0 };
     row.cols[0] = i;
     double mass = 0, perim = 0, mom[3] = {0, 0, 0};
+    double jder0[16] = {0};
+    int64_t jm = 0;
+    int jovf = 0;
     int st = build_cell3(J, i, cp);
     const or_domain *dom = J->dom;
     if (dom->nitems >= 2) {
@@ -1516,10 +1528,15 @@ static void eval_cell3(cells_job *J, int64_t i, or_cp3 *cp, or_cp3 *piece) {
                     if (cp->nv[f] < 3) continue;
                     or_cp3_plane_cut(piece, cp->v[f][0], cp->n[f], cp->id[f]);
                 }
-                if (piece->nf) piece3_accumulate(J, i, piece, dom->it_coeff[k], &mass, &perim, &row, mom);
+                if (piece->nf) {
+                    piece3_accumulate(J, i, piece, dom->it_coeff[k], &mass, &perim, &row, mom);
+                    if (J->jblocks) jac3_items(J, i, piece, dom->it_coeff[k], jder0, &jm, &jovf);
+                }
             }
     } else if (!dom->img) {
-        piece3_accumulate(J, i, cp, dom->nitems == 1 ? dom->it_coeff[0] : dom->coeff, &mass, &perim, &row, mom);
+        const double coeff = dom->nitems == 1 ? dom->it_coeff[0] : dom->coeff;
+        piece3_accumulate(J, i, cp, coeff, &mass, &perim, &row, mom);
+        if (J->jblocks) jac3_items(J, i, cp, coeff, jder0, &jm, &jovf);
     } else if (cp->nf) {
         double mn[3], mx[3];
         for (int d = 0; d < 3; ++d) { mn[d] = INFINITY; mx[d] = -INFINITY; }
@@ -1551,7 +1568,10 @@ static void eval_cell3(cells_job *J, int64_t i, or_cp3 *cp, or_cp3 *piece) {
                         if (cp->nv[f] < 3) continue;
                         or_cp3_plane_cut(piece, cp->v[f][0], cp->n[f], cp->id[f]);
                     }
-                    if (piece->nf) piece3_accumulate(J, i, piece, rho, &mass, &perim, &row, mom);
+                    if (piece->nf) {
+                        piece3_accumulate(J, i, piece, rho, &mass, &perim, &row, mom);
+                        if (J->jblocks) jac3_items(J, i, piece, rho, jder0, &jm, &jovf);
+                    }
                 }
     }
     row.vals[0] = row.der0;
@@ -1559,10 +1579,10 @@ static void eval_cell3(cells_job *J, int64_t i, or_cp3 *cp, or_cp3 *piece) {
     if (J->centroids) {
         for (int d = 0; d < 3; ++d) J->centroids[3 * i + d] = mass != 0 ? J->pos[3 * i + d] + mom[d] / mass : 0.0;
     }
-    if (J->jblocks) {     /* one shape only (checked by or_der_pos) */
+    if (J->jblocks) {
         double cent[3];
         for (int d = 0; d < 3; ++d) cent[d] = mass != 0 ? J->pos[3 * i + d] + mom[d] / mass : 0.0;
-        jac3_cell(J, i, cp, dom->nitems == 1 ? dom->it_coeff[0] : dom->coeff, mass, cent);
+        jac3_close(J, i, jder0, jm, jovf, mass, cent);
     }
     if (J->ball_arc) J->ball_arc[i] = 0;
     if (J->perimeter) J->perimeter[i] = perim;
@@ -1649,7 +1669,6 @@ static int cells_run(int dim, int64_t N, const double *pos, const double *w, con
 int or_der_pos(int dim, int64_t N, const double *pos, const double *w, const or_domain *dom, int flags, int nthreads,
                int64_t cap, int64_t *jlen, int64_t *jcols, double *jblocks, double *jv) {
     if (!dom || dom->dim != dim || !jlen || !jcols || !jblocks || !jv) return OR_INVALID;
-    if (dim == 3 && (dom->img || dom->nitems >= 2)) return OR_INVALID;   /* 3D: one shape */
     if (dom->img && dom->img_nc != 1) return OR_INVALID;                /* constant pixels */
     double *masses = (double *)malloc(sizeof(double) * (N + 1));
     double *cent = (double *)malloc(sizeof(double) * (dim * N + 1));

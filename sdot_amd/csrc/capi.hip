@@ -184,8 +184,8 @@ sdot_status compute_pass(sdot_pd *pd) {
         throw StatusError{SDOT_NOT_SUPPORTED, "ball clipping with a quadratic image (apply_poly has no ball in the reference)"};
     if (pd->ball && pieces && pd->want_jac)
         throw StatusError{SDOT_NOT_SUPPORTED, "position derivatives with ball clipping over several pieces"};
-    if (pd->rf && ((pd->has_image && pd->img_nc != 1) || (pd->dim == 3 && (pd->has_image || pd->shapes.size() > 1))))
-        throw StatusError{SDOT_NOT_SUPPORTED, "R2 / WmR2 / ExpWmR2db integration: constant pieces in 2D, one domain shape in 3D"};
+    if (pd->rf && pd->has_image && pd->img_nc != 1)
+        throw StatusError{SDOT_NOT_SUPPORTED, "R2 / WmR2 / ExpWmR2db integration: constant pieces only"};
     if (pd->rf >= 2 && pd->dim == 3)
         throw StatusError{SDOT_NOT_SUPPORTED, "WmR2 / ExpWmR2db in 3D (ConvexPolyhedron3 integration is TODO in the reference)"};
     if (pd->rf && (pd->want_mom || pd->want_jac))
@@ -752,10 +752,6 @@ sdot_status sdot_pd_der_centroids_and_integrals_wrt_weight_and_positions(sdot_pd
                                                                          double *vals, double *v_values, int64_t cap,
                                                                          int64_t *nnz) {
     if (!pd || !nnz || (cols && (!offsets || !vals))) return SDOT_INVALID_ARG;
-    if (pd->dim == 3 && (pd->has_image || pd->shapes.size() > 1)) {
-        pd->err = "position derivatives in 3D: one domain shape";
-        return SDOT_NOT_SUPPORTED;
-    }
     if (pd->has_image && pd->img_nc != 1) {
         pd->err = "position derivatives of quadratic pixels are not offered";
         return SDOT_NOT_SUPPORTED;

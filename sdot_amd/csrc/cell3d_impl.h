@@ -49,6 +49,9 @@ struct Poly3Mem {
     // per-plane first moments (3) and second moments (xx, xy, xz, yy, yz, zz) of the
     // face, local coordinates, for the position derivatives; null: not wanted
     double *jmom = nullptr;   // [(k * 9 + c) * S]
+    // with domain pieces: per cell plane, the face's area and its 9 moments summed over
+    // the pieces (each piece integrates its own part of the face)
+    double *jsum = nullptr;   // [(k * 10 + c) * S]
 };
 
 // voxel / box plane ids: box side k -> -1 - k (k = 0..5: -x,+x,-y,+y,-z,+z);
@@ -538,6 +541,37 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
     double *const want = a.mom ? off : nullptr;
     for (int k = 0; k < PCAP; ++k) C.m.hacc[k * C.m.S] = 0;
     const int npc = C.np;
+    const bool pieces = a.img || a.nitems >= 2;
+    // pieces: the cell planes whose face some piece covers in part; only those faces are
+    // boundary items (a face outside every shape or voxel gives no entry)
+    unsigned long long hit[4] = {0, 0, 0, 0};   // PCAP <= 256
+    if (C.m.jsum && pieces)
+        for (int k = 0; k < npc; ++k)
+            for (int q = 0; q < 10; ++q) C.m.jsum[(k * 10 + q) * C.m.S] = 0;
+    // one piece of density rho (the cell clipped by a domain item or a voxel): its mass (or
+    // its R2 integral), first moments, the rho-weighted areas of the cell's faces and, for
+    // the position derivatives, those faces' areas and moments (not scaled by rho)
+    auto piece_integrals = [&](double rho) {
+        if (a.rf == 1) {
+            double r2 = 0;
+            C.integrate(nullptr, &r2);
+            mass += rho * r2;
+            return;
+        }
+        const double vol = C.integrate(want);
+        mass += rho * vol;
+        if (want)
+            for (int d = 0; d < 3; ++d) mom[d] += rho * vol * off[d];
+        for (int k = 0; k < npc; ++k) {
+            C.m.hacc[k * C.m.S] += rho * C.m.acc[k * C.m.S];
+            if (C.m.acc[k * C.m.S] > 0) hit[k >> 6] |= 1ull << (k & 63);
+        }
+        if (C.m.jsum)
+            for (int k = 0; k < npc; ++k) {
+                C.m.jsum[(k * 10) * C.m.S] += C.m.acc[k * C.m.S];
+                for (int q = 0; q < 9; ++q) C.m.jsum[(k * 10 + 1 + q) * C.m.S] += C.m.jmom[(k * 9 + q) * C.m.S];
+            }
+    };
     if (C.nv > 0) {
         if (!a.img && a.nitems < 2 && a.rf == 1) {
             // R2 (integration(Constant, R2), ConvexPolyhedron3.tcc:1120-1123): integrals only
@@ -592,14 +626,7 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
                     if (C.nv == 0) empty = true;
                 }
                 if (overflow) break;
-                if (!empty) {
-                    const double rho = a.it_coeff[it];
-                    const double vol = C.integrate(want);
-                    mass += rho * vol;
-                    if (want)
-                        for (int d = 0; d < 3; ++d) mom[d] += rho * vol * off[d];
-                    for (int k = 0; k < npc; ++k) C.m.hacc[k * C.m.S] += rho * C.m.acc[k * C.m.S];
-                }
+                if (!empty) piece_integrals(a.it_coeff[it]);
                 C.restore(nv0, np0);
             }
             if (overflow) return 2;
@@ -646,13 +673,7 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
                             if (C.nv == 0) empty = true;
                         }
                         if (overflow) break;
-                        if (!empty) {
-                            const double vol = C.integrate(want);
-                            mass += rho * vol;
-                            if (want)
-                                for (int d = 0; d < 3; ++d) mom[d] += rho * vol * off[d];
-                            for (int k = 0; k < npc; ++k) C.m.hacc[k * C.m.S] += rho * C.m.acc[k * C.m.S];
-                        }
+                        if (!empty) piece_integrals(rho);
                         C.restore(nv0, np0);
                     }
             if (overflow) return 2;
@@ -666,6 +687,7 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
     for (int k = 0; k < npc && a.rf != 1; ++k) {   // R2: no boundary items
         const double ar = C.m.hacc[k * C.m.S];
         const int id = C.m.pid[k * C.m.S];
+        if (pieces && !((hit[k >> 6] >> (k & 63)) & 1)) continue;
         if (ar == 0 && C.nv > 0) {
             // a plane that bounds no face (removed by later cuts) is skipped
             bool used = false;
@@ -687,13 +709,15 @@ SDOT_HD inline int cell3_body(const CellArgs &a, const int64_t t, const Poly3Mem
             // face's moments in absolute coordinates (ConvexPolyhedron3.tcc:226-265: not
             // scaled by the density; measure = the rho-weighted area)
             const double c0a[3] = {c0x, c0y, c0z};
-            const double ag = C.m.acc[k * C.m.S];
+            // one shape: the face as integrated; pieces: the sums over the pieces
+            auto jm = [&](int q) { return pieces ? C.m.jsum[(k * 10 + 1 + q) * C.m.S] : C.m.jmom[(k * 9 + q) * C.m.S]; };
+            const double ag = pieces ? C.m.jsum[(k * 10) * C.m.S] : C.m.acc[k * C.m.S];
             double Cl[3], Ml[3][3], Ca[3], Ma[3][3];
-            for (int u = 0; u < 3; ++u) Cl[u] = C.m.jmom[(k * 9 + u) * C.m.S];
+            for (int u = 0; u < 3; ++u) Cl[u] = jm(u);
             {
                 int c = 3;
                 for (int u = 0; u < 3; ++u)
-                    for (int v = u; v < 3; ++v, ++c) Ml[u][v] = Ml[v][u] = C.m.jmom[(k * 9 + c) * C.m.S];
+                    for (int v = u; v < 3; ++v, ++c) Ml[u][v] = Ml[v][u] = jm(c);
             }
             for (int u = 0; u < 3; ++u) Ca[u] = ag * c0a[u] + Cl[u];
             for (int u = 0; u < 3; ++u)

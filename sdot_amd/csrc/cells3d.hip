@@ -18,7 +18,7 @@ constexpr int BS3 = 256;
 template <class T>
 size_t slot_bytes(int vcap, int pcap, bool jac = false) {
     return (size_t)vcap * (7 * sizeof(T) + 2 * sizeof(int)) + (size_t)pcap * (4 * sizeof(T) + sizeof(int) + 2 * sizeof(double)) +
-           (size_t)pcap * pcap * sizeof(unsigned short) + (jac ? (size_t)pcap * 9 * sizeof(double) : 0);
+           (size_t)pcap * pcap * sizeof(unsigned short) + (jac ? (size_t)pcap * 19 * sizeof(double) : 0);
 }
 
 template <class T>
@@ -48,7 +48,10 @@ __host__ __device__ c3::Poly3Mem<T> slot(char *base, int64_t S, int64_t s, int v
     take(m.acc, sizeof(double), pcap);
     take(m.hacc, sizeof(double), pcap);
     take(m.emap, sizeof(unsigned short), pcap * pcap);
-    if (jac) take(m.jmom, sizeof(double), pcap * 9);
+    if (jac) {
+        take(m.jmom, sizeof(double), pcap * 9);
+        take(m.jsum, sizeof(double), pcap * 10);
+    }
     m.S = (int)S;
     return m;
 }

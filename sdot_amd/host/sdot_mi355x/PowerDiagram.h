@@ -492,8 +492,8 @@ int get_der_integrals_wrt_weights(std::vector<TI> &m_offsets, std::vector<TI> &m
 }
 
 /// get_der_centroids_and_integrals_wrt_weight_and_positions
-/// (get_der_centroids_and_integrals_wrt_weight_and_positions.h:16; 2D: one domain shape,
-/// several shapes or constant pixels; 3D: one shape):
+/// (get_der_centroids_and_integrals_wrt_weight_and_positions.h:16; one domain shape,
+/// several shapes or constant pixels / voxels):
 /// rows nu n + d (centroid coordinates, then the mass of dirac n), columns nu m + e
 /// (position coordinates, then the weight of dirac m), nu = dim + 1; v_values = centroids
 /// and masses; returns 1 on an empty cell, 0 otherwise.

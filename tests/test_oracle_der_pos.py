@@ -164,3 +164,44 @@ def test_pieces_of_one_density_match_the_whole_box():
     assert np.abs(v - v2).max() < 1e-13
     F = finite_differences(pos, w, 1e-7, img=np.full((3, 4), c))
     assert np.abs(J - F).max() > 0.1 * np.abs(F).max()
+
+
+@pytest.mark.parametrize("case", ["image", "two_boxes"])
+def test_3d_pieces_against_finite_differences(case):
+    """3D domain of pieces (voxels of an image; two overlapping boxes) of density 1 each:
+    every piece adds the items of its faces (for_each_intersection,
+    get_der_centroids_and_integrals_wrt_weight_and_positions.h:63), so finite differences
+    of the centroids and masses pin the blocks."""
+    n = 9
+    u = O.rng_uniform(116, 4 * n)
+    pos = u[: 3 * n].reshape(n, 3).copy()
+    w = 0.01 * u[3 * n:]
+    if case == "image":
+        kw = dict(img=np.ones((2, 3, 2)))
+    else:
+        kw = dict(items=[O.box_item(np.array([0.0, 0.0, 0.0]), np.array([0.6, 0.7, 1.0]), 1.0),
+                         O.box_item(np.array([0.3, 0.4, 0.0]), np.array([1.0, 1.0, 1.0]), 1.0)])
+    st, off, cols, vals, v = O.der_pos(pos, w, **kw)
+    assert st == O.OR_OK
+    J = dense(off, cols, vals, n, 4)
+    F = finite_differences(pos, w, 1e-7, **kw)
+    assert np.abs(J - F).max() < 1e-6 * np.abs(F).max()
+
+
+def test_3d_pieces_of_one_density_match_the_whole_box():
+    """A 3D image of one density c and a box of coefficient c give the same blocks (the
+    pieces' items of a face sum to the whole face's); with c != 1 they differ from finite
+    differences (measure scaled, moments not)."""
+    n = 9
+    u = O.rng_uniform(118, 4 * n)
+    pos = u[: 3 * n].reshape(n, 3).copy()
+    w = 0.01 * u[3 * n:]
+    c = 2.5
+    st, off, cols, vals, v = O.der_pos(pos, w, img=np.full((2, 3, 2), c))
+    st2, off2, cols2, vals2, v2 = O.der_pos(pos, w, items=[O.box_item(np.zeros(3), np.ones(3), c)])
+    assert st == st2 == O.OR_OK
+    J, J2 = dense(off, cols, vals, n, 4), dense(off2, cols2, vals2, n, 4)
+    assert np.abs(J - J2).max() < 1e-11 * np.abs(J2).max()
+    assert np.abs(v - v2).max() < 1e-13
+    F = finite_differences(pos, w, 1e-7, img=np.full((2, 3, 2), c))
+    assert np.abs(J - F).max() > 0.1 * np.abs(F).max()

@@ -211,3 +211,22 @@ def test_expwmr2db_hessian_against_finite_differences():
         fd = (O.cells(pos, wp, radial="expwmr2db", eps=eps)["masses"] -
               O.cells(pos, wm, radial="expwmr2db", eps=eps)["masses"]) / (2 * h)
         assert np.abs(fd - M[:, j]).max() < 1e-6 * np.abs(M).max()
+
+
+def test_r2_3d_pieces_add_up_to_the_cell():
+    """R2 in 3D over pieces: the voxels of a constant image, or two boxes splitting the
+    cube, give the one-box integral (each piece's second moment about the dirac adds up to
+    the cell's), scaled by the density; a planted error in a piece's integral (another
+    centre, or a dropped piece) fails it."""
+    n = 60
+    u = O.rng_uniform(131, 4 * n)
+    pos = u[: 3 * n].reshape(n, 3).copy()
+    w = 0.02 * u[3 * n:]
+    one = O.cells(pos, w, radial="r2")["masses"]
+    c = 1.75
+    img = O.cells(pos, w, radial="r2", img=np.full((3, 2, 4), c))["masses"]
+    assert np.abs(img - c * one).max() < 1e-13
+    split = [O.box_item(np.array([0.0, 0.0, 0.0]), np.array([0.45, 1.0, 1.0]), 1.0),
+             O.box_item(np.array([0.45, 0.0, 0.0]), np.array([1.0, 1.0, 1.0]), 1.0)]
+    two = O.cells(pos, w, radial="r2", items=split)["masses"]
+    assert np.abs(two - one).max() < 1e-13
