@@ -455,6 +455,7 @@ struct BatchEngine::Impl {
 BatchEngine::BatchEngine(int precision, int ball, int dpc_x100, bool use_graph)
     : im(new Impl), ball_(ball ? 1 : 0), dpc_x100_(dpc_x100 > 0 ? dpc_x100 : 250), use_graph_(use_graph) {
     im->pcg.set_options(true, use_graph);
+    im->pcg.set_dim(2);
     const sdot_status st = sdot_pd_create(&pd, 2, precision);
     if (st != SDOT_OK) {
         delete im;

@@ -58,6 +58,7 @@ public:
     void set_layout(const std::vector<int64_t> &off, hipStream_t s);
     void set_options(bool use_amg, bool use_graph);
     void invalidate();          // the matrix or vector buffers moved: recapture the graph
+    void set_dim(int dim);      // the problems' dimension: picks how the V-cycle walks the rows
     // Solves A x = b for every problem p with active[p] != 0 (the others keep x
     // untouched), relative tolerance max(1e-14, min(1e-3, 1e-3 * err[p])) on the
     // residual 2-norm, err = d_err (device, K).  iters[p]: iterations of p.

@@ -191,24 +191,34 @@ __device__ inline bool skip(const double *S, const int *amg_off, int p, int gate
     return gated_out(S, p, gate) || amg_off[p];
 }
 
-// pre-smooth level l from zero (x = w r / D) and its residual r - A x, one thread per
-// row (rows of problems that have a coarser level)
+// sum over the LPR consecutive lanes of a row (LPR a power of two, at most 64)
+template <int LPR>
+__device__ inline double row_sum(double v) {
+    for (int o = LPR / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, LPR);
+    return v;
+}
+
+
+// residual of the pre-smoothed iterate x = w r / D of level l (x written with r by its
+// producer: the CG update on level 0, the restriction on the coarse levels):
+// res = r - A x, for the rows of problems that have a coarser level
+template <int LPR>
 __global__ __launch_bounds__(BB) void kp_smooth_res(LevelView F, const int *plev, double *res, const double *S,
                                                     const int *amg_off, int gate) {
-    const int64_t t = blockIdx.x * (int64_t)BB + threadIdx.x;
+    const int64_t g = blockIdx.x * (int64_t)BB + threadIdx.x;
+    const int64_t t = g / LPR;
+    const int sub = (int)(g % LPR);
     if (t >= F.n) return;
     const int p = F.prob[t];
-    if (skip(S, amg_off, p, gate) || plev[p] <= F.lev) return;
-    const double rt = F.r[t], dt = F.A.diag[t];
-    const double xt = OMEGA * rt / dt;
-    F.xpre[t] = xt;
-    double ax = dt * xt;
+    if (skip(S, amg_off, p, gate) || plev[p] <= F.lev) return;   // uniform over the row's lanes
     const int L = F.A.len[t];
-    for (int e = 0; e < L; ++e) {
+    double ax = 0;
+    for (int e = sub; e < L; e += LPR) {
         const int c = F.A.col[e * F.A.stride + t];
-        ax += F.A.val[e * F.A.stride + t] * (OMEGA * F.r[c] / F.A.diag[c]);
+        ax += F.A.val[e * F.A.stride + t] * F.xpre[c];
     }
-    res[t] = rt - ax;
+    ax = row_sum<LPR>(ax);
+    if (sub == 0) res[t] = F.r[t] - (F.A.diag[t] * F.xpre[t] + ax);
 }
 
 // restriction: coarse row I = sum of the residuals of its aggregate, in row order
@@ -223,6 +233,18 @@ __global__ __launch_bounds__(BB) void kp_restrict(LevelView F, LevelView C, cons
     double sum = 0;
     for (int64_t t = f0; t < f1; ++t) sum += res[t];
     C.r[I] = sum;
+    C.xpre[I] = OMEGA * sum / C.A.diag[I];
+}
+
+// the corrected iterate before post-smoothing, y = x + s P e (P: the aggregates' constant
+// prolongation, e the coarse level's result), for the rows the post-smoothing reads
+__global__ __launch_bounds__(BB) void kp_prolong(LevelView F, LevelView C, const int *plev, double *y, const double *S,
+                                                 const int *amg_off, int gate, int jacobi) {
+    const int64_t t = blockIdx.x * (int64_t)BB + threadIdx.x;
+    if (t >= F.n) return;
+    const int p = F.prob[t];
+    if (jacobi || skip(S, amg_off, p, gate) || plev[p] == F.lev) return;
+    y[t] = F.xpre[t] + CSCALE * C.out[C.off[p] + (t - F.off[p]) / AGG];
 }
 
 // coarsest level of each problem: out = inverse * r (level 0 problems write z)
@@ -247,28 +269,42 @@ __global__ __launch_bounds__(CMIN) void kp_coarse_solve(const int64_t *cbase, co
     }
 }
 
-// prolong the coarse correction, post-smooth (levels >= 1)
-__global__ __launch_bounds__(BB) void kp_up(LevelView F, LevelView C, const int *plev, const double *S,
-                                            const int *amg_off, int gate) {
-    const int64_t t = blockIdx.x * (int64_t)BB + threadIdx.x;
+// the prolonged iterate at row t: precomputed (YIN, kp_prolong) or formed here
+template <bool YIN>
+__device__ inline double prolonged(const LevelView &F, const LevelView &C, const double *y, int p, int64_t t) {
+    if (YIN) return y[t];
+    return F.xpre[t] + CSCALE * C.out[C.off[p] + (t - F.off[p]) / AGG];
+}
+
+// post-smooth the prolonged iterate y (levels >= 1)
+template <int LPR, bool YIN>
+__global__ __launch_bounds__(BB) void kp_up(LevelView F, LevelView C, const double *y, const int *plev,
+                                            const double *S, const int *amg_off, int gate) {
+    const int64_t g = blockIdx.x * (int64_t)BB + threadIdx.x;
+    const int64_t t = g / LPR;
+    const int sub = (int)(g % LPR);
     if (t >= F.n) return;
     const int p = F.prob[t];
     if (skip(S, amg_off, p, gate) || plev[p] == F.lev) return;   // coarsest of p: set by the coarse solve
-    const int64_t fo = F.off[p], co = C.off[p];
-    const double y = F.xpre[t] + CSCALE * C.out[co + (t - fo) / AGG];
-    double ay = F.A.diag[t] * y;
     const int L = F.A.len[t];
-    for (int e = 0; e < L; ++e) {
+    double ay = 0;
+    for (int e = sub; e < L; e += LPR) {
         const int c = F.A.col[e * F.A.stride + t];
-        ay += F.A.val[e * F.A.stride + t] * (F.xpre[c] + CSCALE * C.out[co + (c - fo) / AGG]);
+        ay += F.A.val[e * F.A.stride + t] * prolonged<YIN>(F, C, y, p, c);
     }
-    F.out[t] = y + OMEGA * (F.r[t] - ay) / F.A.diag[t];
+    ay = row_sum<LPR>(ay);
+    if (sub == 0) {
+        const double yt = prolonged<YIN>(F, C, y, p, t), dt = F.A.diag[t];
+        F.out[t] = yt + OMEGA * (F.r[t] - (dt * yt + ay)) / dt;
+    }
 }
 
 // level 0: z = M r (multigrid, the coarse solve of a small problem, or Jacobi),
 // per-block partial of r.z; on the first application also p = z
-__global__ __launch_bounds__(BB) void kp_up0(LevelView F, LevelView C, const int *plev, Blocks B, double *part,
-                                             double *pvec, int init, int jacobi, const double *S, const int *amg_off) {
+template <bool YIN>
+__global__ __launch_bounds__(BB) void kp_up0(LevelView F, LevelView C, const double *y, const int *plev, Blocks B,
+                                             double *part, double *pvec, int init, int jacobi, const double *S,
+                                             const int *amg_off) {
     __shared__ double sm[BB];
     const BlockRows br = block_rows(B, blockIdx.x);
     const int p = br.p;
@@ -285,15 +321,14 @@ __global__ __launch_bounds__(BB) void kp_up0(LevelView F, LevelView C, const int
         } else if (coarse) {
             z = F.out[t];
         } else {
-            const int64_t fo = F.off[p], co = C.off[p];
-            const double y = F.xpre[t] + CSCALE * C.out[co + (t - fo) / AGG];
-            double ay = F.A.diag[t] * y;
+            const double yt = prolonged<YIN>(F, C, y, p, t);
+            double ay = F.A.diag[t] * yt;
             const int L = F.A.len[t];
             for (int e = 0; e < L; ++e) {
                 const int c = F.A.col[e * F.A.stride + t];
-                ay += F.A.val[e * F.A.stride + t] * (F.xpre[c] + CSCALE * C.out[co + (c - fo) / AGG]);
+                ay += F.A.val[e * F.A.stride + t] * prolonged<YIN>(F, C, y, p, c);
             }
-            z = y + OMEGA * (F.r[t] - ay) / F.A.diag[t];
+            z = yt + OMEGA * (F.r[t] - ay) / F.A.diag[t];
             F.out[t] = z;
         }
         if (init) pvec[t] = z;
@@ -305,7 +340,7 @@ __global__ __launch_bounds__(BB) void kp_up0(LevelView F, LevelView C, const int
 
 // ---- CG ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BB) void kp_init(Blocks B, const double *b, double *x, double *r, double *part,
-                                              const double *S) {
+                                              const double *S, const double *diag, double *xpre) {
     __shared__ double sm[BB];
     const BlockRows br = block_rows(B, blockIdx.x);
     if (gated_out(S, br.p, 1)) return;
@@ -314,6 +349,7 @@ __global__ __launch_bounds__(BB) void kp_init(Blocks B, const double *b, double 
         const double rv = b[t];
         x[t] = 0;
         r[t] = rv;
+        xpre[t] = OMEGA * rv / diag[t];   // level 0's pre-smoothed iterate
         bb += rv * rv;
     }
     bb = bsum(bb, sm);
@@ -373,7 +409,7 @@ __global__ __launch_bounds__(BB) void kp_alpha(const double *part, const int64_t
 }
 
 __global__ __launch_bounds__(BB) void kp_axpy(Blocks B, const double *p, const double *q, double *x, double *r,
-                                              double *part, const double *S) {
+                                              double *part, const double *S, const double *diag, double *xpre) {
     __shared__ double sm[BB];
     const BlockRows br = block_rows(B, blockIdx.x);
     const double *sp = S + br.p * S_NS;
@@ -384,6 +420,7 @@ __global__ __launch_bounds__(BB) void kp_axpy(Blocks B, const double *p, const d
         x[t] += alpha * p[t];
         const double rv = r[t] - alpha * q[t];
         r[t] = rv;
+        xpre[t] = OMEGA * rv / diag[t];   // level 0's pre-smoothed iterate
         rr += rv * rv;
     }
     rr = bsum(rr, sm);
@@ -427,9 +464,49 @@ __global__ __launch_bounds__(BB) void kp_count_active(const double *S, int64_t K
     if (threadIdx.x == 0) *out = (int)c;
 }
 
+// How the V-cycle walks the rows: lanes per row on the coarse levels (rows of up to
+// CCAP entries; one lane per row would wait on one gather after the other) and whether
+// the prolonged iterate is formed once per row (a pass, then one gather per entry) or
+// per entry (two gathers).  Chosen by the problems' dimension only, so that a problem's
+// iterates are the same alone and in a batch.
+struct RowPlan {
+    int lpr_coarse = 8;
+    bool prolong0 = true, prolong_coarse = true;
+};
+
+void launch_smooth_res(int lpr, const LevelView &F, const int *plev, double *res, const double *S, const int *amg_off,
+                       int gate, hipStream_t s) {
+    const unsigned nb = nblocks(F.n * lpr, BB);
+    switch (lpr) {
+        case 1: hipLaunchKernelGGL(kp_smooth_res<1>, dim3(nb), dim3(BB), 0, s, F, plev, res, S, amg_off, gate); break;
+        case 2: hipLaunchKernelGGL(kp_smooth_res<2>, dim3(nb), dim3(BB), 0, s, F, plev, res, S, amg_off, gate); break;
+        case 4: hipLaunchKernelGGL(kp_smooth_res<4>, dim3(nb), dim3(BB), 0, s, F, plev, res, S, amg_off, gate); break;
+        default: hipLaunchKernelGGL(kp_smooth_res<8>, dim3(nb), dim3(BB), 0, s, F, plev, res, S, amg_off, gate); break;
+    }
+}
+
+template <bool YIN>
+void launch_up_y(int lpr, const LevelView &F, const LevelView &C, const double *y, const int *plev, const double *S,
+                 const int *amg_off, int gate, hipStream_t s) {
+    const unsigned nb = nblocks(F.n * lpr, BB);
+    switch (lpr) {
+        case 1: hipLaunchKernelGGL((kp_up<1, YIN>), dim3(nb), dim3(BB), 0, s, F, C, y, plev, S, amg_off, gate); break;
+        case 2: hipLaunchKernelGGL((kp_up<2, YIN>), dim3(nb), dim3(BB), 0, s, F, C, y, plev, S, amg_off, gate); break;
+        case 4: hipLaunchKernelGGL((kp_up<4, YIN>), dim3(nb), dim3(BB), 0, s, F, C, y, plev, S, amg_off, gate); break;
+        default: hipLaunchKernelGGL((kp_up<8, YIN>), dim3(nb), dim3(BB), 0, s, F, C, y, plev, S, amg_off, gate); break;
+    }
+}
+
+void launch_up(int lpr, bool yin, const LevelView &F, const LevelView &C, const double *y, const int *plev,
+               const double *S, const int *amg_off, int gate, hipStream_t s) {
+    if (yin) launch_up_y<true>(lpr, F, C, y, plev, S, amg_off, gate, s);
+    else launch_up_y<false>(lpr, F, C, y, plev, S, amg_off, gate, s);
+}
+
 }  // namespace
 
 struct PcgSolver::Impl {
+    RowPlan plan;
     hipStream_t s = nullptr;
     int64_t K = 0, N = 0, nblk = 0;
     std::vector<int64_t> off, blk_off;
@@ -612,24 +689,34 @@ void apply_prec(PcgSolver::Impl &I, const PcgMat &A0, bool amg, bool init) {
     for (int l = 0; l < L; ++l) {
         LevelView F = level_view(I, A0, l, I.r.p, I.z.p), C = level_view(I, A0, l + 1, I.r.p, I.z.p);
         if (C.n == 0) continue;
-        hipLaunchKernelGGL(kp_smooth_res, dim3(nblocks(F.n, BB)), dim3(BB), 0, s, F, I.dplev.p, I.res.p, I.S.p,
-                           I.amg_off.p, gate);
+        launch_smooth_res(l == 0 ? 1 : I.plan.lpr_coarse, F, I.dplev.p, I.res.p, I.S.p, I.amg_off.p, gate, s);
         hipLaunchKernelGGL(kp_restrict, dim3(nblocks(C.n, BB)), dim3(BB), 0, s, F, C, I.res.p, I.S.p, I.amg_off.p,
                            gate);
     }
     if (amg)
         hipLaunchKernelGGL(kp_coarse_solve, dim3((unsigned)I.K), dim3(CMIN), 0, s, I.dcbase.p, I.dcn.p, I.dplev.p,
                            I.inv.p, I.vr.p, I.vout.p, I.r.p, I.z.p, I.S.p, I.amg_off.p, gate);
+    // the restriction's residual buffer is free again: it can hold each level's prolonged iterate
+    const RowPlan &rp = I.plan;
     for (int l = L - 1; l >= 1; --l) {
         LevelView F = level_view(I, A0, l, I.r.p, I.z.p), C = level_view(I, A0, l + 1, I.r.p, I.z.p);
-        if (F.n > 0)
-            hipLaunchKernelGGL(kp_up, dim3(nblocks(F.n, BB)), dim3(BB), 0, s, F, C, I.dplev.p, I.S.p, I.amg_off.p,
-                               gate);
+        if (F.n == 0) continue;
+        if (rp.prolong_coarse)
+            hipLaunchKernelGGL(kp_prolong, dim3(nblocks(F.n, BB)), dim3(BB), 0, s, F, C, I.dplev.p, I.res.p, I.S.p,
+                               I.amg_off.p, gate, 0);
+        launch_up(rp.lpr_coarse, rp.prolong_coarse, F, C, I.res.p, I.dplev.p, I.S.p, I.amg_off.p, gate, s);
     }
     LevelView F = level_view(I, A0, 0, I.r.p, I.z.p);
     LevelView C = L >= 1 ? level_view(I, A0, 1, I.r.p, I.z.p) : F;
-    hipLaunchKernelGGL(kp_up0, dim3((unsigned)I.nblk), dim3(BB), 0, s, F, C, I.dplev.p, I.blocks(), I.red.p, I.p.p,
-                       init ? 1 : 0, amg ? 0 : 1, I.S.p, I.amg_off.p);
+    if (L >= 1 && rp.prolong0) {
+        hipLaunchKernelGGL(kp_prolong, dim3(nblocks(F.n, BB)), dim3(BB), 0, s, F, C, I.dplev.p, I.res.p, I.S.p,
+                           I.amg_off.p, gate, 0);
+        hipLaunchKernelGGL(kp_up0<true>, dim3((unsigned)I.nblk), dim3(BB), 0, s, F, C, I.res.p, I.dplev.p, I.blocks(),
+                           I.red.p, I.p.p, init ? 1 : 0, amg ? 0 : 1, I.S.p, I.amg_off.p);
+    } else {
+        hipLaunchKernelGGL(kp_up0<false>, dim3((unsigned)I.nblk), dim3(BB), 0, s, F, C, I.res.p, I.dplev.p,
+                           I.blocks(), I.red.p, I.p.p, init ? 1 : 0, amg ? 0 : 1, I.S.p, I.amg_off.p);
+    }
 }
 
 void cg_iteration(PcgSolver::Impl &I, const PcgMat &A, double *x, bool amg, int max_iter) {
@@ -638,7 +725,8 @@ void cg_iteration(PcgSolver::Impl &I, const PcgMat &A, double *x, bool amg, int 
     const Blocks bl = I.blocks();
     hipLaunchKernelGGL(kp_spmv_dot, dim3(nb), dim3(BB), 0, s, A, bl, I.p.p, I.q.p, I.red.p, I.S.p);
     hipLaunchKernelGGL(kp_alpha, dim3(K), dim3(BB), 0, s, I.red.p, I.dblk.p, I.S.p);
-    hipLaunchKernelGGL(kp_axpy, dim3(nb), dim3(BB), 0, s, bl, I.p.p, I.q.p, x, I.r.p, I.red.p, I.S.p);
+    hipLaunchKernelGGL(kp_axpy, dim3(nb), dim3(BB), 0, s, bl, I.p.p, I.q.p, x, I.r.p, I.red.p, I.S.p, A.diag,
+                       I.xpre0.p);
     apply_prec(I, A, amg, false);
     hipLaunchKernelGGL(kp_beta, dim3(K), dim3(BB), 0, s, I.red.p, I.dblk.p, I.S.p, max_iter);
     hipLaunchKernelGGL(kp_pupdate, dim3(nb), dim3(BB), 0, s, bl, I.z.p, I.p.p, I.S.p);
@@ -667,7 +755,8 @@ void PcgSolver::enqueue_start(const PcgMat &A, const double *b, double *x, const
                                I.inv.p, I.amg_off.p, I.S.p);
         }
     }
-    hipLaunchKernelGGL(kp_init, dim3((unsigned)I.nblk), dim3(BB), 0, s, I.blocks(), b, x, I.r.p, I.red.p, I.S.p);
+    hipLaunchKernelGGL(kp_init, dim3((unsigned)I.nblk), dim3(BB), 0, s, I.blocks(), b, x, I.r.p, I.red.p, I.S.p, A.diag,
+                       I.xpre0.p);
     apply_prec(I, A, amg, true);
     hipLaunchKernelGGL(kp_init_red, dim3((unsigned)K), dim3(BB), 0, s, I.red.p, I.dblk.p, d_err, I.S.p);
     HIPCK(hipGetLastError());
@@ -680,6 +769,22 @@ void PcgSolver::enqueue_iterations(const PcgMat &A, double *x, int n, int max_it
 }
 
 double *PcgSolver::state() { return im->S.p; }
+
+void PcgSolver::set_dim(int dim) {
+    // measured on the 3D 1M solve and the 512-problem 2D batch: 3D coarse rows (aggregates
+    // of 4 neighbouring cells, up to 64 entries) want 8 lanes and the prolonged iterate
+    // formed once per row (CG 0.27 -> 0.20 s per solve); 2D rows are short (about 6 entries
+    // on level 0), where one lane per row and the iterate formed per entry are fastest
+    RowPlan &rp = im->plan;
+    if (dim == 3) {
+        rp.lpr_coarse = 8;
+        rp.prolong0 = rp.prolong_coarse = true;
+    } else {
+        rp.lpr_coarse = 1;
+        rp.prolong0 = rp.prolong_coarse = false;
+    }
+    im->drop_graph();
+}
 int PcgSolver::levels() const { return use_amg_ && im->L > 0 ? im->L + 1 : 0; }
 int *PcgSolver::fallback() { return im->amg_off.p; }
 

@@ -103,7 +103,10 @@ inline int red_blocks(int64_t n) { return (int)std::min<int64_t>(MAXB, std::max<
 // relative residual rtol.  Returns the iteration count.
 // M x = b on the handle's Hessian rows; relative tolerance from the mass error at d_err
 int pcg_solve(sdot_pd *pd, const double *b, double *x, const double *d_err) {
-    if (!pd->pcg) pd->pcg = new PcgSolver();
+    if (!pd->pcg) {
+        pd->pcg = new PcgSolver();
+        pd->pcg->set_dim(pd->dim);
+    }
     if (pd->pcg_n != pd->N) {
         pd->pcg->set_layout({0, pd->N}, pd->stream);
         pd->pcg_n = pd->N;
