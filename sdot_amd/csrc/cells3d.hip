@@ -419,17 +419,11 @@ struct WaveCell {
         return 1;
     }
 
+    // every plane of the polyhedron, voxel sides included, is in the LDS table (set_plane
+    // at its cut); reading it there keeps the voxel bounds out of a runtime-indexed
+    // selection, which the compiler turned into an indexed load of the whole cell state
+    // from scratch memory
     __device__ void plane64(int k, double &nx, double &ny, double &nz, double &h) const {
-        const int id = L.pid[k];
-        if (id <= -7 && id >= -12) {   // voxel side
-            const int s = -7 - id, axis = s >> 1, up = s & 1;
-            const double off = up ? vhi(axis) : vlo(axis);
-            nx = axis == 0 ? (up ? 1.0 : -1.0) : 0.0;
-            ny = axis == 1 ? (up ? 1.0 : -1.0) : 0.0;
-            nz = axis == 2 ? (up ? 1.0 : -1.0) : 0.0;
-            h = up ? off : -off;
-            return;
-        }
         nx = L.pn[k][0];
         ny = L.pn[k][1];
         nz = L.pn[k][2];
