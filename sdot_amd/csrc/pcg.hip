@@ -19,7 +19,6 @@ constexpr int CHUNK = 8;         // PCG iterations per graph launch
 constexpr int AGG = 4;           // rows per aggregate
 constexpr int CMIN = 64;         // coarsest level: at most CMIN rows per problem
 constexpr int CCAP = 64;         // off-diagonal entries per coarse row
-constexpr int CBS = 64;          // threads per block of the coarsening kernel
 constexpr double OMEGA = 0.8;    // damped-Jacobi smoothing
 constexpr double CSCALE = 1.5;   // coarse-correction scaling
 
@@ -68,16 +67,31 @@ __global__ __launch_bounds__(BB) void kp_owner(const int64_t *off, int64_t K, in
 
 // ---- multigrid setup ------------------------------------------------------------
 // Coarse row I of level l+1 = sum of the AGG fine rows of its aggregate, columns
-// mapped to aggregates, duplicates merged in a fixed order (one thread per row).
-__global__ __launch_bounds__(CBS) void kp_coarsen(PcgMat A, const int64_t *offl, const int64_t *offc,
+// mapped to aggregates, duplicates merged in order of first appearance.  A group of
+// CLPR lanes builds one row: the entries are taken one after the other (the same order,
+// hence the same row, as one thread would build), each looked up in the row so far by
+// all lanes at once (a ballot finds it), so a lookup costs CCAP / CLPR steps instead of
+// a scan of up to CCAP dependent LDS reads.
+constexpr int CLPR = 16;
+constexpr int CGRP = 256 / CLPR;   // coarse rows per block
+
+__device__ inline void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(256) void kp_coarsen(PcgMat A, const int64_t *offl, const int64_t *offc,
                                                   const int *cprob, int64_t nc, double *cdiag, int *clen, int *ccol,
                                                   double *cval, int *amg_off, const double *S) {
-    __shared__ int sc[CCAP * CBS];
-    __shared__ double sv[CCAP * CBS];
-    const int64_t I = blockIdx.x * (int64_t)CBS + threadIdx.x;
-    if (I >= nc) return;
-    int *uc = sc + threadIdx.x;
-    double *uv = sv + threadIdx.x;
+    __shared__ int sc[CGRP][CCAP];
+    __shared__ double sv[CGRP][CCAP];
+    const int grp = threadIdx.x / CLPR, sub = threadIdx.x % CLPR;
+    const int lane = threadIdx.x & 63, gbase = lane - sub;   // the group's first lane in the wavefront
+    const int64_t I = blockIdx.x * (int64_t)CGRP + grp;
+    if (I >= nc) return;   // uniform over the group
+    int *uc = sc[grp];
+    double *uv = sv[grp];
     const int p = cprob[I];
     if (S[p * S_NS + S_START] == 0) return;   // only the problems that start a solve
     const int64_t li = I - offc[p];
@@ -97,26 +111,35 @@ __global__ __launch_bounds__(CBS) void kp_coarsen(PcgMat A, const int64_t *offl,
                 d += v;
                 continue;
             }
-            int k = 0;
-            while (k < u && uc[k * CBS] != cc) ++k;
-            if (k < u) {
-                uv[k * CBS] += v;
+            int found = -1;
+            for (int k = sub; k < u; k += CLPR)
+                if (uc[k] == cc) found = k;
+            const unsigned long long gm = (__ballot(found >= 0) >> gbase) & ((1ull << CLPR) - 1);
+            if (gm) {
+                const int k = __shfl(found, gbase + __ffsll((long long)gm) - 1);
+                if (sub == 0) uv[k] += v;
             } else if (u < CCAP) {
-                uc[u * CBS] = cc;
-                uv[u * CBS] = v;
+                if (sub == 0) {
+                    uc[u] = cc;
+                    uv[u] = v;
+                }
                 ++u;
             } else {
                 ovf = true;
             }
+            wave_lds_order();
         }
     }
-    cdiag[I] = d;
-    clen[I] = u;
-    for (int k = 0; k < u; ++k) {
-        ccol[k * nc + I] = uc[k * CBS];
-        cval[k * nc + I] = uv[k * CBS];
+    wave_lds_order();
+    if (sub == 0) {
+        cdiag[I] = d;
+        clen[I] = u;
+        if (ovf || !(d > 0)) amg_off[p] = 1;
     }
-    if (ovf || !(d > 0)) amg_off[p] = 1;
+    for (int k = sub; k < u; k += CLPR) {
+        ccol[k * nc + I] = uc[k];
+        cval[k * nc + I] = uv[k];
+    }
 }
 
 // dense inverse of each problem's coarsest matrix (at most CMIN rows), in-place
@@ -745,8 +768,8 @@ void PcgSolver::enqueue_start(const PcgMat &A, const double *b, double *x, const
             LevelView F = level_view(I, A, l, nullptr, nullptr);
             Impl::Level &V = I.lv[l + 1];
             if (V.n > 0)
-                hipLaunchKernelGGL(kp_coarsen, dim3(nblocks(V.n, CBS)), dim3(CBS), 0, s, F.A, F.off, V.doff.p, V.prob.p,
-                                   V.n, V.diag.p, V.len.p, V.col.p, V.val.p, I.amg_off.p, I.S.p);
+                hipLaunchKernelGGL(kp_coarsen, dim3(nblocks(V.n, CGRP)), dim3(256), 0, s, F.A, F.off, V.doff.p,
+                                   V.prob.p, V.n, V.diag.p, V.len.p, V.col.p, V.val.p, I.amg_off.p, I.S.p);
         }
         // dense inverses: each problem's coarsest matrix lives at its own level
         for (int l = 0; l <= I.L; ++l) {
