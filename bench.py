@@ -38,7 +38,7 @@ PEAK = {"fp64": 78.6, "mixed": 157.3}        # MI355X vector FP64 / FP32, TFLOP/
 # FETCH_SIZE and WRITE_SIZE of k_cells2, each scaled by the same counter on the library's copy
 # kernel moving 1 GiB (bench.py --copy-calibration 1024: FETCH reads half the copied bytes,
 # WRITE all of them, so FETCH x 2, WRITE x 1)
-TRAFFIC = {"mixed": 184.2e6, "fp64": 576.1e6}
+TRAFFIC = {"mixed": 184.2e6, "fp64": 562.8e6}
 # VALU instructions (wavefront instructions, SQ_INSTS_VALU) per pass of the same kernels, and
 # the issue rate they are compared with: one wave64 VALU instruction per SIMD every 4 cycles,
 # 1024 SIMDs at 2.4 GHz

@@ -492,8 +492,9 @@ BatchEngine::~BatchEngine() {
 
 double BatchEngine::bytes_per_dirac() {
     // dirac arrays (pos, w, w0, old, tgt, spare copies for a relayout), sort keys and
-    // permutations, sorted copies, pass outputs, ELL rows at the wide capacity, PCG vectors
-    return 16 + 8 * 4 + 8 * 5 + 8 + 8 + 24 + 32 + 4 + ELL_CAP2_WIDE * 12.0 + 8 * 7 + 8 + 16;
+    // permutations, sorted copies, pass outputs, ELL rows at the wide capacity, PCG vectors,
+    // FP64 resume slots (184 bytes for a quarter of the cells)
+    return 16 + 8 * 4 + 8 * 5 + 8 + 8 + 24 + 32 + 4 + ELL_CAP2_WIDE * 12.0 + 8 * 7 + 8 + 16 + 46;
 }
 
 size_t BatchEngine::device_bytes() const { return (size_t)(bytes_per_dirac() * (double)N_); }
@@ -621,6 +622,7 @@ void BatchEngine::relayout(const std::vector<int> &keep, const std::vector<NewPr
     pd->ell_col.ensure((size_t)pd->ell_cap * N);
     pd->ell_val.ensure((size_t)pd->ell_cap * N);
     pd->ovf_list.ensure(2 * (size_t)N + 2);
+    if (pd->precision != SDOT_MIXED) ensure_resume(pd, N);
     probs.swap(np);
     N_ = N;
     std::vector<int> done(K);
@@ -737,6 +739,7 @@ void enqueue_pass(sdot_pd *pd, BatchEngine::Impl *im, int64_t K, int64_t N, cons
     a.flags = pd->flags.p;
     a.ovf_list = pd->ovf_list.p;
     a.ovf_count = pd->flags.p + 1;
+    if (pd->precision != SDOT_MIXED) use_resume(pd, a);
     a.prob = im->prob.p;
     a.grids = im->dgrids.p;
     a.code_base = im->dcode_base.p;

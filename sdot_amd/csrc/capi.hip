@@ -252,6 +252,10 @@ sdot_status compute_pass(sdot_pd *pd) {
     }
     a.ovf_list = pd->ovf_list.p;
     a.ovf_count = pd->flags.p + 1;
+    if (pd->dim == 2 && pd->precision != SDOT_MIXED && !pd->want_jac && !pd->rf) {
+        ensure_resume(pd, N);
+        use_resume(pd, a);
+    }
     pd->wctr.ensure(3 * WCTR);
     fill_u32((uint32_t *)pd->wctr.p, 6 * WCTR, 0u, s);
     a.ctr = pd->wctr.p;
@@ -377,6 +381,7 @@ sdot_status sdot_pd_destroy(sdot_pd *pd) {
     pd->spos.release(); pd->sw.release(); pd->cell_wmax.release(); pd->bbox_part.release(); pd->wpyr.release(); pd->wglob.release();
     pd->mass.release(); pd->arc.release(); pd->perim.release(); pd->diag.release(); pd->ell_val.release();
     pd->row_len.release(); pd->mom.release(); pd->ell_col.release(); pd->ovf_list.release(); pd->flags.release(); pd->wctr.release();
+    pd->rs_hdr.release(); pd->rs_i.release(); pd->rs_d.release();
     pd->csr_off.release(); pd->csr_col.release(); pd->csr_cnt.release(); pd->csr_tmp.release();
     pd->csr_val.release(); pd->out_tmp.release(); pd->scratch3.release();
     pd->csr_off2.release(); pd->csr_col2.release(); pd->csr_val2.release();

@@ -356,6 +356,13 @@ struct Integrator {
 };
 
 // half side of the square that encloses the ball of radius sqrt(w0) with a margin
+// Sums of two products with the rounding written out (fma): the compiler's own
+// contraction may fuse either product, and may choose differently in two
+// instantiations of the cell body; written out, every capacity tier rounds alike, so a
+// cell resumed in a larger tier matches one started there bit for bit.
+SDOT_HD inline double sq2(double x, double y) { return fma(x, x, y * y); }
+SDOT_HD inline double det2(double a, double b, double c, double d) { return fma(a, d, -(b * c)); }   // a d - b c
+
 SDOT_HD inline double ball_square(double w0) { return sqrt(fmax(w0, 0.0)) * (1 + 1e-4); }
 
 // The line a*x + b*y = c (coordinates local to the dirac) that carries an edge:
@@ -369,14 +376,14 @@ SDOT_HD inline void line_of(const CellArgs &a, const ProbView &pv, int id, doubl
         const int64_t j = cut_dirac<2, PER>(a, id, q);
         la = q[0] - c0x;
         lb = q[1] - c0y;
-        lc = 0.5 * (la * la + lb * lb + w0 - a.sw[j]);
+        lc = 0.5 * (sq2(la, lb) + w0 - a.sw[j]);
         return;
     }
     if (id <= -100) {   // plane -100 - id of the domain item (n . x <= h)
         const double *pl = a.it_planes + 4 * (-100 - id);
         la = pl[0];
         lb = pl[1];
-        lc = pl[3] - pl[0] * c0x - pl[1] * c0y;
+        lc = fma(-pl[1], c0y, fma(-pl[0], c0x, pl[3]));
         return;
     }
     const double sq = ball_square(w0);
@@ -435,6 +442,17 @@ SDOT_HD inline bool clip_line(const IdPoly2<PC> &in, IdPoly2<PC> &out, double la
     return fits;
 }
 
+// Resume slots (CellArgs::rs_i / rs_d): the FP64 tier of RS_CAP vertices saves a
+// cell that outgrows it (its polygon before the cut that did not fit and where the
+// traversal stood), and the FP64 tier of 2 RS_CAP vertices goes on from there instead
+// of starting the cell again.  Bitwise the result of starting again: the same cuts on
+// the same polygon in the same order.
+constexpr int RS_CAP = 8;
+template <class T, int CAP>
+constexpr bool rs_save() { return sizeof(T) == 8 && CAP == RS_CAP; }
+template <class T, int CAP>
+constexpr bool rs_load() { return sizeof(T) == 8 && CAP == 2 * RS_CAP; }
+
 // BATCH = false: a single problem (CellArgs::prob is null), whose grid and domain
 // then stay uniform values (scalar registers) instead of per-thread copies.
 // PER: the cell is also cut by the translated diracs (a.ntrans translations).
@@ -442,10 +460,12 @@ SDOT_HD inline bool clip_line(const IdPoly2<PC> &in, IdPoly2<PC> &out, double la
 // RF: the final integration function (Integrator).
 template <class T, int CAP, bool BALL, bool IMG = false, bool BATCH = true, bool PER = false, bool JAC = false,
           int RF = 0>
-SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, int *I, const int S, const int SI) {
+SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, int *I, const int S, const int SI,
+                             const int64_t wk = -1) {
 #define PX(k) X[(k) * S]
 #define PY(k) Y[(k) * S]
 #define PI_(k) I[(k) * SI]
+    constexpr bool SAVE = rs_save<T, CAP>(), LOAD = rs_load<T, CAP>();
 
     const double c0x = a.spos[2 * t], c0y = a.spos[2 * t + 1], w0 = a.sw[t];
     ProbView pv;
@@ -461,27 +481,52 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     // square slightly larger than the disc (sides -5..-8): P & square & disc = P & disc,
     // and every edge left is short, which keeps the chord/circle intersections of the
     // ball clip well conditioned.
-    double bx0 = pv.lo[0] - c0x, by0 = pv.lo[1] - c0y, bx1 = pv.hi[0] - c0x, by1 = pv.hi[1] - c0y;
-    int eb = -1, er = -2, et = -3, el = -4;
     int n = 4;
-    if (BALL) {
-        const double sq = ball_square(w0);
-        if (-sq > by0) { by0 = -sq; eb = -8; }
-        if (sq < bx1) { bx1 = sq; er = -5; }
-        if (sq < by1) { by1 = sq; et = -6; }
-        if (-sq > bx0) { bx0 = -sq; el = -7; }
-        if (!(w0 > 0) || !(bx0 < bx1) || !(by0 < by1)) n = 0;   // no disc, or disjoint from the box
+    // a resumed cell: its saved polygon and the candidate to go on from
+    bool res = false;
+    int rs_s = -1, rs_r = 0, rs_gy = 0, rs_gx = 0, rs_j = 0;
+    if (LOAD && a.rs_hdr && wk >= 0 && wk < a.rs_cap) {
+        const int4 h = *reinterpret_cast<const int4 *>(a.rs_hdr + 4 * t);
+        const int rn = h.x & 255;
+        if (rn > 0) {
+            const double *d = a.rs_d + 2 * RS_CAP * wk;
+            const int *e = a.rs_i + RS_CAP * wk;
+            res = true;
+            n = rn;
+            rs_s = (h.x >> 8) - 1;
+            rs_r = h.y;
+            rs_gy = h.z & 0xffff;
+            rs_gx = h.z >> 16;
+            rs_j = h.w;
+            for (int k = 0; k < rn; ++k) {
+                PX(k) = T(d[2 * k]);
+                PY(k) = T(d[2 * k + 1]);
+                PI_(k) = e[k];
+            }
+        }
     }
-    PX(0) = T(bx0); PY(0) = T(by0); PI_(0) = eb;
-    PX(1) = T(bx1); PY(1) = T(by0); PI_(1) = er;
-    PX(2) = T(bx1); PY(2) = T(by1); PI_(2) = et;
-    PX(3) = T(bx0); PY(3) = T(by1); PI_(3) = el;
+    if (!res) {
+        double bx0 = pv.lo[0] - c0x, by0 = pv.lo[1] - c0y, bx1 = pv.hi[0] - c0x, by1 = pv.hi[1] - c0y;
+        int eb = -1, er = -2, et = -3, el = -4;
+        if (BALL) {
+            const double sq = ball_square(w0);
+            if (-sq > by0) { by0 = -sq; eb = -8; }
+            if (sq < bx1) { bx1 = sq; er = -5; }
+            if (sq < by1) { by1 = sq; et = -6; }
+            if (-sq > bx0) { bx0 = -sq; el = -7; }
+            if (!(w0 > 0) || !(bx0 < bx1) || !(by0 < by1)) n = 0;   // no disc, or disjoint from the box
+        }
+        PX(0) = T(bx0); PY(0) = T(by0); PI_(0) = eb;
+        PX(1) = T(bx1); PY(1) = T(by0); PI_(1) = er;
+        PX(2) = T(bx1); PY(2) = T(by1); PI_(2) = et;
+        PX(3) = T(bx0); PY(3) = T(by1); PI_(3) = el;
+    }
 
     auto radius2 = [&]() {
         double R2 = 0;
         for (int k = 0; k < n; ++k) {
             double x = PX(k), y = PY(k);
-            R2 = fmax(R2, x * x + y * y);
+            R2 = fmax(R2, sq2(x, y));
         }
         if (BALL) R2 = fmin(R2, fmax(w0, 0.0) * (1 + 1e-6));
         return R2;
@@ -493,10 +538,10 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         line_of<PER>(a, pv, PI_(k == 0 ? n - 1 : k - 1), c0x, c0y, w0, a1, b1, e1);
         line_of<PER>(a, pv, PI_(k), c0x, c0y, w0, a2, b2, e2);
         double x = (double)PX(k), y = (double)PY(k);
-        const double det = a1 * b2 - a2 * b1;
+        const double det = det2(a1, a2, b1, b2);
         if (fabs(det) > 1e-12 * (fabs(a1 * b2) + fabs(a2 * b1))) {
-            x = (e1 * b2 - e2 * b1) / det;
-            y = (a1 * e2 - a2 * e1) / det;
+            x = det2(e1, e2, b1, b2) / det;
+            y = det2(a1, a2, e1, e2) / det;
         }
         return hh - (x * vx + y * vy);
     };
@@ -547,9 +592,9 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         auto dval = [&](int k) { return exact ? T(side64(k, vx64, vy64, h64)) : dist(k); };
         const T dAin = dval(i0), dAout = dval(i1), dBout = dval(i2), dBin = dval(i3);
         const T m0 = dAin / (dAout - dAin);
-        const T Ax = PX(i0) - m0 * (PX(i1) - PX(i0)), Ay = PY(i0) - m0 * (PY(i1) - PY(i0));
+        const T Ax = fma(-m0, PX(i1) - PX(i0), PX(i0)), Ay = fma(-m0, PY(i1) - PY(i0), PY(i0));
         const T m2 = dBin / (dBout - dBin);
-        const T Bx = PX(i3) - m2 * (PX(i2) - PX(i3)), By = PY(i3) - m2 * (PY(i2) - PY(i3));
+        const T Bx = fma(-m2, PX(i2) - PX(i3), PX(i3)), By = fma(-m2, PY(i2) - PY(i3), PY(i3));
         const int idB = PI_(i2);
         if (i1 <= i2) {
             const int shift = nout - 2;
@@ -571,23 +616,32 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
     };
     bool overflow = false;
     bool bad = false;
-    if (!BATCH && a.nitems == 1)   // one domain item: the starting cell is its box cut by its planes
+    // SAVE: where the traversal stands when the cell outgrows the tier (rn = 0: before
+    // the traversal, the next tier starts again); the caller keeps the polygon
+    auto save = [&](int rn, int sv_s, int sv_r, int sv_gy, int sv_gx, int sv_j) {
+        if (a.rs_hdr)
+            *reinterpret_cast<int4 *>(a.rs_hdr + 4 * t) = make_int4(rn | (sv_s + 1) << 8, sv_r, sv_gy | sv_gx << 16, sv_j);
+    };
+    if (!BATCH && a.nitems == 1 && !res)   // one domain item: the starting cell is its box cut by its planes
         for (int q = a.it_off[0]; q < a.it_off[1] && n > 0 && !overflow; ++q) {
             const double *pl = a.it_planes + 4 * q;
-            const double hl = pl[3] - pl[0] * c0x - pl[1] * c0y;
-            if (cut(T(pl[0]), T(pl[1]), T(hl), -100 - q, pl[0], pl[1], hl) == 2) overflow = true;
+            const double hl = fma(-pl[1], c0y, fma(-pl[0], c0x, pl[3]));
+            if (cut(T(pl[0]), T(pl[1]), T(hl), -100 - q, pl[0], pl[1], hl) == 2) {
+                overflow = true;
+                if (SAVE) save(0, 0, 0, 0, 0, 0);
+            }
         }
     double R2 = radius2();
 
     const int maxr = g.ncell[0] > g.ncell[1] ? g.ncell[0] : g.ncell[1];
     // s = -1: the diracs themselves; s >= 0 (PER): the diracs translated by trans[s],
     // found around q = c0 - trans[s] in the grid's frame (SpZGrid.tcc:558-561,610)
-    for (int s = -1; s < (PER ? a.ntrans : 0) && n > 0 && !overflow; ++s) {
+    for (int s = res ? rs_s : -1; s < (PER ? a.ntrans : 0) && n > 0 && !overflow; ++s) {
     const double qx = s < 0 ? c0x : c0x - a.trans[2 * s], qy = s < 0 ? c0y : c0y - a.trans[2 * s + 1];
     const int idbase = s < 0 ? 0 : (s + 1) * (int)a.N;
     const int hx = grid_coord(g, 0, qx), hy = grid_coord(g, 1, qy);
-    for (int r = 0; r <= maxr && n > 0 && !overflow; ++r) {
-        if (r > 0) {
+    for (int r = res ? rs_r : 0; r <= maxr && n > 0 && !overflow; ++r) {
+        if (r > 0 && !res) {
             bool lf = hx - r + 1 > 0, rt = hx + r < g.ncell[0], bt = hy - r + 1 > 0, tp = hy + r < g.ncell[1];
             if (!(lf || rt || bt || tp)) break;
             double dm = DBL_MAX;
@@ -600,26 +654,28 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         }
         // the in-range cells of ring r: full rows at dy = +-r, the two side columns in between
         const int gy0 = hy - r < 0 ? 0 : hy - r, gy1 = hy + r >= g.ncell[1] ? g.ncell[1] - 1 : hy + r;
-        for (int gy = gy0; gy <= gy1 && n > 0 && !overflow; ++gy) {
+        for (int gy = res ? rs_gy : gy0; gy <= gy1 && n > 0 && !overflow; ++gy) {
           const bool full_row = gy == hy - r || gy == hy + r;
           const int gxa = full_row ? (hx - r < 0 ? 0 : hx - r) : hx - r;
           const int gxb = full_row ? (hx + r >= g.ncell[0] ? g.ncell[0] - 1 : hx + r) : hx + r;
           const int step = full_row ? 1 : 2 * r;
-          for (int gx = gxa; gx <= gxb && n > 0 && !overflow; gx += step) {
+          for (int gx = res ? rs_gx : gxa; gx <= gxb && n > 0 && !overflow; gx += step) {
             if (gx < 0 || gx >= g.ncell[0]) continue;
             const uint32_t code = morton2(gx, gy);
             const uint32_t cs = pv.cell_start[code], ce = pv.cell_start[code + 1];
             if (cs == ce) continue;
-            {
+            if (!res) {
                 double bx0 = g.lo[0] + gx * g.h[0], by0 = g.lo[1] + gy * g.h[1];
                 double ex = fmax(0.0, fmax(bx0 - qx, qx - (bx0 + g.h[0])));
                 double ey = fmax(0.0, fmax(by0 - qy, qy - (by0 + g.h[1])));
                 if (region_skippable<T>(ex * ex + ey * ey, R2, w0, pv.cell_wmax[code])) continue;
             }
+            const uint32_t j0 = res ? (uint32_t)rs_j : cs;
+            res = false;
             // the next candidate's position and weight are loaded while the current one is
             // tested and cut (software pipelining: the loads' latency overlaps the cut)
-            double nx = a.spos[2 * (int64_t)cs], ny = a.spos[2 * (int64_t)cs + 1], nw = a.sw[cs];
-            for (uint32_t j = cs; j < ce; ++j) {
+            double nx = a.spos[2 * (int64_t)j0], ny = a.spos[2 * (int64_t)j0 + 1], nw = a.sw[j0];
+            for (uint32_t j = j0; j < ce; ++j) {
                 const double px = nx, py = ny, pw = nw;
                 if (j + 1 < ce) {
                     nx = a.spos[2 * (int64_t)(j + 1)];
@@ -628,13 +684,17 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                 }
                 if ((int64_t)j == t && s < 0) continue;
                 const double vx = px - qx, vy = py - qy;
-                const double vv = vx * vx + vy * vy;
+                const double vv = sq2(vx, vy);
                 const double hh = 0.5 * (vv + w0 - pw);
                 if (hh > 0 && R2 * vv <= hh * hh * (1 - Eps<T>::cand)) continue;   // cannot cut
                 if (vv == 0) { bad = true; continue; }                             // coincident diracs
                 const T Vx = T(vx), Vy = T(vy), H = T(hh);
                 const int rc = cut(Vx, Vy, H, (int)j + idbase, vx, vy, hh);
-                if (rc == 2) { overflow = true; break; }
+                if (rc == 2) {
+                    overflow = true;
+                    if (SAVE) save(n, s, r, gy, gx, (int)j);
+                    break;
+                }
                 if (rc == 0) continue;
                 if (n == 0) break;
                 R2 = radius2();
@@ -666,10 +726,10 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             double a1, b1, e1, a2, b2, e2;
             line_of<PER>(a, pv, PI_(k == 0 ? n - 1 : k - 1), c0x, c0y, w0, a1, b1, e1);
             line_of<PER>(a, pv, PI_(k), c0x, c0y, w0, a2, b2, e2);
-            double det = a1 * b2 - a2 * b1;
+            double det = det2(a1, a2, b1, b2);
             if (fabs(det) > 1e-12 * (fabs(a1 * b2) + fabs(a2 * b1))) {
-                x = (e1 * b2 - e2 * b1) / det;
-                y = (a1 * e2 - a2 * e1) / det;
+                x = det2(e1, e2, b1, b2) / det;
+                y = det2(a1, a2, e1, e2) / det;
             }
         }
     };

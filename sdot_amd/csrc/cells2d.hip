@@ -116,7 +116,7 @@ __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
         }
         const bool ovf = t >= 0 && cell2_body<T, CAP, BALL, IMG, BATCH, PER, JAC, RF>(
                                        a, t, sxy + 2 * threadIdx.x, sxy + 2 * threadIdx.x + 1, sid + threadIdx.x,
-                                       2 * BS, BS);
+                                       2 * BS, BS, a.work ? k : -1);
         if (a.ovf_list) {
             // the wavefront's overflowing cells go to the list together, in lane (hence
             // grid) order: the next tier's wavefronts then read runs of nearby cells
@@ -126,7 +126,22 @@ __global__ __launch_bounds__(BS, WAVES) void k_cells2(CellArgs a) {
                 unsigned base = 0;
                 if (lane == leader) base = atomicAdd(a.ovf_count, (unsigned)__popcll(m));
                 base = __shfl(base, leader);
-                if (ovf) a.ovf_list[base + __popcll(m & ((1ull << lane) - 1))] = (int)t;
+                const unsigned q = base + __popcll(m & ((1ull << lane) - 1));
+                if (ovf) {
+                    a.ovf_list[q] = (int)t;
+                    if (rs_save<T, CAP>() && a.rs_hdr && (int64_t)q < a.rs_cap) {
+                        // the polygon before the cut that did not fit is still in the slot
+                        const T *xy = sxy + 2 * threadIdx.x;
+                        const int *id = sid + threadIdx.x;
+                        double *d = a.rs_d + 2 * RS_CAP * (int64_t)q;
+                        int *e = a.rs_i + RS_CAP * (int64_t)q;
+                        for (int f = 0; f < CAP; ++f) {
+                            d[2 * f] = (double)xy[f * 2 * BS];
+                            d[2 * f + 1] = (double)xy[f * 2 * BS + 1];
+                            e[f] = id[f * BS];
+                        }
+                    }
+                }
             }
         } else if (ovf) {
             flag_or(a.flags, FLAG_CELL_OVERFLOW);
@@ -363,6 +378,35 @@ void dispatch2(bool ball, bool img, const CellArgs &a, unsigned nb, hipStream_t 
 }
 
 }  // namespace
+
+namespace {
+// the first N / 4 + 1024 cells that outgrow the FP64 8-vertex tier get a resume slot
+// (about 9 % of the cells of a 1M ball-clipped pass outgrow it); the others start again
+int64_t resume_cap(int64_t N) { return std::min<int64_t>(N, N / 4 + 1024); }
+}  // namespace
+
+void ensure_resume(sdot_pd *pd, int64_t N) {
+    const int64_t C = resume_cap(N);
+    pd->rs_hdr.ensure(4 * (size_t)N);
+    pd->rs_i.ensure((size_t)RS_CAP * C);
+    pd->rs_d.ensure((size_t)(2 * RS_CAP) * C);
+}
+
+void use_resume(const sdot_pd *pd, CellArgs &a) {
+    const int64_t C = resume_cap(a.N);
+    if (pd->rs_hdr.p && pd->rs_i.p && pd->rs_d.p && pd->rs_hdr.n >= 4 * (size_t)a.N &&
+        pd->rs_i.n >= (size_t)RS_CAP * C && pd->rs_d.n >= (size_t)(2 * RS_CAP) * C) {
+        a.rs_hdr = pd->rs_hdr.p;
+        a.rs_i = pd->rs_i.p;
+        a.rs_d = pd->rs_d.p;
+        a.rs_cap = C;
+    } else {
+        a.rs_hdr = nullptr;
+        a.rs_i = nullptr;
+        a.rs_d = nullptr;
+        a.rs_cap = 0;
+    }
+}
 
 // Capacity tiers, all on the stream with no host round trip: every cell with a
 // small polygon slot in LDS, the cells that outgrew it again with a larger one,

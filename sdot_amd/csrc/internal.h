@@ -230,6 +230,16 @@ struct CellArgs {
     int stop_if_void;
     int *ovf_list;
     unsigned *ovf_count;
+    // resume slots of the FP64 8-vertex tier (null: a cell that outgrows it starts
+    // again in the next tier).  Sorted cell t that outgrows it leaves where its
+    // traversal stood in rs_hdr[4 t .. 4 t + 4) (vertex count | (image + 1) << 8, or 0
+    // to start again; ring; grid row | column << 16; candidate), and, at position
+    // k < rs_cap of the overflow list, its polygon: x and y of vertex f < 8 at
+    // rs_d[16 k + 2 f], rs_d[16 k + 2 f + 1], its edge id at rs_i[8 k + f]
+    int *rs_hdr;
+    int *rs_i;
+    double *rs_d;
+    int64_t rs_cap;
     const int *work;             // optional list of sorted indices to process
     int64_t work_n;              // its length (3D; host-known)
     const unsigned *work_count;  // its length in device memory (2D tiers; no host round trip)
@@ -398,6 +408,8 @@ struct sdot_pd {
     bool want_mom = false;          // the next pass computes the first moments
     bool pass_has_mom = false;      // the current pass did
     sdot_amd::DevBuf<int> row_len, ell_col, ovf_list;
+    sdot_amd::DevBuf<int> rs_hdr, rs_i;   // resume slots of the FP64 8-vertex 2D tier
+    sdot_amd::DevBuf<double> rs_d;
     sdot_amd::DevBuf<unsigned> flags;  // [0] flags, [1] overflow count, [2] second-tier overflow count
     int ell_cap = 0;
     unsigned last_flags = 0;
@@ -474,6 +486,10 @@ int64_t build_csr_device(sdot_pd *pd);
 // display.hip: the cells of the last pass as a legacy ASCII VTK file (pass valid)
 void write_vtk(sdot_pd *pd, const char *filename, const char *field_name);  // dirac-order CSR in HBM (csr_off/csr_col/csr_val); returns slot bound
 void run_cells2(sdot_pd *pd, CellArgs &a);
+// resume slots of the FP64 2D tiers for N cells: ensure_resume allocates them (not
+// under graph capture), use_resume points the pass's arguments at them when they fit
+void ensure_resume(sdot_pd *pd, int64_t N);
+void use_resume(const sdot_pd *pd, CellArgs &a);
 void run_cells3(sdot_pd *pd, CellArgs &a);
 void fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
 void export_csr(sdot_pd *pd, int64_t *h_off, int64_t *h_col, double *h_val, int64_t cap, int64_t *nnz);

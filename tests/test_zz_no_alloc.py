@@ -2,7 +2,10 @@
 sdot_pd_integrals on the same inputs: bitwise-equal masses (the same grid and cell
 kernels run on the same data), on two calls in a row with the second smaller, in
 both precisions, with and without the ball clip; no change of the device's free
-memory across a call; sizes beyond the fixed workspace rejected."""
+memory across a call; sizes beyond the fixed workspace rejected.  In FP64 the
+handle's pass goes on with the cells that outgrow the 8-vertex tier from their saved
+polygons (resume slots) while the no-alloc pass, which has no slots, starts them
+again: the masses are bitwise equal all the same."""
 import numpy as np
 import pytest
 
@@ -31,6 +34,8 @@ def test_noalloc_masses_equal_handle_masses(precision, ball):
         ref.set_diracs(pos, w)
         m_ref = ref.integrals()
         st_ref = ref.last_status
+        if precision == "fp64":   # cells resumed in the handle's pass, started again here
+            assert ref.last_timings()["nb_cells_overflow"] > 0
         dp, dw, dm = DeviceArray.from_host(pos), DeviceArray.from_host(w), DeviceArray(n)
         dm.fill(-1.0)
         free0 = free_device_memory()
