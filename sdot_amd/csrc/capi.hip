@@ -289,6 +289,7 @@ sdot_status compute_pass(sdot_pd *pd) {
     // a pass that ends in an error is never reused: the next getter runs it again and
     // reports the same error instead of returning the partial results
     pd->pass_valid = !(fl & (FLAG_BAD_INPUT | FLAG_CELL_OVERFLOW | FLAG_ROW_OVERFLOW));
+    if (!pd->pass_valid) pd->hint_ok = false;
     if (fl & FLAG_BAD_INPUT) throw StatusError{SDOT_INVALID_ARG, "non-finite weight or coincident diracs"};
     if (pd->stop_if_void && (fl & FLAG_EMPTY)) {   // stopped early: the per-cell results are partial
         pd->pass_valid = false;
@@ -303,6 +304,12 @@ sdot_status compute_pass(sdot_pd *pd) {
             return compute_pass(pd);
         }
         throw StatusError{SDOT_CAPACITY_OVERFLOW, "Hessian row longer than the row capacity"};
+    }
+    // a complete pass of a Newton solve in 3D: its neighbours start the next passes (a pass
+    // stopped at an empty cell leaves the snapshot as it was, whatever rows it rewrote)
+    if (pd->warm && pd->dim == 3) {
+        snapshot_hints(pd);
+        pd->hint_ok = true;
     }
     return (fl & FLAG_EMPTY) ? SDOT_EMPTY_CELL : SDOT_OK;
 }
@@ -381,7 +388,7 @@ sdot_status sdot_pd_destroy(sdot_pd *pd) {
     pd->spos.release(); pd->sw.release(); pd->cell_wmax.release(); pd->bbox_part.release(); pd->wpyr.release(); pd->wglob.release();
     pd->mass.release(); pd->arc.release(); pd->perim.release(); pd->diag.release(); pd->ell_val.release();
     pd->row_len.release(); pd->mom.release(); pd->ell_col.release(); pd->ovf_list.release(); pd->flags.release(); pd->wctr.release();
-    pd->rs_hdr.release(); pd->rs_i.release(); pd->rs_d.release();
+    pd->rs_hdr.release(); pd->rs_i.release(); pd->rs_d.release(); pd->hint_col.release();
     pd->csr_off.release(); pd->csr_col.release(); pd->csr_cnt.release(); pd->csr_tmp.release();
     pd->csr_val.release(); pd->out_tmp.release(); pd->scratch3.release();
     pd->csr_off2.release(); pd->csr_col2.release(); pd->csr_val2.release();

@@ -555,9 +555,55 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
         }
         double R2 = C.radius2();
         const int hh[3] = {grid_coord(g, 0, c0x), grid_coord(g, 1, c0y), grid_coord(g, 2, c0z)};
+        // Warm start (CellArgs::hint_col): this lane's neighbour of the cell in the last
+        // complete pass over the same sorted diracs.  They are cut first and skipped by the
+        // traversal: every dirac's plane bounds the cell, so the cut order changes, not the
+        // cell.
+        const bool use_hints = a.hint_col != nullptr;
+        int hj = -1;
+        if (use_hints && lane < HCAP3) {
+            const int j = a.hint_col[HCAP3 * t + lane];
+            if (j >= 0 && (int64_t)j < a.N && (int64_t)j != t) hj = j;
+        }
+        // the lanes' candidates j (-1: none): tested against the current radius, and only
+        // the ballot of possible cutters applied, in lane order
+        auto cut_lanes = [&](int j, bool skip_hints) {
+            double vx = 0, vy = 0, vz = 0, hv = 0, vv = 0;
+            bool cand = j >= 0 && (int64_t)j != t;
+            if (cand) {
+                vx = a.spos[3 * (int64_t)j] - c0x;
+                vy = a.spos[3 * (int64_t)j + 1] - c0y;
+                vz = a.spos[3 * (int64_t)j + 2] - c0z;
+                vv = vx * vx + vy * vy + vz * vz;
+                hv = 0.5 * (vv + w0 - a.sw[j]);
+                if (vv == 0) {
+                    bad = true;
+                    cand = false;
+                } else if (hv > 0 && R2 * vv <= hv * hv * (1 - c3::Eps3<T>::cand)) {
+                    cand = false;
+                }
+            }
+            unsigned long long mask = __ballot(cand);
+            while (mask && C.nv > 0) {
+                const int k = __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                const double ux = lane_value(vx, k), uy = lane_value(vy, k), uz = lane_value(vz, k),
+                             uh = lane_value(hv, k), uv = lane_value(vv, k);
+                const int uj = lane_value(j, k);
+                if (skip_hints && use_hints && __ballot(hj == uj)) continue;   // cut by the warm start
+                if (uh > 0 && R2 * uv <= uh * uh * (1 - c3::Eps3<T>::cand)) continue;   // R shrank
+                const double tie = c3::Eps3<T>::tie * sqrt(uv * R2);
+                const int rc = C.cut(T(ux), T(uy), T(uz), T(uh), uj, ux, uy, uz, uh, tie, true);
+                if (rc == 2) {
+                    overflow = true;
+                    break;
+                }
+                if (rc == 1 && C.nv > 0) R2 = C.radius2();
+            }
+        };
+        if (use_hints && C.nv > 0 && !overflow) cut_lanes(hj, false);
         // The diracs of the grid cells held by the lanes (ok, [cs, ce)) are staged 64 at a
-        // time (one per lane, loaded in parallel), tested against the current radius,
-        // and only the ballot of possible cutters is applied, in order.
+        // time (one per lane, loaded in parallel) and handed to cut_lanes.
         auto process_cells = [&](bool ok, uint32_t cs, uint32_t ce) {
             const int cnt = ok ? (int)(ce - cs) : 0;
             const int pre = wave_excl_scan(cnt, lane);
@@ -580,37 +626,7 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
                     while (L.ccnt[lo] == 0) --lo;
                     j = L.ccs[lo] + (gi - L.cpre[lo]);
                 }
-                double vx = 0, vy = 0, vz = 0, hv = 0, vv = 0;
-                bool cand = j >= 0 && (int64_t)j != t;
-                if (cand) {
-                    vx = a.spos[3 * (int64_t)j] - c0x;
-                    vy = a.spos[3 * (int64_t)j + 1] - c0y;
-                    vz = a.spos[3 * (int64_t)j + 2] - c0z;
-                    vv = vx * vx + vy * vy + vz * vz;
-                    hv = 0.5 * (vv + w0 - a.sw[j]);
-                    if (vv == 0) {
-                        bad = true;
-                        cand = false;
-                    } else if (hv > 0 && R2 * vv <= hv * hv * (1 - c3::Eps3<T>::cand)) {
-                        cand = false;
-                    }
-                }
-                unsigned long long mask = __ballot(cand);
-                while (mask && C.nv > 0) {
-                    const int k = __ffsll((long long)mask) - 1;
-                    mask &= mask - 1;
-                    const double ux = lane_value(vx, k), uy = lane_value(vy, k), uz = lane_value(vz, k),
-                                 uh = lane_value(hv, k), uv = lane_value(vv, k);
-                    const int uj = lane_value(j, k);
-                    if (uh > 0 && R2 * uv <= uh * uh * (1 - c3::Eps3<T>::cand)) continue;   // R shrank
-                    const double tie = c3::Eps3<T>::tie * sqrt(uv * R2);
-                    const int rc = C.cut(T(ux), T(uy), T(uz), T(uh), uj, ux, uy, uz, uh, tie, true);
-                    if (rc == 2) {
-                        overflow = true;
-                        break;
-                    }
-                    if (rc == 1 && C.nv > 0) R2 = C.radius2();
-                }
+                cut_lanes(j, true);
             }
             wave_sync();
         };
@@ -938,6 +954,18 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
     }
 }
 
+// hint_col[HCAP3 t + e] = sorted neighbour e of row t (-1 past the row), one thread per entry
+__global__ __launch_bounds__(256) void k_hint_copy(const int *row_len, const int *ell_col, int64_t stride, int cap,
+                                                  int64_t n, int *hint_col) {
+    const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (i >= n * HCAP3) return;
+    const int64_t t = i / HCAP3;
+    const int e = (int)(i % HCAP3);
+    int L = row_len[t];
+    L = L < cap ? L : cap;
+    hint_col[i] = e < L ? ell_col[e * stride + t] : -1;
+}
+
 template <class T>
 void launch_wave3(sdot_pd *pd, const CellArgs &a) {
     int dev = 0, ncu = 0;
@@ -953,6 +981,15 @@ void launch_wave3(sdot_pd *pd, const CellArgs &a) {
 }
 
 }  // namespace
+
+void snapshot_hints(sdot_pd *pd) {
+    const int64_t N = pd->N;
+    pd->hint_col.ensure((size_t)HCAP3 * N);
+    const int64_t tot = (int64_t)HCAP3 * N;
+    hipLaunchKernelGGL(k_hint_copy, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, pd->stream, pd->row_len.p,
+                       pd->ell_col.p, (int64_t)N, pd->ell_cap, N, pd->hint_col.p);
+    HIPCK(hipGetLastError());
+}
 
 void run_cells3(sdot_pd *pd, CellArgs &a) {
     hipStream_t s = pd->stream;
@@ -984,8 +1021,10 @@ void run_cells3(sdot_pd *pd, CellArgs &a) {
         if (mixed) launch_thread3<float, VCAP3, PCAP3, false>(pd, a);
         else launch_thread3<double, VCAP3, PCAP3, false>(pd, a);
     } else {
+        a.hint_col = pd->warm && pd->hint_ok && pd->hint_col.n >= (size_t)HCAP3 * a.N ? pd->hint_col.p : nullptr;
         if (mixed) launch_wave3<float>(pd, a);
         else launch_wave3<double>(pd, a);
+        a.hint_col = nullptr;
     }
     HIPCK(hipEventRecord(pd->ev[3], s));
     unsigned cnt[2] = {0, 0};

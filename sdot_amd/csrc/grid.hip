@@ -288,6 +288,7 @@ void sort_by_keys(sdot_pd *pd, int64_t ncodes) {
         hipLaunchKernelGGL(k_gather_pos<3>, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->pos.p, pd->perm.p, N, pd->spos.p, pd->inv_perm.p);
     HIPCK(hipGetLastError());
     pd->grid_valid = true;
+    pd->hint_ok = false;   // a new sorted order: the rows no longer name this order's diracs
 }
 
 GridInfo grid_params(int dim, const double *lo_in, const double *hi_in, int64_t N, double dpc) {

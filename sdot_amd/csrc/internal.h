@@ -150,6 +150,7 @@ enum : unsigned { FLAG_EMPTY = 1u, FLAG_ROW_OVERFLOW = 2u, FLAG_BAD_INPUT = 4u, 
 // Hessian row capacity (ELL slots per row): the first pass uses the narrow one;
 // a row overflow re-runs the pass with the wide one, which the handle then keeps
 constexpr int ELL_CAP2 = 32, ELL_CAP2_WIDE = 64, ELL_CAP3 = 128, ELL_CAP3_WIDE = 255;
+constexpr int HCAP3 = 32;   // warm-start neighbours kept per 3D cell
 
 #define SDOT_HD __host__ __device__
 
@@ -240,6 +241,10 @@ struct CellArgs {
     int *rs_i;
     double *rs_d;
     int64_t rs_cap;
+    // warm start of the 3D wavefront kernel (null: none): sorted cell t's neighbours in the
+    // last complete pass over the same sorted diracs, hint_col[HCAP3 t + e], e < HCAP3
+    // (-1: none), which the cell cuts first
+    const int *hint_col;
     const int *work;             // optional list of sorted indices to process
     int64_t work_n;              // its length (3D; host-known)
     const unsigned *work_count;  // its length in device memory (2D tiers; no host round trip)
@@ -384,6 +389,11 @@ struct sdot_pd {
     int64_t N = 0;
     sdot_amd::DevBuf<double> pos, w;
     bool grid_valid = false;    // positions unchanged since the last sort
+    // hint_col holds the neighbours of a complete 3D pass over the current sorted order
+    // (warm start of the next passes of a Newton solve, which sets `warm`)
+    bool hint_ok = false;
+    bool warm = false;
+    sdot_amd::DevBuf<int> hint_col;
     bool pass_valid = false;    // results match the current diracs/weights/settings
     bool stop_if_void = false;  // passes of a Newton solve: an empty cell ends the pass early
 
@@ -486,6 +496,8 @@ int64_t build_csr_device(sdot_pd *pd);
 // display.hip: the cells of the last pass as a legacy ASCII VTK file (pass valid)
 void write_vtk(sdot_pd *pd, const char *filename, const char *field_name);  // dirac-order CSR in HBM (csr_off/csr_col/csr_val); returns slot bound
 void run_cells2(sdot_pd *pd, CellArgs &a);
+// copies the first HCAP3 neighbours of every row of the pass just completed to hint_col
+void snapshot_hints(sdot_pd *pd);
 // resume slots of the FP64 2D tiers for N cells: ensure_resume allocates them (not
 // under graph capture), use_resume points the pass's arguments at them when they fit
 void ensure_resume(sdot_pd *pd, int64_t N);

@@ -163,6 +163,17 @@ sdot_status ot_solve(sdot_pd *pd, const double *targets, double tol, int stop_ru
         explicit StopIfVoid(sdot_pd *p) : pd(p) { pd->stop_if_void = true; }
         ~StopIfVoid() { pd->stop_if_void = false; }
     } stop_guard(pd);
+    // 3D: each pass after the first starts its cells from their neighbours in the
+    // previous pass (the rows the PCG has just used); the first pass starts cold, so a
+    // solve depends on its inputs only
+    struct WarmStart {
+        sdot_pd *pd;
+        explicit WarmStart(sdot_pd *p) : pd(p) {
+            pd->hint_ok = false;
+            pd->warm = true;
+        }
+        ~WarmStart() { pd->warm = false; }
+    } warm_guard(pd);
     for (int it = 0; it < max_iter; ++it) {
         sdot_status ps = compute_pass(pd);
         ms_cells += pd->last.ms_total;
