@@ -91,3 +91,37 @@ def test_newton_3d_image(precision):
     m = O.cells(pos, w, img=img)["masses"]
     assert np.abs(m / t - 1).max() < 1.5e-6
     print(f"[newton3d] {precision}: {ot.stats}")
+
+
+def test_newton_3d_warm_start_depends_on_the_inputs_only():
+    """The passes of a 3D Newton solve after its first start each cell from its
+    neighbours in the last complete pass (cut first, skipped by the traversal).  Targets
+    far from the start make the solve backtrack, so some of its passes stop at an empty
+    cell.  The weights are bitwise the same on a fresh handle and on a handle that ran
+    other passes and another solve before; a plain pass afterwards is bitwise the pass of
+    a fresh handle; the masses at the weights match the oracle's."""
+    import sdot_amd
+    n = 4000
+    pos, _ = inputs(3, n, 31)
+    t = 1 + 20 * O.rng_uniform(32, n) ** 4
+    t /= t.sum()
+
+    def handle(w):
+        pd = sdot_amd.PowerDiagram(3, "fp64")
+        pd.add_box_shape([0, 0, 0], [1, 1, 1])
+        pd.set_diracs(pos, w)
+        return pd
+
+    pd1 = handle(np.zeros(n))
+    st, w1, s1 = pd1.ot_solve(t, np.zeros(n), tol=1e-7)
+    assert st == 0 and s1["backtracks"] >= 1, s1
+    pd1.set_weights(1e-3 * O.rng_uniform(33, n))
+    pd1.integrals()
+    st, w1b, _ = pd1.ot_solve(t, np.zeros(n), tol=1e-7)
+    st2, w2, _ = handle(np.zeros(n)).ot_solve(t, np.zeros(n), tol=1e-7)
+    assert st == 0 and st2 == 0
+    assert np.array_equal(w1, w1b) and np.array_equal(w1, w2)
+    pd1.set_weights(w1)
+    assert np.array_equal(pd1.integrals(), handle(w1).integrals())
+    m = O.cells(pos, w1)["masses"]
+    assert np.abs(m / t - 1).max() < 2e-7
