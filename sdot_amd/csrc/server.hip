@@ -34,7 +34,8 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr double BYTES_PER_DIRAC_3D = 3600.0;   // rows at the wide capacity, pass outputs, PCG vectors
+// rows at the wide capacity, pass outputs, PCG vectors, warm-start neighbours (128 B)
+constexpr double BYTES_PER_DIRAC_3D = 3750.0;
 constexpr double HANDLE_3D_BYTES = 1.6e9;        // thread-kernel scratch slots of one 3D handle
 
 struct Request {
