@@ -493,8 +493,9 @@ BatchEngine::~BatchEngine() {
 double BatchEngine::bytes_per_dirac() {
     // dirac arrays (pos, w, w0, old, tgt, spare copies for a relayout), sort keys and
     // permutations, sorted copies, pass outputs, ELL rows at the wide capacity, PCG vectors,
-    // FP64 resume slots (184 bytes for a quarter of the cells)
-    return 16 + 8 * 4 + 8 * 5 + 8 + 8 + 24 + 32 + 4 + ELL_CAP2_WIDE * 12.0 + 8 * 7 + 8 + 16 + 46;
+    // FP64 resume slots (184 bytes for a quarter of the cells), FP32 copies of the matrix
+    // values for the multigrid smoothers (32 entries per row on level 0, 64 per coarse row)
+    return 16 + 8 * 4 + 8 * 5 + 8 + 8 + 24 + 32 + 4 + ELL_CAP2_WIDE * 12.0 + 8 * 7 + 8 + 16 + 46 + 128 + 86;
 }
 
 size_t BatchEngine::device_bytes() const { return (size_t)(bytes_per_dirac() * (double)N_); }
@@ -858,7 +859,7 @@ int BatchEngine::tick(const TickParams &tp, std::vector<int> &finished, TickStat
         std::vector<int> active(K), its;
         for (int64_t k = 0; k < K; ++k) active[k] = act[k] == ACT_ACCEPT;
         PcgStats ps;
-        im->pcg.solve(PcgMat{pd_->diag.p, pd_->row_len.p, pd_->ell_col.p, pd_->ell_val.p, N}, im->b.p, im->x.p,
+        im->pcg.solve(PcgMat{pd_->diag.p, pd_->row_len.p, pd_->ell_col.p, pd_->ell_val.p, N, pd_->ell_cap}, im->b.p, im->x.p,
                       im->perr.p, active, 100000, its, ps);
         hipLaunchKernelGGL(kb_update, dim3((unsigned)im->nblk), dim3(BB), 0, s, blocks_of(im), im->x.p, pd_->perm.p,
                            pd_->w.p, im->act.p, im->red.p);
@@ -918,7 +919,7 @@ void BatchEngine::enqueue_replay(const TickParams &tp, int cg_per_replay) {
     hipStream_t s = pd->stream;
     const int64_t K = (int64_t)probs.size(), N = N_;
     const NewtonDev nd = im->newton();
-    const PcgMat A{pd->diag.p, pd->row_len.p, pd->ell_col.p, pd->ell_val.p, N};
+    const PcgMat A{pd->diag.p, pd->row_len.p, pd->ell_col.p, pd->ell_val.p, N, pd->ell_cap};
     enqueue_pass(pd, im, K, N, nd.state);
     hipLaunchKernelGGL(kb_residual, dim3((unsigned)im->nblk), dim3(BB), 0, s, blocks_of(im), pd->mass.p, im->tsort.p,
                        im->b.p, im->red.p, nd.state);

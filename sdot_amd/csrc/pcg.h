@@ -35,6 +35,10 @@ struct PcgMat {                 // rows in grid order
     const int *col;
     const double *val;
     int64_t stride;
+    int cap = 0;                // entries per row of the layout (0: unknown)
+    // FP32 copy of every entry (null: none), written by the coarsening that reads the rows;
+    // the V-cycle's smoothers read it, the CG's own product reads val
+    float *val32 = nullptr;
 };
 
 struct PcgStats {

@@ -19,6 +19,7 @@ constexpr int CHUNK = 8;         // PCG iterations per graph launch
 constexpr int AGG = 4;           // rows per aggregate
 constexpr int CMIN = 64;         // coarsest level: at most CMIN rows per problem
 constexpr int CCAP = 64;         // off-diagonal entries per coarse row
+constexpr int V32_0 = ELL_CAP2;  // 2D level 0: entries per row of the FP32 copy
 constexpr double OMEGA = 0.8;    // damped-Jacobi smoothing
 constexpr double CSCALE = 1.5;   // coarse-correction scaling
 
@@ -106,6 +107,7 @@ __global__ __launch_bounds__(256) void kp_coarsen(PcgMat A, const int64_t *offl,
         for (int e = 0; e < L; ++e) {
             const int c = A.col[e * A.stride + t];
             const double v = A.val[e * A.stride + t];
+            if (A.val32 && sub == 0) A.val32[e * A.stride + t] = (float)v;
             const int cc = (int)(offc[p] + (c - offl[p]) / AGG);
             if (cc == I) {
                 d += v;
@@ -214,6 +216,10 @@ __device__ inline bool skip(const double *S, const int *amg_off, int p, int gate
     return gated_out(S, p, gate) || amg_off[p];
 }
 
+// matrix entry k (= e * stride + t) as the V-cycle's smoothers read it: the FP32 copy when
+// the level has one (the preconditioner approximates A^-1 anyway; the CG uses A in FP64)
+__device__ inline double smooth_val(const PcgMat &A, int64_t k) { return A.val32 ? (double)A.val32[k] : A.val[k]; }
+
 // sum over the LPR consecutive lanes of a row (LPR a power of two, at most 64)
 template <int LPR>
 __device__ inline double row_sum(double v) {
@@ -238,7 +244,7 @@ __global__ __launch_bounds__(BB) void kp_smooth_res(LevelView F, const int *plev
     double ax = 0;
     for (int e = sub; e < L; e += LPR) {
         const int c = F.A.col[e * F.A.stride + t];
-        ax += F.A.val[e * F.A.stride + t] * F.xpre[c];
+        ax += smooth_val(F.A, e * F.A.stride + t) * F.xpre[c];
     }
     ax = row_sum<LPR>(ax);
     if (sub == 0) res[t] = F.r[t] - (F.A.diag[t] * F.xpre[t] + ax);
@@ -313,7 +319,7 @@ __global__ __launch_bounds__(BB) void kp_up(LevelView F, LevelView C, const doub
     double ay = 0;
     for (int e = sub; e < L; e += LPR) {
         const int c = F.A.col[e * F.A.stride + t];
-        ay += F.A.val[e * F.A.stride + t] * prolonged<YIN>(F, C, y, p, c);
+        ay += smooth_val(F.A, e * F.A.stride + t) * prolonged<YIN>(F, C, y, p, c);
     }
     ay = row_sum<LPR>(ay);
     if (sub == 0) {
@@ -349,7 +355,7 @@ __global__ __launch_bounds__(BB) void kp_up0(LevelView F, LevelView C, const dou
             const int L = F.A.len[t];
             for (int e = 0; e < L; ++e) {
                 const int c = F.A.col[e * F.A.stride + t];
-                ay += F.A.val[e * F.A.stride + t] * prolonged<YIN>(F, C, y, p, c);
+                ay += smooth_val(F.A, e * F.A.stride + t) * prolonged<YIN>(F, C, y, p, c);
             }
             z = yt + OMEGA * (F.r[t] - ay) / F.A.diag[t];
             F.out[t] = z;
@@ -530,6 +536,8 @@ void launch_up(int lpr, bool yin, const LevelView &F, const LevelView &C, const 
 
 struct PcgSolver::Impl {
     RowPlan plan;
+    bool v32 = false;                      // FP32 copies of the matrix values for the smoothers (2D)
+    DevBuf<float> val32_0;                 // level 0's copy (V32_0 entries per row)
     hipStream_t s = nullptr;
     int64_t K = 0, N = 0, nblk = 0;
     std::vector<int64_t> off, blk_off;
@@ -543,6 +551,7 @@ struct PcgSolver::Impl {
         DevBuf<int64_t> doff;
         DevBuf<int> prob, len, col;
         DevBuf<double> diag, val;
+        DevBuf<float> val32;
     };
     std::vector<Level> lv;                 // lv[0] unused placeholder for level 0
     std::vector<int> plev;                 // depth (coarsest level) of each problem
@@ -584,7 +593,9 @@ PcgSolver::~PcgSolver() {
         l.col.release();
         l.diag.release();
         l.val.release();
+        l.val32.release();
     }
+    im->val32_0.release();
     delete im;
 }
 
@@ -654,7 +665,9 @@ void PcgSolver::set_layout(const std::vector<int64_t> &off, hipStream_t s) {
         V.len.ensure(V.n);
         V.col.ensure((size_t)CCAP * V.n);
         V.val.ensure((size_t)CCAP * V.n);
+        if (I.v32) V.val32.ensure((size_t)CCAP * V.n);
     }
+    if (I.v32 && I.L > 0) I.val32_0.ensure((size_t)V32_0 * N);
     for (auto *b : {&I.vr, &I.vxpre, &I.vout}) b->ensure(base);
     // coarsest rows of each problem
     std::vector<int64_t> coff(K), cbase(K);
@@ -685,6 +698,10 @@ LevelView level_view(PcgSolver::Impl &I, const PcgMat &A0, int l, double *r0, do
     v.lev = l;
     if (l == 0) {
         v.A = A0;
+        // the FP32 copy, when it holds the caller's whole layout (ELL stride = row count)
+        const bool c32 = I.v32 && A0.cap > 0 && A0.cap <= V32_0 && A0.stride == I.N &&
+                         I.val32_0.n >= (size_t)V32_0 * I.N;
+        v.A.val32 = c32 ? I.val32_0.p : nullptr;
         v.off = I.doff.p;
         v.prob = I.row_prob.p;
         v.r = r0;
@@ -693,7 +710,7 @@ LevelView level_view(PcgSolver::Impl &I, const PcgMat &A0, int l, double *r0, do
         v.n = I.N;
     } else {
         PcgSolver::Impl::Level &V = I.lv[l];
-        v.A = PcgMat{V.diag.p, V.len.p, V.col.p, V.val.p, V.n};
+        v.A = PcgMat{V.diag.p, V.len.p, V.col.p, V.val.p, V.n, CCAP, I.v32 ? V.val32.p : nullptr};
         v.off = V.doff.p;
         v.prob = V.prob.p;
         v.r = I.vr.p + V.base;
@@ -806,6 +823,9 @@ void PcgSolver::set_dim(int dim) {
         rp.lpr_coarse = 1;
         rp.prolong0 = rp.prolong_coarse = false;
     }
+    // 2D: the smoothers read FP32 copies of the values (128 bytes per row on level 0);
+    // 3D rows are long and the copy would cost 512 bytes per row at the row capacity
+    im->v32 = dim == 2;
     im->drop_graph();
 }
 int PcgSolver::levels() const { return use_amg_ && im->L > 0 ? im->L + 1 : 0; }

@@ -114,7 +114,7 @@ int pcg_solve(sdot_pd *pd, const double *b, double *x, const double *d_err) {
     pd->pcg->set_options(pd->precond == SDOT_PRECOND_AMG, pd->use_graph != 0);
     std::vector<int> its;
     PcgStats ps;
-    pd->pcg->solve(PcgMat{pd->diag.p, pd->row_len.p, pd->ell_col.p, pd->ell_val.p, pd->N}, b, x, d_err, {1}, 100000,
+    pd->pcg->solve(PcgMat{pd->diag.p, pd->row_len.p, pd->ell_col.p, pd->ell_val.p, pd->N, pd->ell_cap}, b, x, d_err, {1}, 100000,
                    its, ps);
     pd->last_precond_levels = ps.amg_fallbacks ? 0 : ps.amg_levels;
     return its[0];
