@@ -10,12 +10,16 @@ clip, cell area, boundary measures, Hessian rows) and the dirac-order CSR in HBM
 `value` is the FP64 pass (the reference's arithmetic); the mixed FP32-clip pass is
 reported beside it with its error against the FP64 oracle.
 
-The same run adds two Newton-solve legs as extra keys:
+The same run adds four legs as extra keys:
   c1: 2D Newton, N = 10^4 uniform, w0 = 0, uniform targets, to 1e-6 max relative mass
       error, FP64 and mixed, beside the oracle's own full solve on the host cores;
   c3: 3D Newton, N = 10^6 uniform in [0,1]^3, 64^3 lognormal(0, 0.5) voxel density
       (mass 1), to 1e-6, FP64 and mixed, beside the oracle's cell pass at the converged
-      weights timed on a bounded sample of the cells.
+      weights timed on a bounded sample of the cells;
+  c4: 512 batched 2D Newton solves (N_k log-uniform in [1e3, 1e5], mixed), one solve after
+      one warmup (which runs the autotuner), beside the oracle's solves of the first problems;
+  c5: the serving loop for a 10 s arrival window at 12 requests/s (70 % 2D / 30 % 3D,
+      N log-uniform in [1e3, 2e6]): p50 / p99 time to converged weights.
 
 Other workloads: --workload c3 | c4 | c5 (one line each).
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2] [--no-legs]
@@ -278,28 +282,27 @@ def main_c3(args):
     print(json.dumps(line))
 
 
-def main_c4(args):
+def run_c4(K, steps, warmup, modes, use_graph=True, cpu_budget_s=10.0):
     """512 independent 2D Newton solves in one batch (mixed precision, per-problem
     stop at 1e-6 relative mass error).  Problem k: N_k = floor(exp(U[ln 1e3, ln 1e5]))
     (U from seed 0), positions uniform from seed k+1, w0 = 0, targets 1/N_k, unit box.
-    One step = one sdot_batch_solve of all problems (grids, cells, PCG, Newton control)."""
+    One step = one sdot_batch_solve of all problems (grids, cells, PCG, Newton control).
+    cpu_budget_s = 0: no CPU baseline."""
     import sdot_amd
     from sdot_amd import workloads
     from oracle import oracle as O
-    K = args.problems
     sizes = workloads.batch_sizes(K, 1e3, 1e5, seed=0)
     pos = [workloads.uniform_diracs(int(n), 2, seed=k + 1) for k, n in enumerate(sizes)]
-    b = sdot_amd.Batch("mixed", diracs_per_cell=0.0, use_graph=not args.no_graph)
+    b = sdot_amd.Batch("mixed", diracs_per_cell=0.0, use_graph=use_graph)
     b.set_problems(pos)
-    modes = ["device", "host"] if args.newton == "both" else [args.newton]
     timed = {}
     for mode in modes:   # the first mode's warmup also runs the autotuner
         b.set_schedule(device_newton=mode == "device")
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             r = b.solve(tol=1e-6)
             print(f"# c4 {mode} warmup: status {r[0]} stats {r[5]}", file=sys.stderr, flush=True)
         runs = []
-        for _ in range(args.steps):
+        for _ in range(steps):
             t0 = time.perf_counter()
             res = b.solve(tol=1e-6)
             runs.append(time.perf_counter() - t0)
@@ -307,37 +310,58 @@ def main_c4(args):
                   flush=True)
         timed[mode] = (float(np.sum(runs) / len(runs)), res)
     sec, (st, ws, ms, pst, its, stats) = timed[modes[0]]
-    # CPU: the oracle's own solve on the first problems, about 10 s, scaled by dirac count
+    # CPU: the oracle's own solve on the first problems, about cpu_budget_s, scaled by dirac count
     cpu = None
-    if not args.no_cpu_baseline:
-        cores = min(16, os.cpu_count() or 1)
+    if cpu_budget_s > 0:
+        cores = CPU_CORES
         t0 = time.perf_counter()
         ns = 0
         k = 0
-        while k < K and (time.perf_counter() - t0 < 10.0 or k == 0):
+        while k < K and (time.perf_counter() - t0 < cpu_budget_s or k == 0):
             n = len(pos[k])
             O.ot_solve(pos[k], np.zeros(n), np.full(n, 1.0 / n), tol=1e-6, nthreads=cores)
             ns += n
             k += 1
         cpu_s = time.perf_counter() - t0
         cpu_est = cpu_s * float(sizes.sum()) / ns
-        cpu = {"value": cpu_est, "unit": "s (estimate)", "cores": cores, "kind": "port",
+        cpu = {"value": cpu_est, "unit": "s (estimate)", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
                "sample": f"oracle Newton solves of the first {k} problems ({ns} diracs) took "
                          f"{cpu_s:.2f} s; scaled by total diracs / sampled diracs"}
     err = max(float(np.abs(ms[i] * len(ms[i]) - 1).max()) for i in range(K))
+    return dict(seconds=sec, modes={m: timed[m][0] for m in modes}, its=its, stats=stats, err=err, cpu=cpu,
+                total_diracs=int(sizes.sum()))
+
+
+def c4_leg(K=512):
+    r = run_c4(K, 1, 1, ["device"], cpu_budget_s=5.0)
+    its = r["its"]
+    return {"workload": f"{K} batched 2D Newton OT solves, N_k log-uniform in [1e3,1e5] (sum {r['total_diracs']}), "
+                        "mixed, per-problem stop at max relative mass error 1e-6, Newton driven from the device",
+            "seconds": r["seconds"], "max_rel_mass_err": r["err"],
+            "newton_iters_per_problem": {"min": int(its.min()), "median": float(np.median(its)), "max": int(its.max())},
+            "pcg_iters_total": r["stats"]["pcg_iters_total"], "cpu_baseline": r["cpu"],
+            "speedup_vs_cpu_estimate": r["cpu"]["value"] / r["seconds"] if r["cpu"] else None}
+
+
+def main_c4(args):
+    K = args.problems
+    modes = ["device", "host"] if args.newton == "both" else [args.newton]
+    r = run_c4(K, args.steps, args.warmup, modes, use_graph=not args.no_graph,
+               cpu_budget_s=0.0 if args.no_cpu_baseline else 10.0)
+    sec, its, stats, err, cpu = r["seconds"], r["its"], r["stats"], r["err"], r["cpu"]
     line = {
         "metric": "solve s @1e-6 mass err (512 batched 2D Newton problems)",
         "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32-clip/f64-solve", "data": "synthetic (xoshiro256** seeds 0..512)",
         "config": {"workload": f"{K} independent 2D OT Newton solves, N_k log-uniform in [1e3,1e5] "
-                               f"(sum {int(sizes.sum())}), unit box, w0=0, targets 1/N_k, per-problem stop at max "
+                               f"(sum {r['total_diracs']}), unit box, w0=0, targets 1/N_k, per-problem stop at max "
                                "relative mass error 1e-6, " + (
                                    "Newton driven from the device (one captured graph per iteration, replayed)"
                                    if modes[0] == "device" else "host-driven Newton iterations, PCG from a captured "
                                    "hipGraph"),
-                   "problems": K, "total_diracs": int(sizes.sum()), "precision": "mixed", "newton": modes[0]},
-        "newton_modes_s": {m: timed[m][0] for m in modes},
+                   "problems": K, "total_diracs": r["total_diracs"], "precision": "mixed", "newton": modes[0]},
+        "newton_modes_s": r["modes"],
         "newton_iters_per_problem": {"min": int(its.min()), "median": float(np.median(its)), "max": int(its.max())},
         "ms_pcg_iteration_graph": stats["ms_pcg_iteration"],
         "max_rel_mass_err": err,
@@ -422,6 +446,18 @@ def c5_run(rate, duration, deadline, precision, seed=7, label=""):
     return out
 
 
+def c5_leg(rate=12.0, window=10.0):
+    """A bounded serving run at a rate the device sustains (every request served)."""
+    r = c5_run(rate, window, window + 120.0, "mixed", seed=8, label=" leg")
+    keys = ("rate_per_s", "arrival_window_s", "offered", "completed", "converged", "throughput_req_per_s",
+            "cells_per_s", "p50_s", "p99_s", "max_s", "backlog_at_end")
+    out = {k: r[k] for k in keys}
+    out["workload"] = ("serving loop: Poisson arrivals, 70 % 2D / 30 % 3D, N = floor(exp(U[ln 1e3, ln 2e6])), unit box, "
+                       "uniform targets, w0 = 0, stop at max relative mass error 1e-6, mixed; latency = time to "
+                       "converged weights")
+    return out
+
+
 def main_c5(args):
     """Serving loop.  Phase 1: the offered load (Poisson 200/s, 70 % 2D / 30 % 3D,
     N log-uniform in [1e3, 2e6]) for a bounded arrival window; the queue grows
@@ -463,7 +499,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-legs", action="store_true", help="c2 only: skip the C1 / C3 solve legs")
+    ap.add_argument("--no-legs", action="store_true", help="c2 only: skip the C1 / C3 / C4 / C5 legs")
     ap.add_argument("--no-graph", action="store_true", help="c4 only: launch the kernels one by one")
     ap.add_argument("--newton", choices=["device", "host", "both"], default="both",
                     help="c4 only: device-driven Newton (the line's value), host-driven, or both")
@@ -507,6 +543,8 @@ def main():
     if not args.no_legs:
         legs["c1"] = run_c1()
         legs["c3"] = c3_leg()
+        legs["c4"] = c4_leg()
+        legs["c5"] = c5_leg()
 
     b = res["fp64"]
     achieved = b["achieved_tflops"]
