@@ -5,6 +5,8 @@
 #include <chrono>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "pcg.h"
 
@@ -885,6 +887,11 @@ void PcgSolver::solve(const PcgMat &A, const double *b, double *x, const double 
         HIPCK(hipStreamSynchronize(s));
         float ms = 0;
         if (use_graph_ && hipEventElapsedTime(&ms, I.gev[0], I.gev[1]) == hipSuccess) st.ms_graph += ms;
+        // SDOT_TRACE=1: one line per graph replay (its device time, the problems still iterating)
+        static const bool trace = getenv("SDOT_TRACE") && getenv("SDOT_TRACE")[0] == '1';
+        if (trace && use_graph_)
+            fprintf(stderr, "[sdot] pcg replay %d (%d iterations): %.3f ms, %d of %lld problems still iterating\n",
+                    it / CHUNK, CHUNK, ms, act, (long long)K);
     }
     HIPCK(hipMemcpyAsync(S.data(), I.S.p, S.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     std::vector<int> off_flags(K);
