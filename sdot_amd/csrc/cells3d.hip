@@ -284,6 +284,17 @@ struct WaveCell {
         return wave_max_dpp(r);
     }
 
+    // a sphere holding the polyhedron: centre the mean of the vertices, squared radius
+    __device__ void sphere(double &mx, double &my, double &mz, double &r2) const {
+        const bool v = lane < nv;
+        const double inv = 1.0 / nv;
+        mx = wave_sum_dpp(v ? (double)x : 0.0) * inv;
+        my = wave_sum_dpp(v ? (double)y : 0.0) * inv;
+        mz = wave_sum_dpp(v ? (double)z : 0.0) * inv;
+        const double dx = (double)x - mx, dy = (double)y - my, dz = (double)z - mz;
+        r2 = wave_max_dpp(v ? dx * dx + dy * dy + dz * dz : 0.0);
+    }
+
     // Side of this lane's vertex for the plane n.q <= h, exactly from the planes:
     // with rows [n_k, h_k] of the vertex's planes a, b, c and the new plane p,
     // det4 = (h_p - n_p.v) det3 and det3 = det(n_a, n_b, n_c) > 0 (orientation).
@@ -565,7 +576,32 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
             const int j = a.hint_col[HCAP3 * t + lane];
             if (j >= 0 && (int64_t)j < a.N && (int64_t)j != t) hj = j;
         }
-        // the lanes' candidates j (-1: none): tested against the current radius, and only
+        // After the warm start the polyhedron is nearly final, and with spread weights it
+        // often lies away from its dirac: a candidate is then also tested against a sphere
+        // about the vertices' mean, which holds the polyhedron as tightly as its own shape
+        // allows.  Centre and squared radius stay in the face-area slots of the LDS slice
+        // (free until the integration), so the cold passes keep their registers.
+        bool centred = false;
+        double *const sph = L.acc + (WV - 4);
+        auto set_sphere = [&]() {
+            double mx, my, mz, r2;
+            C.sphere(mx, my, mz, r2);
+            if (lane == 0) {
+                sph[0] = mx;
+                sph[1] = my;
+                sph[2] = mz;
+                sph[3] = r2;
+            }
+            wave_sync();
+        };
+        // true when the plane v.q <= h cannot cut the polyhedron (no vertex beyond it)
+        auto cannot_cut = [&](double vx, double vy, double vz, double h, double vv) {
+            if (h > 0 && R2 * vv <= h * h * (1 - c3::Eps3<T>::cand)) return true;   // sphere about the dirac
+            if (!centred) return false;
+            const double d = h - (vx * sph[0] + vy * sph[1] + vz * sph[2]);
+            return d > 0 && sph[3] * vv <= d * d * (1 - c3::Eps3<T>::cand);
+        };
+        // the lanes' candidates j (-1: none): tested against the current spheres, and only
         // the ballot of possible cutters applied, in lane order
         auto cut_lanes = [&](int j, bool skip_hints) {
             double vx = 0, vy = 0, vz = 0, hv = 0, vv = 0;
@@ -579,7 +615,7 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
                 if (vv == 0) {
                     bad = true;
                     cand = false;
-                } else if (hv > 0 && R2 * vv <= hv * hv * (1 - c3::Eps3<T>::cand)) {
+                } else if (cannot_cut(vx, vy, vz, hv, vv)) {
                     cand = false;
                 }
             }
@@ -591,17 +627,26 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
                              uh = lane_value(hv, k), uv = lane_value(vv, k);
                 const int uj = lane_value(j, k);
                 if (skip_hints && use_hints && __ballot(hj == uj)) continue;   // cut by the warm start
-                if (uh > 0 && R2 * uv <= uh * uh * (1 - c3::Eps3<T>::cand)) continue;   // R shrank
+                if (cannot_cut(ux, uy, uz, uh, uv)) continue;                   // the polyhedron shrank
                 const double tie = c3::Eps3<T>::tie * sqrt(uv * R2);
                 const int rc = C.cut(T(ux), T(uy), T(uz), T(uh), uj, ux, uy, uz, uh, tie, true);
                 if (rc == 2) {
                     overflow = true;
                     break;
                 }
-                if (rc == 1 && C.nv > 0) R2 = C.radius2();
+                if (rc == 1 && C.nv > 0) {
+                    R2 = C.radius2();
+                    if (centred) set_sphere();
+                }
             }
         };
-        if (use_hints && C.nv > 0 && !overflow) cut_lanes(hj, false);
+        if (use_hints && C.nv > 0 && !overflow) {
+            cut_lanes(hj, false);
+            if (C.nv > 0 && !overflow) {
+                set_sphere();
+                centred = true;
+            }
+        }
         // The diracs of the grid cells held by the lanes (ok, [cs, ce)) are staged 64 at a
         // time (one per lane, loaded in parallel) and handed to cut_lanes.
         auto process_cells = [&](bool ok, uint32_t cs, uint32_t ce) {
