@@ -202,7 +202,11 @@ typedef struct sdot_solve_stats {
 /* OptimalTransportSolver::solve(pos, w_inout, N, masses, rf) -- OptimalTransportSolver.tcc:27-119.
  * Newton with the reference's rules (M00 doubled without ball; 0.9/0.1 backtrack on an
  * empty cell).  stop_rule 0: max_i |m_i - t_i| / t_i < tol; 1: max |dw| < tol (reference).
- * weights (dirac order) are read as the start and overwritten with the result. */
+ * weights (dirac order) are read as the start and overwritten with the result.
+ * In 3D the passes after the first start each cell from its neighbours in the solve's last
+ * complete pass (the same cells; only the rounding of their vertices follows the solve's
+ * history): the result depends on the call's inputs only, bitwise, whatever the handle
+ * computed before. */
 sdot_status sdot_ot_solve(sdot_pd *pd, const double *target_masses, double tol, int stop_rule, int max_iter,
                           double *weights_inout, sdot_solve_stats *stats);
 
