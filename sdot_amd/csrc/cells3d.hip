@@ -353,8 +353,11 @@ struct WaveCell {
     }
 
     // cut by q.n <= h: 0 no change, 1 cut (nv may be 0), 2 capacity exceeded
-    __device__ int cut(T nx, T ny, T nz, T h, int id, double nx64, double ny64, double nz64, double h64,
-                       double tie_tol, bool may_compact) {
+    // always inlined: an out-of-line call saves the wavefront's live registers to scratch
+    // around every cut (the compiler chose that for the warm-started instantiation, which
+    // ran twice as slow)
+    __device__ __attribute__((always_inline)) int cut(T nx, T ny, T nz, T h, int id, double nx64, double ny64,
+                                                      double nz64, double h64, double tie_tol, bool may_compact) {
         const bool valid = lane < nv;
         T d = valid ? fma(-x, nx, fma(-y, ny, fma(-z, nz, h))) : T(1);
         bool out = valid && d < 0;
@@ -523,8 +526,10 @@ struct WaveCell {
     }
 };
 
-template <class T>
-// three wavefronts per SIMD: registers capped at 168, LDS 3 x 4 x 11.5 KB per CU
+// WARM: the instantiation for warm-started passes (CellArgs::hint_col set); the cold
+// one carries none of their code.  Three wavefronts per SIMD: registers capped at 168,
+// LDS 3 x 4 x 11.5 KB per CU
+template <class T, bool WARM>
 __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__restrict__ ap) {
     const CellArgs &a = *ap;
     __shared__ WaveLds lds[WPB];
@@ -570,7 +575,7 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
         // complete pass over the same sorted diracs.  They are cut first and skipped by the
         // traversal: every dirac's plane bounds the cell, so the cut order changes, not the
         // cell.
-        const bool use_hints = a.hint_col != nullptr;
+        const bool use_hints = WARM && a.hint_col != nullptr;
         int hj = -1;
         if (use_hints && lane < HCAP3) {
             const int j = a.hint_col[HCAP3 * t + lane];
@@ -597,7 +602,7 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
         // true when the plane v.q <= h cannot cut the polyhedron (no vertex beyond it)
         auto cannot_cut = [&](double vx, double vy, double vz, double h, double vv) {
             if (h > 0 && R2 * vv <= h * h * (1 - c3::Eps3<T>::cand)) return true;   // sphere about the dirac
-            if (!centred) return false;
+            if (!WARM || !centred) return false;
             const double d = h - (vx * sph[0] + vy * sph[1] + vz * sph[2]);
             return d > 0 && sph[3] * vv <= d * d * (1 - c3::Eps3<T>::cand);
         };
@@ -626,7 +631,7 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
                 const double ux = lane_value(vx, k), uy = lane_value(vy, k), uz = lane_value(vz, k),
                              uh = lane_value(hv, k), uv = lane_value(vv, k);
                 const int uj = lane_value(j, k);
-                if (skip_hints && use_hints && __ballot(hj == uj)) continue;   // cut by the warm start
+                if (WARM && skip_hints && use_hints && __ballot(hj == uj)) continue;   // cut by the warm start
                 if (cannot_cut(ux, uy, uz, uh, uv)) continue;                   // the polyhedron shrank
                 const double tie = c3::Eps3<T>::tie * sqrt(uv * R2);
                 const int rc = C.cut(T(ux), T(uy), T(uz), T(uh), uj, ux, uy, uz, uh, tie, true);
@@ -636,11 +641,11 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
                 }
                 if (rc == 1 && C.nv > 0) {
                     R2 = C.radius2();
-                    if (centred) set_sphere();
+                    if (WARM && centred) set_sphere();
                 }
             }
         };
-        if (use_hints && C.nv > 0 && !overflow) {
+        if (WARM && use_hints && C.nv > 0 && !overflow) {
             cut_lanes(hj, false);
             if (C.nv > 0 && !overflow) {
                 set_sphere();
@@ -1021,7 +1026,10 @@ void launch_wave3(sdot_pd *pd, const CellArgs &a) {
     if (nblocks > need) nblocks = need;
     if (nblocks < 1) return;
     const CellArgs *da = upload_args(pd, a, 0);
-    hipLaunchKernelGGL(k_cells3_wave<T>, dim3((unsigned)nblocks), dim3(WV * WPB), 0, pd->stream, da);
+    if (a.hint_col)
+        hipLaunchKernelGGL((k_cells3_wave<T, true>), dim3((unsigned)nblocks), dim3(WV * WPB), 0, pd->stream, da);
+    else
+        hipLaunchKernelGGL((k_cells3_wave<T, false>), dim3((unsigned)nblocks), dim3(WV * WPB), 0, pd->stream, da);
     HIPCK(hipGetLastError());
 }
 
