@@ -354,26 +354,26 @@ void launch2(const CellArgs &a, unsigned nb, hipStream_t s) {
 
 // batches carry no image and no translations; ball clipping over pieces (pixels or
 // several domain items) clips each piece by the disc
-template <class T, int CAP, int WAVES>
+template <class T, int CAP, int WAVES, int BS = 64>
 void dispatch2(bool ball, bool img, const CellArgs &a, unsigned nb, hipStream_t s) {
     if constexpr (std::is_same<T, float>::value && CAP == 8) {   // a batch-only tier
-        if (ball) launch2<T, CAP, 64, true, false, WAVES, true>(a, nb, s);
-        else launch2<T, CAP, 64, false, false, WAVES, true>(a, nb, s);
+        if (ball) launch2<T, CAP, BS, true, false, WAVES, true>(a, nb, s);
+        else launch2<T, CAP, BS, false, false, WAVES, true>(a, nb, s);
         return;
     }
     if (a.prob) {
-        if (ball) launch2<T, CAP, 64, true, false, WAVES, true>(a, nb, s);
-        else launch2<T, CAP, 64, false, false, WAVES, true>(a, nb, s);
+        if (ball) launch2<T, CAP, BS, true, false, WAVES, true>(a, nb, s);
+        else launch2<T, CAP, BS, false, false, WAVES, true>(a, nb, s);
     } else if (a.ntrans > 0) {
-        if (ball && img) launch2<T, CAP, 64, true, true, WAVES, false, true>(a, nb, s);
-        else if (ball) launch2<T, CAP, 64, true, false, WAVES, false, true>(a, nb, s);
-        else if (img) launch2<T, CAP, 64, false, true, WAVES, false, true>(a, nb, s);
-        else launch2<T, CAP, 64, false, false, WAVES, false, true>(a, nb, s);
+        if (ball && img) launch2<T, CAP, BS, true, true, WAVES, false, true>(a, nb, s);
+        else if (ball) launch2<T, CAP, BS, true, false, WAVES, false, true>(a, nb, s);
+        else if (img) launch2<T, CAP, BS, false, true, WAVES, false, true>(a, nb, s);
+        else launch2<T, CAP, BS, false, false, WAVES, false, true>(a, nb, s);
     } else {
-        if (ball && img) launch2<T, CAP, 64, true, true, WAVES, false>(a, nb, s);
-        else if (ball) launch2<T, CAP, 64, true, false, WAVES, false>(a, nb, s);
-        else if (img) launch2<T, CAP, 64, false, true, WAVES, false>(a, nb, s);
-        else launch2<T, CAP, 64, false, false, WAVES, false>(a, nb, s);
+        if (ball && img) launch2<T, CAP, BS, true, true, WAVES, false>(a, nb, s);
+        else if (ball) launch2<T, CAP, BS, true, false, WAVES, false>(a, nb, s);
+        else if (img) launch2<T, CAP, BS, false, true, WAVES, false>(a, nb, s);
+        else launch2<T, CAP, BS, false, false, WAVES, false>(a, nb, s);
     }
 }
 
@@ -482,7 +482,10 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
             dispatch2<double, 8, 4>(ball, img, a, nb, s);
             nb = next(false);
         }
-        dispatch2<double, 16, 2>(ball, img, a, nb, s);
+        // the cells that outgrew the first tier run 32 to a wavefront (the wavefront's time
+        // is its slowest lanes' chain); a batch's persistent first tier keeps 64
+        if (a.work) dispatch2<double, 16, 2, 32>(ball, img, a, nb, s);
+        else dispatch2<double, 16, 2>(ball, img, a, nb, s);
         nb = next(true);
         dispatch2<double, 64, 1>(ball, img, a, nb, s);
     }
