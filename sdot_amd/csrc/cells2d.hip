@@ -473,7 +473,8 @@ void run_cells2(sdot_pd *pd, CellArgs &a) {
             dispatch2<float, 8, 4>(ball, img, a, nb, s);
             nb = next(false);
         }
-        dispatch2<float, 16, 3>(ball, img, a, nb, s);
+        if (a.work) dispatch2<float, 16, 3, 32>(ball, img, a, nb, s);   // a second tier: 32 cells a wavefront
+        else dispatch2<float, 16, 3>(ball, img, a, nb, s);
         nb = next(true);
         dispatch2<float, 64, 1>(ball, img, a, nb, s);
     } else {
