@@ -493,9 +493,10 @@ BatchEngine::~BatchEngine() {
 double BatchEngine::bytes_per_dirac() {
     // dirac arrays (pos, w, w0, old, tgt, spare copies for a relayout), sort keys and
     // permutations, sorted copies, pass outputs, ELL rows at the wide capacity, PCG vectors,
-    // FP64 resume slots (184 bytes for a quarter of the cells), FP32 copies of the matrix
-    // values for the multigrid smoothers (32 entries per row on level 0, 64 per coarse row)
-    return 16 + 8 * 4 + 8 * 5 + 8 + 8 + 24 + 32 + 4 + ELL_CAP2_WIDE * 12.0 + 8 * 7 + 8 + 16 + 46 + 128 + 86;
+    // FP64 resume slots (a 16-byte header per cell, 160 bytes for a quarter of the cells),
+    // FP32 copies of the matrix values for the multigrid smoothers (32 entries per row on
+    // level 0, 64 per coarse row)
+    return 16 + 8 * 4 + 8 * 5 + 8 + 8 + 24 + 32 + 4 + ELL_CAP2_WIDE * 12.0 + 8 * 7 + 8 + 16 + 56 + 128 + 86;
 }
 
 size_t BatchEngine::device_bytes() const { return (size_t)(bytes_per_dirac() * (double)N_); }
