@@ -554,8 +554,10 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
         constexpr int WB = 8 * sizeof(M);
         M mask = 0;
         if (CAP <= 8) {
+            // both sides evaluated (a '&&' would branch around each LDS read; reading the
+            // slot past n is in bounds and masked out)
 #pragma unroll
-            for (int k = 0; k < CAP; ++k) mask |= (k < n && dist(k) < 0) ? (M(1) << k) : M(0);
+            for (int k = 0; k < CAP; ++k) mask |= ((k < n) & (dist(k) < 0)) ? (M(1) << k) : M(0);
         } else {
             for (int k = 0; k < n; ++k) mask |= dist(k) < 0 ? (M(1) << k) : M(0);
         }
@@ -686,7 +688,7 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                 const double vx = px - qx, vy = py - qy;
                 const double vv = sq2(vx, vy);
                 const double hh = 0.5 * (vv + w0 - pw);
-                if (hh > 0 && R2 * vv <= hh * hh * (1 - Eps<T>::cand)) continue;   // cannot cut
+                if ((hh > 0) & (R2 * vv <= hh * hh * (1 - Eps<T>::cand))) continue;   // cannot cut (no branch inside)
                 if (vv == 0) { bad = true; continue; }                             // coincident diracs
                 const T Vx = T(vx), Vy = T(vy), H = T(hh);
                 const int rc = cut(Vx, Vy, H, (int)j + idbase, vx, vy, hh);
