@@ -559,7 +559,10 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
 #pragma unroll
             for (int k = 0; k < CAP; ++k) mask |= ((k < n) & (dist(k) < 0)) ? (M(1) << k) : M(0);
         } else {
-            for (int k = 0; k < n; ++k) mask |= dist(k) < 0 ? (M(1) << k) : M(0);
+            // the first 8 as above (most cells have fewer vertices), the rest in a loop
+#pragma unroll
+            for (int k = 0; k < 8; ++k) mask |= ((k < n) & (dist(k) < 0)) ? (M(1) << k) : M(0);
+            for (int k = 8; k < n; ++k) mask |= dist(k) < 0 ? (M(1) << k) : M(0);
         }
         if (!mask) return 0;
         const M full = n >= WB ? ~M(0) : (M(1) << n) - 1;
