@@ -682,10 +682,11 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
             double nx = a.spos[2 * (int64_t)j0], ny = a.spos[2 * (int64_t)j0 + 1], nw = a.sw[j0];
             for (uint32_t j = j0; j < ce; ++j) {
                 const double px = nx, py = ny, pw = nw;
-                if (j + 1 < ce) {
-                    nx = a.spos[2 * (int64_t)(j + 1)];
-                    ny = a.spos[2 * (int64_t)(j + 1) + 1];
-                    nw = a.sw[j + 1];
+                {   // unconditional (the last candidate loads itself again): no branch around the loads
+                    const uint32_t jn = j + 1 < ce ? j + 1 : j;
+                    nx = a.spos[2 * (int64_t)jn];
+                    ny = a.spos[2 * (int64_t)jn + 1];
+                    nw = a.sw[jn];
                 }
                 if ((int64_t)j == t && s < 0) continue;
                 const double vx = px - qx, vy = py - qy;
