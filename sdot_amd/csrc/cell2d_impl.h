@@ -688,11 +688,12 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
                     ny = a.spos[2 * (int64_t)jn + 1];
                     nw = a.sw[jn];
                 }
-                if ((int64_t)j == t && s < 0) continue;
                 const double vx = px - qx, vy = py - qy;
                 const double vv = sq2(vx, vy);
                 const double hh = 0.5 * (vv + w0 - pw);
-                if ((hh > 0) & (R2 * vv <= hh * hh * (1 - Eps<T>::cand))) continue;   // cannot cut (no branch inside)
+                // the cell's own dirac, or a dirac whose line cannot reach the polygon: one
+                // branch for both (no short-circuit)
+                if ((((int64_t)j == t) & (s < 0)) | ((hh > 0) & (R2 * vv <= hh * hh * (1 - Eps<T>::cand)))) continue;
                 if (vv == 0) { bad = true; continue; }                             // coincident diracs
                 const T Vx = T(vx), Vy = T(vy), H = T(hh);
                 const int rc = cut(Vx, Vy, H, (int)j + idbase, vx, vy, hh);
