@@ -664,16 +664,39 @@ SDOT_HD inline int cell2_body(const CellArgs &a, const int64_t t, T *X, T *Y, in
           const int gxa = full_row ? (hx - r < 0 ? 0 : hx - r) : hx - r;
           const int gxb = full_row ? (hx + r >= g.ncell[0] ? g.ncell[0] - 1 : hx + r) : hx + r;
           const int step = full_row ? 1 : 2 * r;
+          // the bounds and the largest weight of the row's next grid cell are loaded while this
+          // one is processed (one load latency per grid cell instead of two or three)
+          const int gxe = gxb < g.ncell[0] - 1 ? gxb : g.ncell[0] - 1;   // last in-range cell of the row
+          uint32_t ncs = 0, nce = 0;
+          double nwm = 0;
+          bool pre = false;
           for (int gx = res ? rs_gx : gxa; gx <= gxb && n > 0 && !overflow; gx += step) {
-            if (gx < 0 || gx >= g.ncell[0]) continue;
-            const uint32_t code = morton2(gx, gy);
-            const uint32_t cs = pv.cell_start[code], ce = pv.cell_start[code + 1];
+            if (gx < 0 || gx >= g.ncell[0]) {
+                pre = false;
+                continue;
+            }
+            uint32_t cs = ncs, ce = nce;
+            double wm = nwm;
+            if (!pre) {
+                const uint32_t code = morton2(gx, gy);
+                cs = pv.cell_start[code];
+                ce = pv.cell_start[code + 1];
+                wm = pv.cell_wmax[code];
+            }
+            {   // unconditional (the row's last cell loads itself again): no branch around the loads
+                const int gn = gx + step <= gxe ? gx + step : gx;
+                pre = gx + step <= gxe;
+                const uint32_t cn = morton2(gn, gy);
+                ncs = pv.cell_start[cn];
+                nce = pv.cell_start[cn + 1];
+                nwm = pv.cell_wmax[cn];
+            }
             if (cs == ce) continue;
             if (!res) {
                 double bx0 = g.lo[0] + gx * g.h[0], by0 = g.lo[1] + gy * g.h[1];
                 double ex = fmax(0.0, fmax(bx0 - qx, qx - (bx0 + g.h[0])));
                 double ey = fmax(0.0, fmax(by0 - qy, qy - (by0 + g.h[1])));
-                if (region_skippable<T>(ex * ex + ey * ey, R2, w0, pv.cell_wmax[code])) continue;
+                if (region_skippable<T>(ex * ex + ey * ey, R2, w0, wm)) continue;
             }
             const uint32_t j0 = res ? (uint32_t)rs_j : cs;
             res = false;
