@@ -42,11 +42,11 @@ PEAK = {"fp64": 78.6, "mixed": 157.3}        # MI355X vector FP64 / FP32, TFLOP/
 # FETCH_SIZE and WRITE_SIZE of k_cells2, each scaled by the same counter on the library's copy
 # kernel moving 1 GiB (bench.py --copy-calibration 1024: FETCH reads half the copied bytes,
 # WRITE all of them, so FETCH x 2, WRITE x 1)
-TRAFFIC = {"mixed": 184.2e6, "fp64": 564.6e6}
+TRAFFIC = {"mixed": 184.3e6, "fp64": 577.9e6}
 # VALU instructions (wavefront instructions, SQ_INSTS_VALU) per pass of the same kernels, and
 # the issue rate they are compared with: one wave64 VALU instruction per SIMD every 4 cycles,
 # 1024 SIMDs at 2.4 GHz
-VALU_INSTS = {"mixed": 4.303e8, "fp64": 4.342e8}
+VALU_INSTS = {"mixed": 4.099e8, "fp64": 4.174e8}
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 CPU_CORES = min(16, os.cpu_count() or 1)     # the host share of one GPU on the box
 
