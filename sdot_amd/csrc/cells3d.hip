@@ -609,21 +609,16 @@ __global__ __launch_bounds__(WV * WPB, 3) void k_cells3_wave(const CellArgs *__r
         // the lanes' candidates j (-1: none): tested against the current spheres, and only
         // the ballot of possible cutters applied, in lane order
         auto cut_lanes = [&](int j, bool skip_hints) {
-            double vx = 0, vy = 0, vz = 0, hv = 0, vv = 0;
-            bool cand = j >= 0 && (int64_t)j != t;
-            if (cand) {
-                vx = a.spos[3 * (int64_t)j] - c0x;
-                vy = a.spos[3 * (int64_t)j + 1] - c0y;
-                vz = a.spos[3 * (int64_t)j + 2] - c0z;
-                vv = vx * vx + vy * vy + vz * vz;
-                hv = 0.5 * (vv + w0 - a.sw[j]);
-                if (vv == 0) {
-                    bad = true;
-                    cand = false;
-                } else if (cannot_cut(vx, vy, vz, hv, vv)) {
-                    cand = false;
-                }
-            }
+            // every lane loads (a lane without a candidate loads the cell's own dirac): no
+            // branch around the loads; the tests below are evaluated without short-circuits
+            const int64_t jl = j >= 0 ? (int64_t)j : t;
+            const double vx = a.spos[3 * jl] - c0x, vy = a.spos[3 * jl + 1] - c0y, vz = a.spos[3 * jl + 2] - c0z;
+            const double vv = vx * vx + vy * vy + vz * vz;
+            const double hv = 0.5 * (vv + w0 - a.sw[jl]);
+            bool cand = (j >= 0) & (jl != t);
+            if (cand & (vv == 0)) bad = true;
+            cand = cand & (vv != 0);
+            if (cand && cannot_cut(vx, vy, vz, hv, vv)) cand = false;
             unsigned long long mask = __ballot(cand);
             while (mask && C.nv > 0) {
                 const int k = __ffsll((long long)mask) - 1;
