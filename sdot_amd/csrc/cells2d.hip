@@ -214,7 +214,23 @@ __global__ __launch_bounds__(256) void k_csr_fill(const int *row_len, const int 
             }
         };
         insert((int)i, diag[t]);
-        for (int e = 0; e < L; ++e) insert(perm[ell_col[e * stride + t]], ell_val[e * stride + t]);
+        // slots four at a time: their column, permutation and value loads are issued
+        // together (slots past the row re-read slot e0 and insert the key INT_MAX, which
+        // leaves the row as it is), so a row waits on two load latencies per four entries
+        for (int e0 = 0; e0 < L; e0 += 4) {
+            int cl[4];
+            double vl[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t e = e0 + u < L ? e0 + u : e0;
+                cl[u] = ell_col[e * stride + t];
+                vl[u] = ell_val[e * stride + t];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cl[u] = perm[cl[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) insert(e0 + u < L ? cl[u] : INT_MAX, vl[u]);
+        }
         int w = 0;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -250,7 +266,16 @@ __global__ __launch_bounds__(256) void k_csr_fill(const int *row_len, const int 
             }
         };
         insert((uint64_t)i << 8 | 255u);
-        for (int e = 0; e < L; ++e) insert((uint64_t)perm[ell_col[e * stride + t]] << 8 | (unsigned)e);
+        for (int e0 = 0; e0 < L; e0 += 4) {   // as above; the key ~0 leaves the row as it is
+            int cl[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cl[u] = ell_col[(int64_t)(e0 + u < L ? e0 + u : e0) * stride + t];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cl[u] = perm[cl[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                insert(e0 + u < L ? (uint64_t)cl[u] << 8 | (unsigned)(e0 + u) : ~0ull);
+        }
         int w = 0;
         int64_t pc = -1;
         double acc = 0;
