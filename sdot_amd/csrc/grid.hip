@@ -103,18 +103,12 @@ __global__ __launch_bounds__(BS) void k_gather_pos(const double *pos, const int 
 // diagram stays a consistent power diagram, moved by ~1e-12 relative, and
 // exact ties (lattices: four or more planes through one point, three planes
 // through one line) no longer reach the FP64 clipping.
-__global__ __launch_bounds__(BS) void k_gather_w(const double *w, const int *perm, int64_t n, double *sw,
-                                                 unsigned *flags, double tie) {
-    int64_t p = blockIdx.x * (int64_t)BS + threadIdx.x;
-    if (p >= n) return;
-    const int i = perm[p];
-    double v = w[i];
-    if (!isfinite(v)) atomicOr(flags, FLAG_BAD_INPUT);
+__device__ inline double tie_unit(int i) {   // in [0, 1)
     uint64_t z = (uint64_t)i * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     z ^= z >> 31;
-    sw[p] = v + tie * ((double)(z >> 11) * 0x1.0p-53);
+    return (double)(z >> 11) * 0x1.0p-53;
 }
 
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
@@ -128,23 +122,47 @@ __global__ __launch_bounds__(BS) void k_pyr_level(const double *fine, int64_t nn
     coarse[n] = m;
 }
 
-__global__ __launch_bounds__(BS) void k_cell_wmax(const uint32_t *start, int64_t ncodes, const double *sw,
-                                                  double *cell_wmax, unsigned long long *wglob) {
-    __shared__ double sm[BS];
-    int64_t c = blockIdx.x * (int64_t)BS + threadIdx.x;
+// One thread per grid cell: its sorted weights (gathered through the permutation,
+// with their tie offsets), their maximum, and the maximum over all cells (one atomic
+// per block).  The cell's diracs are taken four at a time so that their permutation
+// and weight loads are issued together.
+__global__ __launch_bounds__(BS) void k_weights(const uint32_t *start, int64_t ncodes, const double *w, const int *perm,
+                                                double *sw, double *cell_wmax, unsigned long long *wglob,
+                                                unsigned *flags, double tie) {
+    __shared__ double sm[BS / 64];
+    const int64_t c = blockIdx.x * (int64_t)BS + threadIdx.x;
     double m = -DBL_MAX;
+    bool bad = false;
     if (c < ncodes) {
-        uint32_t s = start[c], e = start[c + 1];
-        for (uint32_t k = s; k < e; ++k) m = fmax(m, sw[k]);
+        const uint32_t s = start[c], e = start[c + 1];
+        for (uint32_t k0 = s; k0 < e; k0 += 4) {
+            int i[4];
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) i[u] = perm[k0 + u < e ? k0 + u : k0];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = w[i[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (k0 + u < e) {
+                    bad |= !isfinite(v[u]);
+                    const double x = v[u] + tie * tie_unit(i[u]);
+                    sw[k0 + u] = x;
+                    m = fmax(m, x);
+                }
+            }
+        }
         cell_wmax[c] = m;
     }
-    sm[threadIdx.x] = m;
+    if (bad) atomicOr(flags, FLAG_BAD_INPUT);
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
     __syncthreads();
-    for (int s = BS / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = sm[0];
+        for (int q = 1; q < BS / 64; ++q) b = fmax(b, sm[q]);
+        if (b > -DBL_MAX) atomicMax(wglob, dbl_order(b));
     }
-    if (threadIdx.x == 0 && sm[0] > -DBL_MAX) atomicMax(wglob, dbl_order(sm[0]));
 }
 
 // ---- exclusive scan (out[0] = 0, out[k+1] = in[0] + ... + in[k]) ----
@@ -360,11 +378,10 @@ void refresh_weights(sdot_pd *pd) {
     double vol = 1;
     for (int d = 0; d < pd->dim; ++d) vol *= g.h[d] * g.ncell[d];
     const double spacing2 = pow(vol / (double)(N > 0 ? N : 1), 2.0 / pd->dim);
-    hipLaunchKernelGGL(k_gather_w, dim3(nblk(N, BS)), dim3(BS), 0, s, pd->w.p, pd->perm.p, N, pd->sw.p, pd->flags.p,
-                       N > 1 ? 1e-12 * spacing2 : 0.0);   // a lone dirac has no ties to break
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, pd->wglob.p, dbl_order(-DBL_MAX));
-    hipLaunchKernelGGL(k_cell_wmax, dim3(nblk(g.ncodes, BS)), dim3(BS), 0, s, pd->cell_start.p, g.ncodes, pd->sw.p,
-                       pd->cell_wmax.p, pd->wglob.p);
+    hipLaunchKernelGGL(k_weights, dim3(nblk(g.ncodes, BS)), dim3(BS), 0, s, pd->cell_start.p, g.ncodes, pd->w.p,
+                       pd->perm.p, pd->sw.p, pd->cell_wmax.p, pd->wglob.p, pd->flags.p,
+                       N > 1 ? 1e-12 * spacing2 : 0.0);   // a lone dirac has no ties to break
     HIPCK(hipGetLastError());
     if (pd->dim == 3) {   // max-weight pyramid (level 0 = a copy of cell_wmax) for the far-field traversal
         pd->wpyr.ensure(pyr_off(g.ncodes, 3, g.bits + 1));
