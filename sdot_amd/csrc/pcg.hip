@@ -21,7 +21,6 @@ constexpr int CHUNK = 8;         // PCG iterations per graph launch
 constexpr int AGG = 4;           // rows per aggregate
 constexpr int CMIN = 64;         // coarsest level: at most CMIN rows per problem
 constexpr int CCAP = 64;         // off-diagonal entries per coarse row
-constexpr int V32_0 = ELL_CAP2;  // 2D level 0: entries per row of the FP32 copy
 constexpr double OMEGA = 0.8;    // damped-Jacobi smoothing
 constexpr double CSCALE = 1.5;   // coarse-correction scaling
 
@@ -538,8 +537,9 @@ void launch_up(int lpr, bool yin, const LevelView &F, const LevelView &C, const 
 
 struct PcgSolver::Impl {
     RowPlan plan;
-    bool v32 = false;                      // FP32 copies of the matrix values for the smoothers (2D)
-    DevBuf<float> val32_0;                 // level 0's copy (V32_0 entries per row)
+    bool v32 = false;                      // FP32 copies of the matrix values for the smoothers
+    int v32w = 0;                          // level 0's copy: entries per row (the ELL capacity)
+    DevBuf<float> val32_0;                 // level 0's copy, slot-major (slot e of row t at e * N + t)
     hipStream_t s = nullptr;
     int64_t K = 0, N = 0, nblk = 0;
     std::vector<int64_t> off, blk_off;
@@ -669,7 +669,7 @@ void PcgSolver::set_layout(const std::vector<int64_t> &off, hipStream_t s) {
         V.val.ensure((size_t)CCAP * V.n);
         if (I.v32) V.val32.ensure((size_t)CCAP * V.n);
     }
-    if (I.v32 && I.L > 0) I.val32_0.ensure((size_t)V32_0 * N);
+    if (I.v32 && I.L > 0) I.val32_0.ensure((size_t)I.v32w * N);
     for (auto *b : {&I.vr, &I.vxpre, &I.vout}) b->ensure(base);
     // coarsest rows of each problem
     std::vector<int64_t> coff(K), cbase(K);
@@ -701,8 +701,8 @@ LevelView level_view(PcgSolver::Impl &I, const PcgMat &A0, int l, double *r0, do
     if (l == 0) {
         v.A = A0;
         // the FP32 copy, when it holds the caller's whole layout (ELL stride = row count)
-        const bool c32 = I.v32 && A0.cap > 0 && A0.cap <= V32_0 && A0.stride == I.N &&
-                         I.val32_0.n >= (size_t)V32_0 * I.N;
+        const bool c32 = I.v32 && A0.cap > 0 && A0.cap <= I.v32w && A0.stride == I.N &&
+                         I.val32_0.n >= (size_t)I.v32w * I.N;
         v.A.val32 = c32 ? I.val32_0.p : nullptr;
         v.off = I.doff.p;
         v.prob = I.row_prob.p;
@@ -825,9 +825,11 @@ void PcgSolver::set_dim(int dim) {
         rp.lpr_coarse = 1;
         rp.prolong0 = rp.prolong_coarse = false;
     }
-    // 2D: the smoothers read FP32 copies of the values (128 bytes per row on level 0);
-    // 3D rows are long and the copy would cost 512 bytes per row at the row capacity
-    im->v32 = dim == 2;
+    // the smoothers read FP32 copies of the values; level 0's copy is slot-major like the
+    // ELL arrays, so a row moves only its used slots (its allocation is the row capacity:
+    // 128 bytes per row in 2D, 512 in 3D)
+    im->v32 = true;
+    im->v32w = dim == 2 ? ELL_CAP2 : ELL_CAP3;
     im->drop_graph();
 }
 int PcgSolver::levels() const { return use_amg_ && im->L > 0 ? im->L + 1 : 0; }

@@ -34,8 +34,9 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-// rows at the wide capacity, pass outputs, PCG vectors, warm-start neighbours (128 B)
-constexpr double BYTES_PER_DIRAC_3D = 3750.0;
+// rows at the wide capacity, pass outputs, PCG vectors, warm-start neighbours (128 B),
+// the smoothers' FP32 copy of the level-0 values (512 B)
+constexpr double BYTES_PER_DIRAC_3D = 4262.0;
 constexpr double HANDLE_3D_BYTES = 1.6e9;        // thread-kernel scratch slots of one 3D handle
 
 struct Request {
